@@ -1,0 +1,159 @@
+/*
+ * C ABI of the MI355X-native SEISMIC / SEISMIC-SQ index.
+ *
+ * Each entry point stands in for one member of the reference's nsparse::Index
+ * surface (opensearch-project/neural-sparse-cpp, paths below are relative to
+ * that repository's root). No C++ exception crosses this boundary: every call
+ * returns an nsparse_status and leaves a message for nsparse_last_error().
+ * The status codes map one-to-one onto the exception classes the reference
+ * throws, so a C++ or SWIG adapter can re-throw the same types
+ * (nsparse/utils/checks.h:19-84, nsparse/python/swignsparse.swig:47-59).
+ *
+ * Pointers are plain host pointers unless a name says "device". Output arrays
+ * are caller-allocated, n*k, row-major, best first, padded with (-1.0f, -1)
+ * exactly as the reference pads them (nsparse/seismic_index.cpp:270-273).
+ */
+#ifndef NSPARSE_C_H
+#define NSPARSE_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t nsparse_idx_t;   /* nsparse/types.h:18 (idx_t)  */
+typedef uint16_t nsparse_term_t; /* nsparse/types.h:19 (term_t) */
+typedef struct nsparse_index nsparse_index;
+
+typedef enum nsparse_status {
+    NSPARSE_OK = 0,
+    NSPARSE_ERR_INVALID_ARGUMENT = 1, /* std::invalid_argument -> ValueError   */
+    NSPARSE_ERR_RUNTIME = 2,          /* std::runtime_error    -> RuntimeError */
+    NSPARSE_ERR_OUT_OF_RANGE = 3,     /* std::out_of_range     -> IndexError   */
+    NSPARSE_ERR_BAD_ALLOC = 4,        /* std::bad_alloc        -> MemoryError  */
+    NSPARSE_ERR_DEVICE = 5,           /* HIP runtime failure (no reference twin) */
+    NSPARSE_ERR_UNKNOWN = 6
+} nsparse_status;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* nsparse_last_error(void);
+
+/* index_factory(int dim, const char* desc) -- nsparse/index_factory.cpp:63-153.
+ * Accepts the reference's strings unchanged: "seismic,lambda=..|beta=..|alpha=..|seed=..",
+ * "seismic_sq,quantizer=8bit|vmin=..|vmax=..|lambda=..|beta=..|alpha=..|seed=..",
+ * and the "idmap,<delegate>" prefix. Unknown keys are ignored, as there.
+ * "brutal" and "inverted" are other index types and are rejected with
+ * NSPARSE_ERR_RUNTIME ("not implemented on this device path"). */
+int nsparse_index_factory(int dim, const char* description, nsparse_index** out);
+
+/* delete index (SWIG %newobject ownership, nsparse/python/swignsparse.swig:182-184). */
+int nsparse_free(nsparse_index* index);
+
+/* Index::id() fourcc characters: SEIS / SESQ / IDMP (nsparse/seismic_index.h:36,
+ * nsparse/seismic_scalar_quantized_index.h:37, nsparse/id_map_index.h:81). */
+int nsparse_index_id(const nsparse_index* index, char out_fourcc[4]);
+
+/* Index::get_dimension() -- nsparse/index.h:47. */
+int nsparse_get_dimension(const nsparse_index* index, int* out);
+
+/* Index::num_vectors() -- nsparse/index.h:51-54. */
+int nsparse_num_vectors(const nsparse_index* index, int64_t* out);
+
+/* Index::add -- nsparse/seismic_index.cpp:127-142, seismic_scalar_quantized_index.cpp:159-177.
+ * Assumes indptr[0] == 0 like the reference (sparse_vectors.cpp:92-126). Term ids
+ * must be < dim (checked here; the reference does not check). */
+int nsparse_add(nsparse_index* index, nsparse_idx_t n, const nsparse_idx_t* indptr,
+                const nsparse_term_t* indices, const float* values);
+
+/* Index::add_with_ids -- nsparse/id_map_index.cpp:63-75 (IDMapIndex only; others
+ * report "add_with_ids not implemented in Index", index.cpp:57-60). */
+int nsparse_add_with_ids(nsparse_index* index, nsparse_idx_t n, const nsparse_idx_t* indptr,
+                         const nsparse_term_t* indices, const float* values, const nsparse_idx_t* ids);
+
+/* Index::build -- nsparse/seismic_common.h:146-184, run as HIP kernels. */
+int nsparse_build(nsparse_index* index);
+
+/* ---- search parameters and selectors (nsparse/index.h:21-28, id_selector.h:22-86) ---- */
+
+typedef enum nsparse_selector_kind {
+    NSPARSE_SELECTOR_ARRAY = 0,  /* ArrayIDSelector(n, ids)  id_selector.h:51-66 */
+    NSPARSE_SELECTOR_SET = 1,    /* SetIDSelector(n, ids)    id_selector.h:33-49 */
+    NSPARSE_SELECTOR_NOT = 2,    /* NotIDSelector(inner)     id_selector.h:68-78 */
+    NSPARSE_SELECTOR_BITMAP = 3  /* membership bitmap over ids; not enumerable (extension) */
+} nsparse_selector_kind;
+
+typedef struct nsparse_selector {
+    int kind;                              /* nsparse_selector_kind */
+    int64_t n;                             /* ARRAY/SET: id count; BITMAP: bit count */
+    const nsparse_idx_t* ids;              /* ARRAY/SET */
+    const uint32_t* bitmap;                /* BITMAP: bit i of word i/32 set => member */
+    const struct nsparse_selector* inner;  /* NOT: non-owning, like the reference */
+} nsparse_selector;
+
+typedef enum nsparse_params_kind {
+    NSPARSE_PARAMS_BASE = 0,      /* SearchParameters: selector only (index.h:21-28)      */
+    NSPARSE_PARAMS_SEISMIC = 1,   /* SeismicSearchParameters(cut, heap_factor)           */
+    NSPARSE_PARAMS_SEISMIC_SQ = 2 /* SeismicSQSearchParameters(vmin, vmax, cut, hf)      */
+} nsparse_params_kind;
+
+typedef struct nsparse_search_params {
+    int kind;              /* nsparse_params_kind; BASE uses cut=10, heap_factor=1.0 */
+    int cut;               /* seismic_index.h:27 default 10 */
+    float heap_factor;     /* seismic_index.h:28 default 1.0 */
+    float vmin, vmax;      /* SEISMIC_SQ: query quantizer range (seismic_scalar_quantized_index.cpp:182-191) */
+    const nsparse_selector* selector; /* may be NULL */
+} nsparse_search_params;
+
+/* Index::search(n, indptr, indices, values, k, distances, labels, params) --
+ * nsparse/index.cpp:32-49 (+ SeismicIndex::search seismic_index.cpp:152-220,
+ * SeismicScalarQuantizedIndex::search seismic_scalar_quantized_index.cpp:201-305,
+ * IDMapIndex::search id_map_index.cpp:29-60). params may be NULL. */
+int nsparse_search(nsparse_index* index, nsparse_idx_t n, const nsparse_idx_t* indptr,
+                   const nsparse_term_t* indices, const float* values, int k, float* distances,
+                   nsparse_idx_t* labels, const nsparse_search_params* params);
+
+/* Batched extension (no single reference twin: the reference API takes one k per
+ * call). Queries and outputs are DEVICE pointers already resident in HBM; query i
+ * asks for ks[i] results (ks may be NULL: all k_max), written to row i of an
+ * n x k_max output and padded with (-1.0f, -1). selector_of_query[i] (may be NULL)
+ * indexes bitmaps[] (device pointers, NULL entry = no filter) or is -1. Runs as one
+ * persistent launch on `stream` (hipStream_t, NULL = default) and does not
+ * synchronize. cut/heap_factor/vmin/vmax come from params (selector field ignored). */
+int nsparse_search_device(nsparse_index* index, int64_t n, const int32_t* d_indptr,
+                          const uint16_t* d_indices, const float* d_values, const int32_t* d_ks,
+                          int k_max, const nsparse_search_params* params,
+                          const int32_t* d_selector_of_query, const uint32_t* const* d_bitmaps,
+                          int n_bitmaps, float* d_distances, int32_t* d_labels, void* stream);
+
+/* write_index(Index*, const char*) -- nsparse/io/index_io.cpp:88-131,170-173.
+ * Byte layout identical to the reference writer. */
+int nsparse_write_index(nsparse_index* index, const char* path);
+
+/* In-memory writer (BufferedIOWriter, nsparse/io/buffered_io.h:20-62). *out_buf is
+ * released with nsparse_buffer_free. */
+int nsparse_write_index_buffer(nsparse_index* index, uint8_t** out_buf, size_t* out_size);
+void nsparse_buffer_free(uint8_t* buf);
+
+/* read_index(const char*, int flags) -- nsparse/io/index_io.cpp:133-163,175-183.
+ * flags & NSPARSE_IO_FLAG_MMAP maps the file while parsing; the index is then
+ * uploaded to HBM either way. */
+#define NSPARSE_IO_FLAG_MMAP 1 /* IndexIoFlag::kUseMmap, nsparse/io/index_io.h:17-22 */
+int nsparse_read_index(const char* path, int flags, nsparse_index** out);
+int nsparse_read_index_buffer(const uint8_t* buf, size_t size, int flags, nsparse_index** out);
+
+/* Index::read_csr(path, residency) -- nsparse/index.cpp:63-142 (interchange layout). */
+int nsparse_read_csr(nsparse_index* index, const char* path, int residency);
+
+/* Index::get_vectors() (nsparse/index.h:44): copies the stored forward index out.
+ * Call with NULL arrays to learn n_rows / nnz / element_size first. Values are the
+ * stored bytes (uint8 codes for SQ, float for FP32). */
+int nsparse_get_vectors(const nsparse_index* index, int64_t* n_rows, int64_t* nnz, int* element_size,
+                        nsparse_idx_t* indptr, nsparse_term_t* indices, uint8_t* values);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NSPARSE_C_H */

@@ -1,0 +1,927 @@
+// ORACLE -- test infrastructure, not product code.
+//
+// A CPU restatement of neural-sparse-cpp's SEISMIC / SEISMIC-SQ build, search
+// and index writer, used only as the checker: by tests/, by
+// __graft_entry__.smoke() and by bench.py's cpu_baseline leg. Nothing in the
+// shipped library links or calls it.
+//
+// It follows the reference line by line (file:line cited per function) and uses
+// the same libstdc++ routines the reference uses (std::ranges::sort,
+// std::priority_queue, std::mt19937, std::uniform_int_distribution,
+// std::lround), so with tie_mode == 0 it reproduces the reference's tie order
+// as well. tie_mode == 1 is the deterministic variant the device path
+// implements: exact float ties are broken by id (doc id in lambda pruning and
+// in the result top-k, term id in summary pruning). Both modes take identical
+// traversal decisions; they differ only where two values compare equal.
+//
+// Pinning (what each part is checked against) is listed in the header comment
+// of tests/test_oracle.py. Parts pinned only by invariants there are marked
+// "parity unpinned" in that file and in DESIGN.md.
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <numeric>
+#include <queue>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <unordered_set>
+#include <utility>
+#include <vector>
+
+#include <omp.h>
+
+namespace oracle {
+
+using idx_t = int32_t;
+using term_t = uint16_t;
+constexpr idx_t kInvalid = -1;
+
+// ---------------------------------------------------------------- quantizer
+// utils/scalar_quantizer.h:65-77, 97-101, 118-126 (8-bit only).
+struct Quantizer {
+    float vmin = 0.0F, vmax = 1.0F;
+    Quantizer() = default;
+    Quantizer(float lo, float hi) : vmin(lo), vmax(hi) {
+        if (!std::isfinite(lo) || !std::isfinite(hi)) throw std::invalid_argument("vmin and vmax must be finite");
+        if (hi <= lo) throw std::invalid_argument("vmax must be greater than vmin");
+    }
+    uint8_t encode(float val) const {
+        constexpr float kMax = 255.0F;
+        float scaled = (val - vmin) * (kMax / (vmax - vmin));
+        scaled = std::max(0.0F, std::min(kMax, scaled));
+        return static_cast<uint8_t>(std::lround(scaled));
+    }
+    float decode_dot_product(float s, const Quantizer& q) const {
+        const float ingest_range = vmax - vmin;
+        const float query_range = q.vmax - q.vmin;
+        const float max_q = 255.0F;
+        const float scale = (ingest_range * query_range) / (max_q * max_q);
+        return s * scale;
+    }
+};
+
+// ---------------------------------------------------------------- storage
+// sparse_vectors.cpp:92-126 (add_vectors: copy + rebase offsets).
+struct Fwd {
+    size_t esz = 0, dim = 0;
+    std::vector<idx_t> indptr;
+    std::vector<term_t> indices;
+    std::vector<uint8_t> values;
+    size_t n() const { return indptr.empty() ? 0 : indptr.size() - 1; }
+    void add(const idx_t* ip, size_t ip_size, const term_t* idx, size_t nnz, const uint8_t* w) {
+        if (ip_size < 2) return;
+        if (indptr.empty()) indptr.push_back(0);
+        indices.insert(indices.end(), idx, idx + nnz);
+        values.insert(values.end(), w, w + nnz * esz);
+        const idx_t off = indptr.back();
+        for (size_t i = 1; i < ip_size; ++i) indptr.push_back(ip[i] + off);
+    }
+};
+
+// cluster/inverted_list_clusters.h:75-83.
+struct ListClusters {
+    std::vector<idx_t> docs, offsets;
+    size_t n_clusters = 0, esz = 4;
+    std::vector<term_t> term_ids;
+    std::vector<idx_t> term_ptr;
+    std::vector<uint16_t> csc_cluster;
+    std::vector<uint8_t> csc_value;
+};
+
+struct Stats {  // traversal counters, per query
+    int64_t lists, terms_found, csc_entries, clusters_tested, clusters_opened, docs_read, docs_scored,
+        nnz_scored;
+};
+
+// ---------------------------------------------------------------- top-k
+// utils/ranker.h:34-138 (TopKHolder with a std::priority_queue min-heap).
+template <class T>
+struct HeapTopK {
+    struct Item {
+        float score;
+        T id;
+    };
+    struct Cmp {
+        bool operator()(const Item& a, const Item& b) const { return std::greater<float>()(a.score, b.score); }
+    };
+    int k;
+    std::priority_queue<Item, std::vector<Item>, Cmp> pq;
+    float threshold = std::numeric_limits<float>::infinity();
+    explicit HeapTopK(int kk) : k(kk) {
+        if (k <= 0) throw std::invalid_argument("value must be positive");
+        std::vector<Item> v;
+        v.reserve(k);
+        pq = std::priority_queue<Item, std::vector<Item>, Cmp>(Cmp(), std::move(v));
+    }
+    void add(float score, T id) {
+        if (static_cast<int>(pq.size()) >= k && score <= threshold) return;
+        if (static_cast<int>(pq.size()) < k) {
+            pq.push({score, id});
+            if (static_cast<int>(pq.size()) == k) threshold = pq.top().score;
+        } else {
+            pq.pop();
+            pq.push({score, id});
+            threshold = pq.top().score;
+        }
+    }
+    bool full() const { return static_cast<int>(pq.size()) == k; }
+    float peek() const { return pq.top().score; }
+    // ranker.h:96-108
+    std::vector<T> ids_descending() {
+        std::vector<T> ret(pq.size());
+        int i = static_cast<int>(pq.size()) - 1;
+        while (!pq.empty() && i >= 0) {
+            ret[i--] = pq.top().id;
+            pq.pop();
+        }
+        return ret;
+    }
+    // ranker.h:110-124
+    void items_descending(std::vector<float>& s, std::vector<T>& ids) {
+        const size_t n = pq.size();
+        s.assign(n, 0.0F);
+        ids.assign(n, T{});
+        for (int i = static_cast<int>(n) - 1; i >= 0; --i) {
+            s[i] = pq.top().score;
+            ids[i] = pq.top().id;
+            pq.pop();
+        }
+    }
+};
+
+// Same contract with exact ties broken by id: keeps the k best of all distinct
+// (score, id) pairs added under (score desc, id asc). Its kth score is always
+// the heap's (the k largest scores form one multiset however ties are kept).
+template <class T>
+struct IdTopK {
+    int k;
+    std::vector<std::pair<float, T>> v;  // sorted best first
+    explicit IdTopK(int kk) : k(kk) {
+        if (k <= 0) throw std::invalid_argument("value must be positive");
+    }
+    static bool better(const std::pair<float, T>& a, const std::pair<float, T>& b) {
+        return a.first > b.first || (a.first == b.first && a.second < b.second);
+    }
+    void add(float score, T id) {
+        std::pair<float, T> it{score, id};
+        if (static_cast<int>(v.size()) == k && !better(it, v.back())) return;
+        auto pos = std::lower_bound(v.begin(), v.end(), it, better);
+        v.insert(pos, it);
+        if (static_cast<int>(v.size()) > k) v.pop_back();
+    }
+    bool full() const { return static_cast<int>(v.size()) == k; }
+    float peek() const { return v.back().first; }
+    void items_descending(std::vector<float>& s, std::vector<T>& ids) {
+        s.clear();
+        ids.clear();
+        for (auto& p : v) {
+            s.push_back(p.first);
+            ids.push_back(p.second);
+        }
+    }
+};
+
+// utils/vector_process.h:18-30 (query cut). Ties follow the libstdc++ heap in
+// both modes: the device path reproduces that heap exactly.
+template <class W>
+std::vector<term_t> top_k_tokens(const term_t* indices, const W* weights, int size, int k) {
+    if (k >= size) return std::vector<term_t>(indices, indices + size);
+    HeapTopK<term_t> h(k);
+    for (int i = 0; i < size; ++i) h.add(static_cast<float>(weights[i]), indices[i]);
+    return h.ids_descending();
+}
+
+// seismic_common.h:105-115 (first list: ranges::sort by summary score desc).
+std::vector<size_t> reorder_clusters(const std::vector<float>& s, bool first_list) {
+    std::vector<size_t> order(s.size());
+    std::iota(order.begin(), order.end(), 0);
+    if (first_list) std::ranges::sort(order, [&](size_t a, size_t b) { return s[a] > s[b]; });
+    return order;
+}
+
+// ---------------------------------------------------------------- build
+// seismic_common.h:130-144
+int calculate_lambda(int lambda, size_t n) {
+    if (lambda == -1) return std::max(static_cast<int>(0.0005F * static_cast<float>(n)), 160);
+    return lambda;
+}
+int calculate_beta(int beta, int lambda) {
+    if (beta == -1) return static_cast<int>(static_cast<float>(lambda) * 0.1F);
+    return beta;
+}
+
+float value_as_float(const uint8_t* p, size_t esz) {
+    if (esz == 4) {
+        float f;
+        std::memcpy(&f, p, 4);
+        return f;
+    }
+    return static_cast<float>(*p);
+}
+
+// invlists/inverted_lists.cpp:110-132
+std::vector<idx_t> prune(const std::vector<idx_t>& docs, const std::vector<uint8_t>& codes, size_t esz,
+                         size_t lambda, int tie_mode) {
+    const size_t n = docs.size();
+    if (lambda <= 0 || n == 0 || lambda >= n) return docs;
+    std::vector<std::pair<float, idx_t>> p;
+    p.reserve(n);
+    for (size_t i = 0; i < n; ++i) p.emplace_back(value_as_float(codes.data() + i * esz, esz), docs[i]);
+    auto cmp = [](const auto& a, const auto& b) { return a.first > b.first; };
+    if (tie_mode == 0)
+        std::ranges::sort(p, cmp);
+    else
+        std::stable_sort(p.begin(), p.end(), cmp);  // ties keep list order = doc id order
+    std::vector<idx_t> kept;
+    kept.reserve(lambda);
+    for (size_t i = 0; i < lambda; ++i) kept.push_back(p[i].second);
+    return kept;
+}
+
+// cluster/random_kmeans.cpp:28-63
+std::vector<std::vector<idx_t>> select_centroids(std::vector<idx_t> docs, size_t n_clusters, uint32_t seed) {
+    const size_t n = docs.size();
+    if (n_clusters > n || n_clusters <= 0) return {};
+    std::mt19937 gen(seed);
+    std::vector<std::vector<idx_t>> clusters(n_clusters);
+    for (size_t i = 0; i < n_clusters; i++) {
+        std::uniform_int_distribution<> distrib(0, n - i - 1);
+        size_t r = distrib(gen);
+        clusters[i].push_back(docs[r]);
+        std::swap(docs[r], docs[n - 1 - i]);
+    }
+    return clusters;
+}
+size_t boundary_check_n_clusters(size_t n_docs, size_t n_clusters) {
+    if (n_clusters <= 0) n_clusters = static_cast<size_t>(std::sqrt(n_docs));
+    n_clusters = n_clusters > n_docs ? n_docs : n_clusters;
+    return std::max(static_cast<size_t>(1), n_clusters);
+}
+
+// cluster/kmeans_utils.cpp:51-158 (centroid CSC + argmax assignment).
+template <class T, class Acc>
+void map_docs_to_clusters(const Fwd& f, const std::vector<idx_t>& docs, std::vector<std::vector<idx_t>>& clusters) {
+    if (clusters.empty() || docs.empty()) return;
+    const idx_t* ip = f.indptr.data();
+    const term_t* ix = f.indices.data();
+    const T* vals = reinterpret_cast<const T*>(f.values.data());
+    const size_t nc = clusters.size();
+    size_t max_term = 0, nnz = 0;
+    for (auto& c : clusters)
+        for (idx_t j = ip[c.at(0)]; j < ip[c.at(0) + 1]; ++j) {
+            max_term = std::max<size_t>(max_term, ix[j]);
+            ++nnz;
+        }
+    const size_t n_cols = nnz == 0 ? 0 : max_term + 1;
+    std::vector<idx_t> tp(n_cols + 1, 0);
+    for (auto& c : clusters)
+        for (idx_t j = ip[c.at(0)]; j < ip[c.at(0) + 1]; ++j) tp[ix[j] + 1]++;
+    for (size_t t = 0; t < n_cols; ++t) tp[t + 1] += tp[t];
+    std::vector<uint32_t> ccl(nnz);
+    std::vector<T> cw(nnz);
+    std::vector<idx_t> cur(tp.begin(), tp.end() - 1);
+    for (size_t c = 0; c < nc; ++c) {
+        const idx_t cen = clusters[c].at(0);
+        for (idx_t j = ip[cen]; j < ip[cen + 1]; ++j) {
+            const idx_t pos = cur[ix[j]]++;
+            ccl[pos] = static_cast<uint32_t>(c);
+            cw[pos] = vals[j];
+        }
+    }
+    std::vector<idx_t> cids;
+    for (auto& c : clusters) cids.push_back(c.at(0));
+    std::ranges::sort(cids);
+    std::vector<Acc> sims(nc, Acc(0));
+    for (const idx_t d : docs) {
+        if (std::ranges::binary_search(cids, d)) continue;
+        std::ranges::fill(sims, Acc(0));
+        for (idx_t j = ip[d]; j < ip[d + 1]; ++j) {
+            const size_t t = ix[j];
+            if (t >= n_cols) continue;
+            const Acc dv = static_cast<Acc>(vals[j]);
+            for (idx_t p = tp[t]; p < tp[t + 1]; ++p) sims[ccl[p]] += dv * static_cast<Acc>(cw[p]);
+        }
+        size_t best_c = 0;
+        Acc best = sims[0];
+        for (size_t c = 1; c < nc; ++c)
+            if (sims[c] > best) {
+                best = sims[c];
+                best_c = c;
+            }
+        clusters[best_c].push_back(d);
+    }
+}
+
+// cluster/inverted_list_clusters.cpp:37-65 (alpha-mass cut), 72-124 (max-pool).
+template <class T>
+void summarize(const Fwd& f, ListClusters& lc, float alpha, int tie_mode, std::vector<std::vector<term_t>>& s_terms,
+               std::vector<std::vector<T>>& s_vals) {
+    const size_t nc = lc.offsets.size() <= 1 ? 0 : lc.offsets.size() - 1;
+    s_terms.assign(nc, {});
+    s_vals.assign(nc, {});
+    std::vector<T> acc(f.dim, T(0));
+    std::vector<uint32_t> epoch(f.dim, 0);
+    std::vector<term_t> touched;
+    uint32_t cur = 0;
+    const T* vals = reinterpret_cast<const T*>(f.values.data());
+    for (size_t i = 0; i < nc; ++i) {
+        ++cur;
+        touched.clear();
+        float sum = 0.0F;
+        for (idx_t q = lc.offsets[i]; q < lc.offsets[i + 1]; ++q) {
+            const idx_t d = lc.docs[q];
+            for (idx_t j = f.indptr[d]; j < f.indptr[d + 1]; ++j) {
+                const term_t t = f.indices[j];
+                const T v = vals[j];
+                if (epoch[t] != cur) {
+                    epoch[t] = cur;
+                    const T value = std::max(T(0), v);
+                    acc[t] = value;
+                    touched.push_back(t);
+                    sum += value;
+                } else if (v > acc[t]) {
+                    sum += v - acc[t];
+                    acc[t] = v;
+                }
+            }
+        }
+        std::vector<std::pair<term_t, T>> pairs;
+        for (term_t t : touched) pairs.emplace_back(t, acc[t]);
+        if (tie_mode == 0)
+            std::ranges::sort(pairs, [](const auto& a, const auto& b) { return a.second > b.second; });
+        else
+            std::ranges::sort(pairs, [](const auto& a, const auto& b) {
+                return a.second > b.second || (a.second == b.second && a.first < b.first);
+            });
+        float addup = 0.0F;
+        for (size_t j = 0; j < pairs.size(); ++j) {
+            addup += pairs[j].second;
+            if (addup / sum >= alpha) {
+                pairs.erase(pairs.begin() + j + 1, pairs.end());
+                break;
+            }
+        }
+        std::ranges::sort(pairs, [](const auto& a, const auto& b) { return a.first < b.first; });
+        for (auto& [t, v] : pairs) {
+            s_terms[i].push_back(t);
+            s_vals[i].push_back(v);
+        }
+    }
+}
+
+// cluster/inverted_list_clusters.cpp:230-293 (term-major transpose).
+template <class T>
+void build_transpose(ListClusters& lc, const std::vector<std::vector<term_t>>& st, const std::vector<std::vector<T>>& sv) {
+    lc.n_clusters = st.size();
+    lc.term_ids.clear();
+    lc.term_ptr.clear();
+    lc.csc_cluster.clear();
+    lc.csc_value.clear();
+    if (lc.n_clusters == 0) return;
+    std::vector<term_t> ids;
+    for (auto& v : st) ids.insert(ids.end(), v.begin(), v.end());
+    const size_t nnz = ids.size();
+    std::ranges::sort(ids);
+    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+    auto col = [&](term_t t) { return static_cast<size_t>(std::lower_bound(ids.begin(), ids.end(), t) - ids.begin()); };
+    std::vector<idx_t> tp(ids.size() + 1, 0);
+    for (auto& v : st)
+        for (term_t t : v) tp[col(t) + 1]++;
+    for (size_t t = 0; t < ids.size(); ++t) tp[t + 1] += tp[t];
+    std::vector<uint16_t> cc(nnz);
+    std::vector<uint8_t> cv(nnz * sizeof(T));
+    std::vector<idx_t> cursor(tp.begin(), tp.end() - 1);
+    for (size_t c = 0; c < st.size(); ++c)
+        for (size_t j = 0; j < st[c].size(); ++j) {
+            const idx_t pos = cursor[col(st[c][j])]++;
+            cc[pos] = static_cast<uint16_t>(c);
+            std::memcpy(cv.data() + pos * sizeof(T), &sv[c][j], sizeof(T));
+        }
+    lc.term_ids = std::move(ids);
+    lc.term_ptr = std::move(tp);
+    lc.csc_cluster = std::move(cc);
+    lc.csc_value = std::move(cv);
+}
+
+template <class T, class Acc>
+ListClusters build_one_list(const Fwd& f, const std::vector<idx_t>& kept, int beta, uint32_t seed, float alpha,
+                            int tie_mode) {
+    ListClusters lc;
+    lc.esz = f.esz;
+    // RandomKMeans::train, random_kmeans.cpp:65-78
+    std::vector<std::vector<idx_t>> clusters;
+    if (!kept.empty()) {
+        const size_t nc = boundary_check_n_clusters(kept.size(), static_cast<size_t>(beta));
+        clusters = select_centroids(kept, nc, seed);
+        map_docs_to_clusters<T, Acc>(f, kept, clusters);
+    }
+    // InvertedListClusters(docs), inverted_list_clusters.cpp:193-208
+    if (!clusters.empty()) {
+        lc.offsets.push_back(0);
+        for (auto& c : clusters) {
+            lc.docs.insert(lc.docs.end(), c.begin(), c.end());
+            lc.offsets.push_back(static_cast<idx_t>(lc.docs.size()));
+        }
+    }
+    std::vector<std::vector<term_t>> st;
+    std::vector<std::vector<T>> sv;
+    summarize<T>(f, lc, alpha, tie_mode, st, sv);
+    build_transpose<T>(lc, st, sv);
+    return lc;
+}
+
+// seismic_common.h:146-184
+std::vector<ListClusters> build_lists(const Fwd& f, int lambda_p, int beta_p, float alpha, int seed, int tie_mode,
+                                      int n_threads) {
+    const size_t dim = f.dim;
+    std::vector<std::vector<idx_t>> ldocs(dim);
+    std::vector<std::vector<uint8_t>> lcodes(dim);
+    for (size_t i = 0; i < f.n(); ++i)
+        for (idx_t j = f.indptr[i]; j < f.indptr[i + 1]; ++j) {
+            const term_t t = f.indices[j];
+            if (t >= dim) throw std::invalid_argument("term_id out of range");
+            ldocs[t].push_back(static_cast<idx_t>(i));
+            lcodes[t].insert(lcodes[t].end(), f.values.begin() + static_cast<size_t>(j) * f.esz,
+                             f.values.begin() + static_cast<size_t>(j + 1) * f.esz);
+        }
+    const int lambda = calculate_lambda(lambda_p, f.n());
+    const int beta = calculate_beta(beta_p, lambda);
+    const uint32_t base_seed = seed == -1 ? std::random_device{}() : static_cast<uint32_t>(seed);
+    std::vector<ListClusters> out(dim);
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t idx = 0; idx < static_cast<int64_t>(dim); ++idx) {
+        auto kept = prune(ldocs[idx], lcodes[idx], f.esz, static_cast<size_t>(lambda), tie_mode);
+        const uint32_t s = base_seed + static_cast<uint32_t>(idx);
+        if (f.esz == 4)
+            out[idx] = build_one_list<float, float>(f, kept, beta, s, alpha, tie_mode);
+        else
+            out[idx] = build_one_list<uint8_t, int64_t>(f, kept, beta, s, alpha, tie_mode);
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------- search
+struct Index {
+    bool sq = false;
+    Quantizer q;
+    int lambda = 10, beta = 5, seed = -1;
+    float alpha = 0.5F;
+    Fwd fwd;
+    std::vector<ListClusters> lists;
+    int tie_mode = 0;
+    bool has_vectors = false;
+};
+
+// InvertedListClusters::score_summaries_transposed, inverted_list_clusters.cpp:295-331
+template <class QT>
+void score_summaries(const ListClusters& lc, const term_t* qi, const QT* qv, size_t qn, std::vector<float>& out,
+                     Stats& st) {
+    out.assign(lc.n_clusters, 0.0F);
+    if (lc.n_clusters == 0 || lc.term_ids.empty()) return;
+    const QT* cv = reinterpret_cast<const QT*>(lc.csc_value.data());
+    for (size_t i = 0; i < qn; ++i) {
+        auto it = std::lower_bound(lc.term_ids.begin(), lc.term_ids.end(), qi[i]);
+        if (it == lc.term_ids.end() || *it != qi[i]) continue;
+        const size_t col = static_cast<size_t>(it - lc.term_ids.begin());
+        const float v = static_cast<float>(qv[i]);
+        st.terms_found++;
+        st.csc_entries += lc.term_ptr[col + 1] - lc.term_ptr[col];
+        for (idx_t j = lc.term_ptr[col]; j < lc.term_ptr[col + 1]; ++j)
+            out[lc.csc_cluster[j]] += v * static_cast<float>(cv[j]);
+    }
+}
+
+struct Selector {
+    int kind = -1;  // -1 none, 0 array, 1 set, 2 not, 3 bitmap
+    std::vector<idx_t> ids;
+    std::unordered_set<idx_t> set;
+    const uint32_t* bitmap = nullptr;
+    int64_t nbits = 0;
+    const Selector* inner = nullptr;
+    bool is_member(idx_t id) const {
+        switch (kind) {
+            case 0: return std::find(ids.begin(), ids.end(), id) != ids.end();
+            case 1: return set.count(id) != 0;
+            case 2: return !inner->is_member(id);
+            case 3: return id >= 0 && id < nbits && ((bitmap[id >> 5] >> (id & 31)) & 1U);
+            default: return true;
+        }
+    }
+    bool enumerable() const { return kind == 0 || kind == 1; }
+    size_t size() const { return kind == 0 ? ids.size() : set.size(); }
+};
+
+template <class TopK>
+void finish(TopK& h, int k, bool sq, const Quantizer& iq, const Quantizer& qq, float* d, idx_t* l) {
+    std::vector<float> s;
+    std::vector<idx_t> ids;
+    h.items_descending(s, ids);
+    if (sq)
+        for (auto& x : s) x = iq.decode_dot_product(x, qq);
+    for (int i = 0; i < k; ++i) {
+        d[i] = i < static_cast<int>(s.size()) ? s[i] : -1.0F;
+        l[i] = i < static_cast<int>(ids.size()) ? ids[i] : kInvalid;
+    }
+}
+
+// seismic_scalar_quantized_index.cpp:67-135 / seismic_index.cpp:53-118 and
+// single_query :307-357 / :231-274.
+template <class TopK, class VT>
+void single_query(const Index& ix, const std::vector<VT>& dense, const term_t* qi, const VT* qv, size_t qn,
+                  const std::vector<term_t>& cuts, float hf, const Selector* sel, TopK& heap, Stats& st) {
+    std::unordered_set<idx_t> visited;
+    std::vector<float> scores;
+    bool first_list = true;
+    const Fwd& f = ix.fwd;
+    for (const term_t term : cuts) {
+        if (term >= ix.lists.size()) continue;
+        const ListClusters& lc = ix.lists[term];
+        st.lists++;
+        if (lc.n_clusters != 0) {
+            score_summaries<VT>(lc, qi, qv, qn, scores, st);
+            auto order = reorder_clusters(scores, first_list);
+            for (size_t c : order) {
+                st.clusters_tested++;
+                if (heap.full() && (scores[c] * hf < heap.peek())) {
+                    if (first_list) break;
+                    continue;
+                }
+                st.clusters_opened++;
+                for (idx_t p = lc.offsets[c]; p < lc.offsets[c + 1]; ++p) {
+                    const idx_t d = lc.docs[p];
+                    st.docs_read++;
+                    if (!visited.insert(d).second) continue;
+                    if (sel != nullptr && !sel->is_member(d)) continue;
+                    const idx_t s0 = f.indptr[d], s1 = f.indptr[d + 1];
+                    st.docs_scored++;
+                    st.nnz_scored += s1 - s0;
+                    float score;
+                    if constexpr (std::is_same_v<VT, float>) {
+                        const float* w = reinterpret_cast<const float*>(f.values.data());
+                        float r = 0.0F;
+                        for (idx_t j = s0; j < s1; ++j) r += w[j] * dense[f.indices[j]];
+                        score = r;
+                    } else {
+                        int r = 0;
+                        for (idx_t j = s0; j < s1; ++j) r += f.values[j] * dense[f.indices[j]];
+                        score = static_cast<float>(r);
+                    }
+                    heap.add(score, d);
+                }
+            }
+        }
+        first_list = false;
+    }
+}
+
+// exact_matcher.h:23-41
+template <class TopK>
+void exact_query(const Index& ix, const Selector& sel, const std::vector<uint8_t>& qbytes_dense, int k, TopK& h) {
+    std::vector<idx_t> ids = sel.kind == 0 ? sel.ids : std::vector<idx_t>(sel.set.begin(), sel.set.end());
+    std::ranges::sort(ids);
+    const Fwd& f = ix.fwd;
+    for (idx_t d : ids) {
+        if (d < 0 || static_cast<size_t>(d) >= f.n()) throw std::invalid_argument("selector id out of range");
+        float score;
+        if (f.esz == 4) {
+            const float* w = reinterpret_cast<const float*>(f.values.data());
+            const float* dn = reinterpret_cast<const float*>(qbytes_dense.data());
+            float r = 0.0F;
+            for (idx_t j = f.indptr[d]; j < f.indptr[d + 1]; ++j) r += w[j] * dn[f.indices[j]];
+            score = r;
+        } else {
+            int r = 0;
+            for (idx_t j = f.indptr[d]; j < f.indptr[d + 1]; ++j) r += f.values[j] * qbytes_dense[f.indices[j]];
+            score = static_cast<float>(r);
+        }
+        h.add(score, d);
+    }
+}
+
+template <class TopK>
+void search_impl(const Index& ix, int n, const idx_t* ip, const term_t* qi, const float* qv, int k, int cut, float hf,
+                 const Quantizer& qq, const Selector* sel, float* D, idx_t* L, Stats* stats, int n_threads) {
+    const size_t nnz = ip[n];
+    const bool sq = ix.sq;
+    std::vector<uint8_t> codes;
+    if (sq) {
+        codes.resize(nnz);
+        for (size_t i = 0; i < nnz; ++i) codes[i] = qq.encode(qv[i]);
+    }
+    for (size_t i = 0; i < nnz; ++i)
+        if (qi[i] >= ix.fwd.dim) throw std::invalid_argument("query term id out of range");
+    // exact-match gate: seismic_common.h:117-128
+    if (sel != nullptr && sel->enumerable() && sel->size() <= static_cast<size_t>(k)) {
+        const size_t esz = ix.fwd.esz;
+        for (int qidx = 0; qidx < n; ++qidx) {
+            std::vector<uint8_t> dense(ix.fwd.dim * esz, 0);
+            for (idx_t j = ip[qidx]; j < ip[qidx + 1]; ++j) {
+                if (sq)
+                    dense[qi[j]] = codes[j];
+                else
+                    std::memcpy(dense.data() + static_cast<size_t>(qi[j]) * 4, qv + j, 4);
+            }
+            TopK h(k);
+            exact_query(ix, *sel, dense, k, h);
+            std::vector<float> s;
+            std::vector<idx_t> ids;
+            h.items_descending(s, ids);
+            for (int i = 0; i < k; ++i) {
+                float v = -1.0F;
+                idx_t l = kInvalid;
+                if (i < static_cast<int>(s.size())) {
+                    v = sq ? ix.q.decode_dot_product(s[i], qq) : s[i];
+                    l = ids[i];
+                }
+                D[static_cast<size_t>(qidx) * k + i] = v;
+                L[static_cast<size_t>(qidx) * k + i] = l;
+            }
+        }
+        return;
+    }
+    if (cut <= 0) throw std::invalid_argument("cut must be positive");
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel
+    {
+        std::vector<float> densef;
+        std::vector<uint8_t> denseq;
+        if (sq)
+            denseq.assign(ix.fwd.dim, 0);
+        else
+            densef.assign(ix.fwd.dim, 0.0F);
+#pragma omp for schedule(dynamic, 64)
+        for (int qidx = 0; qidx < n; ++qidx) {
+            const idx_t s0 = ip[qidx];
+            const size_t len = ip[qidx + 1] - s0;
+            Stats st{};
+            TopK heap(k);
+            if (sq) {
+                const uint8_t* qc = codes.data() + s0;
+                auto cuts = top_k_tokens<uint8_t>(qi + s0, qc, static_cast<int>(len), cut);
+                for (size_t i = 0; i < len; ++i) denseq[qi[s0 + i]] = qc[i];
+                single_query<TopK, uint8_t>(ix, denseq, qi + s0, qc, len, cuts, hf, sel, heap, st);
+                for (size_t i = 0; i < len; ++i) denseq[qi[s0 + i]] = 0;
+            } else {
+                const float* qf = qv + s0;
+                auto cuts = top_k_tokens<float>(qi + s0, qf, static_cast<int>(len), cut);
+                for (size_t i = 0; i < len; ++i) densef[qi[s0 + i]] = qf[i];
+                single_query<TopK, float>(ix, densef, qi + s0, qf, len, cuts, hf, sel, heap, st);
+                for (size_t i = 0; i < len; ++i) densef[qi[s0 + i]] = 0.0F;
+            }
+            finish(heap, k, sq, ix.q, qq, D + static_cast<size_t>(qidx) * k, L + static_cast<size_t>(qidx) * k);
+            if (stats != nullptr) stats[qidx] = st;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- writer
+// io/index_io.cpp:88-95, seismic_scalar_quantized_index.cpp:421-428,
+// sparse_vectors.cpp:206-231, io/seismic_invlists_writer.h:36-43,
+// cluster/inverted_list_clusters.cpp:333-363, io/align.h:43-88.
+struct Writer {
+    std::vector<uint8_t> buf;
+    void raw(const void* p, size_t n) {
+        const uint8_t* b = static_cast<const uint8_t*>(p);
+        buf.insert(buf.end(), b, b + n);
+    }
+    template <class T>
+    void scalar(T v) {
+        raw(&v, sizeof(T));
+    }
+    void pad(size_t a) {
+        const size_t n = (a - buf.size() % a) % a;
+        buf.insert(buf.end(), n, 0);
+    }
+    template <class T>
+    void padded(const T* p, size_t count, size_t align = alignof(T)) {
+        pad(align);
+        if (count > 0) raw(p, sizeof(T) * count);
+    }
+};
+
+void write_index(const Index& ix, Writer& w) {
+    const char* id = ix.sq ? "SESQ" : "SEIS";
+    uint32_t fcc = static_cast<uint32_t>(id[0]) | static_cast<uint32_t>(id[1]) << 8 |
+                   static_cast<uint32_t>(id[2]) << 16 | static_cast<uint32_t>(id[3]) << 24;
+    w.scalar<uint32_t>(fcc);
+    w.scalar<int>(static_cast<int>(ix.fwd.dim));
+    if (ix.sq) {
+        w.scalar<uint8_t>(0);  // QuantizerType::QT_8bit
+        w.scalar<float>(ix.q.vmin);
+        w.scalar<float>(ix.q.vmax);
+    }
+    const size_t count = ix.has_vectors ? ix.fwd.n() : 0;
+    w.scalar<size_t>(count);
+    if (count > 0) {
+        w.scalar<size_t>(ix.fwd.dim);
+        w.scalar<size_t>(ix.fwd.esz);
+        w.padded(ix.fwd.indptr.data(), count + 1);
+        const size_t nnz = ix.fwd.indptr[count];
+        w.padded(ix.fwd.indices.data(), nnz);
+        w.padded(ix.fwd.values.data(), nnz * ix.fwd.esz, ix.fwd.esz);
+    }
+    w.scalar<size_t>(ix.lists.size());
+    for (const auto& lc : ix.lists) {
+        w.scalar<size_t>(lc.docs.size());
+        w.padded(lc.docs.data(), lc.docs.size());
+        w.scalar<size_t>(lc.offsets.size());
+        w.padded(lc.offsets.data(), lc.offsets.size());
+        w.scalar<size_t>(lc.n_clusters);
+        w.scalar<size_t>(lc.esz);
+        w.scalar<size_t>(lc.term_ids.size());
+        if (!lc.term_ids.empty()) {
+            w.padded(lc.term_ids.data(), lc.term_ids.size());
+            w.padded(lc.term_ptr.data(), lc.term_ids.size() + 1);
+        }
+        w.scalar<size_t>(lc.csc_cluster.size());
+        w.padded(lc.csc_cluster.data(), lc.csc_cluster.size());
+        w.padded(lc.csc_value.data(), lc.csc_value.size(), lc.esz);
+    }
+}
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::invalid_argument& e) {
+        g_err = e.what();
+        return 1;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 2;
+    }
+}
+
+Selector make_selector(int kind, int64_t n, const idx_t* ids, const uint32_t* bitmap, int negate, Selector& inner) {
+    Selector s;
+    if (kind < 0) return s;
+    Selector base;
+    base.kind = kind;
+    if (kind == 0) base.ids.assign(ids, ids + n);
+    if (kind == 1) base.set.insert(ids, ids + n);
+    if (kind == 3) {
+        base.bitmap = bitmap;
+        base.nbits = n;
+    }
+    if (!negate) return base;
+    inner = std::move(base);
+    s.kind = 2;
+    s.inner = &inner;
+    return s;
+}
+
+}  // namespace oracle
+
+using namespace oracle;
+
+extern "C" {
+
+const char* oracle_last_error() { return g_err.c_str(); }
+
+// kind: 0 = seismic (FP32), 1 = seismic_sq (8-bit).
+int oracle_create(int dim, int sq, float vmin, float vmax, int lambda, int beta, float alpha, int seed, int tie_mode,
+                  void** out) {
+    return guard([&] {
+        auto* ix = new Index;
+        ix->sq = sq != 0;
+        if (ix->sq) ix->q = Quantizer(vmin, vmax);
+        ix->lambda = lambda;
+        ix->beta = beta;
+        ix->alpha = alpha;
+        ix->seed = seed;
+        ix->tie_mode = tie_mode;
+        ix->fwd.esz = ix->sq ? 1 : 4;
+        ix->fwd.dim = static_cast<size_t>(dim);
+        *out = ix;
+    });
+}
+
+void oracle_free(void* h) { delete static_cast<Index*>(h); }
+
+int oracle_add(void* h, int n, const idx_t* indptr, const term_t* indices, const float* values) {
+    return guard([&] {
+        auto* ix = static_cast<Index*>(h);
+        if (n <= 0) throw std::invalid_argument("value must be positive");
+        const size_t nnz = indptr[n];
+        if (ix->sq) {
+            std::vector<uint8_t> codes(nnz);
+            for (size_t i = 0; i < nnz; ++i) codes[i] = ix->q.encode(values[i]);
+            ix->fwd.add(indptr, n + 1, indices, nnz, codes.data());
+        } else {
+            ix->fwd.add(indptr, n + 1, indices, nnz, reinterpret_cast<const uint8_t*>(values));
+        }
+        ix->has_vectors = ix->fwd.n() > 0;
+    });
+}
+
+int oracle_build(void* h, int n_threads) {
+    return guard([&] {
+        auto* ix = static_cast<Index*>(h);
+        ix->lists = build_lists(ix->fwd, ix->lambda, ix->beta, ix->alpha, ix->seed, ix->tie_mode, n_threads);
+    });
+}
+
+// sel_kind: -1 none, 0 array, 1 set, 3 bitmap; sel_negate wraps it in NotIDSelector.
+// qsq: 1 => SeismicSQSearchParameters(qvmin, qvmax, ...), else the index range.
+int oracle_search(void* h, int n, const idx_t* indptr, const term_t* indices, const float* values, int k, int cut,
+                  float hf, int qsq, float qvmin, float qvmax, int sel_kind, int64_t sel_n, const idx_t* sel_ids,
+                  const uint32_t* sel_bitmap, int sel_negate, float* D, idx_t* L, int64_t* stats, int n_threads) {
+    return guard([&] {
+        auto* ix = static_cast<Index*>(h);
+        if (n <= 0 || k <= 0) throw std::invalid_argument("value must be positive");
+        if (!ix->has_vectors) return;
+        Quantizer qq = ix->q;
+        if (ix->sq && qsq) qq = Quantizer(qvmin, qvmax);
+        Selector inner;
+        Selector sel = make_selector(sel_kind, sel_n, sel_ids, sel_bitmap, sel_negate, inner);
+        const Selector* sp = sel.kind < 0 ? nullptr : &sel;
+        auto* st = reinterpret_cast<Stats*>(stats);
+        if (ix->tie_mode == 0)
+            search_impl<HeapTopK<idx_t>>(*ix, n, indptr, indices, values, k, cut, hf, qq, sp, D, L, st, n_threads);
+        else
+            search_impl<IdTopK<idx_t>>(*ix, n, indptr, indices, values, k, cut, hf, qq, sp, D, L, st, n_threads);
+    });
+}
+
+int oracle_serialize(void* h, uint8_t** out, size_t* size) {
+    return guard([&] {
+        Writer w;
+        write_index(*static_cast<Index*>(h), w);
+        *size = w.buf.size();
+        *out = static_cast<uint8_t*>(std::malloc(w.buf.size()));
+        std::memcpy(*out, w.buf.data(), w.buf.size());
+    });
+}
+
+void oracle_buffer_free(uint8_t* p) { std::free(p); }
+
+// ---- component hooks for the pinning tests ----
+
+int oracle_top_k_tokens_u8(const term_t* idx, const uint8_t* w, int n, int k, term_t* out, int* out_n) {
+    return guard([&] {
+        auto r = top_k_tokens<uint8_t>(idx, w, n, k);
+        std::copy(r.begin(), r.end(), out);
+        *out_n = static_cast<int>(r.size());
+    });
+}
+
+int oracle_top_k_tokens_f32(const term_t* idx, const float* w, int n, int k, term_t* out, int* out_n) {
+    return guard([&] {
+        auto r = top_k_tokens<float>(idx, w, n, k);
+        std::copy(r.begin(), r.end(), out);
+        *out_n = static_cast<int>(r.size());
+    });
+}
+
+int oracle_cluster_order(const float* scores, int n, int64_t* out) {
+    return guard([&] {
+        auto o = reorder_clusters(std::vector<float>(scores, scores + n), true);
+        for (int i = 0; i < n; ++i) out[i] = static_cast<int64_t>(o[i]);
+    });
+}
+
+int oracle_sq_encode(float vmin, float vmax, const float* v, size_t n, uint8_t* out) {
+    return guard([&] {
+        Quantizer q(vmin, vmax);
+        for (size_t i = 0; i < n; ++i) out[i] = q.encode(v[i]);
+    });
+}
+
+float oracle_sq_decode(float ivmin, float ivmax, float qvmin, float qvmax, float s) {
+    return Quantizer(ivmin, ivmax).decode_dot_product(s, Quantizer(qvmin, qvmax));
+}
+
+int oracle_select_centroids(int n_docs, int n_clusters, uint32_t seed, int32_t* out_positions) {
+    return guard([&] {
+        std::vector<idx_t> docs(n_docs);
+        std::iota(docs.begin(), docs.end(), 0);
+        auto c = select_centroids(docs, static_cast<size_t>(n_clusters), seed);
+        for (size_t i = 0; i < c.size(); ++i) out_positions[i] = c[i][0];
+    });
+}
+
+uint32_t oracle_mt19937_nth(uint32_t seed, int n) {
+    std::mt19937 g(seed);
+    uint32_t v = 0;
+    for (int i = 0; i < n; ++i) v = static_cast<uint32_t>(g());
+    return v;
+}
+
+int oracle_lambda_beta(int lambda, int beta, int64_t n_vectors, int* out_lambda, int* out_beta) {
+    return guard([&] {
+        *out_lambda = calculate_lambda(lambda, static_cast<size_t>(n_vectors));
+        *out_beta = calculate_beta(beta, *out_lambda);
+    });
+}
+
+}  // extern "C"

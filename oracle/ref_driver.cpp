@@ -1,0 +1,131 @@
+// ORACLE (test infrastructure): a thin C entry layer over the reference's own
+// build-side sources (cluster/, invlists/, sparse_vectors.cpp, io/), compiled
+// straight from /root/reference into oracle/_ref/ by oracle/Makefile. Used only
+// by tests/ to check the restatement in oracle/oracle.cpp. It calls the
+// reference's functions; the only code here is argument plumbing plus the
+// per-list loop of build_inverted_lists_clusters (seismic_common.h:146-184) and
+// the header writes of write_index (io/index_io.cpp:88-95,
+// seismic_scalar_quantized_index.cpp:421-428), which live in files that pull in
+// Abseil (absent here) and so cannot be compiled themselves.
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "nsparse/cluster/inverted_list_clusters.h"
+#include "nsparse/cluster/random_kmeans.h"
+#include "nsparse/invlists/inverted_lists.h"
+#include "nsparse/io/buffered_io.h"
+#include "nsparse/sparse_vectors.h"
+#include "nsparse/utils/distance.h"
+#include "nsparse/utils/scalar_quantizer.h"
+
+using namespace nsparse;
+
+namespace {
+thread_local std::string g_err;
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+const char* ref_last_error() { return g_err.c_str(); }
+
+// Builds a SEIS (sq == 0) or SESQ (sq == 1, 8-bit) index with final lambda and
+// beta (already resolved) and writes its full write_index byte stream.
+int ref_build_serialize(int dim, int sq, float vmin, float vmax, int n, const int32_t* indptr,
+                        const uint16_t* indices, const float* values, int lambda, int beta, float alpha,
+                        uint32_t base_seed, uint8_t** out, size_t* out_size) {
+    return guard([&] {
+        const size_t esz = sq ? 1 : 4;
+        const size_t nnz = indptr[n];
+        SparseVectors vectors({.element_size = esz, .dimension = static_cast<size_t>(dim)});
+        ScalarQuantizer q(QuantizerType::QT_8bit, sq ? vmin : 0.0F, sq ? vmax : 1.0F);
+        std::vector<uint8_t> codes;
+        if (sq) {
+            codes.resize(nnz);
+            q.encode(values, codes.data(), nnz);
+        } else {
+            codes.resize(nnz * 4);
+            std::memcpy(codes.data(), values, nnz * 4);
+        }
+        vectors.add_vectors(indptr, n + 1, indices, nnz, codes.data(), codes.size());
+
+        auto inv = ArrayInvertedLists::build_inverted_lists(dim, esz, &vectors);
+        std::vector<InvertedListClusters> lists(inv->size());
+        for (size_t idx = 0; idx < inv->size(); ++idx) {
+            auto& il = (*inv)[idx];
+            const auto doc_ids = il.prune_and_keep_doc_ids(lambda);
+            InvertedListClusters c(
+                detail::RandomKMeans::train(&vectors, doc_ids, beta, base_seed + static_cast<uint32_t>(idx)));
+            c.summarize(&vectors, alpha);
+            lists[idx] = std::move(c);
+            il.clear();
+        }
+
+        BufferedIOWriter w;
+        const char* id = sq ? "SESQ" : "SEIS";
+        uint32_t fcc = static_cast<uint32_t>(id[0]) | static_cast<uint32_t>(id[1]) << 8 |
+                       static_cast<uint32_t>(id[2]) << 16 | static_cast<uint32_t>(id[3]) << 24;
+        w.write(&fcc, sizeof(uint32_t), 1);
+        int d = dim;
+        w.write(&d, sizeof(int), 1);
+        if (sq) {
+            auto t = q.get_quantizer_type();
+            w.write(&t, sizeof(QuantizerType), 1);
+            float lo = q.get_min(), hi = q.get_max();
+            w.write(&lo, sizeof(float), 1);
+            w.write(&hi, sizeof(float), 1);
+        }
+        vectors.serialize(&w);
+        size_t n_lists = lists.size();
+        w.write(&n_lists, sizeof(size_t), 1);
+        for (const auto& l : lists) l.serialize(&w);
+        *out_size = w.size();
+        *out = static_cast<uint8_t*>(std::malloc(w.size()));
+        std::memcpy(*out, w.data().data(), w.size());
+    });
+}
+
+void ref_buffer_free(uint8_t* p) { std::free(p); }
+
+// InvertedList::prune_and_keep_doc_ids on a hand-made list (inverted_lists.cpp:110-132).
+int ref_prune(int n, const int32_t* docs, const uint8_t* codes, int esz, int64_t lambda, int32_t* out, int* out_n) {
+    return guard([&] {
+        InvertedList il(static_cast<size_t>(esz));
+        il.add_entries(n, docs, codes);
+        auto kept = il.prune_and_keep_doc_ids(static_cast<size_t>(lambda));
+        std::copy(kept.begin(), kept.end(), out);
+        *out_n = static_cast<int>(kept.size());
+    });
+}
+
+// ScalarQuantizer encode / decode_dot_product (utils/scalar_quantizer.h).
+int ref_sq_encode(float vmin, float vmax, const float* v, size_t n, uint8_t* out) {
+    return guard([&] { ScalarQuantizer(QuantizerType::QT_8bit, vmin, vmax).encode(v, out, n); });
+}
+float ref_sq_decode(float ivmin, float ivmax, float qvmin, float qvmax, float s) {
+    ScalarQuantizer a(QuantizerType::QT_8bit, ivmin, ivmax), b(QuantizerType::QT_8bit, qvmin, qvmax);
+    return a.decode_dot_product(s, b);
+}
+
+// detail::dot_product_{uint8,float}_dense (utils/distance.h:29-47), generic build.
+float ref_dot_u8(const uint16_t* idx, const uint8_t* w, size_t len, const uint8_t* dense) {
+    return detail::dot_product_uint8_dense(idx, w, len, dense);
+}
+float ref_dot_f32(const uint16_t* idx, const float* w, size_t len, const float* dense) {
+    return detail::dot_product_float_dense(idx, w, len, dense);
+}
+
+}  // extern "C"
