@@ -127,6 +127,20 @@ int nsparse_search_device(nsparse_index* index, int64_t n, const int32_t* d_indp
                           const int32_t* d_selector_of_query, const uint32_t* const* d_bitmaps,
                           int n_bitmaps, float* d_distances, int32_t* d_labels, void* stream);
 
+/* nsparse_search_device plus per-query traversal counters: d_stats (device,
+ * n x 8 int64) receives lists visited, query entries found in summaries,
+ * summary entries read, prune tests, clusters opened, doc ids read, rows scored
+ * and their non-zeros -- the terms of the algorithmic-byte model. Extension. */
+int nsparse_search_device_stats(nsparse_index* index, int64_t n, const int32_t* d_indptr,
+                                const uint16_t* d_indices, const float* d_values, const int32_t* d_ks,
+                                int k_max, const nsparse_search_params* params, float* d_distances,
+                                int32_t* d_labels, int64_t* d_stats, void* stream);
+
+/* Error code (0 = none) a search kernel left for the last nsparse_search_device*
+ * call on this index: 1 query too long, 2 term id >= dim, 4 per-query k out of range.
+ * Reading it synchronizes the device. Extension. */
+int nsparse_device_error(nsparse_index* index, int32_t* out);
+
 /* write_index(Index*, const char*) -- nsparse/io/index_io.cpp:88-131,170-173.
  * Byte layout identical to the reference writer. */
 int nsparse_write_index(nsparse_index* index, const char* path);

@@ -1,0 +1,28 @@
+/*
+ * Host-side hooks that expose the library's re-implementations of libstdc++
+ * orderings so tests can compare them with std::ranges::sort and
+ * std::priority_queue (no reference counterpart: test surface only).
+ */
+#ifndef NSPARSE_TESTING_H
+#define NSPARSE_TESTING_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Cluster visiting order of the first list (seismic_common.h:105-115):
+ * writes the indices 0..n-1 sorted by keys descending, ties as libstdc++ leaves them. */
+int nsparse_testing_cluster_order(const float* keys, int n, int64_t* out_order);
+
+/* top_k_tokens (utils/vector_process.h:18-30) on float weights: writes the cut
+ * terms best-first and their count. */
+int nsparse_testing_top_k_tokens(const uint16_t* terms, const float* weights, int n, int k, uint16_t* out,
+                                 int* out_n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NSPARSE_TESTING_H */

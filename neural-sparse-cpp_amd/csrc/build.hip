@@ -1,0 +1,1099 @@
+// GPU side of add() and build(): scalar quantization and the SEISMIC index
+// build (build_inverted_lists_clusters, seismic_common.h:146-184) as HIP
+// kernels on one device.
+//
+// Phases (all in HBM; only per-list sizes come back to the host):
+//  1. postings: stable radix sort of the forward index by term (doc order kept
+//     inside a term) -- ArrayInvertedLists::build_inverted_lists;
+//  2. lambda pruning: a second stable sort by (term, value desc) gives each
+//     long list's kept docs in value order, ties by doc id --
+//     InvertedList::prune_and_keep_doc_ids;
+//  3. centroid draws: std::mt19937(seed + list) with libstdc++'s
+//     uniform_int_distribution (Lemire) and the swap-on-a-copy of
+//     random_select_initial_centroids, one lane per list;
+//  4. assignment: one wavefront per document walks the document's terms in CSR
+//     order against a term-major index of the list's centroids, adding into an
+//     LDS similarity row (int for uint8 codes, float in term order for FP32) and
+//     taking the strict-> argmax -- map_docs_to_clusters;
+//  5. clusters: centroid first, then assigned docs in pruned order;
+//  6. summaries: per-cluster max-pool and alpha-mass cut (summarize_*), uint8
+//     via an LDS byte-max table and a 256-bin value histogram, FP32 with the
+//     reference's sequential float sums;
+//  7. transpose: stable sort of the summary entries by (list, term) gives the
+//     term-major CSC of every list -- InvertedListClusters::build_transpose.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <numeric>
+#include <random>
+#include <stdexcept>
+#include <vector>
+
+#include "device_mem.h"
+#include "kernels.h"
+#include "stl_order.h"
+
+namespace nsparse_amd {
+
+namespace {
+
+constexpr int kB = 256;
+
+unsigned grid_for(size_t n, size_t per = kB) {
+    return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>((n + per - 1) / per, 1u << 20)));
+}
+
+#define GRID_STRIDE(i, n) \
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < (n); \
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+
+template <class K, class V>
+void radix_sort_pairs(const K* kin, K* kout, const V* vin, V* vout, size_t n, int end_bit) {
+    if (n == 0) return;
+    size_t tmp = 0;
+    hip_check(rocprim::radix_sort_pairs(nullptr, tmp, kin, kout, vin, vout, n, 0, end_bit), "radix sort (size)");
+    DeviceBuffer t(tmp);
+    hip_check(rocprim::radix_sort_pairs(t.get(), tmp, kin, kout, vin, vout, n, 0, end_bit), "radix sort");
+}
+
+// out[i] = sum(in[0..i)), out[n] = total; `in` and `out` are int64.
+void exclusive_scan_i64(const int64_t* in, int64_t* out, size_t n) {
+    if (n == 0) {
+        hip_check(hipMemset(out, 0, 8), "hipMemset");
+        return;
+    }
+    size_t tmp = 0;
+    hip_check(rocprim::inclusive_scan(nullptr, tmp, in, out + 1, n, rocprim::plus<int64_t>()), "scan (size)");
+    DeviceBuffer t(tmp);
+    hip_check(rocprim::inclusive_scan(t.get(), tmp, in, out + 1, n, rocprim::plus<int64_t>()), "scan");
+    hip_check(hipMemset(out, 0, 8), "hipMemset");
+}
+
+void sync(const char* what) {
+    hip_check(hipGetLastError(), what);
+    hip_check(hipDeviceSynchronize(), what);
+}
+
+// ------------------------------------------------------------------ kernels
+
+// ScalarQuantizer::encode_8bit (utils/scalar_quantizer.h:97-101): the scale is
+// computed on the host as `255.0f / (vmax - vmin)`, exactly as the reference
+// evaluates it; the device applies (v - vmin) * scale, clamps with
+// std::min/std::max semantics and rounds half away from zero (lround).
+__global__ void sq_encode_kernel(const float* __restrict__ v, uint8_t* __restrict__ out, size_t n, float vmin,
+                                 float scale) {
+    GRID_STRIDE(i, n) {
+        float s = (v[i] - vmin) * scale;
+        s = (s < 255.0F) ? s : 255.0F;
+        s = (0.0F < s) ? s : 0.0F;
+        out[i] = static_cast<uint8_t>(static_cast<uint32_t>(roundf(s)));
+    }
+}
+
+__global__ void iota_kernel(int32_t* v, size_t n) {
+    GRID_STRIDE(i, n) v[i] = static_cast<int32_t>(i);
+}
+
+__global__ void term_hist_kernel(const uint16_t* __restrict__ idx, size_t n, int32_t* __restrict__ hist) {
+    GRID_STRIDE(i, n) atomicAdd(&hist[idx[i]], 1);
+}
+
+__global__ void doc_of_kernel(const int32_t* __restrict__ indptr, int64_t n_docs, int32_t* __restrict__ doc_of) {
+    GRID_STRIDE(d, static_cast<size_t>(n_docs)) {
+        for (int32_t p = indptr[d]; p < indptr[d + 1]; ++p) doc_of[p] = static_cast<int32_t>(d);
+    }
+}
+
+// Pruning key: term in the high bits, value descending below it.
+__global__ void prune_key_u8_kernel(const uint16_t* __restrict__ idx, const uint8_t* __restrict__ v, size_t n,
+                                    uint32_t* __restrict__ key) {
+    GRID_STRIDE(i, n) key[i] = (static_cast<uint32_t>(idx[i]) << 8) | (255u - v[i]);
+}
+
+__device__ __forceinline__ uint32_t float_desc_bits(float f) {
+    if (f == 0.0F) f = 0.0F;  // -0.0 and +0.0 compare equal in the reference
+    const uint32_t b = __float_as_uint(f);
+    const uint32_t asc = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    return ~asc;
+}
+
+__global__ void prune_key_f32_kernel(const uint16_t* __restrict__ idx, const float* __restrict__ v, size_t n,
+                                     uint64_t* __restrict__ key) {
+    GRID_STRIDE(i, n) key[i] = (static_cast<uint64_t>(idx[i]) << 32) | float_desc_bits(v[i]);
+}
+
+// kept docs of every list: doc-ordered postings when the whole list is kept,
+// value-ordered prefix otherwise.
+__global__ void kept_docs_kernel(const int64_t* __restrict__ list_start, const int64_t* __restrict__ kept_base,
+                                 const int32_t* __restrict__ spos, const int32_t* __restrict__ spos2,
+                                 const int32_t* __restrict__ doc_of, int32_t* __restrict__ kept, int n_lists) {
+    for (int l = blockIdx.x; l < n_lists; l += gridDim.x) {
+        const int64_t s = list_start[l], len = list_start[l + 1] - s;
+        const int64_t kb = kept_base[l], kl = kept_base[l + 1] - kb;
+        const int32_t* src = (kl < len) ? spos2 : spos;
+        for (int64_t i = threadIdx.x; i < kl; i += blockDim.x) kept[kb + i] = doc_of[src[s + i]];
+    }
+}
+
+// std::mt19937 (standard-specified) and libstdc++'s uniform_int_distribution
+// for a 32-bit engine (Lemire's nearly-divisionless method with a 64-bit
+// product, bits/uniform_int_dist.h _S_nd).
+struct Mt19937 {
+    uint32_t* mt;
+    int idx;
+    __device__ void seed(uint32_t s) {
+        mt[0] = s;
+        for (int i = 1; i < 624; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + static_cast<uint32_t>(i);
+        idx = 624;
+    }
+    __device__ uint32_t next() {
+        if (idx >= 624) {
+            for (int i = 0; i < 624; ++i) {
+                const uint32_t y = (mt[i] & 0x80000000u) | (mt[(i + 1) % 624] & 0x7fffffffu);
+                mt[i] = mt[(i + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+            }
+            idx = 0;
+        }
+        uint32_t y = mt[idx++];
+        y ^= y >> 11;
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= y >> 18;
+        return y;
+    }
+    // uniform_int_distribution<int>(0, range - 1)
+    __device__ uint32_t below(uint32_t range) {
+        uint64_t product = static_cast<uint64_t>(next()) * range;
+        uint32_t low = static_cast<uint32_t>(product);
+        if (low < range) {
+            const uint32_t threshold = (0u - range) % range;
+            while (low < threshold) {
+                product = static_cast<uint64_t>(next()) * range;
+                low = static_cast<uint32_t>(product);
+            }
+        }
+        return static_cast<uint32_t>(product >> 32);
+    }
+};
+
+// random_select_initial_centroids (random_kmeans.cpp:28-51) on a copy of the
+// kept docs; writes centroid doc ids in draw order.
+__global__ void centroid_kernel(const int64_t* __restrict__ kept_base, const int32_t* __restrict__ kept,
+                                int32_t* __restrict__ work, const int32_t* __restrict__ lcb,
+                                int32_t* __restrict__ centroid_doc, uint32_t base_seed, int n_lists) {
+    __shared__ uint32_t state[624];
+    for (int l = blockIdx.x; l < n_lists; l += gridDim.x) {
+        const int64_t kb = kept_base[l];
+        const int64_t n = kept_base[l + 1] - kb;
+        const int nc = lcb[l + 1] - lcb[l];
+        if (n == 0 || nc == 0) continue;
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) work[kb + i] = kept[kb + i];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            Mt19937 g{state, 0};
+            g.seed(base_seed + static_cast<uint32_t>(l));
+            for (int i = 0; i < nc; ++i) {
+                const uint32_t r = g.below(static_cast<uint32_t>(n - i));
+                const int32_t d = work[kb + r];
+                centroid_doc[lcb[l] + i] = d;
+                work[kb + r] = work[kb + n - 1 - i];
+                work[kb + n - 1 - i] = d;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void cluster_list_kernel(const int32_t* __restrict__ lcb, int32_t* __restrict__ cluster_list, int n_lists) {
+    for (int l = blockIdx.x; l < n_lists; l += gridDim.x)
+        for (int c = lcb[l] + threadIdx.x; c < lcb[l + 1]; c += blockDim.x) cluster_list[c] = l;
+}
+
+__global__ void row_len_kernel(const int32_t* __restrict__ indptr, const int32_t* __restrict__ docs, size_t n,
+                               int64_t* __restrict__ out) {
+    GRID_STRIDE(i, n) out[i] = indptr[docs[i] + 1] - indptr[docs[i]];
+}
+
+// One posting per (centroid term): key (list, term), value = entry id.
+template <class T>
+__global__ void centroid_entries_kernel(const int32_t* __restrict__ indptr, const uint16_t* __restrict__ indices,
+                                        const T* __restrict__ values, const int32_t* __restrict__ centroid_doc,
+                                        const int32_t* __restrict__ cluster_list, const int32_t* __restrict__ lcb,
+                                        const int64_t* __restrict__ cent_off, size_t n_clusters,
+                                        uint32_t* __restrict__ key, int32_t* __restrict__ val,
+                                        uint16_t* __restrict__ ent_c, T* __restrict__ ent_w) {
+    GRID_STRIDE(g, n_clusters) {
+        const int32_t d = centroid_doc[g];
+        const int l = cluster_list[g];
+        const uint16_t c = static_cast<uint16_t>(g - lcb[l]);
+        int64_t e = cent_off[g];
+        for (int32_t p = indptr[d]; p < indptr[d + 1]; ++p, ++e) {
+            key[e] = (static_cast<uint32_t>(l) << 16) | indices[p];
+            val[e] = static_cast<int32_t>(e);
+            ent_c[e] = c;
+            ent_w[e] = values[p];
+        }
+    }
+}
+
+template <class T>
+__global__ void gather_csc_kernel(const int32_t* __restrict__ perm, const uint16_t* __restrict__ ent_c,
+                                  const T* __restrict__ ent_w, size_t n, uint16_t* __restrict__ c_out,
+                                  T* __restrict__ w_out) {
+    GRID_STRIDE(i, n) {
+        c_out[i] = ent_c[perm[i]];
+        w_out[i] = ent_w[perm[i]];
+    }
+}
+
+__global__ void fill_i32_kernel(int32_t* p, size_t n, int32_t v) {
+    GRID_STRIDE(i, n) p[i] = v;
+}
+
+template <class Acc>
+__device__ __forceinline__ void argmax_merge(Acc& v, int& i, Acc ov, int oi) {
+    if (ov > v || (ov == v && oi < i)) {
+        v = ov;
+        i = oi;
+    }
+}
+
+// map_docs_to_clusters_typed (cluster/kmeans_utils.cpp:107-158). One
+// workgroup per list (grid-stride), one wavefront per document. `tmap` is this
+// workgroup's dense term -> [begin, end) table of the list's centroid CSC.
+template <class T, class Acc>
+__global__ __launch_bounds__(kB) void assign_kernel(
+    const int32_t* __restrict__ indptr, const uint16_t* __restrict__ indices, const T* __restrict__ values,
+    const int64_t* __restrict__ kept_base, const int32_t* __restrict__ kept, const int32_t* __restrict__ lcb,
+    const int32_t* __restrict__ centroid_doc, const int64_t* __restrict__ cent_off, const uint32_t* __restrict__ ckey,
+    const uint16_t* __restrict__ cc, const T* __restrict__ cw, int2* __restrict__ tmap_all, int dim,
+    int32_t* __restrict__ assign, int n_lists, int mc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    int32_t* cent = reinterpret_cast<int32_t*>(smem);                    // sorted centroid ids [mc]
+    Acc* sims_all = reinterpret_cast<Acc*>(smem + ((mc * 4 + 15) & ~15));  // [4][mc]
+    int2* tmap = tmap_all + static_cast<size_t>(blockIdx.x) * dim;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    Acc* sims = sims_all + wave * mc;
+    for (int l = blockIdx.x; l < n_lists; l += gridDim.x) {
+        const int64_t kb = kept_base[l], n = kept_base[l + 1] - kb;
+        const int c0 = lcb[l], nc = lcb[l + 1] - c0;
+        if (n == 0 || nc == 0) continue;
+        const int64_t e0 = cent_off[c0], e1 = cent_off[c0 + nc];
+        for (int64_t e = e0 + tid; e < e1; e += kB) {
+            const uint32_t k = ckey[e];
+            const int t = static_cast<int>(k & 0xFFFFu);
+            if (e == e0 || ckey[e - 1] != k) tmap[t].x = static_cast<int>(e);
+            if (e == e1 - 1 || ckey[e + 1] != k) tmap[t].y = static_cast<int>(e + 1);
+        }
+        // centroid ids, sorted (odd-even merge over the small array)
+        for (int i = tid; i < nc; i += kB) cent[i] = centroid_doc[c0 + i];
+        __syncthreads();
+        for (int pass = 0; pass < nc; ++pass) {
+            for (int i = 2 * tid + (pass & 1); i + 1 < nc; i += 2 * kB) {
+                const int32_t a = cent[i], b = cent[i + 1];
+                if (a > b) {
+                    cent[i] = b;
+                    cent[i + 1] = a;
+                }
+            }
+            __syncthreads();
+        }
+        for (int64_t j = wave; j < n; j += kB / 64) {
+            const int32_t d = kept[kb + j];
+            int lo = 0, hi = nc;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (cent[mid] < d)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            if (lo < nc && cent[lo] == d) {
+                if (lane == 0) assign[kb + j] = -1;
+                continue;
+            }
+            for (int c = lane; c < nc; c += 64) sims[c] = Acc(0);
+            __builtin_amdgcn_wave_barrier();
+            const int32_t r0 = indptr[d], r1 = indptr[d + 1];
+            for (int b = r0; b < r1; b += 64) {
+                const int p = b + lane;
+                int2 col = make_int2(0, 0);
+                Acc dv = Acc(0);
+                if (p < r1) {
+                    col = tmap[indices[p]];
+                    dv = static_cast<Acc>(values[p]);
+                }
+                const int m = min(64, r1 - b);
+                for (int j2 = 0; j2 < m; ++j2) {
+                    const int cs = __shfl(col.x, j2, 64), ce = __shfl(col.y, j2, 64);
+                    const Acc v = __shfl(dv, j2, 64);
+                    if (cs < 0) continue;
+                    for (int e = cs + lane; e < ce; e += 64) sims[cc[e]] += v * static_cast<Acc>(cw[e]);
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            Acc best = sims[0];
+            int bi = 0;
+            for (int c = lane; c < nc; c += 64) argmax_merge(best, bi, sims[c], c);
+            for (int off = 32; off > 0; off >>= 1) {
+                const Acc ov = __shfl_xor(best, off, 64);
+                const int oi = __shfl_xor(bi, off, 64);
+                argmax_merge(best, bi, ov, oi);
+            }
+            if (lane == 0) assign[kb + j] = bi;
+            __builtin_amdgcn_wave_barrier();
+        }
+        __syncthreads();
+        for (int64_t e = e0 + tid; e < e1; e += kB) tmap[ckey[e] & 0xFFFFu] = make_int2(-1, -1);
+        __syncthreads();
+    }
+}
+
+// Cluster sizes: centroid + assigned docs (per global cluster).
+__global__ void cluster_count_kernel(const int64_t* __restrict__ kept_base, const int32_t* __restrict__ assign,
+                                     const int32_t* __restrict__ lcb, int64_t* __restrict__ count, int n_lists) {
+    for (int l = blockIdx.x; l < n_lists; l += gridDim.x) {
+        const int c0 = lcb[l], nc = lcb[l + 1] - c0;
+        for (int c = threadIdx.x; c < nc; c += blockDim.x) count[c0 + c] = 1;
+        __syncthreads();
+        const int64_t kb = kept_base[l], n = kept_base[l + 1] - kb;
+        for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
+            const int a = assign[kb + j];
+            if (a >= 0) atomicAdd(reinterpret_cast<unsigned long long*>(&count[c0 + a]), 1ULL);
+        }
+        __syncthreads();
+    }
+}
+
+// Stable placement: cluster c = [centroid_c, its docs in pruned order].
+__global__ __launch_bounds__(64) void cluster_place_kernel(const int64_t* __restrict__ kept_base,
+                                                           const int32_t* __restrict__ kept,
+                                                           const int32_t* __restrict__ assign,
+                                                           const int32_t* __restrict__ lcb,
+                                                           const int32_t* __restrict__ centroid_doc,
+                                                           const int64_t* __restrict__ cdp, int32_t* __restrict__ docs,
+                                                           int n_lists) {
+    __shared__ int32_t fill[4096];
+    const int lane = threadIdx.x;
+    for (int l = blockIdx.x; l < n_lists; l += gridDim.x) {
+        const int c0 = lcb[l], nc = lcb[l + 1] - c0;
+        for (int c = lane; c < nc; c += 64) {
+            docs[cdp[c0 + c]] = centroid_doc[c0 + c];
+            fill[c] = 1;
+        }
+        __builtin_amdgcn_wave_barrier();
+        const int64_t kb = kept_base[l], n = kept_base[l + 1] - kb;
+        for (int64_t b = 0; b < n; b += 64) {
+            const int64_t j = b + lane;
+            const int a = j < n ? assign[kb + j] : -1;
+            int before = 0, after = 0;
+            for (int o = 0; o < 64; ++o) {
+                const int oa = __shfl(a, o, 64);
+                if (a >= 0 && oa == a) {
+                    if (o < lane) ++before;
+                    if (o > lane) ++after;
+                }
+            }
+            int64_t base = 0;
+            if (a >= 0) base = fill[a];
+            __builtin_amdgcn_wave_barrier();
+            if (a >= 0) {
+                docs[cdp[c0 + a] + base + before] = kept[kb + j];
+                if (after == 0) fill[a] = static_cast<int32_t>(base + before + 1);
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// ---- uint8 summaries: summarize_with_cpu_<uint8_t> + summarize_emit_cluster_
+// (cluster/inverted_list_clusters.cpp:37-124). With integer values the running
+// float sum is exact, so the sum is the sum of per-term maxima; the alpha cut
+// is found from a 256-bin histogram of the maxima, and ties at the cut are
+// kept in term order.
+struct SumLds8 {
+    uint32_t* maxw;   // [nw*8] four uint8 maxima per word
+    uint32_t* touch;  // [nw]
+    int32_t* hist;    // [256]
+    int32_t* wscan;   // [nw] per-word kept counts, scanned
+    int32_t* wtie;    // [nw] cut-valued terms before each word
+    int32_t* misc;    // [8]
+};
+
+__device__ __forceinline__ void byte_max(uint32_t* w, int t, uint32_t v) {
+    uint32_t* word = w + (t >> 2);
+    const int sh = (t & 3) * 8;
+    uint32_t old = *word;
+    while (((old >> sh) & 0xFFu) < v) {
+        const uint32_t nw = (old & ~(0xFFu << sh)) | (v << sh);
+        const uint32_t prev = atomicCAS(word, old, nw);
+        if (prev == old) break;
+        old = prev;
+    }
+}
+
+__device__ __forceinline__ uint32_t max_of(const uint32_t* w, int t) { return (w[t >> 2] >> ((t & 3) * 8)) & 0xFFu; }
+
+template <bool WRITE>
+__global__ __launch_bounds__(kB) void summarize_u8_kernel(const int32_t* __restrict__ indptr,
+                                                          const uint16_t* __restrict__ indices,
+                                                          const uint8_t* __restrict__ values,
+                                                          const int64_t* __restrict__ cdp,
+                                                          const int32_t* __restrict__ docs, const int32_t* __restrict__ cluster_list,
+                                                          const int32_t* __restrict__ lcb, int64_t n_clusters, int nw,
+                                                          float alpha, int64_t* __restrict__ count,
+                                                          const int64_t* __restrict__ out_off,
+                                                          uint16_t* __restrict__ out_term, uint8_t* __restrict__ out_val,
+                                                          uint16_t* __restrict__ out_cl) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    SumLds8 S;
+    S.maxw = reinterpret_cast<uint32_t*>(smem);
+    S.touch = S.maxw + nw * 8;
+    S.hist = reinterpret_cast<int32_t*>(S.touch + nw);
+    S.wscan = S.hist + 256;
+    S.wtie = S.wscan + nw;
+    S.misc = S.wtie + nw;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int w = tid; w < nw * 8; w += kB) S.maxw[w] = 0;
+    for (int w = tid; w < nw; w += kB) S.touch[w] = 0;
+    __syncthreads();
+    for (int64_t g = blockIdx.x; g < n_clusters; g += gridDim.x) {
+        for (int i = tid; i < 256; i += kB) S.hist[i] = 0;
+        // max-pool over the cluster's docs
+        for (int64_t q = cdp[g] + wave; q < cdp[g + 1]; q += kB / 64) {
+            const int32_t d = docs[q];
+            for (int32_t p = indptr[d] + lane; p < indptr[d + 1]; p += 64) {
+                const int t = indices[p];
+                atomicOr(&S.touch[t >> 5], 1u << (t & 31));
+                byte_max(S.maxw, t, values[p]);
+            }
+        }
+        __syncthreads();
+        // histogram of maxima, exact integer sum
+        int local_sum = 0;
+        for (int w = tid; w < nw; w += kB) {
+            uint32_t bits = S.touch[w];
+            while (bits) {
+                const int b = __ffs(bits) - 1;
+                bits &= bits - 1;
+                const uint32_t v = max_of(S.maxw, w * 32 + b);
+                atomicAdd(&S.hist[v], 1);
+                local_sum += static_cast<int>(v);
+            }
+        }
+        if (tid == 0) S.misc[0] = 0;
+        __syncthreads();
+        atomicAdd(&S.misc[0], local_sum);
+        __syncthreads();
+        if (tid == 0) {
+            // pairs sorted by value desc; addup += value; stop at the first
+            // addup / sum >= alpha (inverted_list_clusters.cpp:43-50).
+            const float sum = static_cast<float>(S.misc[0]);
+            int cut_v = -1, cut_x = 0;  // keep values > cut_v, plus cut_x terms == cut_v
+            int64_t a = 0;
+            for (int v = 255; v >= 0; --v) {
+                const int m = S.hist[v];
+                if (m == 0) continue;
+                const float full = static_cast<float>(a + static_cast<int64_t>(v) * m);
+                if (full / sum >= alpha) {
+                    int lo = 1, hi = m;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (static_cast<float>(a + static_cast<int64_t>(v) * mid) / sum >= alpha)
+                            hi = mid;
+                        else
+                            lo = mid + 1;
+                    }
+                    cut_v = v;
+                    cut_x = lo;
+                    break;
+                }
+                a += static_cast<int64_t>(v) * m;
+            }
+            S.misc[1] = cut_v;
+            S.misc[2] = cut_x;
+        }
+        __syncthreads();
+        const int cut_v = S.misc[1], cut_x = S.misc[2];
+        // rank of each cut-valued term in term order -> per-word counts, scan
+        // (single wave, word order), then per-word kept counts and output.
+        if (wave == 0) {
+            int run_tie = 0, run_keep = 0;
+            for (int b = 0; b < nw; b += 64) {
+                const int w = b + lane;
+                int ties = 0;
+                uint32_t bits = w < nw ? S.touch[w] : 0u;
+                uint32_t tb = bits;
+                while (tb) {
+                    const int bb = __ffs(tb) - 1;
+                    tb &= tb - 1;
+                    if (static_cast<int>(max_of(S.maxw, w * 32 + bb)) == cut_v) ++ties;
+                }
+                int incl = ties;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int x = __shfl_up(incl, o, 64);
+                    if (lane >= o) incl += x;
+                }
+                int tie_rank = run_tie + incl - ties;
+                int keep = 0;
+                tb = bits;
+                while (tb) {
+                    const int bb = __ffs(tb) - 1;
+                    tb &= tb - 1;
+                    const int v = static_cast<int>(max_of(S.maxw, w * 32 + bb));
+                    if (v > cut_v)
+                        ++keep;
+                    else if (v == cut_v) {
+                        if (tie_rank < cut_x || cut_v < 0) ++keep;
+                        ++tie_rank;
+                    }
+                }
+                int kincl = keep;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int x = __shfl_up(kincl, o, 64);
+                    if (lane >= o) kincl += x;
+                }
+                if (w < nw) {
+                    S.wscan[w] = run_keep + kincl - keep;
+                    S.wtie[w] = run_tie + incl - ties;
+                }
+                run_tie += __shfl(incl, 63, 64);
+                run_keep += __shfl(kincl, 63, 64);
+            }
+            if (lane == 0) S.misc[3] = run_keep;
+        }
+        __syncthreads();
+        if (!WRITE) {
+            if (tid == 0) count[g] = S.misc[3];
+        } else {
+            const int64_t base = out_off[g];
+            const uint16_t local_c = static_cast<uint16_t>(g - lcb[cluster_list[g]]);
+            // recompute tie ranks per word from the scanned tie counts
+            for (int w = tid; w < nw; w += kB) {
+                uint32_t bits = S.touch[w];
+                if (!bits) continue;
+                int pos = S.wscan[w];
+                int tie_before = S.wtie[w];
+                while (bits) {
+                    const int bb = __ffs(bits) - 1;
+                    bits &= bits - 1;
+                    const int t = w * 32 + bb;
+                    const int v = static_cast<int>(max_of(S.maxw, t));
+                    bool keep = v > cut_v;
+                    if (v == cut_v) {
+                        keep = cut_v < 0 || tie_before < cut_x;
+                        ++tie_before;
+                    }
+                    if (keep) {
+                        out_term[base + pos] = static_cast<uint16_t>(t);
+                        out_val[base + pos] = static_cast<uint8_t>(v);
+                        out_cl[base + pos] = local_c;
+                        ++pos;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // clear the touched words for the next cluster
+        for (int w = tid; w < nw; w += kB) {
+            if (S.touch[w]) {
+                S.touch[w] = 0;
+                for (int j = 0; j < 8; ++j) S.maxw[w * 8 + j] = 0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---- FP32 summaries: the reference's sequential algorithm, one lane per
+// cluster step, LDS dense accumulator (order-sensitive float sums kept exact).
+__global__ __launch_bounds__(kB) void summarize_f32_kernel(const int32_t* __restrict__ indptr,
+                                                           const uint16_t* __restrict__ indices,
+                                                           const float* __restrict__ values,
+                                                           const int64_t* __restrict__ cdp,
+                                                           const int32_t* __restrict__ docs,
+                                                           const int32_t* __restrict__ cluster_list,
+                                                           const int32_t* __restrict__ lcb, int64_t n_clusters, int dim,
+                                                           int cap, float alpha, int64_t* __restrict__ count,
+                                                           const int64_t* __restrict__ out_off, bool write,
+                                                           uint16_t* __restrict__ out_term, float* __restrict__ out_val,
+                                                           uint16_t* __restrict__ out_cl, int32_t* __restrict__ err) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nw = (dim + 31) / 32;
+    float* acc = reinterpret_cast<float*>(smem);
+    uint32_t* touch = reinterpret_cast<uint32_t*>(acc + dim);
+    uint32_t* keepb = touch + nw;
+    const size_t pairs_at = (static_cast<size_t>(dim) * 4 + static_cast<size_t>(nw) * 8 + 7) & ~size_t{7};
+    KeyItem* pairs = reinterpret_cast<KeyItem*>(smem + pairs_at);
+    int32_t* misc = reinterpret_cast<int32_t*>(pairs + cap);
+    const int tid = threadIdx.x;
+    for (int w = tid; w < nw; w += kB) {
+        touch[w] = 0;
+        keepb[w] = 0;
+    }
+    __syncthreads();
+    for (int64_t g = blockIdx.x; g < n_clusters; g += gridDim.x) {
+        if (tid == 0) {
+            float sum = 0.0F;
+            int m = 0;
+            for (int64_t q = cdp[g]; q < cdp[g + 1]; ++q) {
+                const int32_t d = docs[q];
+                for (int32_t p = indptr[d]; p < indptr[d + 1]; ++p) {
+                    const int t = indices[p];
+                    const float v = values[p];
+                    if (!((touch[t >> 5] >> (t & 31)) & 1u)) {
+                        touch[t >> 5] |= 1u << (t & 31);
+                        const float value = (0.0F < v) ? v : 0.0F;
+                        acc[t] = value;
+                        sum += value;
+                        if (m < cap) pairs[m] = KeyItem{0.0F, static_cast<uint32_t>(t)};
+                        ++m;
+                    } else if (v > acc[t]) {
+                        sum += v - acc[t];
+                        acc[t] = v;
+                    }
+                }
+            }
+            misc[0] = m;
+            reinterpret_cast<float*>(misc)[1] = sum;
+            if (m > cap) atomicExch(err, 1);
+        }
+        __syncthreads();
+        const int m = min(misc[0], cap);
+        const float sum = reinterpret_cast<float*>(misc)[1];
+        for (int i = tid; i < m; i += kB) pairs[i].key = acc[pairs[i].id];
+        // bitonic sort of (value desc, term asc) over the next power of two
+        int np = 1;
+        while (np < m) np <<= 1;
+        for (int i = m + tid; i < np; i += kB) pairs[i] = KeyItem{-__builtin_inff(), 0xFFFFFFFFu};
+        __syncthreads();
+        for (int size = 2; size <= np; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int i = tid; i < np; i += kB) {
+                    const int j = i ^ stride;
+                    if (j > i) {
+                        const KeyItem a = pairs[i], b = pairs[j];
+                        const bool a_first = a.key > b.key || (a.key == b.key && a.id < b.id);
+                        const bool up = (i & size) == 0;
+                        if (up ? !a_first : a_first) {
+                            pairs[i] = b;
+                            pairs[j] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (tid == 0) {
+            float addup = 0.0F;
+            int keep_n = m;
+            for (int j = 0; j < m; ++j) {
+                addup += pairs[j].key;
+                if (addup / sum >= alpha) {
+                    keep_n = j + 1;
+                    break;
+                }
+            }
+            misc[2] = keep_n;
+        }
+        __syncthreads();
+        const int keep_n = misc[2];
+        for (int i = tid; i < keep_n; i += kB) atomicOr(&keepb[pairs[i].id >> 5], 1u << (pairs[i].id & 31));
+        __syncthreads();
+        if (!write) {
+            if (tid == 0) count[g] = keep_n;
+        } else if (tid == 0) {
+            int64_t pos = out_off[g];
+            const uint16_t local_c = static_cast<uint16_t>(g - lcb[cluster_list[g]]);
+            for (int w = 0; w < nw; ++w) {
+                uint32_t bits = keepb[w];
+                while (bits) {
+                    const int b = __ffs(bits) - 1;
+                    bits &= bits - 1;
+                    const int t = w * 32 + b;
+                    out_term[pos] = static_cast<uint16_t>(t);
+                    out_val[pos] = acc[t];
+                    out_cl[pos] = local_c;
+                    ++pos;
+                }
+            }
+        }
+        __syncthreads();
+        for (int w = tid; w < nw; w += kB) {
+            touch[w] = 0;
+            keepb[w] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void summary_key_kernel(const uint16_t* __restrict__ term, const int64_t* __restrict__ out_off,
+                                   const int32_t* __restrict__ cluster_list, int64_t n_clusters,
+                                   uint32_t* __restrict__ key, int32_t* __restrict__ val) {
+    GRID_STRIDE(g, static_cast<size_t>(n_clusters)) {
+        const uint32_t l = static_cast<uint32_t>(cluster_list[g]);
+        for (int64_t e = out_off[g]; e < out_off[g + 1]; ++e) {
+            key[e] = (l << 16) | term[e];
+            val[e] = static_cast<int32_t>(e);
+        }
+    }
+}
+
+template <class T>
+__global__ void gather_summary_kernel(const int32_t* __restrict__ perm, const uint16_t* __restrict__ cl,
+                                      const T* __restrict__ v, size_t n, uint16_t* __restrict__ cl_out,
+                                      T* __restrict__ v_out) {
+    GRID_STRIDE(i, n) {
+        cl_out[i] = cl[perm[i]];
+        v_out[i] = v[perm[i]];
+    }
+}
+
+__global__ void head_flag_kernel(const uint32_t* __restrict__ key, size_t n, int64_t* __restrict__ flag) {
+    GRID_STRIDE(i, n) flag[i] = (i == 0 || key[i] != key[i - 1]) ? 1 : 0;
+}
+
+__global__ void head_write_kernel(const uint32_t* __restrict__ key, const int64_t* __restrict__ hscan, size_t n,
+                                  uint16_t* __restrict__ term_ids, int64_t* __restrict__ term_ptr) {
+    GRID_STRIDE(i, n) {
+        if (i == 0 || key[i] != key[i - 1]) {
+            const int64_t h = hscan[i];
+            term_ids[h] = static_cast<uint16_t>(key[i] & 0xFFFFu);
+            term_ptr[h] = static_cast<int64_t>(i);
+        }
+    }
+}
+
+__global__ void list_bases_kernel(const int32_t* __restrict__ lcb, const int64_t* __restrict__ out_off,
+                                  const int64_t* __restrict__ hscan, int64_t E, int64_t T, int n_lists,
+                                  int64_t* __restrict__ leb, int32_t* __restrict__ ltb) {
+    GRID_STRIDE(l, static_cast<size_t>(n_lists) + 1) {
+        const int64_t e = out_off[lcb[l]];
+        leb[l] = e;
+        ltb[l] = static_cast<int32_t>(e < E ? hscan[e] : T);
+    }
+}
+
+int calc_lambda(int lambda, int64_t n) {
+    if (lambda == -1) return std::max(static_cast<int>(0.0005F * static_cast<float>(n)), 160);
+    return lambda;
+}
+int calc_beta(int beta, int lambda) {
+    if (beta == -1) return static_cast<int>(static_cast<float>(lambda) * 0.1F);
+    return beta;
+}
+
+}  // namespace
+
+void encode_sq_on_device(const float* host_values, size_t n, float vmin, float vmax, uint8_t* host_codes) {
+    if (n == 0) return;
+    const float scale = 255.0F / (vmax - vmin);
+    constexpr size_t kChunk = size_t{1} << 28;
+    DeviceBuffer dv(std::min(n, kChunk) * 4), dc(std::min(n, kChunk));
+    for (size_t off = 0; off < n; off += kChunk) {
+        const size_t m = std::min(kChunk, n - off);
+        hip_check(hipMemcpy(dv.get(), host_values + off, m * 4, hipMemcpyHostToDevice), "hipMemcpy H2D");
+        hipLaunchKernelGGL(sq_encode_kernel, dim3(grid_for(m)), dim3(kB), 0, nullptr, dv.get<float>(),
+                           dc.get<uint8_t>(), m, vmin, scale);
+        hip_check(hipGetLastError(), "sq_encode_kernel launch");
+        hip_check(hipMemcpy(host_codes + off, dc.get(), m, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    }
+}
+
+void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int lambda_p, int beta_p, float alpha,
+                     int seed, DeviceLists& out) {
+    const int L = fwd.dim;
+    const bool sq = fwd.esz == 1;
+    const int lambda = calc_lambda(lambda_p, n_docs);
+    const int beta = calc_beta(beta_p, lambda);
+    const uint32_t base_seed = seed == -1 ? std::random_device{}() : static_cast<uint32_t>(seed);
+    const size_t Z = static_cast<size_t>(nnz);
+
+    // ---- 1. postings by term (stable: doc order within a term)
+    DeviceBuffer pos_iota(Z * 4), spos(Z * 4), skeys(Z * 2), hist(static_cast<size_t>(L) * 4), doc_of(Z * 4);
+    hipLaunchKernelGGL(iota_kernel, dim3(grid_for(Z)), dim3(kB), 0, nullptr, pos_iota.get<int32_t>(), Z);
+    radix_sort_pairs(fwd.indices, skeys.get<uint16_t>(), pos_iota.get<int32_t>(), spos.get<int32_t>(), Z, 16);
+    skeys.release();
+    hip_check(hipMemset(hist.get(), 0, static_cast<size_t>(L) * 4), "hipMemset");
+    hipLaunchKernelGGL(term_hist_kernel, dim3(grid_for(Z)), dim3(kB), 0, nullptr, fwd.indices, Z, hist.get<int32_t>());
+    hipLaunchKernelGGL(doc_of_kernel, dim3(grid_for(static_cast<size_t>(n_docs))), dim3(kB), 0, nullptr, fwd.indptr,
+                       n_docs, doc_of.get<int32_t>());
+    sync("postings");
+    const auto h_hist = hist.to_host<int32_t>(L);
+    std::vector<int64_t> list_start(L + 1, 0), kept_base(L + 1, 0);
+    std::vector<int32_t> lcb(L + 1, 0);
+    bool any_long = false;
+    int max_clusters = 0;
+    for (int l = 0; l < L; ++l) {
+        const int64_t len = h_hist[l];
+        list_start[l + 1] = list_start[l] + len;
+        // prune_and_keep_doc_ids: lambda <= 0 (as size_t: 0 or huge) or >= len keeps all
+        const int64_t kl = (lambda <= 0 || lambda >= len) ? len : lambda;
+        any_long = any_long || kl < len;
+        kept_base[l + 1] = kept_base[l] + kl;
+        // boundary_check_n_clusters (random_kmeans.cpp:53-63), beta as size_t
+        int64_t nc = 0;
+        if (kl > 0) {
+            if (beta == 0)
+                nc = static_cast<int64_t>(std::sqrt(static_cast<double>(kl)));
+            else if (beta < 0)
+                nc = kl;
+            else
+                nc = beta;
+            nc = std::min(nc, kl);
+            nc = std::max<int64_t>(1, nc);
+        }
+        if (nc > 65536) throw std::invalid_argument("beta above 65536 clusters per list cannot be stored");
+        lcb[l + 1] = lcb[l] + static_cast<int32_t>(nc);
+        max_clusters = std::max<int>(max_clusters, static_cast<int>(nc));
+    }
+    const int64_t K = kept_base[L];
+    const int64_t C = lcb[L];
+    if (max_clusters > 4096)
+        throw std::invalid_argument("more than 4096 clusters per list is beyond the device build's LDS plan");
+
+    // ---- 2. lambda pruning order
+    DeviceBuffer spos2;
+    if (any_long) {
+        spos2.allocate(Z * 4);
+        if (sq) {
+            DeviceBuffer k_in(Z * 4), k_out(Z * 4);
+            hipLaunchKernelGGL(prune_key_u8_kernel, dim3(grid_for(Z)), dim3(kB), 0, nullptr, fwd.indices,
+                               static_cast<const uint8_t*>(fwd.values), Z, k_in.get<uint32_t>());
+            radix_sort_pairs(k_in.get<uint32_t>(), k_out.get<uint32_t>(), pos_iota.get<int32_t>(),
+                             spos2.get<int32_t>(), Z, 24);
+        } else {
+            DeviceBuffer k_in(Z * 8), k_out(Z * 8);
+            hipLaunchKernelGGL(prune_key_f32_kernel, dim3(grid_for(Z)), dim3(kB), 0, nullptr, fwd.indices,
+                               static_cast<const float*>(fwd.values), Z, k_in.get<uint64_t>());
+            radix_sort_pairs(k_in.get<uint64_t>(), k_out.get<uint64_t>(), pos_iota.get<int32_t>(),
+                             spos2.get<int32_t>(), Z, 48);
+        }
+        sync("pruning sort");
+    }
+    pos_iota.release();
+    DeviceBuffer d_list_start, d_kept_base, kept(static_cast<size_t>(K) * 4), d_lcb;
+    d_list_start.upload(list_start);
+    d_kept_base.upload(kept_base);
+    d_lcb.upload(lcb);
+    hipLaunchKernelGGL(kept_docs_kernel, dim3(std::min(L, 65536)), dim3(kB), 0, nullptr, d_list_start.get<int64_t>(),
+                       d_kept_base.get<int64_t>(), spos.get<int32_t>(),
+                       any_long ? spos2.get<int32_t>() : spos.get<int32_t>(), doc_of.get<int32_t>(),
+                       kept.get<int32_t>(), L);
+    sync("kept docs");
+    spos.release();
+    spos2.release();
+    doc_of.release();
+
+    // ---- 3. centroid draws
+    DeviceBuffer work(static_cast<size_t>(K) * 4), centroid_doc(static_cast<size_t>(C) * 4),
+        cluster_list(static_cast<size_t>(C) * 4);
+    hipLaunchKernelGGL(centroid_kernel, dim3(std::min(L, 65536)), dim3(64), 0, nullptr, d_kept_base.get<int64_t>(),
+                       kept.get<int32_t>(), work.get<int32_t>(), d_lcb.get<int32_t>(), centroid_doc.get<int32_t>(),
+                       base_seed, L);
+    hipLaunchKernelGGL(cluster_list_kernel, dim3(std::min(L, 65536)), dim3(kB), 0, nullptr, d_lcb.get<int32_t>(),
+                       cluster_list.get<int32_t>(), L);
+    sync("centroids");
+    work.release();
+
+    // ---- 4. centroid term-major index + assignment
+    DeviceBuffer clen(static_cast<size_t>(C) * 8), cent_off(static_cast<size_t>(C + 1) * 8);
+    hipLaunchKernelGGL(row_len_kernel, dim3(grid_for(C)), dim3(kB), 0, nullptr, fwd.indptr, centroid_doc.get<int32_t>(),
+                       static_cast<size_t>(C), clen.get<int64_t>());
+    exclusive_scan_i64(clen.get<int64_t>(), cent_off.get<int64_t>(), static_cast<size_t>(C));
+    sync("centroid sizes");
+    clen.release();
+    int64_t CE = 0;
+    cent_off.download(&CE, 8, static_cast<size_t>(C) * 8);
+    const size_t ew = sq ? 1 : 4;
+    DeviceBuffer ckey_in(CE * 4), cval_in(CE * 4), ent_c(CE * 2), ent_w(CE * ew), ckey(CE * 4), cperm(CE * 4);
+    if (sq)
+        hipLaunchKernelGGL((centroid_entries_kernel<uint8_t>), dim3(grid_for(C)), dim3(kB), 0, nullptr, fwd.indptr,
+                           fwd.indices, static_cast<const uint8_t*>(fwd.values), centroid_doc.get<int32_t>(),
+                           cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), cent_off.get<int64_t>(),
+                           static_cast<size_t>(C), ckey_in.get<uint32_t>(), cval_in.get<int32_t>(),
+                           ent_c.get<uint16_t>(), ent_w.get<uint8_t>());
+    else
+        hipLaunchKernelGGL((centroid_entries_kernel<float>), dim3(grid_for(C)), dim3(kB), 0, nullptr, fwd.indptr,
+                           fwd.indices, static_cast<const float*>(fwd.values), centroid_doc.get<int32_t>(),
+                           cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), cent_off.get<int64_t>(),
+                           static_cast<size_t>(C), ckey_in.get<uint32_t>(), cval_in.get<int32_t>(),
+                           ent_c.get<uint16_t>(), ent_w.get<float>());
+    radix_sort_pairs(ckey_in.get<uint32_t>(), ckey.get<uint32_t>(), cval_in.get<int32_t>(), cperm.get<int32_t>(),
+                     static_cast<size_t>(CE), 32);
+    ckey_in.release();
+    cval_in.release();
+    DeviceBuffer cc(CE * 2), cw(CE * ew);
+    if (sq)
+        hipLaunchKernelGGL((gather_csc_kernel<uint8_t>), dim3(grid_for(CE)), dim3(kB), 0, nullptr, cperm.get<int32_t>(),
+                           ent_c.get<uint16_t>(), ent_w.get<uint8_t>(), static_cast<size_t>(CE), cc.get<uint16_t>(),
+                           cw.get<uint8_t>());
+    else
+        hipLaunchKernelGGL((gather_csc_kernel<float>), dim3(grid_for(CE)), dim3(kB), 0, nullptr, cperm.get<int32_t>(),
+                           ent_c.get<uint16_t>(), ent_w.get<float>(), static_cast<size_t>(CE), cc.get<uint16_t>(),
+                           cw.get<float>());
+    sync("centroid index");
+    cperm.release();
+    ent_c.release();
+    ent_w.release();
+    if (CE >= (int64_t{1} << 31)) throw std::invalid_argument("centroid index exceeds 32-bit offsets");
+
+    const int assign_grid = std::min(L, 2048);
+    DeviceBuffer tmap(static_cast<size_t>(assign_grid) * L * 8), assign(static_cast<size_t>(K) * 4);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(grid_for(static_cast<size_t>(assign_grid) * L * 2)), dim3(kB), 0, nullptr,
+                       tmap.get<int32_t>(), static_cast<size_t>(assign_grid) * L * 2, -1);
+    const int mc = std::max(1, max_clusters);
+    const size_t assign_lds = ((mc * 4 + 15) & ~15) + static_cast<size_t>(4) * mc * 4;
+    if (assign_lds > 64 * 1024) {
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(assign_kernel<uint8_t, int32_t>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(assign_lds)),
+                  "hipFuncSetAttribute");
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(assign_kernel<float, float>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(assign_lds)),
+                  "hipFuncSetAttribute");
+    }
+    if (sq)
+        hipLaunchKernelGGL((assign_kernel<uint8_t, int32_t>), dim3(assign_grid), dim3(kB), assign_lds, nullptr,
+                           fwd.indptr, fwd.indices, static_cast<const uint8_t*>(fwd.values), d_kept_base.get<int64_t>(),
+                           kept.get<int32_t>(), d_lcb.get<int32_t>(), centroid_doc.get<int32_t>(),
+                           cent_off.get<int64_t>(), ckey.get<uint32_t>(), cc.get<uint16_t>(), cw.get<uint8_t>(),
+                           tmap.get<int2>(), L, assign.get<int32_t>(), L, mc);
+    else
+        hipLaunchKernelGGL((assign_kernel<float, float>), dim3(assign_grid), dim3(kB), assign_lds, nullptr, fwd.indptr,
+                           fwd.indices, static_cast<const float*>(fwd.values), d_kept_base.get<int64_t>(),
+                           kept.get<int32_t>(), d_lcb.get<int32_t>(), centroid_doc.get<int32_t>(),
+                           cent_off.get<int64_t>(), ckey.get<uint32_t>(), cc.get<uint16_t>(), cw.get<float>(),
+                           tmap.get<int2>(), L, assign.get<int32_t>(), L, mc);
+    sync("assignment");
+    tmap.release();
+    ckey.release();
+    cc.release();
+    cw.release();
+    cent_off.release();
+
+    // ---- 5. clusters
+    DeviceBuffer ccount(static_cast<size_t>(C) * 8), cdp(static_cast<size_t>(C + 1) * 8);
+    hipLaunchKernelGGL(cluster_count_kernel, dim3(std::min(L, 65536)), dim3(kB), 0, nullptr, d_kept_base.get<int64_t>(),
+                       assign.get<int32_t>(), d_lcb.get<int32_t>(), ccount.get<int64_t>(), L);
+    exclusive_scan_i64(ccount.get<int64_t>(), cdp.get<int64_t>(), static_cast<size_t>(C));
+    sync("cluster sizes");
+    int64_t D = 0;
+    cdp.download(&D, 8, static_cast<size_t>(C) * 8);
+    DeviceBuffer docs(static_cast<size_t>(D) * 4);
+    hipLaunchKernelGGL(cluster_place_kernel, dim3(std::min(L, 65536)), dim3(64), 0, nullptr, d_kept_base.get<int64_t>(),
+                       kept.get<int32_t>(), assign.get<int32_t>(), d_lcb.get<int32_t>(), centroid_doc.get<int32_t>(),
+                       cdp.get<int64_t>(), docs.get<int32_t>(), L);
+    sync("cluster placement");
+    kept.release();
+    assign.release();
+    ccount.release();
+    centroid_doc.release();
+
+    // ---- 6. summaries (count pass, scan, write pass)
+    const int nw = (L + 31) / 32;
+    DeviceBuffer scount(static_cast<size_t>(C) * 8), out_off(static_cast<size_t>(C + 1) * 8), err(4);
+    hip_check(hipMemset(err.get(), 0, 4), "hipMemset");
+    const unsigned sgrid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(C, 8192)));
+    const size_t lds8 = static_cast<size_t>(nw) * 8 * 4 + nw * 4 + 256 * 4 + nw * 8 + 8 * 4;
+    const int f32_cap = 4096;
+    const size_t ldsf = static_cast<size_t>(L) * 4 + nw * 8 + 8 + f32_cap * 8 + 16;
+    if (!sq && ldsf > 160 * 1024)
+        throw std::invalid_argument("vocabulary too large for the FP32 device summarizer");
+    if (!sq) {
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(summarize_f32_kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(ldsf)),
+                  "hipFuncSetAttribute");
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        DeviceBuffer sterm, sval, scl;
+        int64_t E = 0;
+        if (pass == 1) {
+            exclusive_scan_i64(scount.get<int64_t>(), out_off.get<int64_t>(), static_cast<size_t>(C));
+            sync("summary offsets");
+            out_off.download(&E, 8, static_cast<size_t>(C) * 8);
+            sterm.allocate(E * 2);
+            sval.allocate(E * ew, 64);
+            scl.allocate(E * 2);
+        }
+        if (sq) {
+            if (pass == 0)
+                hipLaunchKernelGGL((summarize_u8_kernel<false>), dim3(sgrid), dim3(kB), lds8, nullptr, fwd.indptr,
+                                   fwd.indices, static_cast<const uint8_t*>(fwd.values), cdp.get<int64_t>(),
+                                   docs.get<int32_t>(), cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), C, nw, alpha,
+                                   scount.get<int64_t>(), nullptr, nullptr, nullptr, nullptr);
+            else
+                hipLaunchKernelGGL((summarize_u8_kernel<true>), dim3(sgrid), dim3(kB), lds8, nullptr, fwd.indptr,
+                                   fwd.indices, static_cast<const uint8_t*>(fwd.values), cdp.get<int64_t>(),
+                                   docs.get<int32_t>(), cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), C, nw, alpha,
+                                   scount.get<int64_t>(), out_off.get<int64_t>(), sterm.get<uint16_t>(),
+                                   sval.get<uint8_t>(), scl.get<uint16_t>());
+        } else {
+            hipLaunchKernelGGL(summarize_f32_kernel, dim3(sgrid), dim3(kB), ldsf, nullptr, fwd.indptr, fwd.indices,
+                               static_cast<const float*>(fwd.values), cdp.get<int64_t>(), docs.get<int32_t>(),
+                               cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), C, L, f32_cap, alpha,
+                               scount.get<int64_t>(), out_off.get<int64_t>(), pass == 1,
+                               pass == 1 ? sterm.get<uint16_t>() : nullptr, pass == 1 ? sval.get<float>() : nullptr,
+                               pass == 1 ? scl.get<uint16_t>() : nullptr, err.get<int32_t>());
+        }
+        sync("summaries");
+        int32_t e = 0;
+        err.download(&e, 4);
+        if (e != 0) throw std::invalid_argument("a cluster touches more than 4096 terms: beyond the FP32 device summarizer");
+        if (pass == 0) continue;
+
+        // ---- 7. term-major transpose per list
+        DeviceBuffer key_in(E * 4), val_in(E * 4), key(E * 4), perm(E * 4);
+        hipLaunchKernelGGL(summary_key_kernel, dim3(grid_for(C)), dim3(kB), 0, nullptr, sterm.get<uint16_t>(),
+                           out_off.get<int64_t>(), cluster_list.get<int32_t>(), C, key_in.get<uint32_t>(),
+                           val_in.get<int32_t>());
+        radix_sort_pairs(key_in.get<uint32_t>(), key.get<uint32_t>(), val_in.get<int32_t>(), perm.get<int32_t>(),
+                         static_cast<size_t>(E), 32);
+        key_in.release();
+        val_in.release();
+        out.sum_cluster.allocate(E * 2);
+        out.sum_value.allocate(E * ew, 64);
+        if (sq)
+            hipLaunchKernelGGL((gather_summary_kernel<uint8_t>), dim3(grid_for(E)), dim3(kB), 0, nullptr,
+                               perm.get<int32_t>(), scl.get<uint16_t>(), sval.get<uint8_t>(), static_cast<size_t>(E),
+                               out.sum_cluster.get<uint16_t>(), out.sum_value.get<uint8_t>());
+        else
+            hipLaunchKernelGGL((gather_summary_kernel<float>), dim3(grid_for(E)), dim3(kB), 0, nullptr,
+                               perm.get<int32_t>(), scl.get<uint16_t>(), sval.get<float>(), static_cast<size_t>(E),
+                               out.sum_cluster.get<uint16_t>(), out.sum_value.get<float>());
+        DeviceBuffer flag(E * 8), hscan((E + 1) * 8);
+        hipLaunchKernelGGL(head_flag_kernel, dim3(grid_for(E)), dim3(kB), 0, nullptr, key.get<uint32_t>(),
+                           static_cast<size_t>(E), flag.get<int64_t>());
+        exclusive_scan_i64(flag.get<int64_t>(), hscan.get<int64_t>(), static_cast<size_t>(E));
+        sync("summary heads");
+        int64_t T = 0;
+        hscan.download(&T, 8, static_cast<size_t>(E) * 8);
+        if (T >= (int64_t{1} << 31)) throw std::invalid_argument("summary term table exceeds 32-bit offsets");
+        out.sum_term_ids.allocate(T * 2);
+        out.sum_term_ptr.allocate((T + 1) * 8);
+        hipLaunchKernelGGL(head_write_kernel, dim3(grid_for(E)), dim3(kB), 0, nullptr, key.get<uint32_t>(),
+                           hscan.get<int64_t>(), static_cast<size_t>(E), out.sum_term_ids.get<uint16_t>(),
+                           out.sum_term_ptr.get<int64_t>());
+        hip_check(hipMemcpy(out.sum_term_ptr.get<int64_t>() + T, &E, 8, hipMemcpyHostToDevice), "hipMemcpy");
+        out.list_entry_base.allocate(static_cast<size_t>(L + 1) * 8);
+        out.list_term_base.allocate(static_cast<size_t>(L + 1) * 4);
+        hipLaunchKernelGGL(list_bases_kernel, dim3(grid_for(L + 1)), dim3(kB), 0, nullptr, d_lcb.get<int32_t>(),
+                           out_off.get<int64_t>(), hscan.get<int64_t>(), E, T, L, out.list_entry_base.get<int64_t>(),
+                           out.list_term_base.get<int32_t>());
+        sync("summary transpose");
+    }
+    out.list_cluster_base = std::move(d_lcb);
+    out.cluster_doc_ptr = std::move(cdp);
+    out.docs = std::move(docs);
+    out.n_lists = L;
+    out.max_clusters = max_clusters;
+}
+
+}  // namespace nsparse_amd

@@ -1,0 +1,87 @@
+// HBM layout of a SEISMIC / SEISMIC-SQ index and the argument blocks of the
+// search and build kernels. Shared by host (.cpp) and device (.hip) code.
+//
+// Forward index: CSR exactly as the reference stores it (sparse_vectors.h:105-110):
+//   indptr int32[N+1], indices uint16[nnz], values uint8[nnz] (SQ) or float[nnz].
+// Posting lists, all `dim` lists concatenated (list l = term l):
+//   list_cluster_base int32[L+1]  -> global cluster ids of list l
+//   cluster_doc_ptr   int64[C+1]  -> docs of global cluster c (the reference's
+//                                    per-list offsets_, rebased)
+//   docs              int32[D]    -> doc ids, in the reference's docs_ order
+//   list_term_base    int32[L+1]  -> summary columns of list l
+//   sum_term_ids      uint16[T]   -> distinct summary terms of each list, ascending
+//   sum_term_ptr      int64[T+1]  -> entries of column t (term_ptr_, rebased)
+//   sum_cluster       uint16[E]   -> local cluster id per entry (csc_cluster_)
+//   sum_value         uint8/float[E] (csc_value_)
+#pragma once
+
+#include <cstdint>
+
+namespace nsparse_amd {
+
+struct DevIndexView {
+    int32_t n_docs = 0;
+    int32_t dim = 0;
+    int32_t esz = 1;  // 1 = uint8 codes, 4 = float
+    int32_t n_lists = 0;
+    int32_t max_clusters = 0;  // largest cluster count of any list
+    const int32_t* indptr = nullptr;
+    const uint16_t* indices = nullptr;
+    const void* values = nullptr;
+    const int32_t* list_cluster_base = nullptr;
+    const int64_t* cluster_doc_ptr = nullptr;
+    const int32_t* docs = nullptr;
+    const int32_t* list_term_base = nullptr;
+    const uint16_t* sum_term_ids = nullptr;
+    const int64_t* sum_term_ptr = nullptr;
+    const uint16_t* sum_cluster = nullptr;
+    const void* sum_value = nullptr;
+};
+
+// Per-query traversal counters (the algorithmic-byte model): one row of
+// kStatFields int64 per query when SearchArgs::stats is set.
+enum StatField {
+    kStatLists = 0,         // cut lists visited (term < n_lists)
+    kStatTermsFound,        // query entries found in a list's summary columns
+    kStatCscEntries,        // summary (cluster, value) entries read
+    kStatClustersTested,    // prune tests evaluated
+    kStatClustersOpened,    // clusters whose doc ids were read
+    kStatDocsRead,          // doc ids read from opened clusters
+    kStatDocsScored,        // forward-index rows gathered and scored
+    kStatNnzScored,         // non-zeros of those rows
+    kStatFields
+};
+
+struct SearchArgs {
+    DevIndexView ix;
+    const int32_t* q_indptr;   // device, n+1 (absolute offsets)
+    const uint16_t* q_indices;
+    const float* q_values;
+    const int32_t* ks;         // device, per-query k (nullptr: k_max for all)
+    int32_t k_max;             // row stride of the outputs
+    int32_t cut;
+    float heap_factor;
+    float q_vmin;              // SQ query quantizer: code = lround(clamp((v - q_vmin) * q_scale))
+    float q_scale;             // 255 / (vmax - vmin), computed on the host as the reference does
+    float decode_scale;        // (ingest_range * query_range) / (255 * 255)
+    const int32_t* sel_of_query;        // device, per-query selector slot or -1 (nullptr: use sel_all)
+    const uint32_t* const* sel_bitmaps; // device array of device bitmap pointers
+    const uint32_t* sel_all;            // one bitmap for every query (nullptr: none)
+    float* out_d;              // device, n x k_max
+    int32_t* out_l;
+    int64_t* stats;            // device, n x kStatFields, or nullptr
+    int32_t* error_flag;       // device; set to a nonzero code on a rejected query
+    int64_t n_queries;
+    int64_t query_base;        // first query handled by this launch
+};
+
+// Error codes a kernel can leave in SearchArgs::error_flag.
+enum DeviceError : int32_t {
+    kDevOk = 0,
+    kDevQueryTooLong = 1,   // query nnz above the device limit
+    kDevTermOutOfRange = 2, // query term id >= dim
+    kDevTooManyClusters = 3, // list with more clusters than the LDS plan allows
+    kDevBadK = 4            // per-query k outside [1, k_max]
+};
+
+}  // namespace nsparse_amd

@@ -1,0 +1,169 @@
+// Host side of the index: the reference's nsparse::Index family
+// (index.h:30-68, seismic_index.h, seismic_scalar_quantized_index.h,
+// id_map_index.h) re-built around an HBM-resident copy of the index.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "device_index.h"
+#include "device_mem.h"
+#include "nsparse_c.h"
+
+namespace nsparse_amd {
+
+using idx_t = int32_t;
+using term_t = uint16_t;
+
+// Forward index on the host: the reference's SparseVectors CSR
+// (sparse_vectors.h:105-110) with values as raw bytes of width esz.
+struct HostCsr {
+    size_t esz = 1;
+    size_t dim = 0;
+    std::vector<idx_t> indptr;
+    std::vector<term_t> indices;
+    std::vector<uint8_t> values;
+    int64_t n() const { return indptr.empty() ? 0 : static_cast<int64_t>(indptr.size()) - 1; }
+};
+
+// One posting list's clusters + transposed summaries: InvertedListClusters
+// (cluster/inverted_list_clusters.h:75-83).
+struct HostList {
+    std::vector<idx_t> docs, offsets;
+    size_t n_clusters = 0;
+    size_t esz = 4;
+    std::vector<term_t> term_ids;
+    std::vector<idx_t> term_ptr;
+    std::vector<uint16_t> csc_cluster;
+    std::vector<uint8_t> csc_value;
+};
+
+class ByteWriter;
+class ByteReader;
+
+struct SearchRequest {
+    int cut = 10;
+    float heap_factor = 1.0F;
+    bool has_sq_range = false;
+    float vmin = 0.0F, vmax = 1.0F;
+    const nsparse_selector* selector = nullptr;
+};
+
+class Index {
+public:
+    explicit Index(int dim) : dim_(dim) {}
+    virtual ~Index() = default;
+    virtual std::array<char, 4> id() const = 0;
+    int dimension() const { return dim_; }
+    virtual int64_t num_vectors() const = 0;
+    virtual void add(idx_t n, const idx_t* indptr, const term_t* indices, const float* values) = 0;
+    virtual void add_with_ids(idx_t, const idx_t*, const term_t*, const float*, const idx_t*);
+    virtual void build();
+    // Index::search (index.cpp:32-49): validates, then fills n*k outputs.
+    virtual void search(idx_t n, const idx_t* indptr, const term_t* indices, const float* values, int k,
+                        float* distances, idx_t* labels, const nsparse_search_params* params) = 0;
+    virtual void write_payload(ByteWriter& w) const = 0;
+    virtual void read_payload(ByteReader& r) = 0;
+    virtual const HostCsr* vectors() const { return nullptr; }
+
+protected:
+    int dim_;
+};
+
+// SeismicIndex (FP32) and SeismicScalarQuantizedIndex (8-bit) in one class:
+// the two differ only in element width, query encoding and score decoding.
+class SeismicIndex : public Index {
+public:
+    struct Params {
+        int lambda = 10, beta = 5;
+        float alpha = 0.5F;
+        int seed = -1;
+    };
+    SeismicIndex(int dim, bool sq, float vmin, float vmax, Params p);
+    std::array<char, 4> id() const override;
+    int64_t num_vectors() const override { return fwd_.n(); }
+    void add(idx_t n, const idx_t* indptr, const term_t* indices, const float* values) override;
+    void build() override;
+    void search(idx_t n, const idx_t* indptr, const term_t* indices, const float* values, int k, float* distances,
+                idx_t* labels, const nsparse_search_params* params) override;
+    void write_payload(ByteWriter& w) const override;
+    void read_payload(ByteReader& r) override;
+    const HostCsr* vectors() const override { return has_vectors_ ? &fwd_ : nullptr; }
+
+    // Batched search over queries already in HBM (nsparse_search_device).
+    void search_device(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices, const float* d_values,
+                       const int32_t* d_ks, int k_max, const SearchRequest& req, const int32_t* d_sel_of_query,
+                       const uint32_t* const* d_bitmaps, float* d_out_d, int32_t* d_out_l, int64_t* d_stats,
+                       hipStream_t stream);
+    // The search proper over host queries (IDMap passes its mapped selector).
+    void search_host(idx_t n, const idx_t* indptr, const term_t* indices, const float* values, int k,
+                     float* distances, idx_t* labels, const SearchRequest& req, int64_t* stats_out);
+    // ExactMatcher path over an explicit sorted id list (IDMap passes mapped ids).
+    void exact_search(idx_t n, const idx_t* indptr, const term_t* indices, const float* values, int k,
+                      float* distances, idx_t* labels, const SearchRequest& req, const std::vector<idx_t>& ids);
+    // Error code a kernel left from the last search_device call (0 = none).
+    int32_t last_device_error();
+    bool sq() const { return sq_; }
+    const DevIndexView& view();
+    int max_clusters() const { return max_clusters_; }
+    void set_lists(std::vector<HostList>&& lists);
+    const std::vector<HostList>& lists();
+
+private:
+    void ensure_device();
+    void upload_lists();
+    void download_lists();
+
+    bool sq_;
+    float vmin_ = 0.0F, vmax_ = 1.0F;
+    Params params_;
+    HostCsr fwd_;
+    bool has_vectors_ = false;
+    // Lists: host copy and/or device copy; whichever is missing is made on demand.
+    std::vector<HostList> lists_;
+    bool host_lists_valid_ = true;
+    bool dev_lists_valid_ = false;
+    bool dev_fwd_valid_ = false;
+    int32_t n_lists_ = 0;
+    int max_clusters_ = 0;
+    // device copies
+    DeviceBuffer d_indptr_, d_indices_, d_values_;
+    DeviceBuffer d_list_cluster_base_, d_cluster_doc_ptr_, d_docs_, d_list_term_base_, d_sum_term_ids_,
+        d_sum_term_ptr_, d_sum_cluster_, d_sum_value_, d_list_entry_base_;
+    DeviceBuffer d_err_;
+    DevIndexView view_{};
+};
+
+// IDMapIndex (id_map_index.h:78-109, id_map_index.cpp).
+class IDMapIndex : public Index {
+public:
+    explicit IDMapIndex(std::unique_ptr<Index> delegate);
+    IDMapIndex() : Index(0) {}
+    std::array<char, 4> id() const override { return {'I', 'D', 'M', 'P'}; }
+    int64_t num_vectors() const override { return delegate_ ? delegate_->num_vectors() : 0; }
+    void add(idx_t n, const idx_t* indptr, const term_t* indices, const float* values) override;
+    void add_with_ids(idx_t n, const idx_t* indptr, const term_t* indices, const float* values,
+                      const idx_t* ids) override;
+    void build() override;
+    void search(idx_t n, const idx_t* indptr, const term_t* indices, const float* values, int k, float* distances,
+                idx_t* labels, const nsparse_search_params* params) override;
+    void write_payload(ByteWriter& w) const override;
+    void read_payload(ByteReader& r) override;
+    const HostCsr* vectors() const override { return delegate_ ? delegate_->vectors() : nullptr; }
+    Index* delegate() { return delegate_.get(); }
+
+private:
+    std::unique_ptr<Index> delegate_;
+    std::vector<idx_t> i2e_;
+    std::unordered_map<idx_t, idx_t> e2i_;
+};
+
+std::unique_ptr<Index> index_factory(int dim, const char* description);
+void write_index(const Index& index, ByteWriter& w);
+std::unique_ptr<Index> read_index(ByteReader& r);
+
+}  // namespace nsparse_amd

@@ -1,0 +1,53 @@
+// Host-callable entry points of the HIP translation units (search.hip, build.hip).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "device_index.h"
+#include "device_mem.h"
+
+namespace nsparse_amd {
+
+// Longest query (non-zeros) a search workgroup holds in LDS.
+constexpr int kDeviceQueryMax = 512;
+
+// ExactMatcher (exact_matcher.h:23-58): score an explicit sorted id list.
+struct ExactArgs {
+    DevIndexView ix;
+    const int32_t* q_indptr;
+    const uint16_t* q_indices;
+    const float* q_values;
+    const int32_t* ids;  // sorted, may repeat (ArrayIDSelector)
+    int32_t n_ids;       // <= k
+    int32_t k;
+    float q_vmin, q_scale, decode_scale;
+    float* out_d;
+    int32_t* out_l;
+    int64_t n_queries;
+};
+
+// Posting lists produced by the GPU build, in the layout of device_index.h.
+struct DeviceLists {
+    DeviceBuffer list_cluster_base, cluster_doc_ptr, docs, list_term_base, sum_term_ids, sum_term_ptr, sum_cluster,
+        sum_value, list_entry_base;
+    int32_t n_lists = 0;
+    int max_clusters = 0;
+};
+
+// ScalarQuantizer::encode (8-bit) of host values on the GPU; codes come back to the host.
+void encode_sq_on_device(const float* host_values, size_t n, float vmin, float vmax, uint8_t* host_codes);
+
+// build_inverted_lists_clusters (seismic_common.h:146-184) on the GPU.
+void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int lambda, int beta, float alpha,
+                     int seed, DeviceLists& out);
+
+void launch_search_checked(const SearchArgs& a, int max_clusters, int kmax, bool sq, hipStream_t stream);
+void launch_exact_checked(const ExactArgs& a, bool sq, hipStream_t stream);
+int search_lds_bytes(int dim, int max_clusters, int kmax, bool sq);
+std::string device_error_message(int32_t code);
+
+}  // namespace nsparse_amd

@@ -1,0 +1,826 @@
+// SEISMIC / SEISMIC-SQ search on CDNA4 (gfx950).
+//
+// One 256-lane workgroup per query. The query lives in LDS as a presence
+// bitmap over the vocabulary plus a rank table (popcount prefix per 32-term
+// word) into compact per-term values -- about 6 KB for a 30522-term vocabulary
+// instead of a 30-122 KB dense vector -- so several queries share a CU.
+//
+// Per cut list the workgroup (reference: query_single_inverted_list,
+// seismic_scalar_quantized_index.cpp:67-135 and seismic_index.cpp:53-118):
+//  1. scans the list's summary term ids against the query bitmap and
+//     accumulates block-summary scores from the matching CSC columns
+//     (integer-exact LDS atomics for uint8 codes; for FP32 a term-ordered tile
+//     so every cluster adds its products in query-entry order, as the
+//     reference's score_summaries_typed does);
+//  2. orders clusters (first list only) with a step-exact std::sort emulation;
+//  3. walks clusters in that order in speculative chunks: wave 0 picks the
+//     next run of clusters that pass the heap-factor test against the current
+//     k-th score (a cluster failing an earlier, lower k-th score also fails
+//     every later one), all lanes gather those clusters' doc rows from HBM and
+//     score them, then wave 0 replays the chunk in order against the exact,
+//     evolving k-th score -- so every prune decision equals the reference's.
+// The top-k is a sorted LDS array ordered by (score desc, doc id asc); a doc
+// already in it is not inserted twice, which reproduces the reference's
+// visited-set semantics for the result (a revisited doc has the same score).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include <stdexcept>
+#include <string>
+
+#include "device_index.h"
+#include "kernels.h"
+#include "stl_order.h"
+
+namespace nsparse_amd {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kQMax = 512;    // longest query handled on device
+constexpr int kBudget = 512;  // doc slots per speculative chunk
+constexpr int kSegMax = 64;   // clusters examined per chunk (one wave pass)
+constexpr int kTileFloats = 4096;
+
+struct Seg {
+    int32_t cluster;    // local cluster id
+    int32_t doc_begin;  // offset into the list's docs
+    int32_t count;      // docs in this piece
+    int32_t slot;       // first staging slot
+    int32_t test;       // 1 if this piece starts the cluster (prune test runs here)
+    int32_t pos;        // position of the cluster in the visiting order
+};
+
+enum Misc {
+    kMiscErr = 0,
+    kMiscNu,     // unique query terms
+    kMiscNcut,   // cut terms
+    kMiscNf,     // found items in the current list
+    kMiscPos,    // next cluster position
+    kMiscPoff,   // docs of the cluster at kMiscPos already handed out
+    kMiscEnd,    // clusters at positions >= this are never visited (first list break)
+    kMiscNseg,
+    kMiscTotal,  // staged docs in the chunk
+    kMiscDone,   // list finished
+    kMiscTkN,    // entries in the top-k
+    kMiscBreakPending,
+    kMiscCount = 16
+};
+
+// Byte offsets of the dynamic LDS sections (computed on the host).
+struct LdsPlan {
+    int32_t nw;        // 32-bit words of the query bitmap
+    int32_t mc;        // cluster capacity
+    int32_t kmax;      // top-k capacity
+    int32_t uni_bytes; // union scratch (staging / sort buffer / FP32 tile)
+    int32_t o_qbits, o_qpref, o_eterm, o_erank, o_eval, o_rlast, o_rval, o_rsum, o_rcol, o_flist, o_cuts;
+    int32_t o_cstart, o_score, o_order, o_uni, o_tks, o_tki, o_seg, o_misc, o_wsum;
+    int32_t total;
+};
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t group_sum(uint32_t v) {
+#pragma unroll
+    for (int d = G / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, G);
+    return v;
+}
+
+__device__ __forceinline__ bool better(float s, int32_t i, float ts, int32_t ti) {
+    return s > ts || (s == ts && i < ti);
+}
+
+struct QueryLds {
+    const uint32_t* qbits;
+    const uint16_t* qpref;
+    const float* rval;
+    __device__ __forceinline__ int rank_of(uint32_t t, bool& present) const {
+        const uint32_t w = qbits[t >> 5];
+        const uint32_t bit = 1u << (t & 31);
+        present = (w & bit) != 0;
+        return qpref[t >> 5] + __popc(w & (bit - 1));
+    }
+    __device__ __forceinline__ float value(uint32_t t) const {
+        bool p;
+        const int r = rank_of(t, p);
+        return p ? rval[r] : 0.0F;
+    }
+};
+
+template <bool SQ, int G>
+__global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, LdsPlan P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int64_t q = a.query_base + blockIdx.x;
+    if (q >= a.n_queries) return;
+
+    uint32_t* qbits = reinterpret_cast<uint32_t*>(smem + P.o_qbits);
+    uint16_t* qpref = reinterpret_cast<uint16_t*>(smem + P.o_qpref);
+    uint16_t* eterm = reinterpret_cast<uint16_t*>(smem + P.o_eterm);
+    uint16_t* erank = reinterpret_cast<uint16_t*>(smem + P.o_erank);
+    float* evalf = reinterpret_cast<float*>(smem + P.o_eval);
+    int32_t* rlast = reinterpret_cast<int32_t*>(smem + P.o_rlast);
+    float* rval = reinterpret_cast<float*>(smem + P.o_rval);
+    uint32_t* rsum = reinterpret_cast<uint32_t*>(smem + P.o_rsum);
+    int32_t* rcol = reinterpret_cast<int32_t*>(smem + P.o_rcol);
+    int32_t* flist = reinterpret_cast<int32_t*>(smem + P.o_flist);
+    uint16_t* cuts = reinterpret_cast<uint16_t*>(smem + P.o_cuts);
+    int32_t* cstart = reinterpret_cast<int32_t*>(smem + P.o_cstart);
+    float* score = reinterpret_cast<float*>(smem + P.o_score);
+    uint32_t* sacc = reinterpret_cast<uint32_t*>(smem + P.o_score);
+    uint16_t* order = reinterpret_cast<uint16_t*>(smem + P.o_order);
+    unsigned char* uni = smem + P.o_uni;
+    float* tks = reinterpret_cast<float*>(smem + P.o_tks);
+    int32_t* tki = reinterpret_cast<int32_t*>(smem + P.o_tki);
+    Seg* seg = reinterpret_cast<Seg*>(smem + P.o_seg);
+    int32_t* misc = reinterpret_cast<int32_t*>(smem + P.o_misc);
+    int32_t* wsum = reinterpret_cast<int32_t*>(smem + P.o_wsum);
+    // staging view of the union
+    int32_t* st_id = reinterpret_cast<int32_t*>(uni);
+    int32_t* st_rs = st_id + kBudget;
+    int32_t* st_len = st_rs + kBudget;
+    float* st_sc = reinterpret_cast<float*>(st_len + kBudget);
+
+    const DevIndexView& ix = a.ix;
+    const int64_t s0 = a.q_indptr[q];
+    const int qn = static_cast<int>(a.q_indptr[q + 1] - s0);
+    const int k = a.ks != nullptr ? a.ks[q] : a.k_max;
+    float* out_d = a.out_d + q * a.k_max;
+    int32_t* out_l = a.out_l + q * a.k_max;
+    const uint32_t* sel = a.sel_all;
+    if (a.sel_of_query != nullptr) {
+        const int s = a.sel_of_query[q];
+        sel = s >= 0 ? a.sel_bitmaps[s] : nullptr;
+    }
+    const float hf = a.heap_factor;
+
+    if (qn > kQMax || k <= 0 || k > P.kmax) {
+        if (tid == 0) atomicCAS(a.error_flag, 0, qn > kQMax ? kDevQueryTooLong : kDevBadK);
+        for (int i = tid; i < a.k_max; i += kBlock) {
+            out_d[i] = -1.0F;
+            out_l[i] = -1;
+        }
+        return;
+    }
+
+    // ---- query setup ----
+    if (tid < kMiscCount) misc[tid] = 0;
+    for (int w = tid; w < P.nw; w += kBlock) qbits[w] = 0;
+    for (int r = tid; r < qn; r += kBlock) {
+        rlast[r] = -1;
+        rsum[r] = 0;
+    }
+    __syncthreads();
+    for (int i = tid; i < qn; i += kBlock) {
+        const uint32_t t = a.q_indices[s0 + i];
+        const float v = a.q_values[s0 + i];
+        if (t >= static_cast<uint32_t>(ix.dim)) {
+            misc[kMiscErr] = 1;
+            continue;
+        }
+        eterm[i] = static_cast<uint16_t>(t);
+        if constexpr (SQ) {
+            // ScalarQuantizer::encode_8bit (utils/scalar_quantizer.h:97-101)
+            float sc = (v - a.q_vmin) * a.q_scale;
+            sc = (sc < 255.0F) ? sc : 255.0F;
+            sc = (0.0F < sc) ? sc : 0.0F;
+            evalf[i] = static_cast<float>(static_cast<uint32_t>(roundf(sc)));
+        } else {
+            evalf[i] = v;
+        }
+        atomicOr(&qbits[t >> 5], 1u << (t & 31));
+    }
+    __syncthreads();
+    if (misc[kMiscErr] != 0) {
+        if (tid == 0) atomicCAS(a.error_flag, 0, kDevTermOutOfRange);
+        for (int i = tid; i < a.k_max; i += kBlock) {
+            out_d[i] = -1.0F;
+            out_l[i] = -1;
+        }
+        return;
+    }
+    // exclusive popcount prefix over the bitmap words
+    {
+        const int per = (P.nw + kBlock - 1) / kBlock;
+        const int w0 = tid * per;
+        int local = 0;
+        for (int w = w0; w < w0 + per && w < P.nw; ++w) local += __popc(qbits[w]);
+        const int incl = wave_incl_scan(local, lane);
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        int base = 0;
+        for (int w = 0; w < wave; ++w) base += wsum[w];
+        int run = base + incl - local;
+        for (int w = w0; w < w0 + per && w < P.nw; ++w) {
+            qpref[w] = static_cast<uint16_t>(run);
+            run += __popc(qbits[w]);
+        }
+        if (tid == kBlock - 1) misc[kMiscNu] = base + incl;
+    }
+    __syncthreads();
+    const QueryLds QL{qbits, qpref, rval};
+    for (int i = tid; i < qn; i += kBlock) {
+        bool p;
+        const int r = QL.rank_of(eterm[i], p);
+        erank[i] = static_cast<uint16_t>(r);
+        atomicMax(&rlast[r], i);
+        if constexpr (SQ) atomicAdd(&rsum[r], static_cast<uint32_t>(evalf[i]));
+    }
+    __syncthreads();
+    const int nu = misc[kMiscNu];
+    for (int r = tid; r < nu; r += kBlock) rval[r] = evalf[rlast[r]];
+    if (tid == 0) {
+        // top_k_tokens (utils/vector_process.h:18-30) on codes (SQ) or raw values.
+        KeyItem* heap = reinterpret_cast<KeyItem*>(uni);
+        misc[kMiscNcut] = topk_tokens(
+            qn, a.cut, [&](int i) { return evalf[i]; }, [&](int i) { return static_cast<uint32_t>(eterm[i]); },
+            heap, cuts);
+    }
+    __syncthreads();
+
+    int64_t st_lists = 0, st_found = 0, st_csc = 0, st_tested = 0, st_opened = 0, st_read = 0, st_scored = 0,
+            st_nnz = 0;
+    const int ncut = misc[kMiscNcut];
+    bool first_list = true;
+
+    for (int ci = 0; ci < ncut; ++ci) {
+        const uint32_t term = cuts[ci];
+        if (term >= static_cast<uint32_t>(ix.n_lists)) continue;
+        ++st_lists;
+        const int32_t lb = ix.list_cluster_base[term];
+        const int nc = ix.list_cluster_base[term + 1] - lb;
+        if (nc == 0) {
+            first_list = false;
+            continue;
+        }
+        const int64_t dbase = ix.cluster_doc_ptr[lb];
+        for (int j = tid; j <= nc; j += kBlock) cstart[j] = static_cast<int32_t>(ix.cluster_doc_ptr[lb + j] - dbase);
+        const int32_t tb = ix.list_term_base[term];
+        const int32_t te = ix.list_term_base[term + 1];
+        for (int r = tid; r < nu; r += kBlock) rcol[r] = -1;
+        if constexpr (SQ) {
+            for (int c = tid; c < nc; c += kBlock) sacc[c] = 0;
+        } else {
+            for (int c = tid; c < nc; c += kBlock) score[c] = 0.0F;
+        }
+        if (tid == 0) misc[kMiscNf] = 0;
+        __syncthreads();
+        // ---- summary columns matching the query ----
+        for (int i = tb + tid; i < te; i += kBlock) {
+            const uint32_t t = ix.sum_term_ids[i];
+            bool p;
+            const int r = QL.rank_of(t, p);
+            if (p) rcol[r] = i;
+        }
+        __syncthreads();
+        if constexpr (SQ) {
+            for (int r = tid; r < nu; r += kBlock)
+                if (rcol[r] >= 0) flist[atomicAdd(&misc[kMiscNf], 1)] = r;
+        } else {
+            // found query entries in CSR order (the order the reference adds them in)
+            if (wave == 0) {
+                int nf = 0;
+                for (int b = 0; b < qn; b += 64) {
+                    const int i = b + lane;
+                    const bool f = i < qn && rcol[erank[i]] >= 0;
+                    const uint64_t m = ballot(f);
+                    if (f) flist[nf + __popcll(m & ((1ULL << lane) - 1))] = i;
+                    nf += __popcll(m);
+                }
+                if (lane == 0) misc[kMiscNf] = nf;
+            }
+        }
+        __syncthreads();
+        const int nf = misc[kMiscNf];
+        if (a.stats != nullptr && tid == 0) {
+            for (int i = 0; i < qn; ++i) {
+                const int col = rcol[erank[i]];
+                if (col >= 0) {
+                    ++st_found;
+                    st_csc += ix.sum_term_ptr[col + 1] - ix.sum_term_ptr[col];
+                }
+            }
+        }
+        if constexpr (SQ) {
+            const uint8_t* sv = static_cast<const uint8_t*>(ix.sum_value);
+            for (int f = wave; f < nf; f += kWaves) {
+                const int r = flist[f];
+                const int col = rcol[r];
+                const int64_t e0 = ix.sum_term_ptr[col], e1 = ix.sum_term_ptr[col + 1];
+                const uint32_t qs = rsum[r];
+                for (int64_t e = e0 + lane; e < e1; e += 64)
+                    atomicAdd(&sacc[ix.sum_cluster[e]], qs * static_cast<uint32_t>(sv[e]));
+            }
+            __syncthreads();
+            for (int c = tid; c < nc; c += kBlock) score[c] = static_cast<float>(sacc[c]);
+        } else {
+            const float* sv = static_cast<const float*>(ix.sum_value);
+            float* tile = reinterpret_cast<float*>(uni);
+            const int per = max(1, (P.uni_bytes / 4) / nc);
+            for (int f0 = 0; f0 < nf; f0 += per) {
+                const int f1 = min(nf, f0 + per);
+                for (int x = tid; x < (f1 - f0) * nc; x += kBlock) tile[x] = 0.0F;
+                __syncthreads();
+                for (int f = f0 + wave; f < f1; f += kWaves) {
+                    const int i = flist[f];
+                    const int col = rcol[erank[i]];
+                    const float qv = evalf[i];
+                    const int64_t e0 = ix.sum_term_ptr[col], e1 = ix.sum_term_ptr[col + 1];
+                    float* row = tile + (f - f0) * nc;
+                    for (int64_t e = e0 + lane; e < e1; e += 64) row[ix.sum_cluster[e]] = qv * sv[e];
+                }
+                __syncthreads();
+                for (int c = tid; c < nc; c += kBlock) {
+                    float s = score[c];
+                    for (int f = 0; f < f1 - f0; ++f) s += tile[f * nc + c];
+                    score[c] = s;
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+        // ---- visiting order (seismic_common.h:105-115) ----
+        if (first_list) {
+            KeyItem* A = reinterpret_cast<KeyItem*>(uni);
+            for (int c = tid; c < nc; c += kBlock) A[c] = KeyItem{score[c], static_cast<uint32_t>(c)};
+            __syncthreads();
+            if (tid == 0) introsort_desc(A, nc);
+            __syncthreads();
+            for (int c = tid; c < nc; c += kBlock) order[c] = static_cast<uint16_t>(A[c].id);
+        } else {
+            for (int c = tid; c < nc; c += kBlock) order[c] = static_cast<uint16_t>(c);
+        }
+        if (tid == 0) {
+            misc[kMiscPos] = 0;
+            misc[kMiscPoff] = 0;
+            misc[kMiscEnd] = nc;
+            misc[kMiscDone] = 0;
+            misc[kMiscBreakPending] = 0;
+        }
+        __syncthreads();
+
+        // ---- speculative chunks + exact replay ----
+        while (true) {
+            if (wave == 0) {
+                const int pos = misc[kMiscPos], poff = misc[kMiscPoff], end = misc[kMiscEnd];
+                const int tkn = misc[kMiscTkN];
+                const bool full = tkn == k;
+                const float kth = full ? tks[k - 1] : 0.0F;
+                const int p = pos + lane;
+                bool valid = p < end;
+                const int c = valid ? order[p] : 0;
+                const int ctot = valid ? cstart[c + 1] - cstart[c] : 0;
+                const bool first_piece = !(lane == 0 && poff > 0);
+                bool fail = valid && first_piece && full && (score[c] * hf < kth);
+                int new_end = end;
+                if (first_list) {
+                    const uint64_t fm = ballot(fail);
+                    if (fm != 0) {
+                        const int fl = __ffsll(static_cast<unsigned long long>(fm)) - 1;
+                        valid = valid && lane < fl;
+                        fail = false;
+                        new_end = pos + fl;
+                    }
+                }
+                const bool pass = valid && !fail;
+                const int cnt = pass ? ctot - (lane == 0 ? poff : 0) : 0;
+                const int incl = wave_incl_scan(cnt, lane);
+                const int excl = incl - cnt;
+                const bool stopper = pass && (excl + cnt > kBudget);
+                const uint64_t sm = ballot(stopper);
+                const int stl = sm ? __ffsll(static_cast<unsigned long long>(sm)) - 1 : 64;
+                const bool partial = lane == stl && excl < kBudget;
+                const bool is_seg = pass && (lane < stl || partial);
+                const uint64_t segm = ballot(is_seg);
+                if (is_seg) {
+                    const int si = __popcll(segm & ((1ULL << lane) - 1));
+                    Seg s;
+                    s.cluster = c;
+                    s.doc_begin = cstart[c] + (lane == 0 ? poff : 0);
+                    s.count = partial ? kBudget - excl : cnt;
+                    s.slot = excl;
+                    s.test = first_piece ? 1 : 0;
+                    s.pos = p;
+                    seg[si] = s;
+                }
+                // consumed non-first-list failures were prune tests too
+                const uint64_t skipped = ballot(valid && fail && lane < stl);
+                int new_pos, new_poff;
+                if (sm) {
+                    new_pos = pos + stl;
+                    const int before = (stl == 0 ? poff : 0);
+                    new_poff = excl < kBudget ? before + (kBudget - excl) : before;
+                    // `excl`/`poff` above are lane stl's; broadcast them
+                    new_poff = __shfl(new_poff, stl, 64);
+                } else {
+                    new_pos = min(pos + 64, new_end);
+                    new_poff = 0;
+                }
+                const int total = __shfl(incl, 63, 64);
+                if (lane == 0) {
+                    misc[kMiscNseg] = __popcll(segm);
+                    misc[kMiscTotal] = min(total, kBudget);
+                    misc[kMiscPos] = new_pos;
+                    misc[kMiscPoff] = new_poff;
+                    misc[kMiscEnd] = new_end;
+                    st_tested += __popcll(skipped);
+                    if (new_end != end) misc[kMiscBreakPending] = 1;
+                }
+            }
+            __syncthreads();
+            const int nseg = misc[kMiscNseg];
+            const int total = misc[kMiscTotal];
+            // ---- gather doc ids, selector, row ranges ----
+            for (int j = tid; j < total; j += kBlock) {
+                int lo = 0, hi = nseg - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (seg[mid].slot <= j)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                const Seg& s = seg[lo];
+                const int32_t d = ix.docs[dbase + s.doc_begin + (j - s.slot)];
+                st_id[j] = d;
+                const bool keep = sel == nullptr || ((sel[d >> 5] >> (d & 31)) & 1u);
+                if (keep) {
+                    const int32_t r0 = ix.indptr[d];
+                    st_rs[j] = r0;
+                    st_len[j] = ix.indptr[d + 1] - r0;
+                } else {
+                    st_len[j] = -1;
+                }
+            }
+            __syncthreads();
+            // ---- score the staged rows ----
+            if constexpr (SQ) {
+                const uint2* idx2 = reinterpret_cast<const uint2*>(ix.indices);
+                const uint32_t* val4 = static_cast<const uint32_t*>(ix.values);
+                constexpr int NG = kBlock / G;
+                const int g = tid / G, gl = tid % G;
+                for (int j = g; j < total; j += NG) {
+                    const int len = st_len[j];
+                    uint32_t acc = 0;
+                    if (len > 0) {
+                        const int32_t rs = st_rs[j], re = rs + len;
+                        const int w0 = rs >> 2, w1 = (re + 3) >> 2;
+                        for (int w = w0 + gl; w < w1; w += G) {
+                            const uint2 ii = idx2[w];
+                            const uint32_t vv = val4[w];
+                            const int pbase = w << 2;
+                            const uint32_t tt[4] = {ii.x & 0xFFFFu, ii.x >> 16, ii.y & 0xFFFFu, ii.y >> 16};
+#pragma unroll
+                            for (int t = 0; t < 4; ++t) {
+                                const int pp = pbase + t;
+                                if (pp >= rs && pp < re) {
+                                    bool pr;
+                                    const int r = QL.rank_of(tt[t], pr);
+                                    if (pr) acc += ((vv >> (8 * t)) & 0xFFu) * static_cast<uint32_t>(rval[r]);
+                                }
+                            }
+                        }
+                    }
+                    acc = group_sum<G>(acc);
+                    if (gl == 0) st_sc[j] = static_cast<float>(acc);
+                }
+            } else {
+                const float* vf = static_cast<const float*>(ix.values);
+                for (int j = tid; j < total; j += kBlock) {
+                    const int len = st_len[j];
+                    if (len < 0) continue;
+                    const int32_t rs = st_rs[j];
+                    float acc = 0.0F;
+                    for (int pp = rs; pp < rs + len; ++pp) acc += vf[pp] * QL.value(ix.indices[pp]);
+                    st_sc[j] = acc;
+                }
+            }
+            __syncthreads();
+            // ---- exact replay of the chunk (wave 0) ----
+            if (wave == 0) {
+                int tkn = misc[kMiscTkN];
+                bool done = false;
+                for (int si = 0; si < nseg && !done; ++si) {
+                    const Seg s = seg[si];
+                    if (s.test) {
+                        ++st_tested;
+                        if (tkn == k && score[s.cluster] * hf < tks[k - 1]) {
+                            if (first_list) {
+                                done = true;
+                                break;
+                            }
+                            continue;
+                        }
+                        ++st_opened;
+                    }
+                    st_read += s.count;
+                    for (int b = 0; b < s.count; b += 64) {
+                        const int j = s.slot + b + lane;
+                        const bool in = b + lane < s.count;
+                        const int len = in ? st_len[j] : -1;
+                        const bool ok = len >= 0;
+                        const float sc = ok ? st_sc[j] : 0.0F;
+                        const int32_t id = ok ? st_id[j] : 0;
+                        st_scored += __popcll(ballot(ok));
+                        {
+                            int64_t nz = ok ? len : 0;
+#pragma unroll
+                            for (int d = 32; d > 0; d >>= 1) nz += __shfl_xor(nz, d, 64);
+                            st_nnz += nz;
+                        }
+                        bool taken = false;
+                        while (true) {
+                            const bool fullk = tkn == k;
+                            const float ks = fullk ? tks[k - 1] : 0.0F;
+                            const int32_t ki = fullk ? tki[k - 1] : 0;
+                            const bool cand = ok && !taken && (!fullk || better(sc, id, ks, ki));
+                            const uint64_t m = ballot(cand);
+                            if (m == 0) break;
+                            const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
+                            const float bs = __shfl(sc, bl, 64);
+                            const int32_t bi = __shfl(id, bl, 64);
+                            if (lane == bl) taken = true;
+                            bool dup = false, pos_cnt = false;
+                            int npos = 0;
+                            for (int r0 = 0; r0 < tkn; r0 += 64) {
+                                const int e = r0 + lane;
+                                const bool live = e < tkn;
+                                const float es = live ? tks[e] : 0.0F;
+                                const int32_t ei = live ? tki[e] : 0;
+                                dup = dup || (live && ei == bi);
+                                pos_cnt = live && better(es, ei, bs, bi);
+                                npos += __popcll(ballot(pos_cnt));
+                            }
+                            if (ballot(dup) != 0) continue;
+                            const int newn = min(tkn + 1, k);
+                            // shift [npos, newn-1) up by one, highest block first
+                            const int nmove = newn - 1 - npos;
+                            for (int r0 = ((nmove - 1) / 64) * 64; r0 >= 0 && nmove > 0; r0 -= 64) {
+                                const int e = npos + r0 + lane;
+                                const bool mv = r0 + lane < nmove;
+                                float ms = 0.0F;
+                                int32_t mi = 0;
+                                if (mv) {
+                                    ms = tks[e];
+                                    mi = tki[e];
+                                }
+                                __builtin_amdgcn_wave_barrier();
+                                if (mv) {
+                                    tks[e + 1] = ms;
+                                    tki[e + 1] = mi;
+                                }
+                                __builtin_amdgcn_wave_barrier();
+                            }
+                            if (lane == 0) {
+                                tks[npos] = bs;
+                                tki[npos] = bi;
+                            }
+                            __builtin_amdgcn_wave_barrier();
+                            tkn = newn;
+                        }
+                    }
+                }
+                if (lane == 0) {
+                    misc[kMiscTkN] = tkn;
+                    if (done) {
+                        misc[kMiscDone] = 1;
+                    } else if (misc[kMiscPos] >= misc[kMiscEnd] && misc[kMiscPoff] == 0) {
+                        misc[kMiscDone] = 1;
+                        if (misc[kMiscBreakPending]) ++st_tested;
+                    }
+                }
+            }
+            __syncthreads();
+            if (misc[kMiscDone]) break;
+        }
+        first_list = false;
+        __syncthreads();
+    }
+
+    // ---- results: best first, (-1.0f, -1) padding ----
+    const int tkn = misc[kMiscTkN];
+    for (int i = tid; i < a.k_max; i += kBlock) {
+        float d = -1.0F;
+        int32_t l = -1;
+        if (i < k && i < tkn) {
+            d = SQ ? tks[i] * a.decode_scale : tks[i];
+            l = tki[i];
+        }
+        out_d[i] = d;
+        out_l[i] = l;
+    }
+    if (a.stats != nullptr && tid == 0) {
+        int64_t* s = a.stats + q * kStatFields;
+        s[kStatLists] = st_lists;
+        s[kStatTermsFound] = st_found;
+        s[kStatCscEntries] = st_csc;
+        s[kStatClustersTested] = st_tested;
+        s[kStatClustersOpened] = st_opened;
+        s[kStatDocsRead] = st_read;
+        s[kStatDocsScored] = st_scored;
+        s[kStatNnzScored] = st_nnz;
+    }
+}
+
+namespace {
+
+LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq) {
+    LdsPlan P{};
+    P.nw = (dim + 31) / 32;
+    P.mc = max(max_clusters, 1);
+    P.kmax = kmax;
+    const int staging = kBudget * 16;
+    const int sortbuf = P.mc * 8;
+    const int tile = sq ? 0 : kTileFloats * 4;
+    P.uni_bytes = max(max(staging, sortbuf), max(tile, kQMax * 8));
+    int o = 0;
+    auto take = [&](int bytes) {
+        const int at = o;
+        o += (bytes + 15) & ~15;
+        return at;
+    };
+    P.o_qbits = take(P.nw * 4);
+    P.o_qpref = take(P.nw * 2);
+    P.o_eterm = take(kQMax * 2);
+    P.o_erank = take(kQMax * 2);
+    P.o_eval = take(kQMax * 4);
+    P.o_rlast = take(kQMax * 4);
+    P.o_rval = take(kQMax * 4);
+    P.o_rsum = take(kQMax * 4);
+    P.o_rcol = take(kQMax * 4);
+    P.o_flist = take(kQMax * 4);
+    P.o_cuts = take(kQMax * 2);
+    P.o_cstart = take((P.mc + 1) * 4);
+    P.o_score = take(P.mc * 4);
+    P.o_order = take(P.mc * 2);
+    P.o_uni = take(P.uni_bytes);
+    P.o_tks = take(kmax * 4);
+    P.o_tki = take(kmax * 4);
+    P.o_seg = take(kSegMax * static_cast<int>(sizeof(Seg)));
+    P.o_misc = take(kMiscCount * 4);
+    P.o_wsum = take(kWaves * 4);
+    P.total = o;
+    return P;
+}
+
+}  // namespace
+
+// ExactMatcher::single_query (exact_matcher.h:23-41): every id of a small
+// enumerable selector is scored against the dense query and the <= k results
+// are written best first, ties by doc id. One workgroup per query.
+template <bool SQ>
+__global__ __launch_bounds__(kBlock) void exact_match_kernel(ExactArgs a, int nw) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* qbits = reinterpret_cast<uint32_t*>(smem);
+    uint16_t* qpref = reinterpret_cast<uint16_t*>(qbits + nw);
+    float* rval = reinterpret_cast<float*>(smem + ((nw * 6 + 15) & ~15));
+    int32_t* rlast = reinterpret_cast<int32_t*>(rval + kQMax);
+    float* sc = reinterpret_cast<float*>(rlast + kQMax);
+    int32_t* wsum = reinterpret_cast<int32_t*>(sc + kQMax);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t q = blockIdx.x;
+    if (q >= a.n_queries) return;
+    const int64_t s0 = a.q_indptr[q];
+    const int qn = static_cast<int>(a.q_indptr[q + 1] - s0);
+    const DevIndexView& ix = a.ix;
+    for (int w = tid; w < nw; w += kBlock) qbits[w] = 0;
+    for (int r = tid; r < qn; r += kBlock) rlast[r] = -1;
+    __syncthreads();
+    for (int i = tid; i < qn; i += kBlock) {
+        const uint32_t t = a.q_indices[s0 + i];
+        atomicOr(&qbits[t >> 5], 1u << (t & 31));
+    }
+    __syncthreads();
+    {
+        const int per = (nw + kBlock - 1) / kBlock;
+        const int w0 = tid * per;
+        int local = 0;
+        for (int w = w0; w < w0 + per && w < nw; ++w) local += __popc(qbits[w]);
+        const int incl = wave_incl_scan(local, lane);
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        int run = incl - local;
+        for (int w = 0; w < wave; ++w) run += wsum[w];
+        for (int w = w0; w < w0 + per && w < nw; ++w) {
+            qpref[w] = static_cast<uint16_t>(run);
+            run += __popc(qbits[w]);
+        }
+    }
+    __syncthreads();
+    const QueryLds QL{qbits, qpref, rval};
+    for (int i = tid; i < qn; i += kBlock) {
+        bool p;
+        atomicMax(&rlast[QL.rank_of(a.q_indices[s0 + i], p)], i);
+    }
+    __syncthreads();
+    // SparseVectors::get_dense_vector: the last entry of a repeated term wins.
+    for (int r = tid; r < qn; r += kBlock) {
+        const int i = rlast[r];
+        if (i < 0) continue;
+        const float v = a.q_values[s0 + i];
+        if constexpr (SQ) {
+            float x = (v - a.q_vmin) * a.q_scale;
+            x = (x < 255.0F) ? x : 255.0F;
+            x = (0.0F < x) ? x : 0.0F;
+            rval[r] = static_cast<float>(static_cast<uint32_t>(roundf(x)));
+        } else {
+            rval[r] = v;
+        }
+    }
+    __syncthreads();
+    for (int j = tid; j < a.n_ids; j += kBlock) {
+        const int32_t d = a.ids[j];
+        const int32_t r0 = ix.indptr[d], r1 = ix.indptr[d + 1];
+        if constexpr (SQ) {
+            const uint8_t* vv = static_cast<const uint8_t*>(ix.values);
+            int acc = 0;
+            for (int p = r0; p < r1; ++p) acc += static_cast<int>(vv[p]) * static_cast<int>(QL.value(ix.indices[p]));
+            sc[j] = static_cast<float>(acc);
+        } else {
+            const float* vv = static_cast<const float*>(ix.values);
+            float acc = 0.0F;
+            for (int p = r0; p < r1; ++p) acc += vv[p] * QL.value(ix.indices[p]);
+            sc[j] = acc;
+        }
+    }
+    __syncthreads();
+    float* out_d = a.out_d + q * a.k;
+    int32_t* out_l = a.out_l + q * a.k;
+    for (int j = tid; j < a.n_ids; j += kBlock) {
+        const float s = sc[j];
+        const int32_t id = a.ids[j];
+        int rank = 0;
+        for (int o = 0; o < a.n_ids; ++o) {
+            const float os = sc[o];
+            const int32_t oi = a.ids[o];
+            if (better(os, oi, s, id) || (os == s && oi == id && o < j)) ++rank;
+        }
+        out_d[rank] = SQ ? s * a.decode_scale : s;
+        out_l[rank] = id;
+    }
+    for (int j = a.n_ids + tid; j < a.k; j += kBlock) {
+        out_d[j] = -1.0F;
+        out_l[j] = -1;
+    }
+}
+
+int search_lds_bytes(int dim, int max_clusters, int kmax, bool sq) { return make_plan(dim, max_clusters, kmax, sq).total; }
+
+std::string device_error_message(int32_t code) {
+    switch (code) {
+        case kDevQueryTooLong: return "query has more non-zeros than the device path supports (512)";
+        case kDevTermOutOfRange: return "query term id out of range";
+        case kDevTooManyClusters: return "posting list has more clusters than the device plan";
+        case kDevBadK: return "per-query k outside [1, k_max]";
+        default: return "device search reported error " + std::to_string(code);
+    }
+}
+
+void launch_search_checked(const SearchArgs& args, int max_clusters, int kmax, bool sq, hipStream_t stream) {
+    const LdsPlan P = make_plan(args.ix.dim, max_clusters, kmax, sq);
+    if (P.total > 160 * 1024)
+        throw std::invalid_argument("search does not fit in LDS for this dimension / cluster count / k");
+    constexpr int64_t kMaxGrid = 1 << 30;
+    for (int64_t base = 0; base < args.n_queries; base += kMaxGrid) {
+        SearchArgs a = args;
+        a.query_base = base;
+        const int64_t nb = args.n_queries - base < kMaxGrid ? args.n_queries - base : kMaxGrid;
+        const dim3 grid(static_cast<unsigned>(nb));
+        const void* kfn = sq ? reinterpret_cast<const void*>(seismic_search_kernel<true, 16>)
+                             : reinterpret_cast<const void*>(seismic_search_kernel<false, 16>);
+        if (P.total > 64 * 1024)
+            hip_check(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, P.total),
+                      "hipFuncSetAttribute");
+        if (sq)
+            hipLaunchKernelGGL((seismic_search_kernel<true, 16>), grid, dim3(kBlock), P.total, stream, a, P);
+        else
+            hipLaunchKernelGGL((seismic_search_kernel<false, 16>), grid, dim3(kBlock), P.total, stream, a, P);
+        hip_check(hipGetLastError(), "search kernel launch");
+    }
+}
+
+void launch_exact_checked(const ExactArgs& a, bool sq, hipStream_t stream) {
+    if (a.n_ids > kQMax) throw std::invalid_argument("exact-match selector larger than the device path supports");
+    const int nw = (a.ix.dim + 31) / 32;
+    const int lds = ((nw * 6 + 15) & ~15) + kQMax * 12 + kWaves * 4;
+    const dim3 grid(static_cast<unsigned>(a.n_queries));
+    if (sq)
+        hipLaunchKernelGGL((exact_match_kernel<true>), grid, dim3(kBlock), lds, stream, a, nw);
+    else
+        hipLaunchKernelGGL((exact_match_kernel<false>), grid, dim3(kBlock), lds, stream, a, nw);
+    hip_check(hipGetLastError(), "exact-match kernel launch");
+}
+
+}  // namespace nsparse_amd

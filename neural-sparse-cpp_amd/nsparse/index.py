@@ -1,0 +1,269 @@
+"""Index, search parameters and selectors over the C ABI.
+
+Argument handling follows the reference's class_wrappers.py:22-58 (inputs are
+converted with np.ascontiguousarray to int32 / uint16 / float32) and its
+typemaps (outputs are freshly allocated n x k arrays).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import P, SearchParams, Selector, check, lib
+
+kUseMmap = 1  # IndexIoFlag::kUseMmap (nsparse/io/index_io.h:17-22)
+
+_ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+
+
+# ---------------------------------------------------------------- selectors
+class _SelectorBase:
+    """IDSelector (nsparse/id_selector.h:22-86); holds its C struct and buffers."""
+
+    def _struct(self):
+        raise NotImplementedError
+
+
+class _EnumerableSelector(_SelectorBase):
+    _kind = None
+
+    def __init__(self, *args):
+        # SWIG exposes both (ids_array) and (n, ids_array) forms.
+        ids = args[-1]
+        self.ids = np.ascontiguousarray(ids, dtype=np.int32)
+        if len(args) == 2:
+            self.ids = self.ids[: int(args[0])]
+        self._c = Selector(self._kind, len(self.ids), self.ids.ctypes.data_as(P(ctypes.c_int32)), None, None)
+
+    def _struct(self):
+        return self._c
+
+
+class SetIDSelector(_EnumerableSelector):
+    _kind = 1
+
+
+class ArrayIDSelector(_EnumerableSelector):
+    _kind = 0
+
+
+class NotIDSelector(_SelectorBase):
+    """Non-owning like the reference: keep `inner` alive for the search."""
+
+    def __init__(self, inner):
+        self.inner = inner
+        self._c = Selector(2, 0, None, None, ctypes.pointer(inner._struct()))
+
+    def _struct(self):
+        return self._c
+
+
+class BitmapIDSelector(_SelectorBase):
+    """Membership bitmap over doc ids (bit i of word i // 32). Not enumerable.
+
+    Accepts a boolean mask of length n, or uint32 words plus the bit count.
+    """
+
+    def __init__(self, mask_or_words, n_bits=None):
+        arr = np.asarray(mask_or_words)
+        if arr.dtype == np.bool_:
+            n_bits = len(arr)
+            padded = np.zeros(((n_bits + 31) // 32) * 32, dtype=np.uint8)
+            padded[:n_bits] = arr
+            arr = np.packbits(padded.reshape(-1, 32)[:, ::-1].reshape(-1, 8), axis=1, bitorder="big")
+            arr = arr.reshape(-1, 4)[:, ::-1].copy().view(np.uint32).reshape(-1)
+        self.words = np.ascontiguousarray(arr, dtype=np.uint32)
+        self.n_bits = int(n_bits if n_bits is not None else 32 * len(self.words))
+        self._c = Selector(3, self.n_bits, None, self.words.ctypes.data_as(P(ctypes.c_uint32)), None)
+
+    def _struct(self):
+        return self._c
+
+
+# ---------------------------------------------------------------- parameters
+class SearchParameters:
+    """SearchParameters (nsparse/index.h:21-28): an optional, non-owning selector."""
+
+    _kind = 0
+
+    def __init__(self):
+        self.cut = 10
+        self.heap_factor = 1.0
+        self.vmin = 0.0
+        self.vmax = 1.0
+        self._selector = None
+
+    def set_id_selector(self, selector):
+        self._selector = selector
+
+    def get_id_selector(self):
+        return self._selector
+
+    def _struct(self):
+        sel = ctypes.pointer(self._selector._struct()) if self._selector is not None else None
+        return SearchParams(self._kind, int(self.cut), float(self.heap_factor), float(self.vmin),
+                            float(self.vmax), sel)
+
+
+class SeismicSearchParameters(SearchParameters):
+    """SeismicSearchParameters(cut=10, heap_factor=1.0) (nsparse/seismic_index.h:26-32)."""
+
+    _kind = 1
+
+    def __init__(self, cut=10, heap_factor=1.0):
+        super().__init__()
+        self.cut = cut
+        self.heap_factor = heap_factor
+
+
+class SeismicSQSearchParameters(SeismicSearchParameters):
+    """SeismicSQSearchParameters(vmin, vmax, cut, heap_factor)
+    (nsparse/seismic_scalar_quantized_index.h:27-33)."""
+
+    _kind = 2
+
+    def __init__(self, vmin, vmax, cut=10, heap_factor=1.0):
+        super().__init__(cut, heap_factor)
+        self.vmin = vmin
+        self.vmax = vmax
+
+
+# ---------------------------------------------------------------- index
+class Index:
+    """Owning handle of an index created by index_factory / read_index."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.nsparse_free(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def id(self):
+        buf = (ctypes.c_char * 4)()
+        check(lib.nsparse_index_id(self._h, buf))
+        return buf.raw.decode()
+
+    def get_dimension(self):
+        v = ctypes.c_int()
+        check(lib.nsparse_get_dimension(self._h, ctypes.byref(v)))
+        return v.value
+
+    def num_vectors(self):
+        v = ctypes.c_int64()
+        check(lib.nsparse_num_vectors(self._h, ctypes.byref(v)))
+        return v.value
+
+    @staticmethod
+    def _csr(indptr, indices, values):
+        return (np.ascontiguousarray(indptr, dtype=np.int32), np.ascontiguousarray(indices, dtype=np.uint16),
+                np.ascontiguousarray(values, dtype=np.float32))
+
+    def add(self, n, indptr, indices, values):
+        ip, ix, v = self._csr(indptr, indices, values)
+        check(lib.nsparse_add(self._h, int(n), _ptr(ip), _ptr(ix), _ptr(v)))
+
+    def add_with_ids(self, n, indptr, indices, values, ids):
+        ip, ix, v = self._csr(indptr, indices, values)
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        check(lib.nsparse_add_with_ids(self._h, int(n), _ptr(ip), _ptr(ix), _ptr(v), _ptr(ids)))
+
+    def build(self):
+        check(lib.nsparse_build(self._h))
+
+    def search(self, n, indptr, indices, values, k, params=None):
+        """Returns (distances float32 n x k, labels int32 n x k), best first."""
+        ip, ix, v = self._csr(indptr, indices, values)
+        n, k = int(n), int(k)
+        d = np.empty((max(n, 0), max(k, 0)), dtype=np.float32)
+        lab = np.empty((max(n, 0), max(k, 0)), dtype=np.int32)
+        ps = ctypes.byref(params._struct()) if params is not None else None
+        check(lib.nsparse_search(self._h, n, _ptr(ip), _ptr(ix), _ptr(v), k, _ptr(d), _ptr(lab), ps))
+        return d, lab
+
+    def search_device(self, indptr, indices, values, k_max, ks=None, params=None, out_d=None, out_l=None,
+                      stream=None, stats=None, selector_of_query=None, bitmaps=None):
+        """Batched search over torch tensors already on the GPU (HBM-resident
+        queries); returns (distances, labels) tensors of shape n x k_max."""
+        import torch
+
+        n = indptr.numel() - 1
+        dev = indptr.device
+        if out_d is None:
+            out_d = torch.empty((n, k_max), dtype=torch.float32, device=dev)
+        if out_l is None:
+            out_l = torch.empty((n, k_max), dtype=torch.int32, device=dev)
+        ps = ctypes.byref(params._struct()) if params is not None else None
+        s = ctypes.c_void_p(stream.cuda_stream if stream is not None else 0)
+        ksp = ctypes.c_void_p(ks.data_ptr()) if ks is not None else None
+        if stats is not None:
+            check(lib.nsparse_search_device_stats(self._h, n, indptr.data_ptr(), indices.data_ptr(),
+                                                  values.data_ptr(), ksp, int(k_max), ps, out_d.data_ptr(),
+                                                  out_l.data_ptr(), stats.data_ptr(), s))
+        else:
+            sel = ctypes.c_void_p(selector_of_query.data_ptr()) if selector_of_query is not None else None
+            bm = ctypes.c_void_p(bitmaps.data_ptr()) if bitmaps is not None else None
+            nb = int(bitmaps.numel()) if bitmaps is not None else 0
+            check(lib.nsparse_search_device(self._h, n, indptr.data_ptr(), indices.data_ptr(), values.data_ptr(),
+                                            ksp, int(k_max), ps, sel, bm, nb, out_d.data_ptr(), out_l.data_ptr(),
+                                            s))
+        return out_d, out_l
+
+    def device_error(self):
+        v = ctypes.c_int32()
+        check(lib.nsparse_device_error(self._h, ctypes.byref(v)))
+        return v.value
+
+    def read_csr(self, path, residency=0):
+        check(lib.nsparse_read_csr(self._h, str(path).encode(), int(residency)))
+
+    def get_vectors(self):
+        """Stored forward index as (indptr int32, indices uint16, values bytes)."""
+        n, nnz, esz = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        check(lib.nsparse_get_vectors(self._h, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(esz), None, None,
+                                      None))
+        ip = np.empty(n.value + 1 if n.value else 0, dtype=np.int32)
+        ix = np.empty(nnz.value, dtype=np.uint16)
+        v = np.empty(nnz.value * esz.value, dtype=np.uint8)
+        check(lib.nsparse_get_vectors(self._h, None, None, None, _ptr(ip), _ptr(ix), _ptr(v)))
+        return ip, ix, v
+
+
+def index_factory(dim, description):
+    h = ctypes.c_void_p()
+    check(lib.nsparse_index_factory(int(dim), description.encode(), ctypes.byref(h)))
+    return Index(h.value)
+
+
+def write_index(index, path):
+    check(lib.nsparse_write_index(index.handle, str(path).encode()))
+
+
+def serialize_index(index):
+    """write_index into memory; returns bytes."""
+    buf = P(ctypes.c_uint8)()
+    size = ctypes.c_size_t()
+    check(lib.nsparse_write_index_buffer(index.handle, ctypes.byref(buf), ctypes.byref(size)))
+    try:
+        return ctypes.string_at(buf, size.value)
+    finally:
+        lib.nsparse_buffer_free(buf)
+
+
+def read_index(path, io_flags=0):
+    h = ctypes.c_void_p()
+    check(lib.nsparse_read_index(str(path).encode(), int(io_flags), ctypes.byref(h)))
+    return Index(h.value)
+
+
+def read_index_bytes(data, io_flags=0):
+    h = ctypes.c_void_p()
+    buf = ctypes.create_string_buffer(bytes(data), len(data))
+    check(lib.nsparse_read_index_buffer(buf, len(data), int(io_flags), ctypes.byref(h)))
+    return Index(h.value)

@@ -853,6 +853,19 @@ int oracle_search(void* h, int n, const idx_t* indptr, const term_t* indices, co
     });
 }
 
+// Copies the forward index and posting lists of `src` into `dst` (same kind),
+// so one build can be searched under both tie modes.
+int oracle_copy_index(void* dst, const void* src) {
+    return guard([&] {
+        auto* d = static_cast<Index*>(dst);
+        const auto* s = static_cast<const Index*>(src);
+        d->fwd = s->fwd;
+        d->lists = s->lists;
+        d->has_vectors = s->has_vectors;
+        d->q = s->q;
+    });
+}
+
 int oracle_serialize(void* h, uint8_t** out, size_t* size) {
     return guard([&] {
         Writer w;

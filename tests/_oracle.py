@@ -1,0 +1,169 @@
+"""ctypes access to the oracle (oracle/_build/liboracle.so) and, where it was
+built, to the reference's own build-side code (oracle/_ref/libnsparse_ref.so).
+Test infrastructure only."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libnsparse_ref.so")
+P = ctypes.c_void_p
+f32 = ctypes.c_float
+
+
+def _ensure_built():
+    if not os.path.exists(ORACLE_SO):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "oracle"], check=True)
+
+
+_ensure_built()
+lib = ctypes.CDLL(ORACLE_SO)
+lib.oracle_last_error.restype = ctypes.c_char_p
+lib.oracle_sq_decode.restype = ctypes.c_float
+lib.oracle_sq_decode.argtypes = [f32, f32, f32, f32, f32]
+lib.oracle_mt19937_nth.restype = ctypes.c_uint32
+lib.oracle_mt19937_nth.argtypes = [ctypes.c_uint32, ctypes.c_int]
+ref = ctypes.CDLL(REF_SO) if os.path.exists(REF_SO) else None
+if ref is not None:
+    ref.ref_sq_decode.restype = ctypes.c_float
+    ref.ref_sq_decode.argtypes = [f32, f32, f32, f32, f32]
+    ref.ref_dot_u8.restype = ctypes.c_float
+    ref.ref_dot_f32.restype = ctypes.c_float
+
+STAT_FIELDS = ["lists", "terms_found", "csc_entries", "clusters_tested", "clusters_opened", "docs_read",
+               "docs_scored", "nnz_scored"]
+
+
+def _p(a):
+    return a.ctypes.data_as(P)
+
+
+def _chk(rc):
+    if rc != 0:
+        raise RuntimeError(lib.oracle_last_error().decode())
+
+
+class OracleIndex:
+    """The CPU restatement. tie_mode 0 = libstdc++ ties (the reference's),
+    1 = exact float ties broken by id (what the device path implements)."""
+
+    def __init__(self, dim, sq, lam, beta, alpha, seed, vmin=0.0, vmax=1.0, tie_mode=1):
+        self.h = P()
+        self.sq = sq
+        _chk(lib.oracle_create(dim, int(sq), f32(vmin), f32(vmax), lam, beta, f32(alpha), seed, tie_mode,
+                               ctypes.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.oracle_free(self.h)
+
+    def add(self, ip, ix, v):
+        ip = np.ascontiguousarray(ip, np.int32)
+        ix = np.ascontiguousarray(ix, np.uint16)
+        v = np.ascontiguousarray(v, np.float32)
+        _chk(lib.oracle_add(self.h, len(ip) - 1, _p(ip), _p(ix), _p(v)))
+
+    def build(self, threads=8):
+        _chk(lib.oracle_build(self.h, threads))
+
+    def copy_from(self, other):
+        _chk(lib.oracle_copy_index(self.h, other.h))
+
+    def serialize(self):
+        buf = ctypes.POINTER(ctypes.c_uint8)()
+        sz = ctypes.c_size_t()
+        _chk(lib.oracle_serialize(self.h, ctypes.byref(buf), ctypes.byref(sz)))
+        out = ctypes.string_at(buf, sz.value)
+        lib.oracle_buffer_free(buf)
+        return out
+
+    def search(self, ip, ix, v, k, cut=10, hf=1.0, sq_range=None, selector=None, negate=False, threads=8,
+               with_stats=False):
+        """selector: None, ("array", ids), ("set", ids) or ("bitmap", words, nbits)."""
+        ip = np.ascontiguousarray(ip, np.int32)
+        ix = np.ascontiguousarray(ix, np.uint16)
+        v = np.ascontiguousarray(v, np.float32)
+        n = len(ip) - 1
+        d = np.empty((n, k), np.float32)
+        lab = np.empty((n, k), np.int32)
+        stats = np.zeros((n, 8), np.int64)
+        kind, sn, ids, bm = -1, 0, None, None
+        if selector is not None:
+            if selector[0] in ("array", "set"):
+                kind = 0 if selector[0] == "array" else 1
+                ids = np.ascontiguousarray(selector[1], np.int32)
+                sn = len(ids)
+            else:
+                kind = 3
+                bm = np.ascontiguousarray(selector[1], np.uint32)
+                sn = int(selector[2])
+        qsq = 1 if sq_range is not None else 0
+        lo, hi = sq_range if sq_range is not None else (0.0, 1.0)
+        _chk(lib.oracle_search(self.h, n, _p(ip), _p(ix), _p(v), k, cut, f32(hf), qsq, f32(lo), f32(hi), kind,
+                               ctypes.c_int64(sn), _p(ids) if ids is not None else None,
+                               _p(bm) if bm is not None else None, int(negate), _p(d), _p(lab), _p(stats),
+                               threads))
+        return (d, lab, stats) if with_stats else (d, lab)
+
+
+def ref_build_bytes(dim, sq, ip, ix, v, lam, beta, alpha, seed, vmin=0.0, vmax=1.0):
+    """write_index bytes produced by the reference's own build-side code."""
+    ip = np.ascontiguousarray(ip, np.int32)
+    ix = np.ascontiguousarray(ix, np.uint16)
+    v = np.ascontiguousarray(v, np.float32)
+    buf = ctypes.POINTER(ctypes.c_uint8)()
+    sz = ctypes.c_size_t()
+    rc = ref.ref_build_serialize(dim, int(sq), f32(vmin), f32(vmax), len(ip) - 1, _p(ip), _p(ix), _p(v), lam,
+                                 beta, f32(alpha), ctypes.c_uint32(seed), ctypes.byref(buf), ctypes.byref(sz))
+    if rc != 0:
+        raise RuntimeError(ref.ref_last_error().decode())
+    out = ctypes.string_at(buf, sz.value)
+    ref.ref_buffer_free(buf)
+    return out
+
+
+def cluster_order(keys):
+    keys = np.ascontiguousarray(keys, np.float32)
+    out = np.empty(len(keys), np.int64)
+    _chk(lib.oracle_cluster_order(_p(keys), len(keys), _p(out)))
+    return out
+
+
+def top_k_tokens_f32(terms, weights, k):
+    terms = np.ascontiguousarray(terms, np.uint16)
+    weights = np.ascontiguousarray(weights, np.float32)
+    out = np.empty(len(terms) + 1, np.uint16)
+    n = ctypes.c_int()
+    _chk(lib.oracle_top_k_tokens_f32(_p(terms), _p(weights), len(terms), k, _p(out), ctypes.byref(n)))
+    return out[: n.value]
+
+
+def brute_force_top_k(docs, queries, dim, k):
+    """Exact top-k by dot product, score desc then id asc, positive scores only
+    (the reference's python_tests/oracle.py:28-47), in float64 via scipy."""
+    import scipy.sparse as sp
+
+    D = sp.csr_matrix((docs[2].astype(np.float64), docs[1].astype(np.int64), docs[0].astype(np.int64)),
+                      shape=(len(docs[0]) - 1, dim))
+    Q = sp.csr_matrix((queries[2].astype(np.float64), queries[1].astype(np.int64),
+                       queries[0].astype(np.int64)), shape=(len(queries[0]) - 1, dim))
+    S = (Q @ D.T).toarray()
+    labels = np.full((S.shape[0], k), -1, np.int64)
+    for q in range(S.shape[0]):
+        (el,) = np.nonzero(S[q] > 0.0)
+        order = el[np.lexsort((el, -S[q][el]))][:k]
+        labels[q, : len(order)] = order
+    return labels
+
+
+def recall_at_k(got, want):
+    per = []
+    for g, w in zip(got, want):
+        ws = {int(i) for i in w if i >= 0}
+        if not ws:
+            continue
+        per.append(len({int(i) for i in g if i >= 0} & ws) / len(ws))
+    return float(np.mean(per)) if per else 1.0

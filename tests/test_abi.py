@@ -1,0 +1,154 @@
+"""C-ABI surface on the CPU: the library loads, exports every symbol that
+include/*.h declares, keeps the reference's error behaviour, and round-trips
+the reference's serialized layout byte for byte. No kernel runs here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import _oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(nsparse_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol(nsparse):
+    lib = nsparse._lib.lib
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_factory_strings_and_errors(nsparse):
+    ix = nsparse.index_factory(30522, "seismic,lambda=2000|beta=200|alpha=0.4|seed=0|centroid_fraction=0.1")
+    assert ix.id() == "SEIS" and ix.get_dimension() == 30522 and ix.num_vectors() == 0
+    sq = nsparse.index_factory(100, "seismic_sq,quantizer=8bit|vmin=0.5|vmax=3|lambda=10|beta=2|alpha=0.4")
+    assert sq.id() == "SESQ"
+    im = nsparse.index_factory(100, "idmap,seismic_sq,quantizer=8bit|vmin=0.0|vmax=3.0|lambda=10|beta=2")
+    assert im.id() == "IDMP"
+    with pytest.raises(ValueError):
+        nsparse.index_factory(10, "nope")
+    with pytest.raises(ValueError):
+        nsparse.index_factory(10, "")
+    with pytest.raises(ValueError):
+        nsparse.index_factory(10, "idmap")
+    with pytest.raises(ValueError):
+        nsparse.index_factory(10, "seismic_sq,vmin=2|vmax=1")
+    with pytest.raises(ValueError):
+        nsparse.index_factory(10, "seismic,lambda=abc")
+    with pytest.raises(RuntimeError):
+        nsparse.index_factory(10, "brutal")
+
+
+def test_argument_checks(nsparse):
+    ix = nsparse.index_factory(10, "seismic,lambda=10|beta=2")
+    with pytest.raises(ValueError):
+        ix.add(0, [0], [], [])
+    with pytest.raises(ValueError):
+        ix.add(1, [0, 1], [10], [1.0])  # term id >= dim
+    with pytest.raises(RuntimeError):
+        ix.add_with_ids(1, [0, 1], [1], [1.0], [5])
+    with pytest.raises(ValueError):
+        ix.search(0, [0], [], [], 3)
+    with pytest.raises(ValueError):
+        ix.search(1, [0, 1], [1], [1.0], 0)
+    # no vectors: the reference returns without touching the outputs
+    d, lab = ix.search(1, [0, 1], [1], [1.0], 3)
+    assert d.shape == (1, 3)
+
+
+def test_layout_round_trip_is_byte_identical(nsparse):
+    """An oracle-built index (reference layout) read by the library and
+    written back comes out byte for byte the same (host-side only)."""
+    rng = np.random.default_rng(2)
+    n, dim = 600, 300
+    ip = np.arange(0, 12 * n + 1, 12, dtype=np.int32)
+    ix = np.concatenate([np.sort(rng.choice(dim, 12, replace=False)) for _ in range(n)]).astype(np.uint16)
+    v = rng.lognormal(0, 0.7, 12 * n).astype(np.float32)
+    for sq in (0, 1):
+        o = O.OracleIndex(dim, sq, 50, 5, 0.4, 1, 0.0, float(v.max()), tie_mode=1)
+        o.add(ip, ix, v)
+        o.build()
+        blob = o.serialize()
+        got = nsparse.read_index_bytes(blob)
+        assert got.id() == ("SESQ" if sq else "SEIS")
+        assert got.num_vectors() == n
+        assert nsparse.serialize_index(got) == blob
+
+
+def test_corrupt_files_are_rejected(nsparse):
+    rng = np.random.default_rng(3)
+    ip = np.arange(0, 5 * 100 + 1, 5, dtype=np.int32)
+    ix = rng.integers(0, 50, 500).astype(np.uint16)
+    ix = np.concatenate([np.sort(rng.choice(50, 5, replace=False)) for _ in range(100)]).astype(np.uint16)
+    v = rng.random(500).astype(np.float32)
+    o = O.OracleIndex(50, 0, 20, 3, 0.4, 1, tie_mode=1)
+    o.add(ip, ix, v)
+    o.build()
+    blob = bytearray(o.serialize())
+    with pytest.raises(RuntimeError):
+        nsparse.read_index_bytes(bytes(blob[: len(blob) // 2]))
+    bad = bytearray(blob)
+    bad[0:4] = b"XXXX"
+    with pytest.raises(RuntimeError):
+        nsparse.read_index_bytes(bytes(bad))
+
+
+def test_file_round_trip_and_mmap(nsparse, tmp_path):
+    rng = np.random.default_rng(4)
+    ip = np.arange(0, 6 * 200 + 1, 6, dtype=np.int32)
+    ix = np.concatenate([np.sort(rng.choice(80, 6, replace=False)) for _ in range(200)]).astype(np.uint16)
+    v = rng.random(1200).astype(np.float32)
+    o = O.OracleIndex(80, 1, 30, 4, 0.4, 2, 0.0, 1.0, tie_mode=1)
+    o.add(ip, ix, v)
+    o.build()
+    p = tmp_path / "x.idx"
+    p.write_bytes(o.serialize())
+    for flag in (0, nsparse.kUseMmap):
+        got = nsparse.read_index(str(p), flag)
+        out = tmp_path / f"y{flag}.idx"
+        nsparse.write_index(got, str(out))
+        assert out.read_bytes() == p.read_bytes()
+
+
+def test_idmap_layout(nsparse):
+    """IDMP header + i2e map + nested delegate (id_map_index.cpp:76-88)."""
+    im = nsparse.index_factory(20, "idmap,seismic,lambda=10|beta=2|seed=1")
+    im.add_with_ids(2, [0, 1, 2], [1, 3], [0.5, 0.25], [7, 9])
+    blob = nsparse.serialize_index(im)
+    assert blob[:4] == b"IDMP"
+    assert int.from_bytes(blob[8:16], "little") == 2
+    assert np.frombuffer(blob[16:24], np.int32).tolist() == [7, 9]
+    assert blob[24:28] == b"SEIS"
+    back = nsparse.read_index_bytes(blob)
+    assert back.id() == "IDMP" and back.num_vectors() == 2
+    assert nsparse.serialize_index(back) == blob
+
+
+def test_read_csr(nsparse, tmp_path):
+    p = tmp_path / "m.csr"
+    ip = np.array([0, 2, 3], np.int64)
+    ix = np.array([1, 4, 2], np.int32)
+    v = np.array([0.5, 1.0, 2.0], np.float32)
+    p.write_bytes(np.array([2, 5, 3], np.int64).tobytes() + ip.tobytes() + ix.tobytes() + v.tobytes())
+    idx = nsparse.index_factory(5, "seismic,lambda=10|beta=2")
+    idx.read_csr(str(p))
+    gip, gix, gv = idx.get_vectors()
+    assert gip.tolist() == [0, 2, 3] and gix.tolist() == [1, 4, 2]
+    assert np.frombuffer(gv.tobytes(), np.float32).tolist() == [0.5, 1.0, 2.0]
+    with pytest.raises(ValueError):
+        nsparse.index_factory(3, "seismic").read_csr(str(p))

@@ -1,0 +1,79 @@
+"""GPU build parity: add() + build() on the device, then write_index, must give
+the oracle's bytes exactly (id tie mode: value ties broken by doc id / term
+id). For FP32 data, where values do not tie, that is also the reference's
+own layout, checked against oracle/_ref when it is present."""
+import numpy as np
+import pytest
+
+import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DIM = 30522
+
+
+def gpu_build(nsparse, desc, docs):
+    ip, ix, v = docs
+    idx = nsparse.index_factory(DIM, desc)
+    idx.add(len(ip) - 1, ip, ix, v)
+    idx.build()
+    return idx
+
+
+def oracle_bytes(docs, sq, lam, beta, alpha, seed, vmin=0.0, vmax=1.0, tie_mode=1):
+    ip, ix, v = docs
+    o = O.OracleIndex(DIM, sq, lam, beta, alpha, seed, vmin, vmax, tie_mode=tie_mode)
+    o.add(ip, ix, v)
+    o.build()
+    return o.serialize()
+
+
+@pytest.mark.parametrize("n,lam,beta", [(3000, 200, 20), (12000, 400, 40), (2000, -1, -1), (1500, 50, 0)])
+def test_sq_build_matches_oracle(nsparse, n, lam, beta):
+    from nsparse import synth
+
+    docs = synth.make_corpus(synth.doc_spec(seed=n), n)
+    vmax = float(docs[2].max())
+    desc = f"seismic_sq,quantizer=8bit|vmin=0|vmax={vmax:.9g}|lambda={lam}|beta={beta}|alpha=0.4|seed=5"
+    got = nsparse.serialize_index(gpu_build(nsparse, desc, docs))
+    want = oracle_bytes(docs, 1, lam, beta, 0.4, 5, 0.0, np.float32(vmax))
+    assert len(got) == len(want)
+    assert got == want
+
+
+@pytest.mark.parametrize("n,lam,beta,alpha", [(3000, 200, 20, 0.4), (6000, 500, 50, 0.5), (1000, 100, 0, 0.3)])
+def test_f32_build_matches_oracle_and_reference(nsparse, n, lam, beta, alpha):
+    from nsparse import synth
+
+    docs = synth.make_corpus(synth.doc_spec(seed=100 + n), n)
+    desc = f"seismic,lambda={lam}|beta={beta}|alpha={alpha}|seed=9"
+    got = nsparse.serialize_index(gpu_build(nsparse, desc, docs))
+    assert got == oracle_bytes(docs, 0, lam, beta, alpha, 9)
+    if O.ref is not None:
+        assert got == O.ref_build_bytes(DIM, 0, *docs, lam, beta, alpha, 9)
+
+
+def test_build_then_search_matches_oracle(nsparse):
+    from nsparse import synth
+
+    docs = synth.make_corpus(synth.doc_spec(seed=21), 8000)
+    vmax = float(docs[2].max())
+    desc = f"seismic_sq,quantizer=8bit|vmin=0|vmax={vmax:.9g}|lambda=300|beta=30|alpha=0.4|seed=1"
+    dev = gpu_build(nsparse, desc, docs)
+    o = O.OracleIndex(DIM, 1, 300, 30, 0.4, 1, 0.0, np.float32(vmax), tie_mode=1)
+    o.add(*docs)
+    o.build()
+    qip, qix, qv = synth.make_corpus(synth.query_spec(seed=22), 100)
+    want = o.search(qip, qix, qv, 10, cut=10, hf=0.8)
+    got = dev.search(len(qip) - 1, qip, qix, qv, 10, nsparse.SeismicSearchParameters(10, 0.8))
+    assert (got[1] == want[1]).all() and (got[0] == want[0]).all()
+
+
+def test_same_seed_same_index_different_seed_differs(nsparse):
+    from nsparse import synth
+
+    docs = synth.make_corpus(synth.doc_spec(seed=31), 2000)
+    a = nsparse.serialize_index(gpu_build(nsparse, "seismic,lambda=100|beta=10|alpha=0.4|seed=3", docs))
+    b = nsparse.serialize_index(gpu_build(nsparse, "seismic,lambda=100|beta=10|alpha=0.4|seed=3", docs))
+    c = nsparse.serialize_index(gpu_build(nsparse, "seismic,lambda=100|beta=10|alpha=0.4|seed=4", docs))
+    assert a == b and a != c

@@ -1,0 +1,196 @@
+"""GPU search parity: the HIP path (through the C ABI) against the oracle on
+the same index bytes and queries. Integer (SQ) and FP32 scores must be
+bit-identical; labels identical (the oracle runs in its id-tie mode, which is
+the device's documented tie rule); traversal counters identical to the
+oracle's except rows re-scored for docs already seen in an earlier list
+(the device checks those against its top-k instead of a visited set)."""
+import numpy as np
+import pytest
+
+import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DIM = 30522
+
+
+@pytest.fixture(scope="module")
+def data(nsparse):
+    from nsparse import synth
+
+    docs = synth.make_corpus(synth.doc_spec(seed=0), 20000)
+    queries = synth.make_corpus(synth.query_spec(seed=1), 150)
+    mixed = synth.make_corpus(synth.query_spec(seed=2, log_uniform=True), 150)
+    return docs, queries, mixed
+
+
+def build_pair(nsparse, docs, sq, lam=300, beta=30):
+    ip, ix, v = docs
+    vmax = float(v.max())
+    o = O.OracleIndex(DIM, sq, lam, beta, 0.4, 0, 0.0, vmax, tie_mode=1)
+    o.add(ip, ix, v)
+    o.build()
+    dev = nsparse.read_index_bytes(o.serialize())
+    return o, dev, vmax
+
+
+@pytest.fixture(scope="module")
+def sq_pair(nsparse, data):
+    return build_pair(nsparse, data[0], 1)
+
+
+@pytest.fixture(scope="module")
+def f32_pair(nsparse, data):
+    return build_pair(nsparse, data[0], 0)
+
+
+def params(nsparse, sq, cut, hf, vmax, sel=None, sq_range=None):
+    if sq and sq_range is not None:
+        p = nsparse.SeismicSQSearchParameters(sq_range[0], sq_range[1], cut, hf)
+    else:
+        p = nsparse.SeismicSearchParameters(cut, hf)
+    if sel is not None:
+        p.set_id_selector(sel)
+    return p
+
+
+def assert_same(got, want):
+    gd, gl = got
+    wd, wl = want
+    assert (gl == wl).all(), np.argwhere(gl != wl)[:5]
+    assert (gd == wd).all(), np.argwhere(gd != wd)[:5]
+
+
+@pytest.mark.parametrize("sq", [1, 0])
+@pytest.mark.parametrize("cut,hf,k", [(10, 0.8, 10), (10, 1.0, 100), (3, 0.7, 10), (1000, 0.9, 10),
+                                      (10, 1e9, 10), (1, 0.5, 5)])
+def test_search_matches_oracle(nsparse, data, sq_pair, f32_pair, sq, cut, hf, k):
+    o, dev, vmax = sq_pair if sq else f32_pair
+    for qs in (data[1], data[2]):
+        ip, ix, v = qs
+        want = o.search(ip, ix, v, k, cut=cut, hf=hf)
+        got = dev.search(len(ip) - 1, ip, ix, v, k, params(nsparse, sq, cut, hf, vmax))
+        assert_same(got, want)
+
+
+def test_sq_query_range_override(nsparse, data, sq_pair):
+    o, dev, vmax = sq_pair
+    ip, ix, v = data[1]
+    want = o.search(ip, ix, v, 10, cut=10, hf=0.8, sq_range=(0.1, 5.0))
+    got = dev.search(len(ip) - 1, ip, ix, v, 10, params(nsparse, 1, 10, 0.8, vmax, sq_range=(0.1, 5.0)))
+    assert_same(got, want)
+
+
+@pytest.mark.parametrize("sq", [1, 0])
+def test_selectors(nsparse, data, sq_pair, f32_pair, sq):
+    o, dev, vmax = sq_pair if sq else f32_pair
+    ip, ix, v = data[1]
+    rng = np.random.default_rng(7)
+    n_docs = len(data[0][0]) - 1
+    allowed = np.sort(rng.choice(n_docs, n_docs // 2, replace=False)).astype(np.int32)
+    banned = np.sort(rng.choice(n_docs, n_docs // 5, replace=False)).astype(np.int32)
+    # enumerable selector larger than k: the normal path with a filter
+    want = o.search(ip, ix, v, 10, cut=10, hf=0.9, selector=("set", allowed))
+    got = dev.search(len(ip) - 1, ip, ix, v, 10, params(nsparse, sq, 10, 0.9, vmax, nsparse.SetIDSelector(allowed)))
+    assert_same(got, want)
+    # NotIDSelector(Set): not enumerable, never exact
+    inner = nsparse.SetIDSelector(banned)
+    want = o.search(ip, ix, v, 10, cut=10, hf=0.9, selector=("set", banned), negate=True)
+    got = dev.search(len(ip) - 1, ip, ix, v, 10, params(nsparse, sq, 10, 0.9, vmax, nsparse.NotIDSelector(inner)))
+    assert_same(got, want)
+    # bitmap selector
+    mask = np.zeros(n_docs, bool)
+    mask[allowed] = True
+    bm = nsparse.BitmapIDSelector(mask)
+    want = o.search(ip, ix, v, 10, cut=10, hf=0.9, selector=("bitmap", bm.words, n_docs))
+    got = dev.search(len(ip) - 1, ip, ix, v, 10, params(nsparse, sq, 10, 0.9, vmax, bm))
+    assert_same(got, want)
+    # small selector (size <= k): ExactMatcher path, with a repeated id
+    few = np.array([allowed[3], allowed[1], allowed[3], allowed[7]], np.int32)
+    want = o.search(ip, ix, v, 10, cut=10, hf=0.9, selector=("array", few))
+    got = dev.search(len(ip) - 1, ip, ix, v, 10, params(nsparse, sq, 10, 0.9, vmax, nsparse.ArrayIDSelector(few)))
+    assert_same(got, want)
+
+
+def test_queries_with_repeated_terms_and_unsorted_rows(nsparse, sq_pair, f32_pair):
+    rng = np.random.default_rng(11)
+    rows = []
+    for _ in range(60):
+        m = int(rng.integers(1, 40))
+        t = rng.choice(200, m).astype(np.uint16)  # repeats allowed, unsorted
+        rows.append(t)
+    ip = np.r_[0, np.cumsum([len(r) for r in rows])].astype(np.int32)
+    ix = np.concatenate(rows)
+    v = rng.lognormal(0, 0.7, len(ix)).astype(np.float32)
+    for sq, (o, dev, vmax) in ((1, sq_pair), (0, f32_pair)):
+        want = o.search(ip, ix, v, 10, cut=5, hf=0.8)
+        got = dev.search(len(ip) - 1, ip, ix, v, 10, params(nsparse, sq, 5, 0.8, vmax))
+        assert_same(got, want)
+
+
+def test_empty_query_and_padding(nsparse, sq_pair):
+    o, dev, vmax = sq_pair
+    ip = np.array([0, 0, 1], np.int32)
+    ix = np.array([30000], np.uint16)  # rare term: few or no docs
+    v = np.array([1.0], np.float32)
+    want = o.search(ip, ix, v, 50, cut=10, hf=1.0)
+    got = dev.search(2, ip, ix, v, 50, params(nsparse, 1, 10, 1.0, vmax))
+    assert_same(got, want)
+    assert (got[1][0] == -1).all() and (got[0][0] == -1.0).all()
+
+
+def test_batched_device_mixed_k_equals_single(nsparse, data, sq_pair):
+    import torch
+
+    o, dev, vmax = sq_pair
+    ip, ix, v = data[2]
+    n = len(ip) - 1
+    ks = np.where(np.arange(n) % 2 == 0, 10, 100).astype(np.int32)
+    d_ip = torch.from_numpy(ip).cuda()
+    d_ix = torch.from_numpy(ix.view(np.int16)).cuda()
+    d_v = torch.from_numpy(v).cuda()
+    d_ks = torch.from_numpy(ks).cuda()
+    p = params(nsparse, 1, 10, 0.8, vmax)
+    stats = torch.zeros((n, 8), dtype=torch.int64, device="cuda")
+    dd, dl = dev.search_device(d_ip, d_ix, d_v, 100, ks=d_ks, params=p, stats=stats)
+    torch.cuda.synchronize()
+    assert dev.device_error() == 0
+    dd, dl, st = dd.cpu().numpy(), dl.cpu().numpy(), stats.cpu().numpy()
+    w10 = o.search(ip, ix, v, 10, cut=10, hf=0.8, with_stats=True)
+    w100 = o.search(ip, ix, v, 100, cut=10, hf=0.8, with_stats=True)
+    for q in range(n):
+        wd, wl, ws = w10 if ks[q] == 10 else w100
+        k = ks[q]
+        assert (dl[q, :k] == wl[q]).all() and (dd[q, :k] == wd[q]).all(), q
+        assert (dl[q, k:] == -1).all()
+        # lists, terms found, summary entries, prune tests, clusters opened, doc ids read
+        assert (st[q, :6] == ws[q, :6]).all(), (q, st[q], ws[q])
+        assert st[q, 6] >= ws[q, 6]
+
+
+def test_idmap_search(nsparse):
+    """IDMapIndex over a GPU-built SEISMIC-SQ index: labels come back external,
+    selectors are evaluated on external ids (id_map_index.cpp:29-60)."""
+    from nsparse import synth
+
+    ip, ix, v = synth.make_corpus(synth.doc_spec(seed=3), 3000)
+    ids = (np.arange(3000, dtype=np.int32) * 7 + 100)
+    im = nsparse.index_factory(DIM, f"idmap,seismic_sq,quantizer=8bit|vmin=0|vmax={float(v.max()):.9g}|"
+                                    "lambda=200|beta=20|alpha=0.4|seed=0")
+    im.add_with_ids(3000, ip, ix, v, ids)
+    im.build()
+    plain = nsparse.index_factory(DIM, f"seismic_sq,quantizer=8bit|vmin=0|vmax={float(v.max()):.9g}|"
+                                       "lambda=200|beta=20|alpha=0.4|seed=0")
+    plain.add(3000, ip, ix, v)
+    plain.build()
+    qip, qix, qv = synth.make_corpus(synth.query_spec(seed=4), 30)
+    p = nsparse.SeismicSearchParameters(10, 0.9)
+    d1, l1 = im.search(30, qip, qix, qv, 10, p)
+    d0, l0 = plain.search(30, qip, qix, qv, 10, p)
+    assert (d1 == d0).all()
+    assert (l1 == np.where(l0 >= 0, l0 * 7 + 100, -1)).all()
+    # selector on external ids
+    allowed_ext = ids[::3]
+    p.set_id_selector(nsparse.SetIDSelector(allowed_ext))
+    d2, l2 = im.search(30, qip, qix, qv, 10, p)
+    assert np.isin(l2[l2 >= 0], allowed_ext).all()
