@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""Batched SEISMIC-SQ search on one MI355X: QPS, roofline, recall, plain path
+and the host-CPU baseline, printed as one JSON line.
+
+Workload (defaults): 8,841,823 synthetic docs (vocab 30522, Zipf 1.1 terms
+without repeats, doc nnz lognormal median 120 clipped 8..600, FP32
+lognormal(0, 0.7) weights), uint8 scalar quantization over the corpus min/max,
+lambda=6000, beta=600, alpha=0.4; 10k queries with nnz log-uniform over 5..300
+and k in {10, 100} at random, cut=10, heap_factor=0.8. One step = one
+batched search launch over all 10k queries, queries and index resident in HBM.
+
+Beside it: the plain drop-in path (one query per launch, synchronous), recall@10
+against exact FP32 brute force on a query sample (computed on the GPU), and the
+CPU restatement of the reference search timed on this host over a bounded query
+sample on the very same index.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "neural-sparse-cpp_amd"))
+
+METRIC = "QPS at recall@10≥0.95, 8.8M-doc SEISMIC-SQ: batched vs plain; p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def host_threads():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def algorithmic_bytes(stats):
+    """Bytes a query's traversal must read (per row of the device counters):
+    per visited list its cluster range (8 B); per query entry found in the
+    summaries its column range (16 B) and entries (2 B cluster + 1 B value);
+    per opened cluster its doc range (16 B); per doc id read 4 B; per row
+    gathered its indptr pair (8 B) and nnz * (2 B term + 1 B code)."""
+    s = stats.astype(np.float64)
+    return (8 * s[:, 0] + 16 * s[:, 1] + 3 * s[:, 2] + 16 * s[:, 4] + 4 * s[:, 5] + 8 * s[:, 6] + 3 * s[:, 7])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--docs", type=int, default=8_841_823)
+    ap.add_argument("--queries", type=int, default=10_000)
+    ap.add_argument("--lam", type=int, default=6000)
+    ap.add_argument("--beta", type=int, default=600)
+    ap.add_argument("--alpha", type=float, default=0.4)
+    ap.add_argument("--cut", type=int, default=10)
+    ap.add_argument("--hf", type=float, default=0.8)
+    ap.add_argument("--plain-queries", type=int, default=300)
+    ap.add_argument("--recall-queries", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (query-sharded replicas are a "
+              "design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import nsparse
+    from nsparse import synth
+
+    dev = torch.device("cuda:0")
+    threads = host_threads()
+    t0 = time.time()
+    docs = synth.make_corpus(synth.doc_spec(seed=0), args.docs, threads=threads)
+    queries = synth.make_corpus(synth.query_spec(seed=1, log_uniform=True), args.queries, threads=threads)
+    ks = np.where(np.random.default_rng(2).random(args.queries) < 0.5, 10, 100).astype(np.int32)
+    log(f"corpus {args.docs} docs / {int(docs[0][-1])} nnz generated in {time.time() - t0:.1f}s")
+
+    vmin, vmax = float(docs[2].min()), float(docs[2].max())
+    desc = (f"seismic_sq,quantizer=8bit|vmin={vmin:.9g}|vmax={vmax:.9g}|lambda={args.lam}|beta={args.beta}|"
+            f"alpha={args.alpha}|seed=0")
+    index = nsparse.index_factory(30522, desc)
+    t0 = time.time()
+    index.add(args.docs, *docs)
+    add_s = time.time() - t0
+    t0 = time.time()
+    index.build()
+    torch.cuda.synchronize()
+    build_s = time.time() - t0
+    log(f"add {add_s:.1f}s, GPU build {build_s:.1f}s")
+
+    qip = torch.from_numpy(queries[0]).to(dev)
+    qix = torch.from_numpy(queries[1].view(np.int16)).to(dev)
+    qv = torch.from_numpy(queries[2]).to(dev)
+    d_ks = torch.from_numpy(ks).to(dev)
+    k_max = int(ks.max())
+    params = nsparse.SeismicSearchParameters(args.cut, args.hf)
+    out_d = torch.empty((args.queries, k_max), dtype=torch.float32, device=dev)
+    out_l = torch.empty((args.queries, k_max), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        index.search_device(qip, qix, qv, k_max, ks=d_ks, params=params, out_d=out_d, out_l=out_l, stream=stream)
+    torch.cuda.synchronize()
+    if index.device_error() != 0:
+        raise RuntimeError(f"device search error {index.device_error()}")
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        index.search_device(qip, qix, qv, k_max, ks=d_ks, params=params, out_d=out_d, out_l=out_l, stream=stream)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    qps = args.queries * args.steps / wall
+    log(f"batched: {qps:,.0f} QPS, {wall / args.steps * 1e3:.2f} ms/step, kernel {kernel_ms:.2f} ms")
+    labels = out_l.cpu().numpy()
+    dists = out_d.cpu().numpy()
+
+    stats = torch.zeros((args.queries, 8), dtype=torch.int64, device=dev)
+    index.search_device(qip, qix, qv, k_max, ks=d_ks, params=params, stats=stats, stream=stream)
+    torch.cuda.synchronize()
+    st = stats.cpu().numpy()
+    bytes_launch = float(algorithmic_bytes(st).sum())
+    achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
+
+    # ---- plain path: one query per launch, synchronous
+    n_plain = min(args.plain_queries, args.queries)
+    lat = []
+    for i in range(n_plain):
+        k = int(ks[i])
+        od = torch.empty((1, k), dtype=torch.float32, device=dev)
+        ol = torch.empty((1, k), dtype=torch.int32, device=dev)
+        t1 = time.perf_counter()
+        index.search_device(qip[i:i + 2], qix, qv, k, params=params, out_d=od, out_l=ol, stream=stream)
+        stream.synchronize()
+        lat.append(time.perf_counter() - t1)
+        if i < 20:
+            assert (ol.cpu().numpy()[0] == labels[i, :k]).all(), "plain and batched results differ"
+    lat = np.sort(np.array(lat))
+    plain_qps = n_plain / float(lat.sum())
+    pct = lambda f: float(lat[min(len(lat) - 1, int(math.floor(f * len(lat))))] * 1e3)  # noqa: E731
+    log(f"plain: {plain_qps:,.0f} QPS, p50 {pct(0.5):.3f} ms, p99 {pct(0.99):.3f} ms")
+
+    # ---- recall@10 against exact FP32 scores on the unquantized corpus (GPU)
+    import ctypes
+
+    n_rec = min(args.recall_queries, args.queries)
+    d_ip = torch.from_numpy(docs[0]).to(dev)
+    d_ix = torch.from_numpy(docs[1].view(np.int16)).to(dev)
+    d_v = torch.from_numpy(docs[2]).to(dev)
+    scores = torch.empty(args.docs, dtype=torch.float32, device=dev)
+    hits = []
+    for i in range(n_rec):
+        a, b = int(queries[0][i]), int(queries[0][i + 1])
+        nsparse._lib.check(nsparse._lib.lib.nsparse_exact_scores_device(
+            args.docs, d_ip.data_ptr(), d_ix.data_ptr(), d_v.data_ptr(), 30522, b - a,
+            ctypes.c_void_p(qix.data_ptr() + 2 * a), ctypes.c_void_p(qv.data_ptr() + 4 * a), scores.data_ptr(),
+            ctypes.c_void_p(stream.cuda_stream)))
+        top_v, top_i = torch.topk(scores, 64)
+        tv, ti = top_v.cpu().numpy(), top_i.cpu().numpy().astype(np.int64)
+        keep = tv > 0
+        tv, ti = tv[keep], ti[keep]
+        order = np.lexsort((ti, -tv))[:10]
+        want = set(ti[order].tolist())
+        if want:
+            got = set(int(x) for x in labels[i, :10] if x >= 0)
+            hits.append(len(got & want) / len(want))
+    recall = float(np.mean(hits)) if hits else 1.0
+    del d_ip, d_ix, d_v, scores
+    log(f"recall@10 = {recall:.4f} over {n_rec} queries")
+
+    # ---- CPU baseline: the reference's search restated in C++ (oracle/),
+    # on the same index bytes, all host threads, bounded query sample.
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle
+
+        blob = nsparse.serialize_index(index)
+        ref = _oracle.OracleIndex.load(blob, tie_mode=0)
+        del blob
+        done, t_cpu, agree, total = 0, 0.0, 0, 0
+        chunk = 64
+        while t_cpu < args.cpu_seconds and done < args.queries:
+            sl = slice(done, min(done + chunk, args.queries))
+            for k in (10, 100):
+                sel = np.nonzero(ks[sl] == k)[0] + sl.start
+                if len(sel) == 0:
+                    continue
+                rows = [queries[1][queries[0][j]:queries[0][j + 1]] for j in sel]
+                vals = [queries[2][queries[0][j]:queries[0][j + 1]] for j in sel]
+                sip = np.r_[0, np.cumsum([len(r) for r in rows])].astype(np.int32)
+                t1 = time.perf_counter()
+                cd, cl = ref.search(sip, np.concatenate(rows), np.concatenate(vals), k, cut=args.cut, hf=args.hf,
+                                    threads=threads)
+                t_cpu += time.perf_counter() - t1
+                for r, j in enumerate(sel):
+                    total += 1
+                    agree += int((cl[r] == labels[j, :k]).all())
+            done = sl.stop
+        cpu = {"value": done / t_cpu, "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": f"first {done} of the {args.queries} benchmark queries on the same index bytes, "
+                         f"OpenMP dynamic(64) over queries",
+               "topk_identical_frac": agree / max(total, 1)}
+        log(f"cpu: {cpu['value']:,.0f} QPS on {threads} threads, identical top-k {agree}/{total}")
+
+    result = {
+        "metric": METRIC, "value": qps, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": f"seismic_sq_batched_{args.docs}docs", "docs": args.docs, "queries": args.queries,
+                   "k": "10 or 100 per query", "lambda": args.lam, "beta": args.beta, "alpha": args.alpha,
+                   "cut": args.cut, "heap_factor": args.hf, "query_nnz": "log-uniform 5..300"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_launch, "kernel_ms": kernel_ms},
+        "cpu_baseline": cpu,
+        "plain": {"qps": plain_qps, "p50_ms": pct(0.5), "p99_ms": pct(0.99), "queries": n_plain},
+        "batched_over_plain": qps / plain_qps,
+        "batched_over_cpu": (qps / cpu["value"]) if cpu else None,
+        "recall_at_10": recall, "recall_queries": n_rec,
+        "build_s": build_s,
+        "traversal_means": dict(zip(["lists", "terms_found", "csc_entries", "clusters_tested", "clusters_opened",
+                                     "docs_read", "docs_scored", "nnz_scored"], st.mean(axis=0).tolist())),
+    }
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

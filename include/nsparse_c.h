@@ -141,6 +141,13 @@ int nsparse_search_device_stats(nsparse_index* index, int64_t n, const int32_t* 
  * Reading it synchronizes the device. Extension. */
 int nsparse_device_error(nsparse_index* index, int32_t* out);
 
+/* Exact scores of one query (host-free: all pointers DEVICE) against every row of
+ * an FP32 CSR -- the scoring of BrutalIndex::single_query (nsparse/brutal_index.cpp:75-96),
+ * used for recall ground truth. d_scores receives n_docs floats. Extension. */
+int nsparse_exact_scores_device(int64_t n_docs, const int32_t* d_indptr, const uint16_t* d_indices,
+                                const float* d_values, int dim, int q_len, const uint16_t* d_q_indices,
+                                const float* d_q_values, float* d_scores, void* stream);
+
 /* write_index(Index*, const char*) -- nsparse/io/index_io.cpp:88-131,170-173.
  * Byte layout identical to the reference writer. */
 int nsparse_write_index(nsparse_index* index, const char* path);

@@ -613,8 +613,11 @@ __global__ __launch_bounds__(kB) void summarize_u8_kernel(const int32_t* __restr
     }
 }
 
-// ---- FP32 summaries: the reference's sequential algorithm, one lane per
-// cluster step, LDS dense accumulator (order-sensitive float sums kept exact).
+// ---- FP32 summaries: the reference's sequential algorithm (one lane walks the
+// cluster's events in order so the running float sum rounds exactly as the
+// CPU's does), an LDS dense accumulator, and a workgroup bitonic sort of the
+// (max desc, term asc) pairs -- in LDS when they fit, else in this
+// workgroup's global scratch (`spill`, next_pow2(dim) entries).
 __global__ __launch_bounds__(kB) void summarize_f32_kernel(const int32_t* __restrict__ indptr,
                                                            const uint16_t* __restrict__ indices,
                                                            const float* __restrict__ values,
@@ -625,16 +628,19 @@ __global__ __launch_bounds__(kB) void summarize_f32_kernel(const int32_t* __rest
                                                            int cap, float alpha, int64_t* __restrict__ count,
                                                            const int64_t* __restrict__ out_off, bool write,
                                                            uint16_t* __restrict__ out_term, float* __restrict__ out_val,
-                                                           uint16_t* __restrict__ out_cl, int32_t* __restrict__ err) {
+                                                           uint16_t* __restrict__ out_cl, KeyItem* __restrict__ spill,
+                                                           int spill_stride) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nw = (dim + 31) / 32;
     float* acc = reinterpret_cast<float*>(smem);
     uint32_t* touch = reinterpret_cast<uint32_t*>(acc + dim);
     uint32_t* keepb = touch + nw;
     const size_t pairs_at = (static_cast<size_t>(dim) * 4 + static_cast<size_t>(nw) * 8 + 7) & ~size_t{7};
-    KeyItem* pairs = reinterpret_cast<KeyItem*>(smem + pairs_at);
-    int32_t* misc = reinterpret_cast<int32_t*>(pairs + cap);
-    const int tid = threadIdx.x;
+    KeyItem* lds_pairs = reinterpret_cast<KeyItem*>(smem + pairs_at);
+    int32_t* misc = reinterpret_cast<int32_t*>(lds_pairs + cap);
+    int32_t* wsum = misc + 4;
+    KeyItem* my_spill = spill + static_cast<size_t>(blockIdx.x) * spill_stride;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int w = tid; w < nw; w += kB) {
         touch[w] = 0;
         keepb[w] = 0;
@@ -651,10 +657,9 @@ __global__ __launch_bounds__(kB) void summarize_f32_kernel(const int32_t* __rest
                     const float v = values[p];
                     if (!((touch[t >> 5] >> (t & 31)) & 1u)) {
                         touch[t >> 5] |= 1u << (t & 31);
-                        const float value = (0.0F < v) ? v : 0.0F;
+                        const float value = (0.0F < v) ? v : 0.0F;  // std::max(0.0f, v)
                         acc[t] = value;
                         sum += value;
-                        if (m < cap) pairs[m] = KeyItem{0.0F, static_cast<uint32_t>(t)};
                         ++m;
                     } else if (v > acc[t]) {
                         sum += v - acc[t];
@@ -664,15 +669,38 @@ __global__ __launch_bounds__(kB) void summarize_f32_kernel(const int32_t* __rest
             }
             misc[0] = m;
             reinterpret_cast<float*>(misc)[1] = sum;
-            if (m > cap) atomicExch(err, 1);
         }
         __syncthreads();
-        const int m = min(misc[0], cap);
+        const int m = misc[0];
         const float sum = reinterpret_cast<float*>(misc)[1];
-        for (int i = tid; i < m; i += kB) pairs[i].key = acc[pairs[i].id];
-        // bitonic sort of (value desc, term asc) over the next power of two
         int np = 1;
         while (np < m) np <<= 1;
+        KeyItem* pairs = np <= cap ? lds_pairs : my_spill;
+        // touched terms -> pairs (positions from a popcount scan over the words)
+        {
+            const int per = (nw + kB - 1) / kB;
+            const int w0 = tid * per;
+            int local = 0;
+            for (int w = w0; w < w0 + per && w < nw; ++w) local += __popc(touch[w]);
+            int incl = local;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int x = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += x;
+            }
+            if (lane == 63) wsum[wave] = incl;
+            __syncthreads();
+            int run = incl - local;
+            for (int w = 0; w < wave; ++w) run += wsum[w];
+            for (int w = w0; w < w0 + per && w < nw; ++w) {
+                uint32_t bits = touch[w];
+                while (bits) {
+                    const int b = __ffs(bits) - 1;
+                    bits &= bits - 1;
+                    const int t = w * 32 + b;
+                    pairs[run++] = KeyItem{acc[t], static_cast<uint32_t>(t)};
+                }
+            }
+        }
         for (int i = m + tid; i < np; i += kB) pairs[i] = KeyItem{-__builtin_inff(), 0xFFFFFFFFu};
         __syncthreads();
         for (int size = 2; size <= np; size <<= 1) {
@@ -693,6 +721,7 @@ __global__ __launch_bounds__(kB) void summarize_f32_kernel(const int32_t* __rest
             }
         }
         if (tid == 0) {
+            // summarize_emit_cluster_ (inverted_list_clusters.cpp:43-50)
             float addup = 0.0F;
             int keep_n = m;
             for (int j = 0; j < m; ++j) {
@@ -999,12 +1028,15 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
 
     // ---- 6. summaries (count pass, scan, write pass)
     const int nw = (L + 31) / 32;
-    DeviceBuffer scount(static_cast<size_t>(C) * 8), out_off(static_cast<size_t>(C + 1) * 8), err(4);
-    hip_check(hipMemset(err.get(), 0, 4), "hipMemset");
-    const unsigned sgrid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(C, 8192)));
+    DeviceBuffer scount(static_cast<size_t>(C) * 8), out_off(static_cast<size_t>(C + 1) * 8);
+    const unsigned sgrid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(C, sq ? 8192 : 1024)));
     const size_t lds8 = static_cast<size_t>(nw) * 8 * 4 + nw * 4 + 256 * 4 + nw * 8 + 8 * 4;
     const int f32_cap = 4096;
-    const size_t ldsf = static_cast<size_t>(L) * 4 + nw * 8 + 8 + f32_cap * 8 + 16;
+    const size_t ldsf = static_cast<size_t>(L) * 4 + nw * 8 + 8 + f32_cap * 8 + 32;
+    int spill_stride = 1;
+    while (spill_stride < L) spill_stride <<= 1;
+    DeviceBuffer spill;
+    if (!sq) spill.allocate(static_cast<size_t>(sgrid) * spill_stride * sizeof(KeyItem));
     if (!sq && ldsf > 160 * 1024)
         throw std::invalid_argument("vocabulary too large for the FP32 device summarizer");
     if (!sq) {
@@ -1041,12 +1073,9 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
                                cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), C, L, f32_cap, alpha,
                                scount.get<int64_t>(), out_off.get<int64_t>(), pass == 1,
                                pass == 1 ? sterm.get<uint16_t>() : nullptr, pass == 1 ? sval.get<float>() : nullptr,
-                               pass == 1 ? scl.get<uint16_t>() : nullptr, err.get<int32_t>());
+                               pass == 1 ? scl.get<uint16_t>() : nullptr, spill.get<KeyItem>(), spill_stride);
         }
         sync("summaries");
-        int32_t e = 0;
-        err.download(&e, 4);
-        if (e != 0) throw std::invalid_argument("a cluster touches more than 4096 terms: beyond the FP32 device summarizer");
         if (pass == 0) continue;
 
         // ---- 7. term-major transpose per list

@@ -19,6 +19,7 @@
 #include "capi_util.h"
 #include "index.h"
 #include "io.h"
+#include "kernels.h"
 #include "nsparse_c.h"
 
 namespace nsparse_amd {
@@ -89,6 +90,11 @@ struct FileBytes {
     }
     const uint8_t* data() const { return map != nullptr ? static_cast<const uint8_t*>(map) : buf.data(); }
 };
+
+template <class... P>
+void throw_if_null_all(P*... ptrs) {
+    if (((ptrs == nullptr) || ...)) throw std::invalid_argument("unexpected nullptr");
+}
 
 }  // namespace
 
@@ -205,6 +211,19 @@ int nsparse_device_error(nsparse_index* index, int32_t* out) {
     return guarded([&] {
         if (out == nullptr) throw std::invalid_argument("unexpected nullptr");
         *out = seismic_of(index).last_device_error();
+    });
+}
+
+int nsparse_exact_scores_device(int64_t n_docs, const int32_t* d_indptr, const uint16_t* d_indices,
+                                const float* d_values, int dim, int q_len, const uint16_t* d_q_indices,
+                                const float* d_q_values, float* d_scores, void* stream) {
+    return guarded([&] {
+        if (n_docs <= 0 || dim <= 0 || q_len < 0) throw std::invalid_argument("value must be positive");
+        throw_if_null_all(d_indptr, d_indices, d_values, d_scores);
+        if (q_len > 0) throw_if_null_all(d_q_indices, d_q_values);
+        require_device();
+        launch_exact_scores(d_indptr, d_indices, d_values, n_docs, dim, d_q_indices, d_q_values, q_len, d_scores,
+                            static_cast<hipStream_t>(stream));
     });
 }
 

@@ -49,5 +49,7 @@ void launch_search_checked(const SearchArgs& a, int max_clusters, int kmax, bool
 void launch_exact_checked(const ExactArgs& a, bool sq, hipStream_t stream);
 int search_lds_bytes(int dim, int max_clusters, int kmax, bool sq);
 std::string device_error_message(int32_t code);
+void launch_exact_scores(const int32_t* indptr, const uint16_t* indices, const float* values, int64_t n_docs, int dim,
+                         const uint16_t* q_indices, const float* q_values, int qn, float* scores, hipStream_t stream);
 
 }  // namespace nsparse_amd

@@ -26,6 +26,7 @@
 
 #include <cstdint>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 
@@ -774,6 +775,72 @@ __global__ __launch_bounds__(kBlock) void exact_match_kernel(ExactArgs a, int nw
         out_d[j] = -1.0F;
         out_l[j] = -1;
     }
+}
+
+// Exact sparse x sparse scores of one query against every document of an FP32
+// CSR (the scoring loop of BrutalIndex::single_query, brutal_index.cpp:75-96):
+// the query sits in LDS as bitmap + rank table, one lane per document adds
+// weight * query_value in row order. Used as recall ground truth.
+__global__ __launch_bounds__(kBlock) void exact_scores_kernel(const int32_t* __restrict__ indptr,
+                                                              const uint16_t* __restrict__ indices,
+                                                              const float* __restrict__ values, int64_t n_docs,
+                                                              int nw, const uint16_t* __restrict__ q_indices,
+                                                              const float* __restrict__ q_values, int qn,
+                                                              float* __restrict__ scores) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* qbits = reinterpret_cast<uint32_t*>(smem);
+    uint16_t* qpref = reinterpret_cast<uint16_t*>(qbits + nw);
+    float* rval = reinterpret_cast<float*>(smem + ((nw * 6 + 15) & ~15));
+    int32_t* rlast = reinterpret_cast<int32_t*>(rval + kQMax);
+    int32_t* wsum = rlast + kQMax;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int w = tid; w < nw; w += kBlock) qbits[w] = 0;
+    for (int r = tid; r < qn; r += kBlock) rlast[r] = -1;
+    __syncthreads();
+    for (int i = tid; i < qn; i += kBlock) atomicOr(&qbits[q_indices[i] >> 5], 1u << (q_indices[i] & 31));
+    __syncthreads();
+    {
+        const int per = (nw + kBlock - 1) / kBlock;
+        const int w0 = tid * per;
+        int local = 0;
+        for (int w = w0; w < w0 + per && w < nw; ++w) local += __popc(qbits[w]);
+        const int incl = wave_incl_scan(local, lane);
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        int run = incl - local;
+        for (int w = 0; w < wave; ++w) run += wsum[w];
+        for (int w = w0; w < w0 + per && w < nw; ++w) {
+            qpref[w] = static_cast<uint16_t>(run);
+            run += __popc(qbits[w]);
+        }
+    }
+    __syncthreads();
+    const QueryLds QL{qbits, qpref, rval};
+    for (int i = tid; i < qn; i += kBlock) {
+        bool p;
+        atomicMax(&rlast[QL.rank_of(q_indices[i], p)], i);
+    }
+    __syncthreads();
+    for (int r = tid; r < qn; r += kBlock)
+        if (rlast[r] >= 0) rval[r] = q_values[rlast[r]];
+    __syncthreads();
+    for (int64_t d = blockIdx.x * static_cast<int64_t>(kBlock) + tid; d < n_docs;
+         d += static_cast<int64_t>(gridDim.x) * kBlock) {
+        float acc = 0.0F;
+        for (int32_t p = indptr[d]; p < indptr[d + 1]; ++p) acc += values[p] * QL.value(indices[p]);
+        scores[d] = acc;
+    }
+}
+
+void launch_exact_scores(const int32_t* indptr, const uint16_t* indices, const float* values, int64_t n_docs, int dim,
+                         const uint16_t* q_indices, const float* q_values, int qn, float* scores, hipStream_t stream) {
+    if (qn > kQMax) throw std::invalid_argument("query has more non-zeros than the device path supports (512)");
+    const int nw = (dim + 31) / 32;
+    const int lds = ((nw * 6 + 15) & ~15) + kQMax * 8 + kWaves * 4;
+    const int64_t blocks = std::min<int64_t>((n_docs + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(exact_scores_kernel, dim3(static_cast<unsigned>(std::max<int64_t>(blocks, 1))), dim3(kBlock),
+                       lds, stream, indptr, indices, values, n_docs, nw, q_indices, q_values, qn, scores);
+    hip_check(hipGetLastError(), "exact scores launch");
 }
 
 int search_lds_bytes(int dim, int max_clusters, int kmax, bool sq) { return make_plan(dim, max_clusters, kmax, sq).total; }

@@ -85,6 +85,8 @@ _sig("nsparse_search_device", [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, 
 _sig("nsparse_search_device_stats", [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                      P(SearchParams), c_void_p, c_void_p, c_void_p, c_void_p])
 _sig("nsparse_device_error", [c_void_p, P(ctypes.c_int32)])
+_sig("nsparse_exact_scores_device", [c_int64, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                     c_void_p, c_void_p])
 _sig("nsparse_write_index", [c_void_p, c_char_p])
 _sig("nsparse_write_index_buffer", [c_void_p, P(P(ctypes.c_uint8)), P(c_size_t)])
 _sig("nsparse_buffer_free", [c_void_p], None)

@@ -25,6 +25,7 @@
 #include <cstring>
 #include <functional>
 #include <limits>
+#include <memory>
 #include <numeric>
 #include <queue>
 #include <random>
@@ -850,6 +851,80 @@ int oracle_search(void* h, int n, const idx_t* indptr, const term_t* indices, co
             search_impl<HeapTopK<idx_t>>(*ix, n, indptr, indices, values, k, cut, hf, qq, sp, D, L, st, n_threads);
         else
             search_impl<IdTopK<idx_t>>(*ix, n, indptr, indices, values, k, cut, hf, qq, sp, D, L, st, n_threads);
+    });
+}
+
+// Reads a write_index byte stream (SEIS / SESQ) into an oracle index, the
+// inverse of write_index above (sparse_vectors.cpp:233-253,
+// inverted_list_clusters.cpp:365-389). The stream is trusted: it comes from the
+// library under test or from this file's writer.
+int oracle_load(const uint8_t* buf, size_t size, int tie_mode, void** out) {
+    return guard([&] {
+        size_t pos = 0;
+        auto rd = [&](void* dst, size_t n) {
+            if (n > size - pos) throw std::runtime_error("truncated index stream");
+            std::memcpy(dst, buf + pos, n);
+            pos += n;
+        };
+        auto scalar = [&](auto& v) { rd(&v, sizeof(v)); };
+        auto skip = [&](size_t a) { pos += (a - pos % a) % a; };
+        auto arr = [&](auto& vec, size_t n, size_t a) {
+            skip(a);
+            vec.resize(n);
+            rd(vec.data(), n * sizeof(vec[0]));
+        };
+        uint32_t fcc = 0;
+        int dim = 0;
+        scalar(fcc);
+        scalar(dim);
+        auto* ix = new Index;
+        std::unique_ptr<Index> hold(ix);
+        ix->tie_mode = tie_mode;
+        ix->sq = fcc == (uint32_t('S') | uint32_t('E') << 8 | uint32_t('S') << 16 | uint32_t('Q') << 24);
+        if (ix->sq) {
+            uint8_t qt = 0;
+            float lo = 0, hi = 1;
+            scalar(qt);
+            scalar(lo);
+            scalar(hi);
+            ix->q = Quantizer(lo, hi);
+        }
+        ix->fwd.esz = ix->sq ? 1 : 4;
+        ix->fwd.dim = static_cast<size_t>(dim);
+        size_t count = 0;
+        scalar(count);
+        if (count > 0) {
+            size_t fdim = 0, esz = 0;
+            scalar(fdim);
+            scalar(esz);
+            arr(ix->fwd.indptr, count + 1, 4);
+            const size_t nnz = static_cast<size_t>(ix->fwd.indptr[count]);
+            arr(ix->fwd.indices, nnz, 2);
+            arr(ix->fwd.values, nnz * esz, esz);
+        }
+        ix->has_vectors = count > 0;
+        size_t n_lists = 0;
+        scalar(n_lists);
+        ix->lists.resize(n_lists);
+        for (auto& l : ix->lists) {
+            size_t n = 0;
+            scalar(n);
+            arr(l.docs, n, 4);
+            scalar(n);
+            arr(l.offsets, n, 4);
+            scalar(l.n_clusters);
+            scalar(l.esz);
+            size_t nt = 0;
+            scalar(nt);
+            if (nt > 0) {
+                arr(l.term_ids, nt, 2);
+                arr(l.term_ptr, nt + 1, 4);
+            }
+            scalar(n);
+            arr(l.csc_cluster, n, 2);
+            arr(l.csc_value, n * l.esz, l.esz);
+        }
+        *out = hold.release();
     });
 }
 

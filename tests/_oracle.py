@@ -69,6 +69,16 @@ class OracleIndex:
     def build(self, threads=8):
         _chk(lib.oracle_build(self.h, threads))
 
+    @classmethod
+    def load(cls, blob, tie_mode=0):
+        """An oracle index read from write_index bytes (the library's or the oracle's)."""
+        self = cls.__new__(cls)
+        self.h = P()
+        buf = ctypes.create_string_buffer(bytes(blob), len(blob))
+        _chk(lib.oracle_load(buf, ctypes.c_size_t(len(blob)), tie_mode, ctypes.byref(self.h)))
+        self.sq = bytes(blob[:4]) == b"SESQ"
+        return self
+
     def copy_from(self, other):
         _chk(lib.oracle_copy_index(self.h, other.h))
 
