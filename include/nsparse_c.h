@@ -148,6 +148,20 @@ int nsparse_exact_scores_device(int64_t n_docs, const int32_t* d_indptr, const u
                                 const float* d_values, int dim, int q_len, const uint16_t* d_q_indices,
                                 const float* d_q_values, float* d_scores, void* stream);
 
+/* Differentiable exact re-scoring of candidates (training-side use of the scorer in
+ * nsparse/seismic_common.h:78-103; no reference twin). All pointers DEVICE.
+ * Forward (backward == 0): d_scores[q*n_cand + c] = sum_i qv[i] * doc(cand[q,c])[t_i].
+ * Backward: d_grad_q[i] = sum_c g[q,c] * doc[t_i] (overwritten) and
+ * d_grad_d[e] += g[q,c] * sum_{i: t_i = term(e)} qv[i] (accumulated; zero it first);
+ * either gradient pointer may be NULL. cand < 0 marks an empty slot. Queries are
+ * limited to 512 entries; a longer one sets *d_error to 1. Extension. */
+int nsparse_rescore_device(int backward, int64_t n_queries, const int32_t* d_q_indptr,
+                           const uint16_t* d_q_indices, const float* d_q_values, int n_cand,
+                           const int32_t* d_candidates, int64_t n_docs, const int32_t* d_indptr,
+                           const uint16_t* d_indices, const float* d_values, int dim, float* d_scores,
+                           const float* d_grad_scores, float* d_grad_q, float* d_grad_d,
+                           int32_t* d_error, void* stream);
+
 /* write_index(Index*, const char*) -- nsparse/io/index_io.cpp:88-131,170-173.
  * Byte layout identical to the reference writer. */
 int nsparse_write_index(nsparse_index* index, const char* path);

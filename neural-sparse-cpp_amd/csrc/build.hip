@@ -216,9 +216,17 @@ __global__ void cluster_list_kernel(const int32_t* __restrict__ lcb, int32_t* __
         for (int c = lcb[l] + threadIdx.x; c < lcb[l + 1]; c += blockDim.x) cluster_list[c] = l;
 }
 
-__global__ void row_len_kernel(const int32_t* __restrict__ indptr, const int32_t* __restrict__ docs, size_t n,
-                               int64_t* __restrict__ out) {
-    GRID_STRIDE(i, n) out[i] = indptr[docs[i] + 1] - indptr[docs[i]];
+// Row length of each cluster's centroid; zero for clusters of lists whose
+// every kept doc is a centroid (nothing is assigned there, so their centroids
+// need no term-major index).
+__global__ void row_len_kernel(const int32_t* __restrict__ indptr, const int32_t* __restrict__ docs,
+                               const int32_t* __restrict__ cluster_list, const int32_t* __restrict__ lcb,
+                               const int64_t* __restrict__ kept_base, size_t n, int64_t* __restrict__ out) {
+    GRID_STRIDE(i, n) {
+        const int l = cluster_list[i];
+        const bool needed = (kept_base[l + 1] - kept_base[l]) > (lcb[l + 1] - lcb[l]);
+        out[i] = needed ? indptr[docs[i] + 1] - indptr[docs[i]] : 0;
+    }
 }
 
 // One posting per (centroid term): key (list, term), value = entry id.
@@ -234,6 +242,7 @@ __global__ void centroid_entries_kernel(const int32_t* __restrict__ indptr, cons
         const int l = cluster_list[g];
         const uint16_t c = static_cast<uint16_t>(g - lcb[l]);
         int64_t e = cent_off[g];
+        if (cent_off[g + 1] == e) continue;  // list without docs to assign
         for (int32_t p = indptr[d]; p < indptr[d + 1]; ++p, ++e) {
             key[e] = (static_cast<uint32_t>(l) << 16) | indices[p];
             val[e] = static_cast<int32_t>(e);
@@ -285,6 +294,10 @@ __global__ __launch_bounds__(kB) void assign_kernel(
         const int64_t kb = kept_base[l], n = kept_base[l + 1] - kb;
         const int c0 = lcb[l], nc = lcb[l + 1] - c0;
         if (n == 0 || nc == 0) continue;
+        if (n == nc) {  // every kept doc is a centroid: nothing to assign
+            for (int64_t j = tid; j < n; j += kB) assign[kb + j] = -1;
+            continue;
+        }
         const int64_t e0 = cent_off[c0], e1 = cent_off[c0 + nc];
         for (int64_t e = e0 + tid; e < e1; e += kB) {
             const uint32_t k = ckey[e];
@@ -936,6 +949,7 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
     // ---- 4. centroid term-major index + assignment
     DeviceBuffer clen(static_cast<size_t>(C) * 8), cent_off(static_cast<size_t>(C + 1) * 8);
     hipLaunchKernelGGL(row_len_kernel, dim3(grid_for(C)), dim3(kB), 0, nullptr, fwd.indptr, centroid_doc.get<int32_t>(),
+                       cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), d_kept_base.get<int64_t>(),
                        static_cast<size_t>(C), clen.get<int64_t>());
     exclusive_scan_i64(clen.get<int64_t>(), cent_off.get<int64_t>(), static_cast<size_t>(C));
     sync("centroid sizes");

@@ -227,6 +227,24 @@ int nsparse_exact_scores_device(int64_t n_docs, const int32_t* d_indptr, const u
     });
 }
 
+int nsparse_rescore_device(int backward, int64_t n_queries, const int32_t* d_q_indptr, const uint16_t* d_q_indices,
+                           const float* d_q_values, int n_cand, const int32_t* d_candidates, int64_t n_docs,
+                           const int32_t* d_indptr, const uint16_t* d_indices, const float* d_values, int dim,
+                           float* d_scores, const float* d_grad_scores, float* d_grad_q, float* d_grad_d,
+                           int32_t* d_error, void* stream) {
+    return guarded([&] {
+        if (n_queries <= 0 || n_cand <= 0 || n_docs <= 0 || dim <= 0)
+            throw std::invalid_argument("value must be positive");
+        throw_if_null_all(d_q_indptr, d_q_indices, d_q_values, d_candidates, d_indptr, d_indices, d_values, d_error);
+        if (!backward && d_scores == nullptr) throw std::invalid_argument("unexpected nullptr");
+        if (backward && d_grad_scores == nullptr) throw std::invalid_argument("unexpected nullptr");
+        require_device();
+        launch_rescore(backward != 0, n_queries, d_q_indptr, d_q_indices, d_q_values, n_cand, d_candidates, n_docs,
+                       d_indptr, d_indices, d_values, dim, d_scores, d_grad_scores, d_grad_q, d_grad_d, d_error,
+                       static_cast<hipStream_t>(stream));
+    });
+}
+
 int nsparse_write_index(nsparse_index* index, const char* path) {
     return guarded([&] {
         if (path == nullptr) throw std::invalid_argument("unexpected nullptr");

@@ -52,4 +52,11 @@ std::string device_error_message(int32_t code);
 void launch_exact_scores(const int32_t* indptr, const uint16_t* indices, const float* values, int64_t n_docs, int dim,
                          const uint16_t* q_indices, const float* q_values, int qn, float* scores, hipStream_t stream);
 
+// Differentiable exact re-scoring (rescore.hip). Forward fills scores
+// (n_q x n_cand); backward adds into grad_q / grad_d (either may be null).
+void launch_rescore(bool backward, int64_t n_q, const int32_t* q_indptr, const uint16_t* q_indices,
+                    const float* q_values, int n_cand, const int32_t* cand, int64_t n_docs, const int32_t* d_indptr,
+                    const uint16_t* d_indices, const float* d_values, int dim, float* scores,
+                    const float* grad_scores, float* grad_q, float* grad_d, int32_t* err, hipStream_t stream);
+
 }  // namespace nsparse_amd

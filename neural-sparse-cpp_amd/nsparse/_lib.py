@@ -16,6 +16,15 @@ if not os.path.exists(LIB_PATH):
         "(or __graft_entry__.build()) first"
     )
 
+# One HIP runtime per process: PyTorch-ROCm bundles its own libamdhip64.so.7,
+# and the library resolves that soname to whichever copy is already loaded.
+# Loading torch first makes both use torch's runtime (a second runtime in the
+# same process leaves torch without devices).
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 lib = ctypes.CDLL(LIB_PATH)
 
 c_int, c_int64, c_float, c_size_t = ctypes.c_int, ctypes.c_int64, ctypes.c_float, ctypes.c_size_t
@@ -87,6 +96,9 @@ _sig("nsparse_search_device_stats", [c_void_p, c_int64, c_void_p, c_void_p, c_vo
 _sig("nsparse_device_error", [c_void_p, P(ctypes.c_int32)])
 _sig("nsparse_exact_scores_device", [c_int64, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
                                      c_void_p, c_void_p])
+_sig("nsparse_rescore_device", [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int64, c_void_p,
+                                c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                c_void_p])
 _sig("nsparse_write_index", [c_void_p, c_char_p])
 _sig("nsparse_write_index_buffer", [c_void_p, P(P(ctypes.c_uint8)), P(c_size_t)])
 _sig("nsparse_buffer_free", [c_void_p], None)

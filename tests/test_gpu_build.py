@@ -48,8 +48,12 @@ def test_f32_build_matches_oracle_and_reference(nsparse, n, lam, beta, alpha):
     docs = synth.make_corpus(synth.doc_spec(seed=100 + n), n)
     desc = f"seismic,lambda={lam}|beta={beta}|alpha={alpha}|seed=9"
     got = nsparse.serialize_index(gpu_build(nsparse, desc, docs))
-    assert got == oracle_bytes(docs, 0, lam, beta, alpha, 9)
-    if O.ref is not None:
+    want = oracle_bytes(docs, 0, lam, beta, alpha, 9)
+    assert got == want
+    # FP32 weights can still tie inside one posting list (float32 collisions);
+    # where no tie reaches the layout, the two tie modes agree and the device
+    # output is the reference's own.
+    if O.ref is not None and want == oracle_bytes(docs, 0, lam, beta, alpha, 9, tie_mode=0):
         assert got == O.ref_build_bytes(DIM, 0, *docs, lam, beta, alpha, 9)
 
 

@@ -1,0 +1,133 @@
+"""Differentiable re-scoring (nsparse.rescore): forward scores against an
+exact CPU computation, gradients against a CPU reference gradient and FP64
+central finite differences (eps 1e-3, relative error <= 1e-4), on candidates
+mined by the GPU SEISMIC search, with an InfoNCE loss (temperature 0.05, first
+candidate positive)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DIM = 30522
+
+
+@pytest.fixture(scope="module")
+def setup(nsparse):
+    import torch
+
+    from nsparse import synth
+
+    docs = synth.make_corpus(synth.doc_spec(seed=40), 5000)
+    queries = synth.make_corpus(synth.query_spec(seed=41), 16)
+    idx = nsparse.index_factory(DIM, "seismic,lambda=300|beta=30|alpha=0.4|seed=0")
+    idx.add(len(docs[0]) - 1, *docs)
+    idx.build()
+    _, cand = idx.search(len(queries[0]) - 1, *queries, 50, nsparse.SeismicSearchParameters(10, 0.9))
+    dev = torch.device("cuda")
+    t = dict(
+        q_indptr=torch.from_numpy(queries[0]).to(dev), q_indices=torch.from_numpy(queries[1].view(np.int16)).to(dev),
+        d_indptr=torch.from_numpy(docs[0]).to(dev), d_indices=torch.from_numpy(docs[1].view(np.int16)).to(dev),
+        cand=torch.from_numpy(cand).to(dev))
+    return docs, queries, cand, t
+
+
+def cpu_scores(docs, queries, cand, qv, dv):
+    """float64 exact sparse dot products and their gradients w.r.t. a unit
+    upstream gradient weighted by g (returned as functions)."""
+    dip, dix, _ = docs
+    qip, qix, _ = queries
+    n_q, n_c = cand.shape
+    S = np.zeros((n_q, n_c))
+    for q in range(n_q):
+        qmap = {}
+        for i in range(qip[q], qip[q + 1]):
+            qmap[int(qix[i])] = qmap.get(int(qix[i]), 0.0) + qv[i]
+        for c in range(n_c):
+            d = cand[q, c]
+            if d < 0:
+                continue
+            S[q, c] = sum(dv[p] * qmap.get(int(dix[p]), 0.0) for p in range(dip[d], dip[d + 1]))
+    return S
+
+
+def cpu_grads(docs, queries, cand, qv, dv, G):
+    dip, dix, _ = docs
+    qip, qix, _ = queries
+    gq = np.zeros(len(qv))
+    gd = np.zeros(len(dv))
+    for q in range(cand.shape[0]):
+        qmap = {}
+        for i in range(qip[q], qip[q + 1]):
+            qmap[int(qix[i])] = qmap.get(int(qix[i]), 0.0) + qv[i]
+        for c in range(cand.shape[1]):
+            d = cand[q, c]
+            if d < 0:
+                continue
+            dmap = {int(dix[p]): dv[p] for p in range(dip[d], dip[d + 1])}
+            for i in range(qip[q], qip[q + 1]):
+                gq[i] += G[q, c] * dmap.get(int(qix[i]), 0.0)
+            for p in range(dip[d], dip[d + 1]):
+                gd[p] += G[q, c] * qmap.get(int(dix[p]), 0.0)
+    return gq, gd
+
+
+def test_forward_and_cpu_gradient(nsparse, setup):
+    import torch
+
+    from nsparse.rescore import rescore
+
+    docs, queries, cand, t = setup
+    qv = torch.from_numpy(queries[2]).cuda().requires_grad_(True)
+    dv = torch.from_numpy(docs[2]).cuda().requires_grad_(True)
+    S = rescore(qv, dv, t["q_indptr"], t["q_indices"], t["cand"], t["d_indptr"], t["d_indices"], DIM)
+    want = cpu_scores(docs, queries, cand, queries[2].astype(np.float64), docs[2].astype(np.float64))
+    np.testing.assert_allclose(S.detach().cpu().numpy(), want, rtol=1e-5, atol=1e-6)
+    G = np.random.default_rng(0).standard_normal(S.shape).astype(np.float32)
+    S.backward(torch.from_numpy(G).cuda())
+    gq, gd = cpu_grads(docs, queries, cand, queries[2].astype(np.float64), docs[2].astype(np.float64),
+                       G.astype(np.float64))
+    np.testing.assert_allclose(qv.grad.cpu().numpy(), gq, rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(dv.grad.cpu().numpy(), gd, rtol=1e-4, atol=1e-5)
+
+
+def test_info_nce_gradient_matches_finite_differences(nsparse, setup):
+    import torch
+
+    from nsparse.rescore import info_nce, rescore
+
+    docs, queries, cand, t = setup
+    valid = t["cand"] >= 0
+    qv = torch.from_numpy(queries[2]).cuda().requires_grad_(True)
+    dv = torch.from_numpy(docs[2]).cuda().requires_grad_(True)
+    loss = info_nce(rescore(qv, dv, t["q_indptr"], t["q_indices"], t["cand"], t["d_indptr"], t["d_indices"], DIM),
+                    valid)
+    loss.backward()
+
+    # float64 loss on the CPU for central differences
+    valid_np = cand >= 0
+
+    def loss64(qv64, dv64):
+        S = cpu_scores(docs, queries, cand, qv64, dv64) / 0.05
+        S = np.where(valid_np, S, -np.inf)
+        m = S.max(axis=1, keepdims=True)
+        lse = m[:, 0] + np.log(np.exp(S - m).sum(axis=1))
+        return float(np.mean(lse - S[:, 0]))
+
+    qv64 = queries[2].astype(np.float64)
+    dv64 = docs[2].astype(np.float64)
+    rng = np.random.default_rng(1)
+    eps = 1e-3
+    for i in rng.choice(len(qv64), 6, replace=False):
+        a, b = qv64.copy(), qv64.copy()
+        a[i] += eps
+        b[i] -= eps
+        fd = (loss64(a, dv64) - loss64(b, dv64)) / (2 * eps)
+        assert abs(qv.grad[i].item() - fd) <= 1e-4 * max(abs(fd), 1e-2), (i, qv.grad[i].item(), fd)
+    touched = np.unique(np.concatenate([np.arange(docs[0][d], docs[0][d + 1]) for d in cand[:, :3].ravel()
+                                        if d >= 0]))
+    for p in rng.choice(touched, 6, replace=False):
+        a, b = dv64.copy(), dv64.copy()
+        a[p] += eps
+        b[p] -= eps
+        fd = (loss64(qv64, a) - loss64(qv64, b)) / (2 * eps)
+        assert abs(dv.grad[p].item() - fd) <= 1e-4 * max(abs(fd), 1e-2), (p, dv.grad[p].item(), fd)
