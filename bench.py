@@ -67,6 +67,7 @@ def main():
     ap.add_argument("--recall-queries", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (query-sharded replicas are a "
@@ -127,6 +128,13 @@ def main():
     kernel_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
     qps = args.queries * args.steps / wall
     log(f"batched: {qps:,.0f} QPS, {wall / args.steps * 1e3:.2f} ms/step, kernel {kernel_ms:.2f} ms")
+    if args.copy_probe:
+        # a plain copy of known size, for scaling rocprofv3 FETCH/WRITE counters
+        src = torch.empty(256 << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     labels = out_l.cpu().numpy()
     dists = out_d.cpu().numpy()
 
@@ -136,6 +144,9 @@ def main():
     st = stats.cpu().numpy()
     bytes_launch = float(algorithmic_bytes(st).sum())
     achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
+    log(f"algorithmic {bytes_launch / 1e9:.3f} GB/launch -> {achieved:.0f} GB/s; per-query means "
+        + ", ".join(f"{n}={v:.0f}" for n, v in zip(["lists", "found", "csc", "tested", "opened", "read", "scored",
+                                                      "nnz"], st.mean(axis=0))))
 
     # ---- plain path: one query per launch, synchronous
     n_plain = min(args.plain_queries, args.queries)
@@ -190,9 +201,9 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import _oracle
 
-        blob = nsparse.serialize_index(index)
-        ref = _oracle.OracleIndex.load(blob, tie_mode=0)
-        del blob
+        t1 = time.time()
+        ref = _oracle.OracleIndex.load_from_index(nsparse, index, tie_mode=0)
+        log(f"oracle copy of the index loaded in {time.time() - t1:.1f}s")
         done, t_cpu, agree, total = 0, 0.0, 0, 0
         chunk = 64
         while t_cpu < args.cpu_seconds and done < args.queries:

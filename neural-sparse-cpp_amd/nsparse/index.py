@@ -251,7 +251,10 @@ def serialize_index(index):
     size = ctypes.c_size_t()
     check(lib.nsparse_write_index_buffer(index.handle, ctypes.byref(buf), ctypes.byref(size)))
     try:
-        return ctypes.string_at(buf, size.value)
+        out = bytearray(size.value)
+        if size.value:
+            ctypes.memmove((ctypes.c_char * size.value).from_buffer(out), buf, size.value)
+        return bytes(out)
     finally:
         lib.nsparse_buffer_free(buf)
 

@@ -79,6 +79,24 @@ class OracleIndex:
         self.sq = bytes(blob[:4]) == b"SESQ"
         return self
 
+    @classmethod
+    def load_from_index(cls, nsparse, index, tie_mode=0):
+        """Oracle copy of a library index: its write_index stream is handed over
+        as a raw buffer (no Python bytes object, so no 2 GiB limit)."""
+        import ctypes as ct
+
+        buf = ct.POINTER(ct.c_uint8)()
+        size = ct.c_size_t()
+        nsparse._lib.check(nsparse._lib.lib.nsparse_write_index_buffer(index.handle, ct.byref(buf), ct.byref(size)))
+        try:
+            self = cls.__new__(cls)
+            self.h = P()
+            _chk(lib.oracle_load(ct.cast(buf, P), size, tie_mode, ct.byref(self.h)))
+            self.sq = ct.string_at(buf, 4) == b"SESQ"
+            return self
+        finally:
+            nsparse._lib.lib.nsparse_buffer_free(buf)
+
     def copy_from(self, other):
         _chk(lib.oracle_copy_index(self.h, other.h))
 
