@@ -138,12 +138,16 @@ def main():
     labels = out_l.cpu().numpy()
     dists = out_d.cpu().numpy()
 
-    stats = torch.zeros((args.queries, 8), dtype=torch.int64, device=dev)
+    stats = torch.zeros((args.queries, 16), dtype=torch.int64, device=dev)
     index.search_device(qip, qix, qv, k_max, ks=d_ks, params=params, stats=stats, stream=stream)
     torch.cuda.synchronize()
     st = stats.cpu().numpy()
     bytes_launch = float(algorithmic_bytes(st).sum())
     achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
+    tick_names = ["setup", "summary", "order", "plan", "gather", "score", "replay"]
+    tick_means = st[:, 8:15].mean(axis=0) * 10.0 / 1e3  # 100 MHz device wall clock -> us
+    log("per-query phase time (us, one workgroup): " + ", ".join(f"{n}={v:.0f}" for n, v in zip(tick_names,
+                                                                                               tick_means)))
     log(f"algorithmic {bytes_launch / 1e9:.3f} GB/launch -> {achieved:.0f} GB/s; per-query means "
         + ", ".join(f"{n}={v:.0f}" for n, v in zip(["lists", "found", "csc", "tested", "opened", "read", "scored",
                                                       "nnz"], st.mean(axis=0))))
@@ -246,7 +250,8 @@ def main():
         "recall_at_10": recall, "recall_queries": n_rec,
         "build_s": build_s,
         "traversal_means": dict(zip(["lists", "terms_found", "csc_entries", "clusters_tested", "clusters_opened",
-                                     "docs_read", "docs_scored", "nnz_scored"], st.mean(axis=0).tolist())),
+                                     "docs_read", "docs_scored", "nnz_scored"], st[:, :8].mean(axis=0).tolist())),
+        "phase_us_per_query": dict(zip(tick_names, tick_means.tolist())),
     }
     print(json.dumps(result), flush=True)
 

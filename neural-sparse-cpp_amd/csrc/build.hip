@@ -23,6 +23,7 @@
 //     term-major CSC of every list -- InvertedListClusters::build_transpose.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -229,20 +230,23 @@ __global__ void row_len_kernel(const int32_t* __restrict__ indptr, const int32_t
     }
 }
 
-// One posting per (centroid term): key (list, term), value = entry id.
+// One posting per (centroid term): key (list, term), value = entry id. Covers
+// clusters [g0, g0 + n_clusters); entry ids are relative to e_base, the first
+// entry of cluster g0, so one batch of lists fits 32-bit sort values.
 template <class T>
 __global__ void centroid_entries_kernel(const int32_t* __restrict__ indptr, const uint16_t* __restrict__ indices,
                                         const T* __restrict__ values, const int32_t* __restrict__ centroid_doc,
                                         const int32_t* __restrict__ cluster_list, const int32_t* __restrict__ lcb,
-                                        const int64_t* __restrict__ cent_off, size_t n_clusters,
-                                        uint32_t* __restrict__ key, int32_t* __restrict__ val,
+                                        const int64_t* __restrict__ cent_off, size_t g0, size_t n_clusters,
+                                        int64_t e_base, uint32_t* __restrict__ key, int32_t* __restrict__ val,
                                         uint16_t* __restrict__ ent_c, T* __restrict__ ent_w) {
-    GRID_STRIDE(g, n_clusters) {
+    GRID_STRIDE(i, n_clusters) {
+        const size_t g = g0 + i;
         const int32_t d = centroid_doc[g];
         const int l = cluster_list[g];
         const uint16_t c = static_cast<uint16_t>(g - lcb[l]);
-        int64_t e = cent_off[g];
-        if (cent_off[g + 1] == e) continue;  // list without docs to assign
+        int64_t e = cent_off[g] - e_base;
+        if (cent_off[g + 1] - e_base == e) continue;  // list without docs to assign
         for (int32_t p = indptr[d]; p < indptr[d + 1]; ++p, ++e) {
             key[e] = (static_cast<uint32_t>(l) << 16) | indices[p];
             val[e] = static_cast<int32_t>(e);
@@ -275,22 +279,24 @@ __device__ __forceinline__ void argmax_merge(Acc& v, int& i, Acc ov, int oi) {
 }
 
 // map_docs_to_clusters_typed (cluster/kmeans_utils.cpp:107-158). One
-// workgroup per list (grid-stride), one wavefront per document. `tmap` is this
-// workgroup's dense term -> [begin, end) table of the list's centroid CSC.
+// workgroup per list of [l_begin, l_end) (grid-stride), one wavefront per
+// document. `ckey`/`cc`/`cw` hold the batch's centroid CSC, whose entry ids are
+// global ids minus e_base. `tmap` is this workgroup's dense term -> [begin, end)
+// table of the list's centroid CSC (batch-relative entry ids).
 template <class T, class Acc>
 __global__ __launch_bounds__(kB) void assign_kernel(
     const int32_t* __restrict__ indptr, const uint16_t* __restrict__ indices, const T* __restrict__ values,
     const int64_t* __restrict__ kept_base, const int32_t* __restrict__ kept, const int32_t* __restrict__ lcb,
-    const int32_t* __restrict__ centroid_doc, const int64_t* __restrict__ cent_off, const uint32_t* __restrict__ ckey,
-    const uint16_t* __restrict__ cc, const T* __restrict__ cw, int2* __restrict__ tmap_all, int dim,
-    int32_t* __restrict__ assign, int n_lists, int mc) {
+    const int32_t* __restrict__ centroid_doc, const int64_t* __restrict__ cent_off, int64_t e_base,
+    const uint32_t* __restrict__ ckey, const uint16_t* __restrict__ cc, const T* __restrict__ cw,
+    int2* __restrict__ tmap_all, int dim, int32_t* __restrict__ assign, int l_begin, int l_end, int mc) {
     extern __shared__ __align__(16) unsigned char smem[];
     int32_t* cent = reinterpret_cast<int32_t*>(smem);                    // sorted centroid ids [mc]
     Acc* sims_all = reinterpret_cast<Acc*>(smem + ((mc * 4 + 15) & ~15));  // [4][mc]
     int2* tmap = tmap_all + static_cast<size_t>(blockIdx.x) * dim;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     Acc* sims = sims_all + wave * mc;
-    for (int l = blockIdx.x; l < n_lists; l += gridDim.x) {
+    for (int l = l_begin + blockIdx.x; l < l_end; l += gridDim.x) {
         const int64_t kb = kept_base[l], n = kept_base[l + 1] - kb;
         const int c0 = lcb[l], nc = lcb[l + 1] - c0;
         if (n == 0 || nc == 0) continue;
@@ -298,7 +304,7 @@ __global__ __launch_bounds__(kB) void assign_kernel(
             for (int64_t j = tid; j < n; j += kB) assign[kb + j] = -1;
             continue;
         }
-        const int64_t e0 = cent_off[c0], e1 = cent_off[c0 + nc];
+        const int64_t e0 = cent_off[c0] - e_base, e1 = cent_off[c0 + nc] - e_base;
         for (int64_t e = e0 + tid; e < e1; e += kB) {
             const uint32_t k = ckey[e];
             const int t = static_cast<int>(k & 0xFFFFu);
@@ -777,14 +783,17 @@ __global__ __launch_bounds__(kB) void summarize_f32_kernel(const int32_t* __rest
     }
 }
 
+// Transpose keys (list, term) for clusters [g0, g0 + n_clusters); entry ids
+// relative to e_base, the first summary entry of cluster g0.
 __global__ void summary_key_kernel(const uint16_t* __restrict__ term, const int64_t* __restrict__ out_off,
-                                   const int32_t* __restrict__ cluster_list, int64_t n_clusters,
-                                   uint32_t* __restrict__ key, int32_t* __restrict__ val) {
-    GRID_STRIDE(g, static_cast<size_t>(n_clusters)) {
+                                   const int32_t* __restrict__ cluster_list, size_t g0, size_t n_clusters,
+                                   int64_t e_base, uint32_t* __restrict__ key, int32_t* __restrict__ val) {
+    GRID_STRIDE(i, n_clusters) {
+        const size_t g = g0 + i;
         const uint32_t l = static_cast<uint32_t>(cluster_list[g]);
         for (int64_t e = out_off[g]; e < out_off[g + 1]; ++e) {
-            key[e] = (l << 16) | term[e];
-            val[e] = static_cast<int32_t>(e);
+            key[e - e_base] = (l << 16) | term[e];
+            val[e - e_base] = static_cast<int32_t>(e - e_base);
         }
     }
 }
@@ -803,25 +812,62 @@ __global__ void head_flag_kernel(const uint32_t* __restrict__ key, size_t n, int
     GRID_STRIDE(i, n) flag[i] = (i == 0 || key[i] != key[i - 1]) ? 1 : 0;
 }
 
+// One batch of the transpose: batch entry i is global entry e_base + i; its
+// head ordinal within the batch is hscan[i].
 __global__ void head_write_kernel(const uint32_t* __restrict__ key, const int64_t* __restrict__ hscan, size_t n,
-                                  uint16_t* __restrict__ term_ids, int64_t* __restrict__ term_ptr) {
+                                  int64_t e_base, uint16_t* __restrict__ term_ids, int64_t* __restrict__ term_ptr) {
     GRID_STRIDE(i, n) {
         if (i == 0 || key[i] != key[i - 1]) {
             const int64_t h = hscan[i];
             term_ids[h] = static_cast<uint16_t>(key[i] & 0xFFFFu);
-            term_ptr[h] = static_cast<int64_t>(i);
+            term_ptr[h] = e_base + static_cast<int64_t>(i);
         }
     }
 }
 
+// Per-list entry and term bases for lists [l0, l1), whose summary entries are
+// the batch [e_base, e_base + n) with batch head scan `hscan` (n + 1 values);
+// h_base is the number of term heads in earlier batches.
 __global__ void list_bases_kernel(const int32_t* __restrict__ lcb, const int64_t* __restrict__ out_off,
-                                  const int64_t* __restrict__ hscan, int64_t E, int64_t T, int n_lists,
-                                  int64_t* __restrict__ leb, int32_t* __restrict__ ltb) {
-    GRID_STRIDE(l, static_cast<size_t>(n_lists) + 1) {
+                                  const int64_t* __restrict__ hscan, int64_t e_base, int64_t n, int64_t h_base,
+                                  int l0, int l1, int64_t* __restrict__ leb, int32_t* __restrict__ ltb) {
+    GRID_STRIDE(i, static_cast<size_t>(l1 - l0)) {
+        const int l = l0 + static_cast<int>(i);
         const int64_t e = out_off[lcb[l]];
         leb[l] = e;
-        ltb[l] = static_cast<int32_t>(e < E ? hscan[e] : T);
+        ltb[l] = static_cast<int32_t>(h_base + hscan[min(e - e_base, n)]);
     }
+}
+
+// Splits lists [0, L) into consecutive ranges [b[i], b[i + 1]) whose entry
+// counts (start[l1] - start[l0], `start` holding L + 1 prefix offsets) stay
+// at or below `cap`; a single list larger than `cap` forms a range alone.
+std::vector<int> list_batches(const std::vector<int64_t>& start, int64_t cap) {
+    const int L = static_cast<int>(start.size()) - 1;
+    std::vector<int> b{0};
+    for (int l = 0; l < L; ++l)
+        if (l > b.back() && start[l + 1] - start[b.back()] > cap) b.push_back(l);
+    b.push_back(L);
+    return b;
+}
+
+int64_t max_batch_entries(const std::vector<int64_t>& start, const std::vector<int>& b) {
+    int64_t m = 0;
+    for (size_t i = 0; i + 1 < b.size(); ++i) m = std::max(m, start[b[i + 1]] - start[b[i]]);
+    return m;
+}
+
+// Entries per batch of the two list-keyed sorts of the build (centroid index
+// and summary transpose): 32-bit sort values and a bounded scratch footprint.
+// NSPARSE_BUILD_BATCH_ENTRIES lowers it (tests use it to force many batches
+// on small corpora).
+int64_t batch_entries() {
+    constexpr int64_t kMax = int64_t{1} << 29;
+    if (const char* e = std::getenv("NSPARSE_BUILD_BATCH_ENTRIES")) {
+        const long long v = std::atoll(e);
+        if (v > 0) return std::min<int64_t>(v, kMax);
+    }
+    return kMax;
 }
 
 int calc_lambda(int lambda, int64_t n) {
@@ -946,7 +992,8 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
     sync("centroids");
     work.release();
 
-    // ---- 4. centroid term-major index + assignment
+    // ---- 4. centroid term-major index + assignment, in batches of consecutive
+    // lists so that each batch's sort values and CSC offsets fit 32 bits
     DeviceBuffer clen(static_cast<size_t>(C) * 8), cent_off(static_cast<size_t>(C + 1) * 8);
     hipLaunchKernelGGL(row_len_kernel, dim3(grid_for(C)), dim3(kB), 0, nullptr, fwd.indptr, centroid_doc.get<int32_t>(),
                        cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), d_kept_base.get<int64_t>(),
@@ -954,41 +1001,16 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
     exclusive_scan_i64(clen.get<int64_t>(), cent_off.get<int64_t>(), static_cast<size_t>(C));
     sync("centroid sizes");
     clen.release();
-    int64_t CE = 0;
-    cent_off.download(&CE, 8, static_cast<size_t>(C) * 8);
+    std::vector<int64_t> list_ce(L + 1);
+    {
+        const auto h_cent_off = cent_off.to_host<int64_t>(static_cast<size_t>(C) + 1);
+        for (int l = 0; l <= L; ++l) list_ce[l] = h_cent_off[lcb[l]];
+    }
+    const std::vector<int> cbatch = list_batches(list_ce, batch_entries());
+    const int64_t cmax = max_batch_entries(list_ce, cbatch);
     const size_t ew = sq ? 1 : 4;
-    DeviceBuffer ckey_in(CE * 4), cval_in(CE * 4), ent_c(CE * 2), ent_w(CE * ew), ckey(CE * 4), cperm(CE * 4);
-    if (sq)
-        hipLaunchKernelGGL((centroid_entries_kernel<uint8_t>), dim3(grid_for(C)), dim3(kB), 0, nullptr, fwd.indptr,
-                           fwd.indices, static_cast<const uint8_t*>(fwd.values), centroid_doc.get<int32_t>(),
-                           cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), cent_off.get<int64_t>(),
-                           static_cast<size_t>(C), ckey_in.get<uint32_t>(), cval_in.get<int32_t>(),
-                           ent_c.get<uint16_t>(), ent_w.get<uint8_t>());
-    else
-        hipLaunchKernelGGL((centroid_entries_kernel<float>), dim3(grid_for(C)), dim3(kB), 0, nullptr, fwd.indptr,
-                           fwd.indices, static_cast<const float*>(fwd.values), centroid_doc.get<int32_t>(),
-                           cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), cent_off.get<int64_t>(),
-                           static_cast<size_t>(C), ckey_in.get<uint32_t>(), cval_in.get<int32_t>(),
-                           ent_c.get<uint16_t>(), ent_w.get<float>());
-    radix_sort_pairs(ckey_in.get<uint32_t>(), ckey.get<uint32_t>(), cval_in.get<int32_t>(), cperm.get<int32_t>(),
-                     static_cast<size_t>(CE), 32);
-    ckey_in.release();
-    cval_in.release();
-    DeviceBuffer cc(CE * 2), cw(CE * ew);
-    if (sq)
-        hipLaunchKernelGGL((gather_csc_kernel<uint8_t>), dim3(grid_for(CE)), dim3(kB), 0, nullptr, cperm.get<int32_t>(),
-                           ent_c.get<uint16_t>(), ent_w.get<uint8_t>(), static_cast<size_t>(CE), cc.get<uint16_t>(),
-                           cw.get<uint8_t>());
-    else
-        hipLaunchKernelGGL((gather_csc_kernel<float>), dim3(grid_for(CE)), dim3(kB), 0, nullptr, cperm.get<int32_t>(),
-                           ent_c.get<uint16_t>(), ent_w.get<float>(), static_cast<size_t>(CE), cc.get<uint16_t>(),
-                           cw.get<float>());
-    sync("centroid index");
-    cperm.release();
-    ent_c.release();
-    ent_w.release();
-    if (CE >= (int64_t{1} << 31)) throw std::invalid_argument("centroid index exceeds 32-bit offsets");
-
+    DeviceBuffer ckey_in(cmax * 4), cval_in(cmax * 4), ent_c(cmax * 2), ent_w(cmax * ew), ckey(cmax * 4),
+        cperm(cmax * 4), cc(cmax * 2), cw(cmax * ew);
     const int assign_grid = std::min(L, 2048);
     DeviceBuffer tmap(static_cast<size_t>(assign_grid) * L * 8), assign(static_cast<size_t>(K) * 4);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(grid_for(static_cast<size_t>(assign_grid) * L * 2)), dim3(kB), 0, nullptr,
@@ -1003,21 +1025,57 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(assign_lds)),
                   "hipFuncSetAttribute");
     }
-    if (sq)
-        hipLaunchKernelGGL((assign_kernel<uint8_t, int32_t>), dim3(assign_grid), dim3(kB), assign_lds, nullptr,
-                           fwd.indptr, fwd.indices, static_cast<const uint8_t*>(fwd.values), d_kept_base.get<int64_t>(),
-                           kept.get<int32_t>(), d_lcb.get<int32_t>(), centroid_doc.get<int32_t>(),
-                           cent_off.get<int64_t>(), ckey.get<uint32_t>(), cc.get<uint16_t>(), cw.get<uint8_t>(),
-                           tmap.get<int2>(), L, assign.get<int32_t>(), L, mc);
-    else
-        hipLaunchKernelGGL((assign_kernel<float, float>), dim3(assign_grid), dim3(kB), assign_lds, nullptr, fwd.indptr,
-                           fwd.indices, static_cast<const float*>(fwd.values), d_kept_base.get<int64_t>(),
-                           kept.get<int32_t>(), d_lcb.get<int32_t>(), centroid_doc.get<int32_t>(),
-                           cent_off.get<int64_t>(), ckey.get<uint32_t>(), cc.get<uint16_t>(), cw.get<float>(),
-                           tmap.get<int2>(), L, assign.get<int32_t>(), L, mc);
-    sync("assignment");
+    for (size_t bi = 0; bi + 1 < cbatch.size(); ++bi) {
+        const int l0 = cbatch[bi], l1 = cbatch[bi + 1];
+        const size_t g0 = static_cast<size_t>(lcb[l0]), ng = static_cast<size_t>(lcb[l1]) - g0;
+        const int64_t eb = list_ce[l0], n = list_ce[l1] - eb;
+        if (n > 0) {
+            if (sq)
+                hipLaunchKernelGGL((centroid_entries_kernel<uint8_t>), dim3(grid_for(ng)), dim3(kB), 0, nullptr,
+                                   fwd.indptr, fwd.indices, static_cast<const uint8_t*>(fwd.values),
+                                   centroid_doc.get<int32_t>(), cluster_list.get<int32_t>(), d_lcb.get<int32_t>(),
+                                   cent_off.get<int64_t>(), g0, ng, eb, ckey_in.get<uint32_t>(),
+                                   cval_in.get<int32_t>(), ent_c.get<uint16_t>(), ent_w.get<uint8_t>());
+            else
+                hipLaunchKernelGGL((centroid_entries_kernel<float>), dim3(grid_for(ng)), dim3(kB), 0, nullptr,
+                                   fwd.indptr, fwd.indices, static_cast<const float*>(fwd.values),
+                                   centroid_doc.get<int32_t>(), cluster_list.get<int32_t>(), d_lcb.get<int32_t>(),
+                                   cent_off.get<int64_t>(), g0, ng, eb, ckey_in.get<uint32_t>(),
+                                   cval_in.get<int32_t>(), ent_c.get<uint16_t>(), ent_w.get<float>());
+            radix_sort_pairs(ckey_in.get<uint32_t>(), ckey.get<uint32_t>(), cval_in.get<int32_t>(),
+                             cperm.get<int32_t>(), static_cast<size_t>(n), 32);
+            if (sq)
+                hipLaunchKernelGGL((gather_csc_kernel<uint8_t>), dim3(grid_for(n)), dim3(kB), 0, nullptr,
+                                   cperm.get<int32_t>(), ent_c.get<uint16_t>(), ent_w.get<uint8_t>(),
+                                   static_cast<size_t>(n), cc.get<uint16_t>(), cw.get<uint8_t>());
+            else
+                hipLaunchKernelGGL((gather_csc_kernel<float>), dim3(grid_for(n)), dim3(kB), 0, nullptr,
+                                   cperm.get<int32_t>(), ent_c.get<uint16_t>(), ent_w.get<float>(),
+                                   static_cast<size_t>(n), cc.get<uint16_t>(), cw.get<float>());
+        }
+        const int grid = std::min(l1 - l0, assign_grid);
+        if (sq)
+            hipLaunchKernelGGL((assign_kernel<uint8_t, int32_t>), dim3(grid), dim3(kB), assign_lds, nullptr,
+                               fwd.indptr, fwd.indices, static_cast<const uint8_t*>(fwd.values),
+                               d_kept_base.get<int64_t>(), kept.get<int32_t>(), d_lcb.get<int32_t>(),
+                               centroid_doc.get<int32_t>(), cent_off.get<int64_t>(), eb, ckey.get<uint32_t>(),
+                               cc.get<uint16_t>(), cw.get<uint8_t>(), tmap.get<int2>(), L, assign.get<int32_t>(), l0,
+                               l1, mc);
+        else
+            hipLaunchKernelGGL((assign_kernel<float, float>), dim3(grid), dim3(kB), assign_lds, nullptr, fwd.indptr,
+                               fwd.indices, static_cast<const float*>(fwd.values), d_kept_base.get<int64_t>(),
+                               kept.get<int32_t>(), d_lcb.get<int32_t>(), centroid_doc.get<int32_t>(),
+                               cent_off.get<int64_t>(), eb, ckey.get<uint32_t>(), cc.get<uint16_t>(),
+                               cw.get<float>(), tmap.get<int2>(), L, assign.get<int32_t>(), l0, l1, mc);
+        sync("assignment");
+    }
     tmap.release();
+    ckey_in.release();
+    cval_in.release();
+    ent_c.release();
+    ent_w.release();
     ckey.release();
+    cperm.release();
     cc.release();
     cw.release();
     cent_off.release();
@@ -1092,45 +1150,84 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
         sync("summaries");
         if (pass == 0) continue;
 
-        // ---- 7. term-major transpose per list
-        DeviceBuffer key_in(E * 4), val_in(E * 4), key(E * 4), perm(E * 4);
-        hipLaunchKernelGGL(summary_key_kernel, dim3(grid_for(C)), dim3(kB), 0, nullptr, sterm.get<uint16_t>(),
-                           out_off.get<int64_t>(), cluster_list.get<int32_t>(), C, key_in.get<uint32_t>(),
-                           val_in.get<int32_t>());
-        radix_sort_pairs(key_in.get<uint32_t>(), key.get<uint32_t>(), val_in.get<int32_t>(), perm.get<int32_t>(),
-                         static_cast<size_t>(E), 32);
-        key_in.release();
-        val_in.release();
+        // ---- 7. term-major transpose per list, in batches of consecutive lists
+        std::vector<int64_t> list_se(L + 1);
+        {
+            const auto h_out_off = out_off.to_host<int64_t>(static_cast<size_t>(C) + 1);
+            for (int l = 0; l <= L; ++l) list_se[l] = h_out_off[lcb[l]];
+        }
+        const std::vector<int> sbatch = list_batches(list_se, batch_entries());
+        const int64_t smax = max_batch_entries(list_se, sbatch);
+        DeviceBuffer key_in(smax * 4), val_in(smax * 4), key(smax * 4), perm(smax * 4), flag(smax * 8),
+            hscan((smax + 1) * 8);
         out.sum_cluster.allocate(E * 2);
         out.sum_value.allocate(E * ew, 64);
-        if (sq)
-            hipLaunchKernelGGL((gather_summary_kernel<uint8_t>), dim3(grid_for(E)), dim3(kB), 0, nullptr,
-                               perm.get<int32_t>(), scl.get<uint16_t>(), sval.get<uint8_t>(), static_cast<size_t>(E),
-                               out.sum_cluster.get<uint16_t>(), out.sum_value.get<uint8_t>());
-        else
-            hipLaunchKernelGGL((gather_summary_kernel<float>), dim3(grid_for(E)), dim3(kB), 0, nullptr,
-                               perm.get<int32_t>(), scl.get<uint16_t>(), sval.get<float>(), static_cast<size_t>(E),
-                               out.sum_cluster.get<uint16_t>(), out.sum_value.get<float>());
-        DeviceBuffer flag(E * 8), hscan((E + 1) * 8);
-        hipLaunchKernelGGL(head_flag_kernel, dim3(grid_for(E)), dim3(kB), 0, nullptr, key.get<uint32_t>(),
-                           static_cast<size_t>(E), flag.get<int64_t>());
-        exclusive_scan_i64(flag.get<int64_t>(), hscan.get<int64_t>(), static_cast<size_t>(E));
-        sync("summary heads");
-        int64_t T = 0;
-        hscan.download(&T, 8, static_cast<size_t>(E) * 8);
-        if (T >= (int64_t{1} << 31)) throw std::invalid_argument("summary term table exceeds 32-bit offsets");
-        out.sum_term_ids.allocate(T * 2);
-        out.sum_term_ptr.allocate((T + 1) * 8);
-        hipLaunchKernelGGL(head_write_kernel, dim3(grid_for(E)), dim3(kB), 0, nullptr, key.get<uint32_t>(),
-                           hscan.get<int64_t>(), static_cast<size_t>(E), out.sum_term_ids.get<uint16_t>(),
-                           out.sum_term_ptr.get<int64_t>());
-        hip_check(hipMemcpy(out.sum_term_ptr.get<int64_t>() + T, &E, 8, hipMemcpyHostToDevice), "hipMemcpy");
         out.list_entry_base.allocate(static_cast<size_t>(L + 1) * 8);
         out.list_term_base.allocate(static_cast<size_t>(L + 1) * 4);
-        hipLaunchKernelGGL(list_bases_kernel, dim3(grid_for(L + 1)), dim3(kB), 0, nullptr, d_lcb.get<int32_t>(),
-                           out_off.get<int64_t>(), hscan.get<int64_t>(), E, T, L, out.list_entry_base.get<int64_t>(),
-                           out.list_term_base.get<int32_t>());
-        sync("summary transpose");
+        std::vector<DeviceBuffer> head_ids, head_ptr;
+        std::vector<int64_t> head_count;
+        int64_t T = 0;
+        for (size_t bi = 0; bi + 1 < sbatch.size(); ++bi) {
+            const int l0 = sbatch[bi], l1 = sbatch[bi + 1];
+            const size_t g0 = static_cast<size_t>(lcb[l0]), ng = static_cast<size_t>(lcb[l1]) - g0;
+            const int64_t eb = list_se[l0], n = list_se[l1] - eb;
+            if (n > 0) {
+                hipLaunchKernelGGL(summary_key_kernel, dim3(grid_for(ng)), dim3(kB), 0, nullptr, sterm.get<uint16_t>(),
+                                   out_off.get<int64_t>(), cluster_list.get<int32_t>(), g0, ng, eb,
+                                   key_in.get<uint32_t>(), val_in.get<int32_t>());
+                radix_sort_pairs(key_in.get<uint32_t>(), key.get<uint32_t>(), val_in.get<int32_t>(),
+                                 perm.get<int32_t>(), static_cast<size_t>(n), 32);
+                if (sq)
+                    hipLaunchKernelGGL((gather_summary_kernel<uint8_t>), dim3(grid_for(n)), dim3(kB), 0, nullptr,
+                                       perm.get<int32_t>(), scl.get<uint16_t>() + eb, sval.get<uint8_t>() + eb,
+                                       static_cast<size_t>(n), out.sum_cluster.get<uint16_t>() + eb,
+                                       out.sum_value.get<uint8_t>() + eb);
+                else
+                    hipLaunchKernelGGL((gather_summary_kernel<float>), dim3(grid_for(n)), dim3(kB), 0, nullptr,
+                                       perm.get<int32_t>(), scl.get<uint16_t>() + eb, sval.get<float>() + eb,
+                                       static_cast<size_t>(n), out.sum_cluster.get<uint16_t>() + eb,
+                                       out.sum_value.get<float>() + eb);
+                hipLaunchKernelGGL(head_flag_kernel, dim3(grid_for(n)), dim3(kB), 0, nullptr, key.get<uint32_t>(),
+                                   static_cast<size_t>(n), flag.get<int64_t>());
+            }
+            exclusive_scan_i64(flag.get<int64_t>(), hscan.get<int64_t>(), static_cast<size_t>(n));
+            sync("summary heads");
+            int64_t tb = 0;
+            hscan.download(&tb, 8, static_cast<size_t>(n) * 8);
+            head_ids.emplace_back(tb * 2);
+            head_ptr.emplace_back(tb * 8);
+            head_count.push_back(tb);
+            if (n > 0)
+                hipLaunchKernelGGL(head_write_kernel, dim3(grid_for(n)), dim3(kB), 0, nullptr, key.get<uint32_t>(),
+                                   hscan.get<int64_t>(), static_cast<size_t>(n), eb,
+                                   head_ids.back().get<uint16_t>(), head_ptr.back().get<int64_t>());
+            hipLaunchKernelGGL(list_bases_kernel, dim3(grid_for(l1 - l0)), dim3(kB), 0, nullptr, d_lcb.get<int32_t>(),
+                               out_off.get<int64_t>(), hscan.get<int64_t>(), eb, n, T, l0, l1,
+                               out.list_entry_base.get<int64_t>(), out.list_term_base.get<int32_t>());
+            sync("summary transpose");
+            T += tb;
+            if (T >= (int64_t{1} << 31)) throw std::invalid_argument("summary term table exceeds 32-bit offsets");
+        }
+        out.sum_term_ids.allocate(T * 2);
+        out.sum_term_ptr.allocate((T + 1) * 8);
+        int64_t h = 0;
+        for (size_t bi = 0; bi < head_count.size(); ++bi) {
+            const int64_t tb = head_count[bi];
+            if (tb > 0) {
+                hip_check(hipMemcpy(out.sum_term_ids.get<uint16_t>() + h, head_ids[bi].get(), tb * 2,
+                                    hipMemcpyDeviceToDevice),
+                          "hipMemcpy D2D");
+                hip_check(hipMemcpy(out.sum_term_ptr.get<int64_t>() + h, head_ptr[bi].get(), tb * 8,
+                                    hipMemcpyDeviceToDevice),
+                          "hipMemcpy D2D");
+            }
+            h += tb;
+        }
+        const int32_t t32 = static_cast<int32_t>(T);
+        hip_check(hipMemcpy(out.sum_term_ptr.get<int64_t>() + T, &E, 8, hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMemcpy(out.list_entry_base.get<int64_t>() + L, &E, 8, hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMemcpy(out.list_term_base.get<int32_t>() + L, &t32, 4, hipMemcpyHostToDevice), "hipMemcpy");
+        sync("summary term table");
     }
     out.list_cluster_base = std::move(d_lcb);
     out.cluster_doc_ptr = std::move(cdp);

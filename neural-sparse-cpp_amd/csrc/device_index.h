@@ -49,6 +49,15 @@ enum StatField {
     kStatDocsRead,          // doc ids read from opened clusters
     kStatDocsScored,        // forward-index rows gathered and scored
     kStatNnzScored,         // non-zeros of those rows
+    // wall-clock ticks (device wall clock) spent per phase, first lane's view
+    kStatTicksSetup,        // query into LDS + cut
+    kStatTicksSummary,      // summary columns -> block scores
+    kStatTicksOrder,        // first-list cluster order
+    kStatTicksPlan,         // chunk planning (wave 0)
+    kStatTicksGather,       // doc ids, selector, row ranges
+    kStatTicksScore,        // row gathers and dot products
+    kStatTicksReplay,       // exact replay + top-k merge
+    kStatTicksTotal,
     kStatFields
 };
 

@@ -75,7 +75,7 @@ struct LdsPlan {
     int32_t kmax;      // top-k capacity
     int32_t uni_bytes; // union scratch (staging / sort buffer / FP32 tile)
     int32_t o_qbits, o_qpref, o_eterm, o_erank, o_eval, o_rlast, o_rval, o_rsum, o_rcol, o_flist, o_cuts;
-    int32_t o_cstart, o_score, o_order, o_uni, o_tks, o_tki, o_seg, o_misc, o_wsum;
+    int32_t o_cstart, o_score, o_order, o_uni, o_tks, o_tki, o_seg, o_misc, o_wsum, o_wbase;
     int32_t total;
 };
 
@@ -148,11 +148,13 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
     Seg* seg = reinterpret_cast<Seg*>(smem + P.o_seg);
     int32_t* misc = reinterpret_cast<int32_t*>(smem + P.o_misc);
     int32_t* wsum = reinterpret_cast<int32_t*>(smem + P.o_wsum);
+    int32_t* wbase = reinterpret_cast<int32_t*>(smem + P.o_wbase);
     // staging view of the union
     int32_t* st_id = reinterpret_cast<int32_t*>(uni);
     int32_t* st_rs = st_id + kBudget;
     int32_t* st_len = st_rs + kBudget;
     float* st_sc = reinterpret_cast<float*>(st_len + kBudget);
+    uint32_t* st_isc = reinterpret_cast<uint32_t*>(st_len + kBudget);
 
     const DevIndexView& ix = a.ix;
     const int64_t s0 = a.q_indptr[q];
@@ -253,6 +255,17 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
 
     int64_t st_lists = 0, st_found = 0, st_csc = 0, st_tested = 0, st_opened = 0, st_read = 0, st_scored = 0,
             st_nnz = 0;
+    int64_t tk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const bool timing = a.stats != nullptr;
+    uint64_t t_last = timing ? wall_clock64() : 0;
+    auto tick = [&](int slot) {
+        if (timing) {
+            const uint64_t now = wall_clock64();
+            tk[slot] += static_cast<int64_t>(now - t_last);
+            t_last = now;
+        }
+    };
+    tick(0);
     const int ncut = misc[kMiscNcut];
     bool first_list = true;
 
@@ -352,6 +365,7 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
             }
         }
         __syncthreads();
+        tick(1);
         // ---- visiting order (seismic_common.h:105-115) ----
         if (first_list) {
             KeyItem* A = reinterpret_cast<KeyItem*>(uni);
@@ -371,6 +385,7 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
             misc[kMiscBreakPending] = 0;
         }
         __syncthreads();
+        tick(2);
 
         // ---- speculative chunks + exact replay ----
         while (true) {
@@ -441,6 +456,7 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
                 }
             }
             __syncthreads();
+            tick(3);
             const int nseg = misc[kMiscNseg];
             const int total = misc[kMiscTotal];
             // ---- gather doc ids, selector, row ranges ----
@@ -466,37 +482,90 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
                 }
             }
             __syncthreads();
+            tick(4);
             // ---- score the staged rows ----
             if constexpr (SQ) {
+                // Rows are cut into aligned 4-element words (8 B of terms + 4 B of
+                // codes); the chunk's words are dealt to the workgroup's lanes in
+                // order, so a wave reads consecutive words (coalesced) and every
+                // lane keeps kUnroll independent loads in flight. Per-doc sums are
+                // combined with a wave segmented sum and one LDS atomic per run.
                 const uint2* idx2 = reinterpret_cast<const uint2*>(ix.indices);
                 const uint32_t* val4 = static_cast<const uint32_t*>(ix.values);
-                constexpr int NG = kBlock / G;
-                const int g = tid / G, gl = tid % G;
-                for (int j = g; j < total; j += NG) {
-                    const int len = st_len[j];
-                    uint32_t acc = 0;
-                    if (len > 0) {
-                        const int32_t rs = st_rs[j], re = rs + len;
-                        const int w0 = rs >> 2, w1 = (re + 3) >> 2;
-                        for (int w = w0 + gl; w < w1; w += G) {
-                            const uint2 ii = idx2[w];
-                            const uint32_t vv = val4[w];
-                            const int pbase = w << 2;
-                            const uint32_t tt[4] = {ii.x & 0xFFFFu, ii.x >> 16, ii.y & 0xFFFFu, ii.y >> 16};
+                if (wave == 0) {
+                    int run = 0;
+                    for (int b = 0; b < total; b += 64) {
+                        const int j = b + lane;
+                        int nwj = 0;
+                        if (j < total) {
+                            const int len = st_len[j];
+                            if (len > 0) nwj = ((st_rs[j] + len + 3) >> 2) - (st_rs[j] >> 2);
+                            st_isc[j] = 0;
+                        }
+                        const int incl = wave_incl_scan(nwj, lane);
+                        if (j < total) wbase[j] = run + incl - nwj;
+                        run += __shfl(incl, 63, 64);
+                    }
+                    if (lane == 0) wbase[total] = run;
+                }
+                __syncthreads();
+                const int W = wbase[total];
+                constexpr int kUnroll = 4;
+                for (int f0 = 0; f0 < W; f0 += kBlock * kUnroll) {
+                    uint2 ii[kUnroll];
+                    uint32_t vv[kUnroll];
+                    int jj[kUnroll], ww[kUnroll];
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u) {
+                        const int f = f0 + u * kBlock + tid;
+                        jj[u] = -1;
+                        if (f < W) {
+                            int lo = 0, hi = total - 1;
+                            while (lo < hi) {
+                                const int mid = (lo + hi + 1) >> 1;
+                                if (wbase[mid] <= f)
+                                    lo = mid;
+                                else
+                                    hi = mid - 1;
+                            }
+                            jj[u] = lo;
+                            ww[u] = (st_rs[lo] >> 2) + (f - wbase[lo]);
+                            ii[u] = idx2[ww[u]];
+                            vv[u] = val4[ww[u]];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u) {
+                        uint32_t acc = 0;
+                        const int j = jj[u];
+                        if (j >= 0) {
+                            const int32_t rs = st_rs[j], re = rs + st_len[j];
+                            const int pbase = ww[u] << 2;
+                            const uint32_t tt[4] = {ii[u].x & 0xFFFFu, ii[u].x >> 16, ii[u].y & 0xFFFFu,
+                                                    ii[u].y >> 16};
 #pragma unroll
                             for (int t = 0; t < 4; ++t) {
                                 const int pp = pbase + t;
                                 if (pp >= rs && pp < re) {
                                     bool pr;
                                     const int r = QL.rank_of(tt[t], pr);
-                                    if (pr) acc += ((vv >> (8 * t)) & 0xFFu) * static_cast<uint32_t>(rval[r]);
+                                    if (pr) acc += ((vv[u] >> (8 * t)) & 0xFFu) * static_cast<uint32_t>(rval[r]);
                                 }
                             }
                         }
+                        // segmented sum over lanes holding the same doc (keys ascend with the lane)
+#pragma unroll
+                        for (int d = 1; d < 64; d <<= 1) {
+                            const uint32_t o = __shfl_up(acc, d, 64);
+                            const int ok = __shfl_up(j, d, 64);
+                            if (lane >= d && ok == j) acc += o;
+                        }
+                        const int nj = __shfl_down(j, 1, 64);
+                        if (j >= 0 && (lane == 63 || nj != j) && acc != 0) atomicAdd(&st_isc[j], acc);
                     }
-                    acc = group_sum<G>(acc);
-                    if (gl == 0) st_sc[j] = static_cast<float>(acc);
                 }
+                __syncthreads();
+                for (int j = tid; j < total; j += kBlock) st_sc[j] = static_cast<float>(st_isc[j]);
             } else {
                 const float* vf = static_cast<const float*>(ix.values);
                 for (int j = tid; j < total; j += kBlock) {
@@ -509,6 +578,7 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
                 }
             }
             __syncthreads();
+            tick(5);
             // ---- exact replay of the chunk (wave 0) ----
             if (wave == 0) {
                 int tkn = misc[kMiscTkN];
@@ -604,6 +674,7 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
                 }
             }
             __syncthreads();
+            tick(6);
             if (misc[kMiscDone]) break;
         }
         first_list = false;
@@ -632,6 +703,12 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
         s[kStatDocsRead] = st_read;
         s[kStatDocsScored] = st_scored;
         s[kStatNnzScored] = st_nnz;
+        int64_t tot = 0;
+        for (int i = 0; i < 7; ++i) {
+            s[kStatTicksSetup + i] = tk[i];
+            tot += tk[i];
+        }
+        s[kStatTicksTotal] = tot;
     }
 }
 
@@ -672,6 +749,7 @@ LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq) {
     P.o_seg = take(kSegMax * static_cast<int>(sizeof(Seg)));
     P.o_misc = take(kMiscCount * 4);
     P.o_wsum = take(kWaves * 4);
+    P.o_wbase = take((kBudget + 1) * 4);
     P.total = o;
     return P;
 }

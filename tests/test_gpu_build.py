@@ -81,3 +81,24 @@ def test_same_seed_same_index_different_seed_differs(nsparse):
     b = nsparse.serialize_index(gpu_build(nsparse, "seismic,lambda=100|beta=10|alpha=0.4|seed=3", docs))
     c = nsparse.serialize_index(gpu_build(nsparse, "seismic,lambda=100|beta=10|alpha=0.4|seed=4", docs))
     assert a == b and a != c
+
+
+@pytest.mark.parametrize("sq", [1, 0])
+def test_build_in_many_list_batches_matches_oracle(nsparse, monkeypatch, sq):
+    """The centroid index and the summary transpose are built in batches of
+    consecutive posting lists (32-bit sort values per batch). A tiny batch cap
+    forces hundreds of batches, including lists larger than the cap; the bytes
+    must not change."""
+    from nsparse import synth
+
+    docs = synth.make_corpus(synth.doc_spec(seed=41), 4000)
+    vmax = float(docs[2].max())
+    if sq:
+        desc = f"seismic_sq,quantizer=8bit|vmin=0|vmax={vmax:.9g}|lambda=250|beta=25|alpha=0.4|seed=2"
+    else:
+        desc = "seismic,lambda=250|beta=25|alpha=0.4|seed=2"
+    monkeypatch.setenv("NSPARSE_BUILD_BATCH_ENTRIES", "5000")
+    got = nsparse.serialize_index(gpu_build(nsparse, desc, docs))
+    monkeypatch.delenv("NSPARSE_BUILD_BATCH_ENTRIES")
+    want = oracle_bytes(docs, sq, 250, 25, 0.4, 2, 0.0, np.float32(vmax) if sq else 1.0)
+    assert got == want

@@ -151,7 +151,7 @@ def test_batched_device_mixed_k_equals_single(nsparse, data, sq_pair):
     d_v = torch.from_numpy(v).cuda()
     d_ks = torch.from_numpy(ks).cuda()
     p = params(nsparse, 1, 10, 0.8, vmax)
-    stats = torch.zeros((n, 8), dtype=torch.int64, device="cuda")
+    stats = torch.zeros((n, 16), dtype=torch.int64, device="cuda")
     dd, dl = dev.search_device(d_ip, d_ix, d_v, 100, ks=d_ks, params=p, stats=stats)
     torch.cuda.synchronize()
     assert dev.device_error() == 0
