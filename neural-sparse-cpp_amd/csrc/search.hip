@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 
 #include <algorithm>
 #include <stdexcept>
@@ -42,6 +43,8 @@ constexpr int kQMax = 512;    // longest query handled on device
 constexpr int kBudget = 512;  // doc slots per speculative chunk
 constexpr int kSegMax = 64;   // clusters examined per chunk (one wave pass)
 constexpr int kTileFloats = 4096;
+constexpr int kWinMax = 512;      // 64-word windows per first-doc table pass
+constexpr int kWinPerStep = 4;    // windows per wave per scoring step
 
 struct Seg {
     int32_t cluster;    // local cluster id
@@ -75,7 +78,7 @@ struct LdsPlan {
     int32_t kmax;      // top-k capacity
     int32_t uni_bytes; // union scratch (staging / sort buffer / FP32 tile)
     int32_t o_qbits, o_qpref, o_eterm, o_erank, o_eval, o_rlast, o_rval, o_rsum, o_rcol, o_flist, o_cuts;
-    int32_t o_cstart, o_score, o_order, o_uni, o_tks, o_tki, o_seg, o_misc, o_wsum, o_wbase;
+    int32_t o_cstart, o_score, o_order, o_uni, o_tks, o_tki, o_seg, o_misc, o_wsum, o_wbase, o_winfirst;
     int32_t total;
 };
 
@@ -90,10 +93,17 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
     return v;
 }
 
-template <int G>
-__device__ __forceinline__ uint32_t group_sum(uint32_t v) {
-#pragma unroll
-    for (int d = G / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, G);
+// Inclusive prefix sum over the 64 lanes with DPP moves only (no LDS
+// crossbar): Hillis-Steele inside each 16-lane row (row_shr 1, 2, 4, 8), then
+// row_bcast:15 and row_bcast:31 carry the row totals upward. Lanes whose DPP
+// source is outside the row or masked off read the `old` operand, 0.
+__device__ __forceinline__ uint32_t wave_incl_scan_dpp(uint32_t v) {
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x111, 0xf, 0xf, false));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x112, 0xf, 0xf, false));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x114, 0xf, 0xf, false));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x118, 0xf, 0xf, false));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x142, 0xa, 0xf, false));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x143, 0xc, 0xf, false));
     return v;
 }
 
@@ -118,8 +128,13 @@ struct QueryLds {
     }
 };
 
-template <bool SQ, int G>
+// DENSE (uint8 codes only): the query's codes also sit in a dense
+// vocabulary-sized byte table at LDS offset 0, so the doc-row scoring loop
+// does one byte read per element instead of bitmap + rank + value reads; it
+// costs dim bytes of LDS per workgroup.
+template <bool SQ, bool DENSE>
 __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, LdsPlan P) {
+    static_assert(SQ || !DENSE, "the dense code table holds uint8 codes");
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -149,6 +164,8 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
     int32_t* misc = reinterpret_cast<int32_t*>(smem + P.o_misc);
     int32_t* wsum = reinterpret_cast<int32_t*>(smem + P.o_wsum);
     int32_t* wbase = reinterpret_cast<int32_t*>(smem + P.o_wbase);
+    uint16_t* winfirst = reinterpret_cast<uint16_t*>(smem + P.o_winfirst);
+    uint8_t* qcode = smem;  // DENSE only: offset 0
     // staging view of the union
     int32_t* st_id = reinterpret_cast<int32_t*>(uni);
     int32_t* st_rs = st_id + kBudget;
@@ -244,6 +261,14 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
     __syncthreads();
     const int nu = misc[kMiscNu];
     for (int r = tid; r < nu; r += kBlock) rval[r] = evalf[rlast[r]];
+    if constexpr (DENSE) {
+        uint32_t* q32 = reinterpret_cast<uint32_t*>(qcode);
+        for (int w = tid; w < (ix.dim + 3) / 4; w += kBlock) q32[w] = 0;
+        __syncthreads();
+        // get_dense_vector: the last entry of a repeated term wins
+        for (int i = tid; i < qn; i += kBlock)
+            if (rlast[erank[i]] == i) qcode[eterm[i]] = static_cast<uint8_t>(evalf[i]);
+    }
     if (tid == 0) {
         // top_k_tokens (utils/vector_process.h:18-30) on codes (SQ) or raw values.
         KeyItem* heap = reinterpret_cast<KeyItem*>(uni);
@@ -485,13 +510,17 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
             tick(4);
             // ---- score the staged rows ----
             if constexpr (SQ) {
-                // Rows are cut into aligned 4-element words (8 B of terms + 4 B of
-                // codes); the chunk's words are dealt to the workgroup's lanes in
-                // order, so a wave reads consecutive words (coalesced) and every
-                // lane keeps kUnroll independent loads in flight. Per-doc sums are
-                // combined with a wave segmented sum and one LDS atomic per run.
-                const uint2* idx2 = reinterpret_cast<const uint2*>(ix.indices);
-                const uint32_t* val4 = static_cast<const uint32_t*>(ix.values);
+                // Rows are read as aligned 8-element words (16 B of term ids + 8 B
+                // of codes). The chunk's words, doc after doc, form windows of 64
+                // words dealt round-robin to the waves, one word per lane, so
+                // every load is coalesced and each lane keeps kWinPerStep
+                // independent loads in flight. A window's lanes find their doc
+                // from the window's first doc and the doc boundaries inside the
+                // window (a wave-uniform loop); a DPP prefix sum over the window
+                // gives each doc run's sum, added by the run's last lane with one
+                // LDS atomic (integer: exact in any order).
+                const uint4* idx8 = reinterpret_cast<const uint4*>(ix.indices);
+                const uint2* val8 = reinterpret_cast<const uint2*>(ix.values);
                 if (wave == 0) {
                     int run = 0;
                     for (int b = 0; b < total; b += 64) {
@@ -499,72 +528,112 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
                         int nwj = 0;
                         if (j < total) {
                             const int len = st_len[j];
-                            if (len > 0) nwj = ((st_rs[j] + len + 3) >> 2) - (st_rs[j] >> 2);
+                            if (len > 0) nwj = ((st_rs[j] + len + 7) >> 3) - (st_rs[j] >> 3);
                             st_isc[j] = 0;
                         }
                         const int incl = wave_incl_scan(nwj, lane);
                         if (j < total) wbase[j] = run + incl - nwj;
                         run += __shfl(incl, 63, 64);
                     }
-                    if (lane == 0) wbase[total] = run;
+                    // wbase[total] = W; then sentinels above every word index (the
+                    // boundary search below reads up to wbase[total + 64])
+                    wbase[total + lane] = lane == 0 ? run : 0x7FFFFFFF;
+                    wbase[total + 64 + lane] = 0x7FFFFFFF;
                 }
                 __syncthreads();
                 const int W = wbase[total];
-                constexpr int kUnroll = 4;
-                for (int f0 = 0; f0 < W; f0 += kBlock * kUnroll) {
-                    uint2 ii[kUnroll];
-                    uint32_t vv[kUnroll];
-                    int jj[kUnroll], ww[kUnroll];
-#pragma unroll
-                    for (int u = 0; u < kUnroll; ++u) {
-                        const int f = f0 + u * kBlock + tid;
-                        jj[u] = -1;
-                        if (f < W) {
-                            int lo = 0, hi = total - 1;
-                            while (lo < hi) {
-                                const int mid = (lo + hi + 1) >> 1;
-                                if (wbase[mid] <= f)
-                                    lo = mid;
-                                else
-                                    hi = mid - 1;
-                            }
-                            jj[u] = lo;
-                            ww[u] = (st_rs[lo] >> 2) + (f - wbase[lo]);
-                            ii[u] = idx2[ww[u]];
-                            vv[u] = val4[ww[u]];
-                        }
+                const int nwin = (W + 63) >> 6;
+                for (int x0 = 0; x0 < nwin; x0 += kWinMax) {
+                    const int x1 = min(nwin, x0 + kWinMax);
+                    // doc j holds the first word of window x when 64x is in [wbase[j], wbase[j+1])
+                    for (int j = tid; j < total; j += kBlock) {
+                        const int xa = max((wbase[j] + 63) >> 6, x0), xb = min((wbase[j + 1] + 63) >> 6, x1);
+                        for (int x = xa; x < xb; ++x) winfirst[x - x0] = static_cast<uint16_t>(j);
                     }
+                    __syncthreads();
+                    for (int xs = x0 + wave; xs < x1; xs += kWaves * kWinPerStep) {
+                        int jj[kWinPerStep], s0[kWinPerStep], lo[kWinPerStep], hi[kWinPerStep], ww[kWinPerStep];
+                        bool last[kWinPerStep];
+                        uint4 ii[kWinPerStep];
+                        uint2 vv[kWinPerStep];
+                        // 1. each lane's doc and word address (lanes past the end read word 0)
 #pragma unroll
-                    for (int u = 0; u < kUnroll; ++u) {
-                        uint32_t acc = 0;
-                        const int j = jj[u];
-                        if (j >= 0) {
-                            const int32_t rs = st_rs[j], re = rs + st_len[j];
-                            const int pbase = ww[u] << 2;
-                            const uint32_t tt[4] = {ii[u].x & 0xFFFFu, ii[u].x >> 16, ii[u].y & 0xFFFFu,
-                                                    ii[u].y >> 16};
+                        for (int u = 0; u < kWinPerStep; ++u) {
+                            const int x = xs + u * kWaves;
+                            jj[u] = -1;
+                            ww[u] = 0;
+                            lo[u] = 0;
+                            hi[u] = 8;
+                            s0[u] = 0;
+                            last[u] = false;
+                            if (x >= x1) continue;  // wave-uniform
+                            const int F = x << 6, f = F + lane;
+                            int j = winfirst[x - x0], j1 = j;
+                            for (int base = j + 1;; base += 64) {
+                                const int bn = wbase[base + lane];
+                                const uint64_t bm = ballot(bn <= F + 63);
+                                const int nb = __popcll(bm);
+                                for (int i = 0; i < nb; ++i) {
+                                    const int B = __builtin_amdgcn_readlane(bn, i);
+                                    j += f >= B;
+                                    j1 += f + 1 >= B;
+                                }
+                                if (nb < 64) break;
+                            }
+                            if (f < W) {
+                                const int rs = st_rs[j], re = rs + st_len[j], wb = wbase[j];
+                                const int w = (rs >> 3) + (f - wb);
+                                ww[u] = w;
+                                jj[u] = j;
+                                lo[u] = max(rs - (w << 3), 0);
+                                hi[u] = min(re - (w << 3), 8);
+                                last[u] = j1 != j || lane == 63;
+                                s0[u] = max(wb - F, 0);
+                            }
+                        }
+                        // 2. all loads of the step back to back, without branches
 #pragma unroll
-                            for (int t = 0; t < 4; ++t) {
-                                const int pp = pbase + t;
-                                if (pp >= rs && pp < re) {
+                        for (int u = 0; u < kWinPerStep; ++u) {
+                            ii[u] = idx8[ww[u]];
+                            vv[u] = val8[ww[u]];
+                        }
+                        // 3. scores
+#pragma unroll
+                        for (int u = 0; u < kWinPerStep; ++u) {
+                            if (xs + u * kWaves >= x1) continue;  // wave-uniform
+                            // Branch-free: code bytes outside the doc's [rs, re) -- and every
+                            // byte of a lane without a word -- are masked to zero. The masked
+                            // elements' term ids are still valid ids (neighbouring rows or
+                            // the zeroed tail padding), so their lookups stay in bounds.
+                            const bool has = jj[u] >= 0;
+                            const int l8 = 8 * lo[u], h8 = 8 * hi[u];
+                            const uint64_t mlo = has ? (~0ULL << l8) : 0ULL;
+                            const uint64_t mhi = hi[u] < 8 ? ((1ULL << h8) - 1) : ~0ULL;
+                            const uint64_t codes = ((static_cast<uint64_t>(vv[u].y) << 32) | vv[u].x) & mlo & mhi;
+                            const uint32_t tw[4] = {ii[u].x, ii[u].y, ii[u].z, ii[u].w};
+                            uint32_t acc = 0;
+#pragma unroll
+                            for (int t = 0; t < 8; ++t) {
+                                const uint32_t term = (t & 1) ? (tw[t >> 1] >> 16) : (tw[t >> 1] & 0xFFFFu);
+                                const uint32_t c = static_cast<uint32_t>(codes >> (8 * t)) & 0xFFu;
+                                if constexpr (DENSE) {
+                                    acc += c * static_cast<uint32_t>(qcode[term]);
+                                } else {
                                     bool pr;
-                                    const int r = QL.rank_of(tt[t], pr);
-                                    if (pr) acc += ((vv[u] >> (8 * t)) & 0xFFu) * static_cast<uint32_t>(rval[r]);
+                                    const int r = QL.rank_of(term, pr);
+                                    const uint32_t qv = static_cast<uint32_t>(rval[r]);
+                                    acc += pr ? c * qv : 0U;
                                 }
                             }
+                            const uint32_t S = wave_incl_scan_dpp(acc);
+                            const int from = jj[u] >= 0 ? s0[u] - 1 : -1;
+                            const uint32_t before = static_cast<uint32_t>(__shfl(static_cast<int>(S), max(from, 0), 64));
+                            const uint32_t sum = S - (from >= 0 ? before : 0U);
+                            if (jj[u] >= 0 && last[u] && sum != 0) atomicAdd(&st_isc[jj[u]], sum);
                         }
-                        // segmented sum over lanes holding the same doc (keys ascend with the lane)
-#pragma unroll
-                        for (int d = 1; d < 64; d <<= 1) {
-                            const uint32_t o = __shfl_up(acc, d, 64);
-                            const int ok = __shfl_up(j, d, 64);
-                            if (lane >= d && ok == j) acc += o;
-                        }
-                        const int nj = __shfl_down(j, 1, 64);
-                        if (j >= 0 && (lane == 63 || nj != j) && acc != 0) atomicAdd(&st_isc[j], acc);
                     }
+                    __syncthreads();
                 }
-                __syncthreads();
                 for (int j = tid; j < total; j += kBlock) st_sc[j] = static_cast<float>(st_isc[j]);
             } else {
                 const float* vf = static_cast<const float*>(ix.values);
@@ -714,7 +783,7 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
 
 namespace {
 
-LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq) {
+LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq, bool dense) {
     LdsPlan P{};
     P.nw = (dim + 31) / 32;
     P.mc = max(max_clusters, 1);
@@ -729,15 +798,16 @@ LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq) {
         o += (bytes + 15) & ~15;
         return at;
     };
+    if (dense) take(dim);  // the dense code table must sit at offset 0
     P.o_qbits = take(P.nw * 4);
     P.o_qpref = take(P.nw * 2);
     P.o_eterm = take(kQMax * 2);
     P.o_erank = take(kQMax * 2);
     P.o_eval = take(kQMax * 4);
-    P.o_rlast = take(kQMax * 4);
+    P.o_rlast = take(kQMax * 4);  // query setup only; shared with rcol (per list)
+    P.o_rcol = P.o_rlast;
     P.o_rval = take(kQMax * 4);
     P.o_rsum = take(kQMax * 4);
-    P.o_rcol = take(kQMax * 4);
     P.o_flist = take(kQMax * 4);
     P.o_cuts = take(kQMax * 2);
     P.o_cstart = take((P.mc + 1) * 4);
@@ -749,7 +819,8 @@ LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq) {
     P.o_seg = take(kSegMax * static_cast<int>(sizeof(Seg)));
     P.o_misc = take(kMiscCount * 4);
     P.o_wsum = take(kWaves * 4);
-    P.o_wbase = take((kBudget + 1) * 4);
+    P.o_wbase = take((kBudget + 128) * 4);
+    P.o_winfirst = take(kWinMax * 2);
     P.total = o;
     return P;
 }
@@ -921,7 +992,30 @@ void launch_exact_scores(const int32_t* indptr, const uint16_t* indices, const f
     hip_check(hipGetLastError(), "exact scores launch");
 }
 
-int search_lds_bytes(int dim, int max_clusters, int kmax, bool sq) { return make_plan(dim, max_clusters, kmax, sq).total; }
+namespace {
+
+// Doc-row scoring lookup for uint8 indexes: the bitmap + rank table (default:
+// about 40 KB of LDS per workgroup for a 30522-term vocabulary, four queries
+// per CU) or the dense code table (NSPARSE_SEARCH_LOOKUP=dense: fewer LDS
+// reads per element but dim more bytes, two queries per CU). The dense table
+// is dropped when it would not fit in LDS.
+LdsPlan choose_plan(int dim, int max_clusters, int kmax, bool sq, bool& dense) {
+    const char* e = std::getenv("NSPARSE_SEARCH_LOOKUP");
+    dense = sq && e != nullptr && std::string(e) == "dense";
+    LdsPlan P = make_plan(dim, max_clusters, kmax, sq, dense);
+    if (dense && P.total > 160 * 1024) {
+        dense = false;
+        P = make_plan(dim, max_clusters, kmax, sq, false);
+    }
+    return P;
+}
+
+}  // namespace
+
+int search_lds_bytes(int dim, int max_clusters, int kmax, bool sq) {
+    bool dense = false;
+    return choose_plan(dim, max_clusters, kmax, sq, dense).total;
+}
 
 std::string device_error_message(int32_t code) {
     switch (code) {
@@ -934,7 +1028,8 @@ std::string device_error_message(int32_t code) {
 }
 
 void launch_search_checked(const SearchArgs& args, int max_clusters, int kmax, bool sq, hipStream_t stream) {
-    const LdsPlan P = make_plan(args.ix.dim, max_clusters, kmax, sq);
+    bool dense = false;
+    const LdsPlan P = choose_plan(args.ix.dim, max_clusters, kmax, sq, dense);
     if (P.total > 160 * 1024)
         throw std::invalid_argument("search does not fit in LDS for this dimension / cluster count / k");
     constexpr int64_t kMaxGrid = 1 << 30;
@@ -943,15 +1038,18 @@ void launch_search_checked(const SearchArgs& args, int max_clusters, int kmax, b
         a.query_base = base;
         const int64_t nb = args.n_queries - base < kMaxGrid ? args.n_queries - base : kMaxGrid;
         const dim3 grid(static_cast<unsigned>(nb));
-        const void* kfn = sq ? reinterpret_cast<const void*>(seismic_search_kernel<true, 16>)
-                             : reinterpret_cast<const void*>(seismic_search_kernel<false, 16>);
+        const void* kfn = !sq    ? reinterpret_cast<const void*>(seismic_search_kernel<false, false>)
+                          : dense ? reinterpret_cast<const void*>(seismic_search_kernel<true, true>)
+                                  : reinterpret_cast<const void*>(seismic_search_kernel<true, false>);
         if (P.total > 64 * 1024)
             hip_check(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, P.total),
                       "hipFuncSetAttribute");
-        if (sq)
-            hipLaunchKernelGGL((seismic_search_kernel<true, 16>), grid, dim3(kBlock), P.total, stream, a, P);
+        if (!sq)
+            hipLaunchKernelGGL((seismic_search_kernel<false, false>), grid, dim3(kBlock), P.total, stream, a, P);
+        else if (dense)
+            hipLaunchKernelGGL((seismic_search_kernel<true, true>), grid, dim3(kBlock), P.total, stream, a, P);
         else
-            hipLaunchKernelGGL((seismic_search_kernel<false, 16>), grid, dim3(kBlock), P.total, stream, a, P);
+            hipLaunchKernelGGL((seismic_search_kernel<true, false>), grid, dim3(kBlock), P.total, stream, a, P);
         hip_check(hipGetLastError(), "search kernel launch");
     }
 }
