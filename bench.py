@@ -51,6 +51,58 @@ def algorithmic_bytes(stats):
     return (8 * s[:, 0] + 16 * s[:, 1] + 3 * s[:, 2] + 16 * s[:, 4] + 4 * s[:, 5] + 8 * s[:, 6] + 3 * s[:, 7])
 
 
+def exact_truth(args, nsparse, index, docs, queries, qix, qv, vmin, vmax, n_rec, dev, stream):
+    """Exact top-10 (score desc, id asc, positive scores) of the first n_rec
+    queries on the GPU: FP32 on the unquantized corpus, and on the index's own
+    uint8 codes with the query quantized as the index does it."""
+    import ctypes
+
+    import torch
+
+    d_ip = torch.from_numpy(docs[0]).to(dev)
+    d_ix = torch.from_numpy(docs[1].view(np.int16)).to(dev)
+    d_v = torch.from_numpy(docs[2]).to(dev)
+    scores = torch.empty(args.docs, dtype=torch.float32, device=dev)
+    truth = []
+    for i in range(n_rec):
+        a, b = int(queries[0][i]), int(queries[0][i + 1])
+        nsparse._lib.check(nsparse._lib.lib.nsparse_exact_scores_device(
+            args.docs, d_ip.data_ptr(), d_ix.data_ptr(), d_v.data_ptr(), 30522, b - a,
+            ctypes.c_void_p(qix.data_ptr() + 2 * a), ctypes.c_void_p(qv.data_ptr() + 4 * a), scores.data_ptr(),
+            ctypes.c_void_p(stream.cuda_stream)))
+        top_v, top_i = torch.topk(scores, 64)
+        tv, ti = top_v.cpu().numpy(), top_i.cpu().numpy().astype(np.int64)
+        keep = tv > 0
+        tv, ti = tv[keep], ti[keep]
+        order = np.lexsort((ti, -tv))[:10]
+        truth.append(set(ti[order].tolist()))
+    # the same brute force on the index's own uint8 codes (query quantized like the
+    # index does it): the quantization ceiling, separate from SEISMIC's pruning
+    _, _, codes = index.get_vectors()
+    d_v = torch.from_numpy(codes).to(dev).to(torch.float32)
+    del codes
+    scale = np.float32(255.0) / (np.float32(vmax) - np.float32(vmin))
+    qx = (queries[2].astype(np.float32) - np.float32(vmin)) * scale
+    qx = np.minimum(np.maximum(qx, np.float32(0.0)), np.float32(255.0))
+    qcodes = torch.from_numpy(np.floor(qx.astype(np.float64) + 0.5).astype(np.float32)).to(dev)
+    truth_sq = []
+    for i in range(n_rec):
+        a, b = int(queries[0][i]), int(queries[0][i + 1])
+        nsparse._lib.check(nsparse._lib.lib.nsparse_exact_scores_device(
+            args.docs, d_ip.data_ptr(), d_ix.data_ptr(), d_v.data_ptr(), 30522, b - a,
+            ctypes.c_void_p(qix.data_ptr() + 2 * a), ctypes.c_void_p(qcodes.data_ptr() + 4 * a), scores.data_ptr(),
+            ctypes.c_void_p(stream.cuda_stream)))
+        top_v, top_i = torch.topk(scores, 64)
+        tv, ti = top_v.cpu().numpy(), top_i.cpu().numpy().astype(np.int64)
+        keep = tv > 0
+        tv, ti = tv[keep], ti[keep]
+        order = np.lexsort((ti, -tv))[:10]
+        truth_sq.append(set(ti[order].tolist()))
+    del d_ip, d_ix, d_v, scores, qcodes
+
+    return truth, truth_sq
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +120,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--sweep", default="", help="comma-separated cut:heap_factor pairs to time and score")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (query-sharded replicas are a "
@@ -166,37 +219,48 @@ def main():
         if i < 20:
             assert (ol.cpu().numpy()[0] == labels[i, :k]).all(), "plain and batched results differ"
     lat = np.sort(np.array(lat))
-    plain_qps = n_plain / float(lat.sum())
-    pct = lambda f: float(lat[min(len(lat) - 1, int(math.floor(f * len(lat))))] * 1e3)  # noqa: E731
-    log(f"plain: {plain_qps:,.0f} QPS, p50 {pct(0.5):.3f} ms, p99 {pct(0.99):.3f} ms")
+    plain_qps = n_plain / float(lat.sum()) if n_plain else float("nan")
+    # percentile as the reference benchmarks define it: sorted[floor(frac * n)]
+    pct = lambda f: float(lat[min(len(lat) - 1, int(math.floor(f * len(lat))))] * 1e3) if n_plain else None  # noqa
+    if n_plain:
+        log(f"plain: {plain_qps:,.0f} QPS, p50 {pct(0.5):.3f} ms, p99 {pct(0.99):.3f} ms")
 
     # ---- recall@10 against exact FP32 scores on the unquantized corpus (GPU)
     import ctypes
 
     n_rec = min(args.recall_queries, args.queries)
-    d_ip = torch.from_numpy(docs[0]).to(dev)
+    truth, truth_sq = [], []
+    if n_rec:
+        truth, truth_sq = exact_truth(args, nsparse, index, docs, queries, qix, qv, vmin, vmax, n_rec, dev, stream)
+
+    def recall_of(lab, gt=truth):
+        hits = [len(set(int(x) for x in lab[i, :10] if x >= 0) & t) / len(t) for i, t in enumerate(gt) if t]
+        return float(np.mean(hits)) if hits else float("nan")
+
+    recall = recall_of(labels)
+    recall_sq = recall_of(labels, truth_sq)
+    ceiling = float(np.mean([len(a & b) / len(a) for a, b in zip(truth, truth_sq) if a])) if n_rec else float("nan")
+    log(f"recall@10 = {recall:.4f} over {n_rec} queries (vs uint8-exact {recall_sq:.4f}; "
+        f"uint8-exact vs FP32-exact {ceiling:.4f})")
+
     d_ix = torch.from_numpy(docs[1].view(np.int16)).to(dev)
     d_v = torch.from_numpy(docs[2]).to(dev)
-    scores = torch.empty(args.docs, dtype=torch.float32, device=dev)
-    hits = []
-    for i in range(n_rec):
-        a, b = int(queries[0][i]), int(queries[0][i + 1])
-        nsparse._lib.check(nsparse._lib.lib.nsparse_exact_scores_device(
-            args.docs, d_ip.data_ptr(), d_ix.data_ptr(), d_v.data_ptr(), 30522, b - a,
-            ctypes.c_void_p(qix.data_ptr() + 2 * a), ctypes.c_void_p(qv.data_ptr() + 4 * a), scores.data_ptr(),
-            ctypes.c_void_p(stream.cuda_stream)))
-        top_v, top_i = torch.topk(scores, 64)
-        tv, ti = top_v.cpu().numpy(), top_i.cpu().numpy().astype(np.int64)
-        keep = tv > 0
-        tv, ti = tv[keep], ti[keep]
-        order = np.lexsort((ti, -tv))[:10]
-        want = set(ti[order].tolist())
-        if want:
-            got = set(int(x) for x in labels[i, :10] if x >= 0)
-            hits.append(len(got & want) / len(want))
-    recall = float(np.mean(hits)) if hits else 1.0
-    del d_ip, d_ix, d_v, scores
-    log(f"recall@10 = {recall:.4f} over {n_rec} queries")
+    # ---- optional operating-point sweep: batched QPS and recall per (cut, heap_factor)
+    sweep = []
+    for item in filter(None, args.sweep.split(",")):
+        cut, hf = int(item.split(":")[0]), float(item.split(":")[1])
+        p = nsparse.SeismicSearchParameters(cut, hf)
+        index.search_device(qip, qix, qv, k_max, ks=d_ks, params=p, out_d=out_d, out_l=out_l, stream=stream)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            index.search_device(qip, qix, qv, k_max, ks=d_ks, params=p, out_d=out_d, out_l=out_l, stream=stream)
+        torch.cuda.synchronize()
+        sq = args.queries * args.steps / (time.perf_counter() - t1)
+        lab = out_l.cpu().numpy()
+        rc, rcq = recall_of(lab), recall_of(lab, truth_sq)
+        sweep.append({"cut": cut, "heap_factor": hf, "qps": sq, "recall_at_10": rc, "recall_vs_uint8_exact": rcq})
+        log(f"sweep cut={cut} hf={hf}: {sq:,.0f} QPS, recall@10 {rc:.4f} (vs uint8-exact {rcq:.4f})")
 
     # ---- CPU baseline: the reference's search restated in C++ (oracle/),
     # on the same index bytes, all host threads, bounded query sample.
@@ -244,15 +308,20 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": bytes_launch, "kernel_ms": kernel_ms},
         "cpu_baseline": cpu,
-        "plain": {"qps": plain_qps, "p50_ms": pct(0.5), "p99_ms": pct(0.99), "queries": n_plain},
-        "batched_over_plain": qps / plain_qps,
+        "plain": {"qps": plain_qps if n_plain else None, "p50_ms": pct(0.5), "p99_ms": pct(0.99),
+                  "queries": n_plain},
+        "batched_over_plain": qps / plain_qps if n_plain else None,
         "batched_over_cpu": (qps / cpu["value"]) if cpu else None,
-        "recall_at_10": recall, "recall_queries": n_rec,
+        "recall_at_10": recall if n_rec else None, "recall_queries": n_rec,
+        "recall_vs_uint8_exact": recall_sq if n_rec else None,
+        "uint8_exact_vs_fp32_exact": ceiling if n_rec else None,
         "build_s": build_s,
         "traversal_means": dict(zip(["lists", "terms_found", "csc_entries", "clusters_tested", "clusters_opened",
                                      "docs_read", "docs_scored", "nnz_scored"], st[:, :8].mean(axis=0).tolist())),
         "phase_us_per_query": dict(zip(tick_names, tick_means.tolist())),
     }
+    if sweep:
+        result["sweep"] = sweep
     print(json.dumps(result), flush=True)
 
 

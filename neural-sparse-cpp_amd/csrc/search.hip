@@ -44,7 +44,12 @@ constexpr int kBudget = 512;  // doc slots per speculative chunk
 constexpr int kSegMax = 64;   // clusters examined per chunk (one wave pass)
 constexpr int kTileFloats = 4096;
 constexpr int kWinMax = 512;      // 64-word windows per first-doc table pass
-constexpr int kWinPerStep = 4;    // windows per wave per scoring step
+// windows per wave per scoring step: more loads in flight for the dense
+// lookup, whose LDS footprint leaves fewer waves per CU
+template <bool DENSE>
+constexpr int win_per_step() {
+    return DENSE ? 8 : 4;
+}
 
 struct Seg {
     int32_t cluster;    // local cluster id
@@ -519,6 +524,7 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
                 // window (a wave-uniform loop); a DPP prefix sum over the window
                 // gives each doc run's sum, added by the run's last lane with one
                 // LDS atomic (integer: exact in any order).
+                constexpr int kWinPerStep = win_per_step<DENSE>();
                 const uint4* idx8 = reinterpret_cast<const uint4*>(ix.indices);
                 const uint2* val8 = reinterpret_cast<const uint2*>(ix.values);
                 if (wave == 0) {
