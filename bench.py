@@ -42,13 +42,14 @@ def host_threads():
 
 
 def algorithmic_bytes(stats):
-    """Bytes a query's traversal must read (per row of the device counters):
-    per visited list its cluster range (8 B); per query entry found in the
-    summaries its column range (16 B) and entries (2 B cluster + 1 B value);
-    per opened cluster its doc range (16 B); per doc id read 4 B; per row
-    gathered its indptr pair (8 B) and nnz * (2 B term + 1 B code)."""
+    """Bytes the reference's traversal of one query reads (per row of the device
+    counters), uint8 values: per visited list the binary-search probes over its
+    term ids (q_len * 2 B * ceil(log2 T), counter 8); per query entry found in
+    the summaries its term_ptr pair (8 B) and entries (2 B cluster + 1 B value);
+    per prune test the cluster's offsets pair (8 B); per doc id read 4 B; per row
+    scored its indptr pair (8 B) and nnz * (2 B term + 1 B code)."""
     s = stats.astype(np.float64)
-    return (8 * s[:, 0] + 16 * s[:, 1] + 3 * s[:, 2] + 16 * s[:, 4] + 4 * s[:, 5] + 8 * s[:, 6] + 3 * s[:, 7])
+    return (s[:, 8] + 8 * s[:, 1] + 3 * s[:, 2] + 8 * s[:, 3] + 4 * s[:, 5] + 8 * s[:, 6] + 3 * s[:, 7])
 
 
 def exact_truth(args, nsparse, index, docs, queries, qix, qv, vmin, vmax, n_rec, dev, stream):
@@ -198,12 +199,12 @@ def main():
     bytes_launch = float(algorithmic_bytes(st).sum())
     achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
     tick_names = ["setup", "summary", "order", "plan", "gather", "score", "replay"]
-    tick_means = st[:, 8:15].mean(axis=0) * 10.0 / 1e3  # 100 MHz device wall clock -> us
+    tick_means = st[:, 9:16].mean(axis=0) * 10.0 / 1e3  # 100 MHz device wall clock -> us
     log("per-query phase time (us, one workgroup): " + ", ".join(f"{n}={v:.0f}" for n, v in zip(tick_names,
                                                                                                tick_means)))
     log(f"algorithmic {bytes_launch / 1e9:.3f} GB/launch -> {achieved:.0f} GB/s; per-query means "
         + ", ".join(f"{n}={v:.0f}" for n, v in zip(["lists", "found", "csc", "tested", "opened", "read", "scored",
-                                                      "nnz"], st.mean(axis=0))))
+                                                      "nnz", "probe_bytes"], st.mean(axis=0))))
 
     # ---- plain path: one query per launch, synchronous
     n_plain = min(args.plain_queries, args.queries)
@@ -317,7 +318,8 @@ def main():
         "uint8_exact_vs_fp32_exact": ceiling if n_rec else None,
         "build_s": build_s,
         "traversal_means": dict(zip(["lists", "terms_found", "csc_entries", "clusters_tested", "clusters_opened",
-                                     "docs_read", "docs_scored", "nnz_scored"], st[:, :8].mean(axis=0).tolist())),
+                                     "docs_read", "docs_scored", "nnz_scored", "probe_bytes"],
+                                    st[:, :9].mean(axis=0).tolist())),
         "phase_us_per_query": dict(zip(tick_names, tick_means.tolist())),
     }
     if sweep:

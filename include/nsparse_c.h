@@ -129,10 +129,12 @@ int nsparse_search_device(nsparse_index* index, int64_t n, const int32_t* d_indp
 
 /* nsparse_search_device plus per-query traversal counters: d_stats (device,
  * n x 16 int64) receives lists visited, query entries found in summaries,
- * summary entries read, prune tests, clusters opened, doc ids read, rows scored
- * and their non-zeros -- the terms of the algorithmic-byte model -- then the
- * device wall-clock ticks spent in setup, summary scoring, cluster ordering,
- * chunk planning, gathering, scoring, replay, and their total. Extension. */
+ * summary entries read, prune tests, clusters opened, doc ids read, rows scored,
+ * their non-zeros and the bytes of the reference's term_ids binary-search probes
+ * (q_len * 2 * ceil(log2 T) per visited list) -- the terms of the
+ * algorithmic-byte model -- then the device wall-clock ticks (100 MHz) spent in
+ * setup, summary scoring, cluster ordering, chunk planning, gathering, scoring
+ * and replay. Extension. */
 int nsparse_search_device_stats(nsparse_index* index, int64_t n, const int32_t* d_indptr,
                                 const uint16_t* d_indices, const float* d_values, const int32_t* d_ks,
                                 int k_max, const nsparse_search_params* params, float* d_distances,

@@ -49,6 +49,8 @@ enum StatField {
     kStatDocsRead,          // doc ids read from opened clusters
     kStatDocsScored,        // forward-index rows gathered and scored
     kStatNnzScored,         // non-zeros of those rows
+    kStatProbeBytes,        // sum over visited lists of q_len * 2 * ceil(log2 T): the
+                            // reference's binary-search probes over term_ids
     // wall-clock ticks (device wall clock) spent per phase, first lane's view
     kStatTicksSetup,        // query into LDS + cut
     kStatTicksSummary,      // summary columns -> block scores
@@ -57,7 +59,6 @@ enum StatField {
     kStatTicksGather,       // doc ids, selector, row ranges
     kStatTicksScore,        // row gathers and dot products
     kStatTicksReplay,       // exact replay + top-k merge
-    kStatTicksTotal,
     kStatFields
 };
 

@@ -284,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
     __syncthreads();
 
     int64_t st_lists = 0, st_found = 0, st_csc = 0, st_tested = 0, st_opened = 0, st_read = 0, st_scored = 0,
-            st_nnz = 0;
+            st_nnz = 0, st_probe = 0;
     int64_t tk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const bool timing = a.stats != nullptr;
     uint64_t t_last = timing ? wall_clock64() : 0;
@@ -313,6 +313,7 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
         for (int j = tid; j <= nc; j += kBlock) cstart[j] = static_cast<int32_t>(ix.cluster_doc_ptr[lb + j] - dbase);
         const int32_t tb = ix.list_term_base[term];
         const int32_t te = ix.list_term_base[term + 1];
+        if (te - tb > 1) st_probe += static_cast<int64_t>(qn) * 2 * (32 - __clz(te - tb - 1));
         for (int r = tid; r < nu; r += kBlock) rcol[r] = -1;
         if constexpr (SQ) {
             for (int c = tid; c < nc; c += kBlock) sacc[c] = 0;
@@ -778,12 +779,8 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
         s[kStatDocsRead] = st_read;
         s[kStatDocsScored] = st_scored;
         s[kStatNnzScored] = st_nnz;
-        int64_t tot = 0;
-        for (int i = 0; i < 7; ++i) {
-            s[kStatTicksSetup + i] = tk[i];
-            tot += tk[i];
-        }
-        s[kStatTicksTotal] = tot;
+        s[kStatProbeBytes] = st_probe;
+        for (int i = 0; i < 7; ++i) s[kStatTicksSetup + i] = tk[i];
     }
 }
 
