@@ -839,6 +839,110 @@ __global__ void list_bases_kernel(const int32_t* __restrict__ lcb, const int64_t
     }
 }
 
+// ---------------------------------------------------------------- block rows
+
+// 8-element words of each posting's row copy; flags rows too long for the
+// 16-bit length field of row_ref.
+__global__ void block_words_kernel(const int32_t* __restrict__ indptr, const int32_t* __restrict__ docs, size_t n,
+                                   int64_t* __restrict__ words, int32_t* __restrict__ too_long) {
+    GRID_STRIDE(p, n) {
+        const int32_t d = docs[p];
+        const int32_t len = indptr[d + 1] - indptr[d];
+        words[p] = (len + 7) >> 3;
+        if (len > 0xFFFF) atomicOr(too_long, 1);
+    }
+}
+
+// Largest word span of one posting list's row copies.
+__global__ void list_word_span_kernel(const int32_t* __restrict__ lcb, const int64_t* __restrict__ cdp,
+                                      const int64_t* __restrict__ ws, int n_lists,
+                                      unsigned long long* __restrict__ max_span) {
+    GRID_STRIDE(l, static_cast<size_t>(n_lists)) {
+        const int64_t a = ws[cdp[lcb[l]]], b = ws[cdp[lcb[l + 1]]];
+        atomicMax(max_span, static_cast<unsigned long long>(b - a));
+    }
+}
+
+// One 16-lane group per posting copies its row into zero-padded 8-element
+// words (16 B of terms + 8 B of codes, each written whole) and writes row_ref.
+__global__ __launch_bounds__(kB) void block_copy_kernel(const int32_t* __restrict__ indptr,
+                                                        const uint16_t* __restrict__ indices,
+                                                        const uint8_t* __restrict__ values,
+                                                        const int32_t* __restrict__ docs,
+                                                        const int64_t* __restrict__ ws, size_t n,
+                                                        uint4* __restrict__ blk_idx, uint2* __restrict__ blk_val,
+                                                        int64_t* __restrict__ row_ref) {
+    const int g = threadIdx.x >> 4, gl = threadIdx.x & 15;
+    for (size_t p = static_cast<size_t>(blockIdx.x) * (kB / 16) + g; p < n;
+         p += static_cast<size_t>(gridDim.x) * (kB / 16)) {
+        const int32_t d = docs[p];
+        const int32_t r0 = indptr[d], len = indptr[d + 1] - r0;
+        const int64_t w0 = ws[p];
+        if (gl == 0) row_ref[p] = (w0 << 16) | len;
+        const int nw = (len + 7) >> 3;
+        for (int w = gl; w < nw; w += 16) {
+            uint32_t t[4] = {0, 0, 0, 0}, c[2] = {0, 0};
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int i = w * 8 + e;
+                if (i < len) {
+                    t[e >> 1] |= static_cast<uint32_t>(indices[r0 + i]) << (16 * (e & 1));
+                    c[e >> 2] |= static_cast<uint32_t>(values[r0 + i]) << (8 * (e & 3));
+                }
+            }
+            blk_idx[w0 + w] = make_uint4(t[0], t[1], t[2], t[3]);
+            blk_val[w0 + w] = make_uint2(c[0], c[1]);
+        }
+    }
+}
+
+}  // namespace
+
+bool build_block_rows(const DevIndexView& ix, int64_t n_postings, DeviceBuffer& row_ref, DeviceBuffer& blk_indices,
+                      DeviceBuffer& blk_codes) {
+    row_ref.release();
+    blk_indices.release();
+    blk_codes.release();
+    if (ix.esz != 1 || n_postings <= 0) return false;
+    if (ix.indptr == nullptr || ix.indices == nullptr || ix.values == nullptr || ix.docs == nullptr ||
+        ix.list_cluster_base == nullptr || ix.cluster_doc_ptr == nullptr)
+        throw std::logic_error("block rows need the forward index and the posting lists on the device");
+    const size_t D = static_cast<size_t>(n_postings);
+    DeviceBuffer words(D * 8), ws((D + 1) * 8), flag(4), span(8);
+    hip_check(hipMemset(flag.get(), 0, 4), "hipMemset");
+    hip_check(hipMemset(span.get(), 0, 8), "hipMemset");
+    hipLaunchKernelGGL(block_words_kernel, dim3(grid_for(D)), dim3(kB), 0, nullptr, ix.indptr, ix.docs, D,
+                       words.get<int64_t>(), flag.get<int32_t>());
+    exclusive_scan_i64(words.get<int64_t>(), ws.get<int64_t>(), D);
+    hipLaunchKernelGGL(list_word_span_kernel, dim3(grid_for(static_cast<size_t>(ix.n_lists))), dim3(kB), 0, nullptr,
+                       ix.list_cluster_base, ix.cluster_doc_ptr, ws.get<int64_t>(), ix.n_lists,
+                       span.get<unsigned long long>());
+    sync("block row sizes");
+    words.release();
+    int64_t W = 0, max_span = 0;
+    int32_t too_long = 0;
+    ws.download(&W, 8, D * 8);
+    span.download(&max_span, 8);
+    flag.download(&too_long, 4);
+    // the search keeps a list's row offsets as 32-bit element counts
+    if (too_long != 0 || max_span * 8 >= (int64_t{1} << 31)) return false;
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    const size_t need = static_cast<size_t>(W) * 24 + D * 8 + (size_t{1} << 30);
+    if (need > free_b) return false;
+    row_ref.allocate(D * 8);
+    blk_indices.allocate(static_cast<size_t>(W) * 16, 64);
+    blk_codes.allocate(static_cast<size_t>(W) * 8, 64);
+    const unsigned grid = static_cast<unsigned>(std::min<size_t>((D + kB / 16 - 1) / (kB / 16), 1u << 20));
+    hipLaunchKernelGGL(block_copy_kernel, dim3(std::max(grid, 1u)), dim3(kB), 0, nullptr, ix.indptr, ix.indices,
+                       static_cast<const uint8_t*>(ix.values), ix.docs, ws.get<int64_t>(), D,
+                       blk_indices.get<uint4>(), blk_codes.get<uint2>(), row_ref.get<int64_t>());
+    sync("block row copy");
+    return true;
+}
+
+namespace {
+
 // Splits lists [0, L) into consecutive ranges [b[i], b[i + 1]) whose entry
 // counts (start[l1] - start[l0], `start` holding L + 1 prefix offsets) stay
 // at or below `cap`; a single list larger than `cap` forms a range alone.

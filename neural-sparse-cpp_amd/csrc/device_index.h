@@ -13,6 +13,11 @@
 //   sum_term_ptr      int64[T+1]  -> entries of column t (term_ptr_, rebased)
 //   sum_cluster       uint16[E]   -> local cluster id per entry (csc_cluster_)
 //   sum_value         uint8/float[E] (csc_value_)
+// Block-contiguous row copies (uint8 indexes, optional): for posting position p
+// (the order of `docs`), row_ref[p] = word_start << 16 | nnz of docs[p]; its row
+// is copied to 8-element words [word_start, word_start + ceil(nnz / 8)) of
+// blk_indices (8 uint16 per word) and blk_codes (8 bytes per word), zero padded,
+// so one cluster's rows are one contiguous, line-aligned stream.
 #pragma once
 
 #include <cstdint>
@@ -36,6 +41,9 @@ struct DevIndexView {
     const int64_t* sum_term_ptr = nullptr;
     const uint16_t* sum_cluster = nullptr;
     const void* sum_value = nullptr;
+    const int64_t* row_ref = nullptr;
+    const uint16_t* blk_indices = nullptr;
+    const uint8_t* blk_codes = nullptr;
 };
 
 // Per-query traversal counters (the algorithmic-byte model): one row of

@@ -5,6 +5,7 @@
 #include "index.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -218,6 +219,29 @@ void SeismicIndex::build() {
     view_.sum_term_ptr = d_sum_term_ptr_.get<int64_t>();
     view_.sum_cluster = d_sum_cluster_.get<uint16_t>();
     view_.sum_value = d_sum_value_.get<void>();
+    refresh_block_rows();
+}
+
+// Block-contiguous row copies for the uint8 search (device_index.h). They cost
+// about 24 B per 8 non-zeros of every posting; NSPARSE_BLOCK_ROWS=0 turns them
+// off, and they are skipped when HBM cannot hold them. The search reads the
+// forward index directly when they are absent.
+void SeismicIndex::refresh_block_rows() {
+    view_.row_ref = nullptr;
+    view_.blk_indices = nullptr;
+    view_.blk_codes = nullptr;
+    const char* e = std::getenv("NSPARSE_BLOCK_ROWS");
+    const bool on = sq_ && !(e != nullptr && std::string(e) == "0");
+    const int64_t postings = static_cast<int64_t>(d_docs_.bytes() / 4);
+    if (!on || !build_block_rows(view_, postings, d_row_ref_, d_blk_indices_, d_blk_codes_)) {
+        d_row_ref_.release();
+        d_blk_indices_.release();
+        d_blk_codes_.release();
+        return;
+    }
+    view_.row_ref = d_row_ref_.get<int64_t>();
+    view_.blk_indices = d_blk_indices_.get<uint16_t>();
+    view_.blk_codes = d_blk_codes_.get<uint8_t>();
 }
 
 void SeismicIndex::ensure_device() {
@@ -228,13 +252,14 @@ void SeismicIndex::ensure_device() {
         d_values_.upload(fwd_.values.data(), fwd_.values.size(), 64);
         dev_fwd_valid_ = true;
     }
-    if (!dev_lists_valid_ && host_lists_valid_) upload_lists();
+    // the forward-index view first: upload_lists derives the block rows from it
     view_.n_docs = static_cast<int32_t>(fwd_.n());
     view_.dim = dim_;
     view_.esz = static_cast<int32_t>(fwd_.esz);
     view_.indptr = d_indptr_.get<int32_t>();
     view_.indices = d_indices_.get<uint16_t>();
     view_.values = d_values_.get<void>();
+    if (!dev_lists_valid_ && host_lists_valid_) upload_lists();
 }
 
 const DevIndexView& SeismicIndex::view() {
@@ -333,6 +358,7 @@ void SeismicIndex::upload_lists() {
     view_.sum_cluster = d_sum_cluster_.get<uint16_t>();
     view_.sum_value = d_sum_value_.get<void>();
     dev_lists_valid_ = true;
+    refresh_block_rows();
 }
 
 // Rebuilds the per-list host structures from the HBM layout (after a GPU build).

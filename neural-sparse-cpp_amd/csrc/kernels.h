@@ -45,6 +45,13 @@ void encode_sq_on_device(const float* host_values, size_t n, float vmin, float v
 void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int lambda, int beta, float alpha,
                      int seed, DeviceLists& out);
 
+// Block-contiguous row copies of every posting (uint8 indexes) for the device
+// search: row_ref (D int64), blk_indices / blk_codes (8-element words). Returns
+// false, leaving the buffers empty, when HBM cannot hold them or a list's rows
+// exceed 32-bit element offsets.
+bool build_block_rows(const DevIndexView& ix, int64_t n_postings, DeviceBuffer& row_ref, DeviceBuffer& blk_indices,
+                      DeviceBuffer& blk_codes);
+
 void launch_search_checked(const SearchArgs& a, int max_clusters, int kmax, bool sq, hipStream_t stream);
 void launch_exact_checked(const ExactArgs& a, bool sq, hipStream_t stream);
 int search_lds_bytes(int dim, int max_clusters, int kmax, bool sq);

@@ -310,6 +310,10 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
             continue;
         }
         const int64_t dbase = ix.cluster_doc_ptr[lb];
+        // block-contiguous row copies: word offsets are kept relative to the
+        // list's first row
+        const bool blk = SQ && ix.row_ref != nullptr;
+        const int64_t lwb = blk ? (ix.row_ref[dbase] >> 16) : 0;
         for (int j = tid; j <= nc; j += kBlock) cstart[j] = static_cast<int32_t>(ix.cluster_doc_ptr[lb + j] - dbase);
         const int32_t tb = ix.list_term_base[term];
         const int32_t te = ix.list_term_base[term + 1];
@@ -501,10 +505,15 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
                         hi = mid - 1;
                 }
                 const Seg& s = seg[lo];
-                const int32_t d = ix.docs[dbase + s.doc_begin + (j - s.slot)];
+                const int64_t pp = dbase + s.doc_begin + (j - s.slot);
+                const int32_t d = ix.docs[pp];
                 st_id[j] = d;
                 const bool keep = sel == nullptr || ((sel[d >> 5] >> (d & 31)) & 1u);
-                if (keep) {
+                if (keep && blk) {
+                    const int64_t ref = ix.row_ref[pp];
+                    st_rs[j] = static_cast<int32_t>(((ref >> 16) - lwb) << 3);
+                    st_len[j] = static_cast<int32_t>(ref & 0xFFFF);
+                } else if (keep) {
                     const int32_t r0 = ix.indptr[d];
                     st_rs[j] = r0;
                     st_len[j] = ix.indptr[d + 1] - r0;
@@ -526,8 +535,10 @@ __global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, Ld
                 // gives each doc run's sum, added by the run's last lane with one
                 // LDS atomic (integer: exact in any order).
                 constexpr int kWinPerStep = win_per_step<DENSE>();
-                const uint4* idx8 = reinterpret_cast<const uint4*>(ix.indices);
-                const uint2* val8 = reinterpret_cast<const uint2*>(ix.values);
+                const uint4* idx8 = blk ? reinterpret_cast<const uint4*>(ix.blk_indices) + lwb
+                                        : reinterpret_cast<const uint4*>(ix.indices);
+                const uint2* val8 = blk ? reinterpret_cast<const uint2*>(ix.blk_codes) + lwb
+                                        : reinterpret_cast<const uint2*>(ix.values);
                 if (wave == 0) {
                     int run = 0;
                     for (int b = 0; b < total; b += 64) {

@@ -194,3 +194,19 @@ def test_idmap_search(nsparse):
     p.set_id_selector(nsparse.SetIDSelector(allowed_ext))
     d2, l2 = im.search(30, qip, qix, qv, 10, p)
     assert np.isin(l2[l2 >= 0], allowed_ext).all()
+
+
+def test_sq_search_with_and_without_block_rows(nsparse, data, monkeypatch):
+    """uint8 indexes score from block-contiguous row copies by default; with
+    NSPARSE_BLOCK_ROWS=0 the kernel reads the forward index directly. Both must
+    equal the oracle."""
+    o, _, vmax = build_pair(nsparse, data[0], 1)
+    blob = o.serialize()
+    for flag in ("0", "1"):
+        monkeypatch.setenv("NSPARSE_BLOCK_ROWS", flag)
+        dev = nsparse.read_index_bytes(blob)
+        for qs in (data[1], data[2]):
+            ip, ix, v = qs
+            want = o.search(ip, ix, v, 10, cut=10, hf=0.8)
+            got = dev.search(len(ip) - 1, ip, ix, v, 10, params(nsparse, 1, 10, 0.8, vmax))
+            assert_same(got, want)
