@@ -37,18 +37,19 @@
 
 namespace nsparse_amd {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;  // exact-match, brute-force and default search workgroups
 constexpr int kWaves = kBlock / 64;
+constexpr int kMaxWaves = 16;
 constexpr int kQMax = 512;    // longest query handled on device
 constexpr int kBudget = 512;  // doc slots per speculative chunk
 constexpr int kSegMax = 64;   // clusters examined per chunk (one wave pass)
 constexpr int kTileFloats = 4096;
 constexpr int kWinMax = 512;      // 64-word windows per first-doc table pass
-// windows per wave per scoring step: more loads in flight for the dense
-// lookup, whose LDS footprint leaves fewer waves per CU
+// windows per wave per scoring step (loads in flight per lane); 4 keeps the
+// kernel within 128 VGPRs, i.e. four waves per SIMD
 template <bool DENSE>
 constexpr int win_per_step() {
-    return DENSE ? 8 : 4;
+    return 4;
 }
 
 struct Seg {
@@ -137,9 +138,12 @@ struct QueryLds {
 // vocabulary-sized byte table at LDS offset 0, so the doc-row scoring loop
 // does one byte read per element instead of bitmap + rank + value reads; it
 // costs dim bytes of LDS per workgroup.
-template <bool SQ, bool DENSE>
-__global__ __launch_bounds__(kBlock) void seismic_search_kernel(SearchArgs a, LdsPlan P) {
+template <bool SQ, bool DENSE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void seismic_search_kernel(SearchArgs a, LdsPlan P) {
     static_assert(SQ || !DENSE, "the dense code table holds uint8 codes");
+    static_assert(BLOCK % 64 == 0 && BLOCK <= 1024, "whole wavefronts, at most 16");
+    constexpr int kBlock = BLOCK;
+    constexpr int kWaves = BLOCK / 64;
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -832,7 +836,7 @@ LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq, bool dense) {
     P.o_tki = take(kmax * 4);
     P.o_seg = take(kSegMax * static_cast<int>(sizeof(Seg)));
     P.o_misc = take(kMiscCount * 4);
-    P.o_wsum = take(kWaves * 4);
+    P.o_wsum = take(kMaxWaves * 4);
     P.o_wbase = take((kBudget + 128) * 4);
     P.o_winfirst = take(kWinMax * 2);
     P.total = o;
@@ -1008,14 +1012,14 @@ void launch_exact_scores(const int32_t* indptr, const uint16_t* indices, const f
 
 namespace {
 
-// Doc-row scoring lookup for uint8 indexes: the bitmap + rank table (default:
-// about 40 KB of LDS per workgroup for a 30522-term vocabulary, four queries
-// per CU) or the dense code table (NSPARSE_SEARCH_LOOKUP=dense: fewer LDS
-// reads per element but dim more bytes, two queries per CU). The dense table
-// is dropped when it would not fit in LDS.
+// Doc-row scoring lookup for uint8 indexes: the dense code table (default: one
+// LDS byte read per element; dim more bytes of LDS, so two 512-lane queries per
+// CU) or the bitmap + rank table (NSPARSE_SEARCH_LOOKUP=bitmap: about 40 KB of
+// LDS for a 30522-term vocabulary, four 256-lane queries per CU, but ~4x the
+// VALU work per element). The dense table is dropped when it would not fit.
 LdsPlan choose_plan(int dim, int max_clusters, int kmax, bool sq, bool& dense) {
     const char* e = std::getenv("NSPARSE_SEARCH_LOOKUP");
-    dense = sq && e != nullptr && std::string(e) == "dense";
+    dense = sq && !(e != nullptr && std::string(e) == "bitmap");
     LdsPlan P = make_plan(dim, max_clusters, kmax, sq, dense);
     if (dense && P.total > 160 * 1024) {
         dense = false;
@@ -1044,6 +1048,11 @@ std::string device_error_message(int32_t code) {
 void launch_search_checked(const SearchArgs& args, int max_clusters, int kmax, bool sq, hipStream_t stream) {
     bool dense = false;
     const LdsPlan P = choose_plan(args.ix.dim, max_clusters, kmax, sq, dense);
+    // uint8: 512-lane workgroups with the dense table, 256 with the bitmap;
+    // NSPARSE_SEARCH_BLOCK=256|512 overrides
+    const char* eb = std::getenv("NSPARSE_SEARCH_BLOCK");
+    bool wide = sq && dense;
+    if (sq && eb != nullptr) wide = std::string(eb) == "512";
     if (P.total > 160 * 1024)
         throw std::invalid_argument("search does not fit in LDS for this dimension / cluster count / k");
     constexpr int64_t kMaxGrid = 1 << 30;
@@ -1052,18 +1061,24 @@ void launch_search_checked(const SearchArgs& args, int max_clusters, int kmax, b
         a.query_base = base;
         const int64_t nb = args.n_queries - base < kMaxGrid ? args.n_queries - base : kMaxGrid;
         const dim3 grid(static_cast<unsigned>(nb));
-        const void* kfn = !sq    ? reinterpret_cast<const void*>(seismic_search_kernel<false, false>)
-                          : dense ? reinterpret_cast<const void*>(seismic_search_kernel<true, true>)
-                                  : reinterpret_cast<const void*>(seismic_search_kernel<true, false>);
+        const void* kfn = !sq          ? reinterpret_cast<const void*>(seismic_search_kernel<false, false, 256>)
+                          : dense && wide ? reinterpret_cast<const void*>(seismic_search_kernel<true, true, 512>)
+                          : dense         ? reinterpret_cast<const void*>(seismic_search_kernel<true, true, 256>)
+                          : wide          ? reinterpret_cast<const void*>(seismic_search_kernel<true, false, 512>)
+                                          : reinterpret_cast<const void*>(seismic_search_kernel<true, false, 256>);
         if (P.total > 64 * 1024)
             hip_check(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, P.total),
                       "hipFuncSetAttribute");
         if (!sq)
-            hipLaunchKernelGGL((seismic_search_kernel<false, false>), grid, dim3(kBlock), P.total, stream, a, P);
+            hipLaunchKernelGGL((seismic_search_kernel<false, false, 256>), grid, dim3(256), P.total, stream, a, P);
+        else if (dense && wide)
+            hipLaunchKernelGGL((seismic_search_kernel<true, true, 512>), grid, dim3(512), P.total, stream, a, P);
         else if (dense)
-            hipLaunchKernelGGL((seismic_search_kernel<true, true>), grid, dim3(kBlock), P.total, stream, a, P);
+            hipLaunchKernelGGL((seismic_search_kernel<true, true, 256>), grid, dim3(256), P.total, stream, a, P);
+        else if (wide)
+            hipLaunchKernelGGL((seismic_search_kernel<true, false, 512>), grid, dim3(512), P.total, stream, a, P);
         else
-            hipLaunchKernelGGL((seismic_search_kernel<true, false>), grid, dim3(kBlock), P.total, stream, a, P);
+            hipLaunchKernelGGL((seismic_search_kernel<true, false, 256>), grid, dim3(256), P.total, stream, a, P);
         hip_check(hipGetLastError(), "search kernel launch");
     }
 }
