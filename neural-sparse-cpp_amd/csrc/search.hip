@@ -59,6 +59,9 @@ struct Seg {
     int32_t slot;       // first staging slot
     int32_t test;       // 1 if this piece starts the cluster (prune test runs here)
     int32_t pos;        // position of the cluster in the visiting order
+    int32_t n_ok;       // docs that pass the selector (stats runs only)
+    int32_t nnz;        // their non-zeros (stats runs only)
+    int32_t pad;
 };
 
 enum Misc {
@@ -74,6 +77,8 @@ enum Misc {
     kMiscDone,   // list finished
     kMiscTkN,    // entries in the top-k
     kMiscBreakPending,
+    kMiscStatFound,  // counters of the current list (stats runs only)
+    kMiscStatCsc,
     kMiscCount = 16
 };
 
@@ -133,6 +138,139 @@ struct QueryLds {
         return p ? rval[r] : 0.0F;
     }
 };
+
+// std::sort(A, A + n, greater_key) by one wavefront, step-exact with the
+// libstdc++ introsort that stl_order.h restates for one lane (median-of-3
+// pivot, unguarded Hoare partition, depth limit 2*floor(log2 n) with heap-sort
+// fallback, 16-element threshold, final insertion sort).
+//
+// One Hoare partition of [first, last) with pivot p = A[first] (after the
+// median move) is computed in parallel: with L_0 < L_1 < ... the positions in
+// (first, last) whose key is <= p's ("left stoppers") and R_0 > R_1 > ... those
+// whose key is >= p's ("right stoppers"), the sequential scan swaps
+// A[L_i] <-> A[R_i] exactly while L_i < R_i (a swapped element stops the other
+// scanner no later than the next original stopper), and returns
+// min(L_m, R_{m-1}) after m swaps (L_0 when m = 0). The swaps are disjoint.
+// Ranges are pushed on a lane-held stack (lane sp holds entry sp); the order in
+// which disjoint ranges are finished does not change the result. Lpos and Rasc
+// hold n positions each.
+//
+// The final insertion sort is left to the caller: after the partitions every
+// key of an earlier leaf range (<= 16 elements, or a heap-sorted range) is >=
+// every key of a later one, so __final_insertion_sort never moves an element
+// out of its leaf and amounts to a stable sort of each leaf. `leaf_bits` (n
+// bits) marks the first position of every leaf; heap-sorted ranges, already in
+// order, mark every position.
+__device__ void introsort_desc_wave(KeyItem* A, int n, uint16_t* Lpos, uint16_t* Rasc, uint32_t* leaf_bits,
+                                    int lane) {
+    for (int w = lane; w < (n + 31) / 32; w += 64) leaf_bits[w] = w == 0 ? 1u : 0u;
+    __builtin_amdgcn_wave_barrier();
+    if (n <= 1) return;
+    const uint64_t lt = (1ULL << lane) - 1;
+    int s_lo = 0, s_hi = n, s_d = floor_log2(n) * 2;  // entry `lane` of the stack
+    int sp = 1;
+    while (sp > 0) {
+        --sp;
+        int lo = __builtin_amdgcn_readlane(s_lo, sp);
+        int hi = __builtin_amdgcn_readlane(s_hi, sp);
+        int depth = __builtin_amdgcn_readlane(s_d, sp);
+        bool heap = false;
+        while (hi - lo > kIntrosortThreshold) {
+            if (depth == 0) {
+                if (lane == 0) {
+                    heap_sort_range(A, lo, hi);
+                    for (int x = lo; x < hi; ++x) leaf_bits[x >> 5] |= 1u << (x & 31);
+                }
+                __builtin_amdgcn_wave_barrier();
+                heap = true;
+                break;
+            }
+            --depth;
+            if (lane == 0) median_to_first(A, lo, lo + 1, lo + (hi - lo) / 2, hi - 1);
+            __builtin_amdgcn_wave_barrier();
+            const float pk = A[lo].key;
+            int nl = 0, nr = 0;
+            for (int x0 = lo + 1; x0 < hi; x0 += 64) {
+                const int x = x0 + lane;
+                const bool in = x < hi;
+                const float key = in ? A[x].key : 0.0F;
+                const bool isl = in && !(key > pk);
+                const bool isr = in && !(pk > key);
+                const uint64_t ml = ballot(isl), mr = ballot(isr);
+                if (isl) Lpos[nl + __popcll(ml & lt)] = static_cast<uint16_t>(x);
+                if (isr) Rasc[nr + __popcll(mr & lt)] = static_cast<uint16_t>(x);
+                nl += __popcll(ml);
+                nr += __popcll(mr);
+            }
+            __builtin_amdgcn_wave_barrier();
+            // R_i = Rasc[nr - 1 - i]; L_i < R_i holds for a prefix i < m
+            const int pairs = min(nl, nr);
+            int m = 0;
+            for (int i0 = 0; i0 < pairs; i0 += 64) {
+                const int i = i0 + lane;
+                const bool c = i < pairs && Lpos[i] < Rasc[nr - 1 - i];
+                m += __popcll(ballot(c));
+            }
+            for (int i0 = 0; i0 < m; i0 += 64) {
+                const int i = i0 + lane;
+                KeyItem a{}, b{};
+                int pa = 0, pb = 0;
+                if (i < m) {
+                    pa = Lpos[i];
+                    pb = Rasc[nr - 1 - i];
+                    a = A[pa];
+                    b = A[pb];
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (i < m) {
+                    A[pa] = b;
+                    A[pb] = a;
+                }
+            }
+            int cut = m < nl ? static_cast<int>(Lpos[m]) : 0x7FFFFFFF;
+            if (m > 0) cut = min(cut, static_cast<int>(Rasc[nr - m]));
+            __builtin_amdgcn_wave_barrier();
+            // push the right part, continue with the left one
+            if (lane == sp) {
+                s_lo = cut;
+                s_hi = hi;
+                s_d = depth;
+            }
+            ++sp;
+            hi = cut;
+        }
+        if (!heap && lane == 0) leaf_bits[lo >> 5] |= 1u << (lo & 31);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// The final insertion sort as a stable sort of each leaf (see above): element
+// i goes to leaf_start + (elements of its leaf with a greater key, or an equal
+// key before it). Writes out[pos] = A[i].id. Any number of lanes.
+__device__ void introsort_final_pass(const KeyItem* A, int n, const uint32_t* leaf_bits, uint16_t* out, int tid,
+                                     int nthreads) {
+    for (int i = tid; i < n; i += nthreads) {
+        int w = i >> 5;
+        uint32_t m = leaf_bits[w] & ((2u << (i & 31)) - 1u);  // marks at or before i
+        while (m == 0) m = leaf_bits[--w];
+        const int start = w * 32 + 31 - __clz(m);
+        int end = n;
+        const int i1 = i + 1;
+        if (i1 < n) {
+            w = i1 >> 5;
+            m = leaf_bits[w] & ~((1u << (i1 & 31)) - 1u);  // marks at or after i + 1
+            while (m == 0 && (w + 1) * 32 < n) m = leaf_bits[++w];
+            if (m != 0) end = min(n, w * 32 + __ffs(m) - 1);
+        }
+        const float ki = A[i].key;
+        int r = 0;
+        for (int j = start; j < end; ++j) {
+            const float kj = A[j].key;
+            r += (kj > ki || (kj == ki && j < i)) ? 1 : 0;
+        }
+        out[start + r] = static_cast<uint16_t>(A[i].id);
+    }
+}
 
 // DENSE (uint8 codes only): the query's codes also sit in a dense
 // vocabulary-sized byte table at LDS offset 0, so the doc-row scoring loop
@@ -328,16 +466,53 @@ __global__ __launch_bounds__(BLOCK) void seismic_search_kernel(SearchArgs a, Lds
         } else {
             for (int c = tid; c < nc; c += kBlock) score[c] = 0.0F;
         }
-        if (tid == 0) misc[kMiscNf] = 0;
-        __syncthreads();
-        // ---- summary columns matching the query ----
-        for (int i = tb + tid; i < te; i += kBlock) {
-            const uint32_t t = ix.sum_term_ids[i];
-            bool p;
-            const int r = QL.rank_of(t, p);
-            if (p) rcol[r] = i;
+        if (tid == 0) {
+            misc[kMiscNf] = 0;
+            misc[kMiscStatFound] = 0;
+            misc[kMiscStatCsc] = 0;
         }
         __syncthreads();
+        // ---- summary columns matching the query ----
+        {
+            constexpr int kU = 4;  // independent term loads in flight per lane
+            for (int i0 = tb + tid; i0 < te; i0 += kBlock * kU) {
+                uint32_t t[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int i = i0 + u * kBlock;
+                    t[u] = i < te ? ix.sum_term_ids[i] : 0xFFFFFFFFu;
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    if (t[u] == 0xFFFFFFFFu) continue;
+                    bool p;
+                    const int r = QL.rank_of(t[u], p);
+                    if (p) rcol[r] = i0 + u * kBlock;
+                }
+            }
+        }
+        __syncthreads();
+        if (a.stats != nullptr) {
+            // query entries found in this list and their summary entries (a
+            // repeated term counts once per entry, as the reference probes it)
+            int fnd = 0, csc = 0;
+            for (int i = tid; i < qn; i += kBlock) {
+                const int col = rcol[erank[i]];
+                if (col >= 0) {
+                    ++fnd;
+                    csc += static_cast<int>(ix.sum_term_ptr[col + 1] - ix.sum_term_ptr[col]);
+                }
+            }
+#pragma unroll
+            for (int d = 32; d > 0; d >>= 1) {
+                fnd += __shfl_xor(fnd, d, 64);
+                csc += __shfl_xor(csc, d, 64);
+            }
+            if (lane == 0 && fnd != 0) {
+                atomicAdd(&misc[kMiscStatFound], fnd);
+                atomicAdd(&misc[kMiscStatCsc], csc);
+            }
+        }
         if constexpr (SQ) {
             for (int r = tid; r < nu; r += kBlock)
                 if (rcol[r] >= 0) flist[atomicAdd(&misc[kMiscNf], 1)] = r;
@@ -358,23 +533,73 @@ __global__ __launch_bounds__(BLOCK) void seismic_search_kernel(SearchArgs a, Lds
         __syncthreads();
         const int nf = misc[kMiscNf];
         if (a.stats != nullptr && tid == 0) {
-            for (int i = 0; i < qn; ++i) {
-                const int col = rcol[erank[i]];
-                if (col >= 0) {
-                    ++st_found;
-                    st_csc += ix.sum_term_ptr[col + 1] - ix.sum_term_ptr[col];
-                }
-            }
+            st_found += misc[kMiscStatFound];
+            st_csc += misc[kMiscStatCsc];
         }
         if constexpr (SQ) {
+            // The found columns' entries, flattened: one pass loads every
+            // column's range and query-code sum, a prefix over their lengths
+            // numbers the entries, then all lanes take entries with kU loads in
+            // flight, each finding its column by binary search. Integer atomics
+            // make the per-cluster sums exact in any order.
             const uint8_t* sv = static_cast<const uint8_t*>(ix.sum_value);
-            for (int f = wave; f < nf; f += kWaves) {
+            int32_t* fe0 = reinterpret_cast<int32_t*>(uni);  // column start, relative to the list's first entry
+            int32_t* fpre = fe0 + kQMax;                      // [nf + 1] exclusive prefix of column lengths
+            uint32_t* fqs = reinterpret_cast<uint32_t*>(fpre + kQMax + 1);
+            const int64_t lbase = te > tb ? ix.sum_term_ptr[tb] : 0;
+            for (int f = tid; f < nf; f += kBlock) {
                 const int r = flist[f];
                 const int col = rcol[r];
                 const int64_t e0 = ix.sum_term_ptr[col], e1 = ix.sum_term_ptr[col + 1];
-                const uint32_t qs = rsum[r];
-                for (int64_t e = e0 + lane; e < e1; e += 64)
-                    atomicAdd(&sacc[ix.sum_cluster[e]], qs * static_cast<uint32_t>(sv[e]));
+                fe0[f] = static_cast<int32_t>(e0 - lbase);
+                fpre[f + 1] = static_cast<int32_t>(e1 - e0);
+                fqs[f] = rsum[r];
+            }
+            __syncthreads();
+            if (wave == 0) {
+                int run = 0;
+                for (int b = 0; b < nf; b += 64) {
+                    const int f = b + lane;
+                    const int len = f < nf ? fpre[f + 1] : 0;
+                    const int incl = wave_incl_scan(len, lane);
+                    if (f < nf) fpre[f + 1] = run + incl;
+                    run += __shfl(incl, 63, 64);
+                }
+                if (lane == 0) fpre[0] = 0;
+            }
+            __syncthreads();
+            const int total_e = fpre[nf];
+            constexpr int kU = 4;
+            for (int x0 = tid; x0 < total_e; x0 += kBlock * kU) {
+                int fx[kU];
+                int64_t ex[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int x = x0 + u * kBlock;
+                    fx[u] = -1;
+                    ex[u] = lbase;
+                    if (x < total_e) {
+                        int lo = 0, hi = nf - 1;
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (fpre[mid] <= x)
+                                lo = mid;
+                            else
+                                hi = mid - 1;
+                        }
+                        fx[u] = lo;
+                        ex[u] = lbase + fe0[lo] + (x - fpre[lo]);
+                    }
+                }
+                uint32_t cl[kU], vv[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    cl[u] = ix.sum_cluster[ex[u]];
+                    vv[u] = sv[ex[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u)
+                    if (fx[u] >= 0) atomicAdd(&sacc[cl[u]], fqs[fx[u]] * vv[u]);
             }
             __syncthreads();
             for (int c = tid; c < nc; c += kBlock) score[c] = static_cast<float>(sacc[c]);
@@ -410,9 +635,12 @@ __global__ __launch_bounds__(BLOCK) void seismic_search_kernel(SearchArgs a, Lds
             KeyItem* A = reinterpret_cast<KeyItem*>(uni);
             for (int c = tid; c < nc; c += kBlock) A[c] = KeyItem{score[c], static_cast<uint32_t>(c)};
             __syncthreads();
-            if (tid == 0) introsort_desc(A, nc);
+            uint32_t* leaf_bits = reinterpret_cast<uint32_t*>(flist);  // free until the next list
+            if (wave == 0)
+                introsort_desc_wave(A, nc, reinterpret_cast<uint16_t*>(A + P.mc),
+                                    reinterpret_cast<uint16_t*>(A + P.mc) + P.mc, leaf_bits, lane);
             __syncthreads();
-            for (int c = tid; c < nc; c += kBlock) order[c] = static_cast<uint16_t>(A[c].id);
+            introsort_final_pass(A, nc, leaf_bits, order, tid, kBlock);
         } else {
             for (int c = tid; c < nc; c += kBlock) order[c] = static_cast<uint16_t>(c);
         }
@@ -468,6 +696,9 @@ __global__ __launch_bounds__(BLOCK) void seismic_search_kernel(SearchArgs a, Lds
                     s.slot = excl;
                     s.test = first_piece ? 1 : 0;
                     s.pos = p;
+                    s.n_ok = 0;
+                    s.nnz = 0;
+                    s.pad = 0;
                     seg[si] = s;
                 }
                 // consumed non-first-list failures were prune tests too
@@ -513,14 +744,22 @@ __global__ __launch_bounds__(BLOCK) void seismic_search_kernel(SearchArgs a, Lds
                 const int32_t d = ix.docs[pp];
                 st_id[j] = d;
                 const bool keep = sel == nullptr || ((sel[d >> 5] >> (d & 31)) & 1u);
+                int32_t len = -1;
                 if (keep && blk) {
                     const int64_t ref = ix.row_ref[pp];
                     st_rs[j] = static_cast<int32_t>(((ref >> 16) - lwb) << 3);
-                    st_len[j] = static_cast<int32_t>(ref & 0xFFFF);
+                    len = static_cast<int32_t>(ref & 0xFFFF);
                 } else if (keep) {
                     const int32_t r0 = ix.indptr[d];
                     st_rs[j] = r0;
-                    st_len[j] = ix.indptr[d + 1] - r0;
+                    len = ix.indptr[d + 1] - r0;
+                }
+                if (a.stats != nullptr && keep) {
+                    atomicAdd(&seg[lo].n_ok, 1);
+                    atomicAdd(&seg[lo].nnz, len);
+                }
+                if (keep) {
+                    st_len[j] = len;
                 } else {
                     st_len[j] = -1;
                 }
@@ -671,11 +910,40 @@ __global__ __launch_bounds__(BLOCK) void seismic_search_kernel(SearchArgs a, Lds
             __syncthreads();
             tick(5);
             // ---- exact replay of the chunk (wave 0) ----
+            // Only docs that beat the k-th score as it stands at the start of the
+            // chunk can ever enter (the k-th only rises); they are compacted in
+            // slot order. Segments are then replayed in visiting order: prune
+            // test against the exact current k-th, then the segment's candidates
+            // are merged into the sorted top-k in batches of 64. Inserting a batch
+            // one by one with the reference's reject rule keeps the k best of
+            // (top-k U batch) under (score desc, doc id asc) -- an order-free
+            // result -- so the batch is merged in parallel: each candidate's rank
+            // among the top-k (binary search; an equal (score, id) is the same
+            // doc seen in an earlier list) plus its rank among the batch.
             if (wave == 0) {
                 int tkn = misc[kMiscTkN];
                 bool done = false;
+                int32_t* cidx = wbase;  // free after scoring
+                int ncand = 0;
+                {
+                    const bool full0 = tkn == k;
+                    const float ks0 = full0 ? tks[k - 1] : 0.0F;
+                    const int32_t ki0 = full0 ? tki[k - 1] : 0;
+                    for (int b = 0; b < total; b += 64) {
+                        const int j = b + lane;
+                        bool c = j < total && st_len[j] >= 0;
+                        if (c && full0) c = better(st_sc[j], st_id[j], ks0, ki0);
+                        const uint64_t m = ballot(c);
+                        if (c) cidx[ncand + __popcll(m & ((1ULL << lane) - 1))] = j;
+                        ncand += __popcll(m);
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                int cp = 0;  // next compacted candidate
                 for (int si = 0; si < nseg && !done; ++si) {
                     const Seg s = seg[si];
+                    const int send = s.slot + s.count;
+                    bool pruned = false;
                     if (s.test) {
                         ++st_tested;
                         if (tkn == k && score[s.cluster] * hf < tks[k - 1]) {
@@ -683,75 +951,85 @@ __global__ __launch_bounds__(BLOCK) void seismic_search_kernel(SearchArgs a, Lds
                                 done = true;
                                 break;
                             }
-                            continue;
+                            pruned = true;
+                        } else {
+                            ++st_opened;
                         }
-                        ++st_opened;
+                    }
+                    if (pruned) {  // skip the segment's candidates
+                        while (cp < ncand) {
+                            const int v = cp + lane < ncand ? cidx[cp + lane] : 0x7FFFFFFF;
+                            const int n = __popcll(ballot(v < send));
+                            cp += n;
+                            if (n < 64) break;
+                        }
+                        continue;
                     }
                     st_read += s.count;
-                    for (int b = 0; b < s.count; b += 64) {
-                        const int j = s.slot + b + lane;
-                        const bool in = b + lane < s.count;
-                        const int len = in ? st_len[j] : -1;
-                        const bool ok = len >= 0;
-                        const float sc = ok ? st_sc[j] : 0.0F;
-                        const int32_t id = ok ? st_id[j] : 0;
-                        st_scored += __popcll(ballot(ok));
-                        {
-                            int64_t nz = ok ? len : 0;
-#pragma unroll
-                            for (int d = 32; d > 0; d >>= 1) nz += __shfl_xor(nz, d, 64);
-                            st_nnz += nz;
+                    st_scored += s.n_ok;
+                    st_nnz += s.nnz;
+                    while (cp < ncand) {
+                        const int ci = cp + lane;
+                        const int j = ci < ncand ? cidx[ci] : 0x7FFFFFFF;
+                        const bool in = j < send;
+                        const int nb = __popcll(ballot(in));
+                        if (nb == 0) break;
+                        cp += nb;
+                        const float sc = in ? st_sc[j] : 0.0F;
+                        const int32_t id = in ? st_id[j] : 0;
+                        const bool fullk = tkn == k;
+                        const float ks = fullk ? tks[k - 1] : 0.0F;
+                        const int32_t ki = fullk ? tki[k - 1] : 0;
+                        const bool cand = in && (!fullk || better(sc, id, ks, ki));
+                        int rt = 0;
+                        if (cand) {
+                            int hi = tkn;
+                            while (rt < hi) {
+                                const int mid = (rt + hi) >> 1;
+                                if (better(tks[mid], tki[mid], sc, id))
+                                    rt = mid + 1;
+                                else
+                                    hi = mid;
+                            }
                         }
-                        bool taken = false;
-                        while (true) {
-                            const bool fullk = tkn == k;
-                            const float ks = fullk ? tks[k - 1] : 0.0F;
-                            const int32_t ki = fullk ? tki[k - 1] : 0;
-                            const bool cand = ok && !taken && (!fullk || better(sc, id, ks, ki));
-                            const uint64_t m = ballot(cand);
-                            if (m == 0) break;
+                        const bool v = cand && !(rt < tkn && tki[rt] == id);
+                        const uint64_t vm = ballot(v);
+                        if (vm == 0) continue;
+                        int rc = 0;
+                        for (uint64_t m = vm; m != 0; m &= m - 1) {
                             const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
-                            const float bs = __shfl(sc, bl, 64);
-                            const int32_t bi = __shfl(id, bl, 64);
-                            if (lane == bl) taken = true;
-                            bool dup = false, pos_cnt = false;
-                            int npos = 0;
-                            for (int r0 = 0; r0 < tkn; r0 += 64) {
-                                const int e = r0 + lane;
-                                const bool live = e < tkn;
-                                const float es = live ? tks[e] : 0.0F;
-                                const int32_t ei = live ? tki[e] : 0;
-                                dup = dup || (live && ei == bi);
-                                pos_cnt = live && better(es, ei, bs, bi);
-                                npos += __popcll(ballot(pos_cnt));
-                            }
-                            if (ballot(dup) != 0) continue;
-                            const int newn = min(tkn + 1, k);
-                            // shift [npos, newn-1) up by one, highest block first
-                            const int nmove = newn - 1 - npos;
-                            for (int r0 = ((nmove - 1) / 64) * 64; r0 >= 0 && nmove > 0; r0 -= 64) {
-                                const int e = npos + r0 + lane;
-                                const bool mv = r0 + lane < nmove;
-                                float ms = 0.0F;
-                                int32_t mi = 0;
-                                if (mv) {
-                                    ms = tks[e];
-                                    mi = tki[e];
-                                }
-                                __builtin_amdgcn_wave_barrier();
-                                if (mv) {
-                                    tks[e + 1] = ms;
-                                    tki[e + 1] = mi;
-                                }
-                                __builtin_amdgcn_wave_barrier();
-                            }
-                            if (lane == 0) {
-                                tks[npos] = bs;
-                                tki[npos] = bi;
+                            const float os = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sc), bl));
+                            const int32_t oi = __builtin_amdgcn_readlane(id, bl);
+                            rc += better(os, oi, sc, id) ? 1 : 0;
+                        }
+                        // top-k entries move up past the candidates better than
+                        // them, highest tile first (each tile is read whole before
+                        // any of it is written)
+                        for (int t0 = ((tkn - 1) >> 6) << 6; t0 >= 0 && tkn > 0; t0 -= 64) {
+                            const int t = t0 + lane;
+                            const bool live = t < tkn;
+                            const float es = live ? tks[t] : 0.0F;
+                            const int32_t ei = live ? tki[t] : 0;
+                            int up = 0;
+                            for (uint64_t m = vm; m != 0; m &= m - 1) {
+                                const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
+                                const float os = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sc), bl));
+                                const int32_t oi = __builtin_amdgcn_readlane(id, bl);
+                                up += better(os, oi, es, ei) ? 1 : 0;
                             }
                             __builtin_amdgcn_wave_barrier();
-                            tkn = newn;
+                            if (live && up > 0 && t + up < k) {
+                                tks[t + up] = es;
+                                tki[t + up] = ei;
+                            }
+                            __builtin_amdgcn_wave_barrier();
                         }
+                        if (v && rt + rc < k) {
+                            tks[rt + rc] = sc;
+                            tki[rt + rc] = id;
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        tkn = min(k, tkn + __popcll(vm));
                     }
                 }
                 if (lane == 0) {
@@ -807,7 +1085,7 @@ LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq, bool dense) {
     P.mc = max(max_clusters, 1);
     P.kmax = kmax;
     const int staging = kBudget * 16;
-    const int sortbuf = P.mc * 8;
+    const int sortbuf = P.mc * 12;  // KeyItem[mc] + two uint16[mc] position lists
     const int tile = sq ? 0 : kTileFloats * 4;
     P.uni_bytes = max(max(staging, sortbuf), max(tile, kQMax * 8));
     int o = 0;
