@@ -119,8 +119,8 @@ int nsparse_search(nsparse_index* index, nsparse_idx_t n, const nsparse_idx_t* i
  * asks for ks[i] results (ks may be NULL: all k_max), written to row i of an
  * n x k_max output and padded with (-1.0f, -1). selector_of_query[i] (may be NULL)
  * indexes bitmaps[] (device pointers, NULL entry = no filter) or is -1. Runs as one
- * persistent launch on `stream` (hipStream_t, NULL = default) and does not
- * synchronize. cut/heap_factor/vmin/vmax come from params (selector field ignored). */
+ * launch (one workgroup per query) on `stream` (hipStream_t, NULL = default) and does
+ * not synchronize. cut/heap_factor/vmin/vmax come from params (selector field ignored). */
 int nsparse_search_device(nsparse_index* index, int64_t n, const int32_t* d_indptr,
                           const uint16_t* d_indices, const float* d_values, const int32_t* d_ks,
                           int k_max, const nsparse_search_params* params,
@@ -139,6 +139,39 @@ int nsparse_search_device_stats(nsparse_index* index, int64_t n, const int32_t* 
                                 const uint16_t* d_indices, const float* d_values, const int32_t* d_ks,
                                 int k_max, const nsparse_search_params* params, float* d_distances,
                                 int32_t* d_labels, int64_t* d_stats, void* stream);
+
+/* Schedule of a batched search (uint8 indexes; FP32 always runs lookup 0 on
+ * 256-lane workgroups). Every choice computes identical results. Extension. */
+typedef struct nsparse_search_config {
+    int32_t lookup;        /* 1: dense uint8 code table in LDS, 0: bitmap + rank table */
+    int32_t block;         /* lanes per query workgroup: 256 or 512 */
+    int32_t persistent;    /* 1: persistent workgroups pulling queries from a device queue */
+    int32_t longest_first; /* persistent only: queries handed out by descending non-zeros */
+} nsparse_search_config;
+
+typedef struct nsparse_search_plan nsparse_search_plan;
+
+/* The batched search of nsparse_search_device over fixed DEVICE buffers, captured
+ * once in a hipGraph and replayed by nsparse_search_plan_launch. config NULL: the
+ * autotuner times every valid schedule on this batch (on `stream`, synchronizing
+ * it; the outputs are written meanwhile) and keeps the fastest. Between launches
+ * the caller may rewrite the contents of the query and ks buffers (same sizes).
+ * Free the plan before those buffers or the index. Extension. */
+int nsparse_search_plan_create(nsparse_index* index, int64_t n, const int32_t* d_indptr,
+                               const uint16_t* d_indices, const float* d_values, const int32_t* d_ks,
+                               int k_max, const nsparse_search_params* params,
+                               const int32_t* d_selector_of_query, const uint32_t* const* d_bitmaps,
+                               int n_bitmaps, float* d_distances, int32_t* d_labels,
+                               const nsparse_search_config* config, void* stream, nsparse_search_plan** out);
+/* Replays the plan on `stream`; does not synchronize. */
+int nsparse_search_plan_launch(nsparse_search_plan* plan, void* stream);
+/* The schedule in use (*out), and the autotuner's timing of each candidate it
+ * tried: up to max_candidates entries of candidates[] / ms[] (either may be NULL);
+ * *n_candidates receives how many were timed (0 when the config was given). */
+int nsparse_search_plan_config(const nsparse_search_plan* plan, nsparse_search_config* out,
+                               int max_candidates, nsparse_search_config* candidates, float* ms,
+                               int* n_candidates);
+int nsparse_search_plan_free(nsparse_search_plan* plan);
 
 /* Error code (0 = none) a search kernel left for the last nsparse_search_device*
  * call on this index: 1 query too long, 2 term id >= dim, 4 per-query k out of range.

@@ -157,31 +157,105 @@ int nsparse_search(nsparse_index* index, nsparse_idx_t n, const nsparse_idx_t* i
     return guarded([&] { unwrap(index).search(n, indptr, indices, values, k, distances, labels, params); });
 }
 
+namespace {
+
+// cut / heap_factor / SQ range of a device search (the selector field is not used)
+SearchRequest device_request(const nsparse_search_params* params) {
+    SearchRequest req;
+    if (params != nullptr) {
+        if (params->kind == NSPARSE_PARAMS_SEISMIC || params->kind == NSPARSE_PARAMS_SEISMIC_SQ) {
+            req.cut = params->cut;
+            req.heap_factor = params->heap_factor;
+        }
+        if (params->kind == NSPARSE_PARAMS_SEISMIC_SQ) {
+            req.has_sq_range = true;
+            req.vmin = params->vmin;
+            req.vmax = params->vmax;
+        }
+    }
+    return req;
+}
+
+SeismicIndex& device_search_index(nsparse_index* index) {
+    if (dynamic_cast<IDMapIndex*>(&unwrap(index)) != nullptr)
+        throw std::invalid_argument("device searches take the SEISMIC index itself, not an IDMap");
+    return seismic_of(index);
+}
+
+}  // namespace
+
 int nsparse_search_device(nsparse_index* index, int64_t n, const int32_t* d_indptr, const uint16_t* d_indices,
                           const float* d_values, const int32_t* d_ks, int k_max, const nsparse_search_params* params,
                           const int32_t* d_selector_of_query, const uint32_t* const* d_bitmaps, int n_bitmaps,
                           float* d_distances, int32_t* d_labels, void* stream) {
     return guarded([&] {
-        if (dynamic_cast<IDMapIndex*>(&unwrap(index)) != nullptr)
-            throw std::invalid_argument("nsparse_search_device takes the SEISMIC index itself, not an IDMap");
-        SeismicIndex& s = seismic_of(index);
-        SearchRequest req;
-        if (params != nullptr) {
-            if (params->kind == NSPARSE_PARAMS_SEISMIC || params->kind == NSPARSE_PARAMS_SEISMIC_SQ) {
-                req.cut = params->cut;
-                req.heap_factor = params->heap_factor;
-            }
-            if (params->kind == NSPARSE_PARAMS_SEISMIC_SQ) {
-                req.has_sq_range = true;
-                req.vmin = params->vmin;
-                req.vmax = params->vmax;
-            }
-        }
+        SeismicIndex& s = device_search_index(index);
         if (d_selector_of_query != nullptr && (d_bitmaps == nullptr || n_bitmaps <= 0))
             throw std::invalid_argument("per-query selectors need a bitmap table");
-        s.search_device(n, d_indptr, d_indices, d_values, d_ks, k_max, req, d_selector_of_query, d_bitmaps,
-                        d_distances, d_labels, nullptr, static_cast<hipStream_t>(stream));
+        s.search_device(n, d_indptr, d_indices, d_values, d_ks, k_max, device_request(params), d_selector_of_query,
+                        d_bitmaps, d_distances, d_labels, nullptr, static_cast<hipStream_t>(stream));
     });
+}
+
+struct nsparse_search_plan {
+    std::unique_ptr<SearchPlan> impl;
+};
+
+int nsparse_search_plan_create(nsparse_index* index, int64_t n, const int32_t* d_indptr, const uint16_t* d_indices,
+                               const float* d_values, const int32_t* d_ks, int k_max,
+                               const nsparse_search_params* params, const int32_t* d_selector_of_query,
+                               const uint32_t* const* d_bitmaps, int n_bitmaps, float* d_distances,
+                               int32_t* d_labels, const nsparse_search_config* config, void* stream,
+                               nsparse_search_plan** out) {
+    return guarded([&] {
+        if (out == nullptr) throw std::invalid_argument("unexpected nullptr");
+        *out = nullptr;
+        SeismicIndex& s = device_search_index(index);
+        if (d_selector_of_query != nullptr && (d_bitmaps == nullptr || n_bitmaps <= 0))
+            throw std::invalid_argument("per-query selectors need a bitmap table");
+        require_device();
+        const SearchArgs a = s.search_args(n, d_indptr, d_indices, d_values, d_ks, k_max, device_request(params),
+                                           d_selector_of_query, d_bitmaps, d_distances, d_labels, nullptr);
+        SearchConfig c;
+        if (config != nullptr) {
+            c.lookup = config->lookup;
+            c.block = config->block;
+            c.persistent = config->persistent;
+            c.longest_first = config->longest_first;
+        }
+        auto plan = std::make_unique<nsparse_search_plan>();
+        plan->impl = std::make_unique<SearchPlan>(s, a, k_max, config != nullptr ? &c : nullptr,
+                                                  static_cast<hipStream_t>(stream));
+        *out = plan.release();
+    });
+}
+
+int nsparse_search_plan_launch(nsparse_search_plan* plan, void* stream) {
+    return guarded([&] {
+        if (plan == nullptr || !plan->impl) throw std::invalid_argument("search plan handle is null");
+        plan->impl->launch(static_cast<hipStream_t>(stream));
+    });
+}
+
+int nsparse_search_plan_config(const nsparse_search_plan* plan, nsparse_search_config* out, int max_candidates,
+                               nsparse_search_config* candidates, float* ms, int* n_candidates) {
+    return guarded([&] {
+        if (plan == nullptr || !plan->impl) throw std::invalid_argument("search plan handle is null");
+        const auto to_c = [](const SearchConfig& c) {
+            return nsparse_search_config{c.lookup, c.block, c.persistent, c.longest_first};
+        };
+        if (out != nullptr) *out = to_c(plan->impl->config());
+        const auto& t = plan->impl->tuning();
+        if (n_candidates != nullptr) *n_candidates = static_cast<int>(t.size());
+        for (int i = 0; i < max_candidates && i < static_cast<int>(t.size()); ++i) {
+            if (candidates != nullptr) candidates[i] = to_c(t[i].first);
+            if (ms != nullptr) ms[i] = t[i].second;
+        }
+    });
+}
+
+int nsparse_search_plan_free(nsparse_search_plan* plan) {
+    return guarded([&] { delete plan; });
 }
 
 // Extension used by tests and the benchmark: same as nsparse_search_device plus
@@ -192,18 +266,8 @@ int nsparse_search_device_stats(nsparse_index* index, int64_t n, const int32_t* 
                                 int64_t* d_stats, void* stream) {
     return guarded([&] {
         SeismicIndex& s = seismic_of(index);
-        SearchRequest req;
-        if (params != nullptr && (params->kind == NSPARSE_PARAMS_SEISMIC || params->kind == NSPARSE_PARAMS_SEISMIC_SQ)) {
-            req.cut = params->cut;
-            req.heap_factor = params->heap_factor;
-        }
-        if (params != nullptr && params->kind == NSPARSE_PARAMS_SEISMIC_SQ) {
-            req.has_sq_range = true;
-            req.vmin = params->vmin;
-            req.vmax = params->vmax;
-        }
-        s.search_device(n, d_indptr, d_indices, d_values, d_ks, k_max, req, nullptr, nullptr, d_distances, d_labels,
-                        d_stats, static_cast<hipStream_t>(stream));
+        s.search_device(n, d_indptr, d_indices, d_values, d_ks, k_max, device_request(params), nullptr, nullptr,
+                        d_distances, d_labels, d_stats, static_cast<hipStream_t>(stream));
     });
 }
 

@@ -90,7 +90,18 @@ struct SearchArgs {
     int64_t* stats;            // device, n x kStatFields, or nullptr
     int32_t* error_flag;       // device; set to a nonzero code on a rejected query
     int64_t n_queries;
-    int64_t query_base;        // first query handled by this launch
+    int64_t query_base;        // first query handled by this launch (one-workgroup-per-query kernel)
+    int32_t* queue;            // persistent kernel: device counter, zero at launch
+    const int32_t* order;      // persistent kernel: query of each counter value, or nullptr (identity)
+};
+
+// How the batched search runs (uint8 indexes; FP32 always uses the bitmap
+// lookup on 256-lane workgroups). Every choice computes identical results.
+struct SearchConfig {
+    int32_t lookup = 1;         // 1: dense uint8 code table in LDS, 0: bitmap + rank table
+    int32_t block = 512;        // lanes per query workgroup: 256 or 512
+    int32_t persistent = 0;     // 1: persistent workgroups pulling queries from a device queue
+    int32_t longest_first = 0;  // persistent only: hand out queries by descending non-zeros
 };
 
 // Error codes a kernel can leave in SearchArgs::error_flag.

@@ -5,6 +5,7 @@
 #include "index.h"
 
 #include <algorithm>
+#include <functional>
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
@@ -494,10 +495,20 @@ void SeismicIndex::search_device(int64_t n, const int32_t* d_indptr, const uint1
                                  const float* d_values, const int32_t* d_ks, int k_max, const SearchRequest& req,
                                  const int32_t* d_sel_of_query, const uint32_t* const* d_bitmaps, float* d_out_d,
                                  int32_t* d_out_l, int64_t* d_stats, hipStream_t stream) {
+    if (!has_vectors_ && n > 0 && k_max > 0) return;
+    const SearchArgs a = search_args(n, d_indptr, d_indices, d_values, d_ks, k_max, req, d_sel_of_query, d_bitmaps,
+                                     d_out_d, d_out_l, d_stats);
+    launch_search_checked(a, max_clusters_, k_max, sq_, stream);
+}
+
+SearchArgs SeismicIndex::search_args(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices,
+                                     const float* d_values, const int32_t* d_ks, int k_max, const SearchRequest& req,
+                                     const int32_t* d_sel_of_query, const uint32_t* const* d_bitmaps,
+                                     float* d_out_d, int32_t* d_out_l, int64_t* d_stats) {
     if (n <= 0) throw std::invalid_argument("value must be positive");
     if (k_max <= 0) throw std::invalid_argument("value must be positive");
     throw_if_any_null(d_indptr, d_indices, d_values, d_out_d, d_out_l);
-    if (!has_vectors_) return;
+    if (!has_vectors_) throw std::invalid_argument("the index holds no vectors");
     if (req.cut <= 0) throw std::invalid_argument("value must be positive");
     if (req.has_sq_range && sq_) check_quantizer_range(req.vmin, req.vmax);
     require_device();
@@ -529,8 +540,95 @@ void SeismicIndex::search_device(int64_t n, const int32_t* d_indptr, const uint1
     a.error_flag = d_err_.get<int32_t>();
     a.n_queries = n;
     a.query_base = 0;
-    launch_search_checked(a, max_clusters_, k_max, sq_, stream);
+    return a;
 }
+
+// ---- SearchPlan
+
+namespace {
+
+float time_launch(const std::function<void()>& launch, hipStream_t stream, int reps) {
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    float best = std::numeric_limits<float>::infinity();
+    for (int r = 0; r < reps; ++r) {
+        hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+        launch();
+        hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.0F;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        best = std::min(best, ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
+}  // namespace
+
+SearchPlan::SearchPlan(SeismicIndex& index, const SearchArgs& args, int k_max, const SearchConfig* config,
+                       hipStream_t stream)
+    : args_(args), k_max_(k_max), max_clusters_(index.max_clusters()), sq_(index.sq()) {
+    scratch_.allocate(search_scratch_bytes(args.n_queries));
+    const int dim = args.ix.dim;
+    if (config != nullptr) {
+        config_ = resolve_search_config(dim, max_clusters_, k_max_, sq_, *config);
+    } else {
+        // every candidate schedule on this very batch: one warm-up, best of two
+        std::vector<SearchConfig> cands;
+        for (int lookup : {1, 0})
+            for (int block : {512, 256})
+                for (int sched : {0, 1, 2}) {
+                    SearchConfig c;
+                    c.lookup = lookup;
+                    c.block = block;
+                    c.persistent = sched > 0 ? 1 : 0;
+                    c.longest_first = sched == 2 ? 1 : 0;
+                    c = resolve_search_config(dim, max_clusters_, k_max_, sq_, c);
+                    bool seen = false;
+                    for (const auto& o : cands)
+                        seen = seen || (o.lookup == c.lookup && o.block == c.block && o.persistent == c.persistent &&
+                                        o.longest_first == c.longest_first);
+                    if (!seen) cands.push_back(c);
+                }
+        float best = std::numeric_limits<float>::infinity();
+        for (const auto& c : cands) {
+            auto go = [&] { launch_search(args_, max_clusters_, k_max_, sq_, c, scratch_.get(), stream); };
+            go();
+            const float ms = time_launch(go, stream, 2);
+            tuning_.push_back({c, ms});
+            if (ms < best) {
+                best = ms;
+                config_ = c;
+            }
+        }
+    }
+    // capture the launch sequence (queue reset, optional ordering, search)
+    hipStream_t cs = nullptr;
+    hip_check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
+    try {
+        hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+        launch_search(args_, max_clusters_, k_max_, sq_, config_, scratch_.get(), cs);
+        hip_check(hipStreamEndCapture(cs, &graph_), "hipStreamEndCapture");
+        hip_check(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(cs, &g);
+        if (g != nullptr) (void)hipGraphDestroy(g);
+        (void)hipStreamDestroy(cs);
+        throw;
+    }
+    (void)hipStreamDestroy(cs);
+}
+
+SearchPlan::~SearchPlan() {
+    if (exec_ != nullptr) (void)hipGraphExecDestroy(exec_);
+    if (graph_ != nullptr) (void)hipGraphDestroy(graph_);
+}
+
+void SearchPlan::launch(hipStream_t stream) { hip_check(hipGraphLaunch(exec_, stream), "hipGraphLaunch"); }
 
 int32_t SeismicIndex::last_device_error() {
     if (!d_err_.bytes()) return 0;

@@ -99,6 +99,10 @@ public:
                        const int32_t* d_ks, int k_max, const SearchRequest& req, const int32_t* d_sel_of_query,
                        const uint32_t* const* d_bitmaps, float* d_out_d, int32_t* d_out_l, int64_t* d_stats,
                        hipStream_t stream);
+    // The kernel arguments of a batched search (validated), as search_device uses them.
+    SearchArgs search_args(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices, const float* d_values,
+                           const int32_t* d_ks, int k_max, const SearchRequest& req, const int32_t* d_sel_of_query,
+                           const uint32_t* const* d_bitmaps, float* d_out_d, int32_t* d_out_l, int64_t* d_stats);
     // The search proper over host queries (IDMap passes its mapped selector).
     void search_host(idx_t n, const idx_t* indptr, const term_t* indices, const float* values, int k,
                      float* distances, idx_t* labels, const SearchRequest& req, int64_t* stats_out);
@@ -138,6 +142,33 @@ private:
     DeviceBuffer d_row_ref_, d_blk_indices_, d_blk_codes_;  // block-contiguous row copies (uint8)
     DeviceBuffer d_err_;
     DevIndexView view_{};
+};
+
+// A batched search over fixed device buffers, replayed from a hipGraph: the
+// schedule (lookup table, workgroup width, persistent queue, longest-first
+// order) is given or picked by timing every candidate on the batch itself;
+// each launch replays the captured kernels, so the caller may rewrite the query
+// buffers (same sizes) between launches. Results equal search_device's.
+class SearchPlan {
+public:
+    SearchPlan(SeismicIndex& index, const SearchArgs& args, int k_max, const SearchConfig* config, hipStream_t stream);
+    ~SearchPlan();
+    SearchPlan(const SearchPlan&) = delete;
+    SearchPlan& operator=(const SearchPlan&) = delete;
+    void launch(hipStream_t stream);
+    const SearchConfig& config() const { return config_; }
+    const std::vector<std::pair<SearchConfig, float>>& tuning() const { return tuning_; }
+
+private:
+    SearchArgs args_;
+    int k_max_;
+    int max_clusters_;
+    bool sq_;
+    SearchConfig config_{};
+    std::vector<std::pair<SearchConfig, float>> tuning_;  // candidate -> best ms
+    DeviceBuffer scratch_;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
 };
 
 // IDMapIndex (id_map_index.h:78-109, id_map_index.cpp).

@@ -52,6 +52,16 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
 bool build_block_rows(const DevIndexView& ix, int64_t n_postings, DeviceBuffer& row_ref, DeviceBuffer& blk_indices,
                       DeviceBuffer& blk_codes);
 
+// The default (NSPARSE_SEARCH_LOOKUP / NSPARSE_SEARCH_BLOCK override it).
+SearchConfig default_search_config(bool sq);
+// `c` made valid for this index: the dense table is dropped when it does not fit in LDS.
+SearchConfig resolve_search_config(int dim, int max_clusters, int kmax, bool sq, SearchConfig c);
+// Device scratch a persistent launch over n queries needs (queue counter, order).
+size_t search_scratch_bytes(int64_t n);
+// Launches the search of a.n_queries queries with `cfg`; only plain kernels are
+// enqueued on `stream`, so the sequence can be captured in a hipGraph.
+void launch_search(const SearchArgs& a, int max_clusters, int kmax, bool sq, const SearchConfig& cfg, void* scratch,
+                   hipStream_t stream);
 void launch_search_checked(const SearchArgs& a, int max_clusters, int kmax, bool sq, hipStream_t stream);
 void launch_exact_checked(const ExactArgs& a, bool sq, hipStream_t stream);
 int search_lds_bytes(int dim, int max_clusters, int kmax, bool sq);

@@ -89,7 +89,7 @@ struct LdsPlan {
     int32_t kmax;      // top-k capacity
     int32_t uni_bytes; // union scratch (staging / sort buffer / FP32 tile)
     int32_t o_qbits, o_qpref, o_eterm, o_erank, o_eval, o_rlast, o_rval, o_rsum, o_rcol, o_flist, o_cuts;
-    int32_t o_cstart, o_score, o_order, o_uni, o_tks, o_tki, o_seg, o_misc, o_wsum, o_wbase, o_winfirst;
+    int32_t o_cstart, o_score, o_order, o_uni, o_tks, o_tki, o_seg, o_misc, o_wsum, o_wbase, o_winfirst, o_queue;
     int32_t total;
 };
 
@@ -277,17 +277,14 @@ __device__ void introsort_final_pass(const KeyItem* A, int n, const uint32_t* le
 // does one byte read per element instead of bitmap + rank + value reads; it
 // costs dim bytes of LDS per workgroup.
 template <bool SQ, bool DENSE, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void seismic_search_kernel(SearchArgs a, LdsPlan P) {
+__device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan& P, int64_t q, unsigned char* smem) {
     static_assert(SQ || !DENSE, "the dense code table holds uint8 codes");
     static_assert(BLOCK % 64 == 0 && BLOCK <= 1024, "whole wavefronts, at most 16");
     constexpr int kBlock = BLOCK;
     constexpr int kWaves = BLOCK / 64;
-    extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int64_t q = a.query_base + blockIdx.x;
-    if (q >= a.n_queries) return;
 
     uint32_t* qbits = reinterpret_cast<uint32_t*>(smem + P.o_qbits);
     uint16_t* qpref = reinterpret_cast<uint16_t*>(smem + P.o_qpref);
@@ -1077,6 +1074,35 @@ __global__ __launch_bounds__(BLOCK) void seismic_search_kernel(SearchArgs a, Lds
     }
 }
 
+// One workgroup per query: query query_base + blockIdx.x.
+template <bool SQ, bool DENSE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void seismic_search_kernel(SearchArgs a, LdsPlan P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t q = a.query_base + blockIdx.x;
+    if (q >= a.n_queries) return;
+    search_query<SQ, DENSE, BLOCK>(a, P, q, smem);
+}
+
+// Persistent workgroups (as many as fit on the device at once): each takes the
+// next query index from the device counter a.queue (zero at launch) until all
+// n_queries are handed out; a.order, when set, maps the index to a query (for
+// longest-first scheduling). Every workgroup exits once the counter passes n.
+template <bool SQ, bool DENSE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void seismic_search_persistent_kernel(SearchArgs a, LdsPlan P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    int32_t* slot = reinterpret_cast<int32_t*>(smem + P.o_queue);
+    while (true) {
+        if (threadIdx.x == 0) *slot = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int64_t i = *slot;
+        __syncthreads();
+        if (i >= a.n_queries) break;
+        const int64_t q = a.order != nullptr ? a.order[i] : i;
+        search_query<SQ, DENSE, BLOCK>(a, P, q, smem);
+        __syncthreads();
+    }
+}
+
 namespace {
 
 LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq, bool dense) {
@@ -1117,6 +1143,7 @@ LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq, bool dense) {
     P.o_wsum = take(kMaxWaves * 4);
     P.o_wbase = take((kBudget + 128) * 4);
     P.o_winfirst = take(kWinMax * 2);
+    P.o_queue = take(16);
     P.total = o;
     return P;
 }
@@ -1288,29 +1315,9 @@ void launch_exact_scores(const int32_t* indptr, const uint16_t* indices, const f
     hip_check(hipGetLastError(), "exact scores launch");
 }
 
-namespace {
-
-// Doc-row scoring lookup for uint8 indexes: the dense code table (default: one
-// LDS byte read per element; dim more bytes of LDS, so two 512-lane queries per
-// CU) or the bitmap + rank table (NSPARSE_SEARCH_LOOKUP=bitmap: about 40 KB of
-// LDS for a 30522-term vocabulary, four 256-lane queries per CU, but ~4x the
-// VALU work per element). The dense table is dropped when it would not fit.
-LdsPlan choose_plan(int dim, int max_clusters, int kmax, bool sq, bool& dense) {
-    const char* e = std::getenv("NSPARSE_SEARCH_LOOKUP");
-    dense = sq && !(e != nullptr && std::string(e) == "bitmap");
-    LdsPlan P = make_plan(dim, max_clusters, kmax, sq, dense);
-    if (dense && P.total > 160 * 1024) {
-        dense = false;
-        P = make_plan(dim, max_clusters, kmax, sq, false);
-    }
-    return P;
-}
-
-}  // namespace
-
 int search_lds_bytes(int dim, int max_clusters, int kmax, bool sq) {
-    bool dense = false;
-    return choose_plan(dim, max_clusters, kmax, sq, dense).total;
+    const SearchConfig c = resolve_search_config(dim, max_clusters, kmax, sq, default_search_config(sq));
+    return make_plan(dim, max_clusters, kmax, sq, c.lookup == 1).total;
 }
 
 std::string device_error_message(int32_t code) {
@@ -1323,42 +1330,155 @@ std::string device_error_message(int32_t code) {
     }
 }
 
-void launch_search_checked(const SearchArgs& args, int max_clusters, int kmax, bool sq, hipStream_t stream) {
-    bool dense = false;
-    const LdsPlan P = choose_plan(args.ix.dim, max_clusters, kmax, sq, dense);
-    // uint8: 512-lane workgroups with the dense table, 256 with the bitmap;
-    // NSPARSE_SEARCH_BLOCK=256|512 overrides
-    const char* eb = std::getenv("NSPARSE_SEARCH_BLOCK");
-    bool wide = sq && dense;
-    if (sq && eb != nullptr) wide = std::string(eb) == "512";
-    if (P.total > 160 * 1024)
-        throw std::invalid_argument("search does not fit in LDS for this dimension / cluster count / k");
+namespace {
+
+// ---- longest-first order for the persistent schedule (all plain kernels, so a
+// launch sequence can be captured in a graph: no memsets, no library calls)
+constexpr int kOrderBuckets = kQMax + 1;
+
+__global__ void zero_i32_kernel(int32_t* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// bucket b = kQMax - min(nnz, kQMax): bucket 0 holds the longest queries
+__global__ void nnz_bucket_kernel(const int32_t* __restrict__ q_indptr, int64_t n, int32_t* __restrict__ hist) {
+    for (int64_t q = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; q < n;
+         q += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int len = static_cast<int>(min<int64_t>(q_indptr[q + 1] - q_indptr[q], kQMax));
+        atomicAdd(&hist[kQMax - max(len, 0)], 1);
+    }
+}
+
+// exclusive scan of the bucket counts, in place (one wavefront)
+__global__ void bucket_scan_kernel(int32_t* hist, int nb) {
+    const int lane = threadIdx.x;
+    int run = 0;
+    for (int b0 = 0; b0 < nb; b0 += 64) {
+        const int b = b0 + lane;
+        const int v = b < nb ? hist[b] : 0;
+        const int incl = wave_incl_scan(v, lane);
+        if (b < nb) hist[b] = run + incl - v;
+        run += __shfl(incl, 63, 64);
+    }
+}
+
+__global__ void order_scatter_kernel(const int32_t* __restrict__ q_indptr, int64_t n, int32_t* __restrict__ offs,
+                                     int32_t* __restrict__ order) {
+    for (int64_t q = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; q < n;
+         q += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int len = static_cast<int>(min<int64_t>(q_indptr[q + 1] - q_indptr[q], kQMax));
+        order[atomicAdd(&offs[kQMax - max(len, 0)], 1)] = static_cast<int32_t>(q);
+    }
+}
+
+int device_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    }
+    return cus;
+}
+
+template <bool SQ, bool DENSE, int BLOCK>
+void launch_variant(const SearchArgs& args, const LdsPlan& P, bool persistent, hipStream_t stream) {
+    const void* kfn = persistent ? reinterpret_cast<const void*>(seismic_search_persistent_kernel<SQ, DENSE, BLOCK>)
+                                 : reinterpret_cast<const void*>(seismic_search_kernel<SQ, DENSE, BLOCK>);
+    if (P.total > 64 * 1024)
+        hip_check(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, P.total),
+                  "hipFuncSetAttribute");
+    if (persistent) {
+        int per_cu = 0;
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, BLOCK, P.total), "occupancy");
+        const int64_t want = static_cast<int64_t>(std::max(per_cu, 1)) * device_cus();
+        const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(want, args.n_queries)));
+        hipLaunchKernelGGL((seismic_search_persistent_kernel<SQ, DENSE, BLOCK>), dim3(grid), dim3(BLOCK), P.total,
+                           stream, args, P);
+        hip_check(hipGetLastError(), "search kernel launch");
+        return;
+    }
     constexpr int64_t kMaxGrid = 1 << 30;
     for (int64_t base = 0; base < args.n_queries; base += kMaxGrid) {
         SearchArgs a = args;
         a.query_base = base;
         const int64_t nb = args.n_queries - base < kMaxGrid ? args.n_queries - base : kMaxGrid;
-        const dim3 grid(static_cast<unsigned>(nb));
-        const void* kfn = !sq          ? reinterpret_cast<const void*>(seismic_search_kernel<false, false, 256>)
-                          : dense && wide ? reinterpret_cast<const void*>(seismic_search_kernel<true, true, 512>)
-                          : dense         ? reinterpret_cast<const void*>(seismic_search_kernel<true, true, 256>)
-                          : wide          ? reinterpret_cast<const void*>(seismic_search_kernel<true, false, 512>)
-                                          : reinterpret_cast<const void*>(seismic_search_kernel<true, false, 256>);
-        if (P.total > 64 * 1024)
-            hip_check(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, P.total),
-                      "hipFuncSetAttribute");
-        if (!sq)
-            hipLaunchKernelGGL((seismic_search_kernel<false, false, 256>), grid, dim3(256), P.total, stream, a, P);
-        else if (dense && wide)
-            hipLaunchKernelGGL((seismic_search_kernel<true, true, 512>), grid, dim3(512), P.total, stream, a, P);
-        else if (dense)
-            hipLaunchKernelGGL((seismic_search_kernel<true, true, 256>), grid, dim3(256), P.total, stream, a, P);
-        else if (wide)
-            hipLaunchKernelGGL((seismic_search_kernel<true, false, 512>), grid, dim3(512), P.total, stream, a, P);
-        else
-            hipLaunchKernelGGL((seismic_search_kernel<true, false, 256>), grid, dim3(256), P.total, stream, a, P);
+        hipLaunchKernelGGL((seismic_search_kernel<SQ, DENSE, BLOCK>), dim3(static_cast<unsigned>(nb)), dim3(BLOCK),
+                           P.total, stream, a, P);
         hip_check(hipGetLastError(), "search kernel launch");
     }
+}
+
+}  // namespace
+
+SearchConfig default_search_config(bool sq) {
+    SearchConfig c{};
+    c.lookup = sq ? 1 : 0;
+    const char* e = std::getenv("NSPARSE_SEARCH_LOOKUP");
+    if (sq && e != nullptr && std::string(e) == "bitmap") c.lookup = 0;
+    c.block = sq && c.lookup == 1 ? 512 : 256;
+    const char* eb = std::getenv("NSPARSE_SEARCH_BLOCK");
+    if (sq && eb != nullptr) c.block = std::string(eb) == "512" ? 512 : 256;
+    c.persistent = 0;
+    c.longest_first = 0;
+    return c;
+}
+
+SearchConfig resolve_search_config(int dim, int max_clusters, int kmax, bool sq, SearchConfig c) {
+    if (!sq) {
+        c.lookup = 0;
+        c.block = 256;
+    }
+    if (c.block != 256 && c.block != 512) throw std::invalid_argument("search workgroup size must be 256 or 512");
+    if (c.lookup == 1 && make_plan(dim, max_clusters, kmax, sq, true).total > 160 * 1024) c.lookup = 0;
+    if (make_plan(dim, max_clusters, kmax, sq, c.lookup == 1).total > 160 * 1024)
+        throw std::invalid_argument("search does not fit in LDS for this dimension / cluster count / k");
+    if (!c.persistent) c.longest_first = 0;
+    return c;
+}
+
+size_t search_scratch_bytes(int64_t n) { return 16 + 4 * static_cast<size_t>((kOrderBuckets + 15) & ~15) + 4 * n; }
+
+void launch_search(const SearchArgs& args_in, int max_clusters, int kmax, bool sq, const SearchConfig& cfg_in,
+                   void* scratch, hipStream_t stream) {
+    const SearchConfig cfg = resolve_search_config(args_in.ix.dim, max_clusters, kmax, sq, cfg_in);
+    const LdsPlan P = make_plan(args_in.ix.dim, max_clusters, kmax, sq, cfg.lookup == 1);
+    SearchArgs a = args_in;
+    a.queue = nullptr;
+    a.order = nullptr;
+    if (cfg.persistent) {
+        if (scratch == nullptr) throw std::invalid_argument("the persistent schedule needs a scratch buffer");
+        int32_t* base = static_cast<int32_t*>(scratch);
+        int32_t* hist = base + 4;
+        int32_t* order = hist + ((kOrderBuckets + 15) & ~15);
+        a.queue = base;
+        const int nz = cfg.longest_first ? 4 + kOrderBuckets : 1;
+        hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(256), 0, stream, base, nz);
+        if (cfg.longest_first) {
+            const unsigned g = static_cast<unsigned>(std::min<int64_t>((a.n_queries + 255) / 256, 4096));
+            hipLaunchKernelGGL(nnz_bucket_kernel, dim3(g), dim3(256), 0, stream, a.q_indptr, a.n_queries, hist);
+            hipLaunchKernelGGL(bucket_scan_kernel, dim3(1), dim3(64), 0, stream, hist, kOrderBuckets);
+            hipLaunchKernelGGL(order_scatter_kernel, dim3(g), dim3(256), 0, stream, a.q_indptr, a.n_queries, hist,
+                               order);
+            a.order = order;
+        }
+        hip_check(hipGetLastError(), "search queue launch");
+    }
+    const bool pers = cfg.persistent != 0;
+    if (!sq)
+        launch_variant<false, false, 256>(a, P, pers, stream);
+    else if (cfg.lookup == 1 && cfg.block == 512)
+        launch_variant<true, true, 512>(a, P, pers, stream);
+    else if (cfg.lookup == 1)
+        launch_variant<true, true, 256>(a, P, pers, stream);
+    else if (cfg.block == 512)
+        launch_variant<true, false, 512>(a, P, pers, stream);
+    else
+        launch_variant<true, false, 256>(a, P, pers, stream);
+}
+
+void launch_search_checked(const SearchArgs& args, int max_clusters, int kmax, bool sq, hipStream_t stream) {
+    launch_search(args, max_clusters, kmax, sq, default_search_config(sq), nullptr, stream);
 }
 
 void launch_exact_checked(const ExactArgs& a, bool sq, hipStream_t stream) {

@@ -13,6 +13,7 @@ from .index import (  # noqa: F401
     Index,
     NotIDSelector,
     SearchParameters,
+    SearchPlan,
     SeismicSearchParameters,
     SeismicSQSearchParameters,
     SetIDSelector,

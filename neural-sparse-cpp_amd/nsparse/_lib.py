@@ -56,6 +56,18 @@ class SearchParams(ctypes.Structure):
     ]
 
 
+class SearchConfig(ctypes.Structure):
+    _fields_ = [
+        ("lookup", ctypes.c_int32),
+        ("block", ctypes.c_int32),
+        ("persistent", ctypes.c_int32),
+        ("longest_first", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
 class SynthSpec(ctypes.Structure):
     _fields_ = [
         ("seed", ctypes.c_uint64),
@@ -94,6 +106,12 @@ _sig("nsparse_search_device", [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, 
 _sig("nsparse_search_device_stats", [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                      P(SearchParams), c_void_p, c_void_p, c_void_p, c_void_p])
 _sig("nsparse_device_error", [c_void_p, P(ctypes.c_int32)])
+_sig("nsparse_search_plan_create", [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                    P(SearchParams), c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                    P(SearchConfig), c_void_p, P(c_void_p)])
+_sig("nsparse_search_plan_launch", [c_void_p, c_void_p])
+_sig("nsparse_search_plan_config", [c_void_p, P(SearchConfig), c_int, P(SearchConfig), P(c_float), P(c_int)])
+_sig("nsparse_search_plan_free", [c_void_p])
 _sig("nsparse_exact_scores_device", [c_int64, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
                                      c_void_p, c_void_p])
 _sig("nsparse_rescore_device", [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int64, c_void_p,

@@ -215,6 +215,37 @@ class Index:
                                             s))
         return out_d, out_l
 
+    def search_plan(self, indptr, indices, values, k_max, ks=None, params=None, out_d=None, out_l=None,
+                    config=None, stream=None, selector_of_query=None, bitmaps=None):
+        """A SearchPlan: the batched search over these GPU tensors captured in a
+        hipGraph (nsparse_search_plan_create). config None: autotuned on this
+        batch; else a dict with lookup / block / persistent / longest_first."""
+        import torch
+
+        n = indptr.numel() - 1
+        dev = indptr.device
+        if out_d is None:
+            out_d = torch.empty((n, k_max), dtype=torch.float32, device=dev)
+        if out_l is None:
+            out_l = torch.empty((n, k_max), dtype=torch.int32, device=dev)
+        ps = ctypes.byref(params._struct()) if params is not None else None
+        s = ctypes.c_void_p(stream.cuda_stream if stream is not None else 0)
+        ksp = ctypes.c_void_p(ks.data_ptr()) if ks is not None else None
+        sel = ctypes.c_void_p(selector_of_query.data_ptr()) if selector_of_query is not None else None
+        bm = ctypes.c_void_p(bitmaps.data_ptr()) if bitmaps is not None else None
+        nb = int(bitmaps.numel()) if bitmaps is not None else 0
+        cfg = None
+        if config is not None:
+            cfg = ctypes.byref(_lib.SearchConfig(int(config.get("lookup", 1)), int(config.get("block", 512)),
+                                                 int(config.get("persistent", 0)),
+                                                 int(config.get("longest_first", 0))))
+        h = ctypes.c_void_p()
+        check(lib.nsparse_search_plan_create(self._h, n, indptr.data_ptr(), indices.data_ptr(), values.data_ptr(),
+                                             ksp, int(k_max), ps, sel, bm, nb, out_d.data_ptr(), out_l.data_ptr(),
+                                             cfg, s, ctypes.byref(h)))
+        # the plan keeps raw pointers: hold the tensors as long as it lives
+        return SearchPlan(h.value, (indptr, indices, values, ks, out_d, out_l, selector_of_query, bitmaps, self))
+
     def device_error(self):
         v = ctypes.c_int32()
         check(lib.nsparse_device_error(self._h, ctypes.byref(v)))
@@ -233,6 +264,49 @@ class Index:
         v = np.empty(nnz.value * esz.value, dtype=np.uint8)
         check(lib.nsparse_get_vectors(self._h, None, None, None, _ptr(ip), _ptr(ix), _ptr(v)))
         return ip, ix, v
+
+
+class SearchPlan:
+    """A captured batched search (see Index.search_plan)."""
+
+    def __init__(self, handle, keep):
+        self._h = handle
+        self._keep = keep
+        self.out_d, self.out_l = keep[4], keep[5]
+
+    def launch(self, stream=None):
+        s = ctypes.c_void_p(stream.cuda_stream if stream is not None else 0)
+        check(lib.nsparse_search_plan_launch(self._h, s))
+        return self.out_d, self.out_l
+
+    @property
+    def config(self):
+        c = _lib.SearchConfig()
+        n = ctypes.c_int()
+        check(lib.nsparse_search_plan_config(self._h, ctypes.byref(c), 0, None, None, ctypes.byref(n)))
+        return c.as_dict()
+
+    @property
+    def tuning(self):
+        """[(config dict, best ms)] of every schedule the autotuner timed."""
+        n = ctypes.c_int()
+        check(lib.nsparse_search_plan_config(self._h, None, 0, None, None, ctypes.byref(n)))
+        cs = (_lib.SearchConfig * max(n.value, 1))()
+        ms = (ctypes.c_float * max(n.value, 1))()
+        check(lib.nsparse_search_plan_config(self._h, None, n.value, cs, ms, ctypes.byref(n)))
+        return [(cs[i].as_dict(), float(ms[i])) for i in range(n.value)]
+
+    def free(self):
+        if self._h:
+            check(lib.nsparse_search_plan_free(self._h))
+            self._h = None
+            self._keep = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def index_factory(dim, description):
