@@ -136,6 +136,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--sweep", default="", help="comma-separated cut:heap_factor pairs to time and score")
+    ap.add_argument("--schedule", default="auto",
+                    help='"auto" (autotuner) or a JSON search config, e.g. {"lookup":1,"block":512,"persistent":1}')
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (query-sharded replicas are a "
@@ -178,8 +180,19 @@ def main():
     out_l = torch.empty((args.queries, k_max), dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream()
 
+    # one step = one replay of the captured batched search; the schedule (lookup
+    # table, workgroup width, persistent queue, longest-first order) is the
+    # autotuner's pick on this very batch
+    t0 = time.time()
+    plan = index.search_plan(qip, qix, qv, k_max, ks=d_ks, params=params, out_d=out_d, out_l=out_l,
+                             config=None if args.schedule == "auto" else json.loads(args.schedule), stream=stream)
+    torch.cuda.synchronize()
+    tuning = plan.tuning
+    log(f"plan in {time.time() - t0:.1f}s; schedule {plan.config}; tried "
+        + ", ".join(f"{c['lookup']}/{c['block']}/{c['persistent']}/{c['longest_first']}={ms:.1f}ms"
+                    for c, ms in tuning))
     for _ in range(args.warmup):
-        index.search_device(qip, qix, qv, k_max, ks=d_ks, params=params, out_d=out_d, out_l=out_l, stream=stream)
+        plan.launch(stream)
     torch.cuda.synchronize()
     if index.device_error() != 0:
         raise RuntimeError(f"device search error {index.device_error()}")
@@ -189,13 +202,13 @@ def main():
     t0 = time.perf_counter()
     for i in range(args.steps):
         starts[i].record(stream)
-        index.search_device(qip, qix, qv, k_max, ks=d_ks, params=params, out_d=out_d, out_l=out_l, stream=stream)
+        plan.launch(stream)
         ends[i].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kernel_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
     qps = args.queries * args.steps / wall
-    log(f"batched: {qps:,.0f} QPS, {wall / args.steps * 1e3:.2f} ms/step, kernel {kernel_ms:.2f} ms")
+    log(f"batched: {qps:,.0f} QPS, {wall / args.steps * 1e3:.2f} ms/step, graph {kernel_ms:.2f} ms")
     if args.copy_probe:
         # a plain copy of known size, for scaling rocprofv3 FETCH/WRITE counters
         src = torch.empty(256 << 20, dtype=torch.uint8, device=dev)
@@ -318,7 +331,10 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
         "config": {"workload": f"seismic_sq_batched_{args.docs}docs", "docs": args.docs, "queries": args.queries,
                    "k": "10 or 100 per query", "lambda": args.lam, "beta": args.beta, "alpha": args.alpha,
-                   "cut": args.cut, "heap_factor": args.hf, "query_nnz": "log-uniform 5..300"},
+                   "cut": args.cut, "heap_factor": args.hf, "query_nnz": "log-uniform 5..300",
+                   "launch": "hipGraph replay of the autotuned batched search",
+                   "schedule": plan.config,
+                   "autotuner_ms": [dict(c, ms=ms) for c, ms in tuning]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": bytes_launch, "kernel_ms": kernel_ms},

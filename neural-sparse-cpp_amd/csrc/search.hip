@@ -1074,9 +1074,10 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
     }
 }
 
-// One workgroup per query: query query_base + blockIdx.x.
+// One workgroup per query: query query_base + blockIdx.x. Both kernels are held
+// to 128 VGPRs (4 waves per SIMD: two 512-lane or four 256-lane queries per CU).
 template <bool SQ, bool DENSE, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void seismic_search_kernel(SearchArgs a, LdsPlan P) {
+__global__ __launch_bounds__(BLOCK, 4) void seismic_search_kernel(SearchArgs a, LdsPlan P) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int64_t q = a.query_base + blockIdx.x;
     if (q >= a.n_queries) return;
@@ -1088,7 +1089,7 @@ __global__ __launch_bounds__(BLOCK) void seismic_search_kernel(SearchArgs a, Lds
 // n_queries are handed out; a.order, when set, maps the index to a query (for
 // longest-first scheduling). Every workgroup exits once the counter passes n.
 template <bool SQ, bool DENSE, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void seismic_search_persistent_kernel(SearchArgs a, LdsPlan P) {
+__global__ __launch_bounds__(BLOCK, 4) void seismic_search_persistent_kernel(SearchArgs a, LdsPlan P) {
     extern __shared__ __align__(16) unsigned char smem[];
     int32_t* slot = reinterpret_cast<int32_t*>(smem + P.o_queue);
     while (true) {
