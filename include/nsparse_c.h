@@ -173,6 +173,52 @@ int nsparse_search_plan_config(const nsparse_search_plan* plan, nsparse_search_c
                                int* n_candidates);
 int nsparse_search_plan_free(nsparse_search_plan* plan);
 
+/* ---- Serving: continuous batching over a device-resident index (extension) ----
+ * Requests (one query each, own k, optional selector bitmap) are queued from any
+ * thread; a dispatcher packs the oldest pending ones into batches, each run on one
+ * of n_streams default-priority streams as one batched search. A batch is admitted
+ * only while a stream is free and the HBM scratch of all batches in flight
+ * (per request: 6 B per query non-zero + 12 B + 8 B * k_max) stays within
+ * scratch_budget bytes; requests arriving meanwhile wait, then are admitted as
+ * streams and budget free up, so batches overlap on the device. Each query's
+ * results equal nsparse_search_device's for that query alone. */
+typedef struct nsparse_server_config {
+    int32_t max_batch;      /* queries per launch */
+    int32_t n_streams;      /* batches in flight at most (1..16) */
+    int64_t scratch_budget; /* bytes of in-flight request scratch admitted */
+    int32_t k_max;          /* largest k a request may ask for */
+    int32_t cut;
+    float heap_factor;
+} nsparse_server_config;
+
+typedef struct nsparse_server nsparse_server;
+
+int nsparse_server_create(nsparse_index* index, const nsparse_server_config* config, nsparse_server** out);
+/* Host bitmap (bit d of word d/32 set = doc d allowed; at least ceil(n_docs/32)
+ * words) copied into HBM; requests name it by *out_id. Register before submitting. */
+int nsparse_server_add_bitmap(nsparse_server* server, const uint32_t* words, int64_t n_words, int32_t* out_id);
+/* Queues one query (nnz <= 512, 1 <= k <= k_max, bitmap_id -1 = no filter).
+ * arrival_ns < 0 stamps it now; else its arrival time in ns on the server clock
+ * (nsparse_server_now_ns), from which its latency is measured. */
+int nsparse_server_submit(nsparse_server* server, int32_t nnz, const uint16_t* indices, const float* values,
+                          int32_t k, int32_t bitmap_id, int64_t arrival_ns, int64_t* out_ticket);
+/* Blocks until the request is done; writes its k results (either pointer may be
+ * NULL) and its latency (completion - arrival) in microseconds. */
+int nsparse_server_wait(nsparse_server* server, int64_t ticket, float* distances, int32_t* labels,
+                        double* latency_us);
+int nsparse_server_now_ns(nsparse_server* server, int64_t* out);
+/* Replays an arrival trace: query i (CSR rows of indptr/indices/values, ks[i],
+ * bitmap_ids[i] or NULL) is submitted at arrival_s[i] seconds after the call (the
+ * calling thread waits for each arrival time), then every result is collected into
+ * row i of n x k_max distances/labels (padded (-1.0f, -1)) with its latency. */
+int nsparse_server_replay(nsparse_server* server, int64_t n, const double* arrival_s, const int32_t* indptr,
+                          const uint16_t* indices, const float* values, const int32_t* ks,
+                          const int32_t* bitmap_ids, float* distances, int32_t* labels, double* latency_us);
+/* completed, batches, max in-flight scratch bytes, largest batch, admission
+ * rounds stopped by the budget, most batches in flight at once (6 int64). */
+int nsparse_server_stats(nsparse_server* server, int64_t* out6);
+int nsparse_server_free(nsparse_server* server);
+
 /* Error code (0 = none) a search kernel left for the last nsparse_search_device*
  * call on this index: 1 query too long, 2 term id >= dim, 4 per-query k out of range.
  * Reading it synchronizes the device. Extension. */

@@ -18,6 +18,7 @@
 
 #include "capi_util.h"
 #include "index.h"
+#include "server.h"
 #include "io.h"
 #include "kernels.h"
 #include "nsparse_c.h"
@@ -98,6 +99,46 @@ void throw_if_null_all(P*... ptrs) {
 
 }  // namespace
 
+struct nsparse_server {
+    std::unique_ptr<Server> impl;
+};
+
+namespace {
+
+// cut / heap_factor / SQ range of a device search (the selector field is not used)
+SearchRequest device_request(const nsparse_search_params* params) {
+    SearchRequest req;
+    if (params != nullptr) {
+        if (params->kind == NSPARSE_PARAMS_SEISMIC || params->kind == NSPARSE_PARAMS_SEISMIC_SQ) {
+            req.cut = params->cut;
+            req.heap_factor = params->heap_factor;
+        }
+        if (params->kind == NSPARSE_PARAMS_SEISMIC_SQ) {
+            req.has_sq_range = true;
+            req.vmin = params->vmin;
+            req.vmax = params->vmax;
+        }
+    }
+    return req;
+}
+
+SeismicIndex& device_search_index(nsparse_index* index) {
+    if (dynamic_cast<IDMapIndex*>(&unwrap(index)) != nullptr)
+        throw std::invalid_argument("device searches take the SEISMIC index itself, not an IDMap");
+    return seismic_of(index);
+}
+
+}  // namespace
+
+namespace {
+
+Server& server_of(nsparse_server* s) {
+    if (s == nullptr || !s->impl) throw std::invalid_argument("server handle is null");
+    return *s->impl;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* nsparse_last_error(void) { return g_last_error.c_str(); }
@@ -156,33 +197,6 @@ int nsparse_search(nsparse_index* index, nsparse_idx_t n, const nsparse_idx_t* i
                    const nsparse_search_params* params) {
     return guarded([&] { unwrap(index).search(n, indptr, indices, values, k, distances, labels, params); });
 }
-
-namespace {
-
-// cut / heap_factor / SQ range of a device search (the selector field is not used)
-SearchRequest device_request(const nsparse_search_params* params) {
-    SearchRequest req;
-    if (params != nullptr) {
-        if (params->kind == NSPARSE_PARAMS_SEISMIC || params->kind == NSPARSE_PARAMS_SEISMIC_SQ) {
-            req.cut = params->cut;
-            req.heap_factor = params->heap_factor;
-        }
-        if (params->kind == NSPARSE_PARAMS_SEISMIC_SQ) {
-            req.has_sq_range = true;
-            req.vmin = params->vmin;
-            req.vmax = params->vmax;
-        }
-    }
-    return req;
-}
-
-SeismicIndex& device_search_index(nsparse_index* index) {
-    if (dynamic_cast<IDMapIndex*>(&unwrap(index)) != nullptr)
-        throw std::invalid_argument("device searches take the SEISMIC index itself, not an IDMap");
-    return seismic_of(index);
-}
-
-}  // namespace
 
 int nsparse_search_device(nsparse_index* index, int64_t n, const int32_t* d_indptr, const uint16_t* d_indices,
                           const float* d_values, const int32_t* d_ks, int k_max, const nsparse_search_params* params,
@@ -269,6 +283,98 @@ int nsparse_search_device_stats(nsparse_index* index, int64_t n, const int32_t* 
         s.search_device(n, d_indptr, d_indices, d_values, d_ks, k_max, device_request(params), nullptr, nullptr,
                         d_distances, d_labels, d_stats, static_cast<hipStream_t>(stream));
     });
+}
+
+int nsparse_server_create(nsparse_index* index, const nsparse_server_config* config, nsparse_server** out) {
+    return guarded([&] {
+        if (config == nullptr || out == nullptr) throw std::invalid_argument("unexpected nullptr");
+        *out = nullptr;
+        SeismicIndex& s = device_search_index(index);
+        ServerConfig c;
+        c.max_batch = config->max_batch;
+        c.n_streams = config->n_streams;
+        c.scratch_budget = config->scratch_budget;
+        c.k_max = config->k_max;
+        c.req.cut = config->cut;
+        c.req.heap_factor = config->heap_factor;
+        auto h = std::make_unique<nsparse_server>();
+        h->impl = std::make_unique<Server>(s, c);
+        *out = h.release();
+    });
+}
+
+int nsparse_server_add_bitmap(nsparse_server* server, const uint32_t* words, int64_t n_words, int32_t* out_id) {
+    return guarded([&] {
+        if (out_id == nullptr) throw std::invalid_argument("unexpected nullptr");
+        *out_id = server_of(server).add_bitmap(words, n_words);
+    });
+}
+
+int nsparse_server_submit(nsparse_server* server, int32_t nnz, const uint16_t* indices, const float* values,
+                          int32_t k, int32_t bitmap_id, int64_t arrival_ns, int64_t* out_ticket) {
+    return guarded([&] {
+        if (out_ticket == nullptr) throw std::invalid_argument("unexpected nullptr");
+        *out_ticket = server_of(server).submit(nnz, indices, values, k, bitmap_id, arrival_ns);
+    });
+}
+
+int nsparse_server_wait(nsparse_server* server, int64_t ticket, float* distances, int32_t* labels,
+                        double* latency_us) {
+    return guarded([&] {
+        const double us = server_of(server).wait(ticket, distances, labels);
+        if (latency_us != nullptr) *latency_us = us;
+    });
+}
+
+int nsparse_server_now_ns(nsparse_server* server, int64_t* out) {
+    return guarded([&] {
+        if (out == nullptr) throw std::invalid_argument("unexpected nullptr");
+        *out = server_of(server).now_ns();
+    });
+}
+
+int nsparse_server_replay(nsparse_server* server, int64_t n, const double* arrival_s, const int32_t* indptr,
+                          const uint16_t* indices, const float* values, const int32_t* ks,
+                          const int32_t* bitmap_ids, float* distances, int32_t* labels, double* latency_us) {
+    return guarded([&] {
+        Server& srv = server_of(server);
+        if (n <= 0) throw std::invalid_argument("value must be positive");
+        throw_if_null_all(arrival_s, indptr, ks, distances, labels, latency_us);
+        const int32_t k_max = srv.k_max();
+        std::vector<int64_t> tickets(static_cast<size_t>(n));
+        const int64_t t0 = srv.now_ns();
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t at = t0 + static_cast<int64_t>(arrival_s[i] * 1e9);
+            while (srv.now_ns() < at) {
+            }
+            const int32_t a = indptr[i], b = indptr[i + 1];
+            tickets[i] = srv.submit(b - a, indices + a, values + a, ks[i], bitmap_ids ? bitmap_ids[i] : -1, at);
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            float* d = distances + i * k_max;
+            int32_t* l = labels + i * k_max;
+            std::fill(d, d + k_max, -1.0F);
+            std::fill(l, l + k_max, -1);
+            latency_us[i] = srv.wait(tickets[i], d, l);
+        }
+    });
+}
+
+int nsparse_server_stats(nsparse_server* server, int64_t* out6) {
+    return guarded([&] {
+        if (out6 == nullptr) throw std::invalid_argument("unexpected nullptr");
+        const ServerStats st = server_of(server).stats();
+        out6[0] = st.completed;
+        out6[1] = st.batches;
+        out6[2] = st.max_inflight_bytes;
+        out6[3] = st.max_batch_seen;
+        out6[4] = st.deferred_by_budget;
+        out6[5] = st.max_concurrent_batches;
+    });
+}
+
+int nsparse_server_free(nsparse_server* server) {
+    return guarded([&] { delete server; });
 }
 
 int nsparse_device_error(nsparse_index* index, int32_t* out) {

@@ -1,0 +1,263 @@
+// Continuous-batching search server (see server.h).
+#include "server.h"
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "kernels.h"
+
+namespace nsparse_amd {
+
+namespace {
+
+template <class T>
+T* pinned(size_t count) {
+    void* p = nullptr;
+    hip_check(hipHostMalloc(&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault), "hipHostMalloc");
+    return static_cast<T*>(p);
+}
+
+}  // namespace
+
+Server::Server(SeismicIndex& index, const ServerConfig& cfg) : index_(index), cfg_(cfg) {
+    if (cfg_.max_batch <= 0 || cfg_.n_streams <= 0 || cfg_.k_max <= 0 || cfg_.scratch_budget <= 0)
+        throw std::invalid_argument("server sizes must be positive");
+    if (cfg_.n_streams > 16) throw std::invalid_argument("at most 16 server streams");
+    if (cfg_.req.cut <= 0) throw std::invalid_argument("value must be positive");
+    if (request_bytes(kDeviceQueryMax) > cfg_.scratch_budget)
+        throw std::invalid_argument("scratch budget below one full-length request");
+    require_device();
+    (void)index_.view();  // index resident in HBM before the first batch
+    epoch_ = std::chrono::steady_clock::now();
+    slots_.resize(cfg_.n_streams);
+    const size_t B = static_cast<size_t>(cfg_.max_batch), Q = B * kDeviceQueryMax, O = B * cfg_.k_max;
+    for (Slot& s : slots_) {
+        // default priority: a stream of any other priority is never used here
+        hip_check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&s.done, hipEventDisableTiming), "hipEventCreate");
+        s.h_ip = pinned<int32_t>(B + 1);
+        s.h_ks = pinned<int32_t>(B);
+        s.h_sel = pinned<int32_t>(B);
+        s.h_ix = pinned<uint16_t>(Q);
+        s.h_v = pinned<float>(Q);
+        s.h_dist = pinned<float>(O);
+        s.h_lab = pinned<int32_t>(O);
+        s.d_ip.allocate((B + 1) * 4);
+        s.d_ks.allocate(B * 4);
+        s.d_sel.allocate(B * 4);
+        s.d_ix.allocate(Q * 2, 64);
+        s.d_v.allocate(Q * 4);
+        s.d_dist.allocate(O * 4);
+        s.d_lab.allocate(O * 4);
+    }
+    worker_ = std::thread([this] { run(); });
+}
+
+Server::~Server() {
+    stop_ = true;
+    cv_submit_.notify_all();
+    if (worker_.joinable()) worker_.join();
+    for (Slot& s : slots_) {
+        if (s.stream != nullptr) (void)hipStreamSynchronize(s.stream);
+        (void)hipHostFree(s.h_ip);
+        (void)hipHostFree(s.h_ks);
+        (void)hipHostFree(s.h_sel);
+        (void)hipHostFree(s.h_ix);
+        (void)hipHostFree(s.h_v);
+        (void)hipHostFree(s.h_dist);
+        (void)hipHostFree(s.h_lab);
+        if (s.done != nullptr) (void)hipEventDestroy(s.done);
+        if (s.stream != nullptr) (void)hipStreamDestroy(s.stream);
+    }
+}
+
+int64_t Server::now_ns() const {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - epoch_).count();
+}
+
+int64_t Server::request_bytes(int32_t nnz) const {
+    return static_cast<int64_t>(nnz) * 6 + 12 + static_cast<int64_t>(cfg_.k_max) * 8;
+}
+
+int32_t Server::add_bitmap(const uint32_t* words, int64_t n_words) {
+    if (words == nullptr || n_words <= 0) throw std::invalid_argument("empty selector bitmap");
+    const int64_t need = (static_cast<int64_t>(index_.view().n_docs) + 31) / 32;
+    if (n_words < need) throw std::invalid_argument("selector bitmap shorter than the index");
+    std::lock_guard<std::mutex> lk(mu_);
+    if (!pending_.empty() || inflight_bytes_ != 0)
+        throw std::runtime_error("register selector bitmaps before submitting requests");
+    DeviceBuffer b;
+    b.upload(words, static_cast<size_t>(need) * 4);
+    bitmap_ptrs_.push_back(b.get<uint32_t>());
+    bitmaps_.push_back(std::move(b));
+    d_bitmap_table_.upload(bitmap_ptrs_);
+    return static_cast<int32_t>(bitmaps_.size() - 1);
+}
+
+int64_t Server::submit(int32_t nnz, const uint16_t* indices, const float* values, int32_t k, int32_t bitmap_id,
+                       int64_t arrival_ns) {
+    if (nnz < 0 || nnz > kDeviceQueryMax)
+        throw std::invalid_argument("query has more non-zeros than the device path supports (512)");
+    if (nnz > 0 && (indices == nullptr || values == nullptr)) throw std::invalid_argument("unexpected nullptr");
+    if (k <= 0 || k > cfg_.k_max) throw std::invalid_argument("k outside [1, k_max] of the server");
+    const int32_t dim = index_.view().dim;
+    for (int32_t i = 0; i < nnz; ++i)
+        if (indices[i] >= dim) throw std::invalid_argument("query term id out of range");
+    Request r;
+    r.indices.assign(indices, indices + nnz);
+    r.values.assign(values, values + nnz);
+    r.k = k;
+    r.arrival_ns = arrival_ns < 0 ? now_ns() : arrival_ns;
+    std::lock_guard<std::mutex> lk(mu_);
+    if (bitmap_id < -1 || bitmap_id >= static_cast<int32_t>(bitmaps_.size()))
+        throw std::invalid_argument("unknown selector bitmap id");
+    if (!error_.empty()) throw std::runtime_error(error_);
+    r.bitmap_id = bitmap_id;
+    requests_.push_back(std::move(r));
+    const int64_t ticket = first_ticket_ + static_cast<int64_t>(requests_.size()) - 1;
+    pending_.push_back(ticket);
+    cv_submit_.notify_one();
+    return ticket;
+}
+
+double Server::wait(int64_t ticket, float* distances, int32_t* labels) {
+    std::unique_lock<std::mutex> lk(mu_);
+    if (ticket < first_ticket_ || ticket >= first_ticket_ + static_cast<int64_t>(requests_.size()))
+        throw std::invalid_argument("unknown or already collected ticket");
+    cv_done_.wait(lk, [&] { return requests_[ticket - first_ticket_].done_ns >= 0 || !error_.empty(); });
+    Request& r = requests_[ticket - first_ticket_];
+    if (r.done_ns < 0) throw std::runtime_error(error_);
+    if (distances != nullptr) std::copy(r.dist.begin(), r.dist.end(), distances);
+    if (labels != nullptr) std::copy(r.lab.begin(), r.lab.end(), labels);
+    const double us = static_cast<double>(r.done_ns - r.arrival_ns) / 1e3;
+    r.collected = true;
+    r.indices.clear();
+    r.values.clear();
+    while (!requests_.empty() && requests_.front().collected) {
+        requests_.pop_front();
+        ++first_ticket_;
+    }
+    return us;
+}
+
+ServerStats Server::stats() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return stats_;
+}
+
+void Server::run() {
+    try {
+        while (!stop_) {
+            bool progress = reap();
+            progress = admit() || progress;
+            if (!progress) {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_submit_.wait_for(lk, std::chrono::microseconds(20));
+            }
+        }
+    } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(mu_);
+        error_ = std::string("server dispatcher: ") + e.what();
+        cv_done_.notify_all();
+    }
+}
+
+// Finished batches: results out of pinned memory into their requests.
+bool Server::reap() {
+    bool any = false;
+    for (Slot& s : slots_) {
+        if (!s.busy) continue;
+        const hipError_t e = hipEventQuery(s.done);
+        if (e == hipErrorNotReady) continue;
+        hip_check(e, "server batch");
+        const int64_t t_done = now_ns();
+        std::lock_guard<std::mutex> lk(mu_);
+        for (size_t b = 0; b < s.tickets.size(); ++b) {
+            Request& r = requests_[s.tickets[b] - first_ticket_];
+            const size_t row = b * static_cast<size_t>(cfg_.k_max);
+            r.dist.assign(s.h_dist + row, s.h_dist + row + r.k);
+            r.lab.assign(s.h_lab + row, s.h_lab + row + r.k);
+            r.done_ns = t_done;
+        }
+        stats_.completed += static_cast<int64_t>(s.tickets.size());
+        inflight_bytes_ -= s.bytes;
+        s.busy = false;
+        s.tickets.clear();
+        any = true;
+        cv_done_.notify_all();
+    }
+    return any;
+}
+
+// Oldest pending requests into free streams, within the scratch budget.
+bool Server::admit() {
+    bool any = false;
+    for (Slot& s : slots_) {
+        if (s.busy) continue;
+        std::vector<int64_t> batch;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            if (pending_.empty()) return any;
+            int64_t bytes = 0, nnz = 0;
+            while (!pending_.empty() && static_cast<int32_t>(batch.size()) < cfg_.max_batch) {
+                const Request& r = requests_[pending_.front() - first_ticket_];
+                const int64_t need = request_bytes(static_cast<int32_t>(r.indices.size()));
+                if (inflight_bytes_ + bytes + need > cfg_.scratch_budget) {
+                    ++stats_.deferred_by_budget;
+                    break;
+                }
+                // pack into this stream's pinned staging
+                const size_t b = batch.size();
+                s.h_ip[b] = static_cast<int32_t>(nnz);
+                std::memcpy(s.h_ix + nnz, r.indices.data(), r.indices.size() * 2);
+                std::memcpy(s.h_v + nnz, r.values.data(), r.values.size() * 4);
+                s.h_ks[b] = r.k;
+                s.h_sel[b] = r.bitmap_id;
+                nnz += static_cast<int64_t>(r.indices.size());
+                bytes += need;
+                batch.push_back(pending_.front());
+                pending_.pop_front();
+            }
+            if (batch.empty()) return any;
+            s.h_ip[batch.size()] = static_cast<int32_t>(nnz);
+            s.bytes = bytes;
+            inflight_bytes_ += bytes;
+            ++stats_.batches;
+            stats_.max_inflight_bytes = std::max(stats_.max_inflight_bytes, inflight_bytes_);
+            stats_.max_batch_seen = std::max<int64_t>(stats_.max_batch_seen, static_cast<int64_t>(batch.size()));
+            int64_t busy = 1;
+            for (const Slot& o : slots_) busy += o.busy ? 1 : 0;
+            stats_.max_concurrent_batches = std::max(stats_.max_concurrent_batches, busy);
+        }
+        launch(s, batch);
+        any = true;
+    }
+    return any;
+}
+
+void Server::launch(Slot& s, const std::vector<int64_t>& batch) {
+    const int64_t n = static_cast<int64_t>(batch.size());
+    const int64_t nnz = s.h_ip[n];
+    hip_check(hipMemcpyAsync(s.d_ip.get(), s.h_ip, (n + 1) * 4, hipMemcpyHostToDevice, s.stream), "H2D");
+    if (nnz > 0) {
+        hip_check(hipMemcpyAsync(s.d_ix.get(), s.h_ix, nnz * 2, hipMemcpyHostToDevice, s.stream), "H2D");
+        hip_check(hipMemcpyAsync(s.d_v.get(), s.h_v, nnz * 4, hipMemcpyHostToDevice, s.stream), "H2D");
+    }
+    hip_check(hipMemcpyAsync(s.d_ks.get(), s.h_ks, n * 4, hipMemcpyHostToDevice, s.stream), "H2D");
+    const bool with_sel = !bitmaps_.empty();
+    if (with_sel) hip_check(hipMemcpyAsync(s.d_sel.get(), s.h_sel, n * 4, hipMemcpyHostToDevice, s.stream), "H2D");
+    index_.search_device(n, s.d_ip.get<int32_t>(), s.d_ix.get<uint16_t>(), s.d_v.get<float>(), s.d_ks.get<int32_t>(),
+                         cfg_.k_max, cfg_.req, with_sel ? s.d_sel.get<int32_t>() : nullptr,
+                         with_sel ? d_bitmap_table_.get<const uint32_t*>() : nullptr, s.d_dist.get<float>(),
+                         s.d_lab.get<int32_t>(), nullptr, s.stream);
+    const size_t out = static_cast<size_t>(n) * cfg_.k_max;
+    hip_check(hipMemcpyAsync(s.h_dist, s.d_dist.get(), out * 4, hipMemcpyDeviceToHost, s.stream), "D2H");
+    hip_check(hipMemcpyAsync(s.h_lab, s.d_lab.get(), out * 4, hipMemcpyDeviceToHost, s.stream), "D2H");
+    hip_check(hipEventRecord(s.done, s.stream), "hipEventRecord");
+    s.tickets = batch;
+    s.busy = true;
+}
+
+}  // namespace nsparse_amd

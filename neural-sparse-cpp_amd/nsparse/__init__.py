@@ -14,6 +14,7 @@ from .index import (  # noqa: F401
     NotIDSelector,
     SearchParameters,
     SearchPlan,
+    Server,
     SeismicSearchParameters,
     SeismicSQSearchParameters,
     SetIDSelector,

@@ -68,6 +68,17 @@ class SearchConfig(ctypes.Structure):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("max_batch", ctypes.c_int32),
+        ("n_streams", ctypes.c_int32),
+        ("scratch_budget", ctypes.c_int64),
+        ("k_max", ctypes.c_int32),
+        ("cut", ctypes.c_int32),
+        ("heap_factor", c_float),
+    ]
+
+
 class SynthSpec(ctypes.Structure):
     _fields_ = [
         ("seed", ctypes.c_uint64),
@@ -112,6 +123,16 @@ _sig("nsparse_search_plan_create", [c_void_p, c_int64, c_void_p, c_void_p, c_voi
 _sig("nsparse_search_plan_launch", [c_void_p, c_void_p])
 _sig("nsparse_search_plan_config", [c_void_p, P(SearchConfig), c_int, P(SearchConfig), P(c_float), P(c_int)])
 _sig("nsparse_search_plan_free", [c_void_p])
+_sig("nsparse_server_create", [c_void_p, P(ServerConfig), P(c_void_p)])
+_sig("nsparse_server_add_bitmap", [c_void_p, c_void_p, c_int64, P(ctypes.c_int32)])
+_sig("nsparse_server_submit", [c_void_p, ctypes.c_int32, c_void_p, c_void_p, ctypes.c_int32, ctypes.c_int32,
+                               c_int64, P(c_int64)])
+_sig("nsparse_server_wait", [c_void_p, c_int64, c_void_p, c_void_p, P(ctypes.c_double)])
+_sig("nsparse_server_now_ns", [c_void_p, P(c_int64)])
+_sig("nsparse_server_replay", [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                               c_void_p, c_void_p, c_void_p])
+_sig("nsparse_server_stats", [c_void_p, P(c_int64)])
+_sig("nsparse_server_free", [c_void_p])
 _sig("nsparse_exact_scores_device", [c_int64, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
                                      c_void_p, c_void_p])
 _sig("nsparse_rescore_device", [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int64, c_void_p,

@@ -309,6 +309,79 @@ class SearchPlan:
             pass
 
 
+class Server:
+    """Continuous-batching search server over a built SEISMIC index resident in
+    HBM (nsparse_server_*): requests are batched as they arrive, batches run on
+    several streams within a scratch-memory budget."""
+
+    STAT_NAMES = ("completed", "batches", "max_inflight_bytes", "max_batch", "deferred_by_budget",
+                  "max_concurrent_batches")
+
+    def __init__(self, index, max_batch=512, streams=4, scratch_budget=256 << 20, k_max=100, cut=10,
+                 heap_factor=1.0):
+        cfg = _lib.ServerConfig(int(max_batch), int(streams), int(scratch_budget), int(k_max), int(cut),
+                                float(heap_factor))
+        h = ctypes.c_void_p()
+        check(lib.nsparse_server_create(index.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h.value
+        self._index = index
+        self.k_max = int(k_max)
+
+    def add_bitmap(self, words):
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        out = ctypes.c_int32()
+        check(lib.nsparse_server_add_bitmap(self._h, _ptr(w), len(w), ctypes.byref(out)))
+        return out.value
+
+    def submit(self, indices, values, k, bitmap_id=-1, arrival_ns=-1):
+        ix = np.ascontiguousarray(indices, dtype=np.uint16)
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        t = ctypes.c_int64()
+        check(lib.nsparse_server_submit(self._h, len(ix), _ptr(ix), _ptr(v), int(k), int(bitmap_id),
+                                        int(arrival_ns), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket, k):
+        d = np.empty(k, np.float32)
+        lab = np.empty(k, np.int32)
+        us = ctypes.c_double()
+        check(lib.nsparse_server_wait(self._h, int(ticket), _ptr(d), _ptr(lab), ctypes.byref(us)))
+        return d, lab, us.value
+
+    def replay(self, arrival_s, indptr, indices, values, ks, bitmap_ids=None):
+        """Submits query i at arrival_s[i] seconds from now; returns (distances,
+        labels) of shape n x k_max and the latencies in microseconds."""
+        n = len(indptr) - 1
+        a = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        ip = np.ascontiguousarray(indptr, dtype=np.int32)
+        ix = np.ascontiguousarray(indices, dtype=np.uint16)
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        kk = np.ascontiguousarray(ks, dtype=np.int32)
+        bm = np.ascontiguousarray(bitmap_ids, dtype=np.int32) if bitmap_ids is not None else None
+        d = np.empty((n, self.k_max), np.float32)
+        lab = np.empty((n, self.k_max), np.int32)
+        lat = np.empty(n, np.float64)
+        check(lib.nsparse_server_replay(self._h, n, _ptr(a), _ptr(ip), _ptr(ix), _ptr(v), _ptr(kk),
+                                        _ptr(bm) if bm is not None else None, _ptr(d), _ptr(lab), _ptr(lat)))
+        return d, lab, lat
+
+    def stats(self):
+        out = np.zeros(6, np.int64)
+        check(lib.nsparse_server_stats(self._h, _ptr(out)))
+        return dict(zip(self.STAT_NAMES, out.tolist()))
+
+    def close(self):
+        if self._h:
+            check(lib.nsparse_server_free(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def index_factory(dim, description):
     h = ctypes.c_void_p()
     check(lib.nsparse_index_factory(int(dim), description.encode(), ctypes.byref(h)))
