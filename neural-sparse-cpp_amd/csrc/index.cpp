@@ -251,15 +251,20 @@ void SeismicIndex::ensure_device() {
         d_indptr_.upload(fwd_.indptr.data(), fwd_.indptr.size() * 4, 64);
         d_indices_.upload(fwd_.indices.data(), fwd_.indices.size() * 2, 64);
         d_values_.upload(fwd_.values.data(), fwd_.values.size(), 64);
+        // the forward-index view before the lists: upload_lists derives the
+        // block rows from it
+        view_.n_docs = static_cast<int32_t>(fwd_.n());
+        view_.dim = dim_;
+        view_.esz = static_cast<int32_t>(fwd_.esz);
+        view_.indptr = d_indptr_.get<int32_t>();
+        view_.indices = d_indices_.get<uint16_t>();
+        view_.values = d_values_.get<void>();
         dev_fwd_valid_ = true;
     }
-    // the forward-index view first: upload_lists derives the block rows from it
-    view_.n_docs = static_cast<int32_t>(fwd_.n());
-    view_.dim = dim_;
-    view_.esz = static_cast<int32_t>(fwd_.esz);
-    view_.indptr = d_indptr_.get<int32_t>();
-    view_.indices = d_indices_.get<uint16_t>();
-    view_.values = d_values_.get<void>();
+    if (!d_err_.bytes()) {
+        d_err_.allocate(4);
+        hip_check(hipMemset(d_err_.get(), 0, 4), "hipMemset");
+    }
     if (!dev_lists_valid_ && host_lists_valid_) upload_lists();
 }
 
@@ -513,7 +518,6 @@ SearchArgs SeismicIndex::search_args(int64_t n, const int32_t* d_indptr, const u
     if (req.has_sq_range && sq_) check_quantizer_range(req.vmin, req.vmax);
     require_device();
     ensure_device();
-    if (!d_err_.bytes()) d_err_.allocate(4);
     SearchArgs a{};
     a.ix = view_;
     a.q_indptr = d_indptr;

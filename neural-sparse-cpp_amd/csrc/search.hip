@@ -1447,6 +1447,8 @@ void launch_search(const SearchArgs& args_in, int max_clusters, int kmax, bool s
     SearchArgs a = args_in;
     a.queue = nullptr;
     a.order = nullptr;
+    // the error flag reports this launch (a kernel, not a memset: graph-safe)
+    if (a.error_flag != nullptr) hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, a.error_flag, 1);
     if (cfg.persistent) {
         if (scratch == nullptr) throw std::invalid_argument("the persistent schedule needs a scratch buffer");
         int32_t* base = static_cast<int32_t*>(scratch);

@@ -131,7 +131,7 @@ _sig("nsparse_server_wait", [c_void_p, c_int64, c_void_p, c_void_p, P(ctypes.c_d
 _sig("nsparse_server_now_ns", [c_void_p, P(c_int64)])
 _sig("nsparse_server_replay", [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_void_p, c_void_p])
-_sig("nsparse_server_stats", [c_void_p, P(c_int64)])
+_sig("nsparse_server_stats", [c_void_p, c_void_p])
 _sig("nsparse_server_free", [c_void_p])
 _sig("nsparse_exact_scores_device", [c_int64, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
                                      c_void_p, c_void_p])
