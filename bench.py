@@ -52,6 +52,52 @@ def algorithmic_bytes(stats):
     return (s[:, 8] + 8 * s[:, 1] + 3 * s[:, 2] + 8 * s[:, 3] + 4 * s[:, 5] + 8 * s[:, 6] + 3 * s[:, 7])
 
 
+def serving_bench(args, nsparse, index, n_docs, threads):
+    """Continuous batching on the resident index: Poisson arrivals at each rate,
+    k in {10, 100}, 10% of requests carrying a selector bitmap that excludes a
+    random 20% of ids; p50 / p99 latency (arrival to result on the host) and
+    sustained QPS (requests / (last completion - first arrival))."""
+    from nsparse import synth
+
+    rates = [float(r) for r in args.serve_rates.split(",") if r]
+    if not rates:
+        return None
+    rng = np.random.default_rng(3)
+    srv = nsparse.Server(index, max_batch=args.serve_batch, streams=args.serve_streams,
+                         scratch_budget=args.serve_budget_mb << 20, k_max=100, cut=args.cut, heap_factor=args.hf)
+    for _ in range(8):
+        keep = rng.random(n_docs) >= 0.2
+        b = np.packbits(keep.astype(np.uint8), bitorder="little")
+        srv.add_bitmap(np.pad(b, (0, (-len(b)) % 4)).view(np.uint32))
+    out = []
+    for rate in rates:
+        n = int(min(max(rate * args.serve_seconds, 2000), 60000))
+        qip, qix, qv = synth.make_corpus(synth.query_spec(seed=100 + int(rate), log_uniform=True), n,
+                                         threads=threads)
+        r2 = np.random.default_rng(int(rate))
+        ks = np.where(r2.random(n) < 0.5, 10, 100).astype(np.int32)
+        bids = np.where(r2.random(n) < 0.1, r2.integers(0, 8, n), -1).astype(np.int32)
+        arrivals = np.cumsum(r2.exponential(1.0 / rate, n))
+        before = srv.stats()
+        _, _, lat = srv.replay(arrivals, qip, qix, qv, ks, bids)
+        after = srv.stats()
+        done = arrivals + lat * 1e-6
+        lat_s = np.sort(lat) / 1e3
+        pct = lambda f: float(lat_s[min(n - 1, int(math.floor(f * n)))])  # noqa: E731
+        batches = after["batches"] - before["batches"]
+        row = {"offered_qps": rate, "requests": n, "sustained_qps": n / float(done.max() - arrivals[0]),
+               "p50_ms": pct(0.5), "p99_ms": pct(0.99), "batches": batches, "mean_batch": n / max(batches, 1),
+               "max_concurrent_batches": after["max_concurrent_batches"],
+               "deferred_by_budget": after["deferred_by_budget"] - before["deferred_by_budget"]}
+        log(f"serve {rate:,.0f}/s: sustained {row['sustained_qps']:,.0f} QPS, p50 {row['p50_ms']:.2f} ms, "
+            f"p99 {row['p99_ms']:.2f} ms, {batches} batches (mean {row['mean_batch']:.1f})")
+        out.append(row)
+    srv.close()
+    return {"config": {"max_batch": args.serve_batch, "streams": args.serve_streams,
+                       "scratch_budget_mb": args.serve_budget_mb, "k": "10 or 100", "selectors": "10%, 20% excluded",
+                       "arrivals": "Poisson"}, "rates": out}
+
+
 def same_topk_up_to_ties(d_ref, l_ref, d_got, l_got):
     """Top-k identical apart from exact score ties: the same scores in the same
     order, and the same doc ids at every score above the k-th (docs tied with
@@ -136,6 +182,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--sweep", default="", help="comma-separated cut:heap_factor pairs to time and score")
+    ap.add_argument("--serve-rates", default="1000,10000,50000", help="serving loop: offered QPS levels")
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--serve-batch", type=int, default=1024)
+    ap.add_argument("--serve-streams", type=int, default=4)
+    ap.add_argument("--serve-budget-mb", type=int, default=256)
     ap.add_argument("--schedule", default="auto",
                     help='"auto" (autotuner) or a JSON search config, e.g. {"lookup":1,"block":512,"persistent":1}')
     args = ap.parse_args()
@@ -290,6 +341,9 @@ def main():
         sweep.append({"cut": cut, "heap_factor": hf, "qps": sq, "recall_at_10": rc, "recall_vs_uint8_exact": rcq})
         log(f"sweep cut={cut} hf={hf}: {sq:,.0f} QPS, recall@10 {rc:.4f} (vs uint8-exact {rcq:.4f})")
 
+    # ---- serving loop on the same resident index
+    serving = serving_bench(args, nsparse, index, args.docs, threads)
+
     # ---- CPU baseline: the reference's search restated in C++ (oracle/),
     # on the same index bytes, all host threads, bounded query sample.
     cpu = None
@@ -354,6 +408,8 @@ def main():
     }
     if sweep:
         result["sweep"] = sweep
+    if serving:
+        result["serving"] = serving
     print(json.dumps(result), flush=True)
 
 
