@@ -899,7 +899,8 @@ __global__ __launch_bounds__(kB) void block_copy_kernel(const int32_t* __restric
 }  // namespace
 
 bool build_block_rows(const DevIndexView& ix, int64_t n_postings, DeviceBuffer& row_ref, DeviceBuffer& blk_indices,
-                      DeviceBuffer& blk_codes) {
+                      DeviceBuffer& blk_codes, int64_t& n_words) {
+    n_words = 0;
     row_ref.release();
     blk_indices.release();
     blk_codes.release();
@@ -938,6 +939,7 @@ bool build_block_rows(const DevIndexView& ix, int64_t n_postings, DeviceBuffer& 
                        static_cast<const uint8_t*>(ix.values), ix.docs, ws.get<int64_t>(), D,
                        blk_indices.get<uint4>(), blk_codes.get<uint2>(), row_ref.get<int64_t>());
     sync("block row copy");
+    n_words = W;
     return true;
 }
 

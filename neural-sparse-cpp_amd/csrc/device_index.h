@@ -44,6 +44,7 @@ struct DevIndexView {
     const int64_t* row_ref = nullptr;
     const uint16_t* blk_indices = nullptr;
     const uint8_t* blk_codes = nullptr;
+    int64_t blk_words = 0;  // words in the block copies; word blk_words is zero padding
 };
 
 // Per-query traversal counters (the algorithmic-byte model): one row of

@@ -231,10 +231,12 @@ void SeismicIndex::refresh_block_rows() {
     view_.row_ref = nullptr;
     view_.blk_indices = nullptr;
     view_.blk_codes = nullptr;
+    view_.blk_words = 0;
     const char* e = std::getenv("NSPARSE_BLOCK_ROWS");
     const bool on = sq_ && !(e != nullptr && std::string(e) == "0");
     const int64_t postings = static_cast<int64_t>(d_docs_.bytes() / 4);
-    if (!on || !build_block_rows(view_, postings, d_row_ref_, d_blk_indices_, d_blk_codes_)) {
+    int64_t words = 0;
+    if (!on || !build_block_rows(view_, postings, d_row_ref_, d_blk_indices_, d_blk_codes_, words)) {
         d_row_ref_.release();
         d_blk_indices_.release();
         d_blk_codes_.release();
@@ -243,6 +245,7 @@ void SeismicIndex::refresh_block_rows() {
     view_.row_ref = d_row_ref_.get<int64_t>();
     view_.blk_indices = d_blk_indices_.get<uint16_t>();
     view_.blk_codes = d_blk_codes_.get<uint8_t>();
+    view_.blk_words = words;
 }
 
 void SeismicIndex::ensure_device() {

@@ -50,7 +50,7 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
 // false, leaving the buffers empty, when HBM cannot hold them or a list's rows
 // exceed 32-bit element offsets.
 bool build_block_rows(const DevIndexView& ix, int64_t n_postings, DeviceBuffer& row_ref, DeviceBuffer& blk_indices,
-                      DeviceBuffer& blk_codes);
+                      DeviceBuffer& blk_codes, int64_t& n_words);
 
 // The default (NSPARSE_SEARCH_LOOKUP / NSPARSE_SEARCH_BLOCK override it).
 SearchConfig default_search_config(bool sq);

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 #include <cstdlib>
 
 #include <algorithm>
@@ -771,14 +772,14 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                 // every load is coalesced and each lane keeps kWinPerStep
                 // independent loads in flight. A window's lanes find their doc
                 // from the window's first doc and the doc boundaries inside the
-                // window (a wave-uniform loop); a DPP prefix sum over the window
-                // gives each doc run's sum, added by the run's last lane with one
-                // LDS atomic (integer: exact in any order).
+                // window (a wave-uniform loop). A DPP prefix sum S over the window
+                // then yields each doc run's sum: the last lane of a run adds S to
+                // its doc and subtracts S from the next run's doc (LDS integer
+                // atomics: exact in any order, modulo 2^32, and the true sums fit).
+                // Block-contiguous rows start on a word and are zero padded, so
+                // they need no masks; lanes without a word read the zero word
+                // past the end. Forward-index rows mask the bytes outside the row.
                 constexpr int kWinPerStep = win_per_step<DENSE>();
-                const uint4* idx8 = blk ? reinterpret_cast<const uint4*>(ix.blk_indices) + lwb
-                                        : reinterpret_cast<const uint4*>(ix.indices);
-                const uint2* val8 = blk ? reinterpret_cast<const uint2*>(ix.blk_codes) + lwb
-                                        : reinterpret_cast<const uint2*>(ix.values);
                 if (wave == 0) {
                     int run = 0;
                     for (int b = 0; b < total; b += 64) {
@@ -790,7 +791,11 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                             st_isc[j] = 0;
                         }
                         const int incl = wave_incl_scan(nwj, lane);
-                        if (j < total) wbase[j] = run + incl - nwj;
+                        if (j < total) {
+                            wbase[j] = run + incl - nwj;
+                            // block rows: the doc's first word minus its first chunk word
+                            if (blk) st_rs[j] = (st_rs[j] >> 3) - (run + incl - nwj);
+                        }
                         run += __shfl(incl, 63, 64);
                     }
                     // wbase[total] = W; then sentinels above every word index (the
@@ -801,97 +806,116 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                 __syncthreads();
                 const int W = wbase[total];
                 const int nwin = (W + 63) >> 6;
-                for (int x0 = 0; x0 < nwin; x0 += kWinMax) {
-                    const int x1 = min(nwin, x0 + kWinMax);
-                    // doc j holds the first word of window x when 64x is in [wbase[j], wbase[j+1])
-                    for (int j = tid; j < total; j += kBlock) {
-                        const int xa = max((wbase[j] + 63) >> 6, x0), xb = min((wbase[j + 1] + 63) >> 6, x1);
-                        for (int x = xa; x < xb; ++x) winfirst[x - x0] = static_cast<uint16_t>(j);
-                    }
-                    __syncthreads();
-                    for (int xs = x0 + wave; xs < x1; xs += kWaves * kWinPerStep) {
-                        int jj[kWinPerStep], s0[kWinPerStep], lo[kWinPerStep], hi[kWinPerStep], ww[kWinPerStep];
-                        bool last[kWinPerStep];
-                        uint4 ii[kWinPerStep];
-                        uint2 vv[kWinPerStep];
-                        // 1. each lane's doc and word address (lanes past the end read word 0)
+                auto score_windows = [&](auto blk_tag) {
+                    constexpr bool BLK = decltype(blk_tag)::value;
+                    const uint4* idx8 = BLK ? reinterpret_cast<const uint4*>(ix.blk_indices) + lwb
+                                            : reinterpret_cast<const uint4*>(ix.indices);
+                    const uint2* val8 = BLK ? reinterpret_cast<const uint2*>(ix.blk_codes) + lwb
+                                            : reinterpret_cast<const uint2*>(ix.values);
+                    const int64_t zero_word = BLK ? ix.blk_words - lwb : 0;
+                    for (int x0 = 0; x0 < nwin; x0 += kWinMax) {
+                        const int x1 = min(nwin, x0 + kWinMax);
+                        // doc j holds the first word of window x when 64x is in [wbase[j], wbase[j+1])
+                        for (int j = tid; j < total; j += kBlock) {
+                            const int xa = max((wbase[j] + 63) >> 6, x0), xb = min((wbase[j + 1] + 63) >> 6, x1);
+                            for (int x = xa; x < xb; ++x) winfirst[x - x0] = static_cast<uint16_t>(j);
+                        }
+                        __syncthreads();
+                        for (int xs = x0 + wave; xs < x1; xs += kWaves * kWinPerStep) {
+                            int jj[kWinPerStep], j1s[kWinPerStep], lo[kWinPerStep], hi[kWinPerStep];
+                            int64_t ww[kWinPerStep];
+                            uint4 ii[kWinPerStep];
+                            uint2 vv[kWinPerStep];
+                            // 1. each lane's doc and word address
 #pragma unroll
-                        for (int u = 0; u < kWinPerStep; ++u) {
-                            const int x = xs + u * kWaves;
-                            jj[u] = -1;
-                            ww[u] = 0;
-                            lo[u] = 0;
-                            hi[u] = 8;
-                            s0[u] = 0;
-                            last[u] = false;
-                            if (x >= x1) continue;  // wave-uniform
-                            const int F = x << 6, f = F + lane;
-                            int j = winfirst[x - x0], j1 = j;
-                            for (int base = j + 1;; base += 64) {
-                                const int bn = wbase[base + lane];
-                                const uint64_t bm = ballot(bn <= F + 63);
-                                const int nb = __popcll(bm);
-                                for (int i = 0; i < nb; ++i) {
-                                    const int B = __builtin_amdgcn_readlane(bn, i);
-                                    j += f >= B;
-                                    j1 += f + 1 >= B;
+                            for (int u = 0; u < kWinPerStep; ++u) {
+                                const int x = xs + u * kWaves;
+                                jj[u] = -1;
+                                j1s[u] = -1;
+                                ww[u] = zero_word;
+                                lo[u] = 0;
+                                hi[u] = 8;
+                                if (x >= x1) continue;  // wave-uniform
+                                const int F = x << 6, f = F + lane;
+                                int j = winfirst[x - x0], j1 = j;
+                                for (int base = j + 1;; base += 64) {
+                                    const int bn = wbase[base + lane];
+                                    const uint64_t bm = ballot(bn <= F + 63);
+                                    const int nb = __popcll(bm);
+                                    for (int i = 0; i < nb; ++i) {
+                                        const int B = __builtin_amdgcn_readlane(bn, i);
+                                        j += f >= B;
+                                        j1 += f + 1 >= B;
+                                    }
+                                    if (nb < 64) break;
                                 }
-                                if (nb < 64) break;
-                            }
-                            if (f < W) {
-                                const int rs = st_rs[j], re = rs + st_len[j], wb = wbase[j];
-                                const int w = (rs >> 3) + (f - wb);
-                                ww[u] = w;
-                                jj[u] = j;
-                                lo[u] = max(rs - (w << 3), 0);
-                                hi[u] = min(re - (w << 3), 8);
-                                last[u] = j1 != j || lane == 63;
-                                s0[u] = max(wb - F, 0);
-                            }
-                        }
-                        // 2. all loads of the step back to back, without branches
-#pragma unroll
-                        for (int u = 0; u < kWinPerStep; ++u) {
-                            ii[u] = idx8[ww[u]];
-                            vv[u] = val8[ww[u]];
-                        }
-                        // 3. scores
-#pragma unroll
-                        for (int u = 0; u < kWinPerStep; ++u) {
-                            if (xs + u * kWaves >= x1) continue;  // wave-uniform
-                            // Branch-free: code bytes outside the doc's [rs, re) -- and every
-                            // byte of a lane without a word -- are masked to zero. The masked
-                            // elements' term ids are still valid ids (neighbouring rows or
-                            // the zeroed tail padding), so their lookups stay in bounds.
-                            const bool has = jj[u] >= 0;
-                            const int l8 = 8 * lo[u], h8 = 8 * hi[u];
-                            const uint64_t mlo = has ? (~0ULL << l8) : 0ULL;
-                            const uint64_t mhi = hi[u] < 8 ? ((1ULL << h8) - 1) : ~0ULL;
-                            const uint64_t codes = ((static_cast<uint64_t>(vv[u].y) << 32) | vv[u].x) & mlo & mhi;
-                            const uint32_t tw[4] = {ii[u].x, ii[u].y, ii[u].z, ii[u].w};
-                            uint32_t acc = 0;
-#pragma unroll
-                            for (int t = 0; t < 8; ++t) {
-                                const uint32_t term = (t & 1) ? (tw[t >> 1] >> 16) : (tw[t >> 1] & 0xFFFFu);
-                                const uint32_t c = static_cast<uint32_t>(codes >> (8 * t)) & 0xFFu;
-                                if constexpr (DENSE) {
-                                    acc += c * static_cast<uint32_t>(qcode[term]);
-                                } else {
-                                    bool pr;
-                                    const int r = QL.rank_of(term, pr);
-                                    const uint32_t qv = static_cast<uint32_t>(rval[r]);
-                                    acc += pr ? c * qv : 0U;
+                                if (f < W) {
+                                    jj[u] = j;
+                                    // run end (the next word is another doc's, or the window
+                                    // ends): j1s = the doc whose run follows in this window,
+                                    // INT32_MAX when none does, -1 when not a run end
+                                    const bool end = j1 != j || lane == 63;
+                                    j1s[u] = !end ? -1 : (j1 != j && f + 1 < W && lane != 63) ? j1 : INT32_MAX;
+                                    if constexpr (BLK) {
+                                        ww[u] = f + st_rs[j];
+                                    } else {
+                                        const int rs = st_rs[j], re = rs + st_len[j], wb = wbase[j];
+                                        const int w = (rs >> 3) + (f - wb);
+                                        ww[u] = w;
+                                        lo[u] = max(rs - (w << 3), 0);
+                                        hi[u] = min(re - (w << 3), 8);
+                                    }
                                 }
                             }
-                            const uint32_t S = wave_incl_scan_dpp(acc);
-                            const int from = jj[u] >= 0 ? s0[u] - 1 : -1;
-                            const uint32_t before = static_cast<uint32_t>(__shfl(static_cast<int>(S), max(from, 0), 64));
-                            const uint32_t sum = S - (from >= 0 ? before : 0U);
-                            if (jj[u] >= 0 && last[u] && sum != 0) atomicAdd(&st_isc[jj[u]], sum);
+                            // 2. all loads of the step back to back, without branches
+#pragma unroll
+                            for (int u = 0; u < kWinPerStep; ++u) {
+                                ii[u] = idx8[ww[u]];
+                                vv[u] = val8[ww[u]];
+                            }
+                            // 3. scores
+#pragma unroll
+                            for (int u = 0; u < kWinPerStep; ++u) {
+                                if (xs + u * kWaves >= x1) continue;  // wave-uniform
+                                uint64_t codes = (static_cast<uint64_t>(vv[u].y) << 32) | vv[u].x;
+                                if constexpr (!BLK) {
+                                    // bytes outside the doc's [rs, re) -- and every byte of a
+                                    // lane without a word -- are masked; their term ids are
+                                    // still valid ids, so the lookups stay in bounds
+                                    const bool has = jj[u] >= 0;
+                                    const uint64_t mlo = has ? (~0ULL << (8 * lo[u])) : 0ULL;
+                                    const uint64_t mhi = hi[u] < 8 ? ((1ULL << (8 * hi[u])) - 1) : ~0ULL;
+                                    codes &= mlo & mhi;
+                                }
+                                const uint32_t tw[4] = {ii[u].x, ii[u].y, ii[u].z, ii[u].w};
+                                uint32_t acc = 0;
+#pragma unroll
+                                for (int t = 0; t < 8; ++t) {
+                                    const uint32_t term = (t & 1) ? (tw[t >> 1] >> 16) : (tw[t >> 1] & 0xFFFFu);
+                                    const uint32_t c = static_cast<uint32_t>(codes >> (8 * t)) & 0xFFu;
+                                    if constexpr (DENSE) {
+                                        acc += c * static_cast<uint32_t>(qcode[term]);
+                                    } else {
+                                        bool pr;
+                                        const int r = QL.rank_of(term, pr);
+                                        const uint32_t qv = static_cast<uint32_t>(rval[r]);
+                                        acc += pr ? c * qv : 0U;
+                                    }
+                                }
+                                const uint32_t S = wave_incl_scan_dpp(acc);
+                                if (j1s[u] >= 0 && S != 0) {
+                                    atomicAdd(&st_isc[jj[u]], S);
+                                    if (j1s[u] != INT32_MAX) atomicSub(&st_isc[j1s[u]], S);
+                                }
+                            }
                         }
+                        __syncthreads();
                     }
-                    __syncthreads();
-                }
+                };
+                if (blk)
+                    score_windows(std::true_type{});
+                else
+                    score_windows(std::false_type{});
                 for (int j = tid; j < total; j += kBlock) st_sc[j] = static_cast<float>(st_isc[j]);
             } else {
                 const float* vf = static_cast<const float*>(ix.values);
