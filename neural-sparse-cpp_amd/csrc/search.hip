@@ -42,7 +42,11 @@ constexpr int kBlock = 256;  // exact-match, brute-force and default search work
 constexpr int kWaves = kBlock / 64;
 constexpr int kMaxWaves = 16;
 constexpr int kQMax = 512;    // longest query handled on device
-constexpr int kBudget = 512;  // doc slots per speculative chunk
+// doc slots per speculative chunk: 1024 with the dense code table (its 80 KB
+// LDS plan still fits two workgroups per CU), 512 with the bitmap lookup (four
+// workgroups per CU); LdsPlan::budget carries it
+constexpr int kBudgetDense = 1024;
+constexpr int kBudgetCompact = 512;
 constexpr int kSegMax = 64;   // clusters examined per chunk (one wave pass)
 constexpr int kTileFloats = 4096;
 constexpr int kWinMax = 512;      // 64-word windows per first-doc table pass
@@ -86,6 +90,7 @@ enum Misc {
 // Byte offsets of the dynamic LDS sections (computed on the host).
 struct LdsPlan {
     int32_t nw;        // 32-bit words of the query bitmap
+    int32_t budget;    // doc slots per speculative chunk
     int32_t mc;        // cluster capacity
     int32_t kmax;      // top-k capacity
     int32_t uni_bytes; // union scratch (staging / sort buffer / FP32 tile)
@@ -313,6 +318,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
     uint8_t* qcode = smem;  // DENSE only: offset 0
     // staging view of the union
     int32_t* st_id = reinterpret_cast<int32_t*>(uni);
+    const int kBudget = P.budget;
     int32_t* st_rs = st_id + kBudget;
     int32_t* st_len = st_rs + kBudget;
     float* st_sc = reinterpret_cast<float*>(st_len + kBudget);
@@ -1135,7 +1141,8 @@ LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq, bool dense) {
     P.nw = (dim + 31) / 32;
     P.mc = max(max_clusters, 1);
     P.kmax = kmax;
-    const int staging = kBudget * 16;
+    P.budget = dense ? kBudgetDense : kBudgetCompact;
+    const int staging = P.budget * 16;
     const int sortbuf = P.mc * 12;  // KeyItem[mc] + two uint16[mc] position lists
     const int tile = sq ? 0 : kTileFloats * 4;
     P.uni_bytes = max(max(staging, sortbuf), max(tile, kQMax * 8));
@@ -1166,7 +1173,7 @@ LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq, bool dense) {
     P.o_seg = take(kSegMax * static_cast<int>(sizeof(Seg)));
     P.o_misc = take(kMiscCount * 4);
     P.o_wsum = take(kMaxWaves * 4);
-    P.o_wbase = take((kBudget + 128) * 4);
+    P.o_wbase = take((P.budget + 128) * 4);
     P.o_winfirst = take(kWinMax * 2);
     P.o_queue = take(16);
     P.total = o;
