@@ -59,7 +59,7 @@ def serving_bench(args, nsparse, index, n_docs, threads):
     sustained QPS (requests / (last completion - first arrival))."""
     from nsparse import synth
 
-    rates = [float(r) for r in args.serve_rates.split(",") if r]
+    rates = [float(r) for r in args.serve_rates.split(",") if r and r != "none"]
     if not rates:
         return None
     rng = np.random.default_rng(3)
