@@ -98,6 +98,22 @@ def serving_bench(args, nsparse, index, n_docs, threads):
                        "arrivals": "Poisson"}, "rates": out}
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of this workload as measured by the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes summarized under profiles/ (scaled by a copy
+    probe there); None when no summary covers the workload."""
+    import glob
+    import re
+
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"traffic_bytes_per_launch\[(.+)\] = (\d+)", line.strip())
+                if m and m.group(1) == workload:
+                    return float(m.group(2)), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def same_topk_up_to_ties(d_ref, l_ref, d_got, l_got):
     """Top-k identical apart from exact score ties: the same scores in the same
     order, and the same doc ids at every score above the k-th (docs tied with
@@ -379,6 +395,7 @@ def main():
                "topk_identical_up_to_ties_frac": agree / max(total, 1)}
         log(f"cpu: {cpu['value']:,.0f} QPS on {threads} threads, top-k identical up to ties {agree}/{total}")
 
+    traffic, traffic_src = measured_traffic(f"seismic_sq_batched_{args.docs}docs/{args.queries}q")
     result = {
         "metric": METRIC, "value": qps, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
@@ -390,7 +407,8 @@ def main():
                    "schedule": plan.config,
                    "autotuner_ms": [dict(c, ms=ms) for c, ms in tuning]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "bytes_per_launch": bytes_launch, "kernel_ms": kernel_ms},
         "cpu_baseline": cpu,
         "plain": {"qps": plain_qps if n_plain else None, "p50_ms": pct(0.5), "p99_ms": pct(0.99),
