@@ -95,7 +95,8 @@ struct LdsPlan {
     int32_t kmax;      // top-k capacity
     int32_t uni_bytes; // union scratch (staging / sort buffer / FP32 tile)
     int32_t o_qbits, o_qpref, o_eterm, o_erank, o_eval, o_rlast, o_rval, o_rsum, o_rcol, o_flist, o_cuts;
-    int32_t o_cstart, o_score, o_order, o_uni, o_tks, o_tki, o_seg, o_misc, o_wsum, o_wbase, o_winfirst, o_queue;
+    int32_t o_cstart, o_score, o_order, o_uni, o_tks, o_tki, o_seg, o_misc, o_wsum, o_wbase, o_winfirst, o_queue,
+        o_cmask;
     int32_t total;
 };
 
@@ -315,6 +316,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
     int32_t* wsum = reinterpret_cast<int32_t*>(smem + P.o_wsum);
     int32_t* wbase = reinterpret_cast<int32_t*>(smem + P.o_wbase);
     uint16_t* winfirst = reinterpret_cast<uint16_t*>(smem + P.o_winfirst);
+    uint64_t* cmask = reinterpret_cast<uint64_t*>(smem + P.o_cmask);  // candidate docs of the chunk, 64 per word
     uint8_t* qcode = smem;  // DENSE only: offset 0
     // staging view of the union
     int32_t* st_id = reinterpret_cast<int32_t*>(uni);
@@ -922,7 +924,6 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                     score_windows(std::true_type{});
                 else
                     score_windows(std::false_type{});
-                for (int j = tid; j < total; j += kBlock) st_sc[j] = static_cast<float>(st_isc[j]);
             } else {
                 const float* vf = static_cast<const float*>(ix.values);
                 for (int j = tid; j < total; j += kBlock) {
@@ -933,44 +934,141 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                     for (int pp = rs; pp < rs + len; ++pp) acc += vf[pp] * QL.value(ix.indices[pp]);
                     st_sc[j] = acc;
                 }
+                __syncthreads();
+            }
+            // Candidates for the replay: docs that beat the k-th score as it stands
+            // at the start of the chunk (it only rises, so no other doc can enter),
+            // one ballot mask per 64 staging slots. uint8 sums become float here.
+            {
+                const bool full0 = misc[kMiscTkN] == k;
+                const float ks0 = full0 ? tks[k - 1] : 0.0F;
+                const int32_t ki0 = full0 ? tki[k - 1] : 0;
+                for (int j0 = wave * 64; j0 < total; j0 += kBlock) {
+                    const int j = j0 + lane;
+                    bool c = false;
+                    if (j < total) {
+                        float sc;
+                        if constexpr (SQ) {
+                            sc = static_cast<float>(st_isc[j]);
+                            st_sc[j] = sc;
+                        } else {
+                            sc = st_sc[j];
+                        }
+                        c = st_len[j] >= 0 && (!full0 || better(sc, st_id[j], ks0, ki0));
+                    }
+                    const uint64_t m = ballot(c);
+                    if (lane == 0) cmask[j0 >> 6] = m;
+                }
             }
             __syncthreads();
             tick(5);
             // ---- exact replay of the chunk (wave 0) ----
-            // Only docs that beat the k-th score as it stands at the start of the
-            // chunk can ever enter (the k-th only rises); they are compacted in
-            // slot order. Segments are then replayed in visiting order: prune
-            // test against the exact current k-th, then the segment's candidates
-            // are merged into the sorted top-k in batches of 64. Inserting a batch
-            // one by one with the reference's reject rule keeps the k best of
-            // (top-k U batch) under (score desc, doc id asc) -- an order-free
-            // result -- so the batch is merged in parallel: each candidate's rank
-            // among the top-k (binary search; an equal (score, id) is the same
-            // doc seen in an earlier list) plus its rank among the batch.
+            // Segments are replayed in visiting order: the prune test against the
+            // exact current k-th, then the segment's candidates (the masks above)
+            // are merged into the sorted top-k one 64-slot word at a time, lane l
+            // holding slot 64w + l. Inserting a batch one by one with the
+            // reference's reject rule keeps the k best of (top-k U batch) under
+            // (score desc, doc id asc) -- an order-free result -- so the batch is
+            // merged in parallel: each candidate's rank among the top-k (ballots
+            // against the top-k held in registers, or a binary search when it has
+            // more than 128 entries; an equal id is the same doc seen in an
+            // earlier list) plus its rank within the batch.
             if (wave == 0) {
                 int tkn = misc[kMiscTkN];
                 bool done = false;
-                int32_t* cidx = wbase;  // free after scoring
-                int ncand = 0;
-                {
-                    const bool full0 = tkn == k;
-                    const float ks0 = full0 ? tks[k - 1] : 0.0F;
-                    const int32_t ki0 = full0 ? tki[k - 1] : 0;
-                    for (int b = 0; b < total; b += 64) {
-                        const int j = b + lane;
-                        bool c = j < total && st_len[j] >= 0;
-                        if (c && full0) c = better(st_sc[j], st_id[j], ks0, ki0);
-                        const uint64_t m = ballot(c);
-                        if (c) cidx[ncand + __popcll(m & ((1ULL << lane) - 1))] = j;
-                        ncand += __popcll(m);
+                auto merge = [&](bool in, int j) {
+                    const float sc = in ? st_sc[j] : 0.0F;
+                    const int32_t id = in ? st_id[j] : 0;
+                    const bool fullk = tkn == k;
+                    const float ks = fullk ? tks[k - 1] : 0.0F;
+                    const int32_t ki = fullk ? tki[k - 1] : 0;
+                    const bool cand = in && (!fullk || better(sc, id, ks, ki));
+                    const uint64_t cm = ballot(cand);
+                    if (cm == 0) return;
+                    const auto lane_f = [](float v, int l) {
+                        return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+                    };
+                    int rt = 0;
+                    bool dup = false;
+                    const bool small = tkn <= 128;
+                    const bool l0 = lane < tkn, l1 = lane + 64 < tkn;
+                    float e0s = 0.0F, e1s = 0.0F;
+                    int32_t e0i = 0, e1i = 0;
+                    if (small) {
+                        if (l0) {
+                            e0s = tks[lane];
+                            e0i = tki[lane];
+                        }
+                        if (l1) {
+                            e1s = tks[lane + 64];
+                            e1i = tki[lane + 64];
+                        }
+                        for (uint64_t m = cm; m != 0; m &= m - 1) {
+                            const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
+                            const float cs = lane_f(sc, bl);
+                            const int32_t ci = __builtin_amdgcn_readlane(id, bl);
+                            const int cnt = __popcll(ballot(l0 && better(e0s, e0i, cs, ci))) +
+                                            __popcll(ballot(l1 && better(e1s, e1i, cs, ci)));
+                            const bool dp = ballot((l0 && e0i == ci) || (l1 && e1i == ci)) != 0;
+                            if (lane == bl) {
+                                rt = cnt;
+                                dup = dp;
+                            }
+                        }
+                    } else if (cand) {
+                        int hi = tkn;
+                        while (rt < hi) {
+                            const int mid = (rt + hi) >> 1;
+                            if (better(tks[mid], tki[mid], sc, id))
+                                rt = mid + 1;
+                            else
+                                hi = mid;
+                        }
+                        dup = rt < tkn && tki[rt] == id;
                     }
-                }
-                __builtin_amdgcn_wave_barrier();
-                int cp = 0;  // next compacted candidate
+                    const bool v = cand && !dup;
+                    const uint64_t vm = ballot(v);
+                    if (vm == 0) return;
+                    int rc = 0;
+                    for (uint64_t m = vm; m != 0; m &= m - 1) {
+                        const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
+                        rc += better(lane_f(sc, bl), __builtin_amdgcn_readlane(id, bl), sc, id) ? 1 : 0;
+                    }
+                    // top-k entries move up past the candidates better than them,
+                    // highest tile first (a tile is read whole before it is written)
+                    for (int t0 = ((tkn - 1) >> 6) << 6; t0 >= 0 && tkn > 0; t0 -= 64) {
+                        const int t = t0 + lane;
+                        const bool live = t < tkn;
+                        float es;
+                        int32_t ei;
+                        if (small) {
+                            es = t0 == 0 ? e0s : e1s;
+                            ei = t0 == 0 ? e0i : e1i;
+                        } else {
+                            es = live ? tks[t] : 0.0F;
+                            ei = live ? tki[t] : 0;
+                        }
+                        int up = 0;
+                        for (uint64_t m = vm; m != 0; m &= m - 1) {
+                            const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
+                            up += better(lane_f(sc, bl), __builtin_amdgcn_readlane(id, bl), es, ei) ? 1 : 0;
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        if (live && up > 0 && t + up < k) {
+                            tks[t + up] = es;
+                            tki[t + up] = ei;
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                    if (v && rt + rc < k) {
+                        tks[rt + rc] = sc;
+                        tki[rt + rc] = id;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    tkn = min(k, tkn + __popcll(vm));
+                };
                 for (int si = 0; si < nseg && !done; ++si) {
                     const Seg s = seg[si];
-                    const int send = s.slot + s.count;
-                    bool pruned = false;
                     if (s.test) {
                         ++st_tested;
                         if (tkn == k && score[s.cluster] * hf < tks[k - 1]) {
@@ -978,85 +1076,20 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                                 done = true;
                                 break;
                             }
-                            pruned = true;
-                        } else {
-                            ++st_opened;
+                            continue;
                         }
-                    }
-                    if (pruned) {  // skip the segment's candidates
-                        while (cp < ncand) {
-                            const int v = cp + lane < ncand ? cidx[cp + lane] : 0x7FFFFFFF;
-                            const int n = __popcll(ballot(v < send));
-                            cp += n;
-                            if (n < 64) break;
-                        }
-                        continue;
+                        ++st_opened;
                     }
                     st_read += s.count;
                     st_scored += s.n_ok;
                     st_nnz += s.nnz;
-                    while (cp < ncand) {
-                        const int ci = cp + lane;
-                        const int j = ci < ncand ? cidx[ci] : 0x7FFFFFFF;
-                        const bool in = j < send;
-                        const int nb = __popcll(ballot(in));
-                        if (nb == 0) break;
-                        cp += nb;
-                        const float sc = in ? st_sc[j] : 0.0F;
-                        const int32_t id = in ? st_id[j] : 0;
-                        const bool fullk = tkn == k;
-                        const float ks = fullk ? tks[k - 1] : 0.0F;
-                        const int32_t ki = fullk ? tki[k - 1] : 0;
-                        const bool cand = in && (!fullk || better(sc, id, ks, ki));
-                        int rt = 0;
-                        if (cand) {
-                            int hi = tkn;
-                            while (rt < hi) {
-                                const int mid = (rt + hi) >> 1;
-                                if (better(tks[mid], tki[mid], sc, id))
-                                    rt = mid + 1;
-                                else
-                                    hi = mid;
-                            }
-                        }
-                        const bool v = cand && !(rt < tkn && tki[rt] == id);
-                        const uint64_t vm = ballot(v);
-                        if (vm == 0) continue;
-                        int rc = 0;
-                        for (uint64_t m = vm; m != 0; m &= m - 1) {
-                            const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
-                            const float os = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sc), bl));
-                            const int32_t oi = __builtin_amdgcn_readlane(id, bl);
-                            rc += better(os, oi, sc, id) ? 1 : 0;
-                        }
-                        // top-k entries move up past the candidates better than
-                        // them, highest tile first (each tile is read whole before
-                        // any of it is written)
-                        for (int t0 = ((tkn - 1) >> 6) << 6; t0 >= 0 && tkn > 0; t0 -= 64) {
-                            const int t = t0 + lane;
-                            const bool live = t < tkn;
-                            const float es = live ? tks[t] : 0.0F;
-                            const int32_t ei = live ? tki[t] : 0;
-                            int up = 0;
-                            for (uint64_t m = vm; m != 0; m &= m - 1) {
-                                const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
-                                const float os = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sc), bl));
-                                const int32_t oi = __builtin_amdgcn_readlane(id, bl);
-                                up += better(os, oi, es, ei) ? 1 : 0;
-                            }
-                            __builtin_amdgcn_wave_barrier();
-                            if (live && up > 0 && t + up < k) {
-                                tks[t + up] = es;
-                                tki[t + up] = ei;
-                            }
-                            __builtin_amdgcn_wave_barrier();
-                        }
-                        if (v && rt + rc < k) {
-                            tks[rt + rc] = sc;
-                            tki[rt + rc] = id;
-                        }
-                        __builtin_amdgcn_wave_barrier();
-                        tkn = min(k, tkn + __popcll(vm));
+                    const int send = s.slot + s.count;
+                    for (int w = s.slot >> 6; w < (send + 63) >> 6; ++w) {
+                        const int lo = max(s.slot - w * 64, 0), hi = min(send - w * 64, 64);
+                        uint64_t bits = cmask[w] & (~0ULL << lo);
+                        if (hi < 64) bits &= (1ULL << hi) - 1;
+                        if (bits == 0) continue;
+                        merge(((bits >> lane) & 1ULL) != 0, w * 64 + lane);
                     }
                 }
                 if (lane == 0) {
@@ -1176,6 +1209,7 @@ LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq, bool dense) {
     P.o_wbase = take((P.budget + 128) * 4);
     P.o_winfirst = take(kWinMax * 2);
     P.o_queue = take(16);
+    P.o_cmask = take(P.budget / 64 * 8);
     P.total = o;
     return P;
 }
