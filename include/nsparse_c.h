@@ -144,7 +144,7 @@ int nsparse_search_device_stats(nsparse_index* index, int64_t n, const int32_t* 
  * 256-lane workgroups). Every choice computes identical results. Extension. */
 typedef struct nsparse_search_config {
     int32_t lookup;        /* 1: dense uint8 code table in LDS, 0: bitmap + rank table */
-    int32_t block;         /* lanes per query workgroup: 256 or 512 */
+    int32_t block;         /* lanes per query workgroup: 256, 512 or 1024 (1024: dense lookup only) */
     int32_t persistent;    /* 1: persistent workgroups pulling queries from a device queue */
     int32_t longest_first; /* persistent only: queries handed out by descending non-zeros */
 } nsparse_search_config;

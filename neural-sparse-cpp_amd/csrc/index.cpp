@@ -586,7 +586,7 @@ SearchPlan::SearchPlan(SeismicIndex& index, const SearchArgs& args, int k_max, c
         // every candidate schedule on this very batch: one warm-up, best of two
         std::vector<SearchConfig> cands;
         for (int lookup : {1, 0})
-            for (int block : {512, 256})
+            for (int block : {1024, 512, 256})
                 for (int sched : {0, 1, 2}) {
                     SearchConfig c;
                     c.lookup = lookup;

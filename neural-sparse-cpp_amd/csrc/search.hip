@@ -42,14 +42,18 @@ constexpr int kBlock = 256;  // exact-match, brute-force and default search work
 constexpr int kWaves = kBlock / 64;
 constexpr int kMaxWaves = 16;
 constexpr int kQMax = 512;    // longest query handled on device
-// doc slots per speculative chunk: 1024 with the dense code table (its 80 KB
-// LDS plan still fits two workgroups per CU), 512 with the bitmap lookup (four
-// workgroups per CU); LdsPlan::budget carries it
-constexpr int kBudgetDense = 1024;
-constexpr int kBudgetCompact = 512;
+// Doc slots per speculative chunk. Two chunks are staged at once (one is
+// scored while the next is gathered), 20 B per slot each plus a word-start
+// table: 512 slots with the dense code table keep its LDS plan within two
+// 512-lane workgroups per CU, 2048 fill one 1024-lane workgroup per CU, 256
+// with the bitmap lookup or FP32 rows stay within four workgroups per CU.
+// A multiple of 64 up to 4096 (the first-doc search reads every budget/64-th
+// word start with one lane each).
+__host__ __device__ constexpr int chunk_budget(bool dense, int block) {
+    return dense ? (block >= 1024 ? 2048 : 512) : 256;
+}
 constexpr int kSegMax = 64;   // clusters examined per chunk (one wave pass)
 constexpr int kTileFloats = 4096;
-constexpr int kWinMax = 512;      // 64-word windows per first-doc table pass
 // windows per wave per scoring step (loads in flight per lane); 4 keeps the
 // kernel within 128 VGPRs, i.e. four waves per SIMD
 template <bool DENSE>
@@ -57,16 +61,39 @@ constexpr int win_per_step() {
     return 4;
 }
 
+// A piece of one cluster's docs in a chunk. Cluster ids and positions fit 16
+// bits: the LDS plan holds 10 bytes per cluster, so a list has < 16384.
 struct Seg {
-    int32_t cluster;    // local cluster id
     int32_t doc_begin;  // offset into the list's docs
-    int32_t count;      // docs in this piece
-    int32_t slot;       // first staging slot
-    int32_t test;       // 1 if this piece starts the cluster (prune test runs here)
-    int32_t pos;        // position of the cluster in the visiting order
-    int32_t n_ok;       // docs that pass the selector (stats runs only)
-    int32_t nnz;        // their non-zeros (stats runs only)
-    int32_t pad;
+    uint16_t cluster;   // local cluster id
+    uint16_t pos;       // position of the cluster in the visiting order
+    uint16_t count;     // docs in this piece
+    uint16_t slot;      // first staging slot
+    uint16_t test;      // 1 if this piece starts the cluster (prune test runs here)
+    uint16_t pad;
+};
+
+// One staged chunk: per doc slot the doc id, the row start (block rows: first
+// word minus the doc's first chunk word), the row length (-1 when the
+// selector rejects the doc) and the score (uint32 sum for uint8 codes, float
+// for FP32); the word-start table (budget + 128 entries); the pieces; and
+// meta[kChunkMetaCount].
+struct Chunk {
+    int32_t* id;
+    int32_t* len;
+    int32_t* rs;
+    uint32_t* isc;
+    float* sc;
+    int32_t* wbase;
+    Seg* seg;
+    int32_t* meta;
+};
+enum ChunkMeta {
+    kChunkNseg = 0,  // pieces
+    kChunkTotal,     // staged doc slots
+    kChunkWords,     // 8-element words of the staged rows
+    kChunkNext,      // next window (or 64-slot group) to score
+    kChunkMetaCount
 };
 
 enum Misc {
@@ -77,8 +104,6 @@ enum Misc {
     kMiscPos,    // next cluster position
     kMiscPoff,   // docs of the cluster at kMiscPos already handed out
     kMiscEnd,    // clusters at positions >= this are never visited (first list break)
-    kMiscNseg,
-    kMiscTotal,  // staged docs in the chunk
     kMiscDone,   // list finished
     kMiscTkN,    // entries in the top-k
     kMiscBreakPending,
@@ -95,8 +120,7 @@ struct LdsPlan {
     int32_t kmax;      // top-k capacity
     int32_t uni_bytes; // union scratch (staging / sort buffer / FP32 tile)
     int32_t o_qbits, o_qpref, o_eterm, o_erank, o_eval, o_rlast, o_rval, o_rsum, o_rcol, o_flist, o_cuts;
-    int32_t o_cstart, o_score, o_order, o_uni, o_tks, o_tki, o_seg, o_misc, o_wsum, o_wbase, o_winfirst, o_queue,
-        o_cmask;
+    int32_t o_cstart, o_score, o_order, o_uni, o_tks, o_tki, o_seg, o_misc, o_wsum, o_queue, o_chunk, o_qstat;
     int32_t total;
 };
 
@@ -314,17 +338,25 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
     Seg* seg = reinterpret_cast<Seg*>(smem + P.o_seg);
     int32_t* misc = reinterpret_cast<int32_t*>(smem + P.o_misc);
     int32_t* wsum = reinterpret_cast<int32_t*>(smem + P.o_wsum);
-    int32_t* wbase = reinterpret_cast<int32_t*>(smem + P.o_wbase);
-    uint16_t* winfirst = reinterpret_cast<uint16_t*>(smem + P.o_winfirst);
-    uint64_t* cmask = reinterpret_cast<uint64_t*>(smem + P.o_cmask);  // candidate docs of the chunk, 64 per word
+    int32_t* cmeta = reinterpret_cast<int32_t*>(smem + P.o_chunk);
+    int64_t* qstat = reinterpret_cast<int64_t*>(smem + P.o_qstat);  // kStatFields
     uint8_t* qcode = smem;  // DENSE only: offset 0
-    // staging view of the union
-    int32_t* st_id = reinterpret_cast<int32_t*>(uni);
-    const int kBudget = P.budget;
-    int32_t* st_rs = st_id + kBudget;
-    int32_t* st_len = st_rs + kBudget;
-    float* st_sc = reinterpret_cast<float*>(st_len + kBudget);
-    uint32_t* st_isc = reinterpret_cast<uint32_t*>(st_len + kBudget);
+    // the two chunk staging buffers live in the union during the chunk loop;
+    // chunk i uses staging buffer i & 1 and piece list / meta slot i % 3
+    constexpr int kBud = chunk_budget(DENSE, BLOCK);
+    auto chunk_at = [&](int b, int m) {
+        int32_t* base = reinterpret_cast<int32_t*>(uni) + b * (5 * kBud + 128);
+        Chunk c;
+        c.id = base;
+        c.len = base + kBud;
+        c.rs = base + 2 * kBud;
+        c.isc = reinterpret_cast<uint32_t*>(base + 3 * kBud);
+        c.sc = reinterpret_cast<float*>(base + 3 * kBud);
+        c.wbase = base + 4 * kBud;
+        c.seg = seg + m * kSegMax;
+        c.meta = cmeta + m * kChunkMetaCount;
+        return c;
+    };
 
     const DevIndexView& ix = a.ix;
     const int64_t s0 = a.q_indptr[q];
@@ -350,6 +382,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
 
     // ---- query setup ----
     if (tid < kMiscCount) misc[tid] = 0;
+    if (tid < kStatFields) qstat[tid] = 0;
     for (int w = tid; w < P.nw; w += kBlock) qbits[w] = 0;
     for (int r = tid; r < qn; r += kBlock) {
         rlast[r] = -1;
@@ -431,15 +464,17 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
     }
     __syncthreads();
 
-    int64_t st_lists = 0, st_found = 0, st_csc = 0, st_tested = 0, st_opened = 0, st_read = 0, st_scored = 0,
-            st_nnz = 0, st_probe = 0;
-    int64_t tk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // Traversal counters and phase ticks (stats runs only) are kept in LDS by
+    // thread 0, which runs every counted step (wave 0 plans and replays).
     const bool timing = a.stats != nullptr;
+    auto stat_add = [&](int f, int64_t v) {
+        if (timing && tid == 0) qstat[f] += v;
+    };
     uint64_t t_last = timing ? wall_clock64() : 0;
     auto tick = [&](int slot) {
-        if (timing) {
+        if (timing && tid == 0) {
             const uint64_t now = wall_clock64();
-            tk[slot] += static_cast<int64_t>(now - t_last);
+            qstat[kStatTicksSetup + slot] += static_cast<int64_t>(now - t_last);
             t_last = now;
         }
     };
@@ -450,7 +485,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
     for (int ci = 0; ci < ncut; ++ci) {
         const uint32_t term = cuts[ci];
         if (term >= static_cast<uint32_t>(ix.n_lists)) continue;
-        ++st_lists;
+        stat_add(kStatLists, 1);
         const int32_t lb = ix.list_cluster_base[term];
         const int nc = ix.list_cluster_base[term + 1] - lb;
         if (nc == 0) {
@@ -465,7 +500,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
         for (int j = tid; j <= nc; j += kBlock) cstart[j] = static_cast<int32_t>(ix.cluster_doc_ptr[lb + j] - dbase);
         const int32_t tb = ix.list_term_base[term];
         const int32_t te = ix.list_term_base[term + 1];
-        if (te - tb > 1) st_probe += static_cast<int64_t>(qn) * 2 * (32 - __clz(te - tb - 1));
+        if (te - tb > 1) stat_add(kStatProbeBytes, static_cast<int64_t>(qn) * 2 * (32 - __clz(te - tb - 1)));
         for (int r = tid; r < nu; r += kBlock) rcol[r] = -1;
         if constexpr (SQ) {
             for (int c = tid; c < nc; c += kBlock) sacc[c] = 0;
@@ -538,10 +573,8 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
         }
         __syncthreads();
         const int nf = misc[kMiscNf];
-        if (a.stats != nullptr && tid == 0) {
-            st_found += misc[kMiscStatFound];
-            st_csc += misc[kMiscStatCsc];
-        }
+        stat_add(kStatTermsFound, misc[kMiscStatFound]);
+        stat_add(kStatCscEntries, misc[kMiscStatCsc]);
         if constexpr (SQ) {
             // The found columns' entries, flattened: one pass loads every
             // column's range and query-code sum, a prefix over their lengths
@@ -660,13 +693,23 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
         __syncthreads();
         tick(2);
 
-        // ---- speculative chunks + exact replay ----
-        while (true) {
-            if (wave == 0) {
-                const int pos = misc[kMiscPos], poff = misc[kMiscPoff], end = misc[kMiscEnd];
-                const int tkn = misc[kMiscTkN];
-                const bool full = tkn == k;
-                const float kth = full ? tks[k - 1] : 0.0F;
+        // ---- speculative chunks + exact replay, software-pipelined ----
+        // Chunk i is staged in buffer i & 1. While every wave scores chunk i
+        // (windows handed out by an LDS counter), wave 0 replays chunk i - 1
+        // exactly, plans chunk i + 1 against the k-th score that replay leaves,
+        // gathers and scans it, and then joins the scoring. A plan made against
+        // an older (lower) k-th score only takes more clusters, never fewer:
+        // the replay's exact test decides every visit, so results and counters
+        // are those of the sequential traversal.
+        auto plan_chunk = [&](const Chunk& C, int tkn_now) {
+            const bool full = tkn_now == k;
+            const float kth = full ? tks[k - 1] : 0.0F;
+            int pos = misc[kMiscPos], poff = misc[kMiscPoff], end = misc[kMiscEnd];
+            int nseg = 0, total = 0;
+            bool brk = false;
+            // passes that take no piece (every cluster pruned) go on until one
+            // does or the list is exhausted; poff > 0 implies pos < end
+            while (pos < end) {
                 const int p = pos + lane;
                 bool valid = p < end;
                 const int c = valid ? order[p] : 0;
@@ -685,135 +728,171 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                 }
                 const bool pass = valid && !fail;
                 const int cnt = pass ? ctot - (lane == 0 ? poff : 0) : 0;
-                const int incl = wave_incl_scan(cnt, lane);
+                const int incl = static_cast<int>(wave_incl_scan_dpp(static_cast<uint32_t>(cnt)));
                 const int excl = incl - cnt;
-                const bool stopper = pass && (excl + cnt > kBudget);
+                const bool stopper = pass && (excl + cnt > kBud);
                 const uint64_t sm = ballot(stopper);
                 const int stl = sm ? __ffsll(static_cast<unsigned long long>(sm)) - 1 : 64;
-                const bool partial = lane == stl && excl < kBudget;
+                const bool partial = lane == stl && excl < kBud;
                 const bool is_seg = pass && (lane < stl || partial);
                 const uint64_t segm = ballot(is_seg);
                 if (is_seg) {
-                    const int si = __popcll(segm & ((1ULL << lane) - 1));
                     Seg s;
-                    s.cluster = c;
                     s.doc_begin = cstart[c] + (lane == 0 ? poff : 0);
-                    s.count = partial ? kBudget - excl : cnt;
-                    s.slot = excl;
+                    s.cluster = static_cast<uint16_t>(c);
+                    s.pos = static_cast<uint16_t>(p);
+                    s.count = static_cast<uint16_t>(partial ? kBud - excl : cnt);
+                    s.slot = static_cast<uint16_t>(excl);
                     s.test = first_piece ? 1 : 0;
-                    s.pos = p;
-                    s.n_ok = 0;
-                    s.nnz = 0;
                     s.pad = 0;
-                    seg[si] = s;
+                    C.seg[__popcll(segm & ((1ULL << lane) - 1))] = s;
                 }
                 // consumed non-first-list failures were prune tests too
-                const uint64_t skipped = ballot(valid && fail && lane < stl);
+                stat_add(kStatClustersTested, __popcll(ballot(valid && fail && lane < stl)));
                 int new_pos, new_poff;
                 if (sm) {
                     new_pos = pos + stl;
                     const int before = (stl == 0 ? poff : 0);
-                    new_poff = excl < kBudget ? before + (kBudget - excl) : before;
+                    new_poff = excl < kBud ? before + (kBud - excl) : before;
                     // `excl`/`poff` above are lane stl's; broadcast them
                     new_poff = __shfl(new_poff, stl, 64);
                 } else {
                     new_pos = min(pos + 64, new_end);
                     new_poff = 0;
                 }
-                const int total = __shfl(incl, 63, 64);
-                if (lane == 0) {
-                    misc[kMiscNseg] = __popcll(segm);
-                    misc[kMiscTotal] = min(total, kBudget);
-                    misc[kMiscPos] = new_pos;
-                    misc[kMiscPoff] = new_poff;
-                    misc[kMiscEnd] = new_end;
-                    st_tested += __popcll(skipped);
-                    if (new_end != end) misc[kMiscBreakPending] = 1;
-                }
+                brk = brk || new_end != end;
+                pos = new_pos;
+                poff = new_poff;
+                end = new_end;
+                nseg = __popcll(segm);
+                total = min(__builtin_amdgcn_readlane(incl, 63), kBud);
+                if (nseg > 0) break;
             }
-            __syncthreads();
-            tick(3);
-            const int nseg = misc[kMiscNseg];
-            const int total = misc[kMiscTotal];
-            // ---- gather doc ids, selector, row ranges ----
-            for (int j = tid; j < total; j += kBlock) {
-                int lo = 0, hi = nseg - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (seg[mid].slot <= j)
-                        lo = mid;
-                    else
-                        hi = mid - 1;
-                }
-                const Seg& s = seg[lo];
-                const int64_t pp = dbase + s.doc_begin + (j - s.slot);
-                const int32_t d = ix.docs[pp];
-                st_id[j] = d;
-                const bool keep = sel == nullptr || ((sel[d >> 5] >> (d & 31)) & 1u);
-                int32_t len = -1;
-                if (keep && blk) {
-                    const int64_t ref = ix.row_ref[pp];
-                    st_rs[j] = static_cast<int32_t>(((ref >> 16) - lwb) << 3);
-                    len = static_cast<int32_t>(ref & 0xFFFF);
-                } else if (keep) {
-                    const int32_t r0 = ix.indptr[d];
-                    st_rs[j] = r0;
-                    len = ix.indptr[d + 1] - r0;
-                }
-                if (a.stats != nullptr && keep) {
-                    atomicAdd(&seg[lo].n_ok, 1);
-                    atomicAdd(&seg[lo].nnz, len);
-                }
-                if (keep) {
-                    st_len[j] = len;
-                } else {
-                    st_len[j] = -1;
-                }
+            if (lane == 0) {
+                misc[kMiscPos] = pos;
+                misc[kMiscPoff] = poff;
+                misc[kMiscEnd] = end;
+                if (brk) misc[kMiscBreakPending] = 1;
+                C.meta[kChunkNseg] = nseg;
+                C.meta[kChunkTotal] = nseg > 0 ? total : 0;
             }
-            __syncthreads();
-            tick(4);
-            // ---- score the staged rows ----
-            if constexpr (SQ) {
-                // Rows are read as aligned 8-element words (16 B of term ids + 8 B
-                // of codes). The chunk's words, doc after doc, form windows of 64
-                // words dealt round-robin to the waves, one word per lane, so
-                // every load is coalesced and each lane keeps kWinPerStep
-                // independent loads in flight. A window's lanes find their doc
-                // from the window's first doc and the doc boundaries inside the
-                // window (a wave-uniform loop). A DPP prefix sum S over the window
-                // then yields each doc run's sum: the last lane of a run adds S to
-                // its doc and subtracts S from the next run's doc (LDS integer
-                // atomics: exact in any order, modulo 2^32, and the true sums fit).
-                // Block-contiguous rows start on a word and are zero padded, so
-                // they need no masks; lanes without a word read the zero word
-                // past the end. Forward-index rows mask the bytes outside the row.
-                constexpr int kWinPerStep = win_per_step<DENSE>();
-                if (wave == 0) {
-                    int run = 0;
-                    for (int b = 0; b < total; b += 64) {
-                        const int j = b + lane;
-                        int nwj = 0;
-                        if (j < total) {
-                            const int len = st_len[j];
-                            if (len > 0) nwj = ((st_rs[j] + len + 7) >> 3) - (st_rs[j] >> 3);
-                            st_isc[j] = 0;
-                        }
-                        const int incl = wave_incl_scan(nwj, lane);
-                        if (j < total) {
-                            wbase[j] = run + incl - nwj;
-                            // block rows: the doc's first word minus its first chunk word
-                            if (blk) st_rs[j] = (st_rs[j] >> 3) - (run + incl - nwj);
-                        }
-                        run += __shfl(incl, 63, 64);
+            __builtin_amdgcn_wave_barrier();
+        };
+
+        // Doc ids, selector and row ranges of the chunk's slots, by threads
+        // t0, t0 + nt, ...; kUnroll slots per thread are resolved before their
+        // loads are issued. Out-of-range slots load the last slot's doc again.
+        // `overlap` runs exactly once, while the first round's doc-id and row
+        // loads are in flight (rounds are uniform over the threads); it must not
+        // touch the slots being gathered.
+        auto gather_slots = [&](auto unroll_tag, const Chunk& C, int t0, int nt, auto&& overlap) {
+            constexpr int kUnroll = decltype(unroll_tag)::value;
+            const int nseg = C.meta[kChunkNseg], total = C.meta[kChunkTotal];
+            if (total == 0) overlap();
+            for (int r0 = 0; r0 < total; r0 += nt * kUnroll) {
+                const int j0 = r0 + t0;
+                int64_t pp[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    const int j = min(j0 + u * nt, total - 1);
+                    int lo = 0, hi = nseg - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (C.seg[mid].slot <= j)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
                     }
-                    // wbase[total] = W; then sentinels above every word index (the
-                    // boundary search below reads up to wbase[total + 64])
-                    wbase[total + lane] = lane == 0 ? run : 0x7FFFFFFF;
-                    wbase[total + 64 + lane] = 0x7FFFFFFF;
+                    pp[u] = dbase + C.seg[lo].doc_begin + (j - C.seg[lo].slot);
                 }
-                __syncthreads();
-                const int W = wbase[total];
+                int32_t d[kUnroll];
+                int64_t ref[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    d[u] = ix.docs[pp[u]];
+                    ref[u] = blk ? ix.row_ref[pp[u]] : 0;
+                }
+                if (r0 == 0) overlap();
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    const int j = j0 + u * nt;
+                    const bool keep = sel == nullptr || ((sel[d[u] >> 5] >> (d[u] & 31)) & 1u);
+                    int32_t rs, len;
+                    if (blk) {
+                        rs = static_cast<int32_t>(((ref[u] >> 16) - lwb) << 3);
+                        len = static_cast<int32_t>(ref[u] & 0xFFFF);
+                    } else {
+                        rs = ix.indptr[d[u]];
+                        len = ix.indptr[d[u] + 1] - rs;
+                    }
+                    if (j < total) {
+                        C.id[j] = d[u];
+                        C.rs[j] = rs;
+                        C.len[j] = keep ? len : -1;
+                    }
+                }
+            }
+        };
+
+        // Word starts of the staged rows (wave 0). Rows are read as aligned
+        // 8-element words; wbase[j] is doc j's first word within the chunk,
+        // wbase[total] the chunk's word count, followed by sentinels above every
+        // word index (the boundary search reads up to wbase[total + 64]). Block
+        // rows keep, in rs, the doc's first word minus its first chunk word.
+        // Also clears the integer sums and the window counter.
+        auto scan_words = [&](const Chunk& C) {
+            const int total = C.meta[kChunkTotal];
+            int run = 0;
+            if constexpr (SQ) {
+                for (int j0 = 0; j0 < total; j0 += 64) {
+                    const int j = j0 + lane;
+                    int nwj = 0;
+                    if (j < total) {
+                        const int len = C.len[j];
+                        if (len > 0) nwj = ((C.rs[j] + len + 7) >> 3) - (C.rs[j] >> 3);
+                        C.isc[j] = 0;
+                    }
+                    const int incl = static_cast<int>(wave_incl_scan_dpp(static_cast<uint32_t>(nwj)));
+                    if (j < total) {
+                        C.wbase[j] = run + incl - nwj;
+                        if (blk) C.rs[j] = (C.rs[j] >> 3) - (run + incl - nwj);
+                    }
+                    run += __builtin_amdgcn_readlane(incl, 63);
+                }
+                C.wbase[total + lane] = lane == 0 ? run : 0x7FFFFFFF;
+                C.wbase[total + 64 + lane] = 0x7FFFFFFF;
+            }
+            if (lane == 0) {
+                C.meta[kChunkWords] = run;
+                C.meta[kChunkNext] = 0;
+            }
+            __builtin_amdgcn_wave_barrier();
+        };
+
+        // ---- score the staged rows (every wave) ----
+        auto score_chunk = [&](const Chunk& C) {
+            const int total = C.meta[kChunkTotal];
+            if constexpr (SQ) {
+                // The chunk's words, doc after doc, form windows of 64 words, one
+                // word per lane, so every load is coalesced; a wave takes
+                // kWinPerStep consecutive windows per step from the counter and
+                // keeps their loads in flight together. A window's first doc is
+                // found with two ballots over wbase (every kBud/64-th start, then
+                // the kBud/64 starts after the last one <= the window's first
+                // word); its lanes then find their doc from the doc boundaries
+                // inside the window (a wave-uniform loop). A DPP prefix sum S
+                // over the window yields each doc run's sum: the last lane of a
+                // run adds S to its doc and subtracts S from the next run's doc
+                // (LDS integer atomics: exact in any order, modulo 2^32, and the
+                // true sums fit). Block-contiguous rows start on a word and are
+                // zero padded, so they need no masks; lanes without a word read
+                // the zero word past the end. Forward-index rows mask the bytes
+                // outside the row.
+                constexpr int kWinPerStep = win_per_step<DENSE>();
+                constexpr int kS1 = kBud / 64;
+                const int W = C.meta[kChunkWords];
                 const int nwin = (W + 63) >> 6;
+                const int32_t* wbase = C.wbase;
                 auto score_windows = [&](auto blk_tag) {
                     constexpr bool BLK = decltype(blk_tag)::value;
                     const uint4* idx8 = BLK ? reinterpret_cast<const uint4*>(ix.blk_indices) + lwb
@@ -821,103 +900,103 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                     const uint2* val8 = BLK ? reinterpret_cast<const uint2*>(ix.blk_codes) + lwb
                                             : reinterpret_cast<const uint2*>(ix.values);
                     const int64_t zero_word = BLK ? ix.blk_words - lwb : 0;
-                    for (int x0 = 0; x0 < nwin; x0 += kWinMax) {
-                        const int x1 = min(nwin, x0 + kWinMax);
-                        // doc j holds the first word of window x when 64x is in [wbase[j], wbase[j+1])
-                        for (int j = tid; j < total; j += kBlock) {
-                            const int xa = max((wbase[j] + 63) >> 6, x0), xb = min((wbase[j + 1] + 63) >> 6, x1);
-                            for (int x = xa; x < xb; ++x) winfirst[x - x0] = static_cast<uint16_t>(j);
-                        }
-                        __syncthreads();
-                        for (int xs = x0 + wave; xs < x1; xs += kWaves * kWinPerStep) {
-                            int jj[kWinPerStep], j1s[kWinPerStep], lo[kWinPerStep], hi[kWinPerStep];
-                            int64_t ww[kWinPerStep];
-                            uint4 ii[kWinPerStep];
-                            uint2 vv[kWinPerStep];
-                            // 1. each lane's doc and word address
+                    while (true) {
+                        int xs = 0;
+                        if (lane == 0) xs = atomicAdd(&C.meta[kChunkNext], kWinPerStep);
+                        xs = __shfl(xs, 0, 64);
+                        if (xs >= nwin) break;
+                        int jj[kWinPerStep], j1s[kWinPerStep], lo[kWinPerStep], hi[kWinPerStep];
+                        int64_t ww[kWinPerStep];
+                        uint4 ii[kWinPerStep];
+                        uint2 vv[kWinPerStep];
+                        // 1. each lane's doc and word address
 #pragma unroll
-                            for (int u = 0; u < kWinPerStep; ++u) {
-                                const int x = xs + u * kWaves;
-                                jj[u] = -1;
-                                j1s[u] = -1;
-                                ww[u] = zero_word;
-                                lo[u] = 0;
-                                hi[u] = 8;
-                                if (x >= x1) continue;  // wave-uniform
-                                const int F = x << 6, f = F + lane;
-                                int j = winfirst[x - x0], j1 = j;
-                                for (int base = j + 1;; base += 64) {
-                                    const int bn = wbase[base + lane];
-                                    const uint64_t bm = ballot(bn <= F + 63);
-                                    const int nb = __popcll(bm);
-                                    for (int i = 0; i < nb; ++i) {
-                                        const int B = __builtin_amdgcn_readlane(bn, i);
-                                        j += f >= B;
-                                        j1 += f + 1 >= B;
-                                    }
-                                    if (nb < 64) break;
+                        for (int u = 0; u < kWinPerStep; ++u) {
+                            const int x = xs + u;
+                            jj[u] = -1;
+                            j1s[u] = -1;
+                            ww[u] = zero_word;
+                            lo[u] = 0;
+                            hi[u] = 8;
+                            if (x >= nwin) continue;  // wave-uniform
+                            const int F = x << 6, f = F + lane;
+                            // the window's first doc: the last j < total with wbase[j] <= F
+                            const int i1 = lane * kS1;
+                            const uint64_t m1 = ballot(i1 < total && wbase[i1] <= F);
+                            const int sb = (__popcll(m1) - 1) * kS1;
+                            const int i2 = sb + lane;
+                            const uint64_t m2 = ballot(lane < kS1 && i2 < total && wbase[i2] <= F);
+                            int j = sb + __popcll(m2) - 1, j1 = j;
+                            for (int base = j + 1;; base += 64) {
+                                const int bn = wbase[base + lane];
+                                const uint64_t bm = ballot(bn <= F + 63);
+                                const int nb = __popcll(bm);
+                                for (int i = 0; i < nb; ++i) {
+                                    const int B = __builtin_amdgcn_readlane(bn, i);
+                                    j += f >= B;
+                                    j1 += f + 1 >= B;
                                 }
-                                if (f < W) {
-                                    jj[u] = j;
-                                    // run end (the next word is another doc's, or the window
-                                    // ends): j1s = the doc whose run follows in this window,
-                                    // INT32_MAX when none does, -1 when not a run end
-                                    const bool end = j1 != j || lane == 63;
-                                    j1s[u] = !end ? -1 : (j1 != j && f + 1 < W && lane != 63) ? j1 : INT32_MAX;
-                                    if constexpr (BLK) {
-                                        ww[u] = f + st_rs[j];
-                                    } else {
-                                        const int rs = st_rs[j], re = rs + st_len[j], wb = wbase[j];
-                                        const int w = (rs >> 3) + (f - wb);
-                                        ww[u] = w;
-                                        lo[u] = max(rs - (w << 3), 0);
-                                        hi[u] = min(re - (w << 3), 8);
-                                    }
-                                }
+                                if (nb < 64) break;
                             }
-                            // 2. all loads of the step back to back, without branches
-#pragma unroll
-                            for (int u = 0; u < kWinPerStep; ++u) {
-                                ii[u] = idx8[ww[u]];
-                                vv[u] = val8[ww[u]];
-                            }
-                            // 3. scores
-#pragma unroll
-                            for (int u = 0; u < kWinPerStep; ++u) {
-                                if (xs + u * kWaves >= x1) continue;  // wave-uniform
-                                uint64_t codes = (static_cast<uint64_t>(vv[u].y) << 32) | vv[u].x;
-                                if constexpr (!BLK) {
-                                    // bytes outside the doc's [rs, re) -- and every byte of a
-                                    // lane without a word -- are masked; their term ids are
-                                    // still valid ids, so the lookups stay in bounds
-                                    const bool has = jj[u] >= 0;
-                                    const uint64_t mlo = has ? (~0ULL << (8 * lo[u])) : 0ULL;
-                                    const uint64_t mhi = hi[u] < 8 ? ((1ULL << (8 * hi[u])) - 1) : ~0ULL;
-                                    codes &= mlo & mhi;
-                                }
-                                const uint32_t tw[4] = {ii[u].x, ii[u].y, ii[u].z, ii[u].w};
-                                uint32_t acc = 0;
-#pragma unroll
-                                for (int t = 0; t < 8; ++t) {
-                                    const uint32_t term = (t & 1) ? (tw[t >> 1] >> 16) : (tw[t >> 1] & 0xFFFFu);
-                                    const uint32_t c = static_cast<uint32_t>(codes >> (8 * t)) & 0xFFu;
-                                    if constexpr (DENSE) {
-                                        acc += c * static_cast<uint32_t>(qcode[term]);
-                                    } else {
-                                        bool pr;
-                                        const int r = QL.rank_of(term, pr);
-                                        const uint32_t qv = static_cast<uint32_t>(rval[r]);
-                                        acc += pr ? c * qv : 0U;
-                                    }
-                                }
-                                const uint32_t S = wave_incl_scan_dpp(acc);
-                                if (j1s[u] >= 0 && S != 0) {
-                                    atomicAdd(&st_isc[jj[u]], S);
-                                    if (j1s[u] != INT32_MAX) atomicSub(&st_isc[j1s[u]], S);
+                            if (f < W) {
+                                jj[u] = j;
+                                // run end (the next word is another doc's, or the window
+                                // ends): j1s = the doc whose run follows in this window,
+                                // INT32_MAX when none does, -1 when not a run end
+                                const bool run_end = j1 != j || lane == 63;
+                                j1s[u] = !run_end ? -1 : (j1 != j && f + 1 < W && lane != 63) ? j1 : INT32_MAX;
+                                if constexpr (BLK) {
+                                    ww[u] = f + C.rs[j];
+                                } else {
+                                    const int rs = C.rs[j], re = rs + C.len[j], wb = wbase[j];
+                                    const int w = (rs >> 3) + (f - wb);
+                                    ww[u] = w;
+                                    lo[u] = max(rs - (w << 3), 0);
+                                    hi[u] = min(re - (w << 3), 8);
                                 }
                             }
                         }
-                        __syncthreads();
+                        // 2. all loads of the step back to back, without branches
+#pragma unroll
+                        for (int u = 0; u < kWinPerStep; ++u) {
+                            ii[u] = idx8[ww[u]];
+                            vv[u] = val8[ww[u]];
+                        }
+                        // 3. scores
+#pragma unroll
+                        for (int u = 0; u < kWinPerStep; ++u) {
+                            if (xs + u >= nwin) continue;  // wave-uniform
+                            uint64_t codes = (static_cast<uint64_t>(vv[u].y) << 32) | vv[u].x;
+                            if constexpr (!BLK) {
+                                // bytes outside the doc's [rs, re) -- and every byte of a
+                                // lane without a word -- are masked; their term ids are
+                                // still valid ids, so the lookups stay in bounds
+                                const bool has = jj[u] >= 0;
+                                const uint64_t mlo = has ? (~0ULL << (8 * lo[u])) : 0ULL;
+                                const uint64_t mhi = hi[u] < 8 ? ((1ULL << (8 * hi[u])) - 1) : ~0ULL;
+                                codes &= mlo & mhi;
+                            }
+                            const uint32_t tw[4] = {ii[u].x, ii[u].y, ii[u].z, ii[u].w};
+                            uint32_t acc = 0;
+#pragma unroll
+                            for (int t = 0; t < 8; ++t) {
+                                const uint32_t term = (t & 1) ? (tw[t >> 1] >> 16) : (tw[t >> 1] & 0xFFFFu);
+                                const uint32_t c = static_cast<uint32_t>(codes >> (8 * t)) & 0xFFu;
+                                if constexpr (DENSE) {
+                                    acc += c * static_cast<uint32_t>(qcode[term]);
+                                } else {
+                                    bool pr;
+                                    const int r = QL.rank_of(term, pr);
+                                    const uint32_t qv = static_cast<uint32_t>(rval[r]);
+                                    acc += pr ? c * qv : 0U;
+                                }
+                            }
+                            const uint32_t S = wave_incl_scan_dpp(acc);
+                            if (j1s[u] >= 0 && S != 0) {
+                                atomicAdd(&C.isc[jj[u]], S);
+                                if (j1s[u] != INT32_MAX) atomicSub(&C.isc[j1s[u]], S);
+                            }
+                        }
                     }
                 };
                 if (blk)
@@ -925,186 +1004,227 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                 else
                     score_windows(std::false_type{});
             } else {
+                // FP32 rows: one doc per lane, 64-slot groups from the counter;
+                // each row is summed in order (as the reference's loop does)
                 const float* vf = static_cast<const float*>(ix.values);
-                for (int j = tid; j < total; j += kBlock) {
-                    const int len = st_len[j];
+                while (true) {
+                    int g = 0;
+                    if (lane == 0) g = atomicAdd(&C.meta[kChunkNext], 1);
+                    g = __shfl(g, 0, 64);
+                    if (g * 64 >= total) break;
+                    const int j = g * 64 + lane;
+                    const int len = j < total ? C.len[j] : -1;
                     if (len < 0) continue;
-                    const int32_t rs = st_rs[j];
+                    const int32_t rs = C.rs[j];
                     float acc = 0.0F;
                     for (int pp = rs; pp < rs + len; ++pp) acc += vf[pp] * QL.value(ix.indices[pp]);
-                    st_sc[j] = acc;
+                    C.sc[j] = acc;
                 }
-                __syncthreads();
             }
-            // Candidates for the replay: docs that beat the k-th score as it stands
-            // at the start of the chunk (it only rises, so no other doc can enter),
-            // one ballot mask per 64 staging slots. uint8 sums become float here.
-            {
-                const bool full0 = misc[kMiscTkN] == k;
-                const float ks0 = full0 ? tks[k - 1] : 0.0F;
-                const int32_t ki0 = full0 ? tki[k - 1] : 0;
-                for (int j0 = wave * 64; j0 < total; j0 += kBlock) {
-                    const int j = j0 + lane;
-                    bool c = false;
-                    if (j < total) {
-                        float sc;
-                        if constexpr (SQ) {
-                            sc = static_cast<float>(st_isc[j]);
-                            st_sc[j] = sc;
-                        } else {
-                            sc = st_sc[j];
-                        }
-                        c = st_len[j] >= 0 && (!full0 || better(sc, st_id[j], ks0, ki0));
+        };
+
+        // ---- exact replay of a scored chunk (wave 0) ----
+        // Segments are replayed in visiting order: the prune test against the
+        // exact current k-th, then the segment's docs are merged into the
+        // sorted top-k one 64-slot word at a time, lane l holding slot 64w + l.
+        // Inserting a batch one by one with the reference's reject rule keeps
+        // the k best of (top-k U batch) under (score desc, doc id asc) -- an
+        // order-free result -- so the batch is merged in parallel: each
+        // candidate's rank among the top-k (ballots against the top-k held in
+        // registers, or a binary search when it has more than 128 entries; an
+        // equal id is the same doc seen in an earlier list) plus its rank
+        // within the batch. A cluster split over chunks is tested at its first
+        // piece; when that test fails (on a later list) its other pieces are
+        // skipped too (`cluster_failed` carries the outcome across chunks).
+        // Returns true at a first-list break.
+        bool cluster_failed = false;
+        auto replay_chunk = [&](const Chunk& C, int& tkn) -> bool {
+            auto merge = [&](bool in, int j) {
+                float sc = 0.0F;
+                if (in) {
+                    if constexpr (SQ)
+                        sc = static_cast<float>(C.isc[j]);
+                    else
+                        sc = C.sc[j];
+                }
+                const int32_t id = in ? C.id[j] : 0;
+                const bool fullk = tkn == k;
+                const float ks = fullk ? tks[k - 1] : 0.0F;
+                const int32_t ki = fullk ? tki[k - 1] : 0;
+                const bool cand = in && (!fullk || better(sc, id, ks, ki));
+                const uint64_t cm = ballot(cand);
+                if (cm == 0) return;
+                const auto lane_f = [](float v, int l) {
+                    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+                };
+                int rt = 0;
+                bool dup = false;
+                const bool small = tkn <= 128;
+                const bool l0 = lane < tkn, l1 = lane + 64 < tkn;
+                float e0s = 0.0F, e1s = 0.0F;
+                int32_t e0i = 0, e1i = 0;
+                if (small) {
+                    if (l0) {
+                        e0s = tks[lane];
+                        e0i = tki[lane];
                     }
-                    const uint64_t m = ballot(c);
-                    if (lane == 0) cmask[j0 >> 6] = m;
+                    if (l1) {
+                        e1s = tks[lane + 64];
+                        e1i = tki[lane + 64];
+                    }
+                    for (uint64_t m = cm; m != 0; m &= m - 1) {
+                        const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
+                        const float cs = lane_f(sc, bl);
+                        const int32_t ci = __builtin_amdgcn_readlane(id, bl);
+                        const int cnt = __popcll(ballot(l0 && better(e0s, e0i, cs, ci))) +
+                                        __popcll(ballot(l1 && better(e1s, e1i, cs, ci)));
+                        const bool dp = ballot((l0 && e0i == ci) || (l1 && e1i == ci)) != 0;
+                        if (lane == bl) {
+                            rt = cnt;
+                            dup = dp;
+                        }
+                    }
+                } else if (cand) {
+                    int hi = tkn;
+                    while (rt < hi) {
+                        const int mid = (rt + hi) >> 1;
+                        if (better(tks[mid], tki[mid], sc, id))
+                            rt = mid + 1;
+                        else
+                            hi = mid;
+                    }
+                    dup = rt < tkn && tki[rt] == id;
                 }
-            }
-            __syncthreads();
-            tick(5);
-            // ---- exact replay of the chunk (wave 0) ----
-            // Segments are replayed in visiting order: the prune test against the
-            // exact current k-th, then the segment's candidates (the masks above)
-            // are merged into the sorted top-k one 64-slot word at a time, lane l
-            // holding slot 64w + l. Inserting a batch one by one with the
-            // reference's reject rule keeps the k best of (top-k U batch) under
-            // (score desc, doc id asc) -- an order-free result -- so the batch is
-            // merged in parallel: each candidate's rank among the top-k (ballots
-            // against the top-k held in registers, or a binary search when it has
-            // more than 128 entries; an equal id is the same doc seen in an
-            // earlier list) plus its rank within the batch.
-            if (wave == 0) {
-                int tkn = misc[kMiscTkN];
-                bool done = false;
-                auto merge = [&](bool in, int j) {
-                    const float sc = in ? st_sc[j] : 0.0F;
-                    const int32_t id = in ? st_id[j] : 0;
-                    const bool fullk = tkn == k;
-                    const float ks = fullk ? tks[k - 1] : 0.0F;
-                    const int32_t ki = fullk ? tki[k - 1] : 0;
-                    const bool cand = in && (!fullk || better(sc, id, ks, ki));
-                    const uint64_t cm = ballot(cand);
-                    if (cm == 0) return;
-                    const auto lane_f = [](float v, int l) {
-                        return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
-                    };
-                    int rt = 0;
-                    bool dup = false;
-                    const bool small = tkn <= 128;
-                    const bool l0 = lane < tkn, l1 = lane + 64 < tkn;
-                    float e0s = 0.0F, e1s = 0.0F;
-                    int32_t e0i = 0, e1i = 0;
+                const bool v = cand && !dup;
+                const uint64_t vm = ballot(v);
+                if (vm == 0) return;
+                int rc = 0;
+                for (uint64_t m = vm; m != 0; m &= m - 1) {
+                    const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
+                    rc += better(lane_f(sc, bl), __builtin_amdgcn_readlane(id, bl), sc, id) ? 1 : 0;
+                }
+                // top-k entries move up past the candidates better than them,
+                // highest tile first (a tile is read whole before it is written)
+                for (int t0 = ((tkn - 1) >> 6) << 6; t0 >= 0 && tkn > 0; t0 -= 64) {
+                    const int t = t0 + lane;
+                    const bool live = t < tkn;
+                    float es;
+                    int32_t ei;
                     if (small) {
-                        if (l0) {
-                            e0s = tks[lane];
-                            e0i = tki[lane];
-                        }
-                        if (l1) {
-                            e1s = tks[lane + 64];
-                            e1i = tki[lane + 64];
-                        }
-                        for (uint64_t m = cm; m != 0; m &= m - 1) {
-                            const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
-                            const float cs = lane_f(sc, bl);
-                            const int32_t ci = __builtin_amdgcn_readlane(id, bl);
-                            const int cnt = __popcll(ballot(l0 && better(e0s, e0i, cs, ci))) +
-                                            __popcll(ballot(l1 && better(e1s, e1i, cs, ci)));
-                            const bool dp = ballot((l0 && e0i == ci) || (l1 && e1i == ci)) != 0;
-                            if (lane == bl) {
-                                rt = cnt;
-                                dup = dp;
-                            }
-                        }
-                    } else if (cand) {
-                        int hi = tkn;
-                        while (rt < hi) {
-                            const int mid = (rt + hi) >> 1;
-                            if (better(tks[mid], tki[mid], sc, id))
-                                rt = mid + 1;
-                            else
-                                hi = mid;
-                        }
-                        dup = rt < tkn && tki[rt] == id;
+                        es = t0 == 0 ? e0s : e1s;
+                        ei = t0 == 0 ? e0i : e1i;
+                    } else {
+                        es = live ? tks[t] : 0.0F;
+                        ei = live ? tki[t] : 0;
                     }
-                    const bool v = cand && !dup;
-                    const uint64_t vm = ballot(v);
-                    if (vm == 0) return;
-                    int rc = 0;
+                    int up = 0;
                     for (uint64_t m = vm; m != 0; m &= m - 1) {
                         const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
-                        rc += better(lane_f(sc, bl), __builtin_amdgcn_readlane(id, bl), sc, id) ? 1 : 0;
-                    }
-                    // top-k entries move up past the candidates better than them,
-                    // highest tile first (a tile is read whole before it is written)
-                    for (int t0 = ((tkn - 1) >> 6) << 6; t0 >= 0 && tkn > 0; t0 -= 64) {
-                        const int t = t0 + lane;
-                        const bool live = t < tkn;
-                        float es;
-                        int32_t ei;
-                        if (small) {
-                            es = t0 == 0 ? e0s : e1s;
-                            ei = t0 == 0 ? e0i : e1i;
-                        } else {
-                            es = live ? tks[t] : 0.0F;
-                            ei = live ? tki[t] : 0;
-                        }
-                        int up = 0;
-                        for (uint64_t m = vm; m != 0; m &= m - 1) {
-                            const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
-                            up += better(lane_f(sc, bl), __builtin_amdgcn_readlane(id, bl), es, ei) ? 1 : 0;
-                        }
-                        __builtin_amdgcn_wave_barrier();
-                        if (live && up > 0 && t + up < k) {
-                            tks[t + up] = es;
-                            tki[t + up] = ei;
-                        }
-                        __builtin_amdgcn_wave_barrier();
-                    }
-                    if (v && rt + rc < k) {
-                        tks[rt + rc] = sc;
-                        tki[rt + rc] = id;
+                        up += better(lane_f(sc, bl), __builtin_amdgcn_readlane(id, bl), es, ei) ? 1 : 0;
                     }
                     __builtin_amdgcn_wave_barrier();
-                    tkn = min(k, tkn + __popcll(vm));
-                };
-                for (int si = 0; si < nseg && !done; ++si) {
-                    const Seg s = seg[si];
-                    if (s.test) {
-                        ++st_tested;
-                        if (tkn == k && score[s.cluster] * hf < tks[k - 1]) {
-                            if (first_list) {
-                                done = true;
-                                break;
-                            }
-                            continue;
-                        }
-                        ++st_opened;
+                    if (live && up > 0 && t + up < k) {
+                        tks[t + up] = es;
+                        tki[t + up] = ei;
                     }
-                    st_read += s.count;
-                    st_scored += s.n_ok;
-                    st_nnz += s.nnz;
-                    const int send = s.slot + s.count;
-                    for (int w = s.slot >> 6; w < (send + 63) >> 6; ++w) {
-                        const int lo = max(s.slot - w * 64, 0), hi = min(send - w * 64, 64);
-                        uint64_t bits = cmask[w] & (~0ULL << lo);
-                        if (hi < 64) bits &= (1ULL << hi) - 1;
-                        if (bits == 0) continue;
-                        merge(((bits >> lane) & 1ULL) != 0, w * 64 + lane);
-                    }
+                    __builtin_amdgcn_wave_barrier();
                 }
-                if (lane == 0) {
-                    misc[kMiscTkN] = tkn;
-                    if (done) {
-                        misc[kMiscDone] = 1;
-                    } else if (misc[kMiscPos] >= misc[kMiscEnd] && misc[kMiscPoff] == 0) {
-                        misc[kMiscDone] = 1;
-                        if (misc[kMiscBreakPending]) ++st_tested;
+                if (v && rt + rc < k) {
+                    tks[rt + rc] = sc;
+                    tki[rt + rc] = id;
+                }
+                __builtin_amdgcn_wave_barrier();
+                tkn = min(k, tkn + __popcll(vm));
+            };
+            const int nseg = C.meta[kChunkNseg];
+            for (int si = 0; si < nseg; ++si) {
+                const Seg s = C.seg[si];
+                if (s.test) {
+                    stat_add(kStatClustersTested, 1);
+                    cluster_failed = tkn == k && score[s.cluster] * hf < tks[k - 1];
+                    if (cluster_failed) {
+                        if (first_list) return true;
+                        continue;
                     }
+                    stat_add(kStatClustersOpened, 1);
+                } else if (cluster_failed) {
+                    continue;
+                }
+                stat_add(kStatDocsRead, s.count);
+                const int send = s.slot + s.count;
+                for (int w = s.slot >> 6; w < (send + 63) >> 6; ++w) {
+                    const int j = w * 64 + lane;
+                    const int len = (j >= s.slot && j < send) ? C.len[j] : -1;
+                    if (timing) {
+                        stat_add(kStatDocsScored, __popcll(ballot(len >= 0)));
+                        const uint32_t incl = wave_incl_scan_dpp(static_cast<uint32_t>(max(len, 0)));
+                        stat_add(kStatNnzScored,
+                                 static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63)));
+                    }
+                    merge(len >= 0, j);
                 }
             }
+            return false;
+        };
+
+        // prologue: chunk 0 planned by wave 0, gathered by every wave
+        {
+            const Chunk C0 = chunk_at(0, 0);
+            if (wave == 0) plan_chunk(C0, misc[kMiscTkN]);
             __syncthreads();
-            tick(6);
-            if (misc[kMiscDone]) break;
+            tick(3);
+            gather_slots(std::integral_constant<int, (kBud + kBlock - 1) / kBlock>{}, C0, tid, kBlock, [] {});
+            __syncthreads();
+            if (wave == 0) scan_words(C0);
+            __syncthreads();
+            tick(4);
+        }
+        // Iteration i scores chunk i with every wave. Wave 0 first plans chunk
+        // i + 1 (against the k-th score after chunk i - 2's replay), issues its
+        // gather loads, replays chunk i - 1 while they are in flight, then
+        // stages and scans chunk i + 1 and joins the scoring.
+        bool have_prev = false;
+        for (int cb = 0, cm = 0;; cb ^= 1, cm = cm == 2 ? 0 : cm + 1) {
+            const Chunk cur = chunk_at(cb, cm);
+            const bool cur_live = cur.meta[kChunkNseg] > 0;
+            if (wave == 0) {
+                const Chunk next = chunk_at(cb ^ 1, cm == 2 ? 0 : cm + 1);
+                const Chunk prev = chunk_at(cb ^ 1, cm == 0 ? 2 : cm - 1);
+                int tkn = misc[kMiscTkN];
+                bool done = false;
+                auto replay_prev = [&] {
+                    if (!have_prev) return;
+                    tick(4);
+                    done = replay_chunk(prev, tkn);
+                    if (lane == 0) {
+                        misc[kMiscTkN] = tkn;
+                        if (done) misc[kMiscDone] = 1;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    tick(6);
+                };
+                if (cur_live) {
+                    plan_chunk(next, tkn);
+                    tick(3);
+                    gather_slots(std::integral_constant<int, (kBud / 64 < 8 ? kBud / 64 : 8)>{}, next, lane, 64,
+                                 replay_prev);
+                    scan_words(next);
+                    tick(4);
+                } else {
+                    replay_prev();
+                }
+            }
+            score_chunk(cur);
+            __syncthreads();
+            tick(5);
+            if (misc[kMiscDone]) break;  // first-list break found by the replay
+            if (!cur_live) {
+                // every planned chunk is replayed; a break found by the planner
+                // is the prune test at the position where the visit stops
+                if (misc[kMiscBreakPending]) stat_add(kStatClustersTested, 1);
+                break;
+            }
+            have_prev = true;
         }
         first_list = false;
         __syncthreads();
@@ -1122,18 +1242,9 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
         out_d[i] = d;
         out_l[i] = l;
     }
-    if (a.stats != nullptr && tid == 0) {
+    if (timing && tid == 0) {
         int64_t* s = a.stats + q * kStatFields;
-        s[kStatLists] = st_lists;
-        s[kStatTermsFound] = st_found;
-        s[kStatCscEntries] = st_csc;
-        s[kStatClustersTested] = st_tested;
-        s[kStatClustersOpened] = st_opened;
-        s[kStatDocsRead] = st_read;
-        s[kStatDocsScored] = st_scored;
-        s[kStatNnzScored] = st_nnz;
-        s[kStatProbeBytes] = st_probe;
-        for (int i = 0; i < 7; ++i) s[kStatTicksSetup + i] = tk[i];
+        for (int f = 0; f < kStatFields; ++f) s[f] = qstat[f];
     }
 }
 
@@ -1169,13 +1280,13 @@ __global__ __launch_bounds__(BLOCK, 4) void seismic_search_persistent_kernel(Sea
 
 namespace {
 
-LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq, bool dense) {
+LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq, bool dense, int block) {
     LdsPlan P{};
     P.nw = (dim + 31) / 32;
     P.mc = max(max_clusters, 1);
     P.kmax = kmax;
-    P.budget = dense ? kBudgetDense : kBudgetCompact;
-    const int staging = P.budget * 16;
+    P.budget = chunk_budget(dense, block);
+    const int staging = 2 * (5 * P.budget + 128) * 4;  // two chunk buffers
     const int sortbuf = P.mc * 12;  // KeyItem[mc] + two uint16[mc] position lists
     const int tile = sq ? 0 : kTileFloats * 4;
     P.uni_bytes = max(max(staging, sortbuf), max(tile, kQMax * 8));
@@ -1203,13 +1314,12 @@ LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq, bool dense) {
     P.o_uni = take(P.uni_bytes);
     P.o_tks = take(kmax * 4);
     P.o_tki = take(kmax * 4);
-    P.o_seg = take(kSegMax * static_cast<int>(sizeof(Seg)));
+    P.o_seg = take(3 * kSegMax * static_cast<int>(sizeof(Seg)));
     P.o_misc = take(kMiscCount * 4);
     P.o_wsum = take(kMaxWaves * 4);
-    P.o_wbase = take((P.budget + 128) * 4);
-    P.o_winfirst = take(kWinMax * 2);
     P.o_queue = take(16);
-    P.o_cmask = take(P.budget / 64 * 8);
+    P.o_chunk = take(3 * kChunkMetaCount * 4);
+    P.o_qstat = take(kStatFields * 8);
     P.total = o;
     return P;
 }
@@ -1383,7 +1493,7 @@ void launch_exact_scores(const int32_t* indptr, const uint16_t* indices, const f
 
 int search_lds_bytes(int dim, int max_clusters, int kmax, bool sq) {
     const SearchConfig c = resolve_search_config(dim, max_clusters, kmax, sq, default_search_config(sq));
-    return make_plan(dim, max_clusters, kmax, sq, c.lookup == 1).total;
+    return make_plan(dim, max_clusters, kmax, sq, c.lookup == 1, c.block).total;
 }
 
 std::string device_error_message(int32_t code) {
@@ -1484,7 +1594,7 @@ SearchConfig default_search_config(bool sq) {
     if (sq && e != nullptr && std::string(e) == "bitmap") c.lookup = 0;
     c.block = sq && c.lookup == 1 ? 512 : 256;
     const char* eb = std::getenv("NSPARSE_SEARCH_BLOCK");
-    if (sq && eb != nullptr) c.block = std::string(eb) == "512" ? 512 : 256;
+    if (sq && eb != nullptr) c.block = std::string(eb) == "1024" ? 1024 : std::string(eb) == "512" ? 512 : 256;
     c.persistent = 0;
     c.longest_first = 0;
     return c;
@@ -1495,9 +1605,15 @@ SearchConfig resolve_search_config(int dim, int max_clusters, int kmax, bool sq,
         c.lookup = 0;
         c.block = 256;
     }
-    if (c.block != 256 && c.block != 512) throw std::invalid_argument("search workgroup size must be 256 or 512");
-    if (c.lookup == 1 && make_plan(dim, max_clusters, kmax, sq, true).total > 160 * 1024) c.lookup = 0;
-    if (make_plan(dim, max_clusters, kmax, sq, c.lookup == 1).total > 160 * 1024)
+    if (c.block != 256 && c.block != 512 && c.block != 1024)
+        throw std::invalid_argument("search workgroup size must be 256, 512 or 1024");
+    constexpr int kLds = 160 * 1024;
+    // 1024 lanes only with the dense table, and only when its plan fits
+    if (c.block == 1024 && (c.lookup != 1 || make_plan(dim, max_clusters, kmax, sq, true, 1024).total > kLds))
+        c.block = 512;
+    if (c.lookup == 1 && make_plan(dim, max_clusters, kmax, sq, true, c.block).total > kLds) c.lookup = 0;
+    if (c.lookup == 0 && c.block == 1024) c.block = 512;
+    if (make_plan(dim, max_clusters, kmax, sq, c.lookup == 1, c.block).total > kLds)
         throw std::invalid_argument("search does not fit in LDS for this dimension / cluster count / k");
     if (!c.persistent) c.longest_first = 0;
     return c;
@@ -1508,7 +1624,7 @@ size_t search_scratch_bytes(int64_t n) { return 16 + 4 * static_cast<size_t>((kO
 void launch_search(const SearchArgs& args_in, int max_clusters, int kmax, bool sq, const SearchConfig& cfg_in,
                    void* scratch, hipStream_t stream) {
     const SearchConfig cfg = resolve_search_config(args_in.ix.dim, max_clusters, kmax, sq, cfg_in);
-    const LdsPlan P = make_plan(args_in.ix.dim, max_clusters, kmax, sq, cfg.lookup == 1);
+    const LdsPlan P = make_plan(args_in.ix.dim, max_clusters, kmax, sq, cfg.lookup == 1, cfg.block);
     SearchArgs a = args_in;
     a.queue = nullptr;
     a.order = nullptr;
@@ -1535,6 +1651,8 @@ void launch_search(const SearchArgs& args_in, int max_clusters, int kmax, bool s
     const bool pers = cfg.persistent != 0;
     if (!sq)
         launch_variant<false, false, 256>(a, P, pers, stream);
+    else if (cfg.lookup == 1 && cfg.block == 1024)
+        launch_variant<true, true, 1024>(a, P, pers, stream);
     else if (cfg.lookup == 1 && cfg.block == 512)
         launch_variant<true, true, 512>(a, P, pers, stream);
     else if (cfg.lookup == 1)

@@ -9,7 +9,8 @@ pytestmark = pytest.mark.gpu
 
 DIM = 30522
 CONFIGS = [dict(lookup=lk, block=b, persistent=p, longest_first=lf)
-           for lk in (1, 0) for b in (512, 256) for p, lf in ((0, 0), (1, 0), (1, 1))]
+           for lk in (1, 0) for b in ((1024, 512, 256) if lk else (512, 256))
+           for p, lf in ((0, 0), (1, 0), (1, 1))]
 
 
 @pytest.fixture(scope="module")

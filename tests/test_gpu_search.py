@@ -73,6 +73,20 @@ def test_search_matches_oracle(nsparse, data, sq_pair, f32_pair, sq, cut, hf, k)
         assert_same(got, want)
 
 
+@pytest.mark.parametrize("block", ["256", "512", "1024"])
+@pytest.mark.parametrize("cut,hf,k", [(10, 0.8, 10), (10, 1.0, 100), (1000, 0.9, 10)])
+def test_sq_every_workgroup_size_matches_oracle(nsparse, data, sq_pair, monkeypatch, block, cut, hf, k):
+    """The dense-table search at 256, 512 and 1024 lanes per query (chunks of
+    512, 512 and 2048 doc slots) against the oracle."""
+    o, dev, vmax = sq_pair
+    monkeypatch.setenv("NSPARSE_SEARCH_BLOCK", block)
+    for qs in (data[1], data[2]):
+        ip, ix, v = qs
+        want = o.search(ip, ix, v, k, cut=cut, hf=hf)
+        got = dev.search(len(ip) - 1, ip, ix, v, k, params(nsparse, 1, cut, hf, vmax))
+        assert_same(got, want)
+
+
 def test_sq_query_range_override(nsparse, data, sq_pair):
     o, dev, vmax = sq_pair
     ip, ix, v = data[1]
