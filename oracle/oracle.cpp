@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <bit>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -35,6 +36,7 @@
 #include <utility>
 #include <vector>
 
+#include <immintrin.h>
 #include <omp.h>
 
 namespace oracle {
@@ -532,12 +534,109 @@ void finish(TopK& h, int k, bool sq, const Quantizer& iq, const Quantizer& qq, f
     }
 }
 
+// The visited-doc set of one search thread: open addressing with linear
+// probing over doc ids (>= 0; -1 marks a free slot), emptied between queries
+// through the list of slots it filled. Plays the part of the reference's
+// per-thread absl::flat_hash_set (seismic_scalar_quantized_index.cpp:279-281);
+// only membership matters to the traversal.
+class DocSet {
+public:
+    explicit DocSet(size_t expected) { rehash(std::max<size_t>(1024, std::bit_ceil(2 * expected))); }
+    // true if d was not yet in the set
+    bool insert(idx_t d) {
+        if (2 * (used_.size() + 1) > slots_.size()) rehash(2 * slots_.size());
+        uint32_t h = (static_cast<uint32_t>(d) * 0x9E3779B1u) >> shift_;
+        while (true) {
+            const idx_t v = slots_[h];
+            if (v == d) return false;
+            if (v < 0) {
+                slots_[h] = d;
+                used_.push_back(h);
+                return true;
+            }
+            h = (h + 1) & mask_;
+        }
+    }
+    void clear() {
+        for (const uint32_t h : used_) slots_[h] = -1;
+        used_.clear();
+    }
+
+private:
+    void rehash(size_t cap) {
+        std::vector<idx_t> keys;
+        keys.reserve(used_.size());
+        for (const uint32_t h : used_) keys.push_back(slots_[h]);
+        slots_.assign(cap, -1);
+        used_.clear();
+        mask_ = static_cast<uint32_t>(cap - 1);
+        shift_ = 32 - std::countr_zero(cap);
+        for (const idx_t d : keys) insert(d);
+    }
+    std::vector<idx_t> slots_;
+    std::vector<uint32_t> used_;
+    uint32_t mask_ = 0;
+    int shift_ = 0;
+};
+
+// uint8 dot product of a doc row with the dense query codes (integer sums:
+// any order gives the generic loop's result, utils/distance.h:39-47). The
+// AVX-512 form gathers 16 query codes at a time (4-byte loads at byte offsets,
+// so `dense` carries 3 bytes of padding) and is taken when the CPU has it,
+// as the reference's build picks its SIMD dot products (distance_simd.h:15-25).
+__attribute__((target("avx512f,avx512bw"))) int dot_u8_avx512(const term_t* idx, const uint8_t* val, size_t n,
+                                                              const uint8_t* dense) {
+    __m512i acc = _mm512_setzero_si512();
+    const __m512i low = _mm512_set1_epi32(0xFF);
+    size_t j = 0;
+    for (; j + 16 <= n; j += 16) {
+        const __m512i ix = _mm512_cvtepu16_epi32(_mm256_loadu_si256(reinterpret_cast<const __m256i*>(idx + j)));
+        const __m512i q = _mm512_and_si512(_mm512_i32gather_epi32(ix, dense, 1), low);
+        const __m512i c = _mm512_cvtepu8_epi32(_mm_loadu_si128(reinterpret_cast<const __m128i*>(val + j)));
+        acc = _mm512_add_epi32(acc, _mm512_mullo_epi32(q, c));
+    }
+    int r = _mm512_reduce_add_epi32(acc);
+    for (; j < n; ++j) r += val[j] * dense[idx[j]];
+    return r;
+}
+
+inline int dot_u8_generic(const term_t* idx, const uint8_t* val, size_t n, const uint8_t* dense) {
+    int r = 0;
+    for (size_t j = 0; j < n; ++j) r += val[j] * dense[idx[j]];
+    return r;
+}
+
+// 1: take the AVX-512 dot when the CPU has it; 0: always the generic loop
+std::atomic<int> g_simd{1};
+
+inline bool cpu_has_avx512() {
+    static const bool has = __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512bw");
+    return has && g_simd.load(std::memory_order_relaxed) != 0;
+}
+
+// The leading cache lines of a row's term ids and values, fetched one doc
+// ahead of scoring (seismic_scalar_quantized_index.cpp:107-122: distance 1,
+// 4 lines).
+inline void prefetch_row_head(const Fwd& f, idx_t d) {
+    constexpr size_t kLines = 4;
+    const idx_t s0 = f.indptr[d];
+    const size_t len = static_cast<size_t>(f.indptr[d + 1] - s0);
+    const char* ip = reinterpret_cast<const char*>(f.indices.data() + s0);
+    const char* vp = reinterpret_cast<const char*>(f.values.data() + static_cast<size_t>(s0) * f.esz);
+    const size_t il = std::min(kLines, (len * sizeof(term_t) + 63) / 64);
+    const size_t vl = std::min(kLines, (len * f.esz + 63) / 64);
+    for (size_t l = 0; l < il; ++l) __builtin_prefetch(ip + 64 * l);
+    for (size_t l = 0; l < vl; ++l) __builtin_prefetch(vp + 64 * l);
+}
+
 // seismic_scalar_quantized_index.cpp:67-135 / seismic_index.cpp:53-118 and
 // single_query :307-357 / :231-274.
 template <class TopK, class VT>
 void single_query(const Index& ix, const std::vector<VT>& dense, const term_t* qi, const VT* qv, size_t qn,
-                  const std::vector<term_t>& cuts, float hf, const Selector* sel, TopK& heap, Stats& st) {
-    std::unordered_set<idx_t> visited;
+                  const std::vector<term_t>& cuts, float hf, const Selector* sel, TopK& heap, Stats& st,
+                  DocSet& visited) {
+    visited.clear();
+    const bool avx512 = cpu_has_avx512();
     std::vector<float> scores;
     bool first_list = true;
     const Fwd& f = ix.fwd;
@@ -555,10 +654,12 @@ void single_query(const Index& ix, const std::vector<VT>& dense, const term_t* q
                     continue;
                 }
                 st.clusters_opened++;
-                for (idx_t p = lc.offsets[c]; p < lc.offsets[c + 1]; ++p) {
+                const idx_t p1 = lc.offsets[c + 1];
+                for (idx_t p = lc.offsets[c]; p < p1; ++p) {
                     const idx_t d = lc.docs[p];
+                    if (p + 1 < p1) prefetch_row_head(f, lc.docs[p + 1]);
                     st.docs_read++;
-                    if (!visited.insert(d).second) continue;
+                    if (!visited.insert(d)) continue;
                     if (sel != nullptr && !sel->is_member(d)) continue;
                     const idx_t s0 = f.indptr[d], s1 = f.indptr[d + 1];
                     st.docs_scored++;
@@ -570,8 +671,11 @@ void single_query(const Index& ix, const std::vector<VT>& dense, const term_t* q
                         for (idx_t j = s0; j < s1; ++j) r += w[j] * dense[f.indices[j]];
                         score = r;
                     } else {
-                        int r = 0;
-                        for (idx_t j = s0; j < s1; ++j) r += f.values[j] * dense[f.indices[j]];
+                        const size_t n = static_cast<size_t>(s1 - s0);
+                        const int r = avx512 ? dot_u8_avx512(f.indices.data() + s0, f.values.data() + s0, n,
+                                                             dense.data())
+                                             : dot_u8_generic(f.indices.data() + s0, f.values.data() + s0, n,
+                                                              dense.data());
                         score = static_cast<float>(r);
                     }
                     heap.add(score, d);
@@ -653,8 +757,9 @@ void search_impl(const Index& ix, int n, const idx_t* ip, const term_t* qi, cons
     {
         std::vector<float> densef;
         std::vector<uint8_t> denseq;
+        DocSet visited(static_cast<size_t>(std::max(k, 1)) * 4096);
         if (sq)
-            denseq.assign(ix.fwd.dim, 0);
+            denseq.assign(ix.fwd.dim + 3, 0);  // + the 4-byte gathers' padding
         else
             densef.assign(ix.fwd.dim, 0.0F);
 #pragma omp for schedule(dynamic, 64)
@@ -667,13 +772,13 @@ void search_impl(const Index& ix, int n, const idx_t* ip, const term_t* qi, cons
                 const uint8_t* qc = codes.data() + s0;
                 auto cuts = top_k_tokens<uint8_t>(qi + s0, qc, static_cast<int>(len), cut);
                 for (size_t i = 0; i < len; ++i) denseq[qi[s0 + i]] = qc[i];
-                single_query<TopK, uint8_t>(ix, denseq, qi + s0, qc, len, cuts, hf, sel, heap, st);
+                single_query<TopK, uint8_t>(ix, denseq, qi + s0, qc, len, cuts, hf, sel, heap, st, visited);
                 for (size_t i = 0; i < len; ++i) denseq[qi[s0 + i]] = 0;
             } else {
                 const float* qf = qv + s0;
                 auto cuts = top_k_tokens<float>(qi + s0, qf, static_cast<int>(len), cut);
                 for (size_t i = 0; i < len; ++i) densef[qi[s0 + i]] = qf[i];
-                single_query<TopK, float>(ix, densef, qi + s0, qf, len, cuts, hf, sel, heap, st);
+                single_query<TopK, float>(ix, densef, qi + s0, qf, len, cuts, hf, sel, heap, st, visited);
                 for (size_t i = 0; i < len; ++i) densef[qi[s0 + i]] = 0.0F;
             }
             finish(heap, k, sq, ix.q, qq, D + static_cast<size_t>(qidx) * k, L + static_cast<size_t>(qidx) * k);
@@ -788,6 +893,13 @@ using namespace oracle;
 extern "C" {
 
 const char* oracle_last_error() { return g_err.c_str(); }
+
+// Selects the uint8 dot product form of later searches (1 = AVX-512 when
+// available, 0 = generic); returns whether the CPU has AVX-512.
+int oracle_set_simd(int on) {
+    g_simd.store(on != 0 ? 1 : 0);
+    return __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512bw") ? 1 : 0;
+}
 
 // kind: 0 = seismic (FP32), 1 = seismic_sq (8-bit).
 int oracle_create(int dim, int sq, float vmin, float vmax, int lambda, int beta, float alpha, int seed, int tie_mode,

@@ -176,6 +176,25 @@ def test_selector_filters(tie_mode):
     assert list(lab[0]) == [2, 0, -1]
 
 
+def test_simd_and_generic_u8_dot_agree():
+    """The oracle's AVX-512 uint8 dot product (taken when the CPU has it) and
+    the generic loop give identical searches: integer sums, any order."""
+    ip, ix, v = corpus(1500, 2048, 40, seed=11)
+    o = OracleIndex(2048, 1, 200, 20, 0.4, 2, 0.0, float(v.max()), tie_mode=1)
+    o.add(ip, ix, v)
+    o.build()
+    qip, qix, qv = corpus(60, 2048, 25, seed=12)
+    try:
+        O.lib.oracle_set_simd(0)
+        want = o.search(qip, qix, qv, 10, cut=10, hf=0.9, with_stats=True)
+        O.lib.oracle_set_simd(1)
+        got = o.search(qip, qix, qv, 10, cut=10, hf=0.9, with_stats=True)
+    finally:
+        O.lib.oracle_set_simd(1)
+    for a, b in zip(want, got):
+        assert np.array_equal(a, b)
+
+
 def test_no_pruning_equals_brute_force_over_visited_lists():
     """heap_factor huge, lambda >= list length: every doc of every cut list is
     scored, so the result is the exact top-k over docs sharing a cut term."""
