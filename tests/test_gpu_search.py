@@ -153,6 +153,58 @@ def test_empty_query_and_padding(nsparse, sq_pair):
     assert (got[1][0] == -1).all() and (got[0][0] == -1.0).all()
 
 
+@pytest.fixture(scope="module")
+def big_cluster_pairs(nsparse, data):
+    """lambda 3000, beta 2: clusters of ~1500 docs, each split over several
+    chunks of the device traversal."""
+    return {sq: build_pair(nsparse, data[0], sq, lam=3000, beta=2) for sq in (1, 0)}
+
+
+@pytest.mark.parametrize("sq,block", [(1, "512"), (1, "256"), (1, "1024"), (0, "256")])
+@pytest.mark.parametrize("hf,k", [(0.5, 10), (0.9, 100), (1.0, 10)])
+def test_clusters_larger_than_a_chunk(nsparse, data, big_cluster_pairs, monkeypatch, sq, block, hf, k):
+    """A cluster split over several chunks is tested once, at its first piece;
+    when that test fails on a later list, none of its pieces is scored."""
+    o, dev, vmax = big_cluster_pairs[sq]
+    monkeypatch.setenv("NSPARSE_SEARCH_BLOCK", block)
+    for qs in (data[1], data[2]):
+        ip, ix, v = qs
+        want = o.search(ip, ix, v, k, cut=10, hf=hf, with_stats=True)
+        import torch
+
+        n = len(ip) - 1
+        stats = torch.zeros((n, 16), dtype=torch.int64, device="cuda")
+        dd, dl = dev.search_device(torch.from_numpy(ip).cuda(), torch.from_numpy(ix.view(np.int16)).cuda(),
+                                   torch.from_numpy(v).cuda(), k, params=params(nsparse, sq, 10, hf, vmax),
+                                   stats=stats)
+        torch.cuda.synchronize()
+        assert dev.device_error() == 0
+        assert_same((dd.cpu().numpy(), dl.cpu().numpy()), want[:2])
+        st = stats.cpu().numpy()
+        # lists, terms found, summary entries, prune tests, clusters opened, doc ids read
+        assert (st[:, :6] == want[2][:, :6]).all(), np.argwhere(st[:, :6] != want[2][:, :6])[:5]
+
+
+def test_longest_queries(nsparse, data, sq_pair, f32_pair):
+    """Queries of 480..512 non-zeros (the device maximum) against the oracle;
+    513 is rejected."""
+    from nsparse import synth
+
+    spec = synth.SynthSpec(7, DIM, 1.1, 1, 480.0, 512.0, 480, 512, 0.0, 0.7)
+    ip, ix, v = synth.make_corpus(spec, 40)
+    assert (np.diff(ip) <= 512).all() and np.diff(ip).max() >= 500
+    for sq, (o, dev, vmax) in ((1, sq_pair), (0, f32_pair)):
+        want = o.search(ip, ix, v, 10, cut=10, hf=0.8)
+        got = dev.search(len(ip) - 1, ip, ix, v, 10, params(nsparse, sq, 10, 0.8, vmax))
+        assert_same(got, want)
+    o, dev, vmax = sq_pair
+    ip2 = np.array([0, 513], np.int32)
+    ix2 = np.arange(513, dtype=np.uint16)
+    v2 = np.ones(513, np.float32)
+    with pytest.raises(Exception):
+        dev.search(1, ip2, ix2, v2, 10, params(nsparse, 1, 10, 0.8, vmax))
+
+
 def test_batched_device_mixed_k_equals_single(nsparse, data, sq_pair):
     import torch
 
