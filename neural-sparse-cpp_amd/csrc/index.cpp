@@ -796,6 +796,83 @@ void SeismicIndex::read_payload(ByteReader& r) {
     params_ = Params{-1, -1, 0.4F, -1};
 }
 
+// ============================================================== BrutalIndex
+
+BrutalIndex::BrutalIndex(int dim) : Index(dim) {
+    fwd_.esz = 4;
+    fwd_.dim = dim > 0 ? static_cast<size_t>(dim) : 0;
+}
+
+// BrutalIndex::add (brutal_index.cpp:27-42): FP32 rows appended as given.
+void BrutalIndex::add(idx_t n, const idx_t* indptr, const term_t* indices, const float* values) {
+    throw_if_not_positive(n);
+    throw_if_any_null(indptr, indices, values);
+    if (dim_ <= 0) throw std::invalid_argument("value must be positive");
+    check_csr(n, indptr, indices, dim_, "add:");
+    const int64_t nnz = indptr[n];
+    if (static_cast<int64_t>(fwd_.indices.size()) + nnz > std::numeric_limits<idx_t>::max())
+        throw std::invalid_argument("too many non-zeros for 32-bit offsets");
+    if (fwd_.indptr.empty()) fwd_.indptr.push_back(0);
+    const idx_t off = fwd_.indptr.back();
+    fwd_.indices.insert(fwd_.indices.end(), indices, indices + nnz);
+    const size_t old_vals = fwd_.values.size();
+    fwd_.values.resize(old_vals + static_cast<size_t>(nnz) * 4);
+    std::memcpy(fwd_.values.data() + old_vals, values, static_cast<size_t>(nnz) * 4);
+    for (idx_t i = 1; i <= n; ++i) fwd_.indptr.push_back(indptr[i] + off);
+    dev_valid_ = false;
+}
+
+// BrutalIndex::search / single_query (brutal_index.cpp:44-96): every document
+// is a candidate (zero scores included), the k best by (score desc, doc id
+// asc) come back, padded with (-1.0f, -1) when the corpus is smaller than k.
+// The selector is not read. Without vectors nothing is written, as the
+// reference's empty per-query results leave the outputs untouched.
+void BrutalIndex::search(idx_t n, const idx_t* indptr, const term_t* indices, const float* values, int k,
+                         float* distances, idx_t* labels, const nsparse_search_params*) {
+    throw_if_not_positive(n);
+    throw_if_not_positive(k);
+    throw_if_any_null(indptr, indices, values, labels, distances);
+    if (fwd_.n() == 0) return;
+    check_csr(n, indptr, indices, dim_, "query");
+    if (k > kBruteKMax)
+        throw std::invalid_argument("brutal search supports k <= " + std::to_string(kBruteKMax) + " on the device");
+    for (idx_t i = 0; i < n; ++i)
+        if (indptr[i + 1] - indptr[i] > kDeviceQueryMax)
+            throw std::invalid_argument("query has more non-zeros than the device path supports (" +
+                                        std::to_string(kDeviceQueryMax) + ")");
+    require_device();
+    if (!dev_valid_) {
+        d_indptr_.upload(fwd_.indptr.data(), fwd_.indptr.size() * 4, 64);
+        d_indices_.upload(fwd_.indices.data(), fwd_.indices.size() * 2, 64);
+        d_values_.upload(fwd_.values.data(), fwd_.values.size(), 64);
+        dev_valid_ = true;
+    }
+    const int64_t q0 = indptr[0], qnnz = indptr[n] - q0;
+    std::vector<int32_t> qip(n + 1);
+    for (idx_t i = 0; i <= n; ++i) qip[i] = static_cast<int32_t>(indptr[i] - q0);
+    DeviceBuffer d_qip, d_qix, d_qv, d_out_d, d_out_l, d_scores;
+    d_qip.upload(qip);
+    d_qix.upload(indices + q0, qnnz * 2);
+    d_qv.upload(values + q0, qnnz * 4);
+    d_out_d.allocate(static_cast<size_t>(n) * k * 4);
+    d_out_l.allocate(static_cast<size_t>(n) * k * 4);
+    d_scores.allocate(brute_search_scratch_bytes(fwd_.n(), n));
+    launch_brute_search(d_indptr_.get<int32_t>(), d_indices_.get<uint16_t>(), d_values_.get<float>(), fwd_.n(), dim_,
+                        d_qip.get<int32_t>(), d_qix.get<uint16_t>(), d_qv.get<float>(), n, k, d_out_d.get<float>(),
+                        d_out_l.get<int32_t>(), d_scores.get<float>(), nullptr);
+    hip_check(hipDeviceSynchronize(), "brute-force search");
+    d_out_d.download(distances, static_cast<size_t>(n) * k * 4);
+    d_out_l.download(labels, static_cast<size_t>(n) * k * 4);
+}
+
+// write_index / read_index (io/index_io.cpp:120-141): BrutalIndex is not an
+// IndexIO.
+void BrutalIndex::write_payload(ByteWriter&) const {
+    throw std::runtime_error("Index does not support serialization");
+}
+
+void BrutalIndex::read_payload(ByteReader&) { throw std::runtime_error("Index does not support serialization"); }
+
 // ============================================================== IDMapIndex
 
 IDMapIndex::IDMapIndex(std::unique_ptr<Index> delegate) : Index(delegate->dimension()), delegate_(std::move(delegate)) {}
@@ -823,6 +900,13 @@ void IDMapIndex::build() { delegate_->build(); }
 // external ids, the labels are mapped back to external ids.
 void IDMapIndex::search(idx_t n, const idx_t* indptr, const term_t* indices, const float* values, int k,
                         float* distances, idx_t* labels, const nsparse_search_params* params) {
+    if (dynamic_cast<BrutalIndex*>(delegate_.get()) != nullptr) {
+        // the delegate ignores selectors; only the labels are mapped
+        delegate_->search(n, indptr, indices, values, k, distances, labels, params);
+        for (int64_t i = 0; i < static_cast<int64_t>(n) * k; ++i)
+            if (labels[i] >= 0) labels[i] = i2e_.at(labels[i]);
+        return;
+    }
     auto* seismic = dynamic_cast<SeismicIndex*>(delegate_.get());
     if (seismic == nullptr) throw std::runtime_error("IDMapIndex delegate is not a SEISMIC index");
     throw_if_not_positive(n);
@@ -928,7 +1012,8 @@ std::unique_ptr<Index> index_factory(int dim, const char* description) {
             if (k == key) return v;
         return def;
     };
-    if (type == "brutal" || type == "inverted")
+    if (type == "brutal") return std::make_unique<BrutalIndex>(dim);
+    if (type == "inverted")
         throw std::runtime_error("index type '" + type + "' is not part of the SEISMIC device path");
     if (type == "seismic" || type == "seismic_sq") {
         SeismicIndex::Params p;
@@ -983,7 +1068,9 @@ std::unique_ptr<Index> read_index(ByteReader& r) {
         ix = std::make_unique<SeismicIndex>(dim, true, 0.0F, 1.0F, SeismicIndex::Params{-1, -1, 0.4F, -1});
     else if (id == fourcc_of({'I', 'D', 'M', 'P'}))
         ix = std::make_unique<IDMapIndex>();
-    else if (id == fourcc_of({'B', 'R', 'U', 'T'}) || id == fourcc_of({'I', 'N', 'V', 'T'}))
+    else if (id == fourcc_of({'B', 'R', 'U', 'T'}))
+        ix = std::make_unique<BrutalIndex>(dim);  // whose read_payload throws, as the reference's read does
+    else if (id == fourcc_of({'I', 'N', 'V', 'T'}))
         throw std::runtime_error("index type is not part of the SEISMIC device path");
     else
         throw std::runtime_error("Unknown index type");

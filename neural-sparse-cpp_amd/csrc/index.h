@@ -144,6 +144,28 @@ private:
     DevIndexView view_{};
 };
 
+// BrutalIndex (brutal_index.h:22-44, brutal_index.cpp): FP32 vectors kept as
+// they are added, every query scored against every document on the device.
+// Like the reference it has no build step (Index::build throws), ignores a
+// selector, and does not serialize.
+class BrutalIndex : public Index {
+public:
+    explicit BrutalIndex(int dim);
+    std::array<char, 4> id() const override { return {'B', 'R', 'U', 'T'}; }
+    int64_t num_vectors() const override { return fwd_.n(); }
+    void add(idx_t n, const idx_t* indptr, const term_t* indices, const float* values) override;
+    void search(idx_t n, const idx_t* indptr, const term_t* indices, const float* values, int k, float* distances,
+                idx_t* labels, const nsparse_search_params* params) override;
+    void write_payload(ByteWriter& w) const override;
+    void read_payload(ByteReader& r) override;
+    const HostCsr* vectors() const override { return fwd_.n() > 0 ? &fwd_ : nullptr; }
+
+private:
+    HostCsr fwd_;
+    bool dev_valid_ = false;
+    DeviceBuffer d_indptr_, d_indices_, d_values_;
+};
+
 // A batched search over fixed device buffers, replayed from a hipGraph: the
 // schedule (lookup table, workgroup width, persistent queue, longest-first
 // order) is given or picked by timing every candidate on the batch itself;

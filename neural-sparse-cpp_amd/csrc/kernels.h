@@ -66,6 +66,16 @@ void launch_search_checked(const SearchArgs& a, int max_clusters, int kmax, bool
 void launch_exact_checked(const ExactArgs& a, bool sq, hipStream_t stream);
 int search_lds_bytes(int dim, int max_clusters, int kmax, bool sq);
 std::string device_error_message(int32_t code);
+// Exhaustive search (BrutalIndex): every document scored, the k best kept per
+// query (k <= kBruteKMax). Queries run in groups of brute_search_group() rows
+// of n_docs float scores, which d_scores (brute_search_scratch_bytes) holds.
+constexpr int kBruteKMax = 1024;
+int64_t brute_search_group(int64_t n_docs, int64_t n_queries);
+size_t brute_search_scratch_bytes(int64_t n_docs, int64_t n_queries);
+void launch_brute_search(const int32_t* indptr, const uint16_t* indices, const float* values, int64_t n_docs, int dim,
+                         const int32_t* d_q_indptr, const uint16_t* d_q_indices, const float* d_q_values,
+                         int64_t n_queries, int k, float* d_out_d, int32_t* d_out_l, float* d_scores,
+                         hipStream_t stream);
 void launch_exact_scores(const int32_t* indptr, const uint16_t* indices, const float* values, int64_t n_docs, int dim,
                          const uint16_t* q_indices, const float* q_values, int qn, float* scores, hipStream_t stream);
 

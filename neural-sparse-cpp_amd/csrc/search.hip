@@ -1425,16 +1425,21 @@ __global__ __launch_bounds__(kBlock) void exact_match_kernel(ExactArgs a, int nw
     }
 }
 
-// Exact sparse x sparse scores of one query against every document of an FP32
-// CSR (the scoring loop of BrutalIndex::single_query, brutal_index.cpp:75-96):
-// the query sits in LDS as bitmap + rank table, one lane per document adds
-// weight * query_value in row order. Used as recall ground truth.
+// Exact FP32 scores of every document against queries, forward-index CSR
+// (the scoring loop of BrutalIndex::single_query, brutal_index.cpp:75-96):
+// the query sits in LDS as bitmap + rank table (a repeated term keeps its last
+// value, as get_dense_vector_float does), one lane per document adds
+// weight * query_value in row order, as the generic dot product does. Query
+// q0 + blockIdx.y of the CSR batch (q_indptr) writes row blockIdx.y of
+// `scores` (row stride ld); without q_indptr, blockIdx.y is 0 and the query is
+// (q_indices, q_values, qn).
 __global__ __launch_bounds__(kBlock) void exact_scores_kernel(const int32_t* __restrict__ indptr,
                                                               const uint16_t* __restrict__ indices,
                                                               const float* __restrict__ values, int64_t n_docs,
-                                                              int nw, const uint16_t* __restrict__ q_indices,
+                                                              int nw, const int32_t* __restrict__ q_indptr,
+                                                              const uint16_t* __restrict__ q_indices,
                                                               const float* __restrict__ q_values, int qn,
-                                                              float* __restrict__ scores) {
+                                                              int64_t q0, float* __restrict__ scores, int64_t ld) {
     extern __shared__ __align__(16) unsigned char smem[];
     uint32_t* qbits = reinterpret_cast<uint32_t*>(smem);
     uint16_t* qpref = reinterpret_cast<uint16_t*>(qbits + nw);
@@ -1442,6 +1447,13 @@ __global__ __launch_bounds__(kBlock) void exact_scores_kernel(const int32_t* __r
     int32_t* rlast = reinterpret_cast<int32_t*>(rval + kQMax);
     int32_t* wsum = rlast + kQMax;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (q_indptr != nullptr) {
+        const int64_t q = q0 + blockIdx.y;
+        q_indices += q_indptr[q];
+        q_values += q_indptr[q];
+        qn = q_indptr[q + 1] - q_indptr[q];
+        scores += static_cast<int64_t>(blockIdx.y) * ld;
+    }
     for (int w = tid; w < nw; w += kBlock) qbits[w] = 0;
     for (int r = tid; r < qn; r += kBlock) rlast[r] = -1;
     __syncthreads();
@@ -1480,15 +1492,142 @@ __global__ __launch_bounds__(kBlock) void exact_scores_kernel(const int32_t* __r
     }
 }
 
+namespace {
+
+int exact_scores_lds(int dim) { return (((dim + 31) / 32 * 6 + 15) & ~15) + kQMax * 8 + kWaves * 4; }
+
+unsigned exact_scores_blocks(int64_t n_docs) {
+    return static_cast<unsigned>(std::max<int64_t>(std::min<int64_t>((n_docs + kBlock - 1) / kBlock, 4096), 1));
+}
+
+// Top-k selection over rows of scores: key = (order-preserving bits of the
+// score) << 32 | ~doc, so the k largest keys are the k best docs by (score
+// desc, doc id asc), the order in which the reference's reject-when-not-better
+// heap keeps them (earlier docs win ties at the k-th).
+constexpr int kSelectSlots = 4096;  // LDS keys: the sorted top-k, then candidates
+constexpr int kSelectRound = kBlock * 4;
+
+__device__ __forceinline__ uint64_t score_key(float s, int64_t d) {
+    uint32_t u = __float_as_uint(s);
+    u ^= (u >> 31) != 0 ? 0xFFFFFFFFu : 0x80000000u;
+    return (static_cast<uint64_t>(u) << 32) | (0xFFFFFFFFu - static_cast<uint32_t>(d));
+}
+
+__device__ __forceinline__ float key_score(uint64_t key) {
+    uint32_t u = static_cast<uint32_t>(key >> 32);
+    u ^= (u >> 31) != 0 ? 0x80000000u : 0xFFFFFFFFu;
+    return __uint_as_float(u);
+}
+
+// Descending bitonic sort of keys[0, kSelectSlots) by all kBlock threads.
+__device__ void bitonic_desc(uint64_t* keys) {
+    for (int size = 2; size <= kSelectSlots; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = threadIdx.x; t < kSelectSlots / 2; t += kBlock) {
+                const int i = 2 * t - (t & (stride - 1));
+                const int j = i + stride;
+                const bool desc = (i & size) == 0;
+                const uint64_t a = keys[i], b = keys[j];
+                if ((a < b) == desc) {
+                    keys[i] = b;
+                    keys[j] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// One workgroup per row: the row's n_docs scores stream through a filter
+// against the current k-th key; survivors collect in LDS behind the sorted
+// top-kcap and are merged in by a bitonic sort whenever the next round could
+// overflow them. Writes k (score, doc) pairs best first, (-1.0f, -1) past the
+// end of the corpus.
+__global__ __launch_bounds__(kBlock) void topk_select_kernel(const float* __restrict__ scores, int64_t ld,
+                                                             int64_t n_docs, int k, int kcap,
+                                                             float* __restrict__ out_d, int32_t* __restrict__ out_l) {
+    __shared__ uint64_t keys[kSelectSlots];
+    __shared__ int cnt;
+    const float* row = scores + static_cast<int64_t>(blockIdx.x) * ld;
+    const int cap = kSelectSlots - kcap;  // candidate slots
+    for (int i = threadIdx.x; i < kSelectSlots; i += kBlock) keys[i] = 0;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    uint64_t thresh = 0;
+    auto merge = [&]() {
+        const int used = kcap + cnt;
+        for (int i = used + threadIdx.x; i < kSelectSlots; i += kBlock) keys[i] = 0;
+        __syncthreads();
+        bitonic_desc(keys);
+        if (threadIdx.x == 0) cnt = 0;
+        thresh = keys[k - 1];
+        __syncthreads();
+    };
+    for (int64_t base = 0; base < n_docs; base += kSelectRound) {
+#pragma unroll
+        for (int u = 0; u < kSelectRound / kBlock; ++u) {
+            const int64_t d = base + u * kBlock + threadIdx.x;
+            if (d < n_docs) {
+                const uint64_t key = score_key(row[d], d);
+                if (key > thresh) keys[kcap + atomicAdd(&cnt, 1)] = key;
+            }
+        }
+        __syncthreads();
+        const int c = cnt;
+        __syncthreads();  // every thread has read cnt before the next round adds to it
+        if (c > cap - kSelectRound) merge();
+    }
+    if (cnt > 0) merge();
+    float* od = out_d + static_cast<int64_t>(blockIdx.x) * k;
+    int32_t* ol = out_l + static_cast<int64_t>(blockIdx.x) * k;
+    for (int i = threadIdx.x; i < k; i += kBlock) {
+        const uint64_t key = keys[i];
+        od[i] = key != 0 ? key_score(key) : -1.0F;
+        ol[i] = key != 0 ? static_cast<int32_t>(0xFFFFFFFFu - static_cast<uint32_t>(key)) : -1;
+    }
+}
+
+}  // namespace
+
 void launch_exact_scores(const int32_t* indptr, const uint16_t* indices, const float* values, int64_t n_docs, int dim,
                          const uint16_t* q_indices, const float* q_values, int qn, float* scores, hipStream_t stream) {
     if (qn > kQMax) throw std::invalid_argument("query has more non-zeros than the device path supports (512)");
     const int nw = (dim + 31) / 32;
-    const int lds = ((nw * 6 + 15) & ~15) + kQMax * 8 + kWaves * 4;
-    const int64_t blocks = std::min<int64_t>((n_docs + kBlock - 1) / kBlock, 4096);
-    hipLaunchKernelGGL(exact_scores_kernel, dim3(static_cast<unsigned>(std::max<int64_t>(blocks, 1))), dim3(kBlock),
-                       lds, stream, indptr, indices, values, n_docs, nw, q_indices, q_values, qn, scores);
+    hipLaunchKernelGGL(exact_scores_kernel, dim3(exact_scores_blocks(n_docs)), dim3(kBlock), exact_scores_lds(dim),
+                       stream, indptr, indices, values, n_docs, nw, nullptr, q_indices, q_values, qn, int64_t{0},
+                       scores, int64_t{0});
     hip_check(hipGetLastError(), "exact scores launch");
+}
+
+size_t brute_search_scratch_bytes(int64_t n_docs, int64_t n_queries) {
+    return static_cast<size_t>(brute_search_group(n_docs, n_queries)) * static_cast<size_t>(n_docs) * sizeof(float);
+}
+
+int64_t brute_search_group(int64_t n_docs, int64_t n_queries) {
+    constexpr int64_t kScoreBytes = int64_t{256} << 20;  // score rows held at once
+    int64_t per = std::max<int64_t>(1, kScoreBytes / std::max<int64_t>(n_docs * 4, 1));
+    if (const char* e = std::getenv("NSPARSE_BRUTE_GROUP_QUERIES")) per = std::min<int64_t>(per, std::max(1, std::atoi(e)));
+    return std::max<int64_t>(1, std::min<int64_t>({per, n_queries, 65535}));
+}
+
+void launch_brute_search(const int32_t* indptr, const uint16_t* indices, const float* values, int64_t n_docs, int dim,
+                         const int32_t* d_q_indptr, const uint16_t* d_q_indices, const float* d_q_values,
+                         int64_t n_queries, int k, float* d_out_d, int32_t* d_out_l, float* d_scores,
+                         hipStream_t stream) {
+    if (k <= 0 || k > kBruteKMax) throw std::invalid_argument("brutal search supports 1 <= k <= 1024 on the device");
+    int kcap = 1;
+    while (kcap < k) kcap <<= 1;
+    const int64_t group = brute_search_group(n_docs, n_queries);
+    const int nw = (dim + 31) / 32;
+    for (int64_t q0 = 0; q0 < n_queries; q0 += group) {
+        const int64_t g = std::min<int64_t>(group, n_queries - q0);
+        hipLaunchKernelGGL(exact_scores_kernel, dim3(exact_scores_blocks(n_docs), static_cast<unsigned>(g)),
+                           dim3(kBlock), exact_scores_lds(dim), stream, indptr, indices, values, n_docs, nw,
+                           d_q_indptr, d_q_indices, d_q_values, 0, q0, d_scores, n_docs);
+        hipLaunchKernelGGL(topk_select_kernel, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, stream, d_scores,
+                           n_docs, n_docs, k, kcap, d_out_d + q0 * k, d_out_l + q0 * k);
+        hip_check(hipGetLastError(), "brute-force search launch");
+    }
 }
 
 int search_lds_bytes(int dim, int max_clusters, int kmax, bool sq) {

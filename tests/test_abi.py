@@ -51,7 +51,26 @@ def test_factory_strings_and_errors(nsparse):
     with pytest.raises(ValueError):
         nsparse.index_factory(10, "seismic,lambda=abc")
     with pytest.raises(RuntimeError):
-        nsparse.index_factory(10, "brutal")
+        nsparse.index_factory(10, "inverted")
+
+
+def test_brutal_lifecycle(nsparse):
+    """brutal (index_factory.cpp:95-97): created with surrounding whitespace,
+    no build step (Index::build throws "not implemented"), not serializable
+    (python_tests/test_error_paths.py:62-74); add works without a GPU."""
+    ix = nsparse.index_factory(50, "  brutal  ")
+    assert ix.id() == "BRUT" and ix.get_dimension() == 50 and ix.num_vectors() == 0
+    ip = np.array([0, 2, 3], np.int32)
+    ix.add(2, ip, np.array([1, 7, 3], np.uint16), np.array([0.5, 1.0, 2.0], np.float32))
+    assert ix.num_vectors() == 2
+    with pytest.raises(RuntimeError, match="not implemented"):
+        ix.build()
+    with pytest.raises(RuntimeError, match="serialization"):
+        nsparse.serialize_index(ix)
+    im = nsparse.index_factory(50, "idmap,brutal")
+    assert im.id() == "IDMP"
+    with pytest.raises(ValueError):
+        ix.add(1, np.array([0, 1], np.int32), np.array([50], np.uint16), np.array([1.0], np.float32))
 
 
 def test_argument_checks(nsparse):
