@@ -34,6 +34,7 @@
 #include <cstdint>
 #include <numeric>
 #include <random>
+#include <type_traits>
 #include <stdexcept>
 #include <vector>
 
@@ -270,6 +271,19 @@ __global__ void fill_i32_kernel(int32_t* p, size_t n, int32_t v) {
     GRID_STRIDE(i, n) p[i] = v;
 }
 
+// Inclusive prefix sum over the 64 lanes with DPP moves only: Hillis-Steele
+// inside each 16-lane row (row_shr 1, 2, 4, 8), then row_bcast:15 and
+// row_bcast:31 carry the row totals upward.
+__device__ __forceinline__ uint32_t wave_incl_scan_dpp(uint32_t v) {
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x111, 0xf, 0xf, false));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x112, 0xf, 0xf, false));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x114, 0xf, 0xf, false));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x118, 0xf, 0xf, false));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x142, 0xa, 0xf, false));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x143, 0xc, 0xf, false));
+    return v;
+}
+
 template <class Acc>
 __device__ __forceinline__ void argmax_merge(Acc& v, int& i, Acc ov, int oi) {
     if (ov > v || (ov == v && oi < i)) {
@@ -341,21 +355,81 @@ __global__ __launch_bounds__(kB) void assign_kernel(
             for (int c = lane; c < nc; c += 64) sims[c] = Acc(0);
             __builtin_amdgcn_wave_barrier();
             const int32_t r0 = indptr[d], r1 = indptr[d + 1];
-            for (int b = r0; b < r1; b += 64) {
-                const int p = b + lane;
-                int2 col = make_int2(0, 0);
-                Acc dv = Acc(0);
-                if (p < r1) {
-                    col = tmap[indices[p]];
-                    dv = static_cast<Acc>(values[p]);
+            if constexpr (std::is_same_v<T, uint8_t>) {
+                // Integer sims are exact in any order: the doc's (term, centroid
+                // posting) pairs are numbered by a prefix over the terms' posting
+                // counts and dealt to the lanes, kW windows of 64 with their loads
+                // in flight together, and added with LDS atomics. A lane finds
+                // its term as the last lane whose prefix is <= its item.
+                constexpr int kW = 4;
+                for (int b = r0; b < r1; b += 64) {
+                    const int p = b + lane;
+                    int cs = 0, cnt = 0;
+                    Acc dv = Acc(0);
+                    if (p < r1) {
+                        const int2 col = tmap[indices[p]];
+                        if (col.x >= 0) {
+                            cs = col.x;
+                            cnt = col.y - col.x;
+                        }
+                        dv = static_cast<Acc>(values[p]);
+                    }
+                    const int incl = static_cast<int>(wave_incl_scan_dpp(static_cast<uint32_t>(cnt)));
+                    const int excl = incl - cnt;
+                    const int total = __builtin_amdgcn_readlane(incl, 63);
+                    for (int i0 = 0; i0 < total; i0 += 64 * kW) {
+                        int ev[kW];
+                        Acc wv[kW];
+                        bool ok[kW];
+#pragma unroll
+                        for (int u = 0; u < kW; ++u) {
+                            const int w0 = i0 + 64 * u, i = w0 + lane;
+                            ok[u] = i < total;
+                            ev[u] = 0;
+                            wv[u] = Acc(0);
+                            if (w0 >= total) continue;  // wave-uniform
+                            // lanes with excl <= w0 form a prefix whose last lane has postings
+                            int owner = __popcll(__ballot(excl <= w0)) - 1;
+                            const uint64_t bm = __ballot(excl > w0 && excl <= w0 + 63 && cnt > 0);
+                            for (uint64_t m = bm; m != 0; m &= m - 1) {
+                                const int l = __ffsll(static_cast<unsigned long long>(m)) - 1;
+                                if (i >= __builtin_amdgcn_readlane(excl, l)) owner = l;
+                            }
+                            const int ocs = __shfl(cs, owner, 64), oex = __shfl(excl, owner, 64);
+                            wv[u] = __shfl(dv, owner, 64);
+                            ev[u] = ocs + (i - oex);
+                        }
+                        uint32_t cl[kW], cwv[kW];
+#pragma unroll
+                        for (int u = 0; u < kW; ++u) {
+                            cl[u] = ok[u] ? cc[ev[u]] : 0u;
+                            cwv[u] = ok[u] ? static_cast<uint32_t>(cw[ev[u]]) : 0u;
+                        }
+#pragma unroll
+                        for (int u = 0; u < kW; ++u)
+                            if (ok[u]) atomicAdd(&sims[cl[u]], wv[u] * static_cast<Acc>(cwv[u]));
+                    }
                 }
-                const int m = min(64, r1 - b);
-                for (int j2 = 0; j2 < m; ++j2) {
-                    const int cs = __shfl(col.x, j2, 64), ce = __shfl(col.y, j2, 64);
-                    const Acc v = __shfl(dv, j2, 64);
-                    if (cs < 0) continue;
-                    for (int e = cs + lane; e < ce; e += 64) sims[cc[e]] += v * static_cast<Acc>(cw[e]);
-                    __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_wave_barrier();
+            } else {
+                // FP32 sims add in doc-term order, as the reference does: the
+                // terms one after another, lanes over a term's (distinct) clusters
+                for (int b = r0; b < r1; b += 64) {
+                    const int p = b + lane;
+                    int2 col = make_int2(0, 0);
+                    Acc dv = Acc(0);
+                    if (p < r1) {
+                        col = tmap[indices[p]];
+                        dv = static_cast<Acc>(values[p]);
+                    }
+                    const int m = min(64, r1 - b);
+                    for (int j2 = 0; j2 < m; ++j2) {
+                        const int cs = __shfl(col.x, j2, 64), ce = __shfl(col.y, j2, 64);
+                        const Acc v = __shfl(dv, j2, 64);
+                        if (cs < 0) continue;
+                        for (int e = cs + lane; e < ce; e += 64) sims[cc[e]] += v * static_cast<Acc>(cw[e]);
+                        __builtin_amdgcn_wave_barrier();
+                    }
                 }
             }
             Acc best = sims[0];
