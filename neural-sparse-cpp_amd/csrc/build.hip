@@ -103,8 +103,14 @@ __global__ void iota_kernel(int32_t* v, size_t n) {
     GRID_STRIDE(i, n) v[i] = static_cast<int32_t>(i);
 }
 
-__global__ void term_hist_kernel(const uint16_t* __restrict__ idx, size_t n, int32_t* __restrict__ hist) {
-    GRID_STRIDE(i, n) atomicAdd(&hist[idx[i]], 1);
+// Postings per term from the term-sorted keys: each run adds its end and
+// subtracts its start (hist is zero on entry).
+__global__ void term_runs_kernel(const uint16_t* __restrict__ keys, size_t n, int32_t* __restrict__ hist) {
+    GRID_STRIDE(i, n) {
+        const uint16_t t = keys[i];
+        if (i == 0 || keys[i - 1] != t) atomicSub(&hist[t], static_cast<int32_t>(i));
+        if (i + 1 == n || keys[i + 1] != t) atomicAdd(&hist[t], static_cast<int32_t>(i + 1));
+    }
 }
 
 __global__ void doc_of_kernel(const int32_t* __restrict__ indptr, int64_t n_docs, int32_t* __restrict__ doc_of) {
@@ -305,8 +311,10 @@ __global__ __launch_bounds__(kB) void assign_kernel(
     const uint32_t* __restrict__ ckey, const uint16_t* __restrict__ cc, const T* __restrict__ cw,
     int2* __restrict__ tmap_all, int dim, int32_t* __restrict__ assign, int l_begin, int l_end, int mc) {
     extern __shared__ __align__(16) unsigned char smem[];
-    int32_t* cent = reinterpret_cast<int32_t*>(smem);                    // sorted centroid ids [mc]
-    Acc* sims_all = reinterpret_cast<Acc*>(smem + ((mc * 4 + 15) & ~15));  // [4][mc]
+    int mc2 = 1;  // the sort's power-of-two capacity
+    while (mc2 < mc) mc2 <<= 1;
+    int32_t* cent = reinterpret_cast<int32_t*>(smem);                      // sorted centroid ids [mc2]
+    Acc* sims_all = reinterpret_cast<Acc*>(smem + ((mc2 * 4 + 15) & ~15));  // [4][mc]
     int2* tmap = tmap_all + static_cast<size_t>(blockIdx.x) * dim;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     Acc* sims = sims_all + wave * mc;
@@ -325,18 +333,23 @@ __global__ __launch_bounds__(kB) void assign_kernel(
             if (e == e0 || ckey[e - 1] != k) tmap[t].x = static_cast<int>(e);
             if (e == e1 - 1 || ckey[e + 1] != k) tmap[t].y = static_cast<int>(e + 1);
         }
-        // centroid ids, sorted (odd-even merge over the small array)
-        for (int i = tid; i < nc; i += kB) cent[i] = centroid_doc[c0 + i];
+        // centroid ids, sorted ascending (bitonic, padded to a power of two)
+        int n2 = 1;
+        while (n2 < nc) n2 <<= 1;
+        for (int i = tid; i < n2; i += kB) cent[i] = i < nc ? centroid_doc[c0 + i] : INT32_MAX;
         __syncthreads();
-        for (int pass = 0; pass < nc; ++pass) {
-            for (int i = 2 * tid + (pass & 1); i + 1 < nc; i += 2 * kB) {
-                const int32_t a = cent[i], b = cent[i + 1];
-                if (a > b) {
-                    cent[i] = b;
-                    cent[i + 1] = a;
+        for (int size = 2; size <= n2; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int t = tid; t < n2 / 2; t += kB) {
+                    const int i = 2 * t - (t & (stride - 1)), j = i + stride;
+                    const int32_t a = cent[i], b = cent[j];
+                    if ((a > b) == ((i & size) == 0)) {
+                        cent[i] = b;
+                        cent[j] = a;
+                    }
                 }
+                __syncthreads();
             }
-            __syncthreads();
         }
         for (int64_t j = wave; j < n; j += kB / 64) {
             const int32_t d = kept[kb + j];
@@ -586,33 +599,54 @@ __global__ __launch_bounds__(kB) void summarize_u8_kernel(const int32_t* __restr
         __syncthreads();
         atomicAdd(&S.misc[0], local_sum);
         __syncthreads();
-        if (tid == 0) {
+        if (wave == 0) {
             // pairs sorted by value desc; addup += value; stop at the first
-            // addup / sum >= alpha (inverted_list_clusters.cpp:43-50).
+            // addup / sum >= alpha (inverted_list_clusters.cpp:43-50). Over the
+            // value histogram: lane l holds values 255 - 4l - u (u < 4); a DPP
+            // prefix gives each value the exact sum of all higher ones (a), and
+            // every non-empty value runs the sequential loop's test
+            // float(a + v * m) / sum >= alpha; the first value that passes (in
+            // descending order) is the cut, and its lane finds how many of its
+            // equal terms are kept.
             const float sum = static_cast<float>(S.misc[0]);
-            int cut_v = -1, cut_x = 0;  // keep values > cut_v, plus cut_x terms == cut_v
-            int64_t a = 0;
-            for (int v = 255; v >= 0; --v) {
-                const int m = S.hist[v];
-                if (m == 0) continue;
-                const float full = static_cast<float>(a + static_cast<int64_t>(v) * m);
-                if (full / sum >= alpha) {
-                    int lo = 1, hi = m;
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (static_cast<float>(a + static_cast<int64_t>(v) * mid) / sum >= alpha)
-                            hi = mid;
-                        else
-                            lo = mid + 1;
-                    }
-                    cut_v = v;
-                    cut_x = lo;
-                    break;
+            int m4[4];
+            uint32_t part = 0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                m4[u] = S.hist[255 - 4 * lane - u];
+                part += static_cast<uint32_t>((255 - 4 * lane - u) * m4[u]);
+            }
+            int64_t a = static_cast<int64_t>(wave_incl_scan_dpp(part) - part);
+            int hit = -1;
+            int64_t a_hit = 0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int v = 255 - 4 * lane - u, m = m4[u];
+                if (hit < 0 && m != 0 && static_cast<float>(a + static_cast<int64_t>(v) * m) / sum >= alpha) {
+                    hit = u;
+                    a_hit = a;
                 }
                 a += static_cast<int64_t>(v) * m;
             }
-            S.misc[1] = cut_v;
-            S.misc[2] = cut_x;
+            const uint64_t hm = __ballot(hit >= 0);
+            const int first = hm != 0 ? __ffsll(static_cast<unsigned long long>(hm)) - 1 : -1;
+            if (first < 0 && lane == 0) {
+                S.misc[1] = -1;  // keep values > -1: every touched term
+                S.misc[2] = 0;
+            }
+            if (lane == first) {
+                const int v = 255 - 4 * lane - hit, m = m4[hit];
+                int lo = 1, hi = m;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (static_cast<float>(a_hit + static_cast<int64_t>(v) * mid) / sum >= alpha)
+                        hi = mid;
+                    else
+                        lo = mid + 1;
+                }
+                S.misc[1] = v;
+                S.misc[2] = lo;
+            }
         }
         __syncthreads();
         const int cut_v = S.misc[1], cut_x = S.misc[2];
@@ -630,11 +664,7 @@ __global__ __launch_bounds__(kB) void summarize_u8_kernel(const int32_t* __restr
                     tb &= tb - 1;
                     if (static_cast<int>(max_of(S.maxw, w * 32 + bb)) == cut_v) ++ties;
                 }
-                int incl = ties;
-                for (int o = 1; o < 64; o <<= 1) {
-                    const int x = __shfl_up(incl, o, 64);
-                    if (lane >= o) incl += x;
-                }
+                const int incl = static_cast<int>(wave_incl_scan_dpp(static_cast<uint32_t>(ties)));
                 int tie_rank = run_tie + incl - ties;
                 int keep = 0;
                 tb = bits;
@@ -649,17 +679,13 @@ __global__ __launch_bounds__(kB) void summarize_u8_kernel(const int32_t* __restr
                         ++tie_rank;
                     }
                 }
-                int kincl = keep;
-                for (int o = 1; o < 64; o <<= 1) {
-                    const int x = __shfl_up(kincl, o, 64);
-                    if (lane >= o) kincl += x;
-                }
+                const int kincl = static_cast<int>(wave_incl_scan_dpp(static_cast<uint32_t>(keep)));
                 if (w < nw) {
                     S.wscan[w] = run_keep + kincl - keep;
                     S.wtie[w] = run_tie + incl - ties;
                 }
-                run_tie += __shfl(incl, 63, 64);
-                run_keep += __shfl(kincl, 63, 64);
+                run_tie += __builtin_amdgcn_readlane(incl, 63);
+                run_keep += __builtin_amdgcn_readlane(kincl, 63);
             }
             if (lane == 0) S.misc[3] = run_keep;
         }
@@ -1089,12 +1115,13 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
     DeviceBuffer pos_iota(Z * 4), spos(Z * 4), skeys(Z * 2), hist(static_cast<size_t>(L) * 4), doc_of(Z * 4);
     hipLaunchKernelGGL(iota_kernel, dim3(grid_for(Z)), dim3(kB), 0, nullptr, pos_iota.get<int32_t>(), Z);
     radix_sort_pairs(fwd.indices, skeys.get<uint16_t>(), pos_iota.get<int32_t>(), spos.get<int32_t>(), Z, 16);
-    skeys.release();
     hip_check(hipMemset(hist.get(), 0, static_cast<size_t>(L) * 4), "hipMemset");
-    hipLaunchKernelGGL(term_hist_kernel, dim3(grid_for(Z)), dim3(kB), 0, nullptr, fwd.indices, Z, hist.get<int32_t>());
+    hipLaunchKernelGGL(term_runs_kernel, dim3(grid_for(Z)), dim3(kB), 0, nullptr, skeys.get<uint16_t>(), Z,
+                       hist.get<int32_t>());
     hipLaunchKernelGGL(doc_of_kernel, dim3(grid_for(static_cast<size_t>(n_docs))), dim3(kB), 0, nullptr, fwd.indptr,
                        n_docs, doc_of.get<int32_t>());
     sync("postings");
+    skeys.release();
     const auto h_hist = hist.to_host<int32_t>(L);
     std::vector<int64_t> list_start(L + 1, 0), kept_base(L + 1, 0);
     std::vector<int32_t> lcb(L + 1, 0);
@@ -1196,7 +1223,9 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
     hipLaunchKernelGGL(fill_i32_kernel, dim3(grid_for(static_cast<size_t>(assign_grid) * L * 2)), dim3(kB), 0, nullptr,
                        tmap.get<int32_t>(), static_cast<size_t>(assign_grid) * L * 2, -1);
     const int mc = std::max(1, max_clusters);
-    const size_t assign_lds = ((mc * 4 + 15) & ~15) + static_cast<size_t>(4) * mc * 4;
+    int mc2 = 1;
+    while (mc2 < mc) mc2 <<= 1;
+    const size_t assign_lds = ((mc2 * 4 + 15) & ~15) + static_cast<size_t>(4) * mc * 4;
     if (assign_lds > 64 * 1024) {
         hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(assign_kernel<uint8_t, int32_t>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(assign_lds)),
