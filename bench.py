@@ -131,6 +131,18 @@ def measured_traffic(workload):
     return None, None
 
 
+def build_bytes(n_docs, nnz, st, v=1):
+    """Bytes the reference's build moves, uint8 values (v = 1): the forward
+    index read once for list construction ((N + 1) * 4 + nnz * (2 + v)), the
+    postings written and read back (2 * nnz * (4 + v)), and every kept
+    posting's doc row gathered twice, for the assignment and the summaries
+    (2 * (8 * kept + row_nnz * (2 + v)))."""
+    fwd = (n_docs + 1) * 4 + nnz * (2 + v)
+    postings = 2 * nnz * (4 + v)
+    rows = 2 * (8 * st["kept_postings"] + st["kept_row_nnz"] * (2 + v))
+    return float(fwd + postings + rows)
+
+
 def copy_bandwidth(dev, nbytes=2 << 30, reps=5):
     """Rate of a plain device copy (bytes read + bytes written per second, GB/s)
     over buffers far larger than the last-level cache: the measured ceiling
@@ -275,7 +287,14 @@ def main():
     index.build()
     torch.cuda.synchronize()
     build_s = time.time() - t0
-    log(f"add {add_s:.1f}s, GPU build {build_s:.1f}s")
+    bst = index.build_stats()
+    b_bytes = build_bytes(args.docs, int(docs[0][-1]), bst)
+    build = dict(bst, seconds=build_s, algorithmic_bytes=b_bytes, gbps=b_bytes / build_s / 1e9,
+                 frac=b_bytes / build_s / 1e9 / HBM_PEAK_GBS, assign_pairs_per_s=bst["assign_pairs"] / build_s)
+    log(f"add {add_s:.1f}s, GPU build {build_s:.1f}s: {b_bytes / 1e9:.1f} GB algorithmic "
+        f"({build['gbps']:.1f} GB/s), {bst['assign_pairs'] / 1e9:.2f}G assignment pairs "
+        f"({build['assign_pairs_per_s'] / 1e9:.1f}G/s), {bst['kept_postings']:,} kept postings, "
+        f"{bst['clusters']:,} clusters")
 
     qip = torch.from_numpy(queries[0]).to(dev)
     qix = torch.from_numpy(queries[1].view(np.int16)).to(dev)
@@ -479,6 +498,7 @@ def main():
         "recall_vs_uint8_exact": recall_sq if n_rec else None,
         "uint8_exact_vs_fp32_exact": ceiling if n_rec else None,
         "build_s": build_s,
+        "build": build,
         "traversal_means": dict(zip(["lists", "terms_found", "csc_entries", "clusters_tested", "clusters_opened",
                                      "docs_read", "docs_scored", "nnz_scored", "probe_bytes"],
                                     st[:, :9].mean(axis=0).tolist())),

@@ -224,6 +224,12 @@ int nsparse_server_free(nsparse_server* server);
  * Reading it synchronizes the device. Extension. */
 int nsparse_device_error(nsparse_index* index, int32_t* out);
 
+/* What the last nsparse_build of this SEISMIC index did (5 int64): postings kept by
+ * lambda pruning, non-zeros of those postings' doc rows, clusters, summary entries,
+ * and (doc term, centroid posting) pairs scored by the assignment. Zero for an index
+ * read from a file. For the build's byte count and rates. Extension. */
+int nsparse_build_stats(nsparse_index* index, int64_t* out5);
+
 /* Exact scores of one query (host-free: all pointers DEVICE) against every row of
  * an FP32 CSR -- the scoring of BrutalIndex::single_query (nsparse/brutal_index.cpp:75-96),
  * used for recall ground truth. d_scores receives n_docs floats. Extension. */

@@ -309,7 +309,8 @@ __global__ __launch_bounds__(kB) void assign_kernel(
     const int64_t* __restrict__ kept_base, const int32_t* __restrict__ kept, const int32_t* __restrict__ lcb,
     const int32_t* __restrict__ centroid_doc, const int64_t* __restrict__ cent_off, int64_t e_base,
     const uint32_t* __restrict__ ckey, const uint16_t* __restrict__ cc, const T* __restrict__ cw,
-    int2* __restrict__ tmap_all, int dim, int32_t* __restrict__ assign, int l_begin, int l_end, int mc) {
+    int2* __restrict__ tmap_all, int dim, int32_t* __restrict__ assign, int l_begin, int l_end, int mc,
+    unsigned long long* __restrict__ pairs) {
     extern __shared__ __align__(16) unsigned char smem[];
     int mc2 = 1;  // the sort's power-of-two capacity
     while (mc2 < mc) mc2 <<= 1;
@@ -318,6 +319,7 @@ __global__ __launch_bounds__(kB) void assign_kernel(
     int2* tmap = tmap_all + static_cast<size_t>(blockIdx.x) * dim;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     Acc* sims = sims_all + wave * mc;
+    uint64_t n_pairs = 0;  // (doc term, centroid posting) pairs this wave scored
     for (int l = l_begin + blockIdx.x; l < l_end; l += gridDim.x) {
         const int64_t kb = kept_base[l], n = kept_base[l + 1] - kb;
         const int c0 = lcb[l], nc = lcb[l + 1] - c0;
@@ -390,6 +392,7 @@ __global__ __launch_bounds__(kB) void assign_kernel(
                     const int incl = static_cast<int>(wave_incl_scan_dpp(static_cast<uint32_t>(cnt)));
                     const int excl = incl - cnt;
                     const int total = __builtin_amdgcn_readlane(incl, 63);
+                    n_pairs += static_cast<uint64_t>(total);
                     for (int i0 = 0; i0 < total; i0 += 64 * kW) {
                         int ev[kW];
                         Acc wv[kW];
@@ -440,6 +443,7 @@ __global__ __launch_bounds__(kB) void assign_kernel(
                         const int cs = __shfl(col.x, j2, 64), ce = __shfl(col.y, j2, 64);
                         const Acc v = __shfl(dv, j2, 64);
                         if (cs < 0) continue;
+                        n_pairs += static_cast<uint64_t>(ce - cs);
                         for (int e = cs + lane; e < ce; e += 64) sims[cc[e]] += v * static_cast<Acc>(cw[e]);
                         __builtin_amdgcn_wave_barrier();
                     }
@@ -460,6 +464,16 @@ __global__ __launch_bounds__(kB) void assign_kernel(
         for (int64_t e = e0 + tid; e < e1; e += kB) tmap[ckey[e] & 0xFFFFu] = make_int2(-1, -1);
         __syncthreads();
     }
+    if (lane == 0 && n_pairs != 0) atomicAdd(pairs, static_cast<unsigned long long>(n_pairs));
+}
+
+// Non-zeros of the kept postings' rows (for the build's byte count).
+__global__ void kept_row_nnz_kernel(const int32_t* __restrict__ indptr, const int32_t* __restrict__ kept, size_t n,
+                                    unsigned long long* __restrict__ out) {
+    uint64_t local = 0;
+    GRID_STRIDE(i, n) local += static_cast<uint64_t>(indptr[kept[i] + 1] - indptr[kept[i]]);
+    for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o, 64);
+    if ((threadIdx.x & 63) == 0 && local != 0) atomicAdd(out, static_cast<unsigned long long>(local));
 }
 
 // Cluster sizes: centroid + assigned docs (per global cluster).
@@ -1183,6 +1197,14 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
                        d_kept_base.get<int64_t>(), spos.get<int32_t>(),
                        any_long ? spos2.get<int32_t>() : spos.get<int32_t>(), doc_of.get<int32_t>(),
                        kept.get<int32_t>(), L);
+    // build counters: [0] (doc term, centroid posting) pairs scored by the
+    // assignment, [1] non-zeros of the kept postings' rows
+    DeviceBuffer bstat(16);
+    hip_check(hipMemset(bstat.get(), 0, 16), "hipMemset");
+    if (K > 0)
+        hipLaunchKernelGGL(kept_row_nnz_kernel, dim3(grid_for(static_cast<size_t>(K))), dim3(kB), 0, nullptr,
+                           fwd.indptr, kept.get<int32_t>(), static_cast<size_t>(K),
+                           bstat.get<unsigned long long>() + 1);
     sync("kept docs");
     spos.release();
     spos2.release();
@@ -1269,13 +1291,14 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
                                d_kept_base.get<int64_t>(), kept.get<int32_t>(), d_lcb.get<int32_t>(),
                                centroid_doc.get<int32_t>(), cent_off.get<int64_t>(), eb, ckey.get<uint32_t>(),
                                cc.get<uint16_t>(), cw.get<uint8_t>(), tmap.get<int2>(), L, assign.get<int32_t>(), l0,
-                               l1, mc);
+                               l1, mc, bstat.get<unsigned long long>());
         else
             hipLaunchKernelGGL((assign_kernel<float, float>), dim3(grid), dim3(kB), assign_lds, nullptr, fwd.indptr,
                                fwd.indices, static_cast<const float*>(fwd.values), d_kept_base.get<int64_t>(),
                                kept.get<int32_t>(), d_lcb.get<int32_t>(), centroid_doc.get<int32_t>(),
                                cent_off.get<int64_t>(), eb, ckey.get<uint32_t>(), cc.get<uint16_t>(),
-                               cw.get<float>(), tmap.get<int2>(), L, assign.get<int32_t>(), l0, l1, mc);
+                               cw.get<float>(), tmap.get<int2>(), L, assign.get<int32_t>(), l0, l1, mc,
+                               bstat.get<unsigned long long>());
         sync("assignment");
     }
     tmap.release();
@@ -1325,6 +1348,7 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(ldsf)),
                   "hipFuncSetAttribute");
     }
+    int64_t summary_entries = 0;
     for (int pass = 0; pass < 2; ++pass) {
         DeviceBuffer sterm, sval, scl;
         int64_t E = 0;
@@ -1332,6 +1356,7 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
             exclusive_scan_i64(scount.get<int64_t>(), out_off.get<int64_t>(), static_cast<size_t>(C));
             sync("summary offsets");
             out_off.download(&E, 8, static_cast<size_t>(C) * 8);
+            summary_entries = E;
             sterm.allocate(E * 2);
             sval.allocate(E * ew, 64);
             scl.allocate(E * 2);
@@ -1443,6 +1468,13 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
     out.docs = std::move(docs);
     out.n_lists = L;
     out.max_clusters = max_clusters;
+    uint64_t counters[2] = {0, 0};
+    bstat.download(counters, 16);
+    out.stats.kept_postings = K;
+    out.stats.kept_row_nnz = static_cast<int64_t>(counters[1]);
+    out.stats.clusters = C;
+    out.stats.summary_entries = summary_entries;
+    out.stats.assign_pairs = static_cast<int64_t>(counters[0]);
 }
 
 }  // namespace nsparse_amd

@@ -384,6 +384,18 @@ int nsparse_device_error(nsparse_index* index, int32_t* out) {
     });
 }
 
+int nsparse_build_stats(nsparse_index* index, int64_t* out5) {
+    return guarded([&] {
+        if (out5 == nullptr) throw std::invalid_argument("unexpected nullptr");
+        const BuildStats& s = seismic_of(index).build_stats();
+        out5[0] = s.kept_postings;
+        out5[1] = s.kept_row_nnz;
+        out5[2] = s.clusters;
+        out5[3] = s.summary_entries;
+        out5[4] = s.assign_pairs;
+    });
+}
+
 int nsparse_exact_scores_device(int64_t n_docs, const int32_t* d_indptr, const uint16_t* d_indices,
                                 const float* d_values, int dim, int q_len, const uint16_t* d_q_indices,
                                 const float* d_q_values, float* d_scores, void* stream) {

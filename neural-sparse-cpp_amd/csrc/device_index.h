@@ -71,6 +71,15 @@ enum StatField {
     kStatFields
 };
 
+// What a device build did, for its byte count and rates.
+struct BuildStats {
+    int64_t kept_postings = 0;    // postings kept by lambda pruning
+    int64_t kept_row_nnz = 0;     // non-zeros of those postings' doc rows
+    int64_t clusters = 0;
+    int64_t summary_entries = 0;
+    int64_t assign_pairs = 0;     // (doc term, centroid posting) pairs scored by the assignment
+};
+
 struct SearchArgs {
     DevIndexView ix;
     const int32_t* q_indptr;   // device, n+1 (absolute offsets)

@@ -207,6 +207,7 @@ void SeismicIndex::build() {
     d_list_entry_base_ = std::move(out.list_entry_base);
     n_lists_ = out.n_lists;
     max_clusters_ = out.max_clusters;
+    build_stats_ = out.stats;
     lists_.clear();
     host_lists_valid_ = false;
     dev_lists_valid_ = true;

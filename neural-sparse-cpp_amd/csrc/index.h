@@ -114,6 +114,8 @@ public:
     bool sq() const { return sq_; }
     const DevIndexView& view();
     int max_clusters() const { return max_clusters_; }
+    // Counters of the last build() (zero for an index read from a file).
+    const BuildStats& build_stats() const { return build_stats_; }
     void set_lists(std::vector<HostList>&& lists);
     const std::vector<HostList>& lists();
 
@@ -135,6 +137,7 @@ private:
     bool dev_fwd_valid_ = false;
     int32_t n_lists_ = 0;
     int max_clusters_ = 0;
+    BuildStats build_stats_{};
     // device copies
     DeviceBuffer d_indptr_, d_indices_, d_values_;
     DeviceBuffer d_list_cluster_base_, d_cluster_doc_ptr_, d_docs_, d_list_term_base_, d_sum_term_ids_,

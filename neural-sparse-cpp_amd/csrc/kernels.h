@@ -36,6 +36,7 @@ struct DeviceLists {
         sum_value, list_entry_base;
     int32_t n_lists = 0;
     int max_clusters = 0;
+    BuildStats stats;
 };
 
 // ScalarQuantizer::encode (8-bit) of host values on the GPU; codes come back to the host.

@@ -117,6 +117,7 @@ _sig("nsparse_search_device", [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, 
 _sig("nsparse_search_device_stats", [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                      P(SearchParams), c_void_p, c_void_p, c_void_p, c_void_p])
 _sig("nsparse_device_error", [c_void_p, P(ctypes.c_int32)])
+_sig("nsparse_build_stats", [c_void_p, P(ctypes.c_int64)])
 _sig("nsparse_search_plan_create", [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                     P(SearchParams), c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                     P(SearchConfig), c_void_p, P(c_void_p)])

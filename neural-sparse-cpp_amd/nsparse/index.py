@@ -251,6 +251,13 @@ class Index:
         check(lib.nsparse_device_error(self._h, ctypes.byref(v)))
         return v.value
 
+    def build_stats(self):
+        """What the last build() did: kept postings, their rows' non-zeros,
+        clusters, summary entries, assignment (doc term, centroid posting) pairs."""
+        v = (ctypes.c_int64 * 5)()
+        check(lib.nsparse_build_stats(self._h, v))
+        return dict(zip(["kept_postings", "kept_row_nnz", "clusters", "summary_entries", "assign_pairs"], v))
+
     def read_csr(self, path, residency=0):
         check(lib.nsparse_read_csr(self._h, str(path).encode(), int(residency)))
 

@@ -102,3 +102,31 @@ def test_build_in_many_list_batches_matches_oracle(nsparse, monkeypatch, sq):
     monkeypatch.delenv("NSPARSE_BUILD_BATCH_ENTRIES")
     want = oracle_bytes(docs, sq, 250, 25, 0.4, 2, 0.0, np.float32(vmax) if sq else 1.0)
     assert got == want
+
+
+@pytest.mark.parametrize("sq", [1, 0])
+@pytest.mark.parametrize("lam,beta", [(100000, 7), (150, 20)])
+def test_build_counters(nsparse, sq, lam, beta):
+    """The build's counters against closed forms: kept postings are
+    sum_t min(count_t, lambda); clusters are sum_t min(beta, kept_t); without
+    pruning every doc sits in nnz_d lists, so the kept rows hold sum_d nnz_d^2
+    non-zeros; summary entries and assignment pairs are counted."""
+    from nsparse import synth
+
+    ip, ix, v = synth.make_corpus(synth.doc_spec(seed=13), 3000)
+    vmax = float(v.max())
+    spec = (f"seismic_sq,quantizer=8bit|vmin=0|vmax={vmax:.9g}|lambda={lam}|beta={beta}|alpha=0.4|seed=1" if sq
+            else f"seismic,lambda={lam}|beta={beta}|alpha=0.4|seed=1")
+    index = nsparse.index_factory(DIM, spec)
+    index.add(len(ip) - 1, ip, ix, v)
+    index.build()
+    st = index.build_stats()
+    cnt = np.bincount(ix.astype(np.int64), minlength=DIM)
+    kept = np.minimum(cnt, lam)
+    assert st["kept_postings"] == int(kept.sum())
+    assert st["clusters"] == int(np.minimum(kept, beta).sum())
+    if lam >= cnt.max():
+        assert st["kept_row_nnz"] == int((np.diff(ip).astype(np.int64) ** 2).sum())
+    else:
+        assert 0 < st["kept_row_nnz"] < int((np.diff(ip).astype(np.int64) ** 2).sum())
+    assert st["summary_entries"] > 0 and st["assign_pairs"] > 0
