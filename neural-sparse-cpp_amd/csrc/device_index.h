@@ -101,6 +101,7 @@ struct SearchArgs {
     int32_t* error_flag;       // device; set to a nonzero code on a rejected query
     int64_t n_queries;
     int64_t query_base;        // first query handled by this launch (one-workgroup-per-query kernel)
+    int32_t spin_max = 1 << 22;  // polls before a wait between waves gives up (kDevChunkWait)
     int32_t* queue;            // persistent kernel: device counter, zero at launch
     const int32_t* order;      // persistent kernel: query of each counter value, or nullptr (identity)
 };
@@ -120,7 +121,8 @@ enum DeviceError : int32_t {
     kDevQueryTooLong = 1,   // query nnz above the device limit
     kDevTermOutOfRange = 2, // query term id >= dim
     kDevTooManyClusters = 3, // list with more clusters than the LDS plan allows
-    kDevBadK = 4            // per-query k outside [1, k_max]
+    kDevBadK = 4,           // per-query k outside [1, k_max]
+    kDevChunkWait = 5       // a wait between the waves of a search timed out
 };
 
 }  // namespace nsparse_amd

@@ -93,8 +93,19 @@ enum ChunkMeta {
     kChunkTotal,     // staged doc slots
     kChunkWords,     // 8-element words of the staged rows
     kChunkNext,      // next window (or 64-slot group) to score
+    kChunkEnd,       // 1: no chunk here, the list's traversal is over
+    kChunkStamp,     // index of the chunk this slot holds, written last (release)
+    kChunkCheckin,   // waves that finished scoring the chunk
     kChunkMetaCount
 };
+
+__device__ __forceinline__ int lds_load_acquire(const int32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void lds_store_release(int32_t* p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 enum Misc {
     kMiscErr = 0,
@@ -1003,6 +1014,9 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                     score_windows(std::true_type{});
                 else
                     score_windows(std::false_type{});
+                // this wave's sums are in (release: its LDS atomics come first)
+                if (lane == 0)
+                    __hip_atomic_fetch_add(&C.meta[kChunkCheckin], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
             } else {
                 // FP32 rows: one doc per lane, 64-slot groups from the counter;
                 // each row is summed in order (as the reference's loop does)
@@ -1020,6 +1034,8 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                     for (int pp = rs; pp < rs + len; ++pp) acc += vf[pp] * QL.value(ix.indices[pp]);
                     C.sc[j] = acc;
                 }
+                if (lane == 0)
+                    __hip_atomic_fetch_add(&C.meta[kChunkCheckin], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         };
 
@@ -1167,65 +1183,107 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
             return false;
         };
 
+        // Spins (uniform over the wave, s_sleep between polls) until *p == v.
+        // After a.spin_max polls (about a quarter second by default) it reports
+        // kDevChunkWait and returns false; once that is reported, every wait
+        // of the launch returns false at once, so a launch always ends.
+        auto wait_for = [&](const int32_t* p, int v) {
+            for (int it = 0; lds_load_acquire(p) != v; ++it) {
+                if ((it & 1023) == 0 &&
+                    __hip_atomic_load(a.error_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == kDevChunkWait)
+                    return false;
+                if (it >= a.spin_max) {
+                    if (lane == 0) atomicCAS(a.error_flag, 0, kDevChunkWait);
+                    return false;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            return true;
+        };
+
         // prologue: chunk 0 planned by wave 0, gathered by every wave
         {
             const Chunk C0 = chunk_at(0, 0);
-            if (wave == 0) plan_chunk(C0, misc[kMiscTkN]);
+            if (wave == 0) {
+                plan_chunk(C0, misc[kMiscTkN]);
+                if (lane < 3) cmeta[lane * kChunkMetaCount + kChunkStamp] = -1;
+                if (lane == 0) C0.meta[kChunkCheckin] = 0;
+            }
             __syncthreads();
             tick(3);
             gather_slots(std::integral_constant<int, (kBud + kBlock - 1) / kBlock>{}, C0, tid, kBlock, [] {});
             __syncthreads();
-            if (wave == 0) scan_words(C0);
+            if (wave == 0) {
+                scan_words(C0);
+                if (lane == 0) {
+                    C0.meta[kChunkEnd] = C0.meta[kChunkNseg] == 0 ? 1 : 0;
+                    C0.meta[kChunkStamp] = 0;
+                }
+            }
             __syncthreads();
             tick(4);
         }
-        // Iteration i scores chunk i with every wave. Wave 0 first plans chunk
-        // i + 1 (against the k-th score after chunk i - 2's replay), issues its
-        // gather loads, replays chunk i - 1 while they are in flight, then
-        // stages and scans chunk i + 1 and joins the scoring.
-        bool have_prev = false;
-        for (int cb = 0, cm = 0;; cb ^= 1, cm = cm == 2 ? 0 : cm + 1) {
-            const Chunk cur = chunk_at(cb, cm);
-            const bool cur_live = cur.meta[kChunkNseg] > 0;
-            if (wave == 0) {
-                const Chunk next = chunk_at(cb ^ 1, cm == 2 ? 0 : cm + 1);
+        // Chunk i sits in staging buffer i & 1 and slot i % 3, published by its
+        // stamp. Every wave scores chunk i as soon as it is published and then
+        // checks in on it; no barrier separates the chunks. Wave 0, in its
+        // iteration i, plans chunk i + 1 (against the k-th score after chunk
+        // i - 2's replay) and issues its gather loads; meanwhile it waits for
+        // every wave's check-in on chunk i - 1 and replays that chunk; then it
+        // stages and scans chunk i + 1, publishes it (or an end marker after a
+        // first-list break or the last cluster), and joins the scoring of
+        // chunk i. The check-in wait keeps every wave within one chunk of wave
+        // 0, so the buffers chunk i + 1 reuses (those of chunks i - 1 and
+        // i - 2) are free.
+        if (wave == 0) {
+            bool have_prev = false, stop = false;
+            for (int i = 0, cb = 0, cm = 0;; ++i, cb ^= 1, cm = cm == 2 ? 0 : cm + 1) {
+                const Chunk cur = chunk_at(cb, cm);
                 const Chunk prev = chunk_at(cb ^ 1, cm == 0 ? 2 : cm - 1);
+                const Chunk next = chunk_at(cb ^ 1, cm == 2 ? 0 : cm + 1);
                 int tkn = misc[kMiscTkN];
-                bool done = false;
                 auto replay_prev = [&] {
                     if (!have_prev) return;
                     tick(4);
-                    done = replay_chunk(prev, tkn);
-                    if (lane == 0) {
-                        misc[kMiscTkN] = tkn;
-                        if (done) misc[kMiscDone] = 1;
+                    wait_for(&prev.meta[kChunkCheckin], kWaves);
+                    if (!stop) {
+                        stop = replay_chunk(prev, tkn);
+                        if (lane == 0) misc[kMiscTkN] = tkn;
+                        __builtin_amdgcn_wave_barrier();
                     }
-                    __builtin_amdgcn_wave_barrier();
                     tick(6);
                 };
-                if (cur_live) {
-                    plan_chunk(next, tkn);
-                    tick(3);
-                    gather_slots(std::integral_constant<int, (kBud / 64 < 8 ? kBud / 64 : 8)>{}, next, lane, 64,
-                                 replay_prev);
-                    scan_words(next);
-                    tick(4);
-                } else {
+                if (cur.meta[kChunkEnd] != 0) {
                     replay_prev();
+                    // every planned chunk is replayed; a break found by the
+                    // planner is the prune test at the position where the visit stops
+                    if (!stop && misc[kMiscBreakPending]) stat_add(kStatClustersTested, 1);
+                    break;
                 }
+                plan_chunk(next, tkn);
+                if (lane == 0) next.meta[kChunkCheckin] = 0;
+                tick(3);
+                gather_slots(std::integral_constant<int, (kBud / 64 < 8 ? kBud / 64 : 8)>{}, next, lane, 64,
+                             replay_prev);
+                scan_words(next);
+                if (lane == 0) {
+                    next.meta[kChunkEnd] = (stop || next.meta[kChunkNseg] == 0) ? 1 : 0;
+                    lds_store_release(&next.meta[kChunkStamp], i + 1);
+                }
+                __builtin_amdgcn_wave_barrier();
+                tick(4);
+                score_chunk(cur);
+                tick(5);
+                have_prev = true;
             }
-            score_chunk(cur);
-            __syncthreads();
-            tick(5);
-            if (misc[kMiscDone]) break;  // first-list break found by the replay
-            if (!cur_live) {
-                // every planned chunk is replayed; a break found by the planner
-                // is the prune test at the position where the visit stops
-                if (misc[kMiscBreakPending]) stat_add(kStatClustersTested, 1);
-                break;
+        } else {
+            for (int i = 0, cb = 0, cm = 0;; ++i, cb ^= 1, cm = cm == 2 ? 0 : cm + 1) {
+                const Chunk cur = chunk_at(cb, cm);
+                if (!wait_for(&cur.meta[kChunkStamp], i)) break;
+                if (cur.meta[kChunkEnd] != 0) break;
+                score_chunk(cur);
             }
-            have_prev = true;
         }
+        __syncthreads();
         first_list = false;
         __syncthreads();
     }
@@ -1641,6 +1699,7 @@ std::string device_error_message(int32_t code) {
         case kDevTermOutOfRange: return "query term id out of range";
         case kDevTooManyClusters: return "posting list has more clusters than the device plan";
         case kDevBadK: return "per-query k outside [1, k_max]";
+        case kDevChunkWait: return "a wait between the waves of a search workgroup timed out";
         default: return "device search reported error " + std::to_string(code);
     }
 }
@@ -1767,8 +1826,11 @@ void launch_search(const SearchArgs& args_in, int max_clusters, int kmax, bool s
     SearchArgs a = args_in;
     a.queue = nullptr;
     a.order = nullptr;
+    if (const char* e = std::getenv("NSPARSE_SEARCH_SPIN_LIMIT")) a.spin_max = std::max(1, std::atoi(e));
     // the error flag reports this launch (a kernel, not a memset: graph-safe)
-    if (a.error_flag != nullptr) hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, a.error_flag, 1);
+    // kernels read the flag too (a timed-out wait ends the launch's other waits)
+    if (a.error_flag == nullptr) throw std::invalid_argument("search launch without an error flag");
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, a.error_flag, 1);
     if (cfg.persistent) {
         if (scratch == nullptr) throw std::invalid_argument("the persistent schedule needs a scratch buffer");
         int32_t* base = static_cast<int32_t*>(scratch);

@@ -276,3 +276,31 @@ def test_sq_search_with_and_without_block_rows(nsparse, data, monkeypatch):
             want = o.search(ip, ix, v, 10, cut=10, hf=0.8)
             got = dev.search(len(ip) - 1, ip, ix, v, 10, params(nsparse, 1, 10, 0.8, vmax))
             assert_same(got, want)
+
+
+def test_waits_between_waves_end_with_a_one_poll_limit(nsparse, data, sq_pair, monkeypatch):
+    """The waves of a search workgroup wait for each other on LDS flags, each
+    wait bounded. With the bound forced to one poll the launch still ends:
+    every result matches the normal search, or the launch reports error 5
+    (a wait timed out); a normal search afterwards is clean."""
+    import torch
+
+    o, dev, vmax = sq_pair
+    ip, ix, v = data[2]
+    n = len(ip) - 1
+    q = (torch.from_numpy(ip).cuda(), torch.from_numpy(ix.view(np.int16)).cuda(), torch.from_numpy(v).cuda())
+    p = params(nsparse, 1, 10, 0.8, vmax)
+    wd, wl = (t.cpu().numpy() for t in dev.search_device(*q, 10, params=p))
+    torch.cuda.synchronize()
+    assert dev.device_error() == 0
+    monkeypatch.setenv("NSPARSE_SEARCH_SPIN_LIMIT", "1")
+    gd, gl = (t.cpu().numpy() for t in dev.search_device(*q, 10, params=p))
+    err = dev.device_error()
+    assert err in (0, 5)
+    if err == 0:
+        assert np.array_equal(gl, wl) and np.array_equal(gd, wd)
+    monkeypatch.delenv("NSPARSE_SEARCH_SPIN_LIMIT")
+    gd, gl = (t.cpu().numpy() for t in dev.search_device(*q, 10, params=p))
+    assert dev.device_error() == 0
+    assert np.array_equal(gl, wl) and np.array_equal(gd, wd)
+    assert n > 0
