@@ -1865,7 +1865,14 @@ void launch_search(const SearchArgs& args_in, int max_clusters, int kmax, bool s
 }
 
 void launch_search_checked(const SearchArgs& args, int max_clusters, int kmax, bool sq, hipStream_t stream) {
-    launch_search(args, max_clusters, kmax, sq, default_search_config(sq), nullptr, stream);
+    SearchConfig c = default_search_config(sq);
+    // A launch of few queries (the per-query drop-in, small serving batches)
+    // leaves most CUs idle: one 1024-lane workgroup per query, alone on its
+    // CU, finishes it sooner than a 512-lane one. NSPARSE_SEARCH_BLOCK wins.
+    if (sq && c.lookup == 1 && std::getenv("NSPARSE_SEARCH_BLOCK") == nullptr &&
+        args.n_queries <= std::max(1, device_cus() / 4))
+        c.block = 1024;
+    launch_search(args, max_clusters, kmax, sq, c, nullptr, stream);
 }
 
 void launch_exact_checked(const ExactArgs& a, bool sq, hipStream_t stream) {
