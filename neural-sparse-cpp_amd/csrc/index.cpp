@@ -503,10 +503,11 @@ void SeismicIndex::search_host(idx_t n, const idx_t* indptr, const term_t* indic
 void SeismicIndex::search_device(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices,
                                  const float* d_values, const int32_t* d_ks, int k_max, const SearchRequest& req,
                                  const int32_t* d_sel_of_query, const uint32_t* const* d_bitmaps, float* d_out_d,
-                                 int32_t* d_out_l, int64_t* d_stats, hipStream_t stream) {
+                                 int32_t* d_out_l, int64_t* d_stats, hipStream_t stream, int32_t* d_err) {
     if (!has_vectors_ && n > 0 && k_max > 0) return;
-    const SearchArgs a = search_args(n, d_indptr, d_indices, d_values, d_ks, k_max, req, d_sel_of_query, d_bitmaps,
-                                     d_out_d, d_out_l, d_stats);
+    SearchArgs a = search_args(n, d_indptr, d_indices, d_values, d_ks, k_max, req, d_sel_of_query, d_bitmaps,
+                               d_out_d, d_out_l, d_stats);
+    if (d_err != nullptr) a.error_flag = d_err;
     launch_search_checked(a, max_clusters_, k_max, sq_, stream);
 }
 

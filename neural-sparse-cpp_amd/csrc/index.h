@@ -95,10 +95,12 @@ public:
     const HostCsr* vectors() const override { return has_vectors_ ? &fwd_ : nullptr; }
 
     // Batched search over queries already in HBM (nsparse_search_device).
+    // d_err: the launch's device error flag; nullptr uses the index's (which
+    // last_device_error reads). Launches that may overlap pass flags of their own.
     void search_device(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices, const float* d_values,
                        const int32_t* d_ks, int k_max, const SearchRequest& req, const int32_t* d_sel_of_query,
                        const uint32_t* const* d_bitmaps, float* d_out_d, int32_t* d_out_l, int64_t* d_stats,
-                       hipStream_t stream);
+                       hipStream_t stream, int32_t* d_err = nullptr);
     // The kernel arguments of a batched search (validated), as search_device uses them.
     SearchArgs search_args(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices, const float* d_values,
                            const int32_t* d_ks, int k_max, const SearchRequest& req, const int32_t* d_sel_of_query,

@@ -44,6 +44,8 @@ Server::Server(SeismicIndex& index, const ServerConfig& cfg) : index_(index), cf
         s.h_v = pinned<float>(Q);
         s.h_dist = pinned<float>(O);
         s.h_lab = pinned<int32_t>(O);
+        s.h_err = pinned<int32_t>(1);
+        s.d_err.allocate(4);
         s.d_ip.allocate((B + 1) * 4);
         s.d_ks.allocate(B * 4);
         s.d_sel.allocate(B * 4);
@@ -68,6 +70,7 @@ Server::~Server() {
         (void)hipHostFree(s.h_v);
         (void)hipHostFree(s.h_dist);
         (void)hipHostFree(s.h_lab);
+        (void)hipHostFree(s.h_err);
         if (s.done != nullptr) (void)hipEventDestroy(s.done);
         if (s.stream != nullptr) (void)hipStreamDestroy(s.stream);
     }
@@ -172,6 +175,7 @@ bool Server::reap() {
         const hipError_t e = hipEventQuery(s.done);
         if (e == hipErrorNotReady) continue;
         hip_check(e, "server batch");
+        if (*s.h_err != 0) throw std::runtime_error("server batch: " + device_error_message(*s.h_err));
         const int64_t t_done = now_ns();
         std::lock_guard<std::mutex> lk(mu_);
         for (size_t b = 0; b < s.tickets.size(); ++b) {
@@ -251,7 +255,8 @@ void Server::launch(Slot& s, const std::vector<int64_t>& batch) {
     index_.search_device(n, s.d_ip.get<int32_t>(), s.d_ix.get<uint16_t>(), s.d_v.get<float>(), s.d_ks.get<int32_t>(),
                          cfg_.k_max, cfg_.req, with_sel ? s.d_sel.get<int32_t>() : nullptr,
                          with_sel ? d_bitmap_table_.get<const uint32_t*>() : nullptr, s.d_dist.get<float>(),
-                         s.d_lab.get<int32_t>(), nullptr, s.stream);
+                         s.d_lab.get<int32_t>(), nullptr, s.stream, s.d_err.get<int32_t>());
+    hip_check(hipMemcpyAsync(s.h_err, s.d_err.get(), 4, hipMemcpyDeviceToHost, s.stream), "D2H");
     const size_t out = static_cast<size_t>(n) * cfg_.k_max;
     hip_check(hipMemcpyAsync(s.h_dist, s.d_dist.get(), out * 4, hipMemcpyDeviceToHost, s.stream), "D2H");
     hip_check(hipMemcpyAsync(s.h_lab, s.d_lab.get(), out * 4, hipMemcpyDeviceToHost, s.stream), "D2H");

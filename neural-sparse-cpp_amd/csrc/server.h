@@ -88,7 +88,8 @@ private:
         int32_t *h_ip = nullptr, *h_ks = nullptr, *h_sel = nullptr, *h_lab = nullptr;
         uint16_t* h_ix = nullptr;
         float *h_v = nullptr, *h_dist = nullptr;
-        DeviceBuffer d_ip, d_ix, d_v, d_ks, d_sel, d_dist, d_lab;
+        int32_t* h_err = nullptr;  // the batch's device error flag, copied back
+        DeviceBuffer d_ip, d_ix, d_v, d_ks, d_sel, d_dist, d_lab, d_err;
     };
 
     void run();
