@@ -1010,7 +1010,74 @@ __global__ __launch_bounds__(kB) void block_copy_kernel(const int32_t* __restric
     }
 }
 
+// InvertedIndex postings: the term-sorted positions become (doc, value)
+// pairs; each term's maximum positive value is raised through the bit
+// pattern (positive floats order as their bits), with the atomic skipped once
+// the stored maximum is already at least as large; a doc that repeats inside
+// one list (a row listing a term twice) raises `dup`.
+__global__ void gather_postings_kernel(const uint16_t* __restrict__ skeys, const int32_t* __restrict__ spos,
+                                       const int32_t* __restrict__ doc_of, const float* __restrict__ values,
+                                       size_t n, int32_t* __restrict__ docs, float* __restrict__ vals,
+                                       uint32_t* __restrict__ max_bits, int32_t* __restrict__ dup) {
+    GRID_STRIDE(i, n) {
+        const int32_t p = spos[i];
+        const int32_t d = doc_of[p];
+        const float v = values[p];
+        docs[i] = d;
+        vals[i] = v;
+        const uint16_t t = skeys[i];
+        if (v > 0.0F) {
+            const uint32_t b = __float_as_uint(v);
+            if (b > max_bits[t]) atomicMax(&max_bits[t], b);
+        }
+        if (i > 0 && skeys[i - 1] == t && doc_of[spos[i - 1]] == d) *dup = 1;
+    }
+}
+
+__global__ void widen_i32_kernel(const int32_t* __restrict__ in, int64_t* __restrict__ out, size_t n) {
+    GRID_STRIDE(i, n) out[i] = in[i];
+}
+
 }  // namespace
+
+// ArrayInvertedLists::build_inverted_lists (invlists/inverted_lists.cpp:159-180):
+// a stable radix sort of the row entries by term keeps doc order (and row
+// order inside a doc) within each list; InvertedIndex::build then takes each
+// list's maximum from 0 (InvertedList::max_value, inverted_lists.cpp:88-96).
+void build_postings_on_device(const int32_t* indptr, const uint16_t* indices, const float* values, int64_t n_docs,
+                              int64_t nnz, int dim, DevicePostings& out) {
+    const size_t Z = static_cast<size_t>(nnz), L = static_cast<size_t>(dim);
+    out.list_ptr.allocate((L + 1) * 8);
+    out.max_score.allocate(L * 4);
+    out.docs.allocate(Z * 4, 64);
+    out.vals.allocate(Z * 4, 64);
+    out.n_postings = nnz;
+    hip_check(hipMemset(out.max_score.get(), 0, L * 4), "hipMemset");
+    DeviceBuffer dup(4);
+    hip_check(hipMemset(dup.get(), 0, 4), "hipMemset");
+    if (Z == 0) {
+        hip_check(hipMemset(out.list_ptr.get(), 0, (L + 1) * 8), "hipMemset");
+    } else {
+        DeviceBuffer pos_iota(Z * 4), spos(Z * 4), skeys(Z * 2), hist(L * 4), hist64(L * 8), doc_of(Z * 4);
+        hipLaunchKernelGGL(iota_kernel, dim3(grid_for(Z)), dim3(kB), 0, nullptr, pos_iota.get<int32_t>(), Z);
+        radix_sort_pairs(indices, skeys.get<uint16_t>(), pos_iota.get<int32_t>(), spos.get<int32_t>(), Z, 16);
+        hip_check(hipMemset(hist.get(), 0, L * 4), "hipMemset");
+        hipLaunchKernelGGL(term_runs_kernel, dim3(grid_for(Z)), dim3(kB), 0, nullptr, skeys.get<uint16_t>(), Z,
+                           hist.get<int32_t>());
+        hipLaunchKernelGGL(widen_i32_kernel, dim3(grid_for(L)), dim3(kB), 0, nullptr, hist.get<int32_t>(),
+                           hist64.get<int64_t>(), L);
+        exclusive_scan_i64(hist64.get<int64_t>(), out.list_ptr.get<int64_t>(), L);
+        hipLaunchKernelGGL(doc_of_kernel, dim3(grid_for(static_cast<size_t>(n_docs))), dim3(kB), 0, nullptr, indptr,
+                           n_docs, doc_of.get<int32_t>());
+        hipLaunchKernelGGL(gather_postings_kernel, dim3(grid_for(Z)), dim3(kB), 0, nullptr, skeys.get<uint16_t>(),
+                           spos.get<int32_t>(), doc_of.get<int32_t>(), values, Z, out.docs.get<int32_t>(),
+                           out.vals.get<float>(), out.max_score.get<uint32_t>(), dup.get<int32_t>());
+        sync("inverted postings");
+    }
+    int32_t h_dup = 0;
+    dup.download(&h_dup, 4);
+    out.duplicate_docs = h_dup != 0;
+}
 
 bool build_block_rows(const DevIndexView& ix, int64_t n_postings, DeviceBuffer& row_ref, DeviceBuffer& blk_indices,
                       DeviceBuffer& blk_codes, int64_t& n_words) {

@@ -875,6 +875,212 @@ void BrutalIndex::write_payload(ByteWriter&) const {
 
 void BrutalIndex::read_payload(ByteReader&) { throw std::runtime_error("Index does not support serialization"); }
 
+// ============================================================== InvertedIndex
+
+InvertedIndex::InvertedIndex(int dim) : Index(dim) {
+    pending_.esz = 4;
+    pending_.dim = dim > 0 ? static_cast<size_t>(dim) : 0;
+}
+
+// InvertedIndex::add (inverted_index.cpp:218-234): FP32 rows appended as given;
+// term ids are checked by build(), as the reference's are.
+void InvertedIndex::add(idx_t n, const idx_t* indptr, const term_t* indices, const float* values) {
+    throw_if_not_positive(n);
+    throw_if_any_null(indptr, indices, values);
+    for (idx_t i = 0; i < n; ++i)
+        if (indptr[i + 1] < indptr[i]) throw std::invalid_argument("add: indptr is not monotonic");
+    if (indptr[0] < 0) throw std::invalid_argument("add: indptr must not be negative");
+    const int64_t nnz = indptr[n];
+    if (static_cast<int64_t>(pending_.indices.size()) + nnz > std::numeric_limits<idx_t>::max())
+        throw std::invalid_argument("too many non-zeros for 32-bit offsets");
+    if (pending_.indptr.empty()) pending_.indptr.push_back(0);
+    const idx_t off = pending_.indptr.back();
+    pending_.indices.insert(pending_.indices.end(), indices, indices + nnz);
+    const size_t old_vals = pending_.values.size();
+    pending_.values.resize(old_vals + static_cast<size_t>(nnz) * 4);
+    std::memcpy(pending_.values.data() + old_vals, values, static_cast<size_t>(nnz) * 4);
+    for (idx_t i = 1; i <= n; ++i) pending_.indptr.push_back(indptr[i] + off);
+    n_vectors_ += n;
+}
+
+// InvertedIndex::build (inverted_index.cpp:236-252) on the device. Like the
+// reference's, it consumes the vectors added since the last build; with none,
+// where the reference dereferences its released vectors, it raises.
+void InvertedIndex::build() {
+    if (pending_.indptr.empty()) throw std::runtime_error("no vectors added since the last build");
+    if (dim_ <= 0) throw std::invalid_argument("value must be positive");
+    for (term_t t : pending_.indices)
+        if (t >= dim_) throw std::invalid_argument("term_id out of range");  // ArrayInvertedLists::add_entries
+    require_device();
+    const int64_t n = pending_.n(), nnz = static_cast<int64_t>(pending_.indices.size());
+    {
+        DeviceBuffer d_ip, d_ix, d_v;
+        d_ip.upload(pending_.indptr.data(), pending_.indptr.size() * 4);
+        d_ix.upload(pending_.indices.data(), pending_.indices.size() * 2);
+        d_v.upload(pending_.values.data(), pending_.values.size());
+        build_postings_on_device(d_ip.get<int32_t>(), d_ix.get<uint16_t>(), d_v.get<float>(), n, nnz, dim_, dev_);
+    }
+    pending_ = HostCsr{};
+    pending_.esz = 4;
+    pending_.dim = static_cast<size_t>(dim_);
+    list_docs_ = n;
+    n_lists_ = dim_;
+    built_ = true;
+    dev_valid_ = true;
+    host_valid_ = false;
+}
+
+void InvertedIndex::ensure_device() {
+    if (dev_valid_) return;
+    require_device();
+    dev_.list_ptr.upload(h_list_ptr_);
+    dev_.docs.upload(h_docs_, 64);
+    dev_.vals.upload(h_vals_, 64);
+    dev_.n_postings = static_cast<int64_t>(h_docs_.size());
+    dev_.duplicate_docs = false;
+    for (size_t t = 0; t + 1 < h_list_ptr_.size() && !dev_.duplicate_docs; ++t)
+        for (int64_t i = h_list_ptr_[t] + 1; i < h_list_ptr_[t + 1]; ++i)
+            if (h_docs_[i] == h_docs_[i - 1]) {
+                dev_.duplicate_docs = true;
+                break;
+            }
+    dev_valid_ = true;
+}
+
+void InvertedIndex::ensure_host() const {
+    if (host_valid_) return;
+    const size_t L = static_cast<size_t>(n_lists_);
+    h_list_ptr_ = dev_.list_ptr.to_host<int64_t>(L + 1);
+    h_docs_ = dev_.docs.to_host<idx_t>(static_cast<size_t>(dev_.n_postings));
+    h_vals_ = dev_.vals.to_host<float>(static_cast<size_t>(dev_.n_postings));
+    h_max_ = dev_.max_score.to_host<float>(L);
+    host_valid_ = true;
+}
+
+// InvertedIndex::search (inverted_index.cpp:254-293): the selector is read off
+// the base SearchParameters, whatever its kind, and filters every doc.
+void InvertedIndex::search(idx_t n, const idx_t* indptr, const term_t* indices, const float* values, int k,
+                           float* distances, idx_t* labels, const nsparse_search_params* params) {
+    throw_if_not_positive(n);
+    throw_if_not_positive(k);
+    throw_if_any_null(indptr, indices, values, labels, distances);
+    const SearchRequest req = request_from(params);
+    SelectorEval sel(req.selector);
+    if (!built_) return;
+    std::vector<uint32_t> allow;
+    if (sel.active()) {
+        allow.assign(static_cast<size_t>((list_docs_ + 31) / 32), 0);
+        for (int64_t d = 0; d < list_docs_; ++d)
+            if (sel.is_member(d)) allow[d >> 5] |= 1U << (d & 31);
+    }
+    search_allowed(n, indptr, indices, values, k, distances, labels, sel.active() ? &allow : nullptr);
+}
+
+// An index that was never built leaves the outputs as they are: the
+// reference's per-query results are then empty (inverted_index.cpp:258-261).
+void InvertedIndex::search_allowed(idx_t n, const idx_t* indptr, const term_t* indices, const float* values, int k,
+                                   float* distances, idx_t* labels, const std::vector<uint32_t>* allow) {
+    throw_if_not_positive(n);
+    throw_if_not_positive(k);
+    throw_if_any_null(indptr, indices, values, labels, distances);
+    if (!built_) return;
+    check_csr(n, indptr, indices, dim_, "query");
+    if (k > kBruteKMax)
+        throw std::invalid_argument("inverted search supports k <= " + std::to_string(kBruteKMax) + " on the device");
+    for (idx_t i = 0; i < n; ++i)
+        if (indptr[i + 1] - indptr[i] > kDeviceQueryMax)
+            throw std::invalid_argument("query has more non-zeros than the device path supports (" +
+                                        std::to_string(kDeviceQueryMax) + ")");
+    if (list_docs_ == 0) {
+        std::fill(distances, distances + static_cast<int64_t>(n) * k, -1.0F);
+        std::fill(labels, labels + static_cast<int64_t>(n) * k, -1);
+        return;
+    }
+    ensure_device();
+    const int64_t q0 = indptr[0], qnnz = indptr[n] - q0;
+    std::vector<int32_t> qip(n + 1);
+    for (idx_t i = 0; i <= n; ++i) qip[i] = static_cast<int32_t>(indptr[i] - q0);
+    DeviceBuffer d_qip, d_qix, d_qv, d_out_d, d_out_l, d_scores, d_allow;
+    d_qip.upload(qip);
+    d_qix.upload(indices + q0, qnnz * 2);
+    d_qv.upload(values + q0, qnnz * 4);
+    if (allow != nullptr) d_allow.upload(*allow);
+    d_out_d.allocate(static_cast<size_t>(n) * k * 4);
+    d_out_l.allocate(static_cast<size_t>(n) * k * 4);
+    d_scores.allocate(brute_search_scratch_bytes(list_docs_, n));
+    launch_inverted_search(dev_, list_docs_, d_qip.get<int32_t>(), d_qix.get<uint16_t>(), d_qv.get<float>(), n, k,
+                           allow != nullptr ? d_allow.get<uint32_t>() : nullptr, d_out_d.get<float>(),
+                           d_out_l.get<int32_t>(), d_scores.get<float>(), nullptr);
+    hip_check(hipDeviceSynchronize(), "inverted search");
+    d_out_d.download(distances, static_cast<size_t>(n) * k * 4);
+    d_out_l.download(labels, static_cast<size_t>(n) * k * 4);
+}
+
+// InvertedIndex::write_index (inverted_index.cpp:367-402): document count,
+// list count, element width, then each list's doc ids and value bytes, then
+// the per-term maxima; plain writes, no alignment padding.
+void InvertedIndex::write_payload(ByteWriter& w) const {
+    w.scalar<size_t>(static_cast<size_t>(n_vectors_));
+    const size_t n_terms = built_ ? static_cast<size_t>(n_lists_) : 0;
+    w.scalar<size_t>(n_terms);
+    w.scalar<size_t>(4);
+    if (built_) ensure_host();
+    for (size_t t = 0; t < n_terms; ++t) {
+        const int64_t a = h_list_ptr_[t], b = h_list_ptr_[t + 1];
+        w.scalar<size_t>(static_cast<size_t>(b - a));
+        if (b > a) {
+            w.write(h_docs_.data() + a, static_cast<size_t>(b - a) * 4);
+            w.write(h_vals_.data() + a, static_cast<size_t>(b - a) * 4);
+        }
+    }
+    const size_t ns = built_ ? h_max_.size() : 0;
+    w.scalar<size_t>(ns);
+    if (ns > 0) w.write(h_max_.data(), ns * 4);
+}
+
+// InvertedIndex::read_index (inverted_index.cpp:404-447). Each list must hold
+// non-negative doc ids in ascending order, as every writer produces them: the
+// device search relies on it, so a file breaking it is rejected as corrupt.
+void InvertedIndex::read_payload(ByteReader& r) {
+    n_vectors_ = static_cast<int64_t>(r.scalar<size_t>());
+    const size_t n_terms = r.scalar<size_t>();
+    const size_t esz = r.scalar<size_t>();
+    if (esz != 4) throw std::runtime_error("inverted index file declares an unsupported element width");
+    const size_t L = std::max<size_t>(n_terms, dim_ > 0 ? static_cast<size_t>(dim_) : 0);
+    h_list_ptr_.assign(L + 1, 0);
+    h_docs_.clear();
+    h_vals_.clear();
+    int64_t max_doc = -1;
+    for (size_t t = 0; t < n_terms; ++t) {
+        const size_t len = r.scalar<size_t>();
+        if (len > r.remaining() / 8) throw std::runtime_error("index file is truncated");
+        if (h_docs_.size() + len > static_cast<size_t>(std::numeric_limits<idx_t>::max()))
+            throw std::runtime_error("inverted index holds more postings than 32-bit offsets address");
+        const size_t at = h_docs_.size();
+        h_docs_.resize(at + len);
+        h_vals_.resize(at + len);
+        r.read(h_docs_.data() + at, len * 4);
+        r.read(h_vals_.data() + at, len * 4);
+        for (size_t i = at; i < at + len; ++i) {
+            if (h_docs_[i] < 0 || (i > at && h_docs_[i] < h_docs_[i - 1]))
+                throw std::runtime_error("corrupt index file: a posting list is not in ascending doc order");
+        }
+        if (len > 0) max_doc = std::max<int64_t>(max_doc, h_docs_[at + len - 1]);
+        h_list_ptr_[t + 1] = static_cast<int64_t>(h_docs_.size());
+    }
+    for (size_t t = n_terms; t < L; ++t) h_list_ptr_[t + 1] = h_list_ptr_[t];
+    const size_t ns = r.scalar<size_t>();
+    if (ns > r.remaining() / 4) throw std::runtime_error("index file is truncated");
+    std::vector<float> maxima(ns);
+    r.read(maxima.data(), ns * 4);
+    h_max_ = std::move(maxima);
+    n_lists_ = static_cast<int64_t>(n_terms);
+    built_ = n_terms > 0;
+    list_docs_ = max_doc + 1;
+    host_valid_ = true;
+    dev_valid_ = false;
+}
+
 // ============================================================== IDMapIndex
 
 IDMapIndex::IDMapIndex(std::unique_ptr<Index> delegate) : Index(delegate->dimension()), delegate_(std::move(delegate)) {}
@@ -905,6 +1111,28 @@ void IDMapIndex::search(idx_t n, const idx_t* indptr, const term_t* indices, con
     if (dynamic_cast<BrutalIndex*>(delegate_.get()) != nullptr) {
         // the delegate ignores selectors; only the labels are mapped
         delegate_->search(n, indptr, indices, values, k, distances, labels, params);
+        for (int64_t i = 0; i < static_cast<int64_t>(n) * k; ++i)
+            if (labels[i] >= 0) labels[i] = i2e_.at(labels[i]);
+        return;
+    }
+    if (auto* inv = dynamic_cast<InvertedIndex*>(delegate_.get())) {
+        // the selector tests external ids: IDSelectorEnumerableWithIDMap /
+        // is_member through the map (id_map_index.h:28-76)
+        throw_if_not_positive(n);
+        throw_if_not_positive(k);
+        throw_if_any_null(indptr, indices, values, labels, distances);
+        SelectorEval sel(request_from(params).selector);
+        if (!inv->built()) return;
+        std::vector<uint32_t> allow;
+        if (sel.active()) {
+            const int64_t nd = inv->list_docs();
+            allow.assign(static_cast<size_t>((nd + 31) / 32), 0);
+            for (int64_t d = 0; d < nd; ++d) {
+                const int64_t ext = d < static_cast<int64_t>(i2e_.size()) ? i2e_[d] : d;
+                if (sel.is_member(ext)) allow[d >> 5] |= 1U << (d & 31);
+            }
+        }
+        inv->search_allowed(n, indptr, indices, values, k, distances, labels, sel.active() ? &allow : nullptr);
         for (int64_t i = 0; i < static_cast<int64_t>(n) * k; ++i)
             if (labels[i] >= 0) labels[i] = i2e_.at(labels[i]);
         return;
@@ -1015,8 +1243,7 @@ std::unique_ptr<Index> index_factory(int dim, const char* description) {
         return def;
     };
     if (type == "brutal") return std::make_unique<BrutalIndex>(dim);
-    if (type == "inverted")
-        throw std::runtime_error("index type '" + type + "' is not part of the SEISMIC device path");
+    if (type == "inverted") return std::make_unique<InvertedIndex>(dim);
     if (type == "seismic" || type == "seismic_sq") {
         SeismicIndex::Params p;
         bool sq = type == "seismic_sq";
@@ -1073,7 +1300,7 @@ std::unique_ptr<Index> read_index(ByteReader& r) {
     else if (id == fourcc_of({'B', 'R', 'U', 'T'}))
         ix = std::make_unique<BrutalIndex>(dim);  // whose read_payload throws, as the reference's read does
     else if (id == fourcc_of({'I', 'N', 'V', 'T'}))
-        throw std::runtime_error("index type is not part of the SEISMIC device path");
+        ix = std::make_unique<InvertedIndex>(dim);
     else
         throw std::runtime_error("Unknown index type");
     ix->read_payload(r);

@@ -12,6 +12,7 @@
 
 #include "device_index.h"
 #include "device_mem.h"
+#include "kernels.h"
 #include "nsparse_c.h"
 
 namespace nsparse_amd {
@@ -169,6 +170,50 @@ private:
     HostCsr fwd_;
     bool dev_valid_ = false;
     DeviceBuffer d_indptr_, d_indices_, d_values_;
+};
+
+// InvertedIndex (inverted_index.h:25-61, inverted_index.cpp): FP32 posting
+// lists built on the device from the vectors added since the last build (which
+// build then releases), searched exactly: the k best positive scores, ranked
+// score desc then doc id asc -- what the reference's MaxScore traversal
+// returns for non-negative weights. The lists' doc ids number that build's
+// documents from 0, as in the reference; num_vectors counts every add.
+class InvertedIndex : public Index {
+public:
+    explicit InvertedIndex(int dim);
+    std::array<char, 4> id() const override { return {'I', 'N', 'V', 'T'}; }
+    int64_t num_vectors() const override { return n_vectors_; }
+    void add(idx_t n, const idx_t* indptr, const term_t* indices, const float* values) override;
+    void build() override;
+    void search(idx_t n, const idx_t* indptr, const term_t* indices, const float* values, int k, float* distances,
+                idx_t* labels, const nsparse_search_params* params) override;
+    void write_payload(ByteWriter& w) const override;
+    void read_payload(ByteReader& r) override;
+    // The search with the selector already resolved to a bitmap over the
+    // lists' doc ids (nullptr: every doc); IDMapIndex passes its mapped one.
+    void search_allowed(idx_t n, const idx_t* indptr, const term_t* indices, const float* values, int k,
+                        float* distances, idx_t* labels, const std::vector<uint32_t>* allow);
+    bool built() const { return built_; }
+    // Doc ids the lists can hold: [0, list_docs()).
+    int64_t list_docs() const { return list_docs_; }
+
+private:
+    void ensure_device();
+    void ensure_host() const;
+
+    HostCsr pending_;  // vectors_ of the reference
+    int64_t n_vectors_ = 0;
+    bool built_ = false;
+    int64_t list_docs_ = 0;
+    int64_t n_lists_ = 0;  // lists written: dim after a build, the file's count after a read
+    // Lists: on the device after a build, on the host after a read; each side
+    // is made from the other on demand.
+    mutable bool host_valid_ = false;
+    bool dev_valid_ = false;
+    mutable std::vector<int64_t> h_list_ptr_;
+    mutable std::vector<idx_t> h_docs_;
+    mutable std::vector<float> h_vals_, h_max_;  // h_max_: max_term_scores_
+    DevicePostings dev_;  // max_score is filled by a build only
 };
 
 // A batched search over fixed device buffers, replayed from a hipGraph: the

@@ -80,6 +80,29 @@ void launch_brute_search(const int32_t* indptr, const uint16_t* indices, const f
 void launch_exact_scores(const int32_t* indptr, const uint16_t* indices, const float* values, int64_t n_docs, int dim,
                          const uint16_t* q_indices, const float* q_values, int qn, float* scores, hipStream_t stream);
 
+// InvertedIndex posting lists on the device: list_ptr (int64, dim + 1), docs
+// (int32) and vals (float) term-major with doc ids ascending inside a term,
+// and each term's largest posting value from 0 up (float, dim).
+struct DevicePostings {
+    DeviceBuffer list_ptr, docs, vals, max_score;
+    int64_t n_postings = 0;
+    bool duplicate_docs = false;  // some row repeats a term, so a list holds that doc twice
+};
+
+// ArrayInvertedLists::build_inverted_lists + InvertedIndex::build's per-term
+// maxima (inverted_index.cpp:236-252) over a forward CSR already in HBM.
+void build_postings_on_device(const int32_t* indptr, const uint16_t* indices, const float* values, int64_t n_docs,
+                              int64_t nnz, int dim, DevicePostings& out);
+
+// Exact InvertedIndex search: every posting of every query term is scored
+// term by term into LDS windows of doc ids, then the k best positive scores
+// (score desc, doc id asc) are kept, among the docs `allow` marks when it is
+// not null (bit d of word d / 32). Scratch as for launch_brute_search.
+void launch_inverted_search(const DevicePostings& p, int64_t n_docs, const int32_t* d_q_indptr,
+                            const uint16_t* d_q_indices, const float* d_q_values, int64_t n_queries, int k,
+                            const uint32_t* d_allow, float* d_out_d, int32_t* d_out_l, float* d_scores,
+                            hipStream_t stream);
+
 // Differentiable exact re-scoring (rescore.hip). Forward fills scores
 // (n_q x n_cand); backward adds into grad_q / grad_d (either may be null).
 void launch_rescore(bool backward, int64_t n_q, const int32_t* q_indptr, const uint16_t* q_indices,

@@ -1600,9 +1600,11 @@ __device__ void bitonic_desc(uint64_t* keys) {
 // against the current k-th key; survivors collect in LDS behind the sorted
 // top-kcap and are merged in by a bitonic sort whenever the next round could
 // overflow them. Writes k (score, doc) pairs best first, (-1.0f, -1) past the
-// end of the corpus.
+// last candidate. With `positive`, only scores > 0 are candidates; with
+// `allow`, only docs whose bit is set.
 __global__ __launch_bounds__(kBlock) void topk_select_kernel(const float* __restrict__ scores, int64_t ld,
-                                                             int64_t n_docs, int k, int kcap,
+                                                             int64_t n_docs, int k, int kcap, bool positive,
+                                                             const uint32_t* __restrict__ allow,
                                                              float* __restrict__ out_d, int32_t* __restrict__ out_l) {
     __shared__ uint64_t keys[kSelectSlots];
     __shared__ int cnt;
@@ -1626,8 +1628,11 @@ __global__ __launch_bounds__(kBlock) void topk_select_kernel(const float* __rest
         for (int u = 0; u < kSelectRound / kBlock; ++u) {
             const int64_t d = base + u * kBlock + threadIdx.x;
             if (d < n_docs) {
-                const uint64_t key = score_key(row[d], d);
-                if (key > thresh) keys[kcap + atomicAdd(&cnt, 1)] = key;
+                const float sc = row[d];
+                bool keep = !positive || sc > 0.0F;
+                if (allow != nullptr) keep = keep && ((allow[d >> 5] >> (d & 31)) & 1U) != 0;
+                const uint64_t key = score_key(sc, d);
+                if (keep && key > thresh) keys[kcap + atomicAdd(&cnt, 1)] = key;
             }
         }
         __syncthreads();
@@ -1683,8 +1688,139 @@ void launch_brute_search(const int32_t* indptr, const uint16_t* indices, const f
                            dim3(kBlock), exact_scores_lds(dim), stream, indptr, indices, values, n_docs, nw,
                            d_q_indptr, d_q_indices, d_q_values, 0, q0, d_scores, n_docs);
         hipLaunchKernelGGL(topk_select_kernel, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, stream, d_scores,
-                           n_docs, n_docs, k, kcap, d_out_d + q0 * k, d_out_l + q0 * k);
+                           n_docs, n_docs, k, kcap, false, nullptr, d_out_d + q0 * k, d_out_l + q0 * k);
         hip_check(hipGetLastError(), "brute-force search launch");
+    }
+}
+
+namespace {
+
+// ---- InvertedIndex: exact term-at-a-time scores in LDS windows of doc ids
+constexpr int kInvBlock = 512;
+constexpr int kInvWaves = kInvBlock / 64;
+constexpr int kInvWindow = 8192;  // doc ids per window: 32 KB of float scores
+constexpr int kInvSlabTarget = 2048;  // workgroups a group of queries aims for
+
+// One workgroup per (slab of consecutive windows, query). Per query term, LDS
+// keeps the cursor into its posting list (found once per slab by binary
+// search), the list end and the doc id under the cursor. Each window is
+// scored term by term in query order: the whole workgroup takes the term's
+// next kInvBlock postings at a time, lane i adds weight * value of posting
+// cursor + i into its doc's slot while that doc is inside the window, and a
+// barrier separates the rounds. Docs ascend within a list, so the lanes inside
+// the window are a prefix and the first lane past it holds the new cursor.
+// A doc's score is therefore the float sum of its products in query-term
+// order, from 0 -- the same bits however the work is split. The scores of the
+// window are then written to row blockIdx.y of `scores` (row stride ld).
+// kAtomic: some list holds a doc twice, so one round may add to a slot twice.
+template <bool kAtomic>
+__global__ __launch_bounds__(kInvBlock) void inverted_scores_kernel(
+    const int64_t* __restrict__ list_ptr, const int32_t* __restrict__ docs, const float* __restrict__ vals,
+    int64_t n_docs, int windows_per_slab, const int32_t* __restrict__ q_indptr,
+    const uint16_t* __restrict__ q_indices, const float* __restrict__ q_values, int64_t q0,
+    float* __restrict__ scores, int64_t ld) {
+    __shared__ float win[kInvWindow];
+    __shared__ int64_t cur[kQMax], stop[kQMax];
+    __shared__ int32_t head[kQMax];
+    __shared__ float wt[kQMax];
+    __shared__ int32_t inside[2][kInvWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t q = q0 + blockIdx.y;
+    const int32_t qs = q_indptr[q];
+    const int qn = q_indptr[q + 1] - qs;
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * windows_per_slab * kInvWindow;
+    const int64_t last = min(n_docs, first + static_cast<int64_t>(windows_per_slab) * kInvWindow);
+    float* row = scores + static_cast<int64_t>(blockIdx.y) * ld;
+    for (int t = tid; t < qn; t += kInvBlock) {
+        const int term = q_indices[qs + t];
+        int64_t a = list_ptr[term], b = list_ptr[term + 1];
+        const int64_t e = b;
+        while (a < b) {  // lower_bound(docs[a, b), first)
+            const int64_t m = (a + b) >> 1;
+            if (docs[m] < first)
+                a = m + 1;
+            else
+                b = m;
+        }
+        cur[t] = a;
+        stop[t] = e;
+        head[t] = a < e ? docs[a] : INT32_MAX;
+        wt[t] = q_values[qs + t];
+    }
+    for (int i = tid; i < kInvWindow; i += kInvBlock) win[i] = 0.0F;
+    __syncthreads();
+    int round = 0;
+    for (int64_t base = first; base < last; base += kInvWindow) {
+        const int64_t wend = min(base + kInvWindow, last);
+        for (int t = 0; t < qn; ++t) {
+            if (head[t] >= wend) continue;  // nothing of this term in the window
+            const float w = wt[t];
+            const int64_t e = stop[t];
+            for (int64_t p0 = cur[t];; p0 += kInvBlock) {
+                const int64_t p = p0 + tid;
+                const int32_t d = p < e ? docs[p] : INT32_MAX;
+                const bool in = d < wend;
+                if (in) {
+                    const float add = w * vals[p];
+                    float* slot = &win[d - base];
+                    if (kAtomic)
+                        atomicAdd(slot, add);
+                    else
+                        *slot += add;
+                }
+                const uint64_t m = __ballot(in);
+                if (lane == 0) inside[round & 1][wave] = __popcll(m);
+                __syncthreads();
+                int c = 0;
+#pragma unroll
+                for (int v = 0; v < kInvWaves; ++v) c += inside[round & 1][v];
+                ++round;
+                if (c < kInvBlock) {
+                    if (tid == c) {  // first posting past the window
+                        cur[t] = p;
+                        head[t] = d;
+                    }
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < wend - base; i += kInvBlock) {
+            row[base + i] = win[i];
+            win[i] = 0.0F;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+void launch_inverted_search(const DevicePostings& ps, int64_t n_docs, const int32_t* d_q_indptr,
+                            const uint16_t* d_q_indices, const float* d_q_values, int64_t n_queries, int k,
+                            const uint32_t* d_allow, float* d_out_d, int32_t* d_out_l, float* d_scores,
+                            hipStream_t stream) {
+    if (k <= 0 || k > kBruteKMax) throw std::invalid_argument("inverted search supports 1 <= k <= 1024 on the device");
+    if (n_docs <= 0 || n_queries <= 0) throw std::invalid_argument("inverted search needs documents and queries");
+    int kcap = 1;
+    while (kcap < k) kcap <<= 1;
+    const int64_t group = brute_search_group(n_docs, n_queries);
+    const int64_t windows = (n_docs + kInvWindow - 1) / kInvWindow;
+    for (int64_t q0 = 0; q0 < n_queries; q0 += group) {
+        const int64_t g = std::min<int64_t>(group, n_queries - q0);
+        const int64_t wps = std::clamp<int64_t>(windows * g / kInvSlabTarget, 1, windows);
+        const int64_t slabs = (windows + wps - 1) / wps;  // every slab starts below n_docs
+        const dim3 grid(static_cast<unsigned>(slabs), static_cast<unsigned>(g));
+        if (ps.duplicate_docs)
+            hipLaunchKernelGGL(inverted_scores_kernel<true>, grid, dim3(kInvBlock), 0, stream,
+                               ps.list_ptr.get<int64_t>(), ps.docs.get<int32_t>(), ps.vals.get<float>(), n_docs,
+                               static_cast<int>(wps), d_q_indptr, d_q_indices, d_q_values, q0, d_scores, n_docs);
+        else
+            hipLaunchKernelGGL(inverted_scores_kernel<false>, grid, dim3(kInvBlock), 0, stream,
+                               ps.list_ptr.get<int64_t>(), ps.docs.get<int32_t>(), ps.vals.get<float>(), n_docs,
+                               static_cast<int>(wps), d_q_indptr, d_q_indices, d_q_values, q0, d_scores, n_docs);
+        hipLaunchKernelGGL(topk_select_kernel, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, stream, d_scores,
+                           n_docs, n_docs, k, kcap, true, d_allow, d_out_d + q0 * k, d_out_l + q0 * k);
+        hip_check(hipGetLastError(), "inverted search launch");
     }
 }
 

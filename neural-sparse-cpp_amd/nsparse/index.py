@@ -181,8 +181,9 @@ class Index:
         """Returns (distances float32 n x k, labels int32 n x k), best first."""
         ip, ix, v = self._csr(indptr, indices, values)
         n, k = int(n), int(k)
-        d = np.empty((max(n, 0), max(k, 0)), dtype=np.float32)
-        lab = np.empty((max(n, 0), max(k, 0)), dtype=np.int32)
+        # an index with nothing to search writes nothing: the padding stays
+        d = np.full((max(n, 0), max(k, 0)), -1.0, dtype=np.float32)
+        lab = np.full((max(n, 0), max(k, 0)), -1, dtype=np.int32)
         ps = ctypes.byref(params._struct()) if params is not None else None
         check(lib.nsparse_search(self._h, n, _ptr(ip), _ptr(ix), _ptr(v), k, _ptr(d), _ptr(lab), ps))
         return d, lab

@@ -886,6 +886,193 @@ Selector make_selector(int kind, int64_t n, const idx_t* ids, const uint32_t* bi
     return s;
 }
 
+// ---------------------------------------------------------------- InvertedIndex
+// inverted_index.cpp: FP32 posting lists in doc order and a MaxScore traversal
+// over windows of 4096 doc ids. Each step follows the reference, so scores are
+// summed in its order (essential terms into the window in ascending bound
+// order, then the non-essential terms from the highest bound down) and the
+// pruning tests see the same float values.
+struct InvIndex {
+    int dim = 0;
+    int tie_mode = 1;
+    int64_t n_vectors = 0;  // num_vectors_ (inverted_index.cpp:233)
+    Fwd pending;            // vectors_, released by build (:239)
+    bool built = false;     // inverted_lists_ != nullptr
+    std::vector<std::vector<idx_t>> docs;
+    std::vector<std::vector<float>> vals;
+    std::vector<float> max_score;  // max_term_scores_
+};
+
+// InvertedIndex::add (inverted_index.cpp:218-234).
+void inv_add(InvIndex& ix, int n, const idx_t* indptr, const term_t* indices, const float* values) {
+    if (n <= 0) throw std::invalid_argument("value must be positive");
+    if (indptr == nullptr || indices == nullptr || values == nullptr) throw std::invalid_argument("unexpected nullptr");
+    if (ix.pending.indptr.empty()) {
+        ix.pending.esz = 4;
+        ix.pending.dim = static_cast<size_t>(ix.dim);
+    }
+    ix.pending.add(indptr, static_cast<size_t>(n) + 1, indices, static_cast<size_t>(indptr[n]),
+                   reinterpret_cast<const uint8_t*>(values));
+    ix.n_vectors += n;
+}
+
+// ArrayInvertedLists::build_inverted_lists (invlists/inverted_lists.cpp:159-180)
+// and InvertedIndex::build (inverted_index.cpp:236-252): postings appended doc
+// by doc in row order, then the per-term maximum starting from 0
+// (InvertedList::max_value, inverted_lists.cpp:88-96).
+void inv_build(InvIndex& ix) {
+    // The reference dereferences its released vectors here; the oracle refuses.
+    if (ix.pending.indptr.empty()) throw std::runtime_error("no vectors added since the last build");
+    const Fwd& f = ix.pending;
+    ix.docs.assign(ix.dim, {});
+    ix.vals.assign(ix.dim, {});
+    for (size_t d = 0; d < f.n(); ++d)
+        for (idx_t j = f.indptr[d]; j < f.indptr[d + 1]; ++j) {
+            const term_t t = f.indices[j];
+            if (t >= ix.dim) throw std::invalid_argument("term_id out of range");
+            float v;
+            std::memcpy(&v, f.values.data() + static_cast<size_t>(j) * 4, 4);
+            ix.docs[t].push_back(static_cast<idx_t>(d));
+            ix.vals[t].push_back(v);
+        }
+    ix.max_score.assign(ix.dim, 0.0F);
+    for (int t = 0; t < ix.dim; ++t)
+        for (float v : ix.vals[t])
+            if (v > ix.max_score[t]) ix.max_score[t] = v;
+    ix.pending = Fwd{};
+    ix.built = true;
+}
+
+constexpr idx_t kNoDoc = std::numeric_limits<idx_t>::max();
+constexpr int kInvWindow = 1 << 12;  // kScoreWindowSize (inverted_index.cpp:31)
+
+// DirectTermScorer (inverted_index.cpp:38-86): a cursor over one posting list.
+struct TermCursor {
+    const idx_t* docs = nullptr;
+    const float* vals = nullptr;
+    int size = 0;
+    int pos = 0;
+    float weight = 0.0F;
+    float bound = 0.0F;  // weight * max posting value
+    idx_t doc() const { return pos < 0 || pos >= size ? kNoDoc : docs[pos]; }
+    // advance_to: first posting >= target at or after the cursor.
+    void seek(idx_t target) {
+        if (pos >= size) return;
+        if (pos >= 0 && docs[pos] >= target) return;
+        const int from = pos < 0 ? 0 : pos + 1;
+        pos = static_cast<int>(std::lower_bound(docs + from, docs + size, target) - docs);
+    }
+};
+
+// InvertedIndex::single_query (inverted_index.cpp:295-365) with build_scorers
+// (:88-122), score_essential_terms (:124-150) and evaluate_window_candidates
+// (:152-212).
+template <class TopK>
+void inv_query(const InvIndex& ix, const term_t* qi, const float* qv, int size, int k, const Selector* sel, float* D,
+               idx_t* L) {
+    for (int i = 0; i < k; ++i) {
+        D[i] = -1.0F;
+        L[i] = kInvalid;
+    }
+    if (size == 0) return;
+    TopK heap(k);
+    std::vector<TermCursor> cur(size);
+    for (int i = 0; i < size; ++i) {
+        const term_t t = qi[i];
+        cur[i].docs = ix.docs[t].data();
+        cur[i].vals = ix.vals[t].data();
+        cur[i].size = static_cast<int>(ix.docs[t].size());
+        cur[i].weight = qv[i];
+        cur[i].bound = qv[i] * ix.max_score[t];
+    }
+    std::sort(cur.begin(), cur.end(), [](const TermCursor& a, const TermCursor& b) { return a.bound < b.bound; });
+    std::vector<float> prefix(size + 1, 0.0F);
+    for (int i = 0; i < size; ++i) prefix[i + 1] = prefix[i] + cur[i].bound;
+    std::vector<float> acc(kInvWindow, 0.0F);
+    std::array<uint64_t, kInvWindow / 64> touched{};
+    int first = 0;  // first essential cursor
+    for (;;) {
+        if (heap.full()) {
+            const float thr = heap.peek();
+            first = 0;
+            for (int i = 0; i < size; ++i) {
+                if (prefix[i + 1] < thr)
+                    first = i + 1;
+                else
+                    break;
+            }
+        }
+        idx_t low = kNoDoc;
+        for (int i = first; i < size; ++i) low = std::min(low, cur[i].doc());
+        if (low == kNoDoc) break;
+        const idx_t base = low / kInvWindow * kInvWindow, end = base + kInvWindow;
+        touched.fill(0);
+        for (int i = first; i < size; ++i) {
+            TermCursor& c = cur[i];
+            while (c.pos < c.size) {
+                const idx_t d = c.docs[c.pos];
+                if (d >= end) break;
+                if (d >= base) {
+                    const int slot = d - base;
+                    touched[slot >> 6] |= uint64_t{1} << (slot & 63);
+                    acc[slot] += c.weight * c.vals[c.pos];
+                }
+                ++c.pos;
+            }
+        }
+        float thr = heap.full() ? heap.peek() : 0.0F;
+        const float rest = prefix[first];
+        for (int w = 0; w < kInvWindow / 64; ++w) {
+            for (uint64_t bits = touched[w]; bits != 0; bits &= bits - 1) {
+                const int slot = w * 64 + std::countr_zero(bits);
+                const float part = acc[slot];
+                acc[slot] = 0.0F;
+                const idx_t d = base + slot;
+                if (part + rest <= thr) continue;
+                if (sel != nullptr && !sel->is_member(d)) continue;
+                float total = part;
+                for (int j = first - 1; j >= 0; --j) {
+                    TermCursor& c = cur[j];
+                    c.seek(d);
+                    if (c.doc() == d) total += c.weight * c.vals[c.pos];
+                    if (total + prefix[j] <= thr) break;
+                }
+                heap.add(total, d);
+                if (heap.full()) thr = heap.peek();
+            }
+        }
+    }
+    std::vector<float> s;
+    std::vector<idx_t> ids;
+    heap.items_descending(s, ids);
+    for (size_t i = 0; i < s.size() && i < static_cast<size_t>(k); ++i) {
+        D[i] = s[i];
+        L[i] = ids[i];
+    }
+}
+
+// InvertedIndex::write_index (inverted_index.cpp:367-402) after the header
+// (io/index_io.cpp:88-95): plain field writes, no alignment padding.
+void inv_write(const InvIndex& ix, Writer& w) {
+    w.scalar<uint32_t>(static_cast<uint32_t>('I') | static_cast<uint32_t>('N') << 8 |
+                       static_cast<uint32_t>('V') << 16 | static_cast<uint32_t>('T') << 24);
+    w.scalar<int>(ix.dim);
+    w.scalar<size_t>(static_cast<size_t>(ix.n_vectors));
+    const size_t n_terms = ix.built ? ix.docs.size() : 0;
+    w.scalar<size_t>(n_terms);
+    w.scalar<size_t>(4);
+    for (size_t t = 0; t < n_terms; ++t) {
+        w.scalar<size_t>(ix.docs[t].size());
+        if (!ix.docs[t].empty()) {
+            w.raw(ix.docs[t].data(), ix.docs[t].size() * 4);
+            w.raw(ix.vals[t].data(), ix.vals[t].size() * 4);
+        }
+    }
+    const size_t ns = ix.built ? ix.max_score.size() : 0;
+    w.scalar<size_t>(ns);
+    if (ns > 0) w.raw(ix.max_score.data(), ns * 4);
+}
+
 }  // namespace oracle
 
 using namespace oracle;
@@ -1121,6 +1308,69 @@ int oracle_lambda_beta(int lambda, int beta, int64_t n_vectors, int* out_lambda,
     return guard([&] {
         *out_lambda = calculate_lambda(lambda, static_cast<size_t>(n_vectors));
         *out_beta = calculate_beta(beta, *out_lambda);
+    });
+}
+
+// ---- InvertedIndex ----
+
+int oracle_inv_create(int dim, int tie_mode, void** out) {
+    return guard([&] {
+        auto* ix = new InvIndex;
+        ix->dim = dim;
+        ix->tie_mode = tie_mode;
+        *out = ix;
+    });
+}
+
+void oracle_inv_free(void* h) { delete static_cast<InvIndex*>(h); }
+
+int oracle_inv_add(void* h, int n, const idx_t* indptr, const term_t* indices, const float* values) {
+    return guard([&] { inv_add(*static_cast<InvIndex*>(h), n, indptr, indices, values); });
+}
+
+int oracle_inv_build(void* h) {
+    return guard([&] { inv_build(*static_cast<InvIndex*>(h)); });
+}
+
+int64_t oracle_inv_num_vectors(void* h) { return static_cast<InvIndex*>(h)->n_vectors; }
+
+// InvertedIndex::search (inverted_index.cpp:254-293): queries in parallel;
+// an index that was never built writes nothing. Selector arguments as for
+// oracle_search.
+int oracle_inv_search(void* h, int n, const idx_t* indptr, const term_t* indices, const float* values, int k,
+                      int sel_kind, int64_t sel_n, const idx_t* sel_ids, const uint32_t* sel_bitmap, int sel_negate,
+                      float* D, idx_t* L, int n_threads) {
+    return guard([&] {
+        const auto* ix = static_cast<const InvIndex*>(h);
+        if (n <= 0 || k <= 0) throw std::invalid_argument("value must be positive");
+        if (!ix->built) return;
+        Selector inner;
+        Selector sel = make_selector(sel_kind, sel_n, sel_ids, sel_bitmap, sel_negate, inner);
+        const Selector* sp = sel.kind < 0 ? nullptr : &sel;
+        for (int64_t j = indptr[0]; j < indptr[n]; ++j)
+            if (indices[j] >= ix->dim) throw std::invalid_argument("query term id out of range");
+        if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int q = 0; q < n; ++q) {
+            const idx_t s0 = indptr[q];
+            const int len = indptr[q + 1] - s0;
+            if (ix->tie_mode == 0)
+                inv_query<HeapTopK<idx_t>>(*ix, indices + s0, values + s0, len, k, sp, D + int64_t{q} * k,
+                                           L + int64_t{q} * k);
+            else
+                inv_query<IdTopK<idx_t>>(*ix, indices + s0, values + s0, len, k, sp, D + int64_t{q} * k,
+                                         L + int64_t{q} * k);
+        }
+    });
+}
+
+int oracle_inv_serialize(void* h, uint8_t** out, size_t* size) {
+    return guard([&] {
+        Writer w;
+        inv_write(*static_cast<InvIndex*>(h), w);
+        *size = w.buf.size();
+        *out = static_cast<uint8_t*>(std::malloc(w.buf.size()));
+        std::memcpy(*out, w.buf.data(), w.buf.size());
     });
 }
 

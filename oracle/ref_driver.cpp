@@ -100,6 +100,51 @@ int ref_build_serialize(int dim, int sq, float vmin, float vmax, int n, const in
 
 void ref_buffer_free(uint8_t* p) { std::free(p); }
 
+// The posting lists and per-term maxima of InvertedIndex::build
+// (inverted_index.cpp:236-252), made by the reference's own
+// ArrayInvertedLists::build_inverted_lists and InvertedList::max_value
+// (invlists/inverted_lists.cpp:88-96, 159-180), written in the field order of
+// InvertedIndex::write_index (inverted_index.cpp:367-402) after the header.
+// inverted_index.cpp itself pulls in Abseil, so only that loop is restated here.
+int ref_inverted_serialize(int dim, int n, const int32_t* indptr, const uint16_t* indices, const float* values,
+                           uint8_t** out, size_t* out_size) {
+    return guard([&] {
+        const size_t nnz = indptr[n];
+        SparseVectors vectors({.element_size = 4, .dimension = static_cast<size_t>(dim)});
+        vectors.add_vectors(indptr, static_cast<size_t>(n) + 1, indices, nnz,
+                            reinterpret_cast<const uint8_t*>(values), nnz * 4);
+        auto lists = ArrayInvertedLists::build_inverted_lists(static_cast<size_t>(dim), 4, &vectors);
+        std::vector<uint8_t> w;
+        auto put = [&](const void* p, size_t bytes) {
+            const auto* b = static_cast<const uint8_t*>(p);
+            w.insert(w.end(), b, b + bytes);
+        };
+        const uint32_t fcc = 'I' | 'N' << 8 | 'V' << 16 | 'T' << 24;
+        put(&fcc, 4);
+        put(&dim, 4);
+        const size_t nv = static_cast<size_t>(n), n_terms = lists->size(), esz = 4;
+        put(&nv, 8);
+        put(&n_terms, 8);
+        put(&esz, 8);
+        std::vector<float> maxima(n_terms);
+        for (size_t t = 0; t < n_terms; ++t) {
+            const auto& l = (*lists)[t];
+            const size_t len = l.get_doc_ids().size();
+            put(&len, 8);
+            if (len > 0) {
+                put(l.get_doc_ids().data(), len * 4);
+                put(l.get_codes().data(), l.get_codes().size());
+            }
+            maxima[t] = l.max_value();
+        }
+        put(&n_terms, 8);
+        put(maxima.data(), n_terms * 4);
+        *out_size = w.size();
+        *out = static_cast<uint8_t*>(std::malloc(w.size()));
+        std::memcpy(*out, w.data(), w.size());
+    });
+}
+
 // InvertedList::prune_and_keep_doc_ids on a hand-made list (inverted_lists.cpp:110-132).
 int ref_prune(int n, const int32_t* docs, const uint8_t* codes, int esz, int64_t lambda, int32_t* out, int* out_n) {
     return guard([&] {

@@ -195,3 +195,153 @@ def recall_at_k(got, want):
             continue
         per.append(len({int(i) for i in g if i >= 0} & ws) / len(ws))
     return float(np.mean(per)) if per else 1.0
+
+
+class OracleInverted:
+    """The CPU restatement of InvertedIndex (MaxScore over 4096-doc windows).
+    tie_mode as for OracleIndex."""
+
+    def __init__(self, dim, tie_mode=1):
+        self.h = P()
+        _chk(lib.oracle_inv_create(dim, tie_mode, ctypes.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.oracle_inv_free(self.h)
+            self.h = None
+
+    def add(self, ip, ix, v):
+        ip = np.ascontiguousarray(ip, np.int32)
+        ix = np.ascontiguousarray(ix, np.uint16)
+        v = np.ascontiguousarray(v, np.float32)
+        _chk(lib.oracle_inv_add(self.h, len(ip) - 1, _p(ip), _p(ix), _p(v)))
+
+    def build(self):
+        _chk(lib.oracle_inv_build(self.h))
+
+    def num_vectors(self):
+        lib.oracle_inv_num_vectors.restype = ctypes.c_int64
+        return int(lib.oracle_inv_num_vectors(self.h))
+
+    def search(self, ip, ix, v, k, selector=None, negate=False, threads=8, fill=(-1.0, -1)):
+        """selector: None, ("array", ids), ("set", ids) or ("bitmap", words, nbits).
+        Outputs start as `fill`; an unbuilt index leaves them so."""
+        ip = np.ascontiguousarray(ip, np.int32)
+        ix = np.ascontiguousarray(ix, np.uint16)
+        v = np.ascontiguousarray(v, np.float32)
+        n = len(ip) - 1
+        d = np.full((n, k), fill[0], np.float32)
+        lab = np.full((n, k), fill[1], np.int32)
+        kind, sn, ids, bm = -1, 0, None, None
+        if selector is not None:
+            if selector[0] in ("array", "set"):
+                kind = 0 if selector[0] == "array" else 1
+                ids = np.ascontiguousarray(selector[1], np.int32)
+                sn = len(ids)
+            else:
+                kind = 3
+                bm = np.ascontiguousarray(selector[1], np.uint32)
+                sn = int(selector[2])
+        _chk(lib.oracle_inv_search(self.h, n, _p(ip), _p(ix), _p(v), k, kind, ctypes.c_int64(sn),
+                                   _p(ids) if ids is not None else None, _p(bm) if bm is not None else None,
+                                   int(negate), _p(d), _p(lab), threads))
+        return d, lab
+
+    def serialize(self):
+        buf = ctypes.POINTER(ctypes.c_uint8)()
+        sz = ctypes.c_size_t()
+        _chk(lib.oracle_inv_serialize(self.h, ctypes.byref(buf), ctypes.byref(sz)))
+        out = ctypes.string_at(buf, sz.value)
+        lib.oracle_buffer_free(buf)
+        return out
+
+
+class TermMajor:
+    """The corpus grouped by term (stable: doc ids ascend inside a term), for
+    exact FP32 scores summed over a query's entries in query order: for each
+    entry, every posting of its term adds weight * value (float32 product, then
+    a float32 add) -- the order of the device's term-at-a-time scorer. Rows
+    must hold each term at most once."""
+
+    def __init__(self, docs, dim):
+        ip, ix, v = docs
+        self.n = len(ip) - 1
+        order = np.argsort(ix, kind="stable")
+        self.rows = np.repeat(np.arange(self.n), np.diff(ip))[order]
+        self.vals = v[order]
+        self.ptr = np.zeros(dim + 1, np.int64)
+        np.cumsum(np.bincount(ix, minlength=dim), out=self.ptr[1:])
+
+    def scores(self, q_idx, q_val):
+        acc = np.zeros(self.n, np.float32)
+        for t, w in zip(q_idx, q_val):
+            a, b = self.ptr[t], self.ptr[t + 1]
+            r = self.rows[a:b]
+            acc[r] = acc[r] + np.float32(w) * self.vals[a:b]
+        return acc
+
+    def top_k(self, queries, k, allowed=None):
+        """Exact top-k of positive scores (score desc, doc id asc) in that order."""
+        qip, qix, qv = queries
+        nq = len(qip) - 1
+        labels = np.full((nq, k), -1, np.int32)
+        dists = np.full((nq, k), -1.0, np.float32)
+        for q in range(nq):
+            s = self.scores(qix[qip[q]:qip[q + 1]], qv[qip[q]:qip[q + 1]])
+            keep = s > 0
+            if allowed is not None:
+                keep &= allowed
+            (el,) = np.nonzero(keep)
+            order = el[np.lexsort((el, -s[el]))][:k]
+            labels[q, :len(order)] = order
+            dists[q, :len(order)] = s[order]
+        return labels, dists
+
+
+def ref_inverted_bytes(dim, ip, ix, v):
+    """INVT bytes of an index built by the reference's own posting-list code."""
+    ip = np.ascontiguousarray(ip, np.int32)
+    ix = np.ascontiguousarray(ix, np.uint16)
+    v = np.ascontiguousarray(v, np.float32)
+    buf = ctypes.POINTER(ctypes.c_uint8)()
+    sz = ctypes.c_size_t()
+    rc = ref.ref_inverted_serialize(dim, len(ip) - 1, _p(ip), _p(ix), _p(v), ctypes.byref(buf), ctypes.byref(sz))
+    if rc != 0:
+        raise RuntimeError(ref.ref_last_error().decode())
+    out = ctypes.string_at(buf, sz.value)
+    ref.ref_buffer_free(buf)
+    return out
+
+
+def exact_scores64(docs, queries, dim):
+    """Every query's score against every doc in float64 (scipy), n_q x n_docs."""
+    import scipy.sparse as sp
+
+    D = sp.csr_matrix((docs[2].astype(np.float64), docs[1].astype(np.int64), docs[0].astype(np.int64)),
+                      shape=(len(docs[0]) - 1, dim))
+    Q = sp.csr_matrix((queries[2].astype(np.float64), queries[1].astype(np.int64),
+                       queries[0].astype(np.int64)), shape=(len(queries[0]) - 1, dim))
+    return (Q @ D.T).toarray()
+
+
+def assert_topk_matches(labels, dists, scores64, k, allowed=None, rtol=1e-5):
+    """labels/dists against the exact top-k of positive scores (score desc, id
+    asc) from float64 scores: scores agree within rtol, and a label may differ
+    from the exact ranking only where its exact score ties the expected one
+    within rtol (a float32 sum can order such near-ties either way)."""
+    for q in range(scores64.shape[0]):
+        s = scores64[q].copy()
+        if allowed is not None:
+            s[~allowed] = 0.0
+        (el,) = np.nonzero(s > 0.0)
+        order = el[np.lexsort((el, -s[el]))][:k]
+        want_l = np.full(k, -1, np.int64)
+        want_l[: len(order)] = order
+        got_l = labels[q].astype(np.int64)
+        assert np.array_equal(got_l < 0, want_l < 0), (q, got_l, want_l)
+        m = want_l >= 0
+        np.testing.assert_allclose(dists[q][m], s[want_l[m]], rtol=rtol, atol=1e-6)
+        assert np.all(dists[q][~m] == -1.0)
+        for i in np.nonzero(got_l != want_l)[0]:
+            g, w = got_l[i], want_l[i]
+            assert g >= 0 and abs(s[g] - s[w]) <= rtol * abs(s[w]) + 1e-6, (q, i, g, w, s[g], s[w])

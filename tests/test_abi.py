@@ -50,8 +50,8 @@ def test_factory_strings_and_errors(nsparse):
         nsparse.index_factory(10, "seismic_sq,vmin=2|vmax=1")
     with pytest.raises(ValueError):
         nsparse.index_factory(10, "seismic,lambda=abc")
-    with pytest.raises(RuntimeError):
-        nsparse.index_factory(10, "inverted")
+    assert nsparse.index_factory(10, " inverted ").id() == "INVT"
+    assert nsparse.index_factory(10, "idmap,inverted").id() == "IDMP"
 
 
 def test_brutal_lifecycle(nsparse):
@@ -71,6 +71,54 @@ def test_brutal_lifecycle(nsparse):
     assert im.id() == "IDMP"
     with pytest.raises(ValueError):
         ix.add(1, np.array([0, 1], np.int32), np.array([50], np.uint16), np.array([1.0], np.float32))
+
+
+def test_inverted_lifecycle_and_layout(nsparse):
+    """inverted (inverted_index.cpp): num_vectors counts every add, empty rows
+    included; an unbuilt index writes nothing into the outputs and serializes
+    with no lists; a built INVT file (here the oracle's, whose lists are pinned
+    to the reference's own list-building code) reads back and re-writes byte for
+    byte; bad term ids are rejected by build(); a file with another element
+    width, or a list out of doc order, is rejected. No GPU needed."""
+    from _oracle import OracleInverted
+
+    ix = nsparse.index_factory(9, "inverted")
+    ip = np.array([0, 1, 1, 3], np.int32)
+    ix.add(3, ip, np.array([3, 0, 8], np.uint16), np.array([1.0, 0.5, 2.0], np.float32))
+    assert ix.num_vectors() == 3 and ix.get_dimension() == 9
+    d, lab = ix.search(1, np.array([0, 1], np.int32), np.array([3], np.uint16), np.array([1.0], np.float32), 4)
+    assert (lab == -1).all() and (d == -1.0).all()
+    o = OracleInverted(9)
+    o.add(ip, np.array([3, 0, 8], np.uint16), np.array([1.0, 0.5, 2.0], np.float32))
+    assert nsparse.serialize_index(ix) == o.serialize()
+    o.build()
+    built = o.serialize()
+    back = nsparse.read_index_bytes(built)
+    assert back.id() == "INVT" and back.num_vectors() == 3 and back.get_dimension() == 9
+    assert nsparse.serialize_index(back) == built
+    im = nsparse.index_factory(9, "idmap,inverted")
+    im.add_with_ids(3, ip, np.array([3, 0, 8], np.uint16), np.array([1.0, 0.5, 2.0], np.float32),
+                    np.array([70, 71, 72], np.int32))
+    assert im.num_vectors() == 3
+    bad = nsparse.index_factory(9, "inverted")
+    bad.add(1, np.array([0, 1], np.int32), np.array([9], np.uint16), np.array([1.0], np.float32))
+    with pytest.raises(ValueError, match="term_id out of range"):
+        bad.build()
+    with pytest.raises(RuntimeError, match="no vectors"):
+        nsparse.index_factory(9, "inverted").build()
+    wide = bytearray(built)
+    wide[24:32] = np.uint64(8).tobytes()  # element width after fourcc, dim, num_vectors, n_terms
+    with pytest.raises(RuntimeError, match="element width"):
+        nsparse.read_index_bytes(bytes(wide))
+    o2 = OracleInverted(2)
+    o2.add(np.array([0, 1, 2], np.int32), np.array([0, 0], np.uint16), np.array([1.0, 2.0], np.float32))
+    o2.build()
+    raw = bytearray(o2.serialize())
+    # list 0 holds docs [0, 1] right after its length: swap them
+    at = 4 + 4 + 8 * 3 + 8
+    raw[at:at + 8] = np.array([1, 0], np.int32).tobytes()
+    with pytest.raises(RuntimeError, match="ascending doc order"):
+        nsparse.read_index_bytes(bytes(raw))
 
 
 def test_argument_checks(nsparse):
