@@ -166,6 +166,73 @@ def copy_bandwidth(dev, nbytes=2 << 30, reps=5):
     return 2.0 * nbytes / (ms * 1e-3) / 1e9
 
 
+def inverted_bench(args, nsparse, docs, queries, truth, threads):
+    """The exact InvertedIndex on the same FP32 corpus: GPU build, then the
+    drop-in search (host queries, as the reference API takes them, k = 10) of a
+    query sample; recall@10 against the FP32 ground truth; algorithmic bytes =
+    every posting of every query term (doc id + value, 8 B) plus one n_docs
+    float score row written and read per query. Beside it, the oracle's
+    restatement of the reference's MaxScore traversal on all host threads."""
+    n = min(args.inverted_queries, args.queries)
+    if n <= 0:
+        return None
+    import torch
+
+    inv = nsparse.index_factory(30522, "inverted")
+    t0 = time.time()
+    inv.add(args.docs, *docs)
+    inv.build()
+    torch.cuda.synchronize()
+    build_s = time.time() - t0
+    qip = np.ascontiguousarray(queries[0][: n + 1])
+    qix, qv = queries[1][: qip[-1]], queries[2][: qip[-1]]
+    inv.search(min(16, n), qip[: min(16, n) + 1], qix, qv, 10)
+    t0 = time.perf_counter()
+    d, lab = inv.search(n, qip, qix, qv, 10)
+    wall = time.perf_counter() - t0
+    df = np.bincount(docs[1], minlength=30522).astype(np.int64)
+    postings = int(df[qix].sum())
+    alg = postings * 8 + n * args.docs * 8
+    n_rec = min(n, len(truth))
+    hits = [len(set(int(x) for x in lab[i] if x >= 0) & truth[i]) / len(truth[i]) for i in range(n_rec) if truth[i]]
+    out = {"queries": n, "k": 10, "qps": n / wall, "build_s": build_s, "postings_per_query": postings / n,
+           "algorithmic_bytes_per_query": alg / n, "achieved_gbps": alg / wall / 1e9,
+           "frac": alg / wall / 1e9 / HBM_PEAK_GBS, "recall_at_10": float(np.mean(hits)) if hits else None,
+           "recall_queries": n_rec}
+    log(f"inverted (exact): GPU build {build_s:.1f}s; {out['qps']:,.0f} QPS over {n} queries (drop-in, k=10), "
+        f"{postings / n / 1e6:.1f}M postings/query, {out['achieved_gbps']:,.0f} GB/s algorithmic, "
+        f"recall@10 {out['recall_at_10']}")
+    del inv
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle
+
+        t0 = time.time()
+        ref = _oracle.OracleInverted(30522, tie_mode=0)
+        ref.add(*docs)
+        ref.build()
+        ref_build_s = time.time() - t0
+        done, t_cpu, same = 0, 0.0, 0
+        chunk = 16 * threads
+        while t_cpu < args.cpu_seconds / 2 and done < n:
+            hi = min(done + chunk, n)
+            sip = (qip[done:hi + 1] - qip[done]).astype(np.int32)
+            t1 = time.perf_counter()
+            cd, cl = ref.search(sip, qix[qip[done]:qip[hi]], qv[qip[done]:qip[hi]], 10, threads=threads)
+            t_cpu += time.perf_counter() - t1
+            for r in range(hi - done):
+                same += int(set(cl[r].tolist()) == set(lab[done + r].tolist()))
+            done = hi
+        out["cpu_baseline"] = {"value": done / t_cpu, "unit": "queries/s", "cores": threads, "kind": "port",
+                               "sample": f"first {done} of the {n} inverted-benchmark queries, k=10, {chunk} per "
+                                         "call, OpenMP over queries",
+                               "oracle_build_s": ref_build_s, "label_sets_equal_frac": same / max(done, 1)}
+        log(f"inverted cpu (MaxScore, {threads} threads): {done / t_cpu:,.1f} QPS over {done} queries; "
+            f"label sets equal to the GPU's {same}/{done}; oracle build {ref_build_s:.1f}s")
+        del ref
+    return out
+
+
 def same_topk_up_to_ties(d_ref, l_ref, d_got, l_got):
     """Top-k identical apart from exact score ties: the same scores in the same
     order, and the same doc ids at every score above the k-th (docs tied with
@@ -255,6 +322,8 @@ def main():
     ap.add_argument("--serve-batch", type=int, default=1024)
     ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--serve-budget-mb", type=int, default=256)
+    ap.add_argument("--inverted-queries", type=int, default=300,
+                    help="exact InvertedIndex on the same corpus: drop-in search of this many queries (0 skips)")
     ap.add_argument("--schedule", default="auto",
                     help='"auto" (autotuner) or a JSON search config, e.g. {"lookup":1,"block":512,"persistent":1}')
     args = ap.parse_args()
@@ -419,6 +488,9 @@ def main():
     # ---- serving loop on the same resident index
     serving = serving_bench(args, nsparse, index, args.docs, threads)
 
+    # ---- the exact InvertedIndex on the same FP32 corpus
+    inverted = inverted_bench(args, nsparse, docs, queries, truth, threads)
+
     # ---- CPU baseline: the reference's search restated in C++ (oracle/),
     # on the same index bytes, all host threads, bounded query sample.
     cpu = None
@@ -508,6 +580,8 @@ def main():
         result["sweep"] = sweep
     if serving:
         result["serving"] = serving
+    if inverted:
+        result["inverted"] = inverted
     print(json.dumps(result), flush=True)
 
 

@@ -1010,27 +1010,45 @@ __global__ __launch_bounds__(kB) void block_copy_kernel(const int32_t* __restric
     }
 }
 
-// InvertedIndex postings: the term-sorted positions become (doc, value)
-// pairs; each term's maximum positive value is raised through the bit
-// pattern (positive floats order as their bits), with the atomic skipped once
-// the stored maximum is already at least as large; a doc that repeats inside
-// one list (a row listing a term twice) raises `dup`.
-__global__ void gather_postings_kernel(const uint16_t* __restrict__ skeys, const int32_t* __restrict__ spos,
-                                       const int32_t* __restrict__ doc_of, const float* __restrict__ values,
-                                       size_t n, int32_t* __restrict__ docs, float* __restrict__ vals,
-                                       uint32_t* __restrict__ max_bits, int32_t* __restrict__ dup) {
+// InvertedIndex postings: the term-sorted positions become (doc, value) pairs.
+__global__ void gather_postings_kernel(const int32_t* __restrict__ spos, const int32_t* __restrict__ doc_of,
+                                       const float* __restrict__ values, size_t n, int32_t* __restrict__ docs,
+                                       float* __restrict__ vals) {
     GRID_STRIDE(i, n) {
         const int32_t p = spos[i];
-        const int32_t d = doc_of[p];
-        const float v = values[p];
-        docs[i] = d;
-        vals[i] = v;
-        const uint16_t t = skeys[i];
-        if (v > 0.0F) {
-            const uint32_t b = __float_as_uint(v);
-            if (b > max_bits[t]) atomicMax(&max_bits[t], b);
-        }
-        if (i > 0 && skeys[i - 1] == t && doc_of[spos[i - 1]] == d) *dup = 1;
+        docs[i] = doc_of[p];
+        vals[i] = values[p];
+    }
+}
+
+// One workgroup per term: the list's largest value from 0 up (values that
+// are not > the running maximum, NaN included, leave it), and whether a doc
+// repeats inside the list (a row listing the term twice) -- then `dup` is set.
+__global__ __launch_bounds__(kB) void term_max_kernel(const int64_t* __restrict__ list_ptr,
+                                                      const int32_t* __restrict__ docs,
+                                                      const float* __restrict__ vals, float* __restrict__ max_score,
+                                                      int32_t* __restrict__ dup) {
+    __shared__ float wmax[kB / 64];
+    const int64_t a = list_ptr[blockIdx.x], b = list_ptr[blockIdx.x + 1];
+    float m = 0.0F;
+    bool rep = false;
+    for (int64_t i = a + threadIdx.x; i < b; i += kB) {
+        const float v = vals[i];
+        if (v > m) m = v;
+        rep = rep || (i > a && docs[i] == docs[i - 1]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const float o = __shfl_xor(m, off);
+        if (o > m) m = o;
+    }
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    if (rep) *dup = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = wmax[0];
+        for (int w = 1; w < kB / 64; ++w)
+            if (wmax[w] > r) r = wmax[w];
+        max_score[blockIdx.x] = r;
     }
 }
 
@@ -1052,7 +1070,7 @@ void build_postings_on_device(const int32_t* indptr, const uint16_t* indices, co
     out.docs.allocate(Z * 4, 64);
     out.vals.allocate(Z * 4, 64);
     out.n_postings = nnz;
-    hip_check(hipMemset(out.max_score.get(), 0, L * 4), "hipMemset");
+    hip_check(hipMemset(out.max_score.get(), 0, L * 4), "hipMemset");  // lists left empty keep 0
     DeviceBuffer dup(4);
     hip_check(hipMemset(dup.get(), 0, 4), "hipMemset");
     if (Z == 0) {
@@ -1069,9 +1087,12 @@ void build_postings_on_device(const int32_t* indptr, const uint16_t* indices, co
         exclusive_scan_i64(hist64.get<int64_t>(), out.list_ptr.get<int64_t>(), L);
         hipLaunchKernelGGL(doc_of_kernel, dim3(grid_for(static_cast<size_t>(n_docs))), dim3(kB), 0, nullptr, indptr,
                            n_docs, doc_of.get<int32_t>());
-        hipLaunchKernelGGL(gather_postings_kernel, dim3(grid_for(Z)), dim3(kB), 0, nullptr, skeys.get<uint16_t>(),
-                           spos.get<int32_t>(), doc_of.get<int32_t>(), values, Z, out.docs.get<int32_t>(),
-                           out.vals.get<float>(), out.max_score.get<uint32_t>(), dup.get<int32_t>());
+        hipLaunchKernelGGL(gather_postings_kernel, dim3(grid_for(Z)), dim3(kB), 0, nullptr, spos.get<int32_t>(),
+                           doc_of.get<int32_t>(), values, Z, out.docs.get<int32_t>(), out.vals.get<float>());
+        if (L > 0)
+            hipLaunchKernelGGL(term_max_kernel, dim3(static_cast<unsigned>(L)), dim3(kB), 0, nullptr,
+                               out.list_ptr.get<int64_t>(), out.docs.get<int32_t>(), out.vals.get<float>(),
+                               out.max_score.get<float>(), dup.get<int32_t>());
         sync("inverted postings");
     }
     int32_t h_dup = 0;

@@ -1596,51 +1596,84 @@ __device__ void bitonic_desc(uint64_t* keys) {
     }
 }
 
-// One workgroup per row: the row's n_docs scores stream through a filter
-// against the current k-th key; survivors collect in LDS behind the sorted
-// top-kcap and are merged in by a bitonic sort whenever the next round could
-// overflow them. Writes k (score, doc) pairs best first, (-1.0f, -1) past the
-// last candidate. With `positive`, only scores > 0 are candidates; with
-// `allow`, only docs whose bit is set.
-__global__ __launch_bounds__(kBlock) void topk_select_kernel(const float* __restrict__ scores, int64_t ld,
-                                                             int64_t n_docs, int k, int kcap, bool positive,
-                                                             const uint32_t* __restrict__ allow,
-                                                             float* __restrict__ out_d, int32_t* __restrict__ out_l) {
-    __shared__ uint64_t keys[kSelectSlots];
-    __shared__ int cnt;
-    const float* row = scores + static_cast<int64_t>(blockIdx.x) * ld;
+// Keeps the kcap best of n candidate keys (key_of(i); 0 = not a candidate) in
+// keys[0, kcap), best first: the candidates stream through a filter against
+// the current k-th key; survivors collect in LDS behind the sorted top-kcap and
+// are merged in by a bitonic sort whenever the next round could overflow them.
+template <class KeyOf>
+__device__ void stream_topk(uint64_t* keys, int* cnt, int64_t n, int k, int kcap, KeyOf key_of) {
     const int cap = kSelectSlots - kcap;  // candidate slots
     for (int i = threadIdx.x; i < kSelectSlots; i += kBlock) keys[i] = 0;
-    if (threadIdx.x == 0) cnt = 0;
+    if (threadIdx.x == 0) *cnt = 0;
     __syncthreads();
     uint64_t thresh = 0;
     auto merge = [&]() {
-        const int used = kcap + cnt;
+        const int used = kcap + *cnt;
         for (int i = used + threadIdx.x; i < kSelectSlots; i += kBlock) keys[i] = 0;
         __syncthreads();
         bitonic_desc(keys);
-        if (threadIdx.x == 0) cnt = 0;
+        if (threadIdx.x == 0) *cnt = 0;
         thresh = keys[k - 1];
         __syncthreads();
     };
-    for (int64_t base = 0; base < n_docs; base += kSelectRound) {
+    for (int64_t base = 0; base < n; base += kSelectRound) {
 #pragma unroll
         for (int u = 0; u < kSelectRound / kBlock; ++u) {
-            const int64_t d = base + u * kBlock + threadIdx.x;
-            if (d < n_docs) {
-                const float sc = row[d];
-                bool keep = !positive || sc > 0.0F;
-                if (allow != nullptr) keep = keep && ((allow[d >> 5] >> (d & 31)) & 1U) != 0;
-                const uint64_t key = score_key(sc, d);
-                if (keep && key > thresh) keys[kcap + atomicAdd(&cnt, 1)] = key;
+            const int64_t i = base + u * kBlock + threadIdx.x;
+            if (i < n) {
+                const uint64_t key = key_of(i);
+                if (key > thresh) keys[kcap + atomicAdd(cnt, 1)] = key;
             }
         }
         __syncthreads();
-        const int c = cnt;
+        const int c = *cnt;
         __syncthreads();  // every thread has read cnt before the next round adds to it
         if (c > cap - kSelectRound) merge();
     }
-    if (cnt > 0) merge();
+    if (*cnt > 0) merge();
+}
+
+// Doc ids per top-k chunk: each row of scores is cut into chunks of this many
+// docs, selected by workgroups of their own, then merged.
+constexpr int64_t kSelectChunk = 65536;
+
+int64_t select_chunks(int64_t n_docs) { return std::max<int64_t>(1, (n_docs + kSelectChunk - 1) / kSelectChunk); }
+
+// Stage 1, one workgroup per (chunk, row): the k best docs of the chunk by
+// (score desc, doc id asc) -- key = (order-preserving bits of the score) << 32
+// | ~doc -- written as kcap keys (0 = none) to partial[row][chunk]. With
+// `positive`, only scores > 0 are candidates; with `allow`, only docs whose
+// bit is set.
+__global__ __launch_bounds__(kBlock) void topk_chunk_kernel(const float* __restrict__ scores, int64_t ld,
+                                                            int64_t n_docs, int k, int kcap, bool positive,
+                                                            const uint32_t* __restrict__ allow,
+                                                            uint64_t* __restrict__ partial) {
+    __shared__ uint64_t keys[kSelectSlots];
+    __shared__ int cnt;
+    const float* row = scores + static_cast<int64_t>(blockIdx.y) * ld;
+    const int64_t lo = static_cast<int64_t>(blockIdx.x) * kSelectChunk;
+    const int64_t n = min(kSelectChunk, n_docs - lo);
+    stream_topk(keys, &cnt, n, k, kcap, [&](int64_t i) -> uint64_t {
+        const int64_t d = lo + i;
+        const float sc = row[d];
+        bool keep = !positive || sc > 0.0F;
+        if (allow != nullptr) keep = keep && ((allow[d >> 5] >> (d & 31)) & 1U) != 0;
+        return keep ? score_key(sc, d) : 0;
+    });
+    uint64_t* out = partial + (static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x) * kcap;
+    for (int i = threadIdx.x; i < kcap; i += kBlock) out[i] = keys[i];
+}
+
+// Stage 2, one workgroup per row: the k best of the row's n_chunks * kcap
+// partial keys, written as k (score, doc) pairs best first, (-1.0f, -1) past
+// the last candidate.
+__global__ __launch_bounds__(kBlock) void topk_merge_kernel(const uint64_t* __restrict__ partial, int n_chunks,
+                                                            int k, int kcap, float* __restrict__ out_d,
+                                                            int32_t* __restrict__ out_l) {
+    __shared__ uint64_t keys[kSelectSlots];
+    __shared__ int cnt;
+    const uint64_t* in = partial + static_cast<int64_t>(blockIdx.x) * n_chunks * kcap;
+    stream_topk(keys, &cnt, static_cast<int64_t>(n_chunks) * kcap, k, kcap, [&](int64_t i) { return in[i]; });
     float* od = out_d + static_cast<int64_t>(blockIdx.x) * k;
     int32_t* ol = out_l + static_cast<int64_t>(blockIdx.x) * k;
     for (int i = threadIdx.x; i < k; i += kBlock) {
@@ -1648,6 +1681,16 @@ __global__ __launch_bounds__(kBlock) void topk_select_kernel(const float* __rest
         od[i] = key != 0 ? key_score(key) : -1.0F;
         ol[i] = key != 0 ? static_cast<int32_t>(0xFFFFFFFFu - static_cast<uint32_t>(key)) : -1;
     }
+}
+
+// Both stages over g rows of scores; partial holds g * select_chunks(n_docs) * kcap keys.
+void launch_topk(const float* scores, int64_t n_docs, int64_t g, int k, int kcap, bool positive,
+                 const uint32_t* allow, uint64_t* partial, float* out_d, int32_t* out_l, hipStream_t stream) {
+    const int64_t chunks = select_chunks(n_docs);
+    hipLaunchKernelGGL(topk_chunk_kernel, dim3(static_cast<unsigned>(chunks), static_cast<unsigned>(g)),
+                       dim3(kBlock), 0, stream, scores, n_docs, n_docs, k, kcap, positive, allow, partial);
+    hipLaunchKernelGGL(topk_merge_kernel, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, stream, partial,
+                       static_cast<int>(chunks), k, kcap, out_d, out_l);
 }
 
 }  // namespace
@@ -1662,9 +1705,21 @@ void launch_exact_scores(const int32_t* indptr, const uint16_t* indices, const f
     hip_check(hipGetLastError(), "exact scores launch");
 }
 
+// Per group of queries: their rows of n_docs float scores, then the top-k
+// partial keys (select_chunks(n_docs) chunks of up to kBruteKMax keys a row).
 size_t brute_search_scratch_bytes(int64_t n_docs, int64_t n_queries) {
-    return static_cast<size_t>(brute_search_group(n_docs, n_queries)) * static_cast<size_t>(n_docs) * sizeof(float);
+    const size_t g = static_cast<size_t>(brute_search_group(n_docs, n_queries));
+    return ((g * static_cast<size_t>(n_docs) * sizeof(float) + 255) & ~size_t{255}) +
+           g * static_cast<size_t>(select_chunks(n_docs)) * kBruteKMax * sizeof(uint64_t);
 }
+
+namespace {
+uint64_t* partial_keys(float* d_scores, int64_t n_docs, int64_t n_queries) {
+    const size_t g = static_cast<size_t>(brute_search_group(n_docs, n_queries));
+    const size_t off = (g * static_cast<size_t>(n_docs) * sizeof(float) + 255) & ~size_t{255};
+    return reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(d_scores) + off);
+}
+}  // namespace
 
 int64_t brute_search_group(int64_t n_docs, int64_t n_queries) {
     constexpr int64_t kScoreBytes = int64_t{256} << 20;  // score rows held at once
@@ -1687,8 +1742,8 @@ void launch_brute_search(const int32_t* indptr, const uint16_t* indices, const f
         hipLaunchKernelGGL(exact_scores_kernel, dim3(exact_scores_blocks(n_docs), static_cast<unsigned>(g)),
                            dim3(kBlock), exact_scores_lds(dim), stream, indptr, indices, values, n_docs, nw,
                            d_q_indptr, d_q_indices, d_q_values, 0, q0, d_scores, n_docs);
-        hipLaunchKernelGGL(topk_select_kernel, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, stream, d_scores,
-                           n_docs, n_docs, k, kcap, false, nullptr, d_out_d + q0 * k, d_out_l + q0 * k);
+        launch_topk(d_scores, n_docs, g, k, kcap, false, nullptr, partial_keys(d_scores, n_docs, n_queries),
+                    d_out_d + q0 * k, d_out_l + q0 * k, stream);
         hip_check(hipGetLastError(), "brute-force search launch");
     }
 }
@@ -1698,21 +1753,43 @@ namespace {
 // ---- InvertedIndex: exact term-at-a-time scores in LDS windows of doc ids
 constexpr int kInvBlock = 512;
 constexpr int kInvWaves = kInvBlock / 64;
-constexpr int kInvWindow = 8192;  // doc ids per window: 32 KB of float scores
+constexpr int kInvWindow = 8192;      // doc ids per window: 32 KB of float scores
+constexpr int kInvChunks = 4;         // a round takes up to kInvChunks * kInvBlock postings of one term
 constexpr int kInvSlabTarget = 2048;  // workgroups a group of queries aims for
+
+// Postings a round holds in registers: lane tid has positions
+// p0 + j * kInvBlock + tid, j < kInvChunks; unloaded positions read as doc
+// INT32_MAX (never inside a window).
+struct InvRound {
+    int32_t d[kInvChunks];
+    float v[kInvChunks];
+};
+
+__device__ __forceinline__ void inv_load(const int32_t* __restrict__ docs, const float* __restrict__ vals, int64_t p0,
+                                         int64_t e, int nu, int tid, InvRound& r) {
+#pragma unroll
+    for (int j = 0; j < kInvChunks; ++j) {
+        const int64_t p = p0 + static_cast<int64_t>(j) * kInvBlock + tid;
+        const bool ok = j < nu && p < e;
+        r.d[j] = ok ? docs[p] : INT32_MAX;
+        r.v[j] = ok ? vals[p] : 0.0F;
+    }
+}
 
 // One workgroup per (slab of consecutive windows, query). Per query term, LDS
 // keeps the cursor into its posting list (found once per slab by binary
 // search), the list end and the doc id under the cursor. Each window is
-// scored term by term in query order: the whole workgroup takes the term's
-// next kInvBlock postings at a time, lane i adds weight * value of posting
-// cursor + i into its doc's slot while that doc is inside the window, and a
-// barrier separates the rounds. Docs ascend within a list, so the lanes inside
-// the window are a prefix and the first lane past it holds the new cursor.
-// A doc's score is therefore the float sum of its products in query-term
-// order, from 0 -- the same bits however the work is split. The scores of the
-// window are then written to row blockIdx.y of `scores` (row stride ld).
-// kAtomic: some list holds a doc twice, so one round may add to a slot twice.
+// scored term by term in query order: a round of the whole workgroup takes the
+// term's next nu * kInvBlock postings (nu from the list's remaining density,
+// up to kInvChunks), every posting inside the window adds weight * value into
+// its doc's slot, and a barrier ends the round. Docs ascend within a list, so
+// the postings inside the window are a prefix of the round and the first one
+// past it is the new cursor. While a term's round is being added, the next
+// term's first round is already loading. A doc's score is therefore the float
+// sum of its products in query-term order, from 0 -- the same bits however the
+// work is split. The window's scores are then written to row blockIdx.y of
+// `scores` (row stride ld). kAtomic: some list holds a doc twice, so one round
+// may add to a slot twice.
 template <bool kAtomic>
 __global__ __launch_bounds__(kInvBlock) void inverted_scores_kernel(
     const int64_t* __restrict__ list_ptr, const int32_t* __restrict__ docs, const float* __restrict__ vals,
@@ -1720,7 +1797,8 @@ __global__ __launch_bounds__(kInvBlock) void inverted_scores_kernel(
     const uint16_t* __restrict__ q_indices, const float* __restrict__ q_values, int64_t q0,
     float* __restrict__ scores, int64_t ld) {
     __shared__ float win[kInvWindow];
-    __shared__ int64_t cur[kQMax], stop[kQMax];
+    __shared__ int64_t cur[kQMax];
+    __shared__ int64_t stop[kQMax];
     __shared__ int32_t head[kQMax];
     __shared__ float wt[kQMax];
     __shared__ int32_t inside[2][kInvWaves];
@@ -1752,37 +1830,68 @@ __global__ __launch_bounds__(kInvBlock) void inverted_scores_kernel(
     int round = 0;
     for (int64_t base = first; base < last; base += kInvWindow) {
         const int64_t wend = min(base + kInvWindow, last);
-        for (int t = 0; t < qn; ++t) {
-            if (head[t] >= wend) continue;  // nothing of this term in the window
-            const float w = wt[t];
+        // the expected postings of a term in this window, from its remaining
+        // list spread over the remaining doc ids, with a margin
+        const float frac = 1.25F * static_cast<float>(wend - base) / static_cast<float>(n_docs - base);
+        auto rounds_for = [&](int t) {
+            const float est = static_cast<float>(stop[t] - cur[t]) * frac;
+            return min(kInvChunks, static_cast<int>(est * (1.0F / kInvBlock)) + 1);
+        };
+        int t = 0;
+        while (t < qn && head[t] >= wend) ++t;  // terms with nothing in the window are skipped
+        InvRound nx;
+        int nx_nu = 0;
+        if (t < qn) {
+            nx_nu = rounds_for(t);
+            inv_load(docs, vals, cur[t], stop[t], nx_nu, tid, nx);
+        }
+        while (t < qn) {
+            InvRound c = nx;
+            int nu = nx_nu;
+            int64_t p0 = cur[t];
             const int64_t e = stop[t];
-            for (int64_t p0 = cur[t];; p0 += kInvBlock) {
-                const int64_t p = p0 + tid;
-                const int32_t d = p < e ? docs[p] : INT32_MAX;
-                const bool in = d < wend;
-                if (in) {
-                    const float add = w * vals[p];
-                    float* slot = &win[d - base];
-                    if (kAtomic)
-                        atomicAdd(slot, add);
-                    else
-                        *slot += add;
-                }
-                const uint64_t m = __ballot(in);
-                if (lane == 0) inside[round & 1][wave] = __popcll(m);
-                __syncthreads();
-                int c = 0;
+            const float w = wt[t];
+            int t2 = t + 1;
+            while (t2 < qn && head[t2] >= wend) ++t2;
+            if (t2 < qn) {
+                nx_nu = rounds_for(t2);
+                inv_load(docs, vals, cur[t2], stop[t2], nx_nu, tid, nx);
+            }
+            for (;;) {
+                int mine = 0;
 #pragma unroll
-                for (int v = 0; v < kInvWaves; ++v) c += inside[round & 1][v];
-                ++round;
-                if (c < kInvBlock) {
-                    if (tid == c) {  // first posting past the window
-                        cur[t] = p;
-                        head[t] = d;
+                for (int j = 0; j < kInvChunks; ++j) {
+                    const bool in = c.d[j] < wend;
+                    if (in) {
+                        const float add = w * c.v[j];
+                        float* slot = &win[c.d[j] - base];
+                        if (kAtomic)
+                            atomicAdd(slot, add);
+                        else
+                            *slot += add;
                     }
+                    mine += __popcll(__ballot(in));
+                }
+                if (lane == 0) inside[round & 1][wave] = mine;
+                __syncthreads();
+                int cnt = 0;
+#pragma unroll
+                for (int v = 0; v < kInvWaves; ++v) cnt += inside[round & 1][v];
+                ++round;
+                if (cnt < nu * kInvBlock) {
+#pragma unroll
+                    for (int j = 0; j < kInvChunks; ++j)
+                        if (j * kInvBlock + tid == cnt) {  // the first posting past the window
+                            cur[t] = p0 + cnt;
+                            head[t] = c.d[j];
+                        }
                     break;
                 }
+                p0 += static_cast<int64_t>(nu) * kInvBlock;
+                nu = kInvChunks;  // still inside after a whole round: take whole rounds
+                inv_load(docs, vals, p0, e, nu, tid, c);
             }
+            t = t2;
         }
         __syncthreads();
         for (int i = tid; i < wend - base; i += kInvBlock) {
@@ -1818,8 +1927,8 @@ void launch_inverted_search(const DevicePostings& ps, int64_t n_docs, const int3
             hipLaunchKernelGGL(inverted_scores_kernel<false>, grid, dim3(kInvBlock), 0, stream,
                                ps.list_ptr.get<int64_t>(), ps.docs.get<int32_t>(), ps.vals.get<float>(), n_docs,
                                static_cast<int>(wps), d_q_indptr, d_q_indices, d_q_values, q0, d_scores, n_docs);
-        hipLaunchKernelGGL(topk_select_kernel, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, stream, d_scores,
-                           n_docs, n_docs, k, kcap, true, d_allow, d_out_d + q0 * k, d_out_l + q0 * k);
+        launch_topk(d_scores, n_docs, g, k, kcap, true, d_allow, partial_keys(d_scores, n_docs, n_queries),
+                    d_out_d + q0 * k, d_out_l + q0 * k, stream);
         hip_check(hipGetLastError(), "inverted search launch");
     }
 }
