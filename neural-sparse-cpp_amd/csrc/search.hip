@@ -29,6 +29,7 @@
 #include <cstdlib>
 
 #include <algorithm>
+#include <limits>
 #include <stdexcept>
 #include <string>
 
@@ -1758,19 +1759,20 @@ constexpr int kInvChunks = 4;         // a round takes up to kInvChunks * kInvBl
 constexpr int kInvSlabTarget = 2048;  // workgroups a group of queries aims for
 
 // Postings a round holds in registers: lane tid has positions
-// p0 + j * kInvBlock + tid, j < kInvChunks; unloaded positions read as doc
-// INT32_MAX (never inside a window).
+// p0 + j * kInvBlock + tid for j < kInvChunks, loaded below p0 + n and the
+// list end e; the others read as doc INT32_MAX (never inside a window).
 struct InvRound {
     int32_t d[kInvChunks];
     float v[kInvChunks];
 };
 
 __device__ __forceinline__ void inv_load(const int32_t* __restrict__ docs, const float* __restrict__ vals, int64_t p0,
-                                         int64_t e, int nu, int tid, InvRound& r) {
+                                         int64_t e, int n, int tid, InvRound& r) {
 #pragma unroll
     for (int j = 0; j < kInvChunks; ++j) {
-        const int64_t p = p0 + static_cast<int64_t>(j) * kInvBlock + tid;
-        const bool ok = j < nu && p < e;
+        const int off = j * kInvBlock + tid;
+        const int64_t p = p0 + off;
+        const bool ok = off < n && p < e;
         r.d[j] = ok ? docs[p] : INT32_MAX;
         r.v[j] = ok ? vals[p] : 0.0F;
     }
@@ -1780,8 +1782,10 @@ __device__ __forceinline__ void inv_load(const int32_t* __restrict__ docs, const
 // keeps the cursor into its posting list (found once per slab by binary
 // search), the list end and the doc id under the cursor. Each window is
 // scored term by term in query order: a round of the whole workgroup takes the
-// term's next nu * kInvBlock postings (nu from the list's remaining density,
-// up to kInvChunks), every posting inside the window adds weight * value into
+// term's next n postings (the count its remaining list's density predicts in
+// the window, with a margin, up to kInvChunks * kInvBlock: what a round loads
+// past the window is read again by the next one), every posting inside the
+// window adds weight * value into
 // its doc's slot, and a barrier ends the round. Docs ascend within a list, so
 // the postings inside the window are a prefix of the round and the first one
 // past it is the new cursor. While a term's round is being added, the next
@@ -1797,8 +1801,8 @@ __global__ __launch_bounds__(kInvBlock) void inverted_scores_kernel(
     const uint16_t* __restrict__ q_indices, const float* __restrict__ q_values, int64_t q0,
     float* __restrict__ scores, int64_t ld) {
     __shared__ float win[kInvWindow];
-    __shared__ int64_t cur[kQMax];
-    __shared__ int64_t stop[kQMax];
+    __shared__ int32_t cur[kQMax];  // list positions fit 32 bits: the host caps postings at 2^31 - 1
+    __shared__ int32_t stop[kQMax];
     __shared__ int32_t head[kQMax];
     __shared__ float wt[kQMax];
     __shared__ int32_t inside[2][kInvWaves];
@@ -1820,8 +1824,8 @@ __global__ __launch_bounds__(kInvBlock) void inverted_scores_kernel(
             else
                 b = m;
         }
-        cur[t] = a;
-        stop[t] = e;
+        cur[t] = static_cast<int32_t>(a);
+        stop[t] = static_cast<int32_t>(e);
         head[t] = a < e ? docs[a] : INT32_MAX;
         wt[t] = q_values[qs + t];
     }
@@ -1830,32 +1834,33 @@ __global__ __launch_bounds__(kInvBlock) void inverted_scores_kernel(
     int round = 0;
     for (int64_t base = first; base < last; base += kInvWindow) {
         const int64_t wend = min(base + kInvWindow, last);
-        // the expected postings of a term in this window, from its remaining
-        // list spread over the remaining doc ids, with a margin
-        const float frac = 1.25F * static_cast<float>(wend - base) / static_cast<float>(n_docs - base);
-        auto rounds_for = [&](int t) {
+        // postings a term's first round takes: its remaining list spread over
+        // the remaining doc ids predicts the count in this window; 1.5x that
+        // plus 64 rarely falls short (a short round is followed by full ones)
+        const float frac = static_cast<float>(wend - base) / static_cast<float>(n_docs - base);
+        auto round_size = [&](int t) {
             const float est = static_cast<float>(stop[t] - cur[t]) * frac;
-            return min(kInvChunks, static_cast<int>(est * (1.0F / kInvBlock)) + 1);
+            return static_cast<int>(min(1.5F * est + 64.0F, static_cast<float>(kInvChunks * kInvBlock)));
         };
         int t = 0;
         while (t < qn && head[t] >= wend) ++t;  // terms with nothing in the window are skipped
         InvRound nx;
-        int nx_nu = 0;
+        int nx_n = 0;
         if (t < qn) {
-            nx_nu = rounds_for(t);
-            inv_load(docs, vals, cur[t], stop[t], nx_nu, tid, nx);
+            nx_n = round_size(t);
+            inv_load(docs, vals, cur[t], stop[t], nx_n, tid, nx);
         }
         while (t < qn) {
             InvRound c = nx;
-            int nu = nx_nu;
+            int n_ld = nx_n;
             int64_t p0 = cur[t];
             const int64_t e = stop[t];
             const float w = wt[t];
             int t2 = t + 1;
             while (t2 < qn && head[t2] >= wend) ++t2;
             if (t2 < qn) {
-                nx_nu = rounds_for(t2);
-                inv_load(docs, vals, cur[t2], stop[t2], nx_nu, tid, nx);
+                nx_n = round_size(t2);
+                inv_load(docs, vals, cur[t2], stop[t2], nx_n, tid, nx);
             }
             for (;;) {
                 int mine = 0;
@@ -1878,18 +1883,18 @@ __global__ __launch_bounds__(kInvBlock) void inverted_scores_kernel(
 #pragma unroll
                 for (int v = 0; v < kInvWaves; ++v) cnt += inside[round & 1][v];
                 ++round;
-                if (cnt < nu * kInvBlock) {
+                if (cnt < n_ld) {
 #pragma unroll
                     for (int j = 0; j < kInvChunks; ++j)
                         if (j * kInvBlock + tid == cnt) {  // the first posting past the window
-                            cur[t] = p0 + cnt;
+                            cur[t] = static_cast<int32_t>(p0 + cnt);
                             head[t] = c.d[j];
                         }
                     break;
                 }
-                p0 += static_cast<int64_t>(nu) * kInvBlock;
-                nu = kInvChunks;  // still inside after a whole round: take whole rounds
-                inv_load(docs, vals, p0, e, nu, tid, c);
+                p0 += n_ld;
+                n_ld = kInvChunks * kInvBlock;  // still inside after a whole round: take whole rounds
+                inv_load(docs, vals, p0, e, n_ld, tid, c);
             }
             t = t2;
         }
@@ -1910,6 +1915,8 @@ void launch_inverted_search(const DevicePostings& ps, int64_t n_docs, const int3
                             hipStream_t stream) {
     if (k <= 0 || k > kBruteKMax) throw std::invalid_argument("inverted search supports 1 <= k <= 1024 on the device");
     if (n_docs <= 0 || n_queries <= 0) throw std::invalid_argument("inverted search needs documents and queries");
+    if (ps.n_postings > std::numeric_limits<int32_t>::max())
+        throw std::invalid_argument("inverted search addresses at most 2^31 - 1 postings");
     int kcap = 1;
     while (kcap < k) kcap <<= 1;
     const int64_t group = brute_search_group(n_docs, n_queries);
