@@ -37,6 +37,54 @@ def test_c0_fp32_100k_build_and_search(nsparse):
     assert (got[1] == want[1]).all() and (got[0] == want[0]).all()
 
 
+def test_c1_sq_1m_plain_search_and_recall(nsparse):
+    """C1 -- SEISMIC-SQ over 1M docs (per-index min/max), lambda=6000, beta=600
+    (centroid fraction 0.1), alpha=0.4, cut 10: 10k queries (C0's distributions)
+    at k=10 one per launch through the plain drop-in at heap_factor 0.8, and
+    batched at 0.7 and 0.9. Every result equals the oracle's on the device-built
+    index (scores bit for bit). Recall@10 against exact FP32 brute force over 200
+    queries is then the oracle's recall exactly, and clears the reference python
+    tests' SQ floor (test_seismic_sq_index.py:46, 0.82)."""
+    from nsparse import synth
+
+    docs = synth.make_corpus(synth.doc_spec(seed=0), 1_000_000, threads=16)
+    qip, qix, qv = synth.make_corpus(synth.query_spec(seed=1), 10_000)
+    vmin, vmax = float(docs[2].min()), float(docs[2].max())
+    idx = nsparse.index_factory(DIM, f"seismic_sq,quantizer=8bit|vmin={vmin:.9g}|vmax={vmax:.9g}|lambda=6000|"
+                                     "beta=600|alpha=0.4|seed=0")
+    idx.add(len(docs[0]) - 1, *docs)
+    idx.build()
+    o = O.OracleIndex.load_from_index(nsparse, idx, tie_mode=1)
+    nq = len(qip) - 1
+    print("C1: 10k plain searches at hf 0.8", flush=True)
+    p = nsparse.SeismicSearchParameters(10, 0.8)
+    got_d = np.empty((nq, 10), np.float32)
+    got_l = np.empty((nq, 10), np.int32)
+    for i in range(nq):
+        a, b = int(qip[i]), int(qip[i + 1])
+        got_d[i], got_l[i] = (x[0] for x in idx.search(1, np.array([0, b - a], np.int32), qix[a:b], qv[a:b], 10, p))
+    want_d, want_l = o.search(qip, qix, qv, 10, cut=10, hf=0.8, threads=16)
+    assert np.array_equal(got_l, want_l) and np.array_equal(got_d, want_d)
+    for hf in (0.7, 0.9):
+        d, lab = idx.search(nq, qip, qix, qv, 10, nsparse.SeismicSearchParameters(10, hf))
+        wd, wl = o.search(qip, qix, qv, 10, cut=10, hf=hf, threads=16)
+        assert np.array_equal(lab, wl) and np.array_equal(d, wd), hf
+    # recall@10 of the plain results against exact FP32 scores on the unquantized docs
+    D = _csr(*docs)
+    hits = []
+    for q0 in range(0, 200, 20):
+        Q = _csr(qip[q0:q0 + 21] - qip[q0], qix[qip[q0]:qip[q0 + 20]], qv[qip[q0]:qip[q0 + 20]])
+        S = (Q @ D.T).toarray()
+        for r in range(S.shape[0]):
+            (el,) = np.nonzero(S[r] > 0)
+            truth = set(el[np.lexsort((el, -S[r][el]))][:10].tolist())
+            if truth:
+                hits.append(len(truth & set(got_l[q0 + r].tolist())) / len(truth))
+    recall = float(np.mean(hits))
+    print(f"C1 recall@10 (hf 0.8, 200 queries) = {recall:.4f}", flush=True)
+    assert recall >= 0.82, recall
+
+
 def _csr(ip, ix, v, n_cols=DIM):
     import scipy.sparse as sp
 
