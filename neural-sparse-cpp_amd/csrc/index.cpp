@@ -250,6 +250,7 @@ void SeismicIndex::refresh_block_rows() {
 }
 
 void SeismicIndex::ensure_device() {
+    std::lock_guard<std::mutex> lock(dev_mu_);
     if (!dev_fwd_valid_) {
         // vector loads read whole 4-element words, so pad every array by 8 elements
         d_indptr_.upload(fwd_.indptr.data(), fwd_.indptr.size() * 4, 64);
@@ -843,11 +844,14 @@ void BrutalIndex::search(idx_t n, const idx_t* indptr, const term_t* indices, co
             throw std::invalid_argument("query has more non-zeros than the device path supports (" +
                                         std::to_string(kDeviceQueryMax) + ")");
     require_device();
-    if (!dev_valid_) {
-        d_indptr_.upload(fwd_.indptr.data(), fwd_.indptr.size() * 4, 64);
-        d_indices_.upload(fwd_.indices.data(), fwd_.indices.size() * 2, 64);
-        d_values_.upload(fwd_.values.data(), fwd_.values.size(), 64);
-        dev_valid_ = true;
+    {
+        std::lock_guard<std::mutex> lock(dev_mu_);
+        if (!dev_valid_) {
+            d_indptr_.upload(fwd_.indptr.data(), fwd_.indptr.size() * 4, 64);
+            d_indices_.upload(fwd_.indices.data(), fwd_.indices.size() * 2, 64);
+            d_values_.upload(fwd_.values.data(), fwd_.values.size(), 64);
+            dev_valid_ = true;
+        }
     }
     const int64_t q0 = indptr[0], qnnz = indptr[n] - q0;
     std::vector<int32_t> qip(n + 1);
@@ -931,6 +935,7 @@ void InvertedIndex::build() {
 }
 
 void InvertedIndex::ensure_device() {
+    std::lock_guard<std::mutex> lock(mu_);
     if (dev_valid_) return;
     require_device();
     dev_.list_ptr.upload(h_list_ptr_);
@@ -948,6 +953,7 @@ void InvertedIndex::ensure_device() {
 }
 
 void InvertedIndex::ensure_host() const {
+    std::lock_guard<std::mutex> lock(mu_);
     if (host_valid_) return;
     const size_t L = static_cast<size_t>(n_lists_);
     h_list_ptr_ = dev_.list_ptr.to_host<int64_t>(L + 1);

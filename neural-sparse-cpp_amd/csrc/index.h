@@ -6,6 +6,7 @@
 #include <array>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -148,6 +149,9 @@ private:
     DeviceBuffer d_row_ref_, d_blk_indices_, d_blk_codes_;  // block-contiguous row copies (uint8)
     DeviceBuffer d_err_;
     DevIndexView view_{};
+    // Guards the lazy upload of host state, so concurrent searches on one
+    // built index (which the reference allows) find it resident exactly once.
+    std::mutex dev_mu_;
 };
 
 // BrutalIndex (brutal_index.h:22-44, brutal_index.cpp): FP32 vectors kept as
@@ -170,6 +174,7 @@ private:
     HostCsr fwd_;
     bool dev_valid_ = false;
     DeviceBuffer d_indptr_, d_indices_, d_values_;
+    std::mutex dev_mu_;  // the lazy upload, as in SeismicIndex
 };
 
 // InvertedIndex (inverted_index.h:25-61, inverted_index.cpp): FP32 posting
@@ -214,6 +219,7 @@ private:
     mutable std::vector<idx_t> h_docs_;
     mutable std::vector<float> h_vals_, h_max_;  // h_max_: max_term_scores_
     DevicePostings dev_;  // max_score is filled by a build only
+    mutable std::mutex mu_;  // the lazy copies between host and device
 };
 
 // A batched search over fixed device buffers, replayed from a hipGraph: the
