@@ -542,6 +542,8 @@ def main():
         t1 = time.time()
         ref = _oracle.OracleIndex.load_from_index(nsparse, index, tie_mode=0)
         log(f"oracle copy of the index loaded in {time.time() - t1:.1f}s")
+        cpu_labels = np.full((n_rec, 10), -1, np.int64)  # the CPU's top-10 of the recall queries
+
         def cpu_run(n_threads, seconds, chunk):
             """Searches the benchmark queries in order, chunk by chunk, until
             `seconds` of search time; returns (queries, seconds, agree, total)."""
@@ -562,6 +564,8 @@ def main():
                     for r, j in enumerate(sel):
                         total += 1
                         agree += int(same_topk_up_to_ties(cd[r], cl[r], dists[j, :k], labels[j, :k]))
+                        if j < n_rec:
+                            cpu_labels[j] = cl[r, :10]
                 done = sl.stop
             return done, t_cpu, agree, total
 
@@ -578,11 +582,12 @@ def main():
                "single_thread_qps": done1 / t_cpu1,
                "single_thread_sample": f"first {done1} benchmark queries, 1 thread",
                "cpu_model": model, "sockets": sockets, "u8_dot": simd,
+               "recall_at_10": recall_of(cpu_labels) if n_rec and done >= n_rec else None,
                "notes": "the oracle's traversal with the reference's one-doc-ahead row prefetch, a flat visited "
                         "set per thread, and the AVX-512 gather dot product when the CPU has it"}
         log(f"cpu: {cpu['value']:,.0f} QPS on {threads} threads, top-k identical up to ties {agree}/{total}; "
             f"{cpu['single_thread_qps']:,.1f} QPS on 1 thread ({done1} queries); {model}, {sockets} socket(s), "
-            f"{simd} u8 dot")
+            f"{simd} u8 dot; CPU recall@10 {cpu['recall_at_10']}")
 
     traffic, traffic_src = measured_traffic(f"seismic_sq_batched_{args.docs}docs/{args.queries}q")
     copy_gbs = copy_bandwidth(dev)
@@ -609,6 +614,8 @@ def main():
         "batched_over_cpu": (qps / cpu["value"]) if cpu else None,
         "recall_at_10": recall if n_rec else None, "recall_queries": n_rec,
         "recall_vs_uint8_exact": recall_sq if n_rec else None,
+        "recall_minus_cpu_port": (recall - cpu["recall_at_10"]) if cpu and cpu.get("recall_at_10") is not None
+        else None,
         "uint8_exact_vs_fp32_exact": ceiling if n_rec else None,
         "build_s": build_s,
         "build": build,
