@@ -376,7 +376,7 @@ __global__ __launch_bounds__(kB) void assign_kernel(
                 // counts and dealt to the lanes, kW windows of 64 with their loads
                 // in flight together, and added with LDS atomics. A lane finds
                 // its term as the last lane whose prefix is <= its item.
-                constexpr int kW = 8;
+                constexpr int kW = 16;
                 for (int b = r0; b < r1; b += 64) {
                     const int p = b + lane;
                     int cs = 0, cnt = 0;
