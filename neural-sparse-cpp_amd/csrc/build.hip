@@ -562,6 +562,160 @@ __device__ __forceinline__ void byte_max(uint32_t* w, int t, uint32_t v) {
 
 __device__ __forceinline__ uint32_t max_of(const uint32_t* w, int t) { return (w[t >> 2] >> ((t & 3) * 8)) & 0xFFu; }
 
+// The alpha cut of one cluster by one wavefront, from the 256-bin histogram of
+// its per-term maxima and their exact integer sum. The reference sorts the
+// (term, max) pairs by value desc, adds them up in that order and stops at the
+// first addup / sum >= alpha (inverted_list_clusters.cpp:43-50). Lane l holds
+// values 255 - 4l - u (u < 4); a DPP prefix gives each value the exact sum of
+// all higher ones (a), and every non-empty value runs the sequential loop's
+// test float(a + v * m) / sum >= alpha; the first value that passes (in
+// descending order) is the cut, and its lane finds how many of its equal terms
+// are kept. Returns (cut value, terms kept at it), the same in every lane;
+// (-1, 0) when no value passes: every touched term is kept.
+__device__ int2 alpha_cut(const int32_t* hist, int sum_i, float alpha, int lane) {
+    const float sum = static_cast<float>(sum_i);
+    int m4[4];
+    uint32_t part = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        m4[u] = hist[255 - 4 * lane - u];
+        part += static_cast<uint32_t>((255 - 4 * lane - u) * m4[u]);
+    }
+    int64_t a = static_cast<int64_t>(wave_incl_scan_dpp(part) - part);
+    int hit = -1;
+    int64_t a_hit = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int v = 255 - 4 * lane - u, m = m4[u];
+        if (hit < 0 && m != 0 && static_cast<float>(a + static_cast<int64_t>(v) * m) / sum >= alpha) {
+            hit = u;
+            a_hit = a;
+        }
+        a += static_cast<int64_t>(v) * m;
+    }
+    const uint64_t hm = __ballot(hit >= 0);
+    if (hm == 0) return make_int2(-1, 0);
+    const int first = __ffsll(static_cast<unsigned long long>(hm)) - 1;
+    int cv = 0, lo = 1;
+    if (lane == first) {
+        const int v = 255 - 4 * lane - hit, m = m4[hit];
+        int hi = m;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (static_cast<float>(a_hit + static_cast<int64_t>(v) * mid) / sum >= alpha)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        cv = v;
+    }
+    return make_int2(__shfl(cv, first, 64), __shfl(lo, first, 64));
+}
+
+// Clusters whose rows hold at most kSumSmallCap entries are summarized by one
+// wavefront each: the (term << 8 | code) entries of the cluster's rows are
+// bitonic-sorted in LDS, so each term's last entry carries its maximum and the
+// terms come in ascending order. Then come the histogram of the maxima, the
+// alpha cut, and the kept terms -- those above the cut value, and the first
+// cut_x at it in term order -- counted (count pass) or written at out_off[g]
+// (write pass), exactly as summarize_u8_kernel does. Larger clusters are left
+// to it: the count pass appends them to `big`.
+constexpr int kSumSmallCap = 2048;
+constexpr int kSumWaves = kB / 64;
+
+template <bool WRITE>
+__global__ __launch_bounds__(kB) void summarize_small_u8_kernel(
+    const int32_t* __restrict__ indptr, const uint16_t* __restrict__ indices, const uint8_t* __restrict__ values,
+    const int64_t* __restrict__ cdp, const int32_t* __restrict__ docs, const int32_t* __restrict__ cluster_list,
+    const int32_t* __restrict__ lcb, int64_t n_clusters, float alpha, int64_t* __restrict__ count,
+    const int64_t* __restrict__ out_off, uint16_t* __restrict__ out_term, uint8_t* __restrict__ out_val,
+    uint16_t* __restrict__ out_cl, int32_t* __restrict__ big, int32_t* __restrict__ n_big) {
+    __shared__ uint32_t ent_all[kSumWaves][kSumSmallCap];
+    __shared__ int32_t hist_all[kSumWaves][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t* ent = ent_all[wave];
+    int32_t* hist = hist_all[wave];
+    const uint64_t below = (uint64_t{1} << lane) - 1;
+    for (int64_t g = static_cast<int64_t>(blockIdx.x) * kSumWaves + wave; g < n_clusters;
+         g += static_cast<int64_t>(gridDim.x) * kSumWaves) {
+        const int64_t q0 = cdp[g], q1 = cdp[g + 1];
+        int n_e = 0;
+        for (int64_t q = q0 + lane; q < q1; q += 64) {
+            const int32_t d = docs[q];
+            n_e += indptr[d + 1] - indptr[d];
+        }
+        for (int off = 32; off > 0; off >>= 1) n_e += __shfl_xor(n_e, off, 64);
+        if (n_e > kSumSmallCap) {
+            if (!WRITE && lane == 0) big[atomicAdd(n_big, 1)] = static_cast<int32_t>(g);
+            continue;
+        }
+        int pos = 0;
+        for (int64_t q = q0; q < q1; ++q) {
+            const int32_t d = docs[q], r0 = indptr[d], r1 = indptr[d + 1];
+            for (int32_t p = r0 + lane; p < r1; p += 64)
+                ent[pos + (p - r0)] = (static_cast<uint32_t>(indices[p]) << 8) | values[p];
+            pos += r1 - r0;
+        }
+        int n2 = 64;
+        while (n2 < n_e) n2 <<= 1;
+        for (int i = n_e + lane; i < n2; i += 64) ent[i] = 0xFFFFFFFFu;
+        for (int i = lane; i < 256; i += 64) hist[i] = 0;
+        __builtin_amdgcn_wave_barrier();
+        for (int size = 2; size <= n2; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int t = lane; t < n2 / 2; t += 64) {
+                    const int i = 2 * t - (t & (stride - 1)), j = i + stride;
+                    const uint32_t a = ent[i], b = ent[j];
+                    if ((a > b) == ((i & size) == 0)) {
+                        ent[i] = b;
+                        ent[j] = a;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        int local_sum = 0;
+        for (int i = lane; i < n_e; i += 64) {
+            const uint32_t k = ent[i];
+            if (i + 1 == n_e || (ent[i + 1] >> 8) != (k >> 8)) {
+                atomicAdd(&hist[k & 0xFFu], 1);
+                local_sum += static_cast<int>(k & 0xFFu);
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) local_sum += __shfl_xor(local_sum, off, 64);
+        __builtin_amdgcn_wave_barrier();
+        const int2 cut = alpha_cut(hist, local_sum, alpha, lane);
+        int run_tie = 0, run_keep = 0;
+        const int64_t base = WRITE ? out_off[g] : 0;
+        const uint16_t local_c = WRITE ? static_cast<uint16_t>(g - lcb[cluster_list[g]]) : uint16_t{0};
+        for (int i0 = 0; i0 < n_e; i0 += 64) {
+            const int i = i0 + lane;
+            uint32_t k = 0;
+            bool last = false;
+            if (i < n_e) {
+                k = ent[i];
+                last = i + 1 == n_e || (ent[i + 1] >> 8) != (k >> 8);
+            }
+            const int v = static_cast<int>(k & 0xFFu);
+            const bool tie = last && v == cut.x;
+            const uint64_t tb = __ballot(tie);
+            const int tie_rank = run_tie + __popcll(tb & below);
+            const bool keep = last && (v > cut.x || (tie && tie_rank < cut.y));
+            const uint64_t kb = __ballot(keep);
+            if (WRITE && keep) {
+                const int64_t o = base + run_keep + __popcll(kb & below);
+                out_term[o] = static_cast<uint16_t>(k >> 8);
+                out_val[o] = static_cast<uint8_t>(v);
+                out_cl[o] = local_c;
+            }
+            run_tie += __popcll(tb);
+            run_keep += __popcll(kb);
+        }
+        if (!WRITE && lane == 0) count[g] = run_keep;
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 template <bool WRITE>
 __global__ __launch_bounds__(kB) void summarize_u8_kernel(const int32_t* __restrict__ indptr,
                                                           const uint16_t* __restrict__ indices,
@@ -572,7 +726,9 @@ __global__ __launch_bounds__(kB) void summarize_u8_kernel(const int32_t* __restr
                                                           float alpha, int64_t* __restrict__ count,
                                                           const int64_t* __restrict__ out_off,
                                                           uint16_t* __restrict__ out_term, uint8_t* __restrict__ out_val,
-                                                          uint16_t* __restrict__ out_cl) {
+                                                          uint16_t* __restrict__ out_cl,
+                                                          const int32_t* __restrict__ list,
+                                                          const int32_t* __restrict__ n_list) {
     extern __shared__ __align__(16) unsigned char smem[];
     SumLds8 S;
     S.maxw = reinterpret_cast<uint32_t*>(smem);
@@ -585,7 +741,10 @@ __global__ __launch_bounds__(kB) void summarize_u8_kernel(const int32_t* __restr
     for (int w = tid; w < nw * 8; w += kB) S.maxw[w] = 0;
     for (int w = tid; w < nw; w += kB) S.touch[w] = 0;
     __syncthreads();
-    for (int64_t g = blockIdx.x; g < n_clusters; g += gridDim.x) {
+    // every cluster, or those of `list` (n_list[0] of them)
+    const int64_t n_items = list != nullptr ? *n_list : n_clusters;
+    for (int64_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const int64_t g = list != nullptr ? list[it] : it;
         for (int i = tid; i < 256; i += kB) S.hist[i] = 0;
         // max-pool over the cluster's docs
         for (int64_t q = cdp[g] + wave; q < cdp[g + 1]; q += kB / 64) {
@@ -614,52 +773,10 @@ __global__ __launch_bounds__(kB) void summarize_u8_kernel(const int32_t* __restr
         atomicAdd(&S.misc[0], local_sum);
         __syncthreads();
         if (wave == 0) {
-            // pairs sorted by value desc; addup += value; stop at the first
-            // addup / sum >= alpha (inverted_list_clusters.cpp:43-50). Over the
-            // value histogram: lane l holds values 255 - 4l - u (u < 4); a DPP
-            // prefix gives each value the exact sum of all higher ones (a), and
-            // every non-empty value runs the sequential loop's test
-            // float(a + v * m) / sum >= alpha; the first value that passes (in
-            // descending order) is the cut, and its lane finds how many of its
-            // equal terms are kept.
-            const float sum = static_cast<float>(S.misc[0]);
-            int m4[4];
-            uint32_t part = 0;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                m4[u] = S.hist[255 - 4 * lane - u];
-                part += static_cast<uint32_t>((255 - 4 * lane - u) * m4[u]);
-            }
-            int64_t a = static_cast<int64_t>(wave_incl_scan_dpp(part) - part);
-            int hit = -1;
-            int64_t a_hit = 0;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int v = 255 - 4 * lane - u, m = m4[u];
-                if (hit < 0 && m != 0 && static_cast<float>(a + static_cast<int64_t>(v) * m) / sum >= alpha) {
-                    hit = u;
-                    a_hit = a;
-                }
-                a += static_cast<int64_t>(v) * m;
-            }
-            const uint64_t hm = __ballot(hit >= 0);
-            const int first = hm != 0 ? __ffsll(static_cast<unsigned long long>(hm)) - 1 : -1;
-            if (first < 0 && lane == 0) {
-                S.misc[1] = -1;  // keep values > -1: every touched term
-                S.misc[2] = 0;
-            }
-            if (lane == first) {
-                const int v = 255 - 4 * lane - hit, m = m4[hit];
-                int lo = 1, hi = m;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (static_cast<float>(a_hit + static_cast<int64_t>(v) * mid) / sum >= alpha)
-                        hi = mid;
-                    else
-                        lo = mid + 1;
-                }
-                S.misc[1] = v;
-                S.misc[2] = lo;
+            const int2 cut = alpha_cut(S.hist, S.misc[0], alpha, lane);
+            if (lane == 0) {
+                S.misc[1] = cut.x;
+                S.misc[2] = cut.y;
             }
         }
         __syncthreads();
@@ -1436,6 +1553,16 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(ldsf)),
                   "hipFuncSetAttribute");
     }
+    // uint8: small clusters one wavefront each, the others (listed by the count
+    // pass) one workgroup each
+    DeviceBuffer big, n_big;
+    const unsigned small_grid =
+        static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((C + kSumWaves - 1) / kSumWaves, 8192)));
+    if (sq) {
+        big.allocate(static_cast<size_t>(C) * 4);
+        n_big.allocate(4);
+        hip_check(hipMemset(n_big.get(), 0, 4), "hipMemset");
+    }
     int64_t summary_entries = 0;
     for (int pass = 0; pass < 2; ++pass) {
         DeviceBuffer sterm, sval, scl;
@@ -1450,17 +1577,29 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
             scl.allocate(E * 2);
         }
         if (sq) {
-            if (pass == 0)
+            const auto* vals8 = static_cast<const uint8_t*>(fwd.values);
+            if (pass == 0) {
+                hipLaunchKernelGGL((summarize_small_u8_kernel<false>), dim3(small_grid), dim3(kB), 0, nullptr,
+                                   fwd.indptr, fwd.indices, vals8, cdp.get<int64_t>(), docs.get<int32_t>(),
+                                   cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), C, alpha, scount.get<int64_t>(),
+                                   nullptr, nullptr, nullptr, nullptr, big.get<int32_t>(), n_big.get<int32_t>());
                 hipLaunchKernelGGL((summarize_u8_kernel<false>), dim3(sgrid), dim3(kB), lds8, nullptr, fwd.indptr,
-                                   fwd.indices, static_cast<const uint8_t*>(fwd.values), cdp.get<int64_t>(),
-                                   docs.get<int32_t>(), cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), C, nw, alpha,
-                                   scount.get<int64_t>(), nullptr, nullptr, nullptr, nullptr);
-            else
+                                   fwd.indices, vals8, cdp.get<int64_t>(), docs.get<int32_t>(),
+                                   cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), C, nw, alpha,
+                                   scount.get<int64_t>(), nullptr, nullptr, nullptr, nullptr, big.get<int32_t>(),
+                                   n_big.get<int32_t>());
+            } else {
+                hipLaunchKernelGGL((summarize_small_u8_kernel<true>), dim3(small_grid), dim3(kB), 0, nullptr,
+                                   fwd.indptr, fwd.indices, vals8, cdp.get<int64_t>(), docs.get<int32_t>(),
+                                   cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), C, alpha, scount.get<int64_t>(),
+                                   out_off.get<int64_t>(), sterm.get<uint16_t>(), sval.get<uint8_t>(),
+                                   scl.get<uint16_t>(), big.get<int32_t>(), n_big.get<int32_t>());
                 hipLaunchKernelGGL((summarize_u8_kernel<true>), dim3(sgrid), dim3(kB), lds8, nullptr, fwd.indptr,
-                                   fwd.indices, static_cast<const uint8_t*>(fwd.values), cdp.get<int64_t>(),
-                                   docs.get<int32_t>(), cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), C, nw, alpha,
+                                   fwd.indices, vals8, cdp.get<int64_t>(), docs.get<int32_t>(),
+                                   cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), C, nw, alpha,
                                    scount.get<int64_t>(), out_off.get<int64_t>(), sterm.get<uint16_t>(),
-                                   sval.get<uint8_t>(), scl.get<uint16_t>());
+                                   sval.get<uint8_t>(), scl.get<uint16_t>(), big.get<int32_t>(), n_big.get<int32_t>());
+            }
         } else {
             hipLaunchKernelGGL(summarize_f32_kernel, dim3(sgrid), dim3(kB), ldsf, nullptr, fwd.indptr, fwd.indices,
                                static_cast<const float*>(fwd.values), cdp.get<int64_t>(), docs.get<int32_t>(),
