@@ -1752,9 +1752,9 @@ void launch_brute_search(const int32_t* indptr, const uint16_t* indices, const f
 namespace {
 
 // ---- InvertedIndex: exact term-at-a-time scores in LDS windows of doc ids
-constexpr int kInvBlock = 512;
+constexpr int kInvBlock = 1024;
 constexpr int kInvWaves = kInvBlock / 64;
-constexpr int kInvWindow = 8192;      // doc ids per window: 32 KB of float scores
+constexpr int kInvWindow = 16384;     // doc ids per window: 64 KB of float scores
 constexpr int kInvChunks = 4;         // a round takes up to kInvChunks * kInvBlock postings of one term
 constexpr int kInvSlabTarget = 2048;  // workgroups a group of queries aims for
 
