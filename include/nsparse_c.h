@@ -43,9 +43,10 @@ const char* nsparse_last_error(void);
 /* index_factory(int dim, const char* desc) -- nsparse/index_factory.cpp:63-153.
  * Accepts the reference's strings unchanged: "seismic,lambda=..|beta=..|alpha=..|seed=..",
  * "seismic_sq,quantizer=8bit|vmin=..|vmax=..|lambda=..|beta=..|alpha=..|seed=..",
- * and the "idmap,<delegate>" prefix. Unknown keys are ignored, as there.
- * "brutal" and "inverted" are other index types and are rejected with
- * NSPARSE_ERR_RUNTIME ("not implemented on this device path"). */
+ * "brutal" (BrutalIndex, nsparse/brutal_index.cpp), "inverted" (InvertedIndex,
+ * nsparse/inverted_index.cpp) and the "idmap,<delegate>" prefix. Unknown keys
+ * are ignored, as there. "quantizer=16bit" is rejected with NSPARSE_ERR_RUNTIME
+ * (8-bit only on this device path). */
 int nsparse_index_factory(int dim, const char* description, nsparse_index** out);
 
 /* delete index (SWIG %newobject ownership, nsparse/python/swignsparse.swig:182-184). */
@@ -109,7 +110,9 @@ typedef struct nsparse_search_params {
 /* Index::search(n, indptr, indices, values, k, distances, labels, params) --
  * nsparse/index.cpp:32-49 (+ SeismicIndex::search seismic_index.cpp:152-220,
  * SeismicScalarQuantizedIndex::search seismic_scalar_quantized_index.cpp:201-305,
- * IDMapIndex::search id_map_index.cpp:29-60). params may be NULL. */
+ * BrutalIndex::search brutal_index.cpp:44-96, InvertedIndex::search
+ * inverted_index.cpp:254-293, IDMapIndex::search id_map_index.cpp:29-60).
+ * params may be NULL. Safe to call from several threads on one built index. */
 int nsparse_search(nsparse_index* index, nsparse_idx_t n, const nsparse_idx_t* indptr,
                    const nsparse_term_t* indices, const float* values, int k, float* distances,
                    nsparse_idx_t* labels, const nsparse_search_params* params);
