@@ -52,8 +52,9 @@ int nsparse_index_factory(int dim, const char* description, nsparse_index** out)
 /* delete index (SWIG %newobject ownership, nsparse/python/swignsparse.swig:182-184). */
 int nsparse_free(nsparse_index* index);
 
-/* Index::id() fourcc characters: SEIS / SESQ / IDMP (nsparse/seismic_index.h:36,
- * nsparse/seismic_scalar_quantized_index.h:37, nsparse/id_map_index.h:81). */
+/* Index::id() fourcc characters: SEIS / SESQ / IDMP / BRUT / INVT (nsparse/seismic_index.h:36,
+ * nsparse/seismic_scalar_quantized_index.h:37, nsparse/id_map_index.h:81,
+ * nsparse/brutal_index.h:32, nsparse/inverted_index.h:36). */
 int nsparse_index_id(const nsparse_index* index, char out_fourcc[4]);
 
 /* Index::get_dimension() -- nsparse/index.h:47. */
@@ -62,9 +63,11 @@ int nsparse_get_dimension(const nsparse_index* index, int* out);
 /* Index::num_vectors() -- nsparse/index.h:51-54. */
 int nsparse_num_vectors(const nsparse_index* index, int64_t* out);
 
-/* Index::add -- nsparse/seismic_index.cpp:127-142, seismic_scalar_quantized_index.cpp:159-177.
- * Assumes indptr[0] == 0 like the reference (sparse_vectors.cpp:92-126). Term ids
- * must be < dim (checked here; the reference does not check). */
+/* Index::add -- nsparse/seismic_index.cpp:127-142, seismic_scalar_quantized_index.cpp:159-177,
+ * brutal_index.cpp:27-42, inverted_index.cpp:218-234. Assumes indptr[0] == 0 like the
+ * reference (sparse_vectors.cpp:92-126). Term ids must be < dim: checked here (the
+ * reference does not check), except for "inverted", whose build() rejects them as the
+ * reference's does ("term_id out of range"). */
 int nsparse_add(nsparse_index* index, nsparse_idx_t n, const nsparse_idx_t* indptr,
                 const nsparse_term_t* indices, const float* values);
 
@@ -73,7 +76,9 @@ int nsparse_add(nsparse_index* index, nsparse_idx_t n, const nsparse_idx_t* indp
 int nsparse_add_with_ids(nsparse_index* index, nsparse_idx_t n, const nsparse_idx_t* indptr,
                          const nsparse_term_t* indices, const float* values, const nsparse_idx_t* ids);
 
-/* Index::build -- nsparse/seismic_common.h:146-184, run as HIP kernels. */
+/* Index::build -- nsparse/seismic_common.h:146-184 (SEISMIC) and
+ * inverted_index.cpp:236-252 (inverted), run as HIP kernels; "brutal" has none
+ * ("not implemented", as Index::build). */
 int nsparse_build(nsparse_index* index);
 
 /* ---- search parameters and selectors (nsparse/index.h:21-28, id_selector.h:22-86) ---- */
