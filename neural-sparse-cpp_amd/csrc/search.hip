@@ -1604,10 +1604,45 @@ __device__ void bitonic_desc(uint64_t* keys) {
 template <class KeyOf>
 __device__ void stream_topk(uint64_t* keys, int* cnt, int64_t n, int k, int kcap, KeyOf key_of) {
     const int cap = kSelectSlots - kcap;  // candidate slots
+    uint64_t thresh = 0;
+    if (k <= kBlock) {
+        // A first threshold: every thread's best key of the first round is a
+        // real candidate, so the k-th largest of those kBlock keys is at most
+        // the final k-th; starting the filter just below it spares the early
+        // merges of nearly every candidate.
+        uint64_t best = 0;
+#pragma unroll
+        for (int u = 0; u < kSelectRound / kBlock; ++u) {
+            const int64_t i = u * kBlock + threadIdx.x;
+            if (i < n) {
+                const uint64_t key = key_of(i);
+                best = key > best ? key : best;
+            }
+        }
+        keys[threadIdx.x] = best;
+        __syncthreads();
+        for (int size = 2; size <= kBlock; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                const int t = threadIdx.x;
+                if (t < kBlock / 2) {
+                    const int i = 2 * t - (t & (stride - 1)), j = i + stride;
+                    const bool desc = (i & size) == 0;
+                    const uint64_t a = keys[i], b = keys[j];
+                    if ((a < b) == desc) {
+                        keys[i] = b;
+                        keys[j] = a;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        const uint64_t kth = keys[k - 1];
+        thresh = kth > 0 ? kth - 1 : 0;
+        __syncthreads();
+    }
     for (int i = threadIdx.x; i < kSelectSlots; i += kBlock) keys[i] = 0;
     if (threadIdx.x == 0) *cnt = 0;
     __syncthreads();
-    uint64_t thresh = 0;
     auto merge = [&]() {
         const int used = kcap + *cnt;
         for (int i = used + threadIdx.x; i < kSelectSlots; i += kBlock) keys[i] = 0;
