@@ -620,7 +620,7 @@ __device__ int2 alpha_cut(const int32_t* hist, int sum_i, float alpha, int lane)
 // cut_x at it in term order -- counted (count pass) or written at out_off[g]
 // (write pass), exactly as summarize_u8_kernel does. Larger clusters are left
 // to it: the count pass appends them to `big`.
-constexpr int kSumSmallCap = 2048;
+constexpr int kSumSmallCap = 1024;
 constexpr int kSumWaves = kB / 64;
 
 template <bool WRITE>
