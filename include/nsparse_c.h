@@ -52,6 +52,11 @@ int nsparse_index_factory(int dim, const char* description, nsparse_index** out)
 /* delete index (SWIG %newobject ownership, nsparse/python/swignsparse.swig:182-184). */
 int nsparse_free(nsparse_index* index);
 
+/* IDMapIndex(Index*) -- nsparse/id_map_index.h:84, id_map_index.cpp:20: a new IDMap
+ * index owning the delegate's index. `delegate` stays a handle (free it as usual)
+ * that holds no index any more; calls on it report NSPARSE_ERR_INVALID_ARGUMENT. */
+int nsparse_idmap_wrap(nsparse_index* delegate, nsparse_index** out);
+
 /* Index::id() fourcc characters: SEIS / SESQ / IDMP / BRUT / INVT (nsparse/seismic_index.h:36,
  * nsparse/seismic_scalar_quantized_index.h:37, nsparse/id_map_index.h:81,
  * nsparse/brutal_index.h:32, nsparse/inverted_index.h:36). */

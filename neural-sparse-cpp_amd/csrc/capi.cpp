@@ -156,6 +156,16 @@ int nsparse_free(nsparse_index* index) {
     return guarded([&] { delete index; });
 }
 
+int nsparse_idmap_wrap(nsparse_index* delegate, nsparse_index** out) {
+    return guarded([&] {
+        if (out == nullptr) throw std::invalid_argument("unexpected nullptr");
+        unwrap(delegate);  // a live index
+        auto h = std::make_unique<nsparse_index>();
+        h->impl = std::make_unique<IDMapIndex>(std::move(delegate->impl));
+        *out = h.release();
+    });
+}
+
 int nsparse_index_id(const nsparse_index* index, char out_fourcc[4]) {
     return guarded([&] {
         if (out_fourcc == nullptr) throw std::invalid_argument("unexpected nullptr");

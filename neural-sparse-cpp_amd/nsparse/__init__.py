@@ -10,8 +10,15 @@ goes through the C ABI of libnsparse_amd.so; search and build run on the GPU.
 from .index import (  # noqa: F401
     ArrayIDSelector,
     BitmapIDSelector,
+    BrutalIndex,
+    IDMapIndex,
     Index,
+    InvertedIndex,
     NotIDSelector,
+    QuantizerType,
+    SeismicClusterParameters,
+    SeismicIndex,
+    SeismicScalarQuantizedIndex,
     SearchParameters,
     SearchPlan,
     Server,

@@ -104,6 +104,7 @@ def _sig(name, argtypes, restype=c_int):
 _sig("nsparse_last_error", [], c_char_p)
 _sig("nsparse_index_factory", [c_int, c_char_p, P(c_void_p)])
 _sig("nsparse_free", [c_void_p])
+_sig("nsparse_idmap_wrap", [c_void_p, P(c_void_p)])
 _sig("nsparse_index_id", [c_void_p, ctypes.c_char * 4])
 _sig("nsparse_get_dimension", [c_void_p, P(c_int)])
 _sig("nsparse_num_vectors", [c_void_p, P(c_int64)])

@@ -396,6 +396,71 @@ def index_factory(dim, description):
     return Index(h.value)
 
 
+# ---------------------------------------------------------------- constructors
+# The SWIG module also exposes the index classes' own constructors
+# (swignsparse.swig:149-176). Their defaults are the classes' (seismic_common.h:
+# 36, 50-58: lambda and beta -1 = derived from the corpus, alpha 0.4, random
+# seed; scalar_quantizer.h:30: 8-bit over [0, 1]), not the factory's; each
+# returns an Index built through the factory string that says the same.
+class QuantizerType:
+    """utils/scalar_quantizer.h:21-24."""
+
+    QT_8bit = 0
+    QT_16bit = 1
+
+
+class SeismicClusterParameters:
+    """SeismicClusterParameters (seismic_common.h:38-47); `lambda` is a Python
+    keyword, so the field is `lambda_`, as SWIG names it."""
+
+    def __init__(self, lambda_=-1, beta=-1, alpha=0.4, seed=-1, inverted_list_batch_size=1):
+        self.lambda_ = int(lambda_)
+        self.beta = int(beta)
+        self.alpha = float(alpha)
+        self.seed = int(seed)
+        self.inverted_list_batch_size = int(inverted_list_batch_size)
+
+    def _desc(self):
+        return f"lambda={self.lambda_}|beta={self.beta}|alpha={self.alpha:.9g}|seed={self.seed}"
+
+
+def SeismicIndex(dim, parameter=None):  # noqa: N802 -- the reference's class name
+    """SeismicIndex(dim) / SeismicIndex(dim, SeismicClusterParameters) (seismic_index.h:38-39)."""
+    p = parameter if parameter is not None else SeismicClusterParameters()
+    return index_factory(dim, "seismic," + p._desc())
+
+
+def SeismicScalarQuantizedIndex(*args):  # noqa: N802
+    """SeismicScalarQuantizedIndex(dim) or (quantizer_type, vmin, vmax, parameter, dim)
+    (seismic_scalar_quantized_index.h:38-41)."""
+    if len(args) == 1:
+        qt, vmin, vmax, p, dim = QuantizerType.QT_8bit, 0.0, 1.0, SeismicClusterParameters(), args[0]
+    elif len(args) == 5:
+        qt, vmin, vmax, p, dim = args
+    else:
+        raise TypeError("SeismicScalarQuantizedIndex(dim) or (quantizer_type, vmin, vmax, parameter, dim)")
+    q = "8bit" if qt == QuantizerType.QT_8bit else "16bit"
+    return index_factory(dim, f"seismic_sq,quantizer={q}|vmin={float(vmin):.9g}|vmax={float(vmax):.9g}|" + p._desc())
+
+
+def BrutalIndex(dim=0):  # noqa: N802
+    """BrutalIndex(dim = 0) (brutal_index.h:24)."""
+    return index_factory(dim, "brutal")
+
+
+def InvertedIndex(dim):  # noqa: N802
+    """InvertedIndex(dim) (inverted_index.h:27)."""
+    return index_factory(dim, "inverted")
+
+
+def IDMapIndex(index):  # noqa: N802
+    """IDMapIndex(Index*) (id_map_index.h:84): takes ownership of `index`, whose
+    handle holds no index afterwards."""
+    h = ctypes.c_void_p()
+    check(lib.nsparse_idmap_wrap(index.handle, ctypes.byref(h)))
+    return Index(h.value)
+
+
 def write_index(index, path):
     check(lib.nsparse_write_index(index.handle, str(path).encode()))
 

@@ -121,6 +121,31 @@ def test_inverted_lifecycle_and_layout(nsparse):
         nsparse.read_index_bytes(bytes(raw))
 
 
+def test_class_constructors(nsparse):
+    """The SWIG module's class constructors (swignsparse.swig:149-176) with the
+    classes' own defaults; IDMapIndex(index) takes ownership of its delegate."""
+    assert nsparse.SeismicIndex(100).id() == "SEIS"
+    p = nsparse.SeismicClusterParameters(10, 2, 0.4, 0)
+    assert p.lambda_ == 10 and p.beta == 2 and p.seed == 0
+    assert nsparse.SeismicIndex(100, p).get_dimension() == 100
+    assert nsparse.SeismicScalarQuantizedIndex(100).id() == "SESQ"
+    sq = nsparse.SeismicScalarQuantizedIndex(nsparse.QuantizerType.QT_8bit, 0.5, 3.0, p, 50)
+    assert sq.id() == "SESQ" and sq.get_dimension() == 50
+    with pytest.raises(RuntimeError, match="16-bit"):
+        nsparse.SeismicScalarQuantizedIndex(nsparse.QuantizerType.QT_16bit, 0.0, 1.0, p, 10)
+    with pytest.raises(TypeError):
+        nsparse.SeismicScalarQuantizedIndex(1, 2)
+    assert nsparse.BrutalIndex(5).id() == "BRUT" and nsparse.InvertedIndex(9).id() == "INVT"
+    inner = nsparse.InvertedIndex(9)
+    outer = nsparse.IDMapIndex(inner)
+    assert outer.id() == "IDMP" and outer.get_dimension() == 9
+    outer.add_with_ids(1, np.array([0, 1], np.int32), np.array([3], np.uint16), np.array([1.0], np.float32),
+                       np.array([77], np.int32))
+    assert outer.num_vectors() == 1
+    with pytest.raises(ValueError, match="handle is null"):
+        inner.num_vectors()
+
+
 def test_argument_checks(nsparse):
     ix = nsparse.index_factory(10, "seismic,lambda=10|beta=2")
     with pytest.raises(ValueError):

@@ -262,3 +262,23 @@ def test_slabs_of_several_windows(nsparse, corpus, built, major):
     d, lab = search(built, queries, 10)
     wl, wd = major.top_k(queries, 10)
     assert np.array_equal(lab, wl) and np.array_equal(d, wd)
+
+
+def test_class_constructors_build_and_search(nsparse, corpus, major):
+    """IDMapIndex(InvertedIndex(dim)) and SeismicIndex(dim) with the class
+    defaults (lambda and beta derived from the corpus) build and search."""
+    docs, queries = corpus
+    ext = np.arange(30000, dtype=np.int32) + 500
+    index = nsparse.IDMapIndex(nsparse.InvertedIndex(DIM))
+    index.add_with_ids(len(docs[0]) - 1, *docs, ext)
+    index.build()
+    d, lab = search(index, queries, 10)
+    wl, wd = major.top_k(queries, 10)
+    assert np.array_equal(lab, np.where(wl >= 0, ext[wl], -1)) and np.array_equal(d, wd)
+    s = nsparse.SeismicIndex(DIM, nsparse.SeismicClusterParameters(seed=0))
+    s.add(len(docs[0]) - 1, *docs)
+    s.build()
+    o = O.OracleIndex(DIM, 0, -1, -1, 0.4, 0, tie_mode=1)
+    o.add(*docs)
+    o.build()
+    assert nsparse.serialize_index(s) == o.serialize()
