@@ -16,6 +16,8 @@ from .index import (  # noqa: F401
     InvertedIndex,
     NotIDSelector,
     QuantizerType,
+    Residency_kInMemory,
+    Residency_kMmap,
     SeismicClusterParameters,
     SeismicIndex,
     SeismicScalarQuantizedIndex,

@@ -259,6 +259,14 @@ class Index:
         check(lib.nsparse_build_stats(self._h, v))
         return dict(zip(["kept_postings", "kept_row_nnz", "clusters", "summary_entries", "assign_pairs"], v))
 
+    # the SWIG extension methods the reference's wrappers build search() from
+    # (swignsparse.swig, %extend nsparse::Index)
+    def search_c(self, n, indptr, indices, values, k):
+        return self.search(n, indptr, indices, values, k)
+
+    def search_with_params(self, n, indptr, indices, values, k, params):
+        return self.search(n, indptr, indices, values, k, params)
+
     def read_csr(self, path, residency=0):
         check(lib.nsparse_read_csr(self._h, str(path).encode(), int(residency)))
 
@@ -402,6 +410,11 @@ def index_factory(dim, description):
 # 36, 50-58: lambda and beta -1 = derived from the corpus, alpha 0.4, random
 # seed; scalar_quantizer.h:30: 8-bit over [0, 1]), not the factory's; each
 # returns an Index built through the factory string that says the same.
+# Residency (types.h:27-30), as SWIG names a scoped enum's values
+Residency_kInMemory = 0
+Residency_kMmap = 1
+
+
 class QuantizerType:
     """utils/scalar_quantizer.h:21-24."""
 

@@ -144,6 +144,11 @@ def test_class_constructors(nsparse):
     assert outer.num_vectors() == 1
     with pytest.raises(ValueError, match="handle is null"):
         inner.num_vectors()
+    assert (nsparse.Residency_kInMemory, nsparse.Residency_kMmap) == (0, 1)
+    empty = nsparse.InvertedIndex(9)  # never built: search_c / search_with_params leave the padding
+    q = (1, np.array([0, 1], np.int32), np.array([3], np.uint16), np.array([1.0], np.float32), 2)
+    assert (empty.search_c(*q)[1] == -1).all()
+    assert (empty.search_with_params(*q, nsparse.SearchParameters())[1] == -1).all()
 
 
 def test_argument_checks(nsparse):
