@@ -151,6 +151,22 @@ def test_class_constructors(nsparse):
     assert (empty.search_with_params(*q, nsparse.SearchParameters())[1] == -1).all()
 
 
+def test_truncated_inverted_files_are_rejected(nsparse):
+    """Every proper prefix of an INVT file is rejected with RuntimeError; the
+    whole file reads back (io errors are runtime errors in the reference too)."""
+    from _oracle import OracleInverted
+
+    o = OracleInverted(6)
+    o.add(np.array([0, 2, 3, 5], np.int32), np.array([1, 4, 0, 1, 5], np.uint16),
+          np.array([0.5, 1.0, 2.0, 0.25, 3.0], np.float32))
+    o.build()
+    raw = o.serialize()
+    for cut in range(len(raw)):
+        with pytest.raises(RuntimeError):
+            nsparse.read_index_bytes(raw[:cut])
+    assert nsparse.serialize_index(nsparse.read_index_bytes(raw)) == raw
+
+
 def test_argument_checks(nsparse):
     ix = nsparse.index_factory(10, "seismic,lambda=10|beta=2")
     with pytest.raises(ValueError):
