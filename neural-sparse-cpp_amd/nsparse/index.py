@@ -165,8 +165,22 @@ class Index:
         return (np.ascontiguousarray(indptr, dtype=np.int32), np.ascontiguousarray(indices, dtype=np.uint16),
                 np.ascontiguousarray(values, dtype=np.float32))
 
+    @staticmethod
+    def _typed(arr, dtype, name):
+        # the SWIG typemaps of Index.add take only arrays of the exact type
+        # (nsparse_typemaps.i): the same TypeError here, for every argument
+        if not isinstance(arr, np.ndarray) or arr.dtype != dtype:
+            raise TypeError(f"Expected {np.dtype(dtype).name} array for {name}")
+        return np.ascontiguousarray(arr)
+
     def add(self, n, indptr, indices, values):
-        ip, ix, v = self._csr(indptr, indices, values)
+        """Index.add: like the SWIG method (unlike search and add_with_ids,
+        which convert), it takes int32 / uint16 / float32 arrays only."""
+        if isinstance(n, bool) or not isinstance(n, (int, np.integer)):
+            raise TypeError("Expected an integer for idx_t parameter")
+        ip = self._typed(indptr, np.int32, "indptr")
+        ix = self._typed(indices, np.uint16, "indices")
+        v = self._typed(values, np.float32, "values")
         check(lib.nsparse_add(self._h, int(n), _ptr(ip), _ptr(ix), _ptr(v)))
 
     def add_with_ids(self, n, indptr, indices, values, ids):

@@ -169,10 +169,22 @@ def test_truncated_inverted_files_are_rejected(nsparse):
 
 def test_argument_checks(nsparse):
     ix = nsparse.index_factory(10, "seismic,lambda=10|beta=2")
+    i32, u16, f32 = (lambda *a: np.array(a, np.int32)), (lambda *a: np.array(a, np.uint16)), \
+        (lambda *a: np.array(a, np.float32))
     with pytest.raises(ValueError):
-        ix.add(0, [0], [], [])
+        ix.add(0, i32(0), u16(), f32())
     with pytest.raises(ValueError):
-        ix.add(1, [0, 1], [10], [1.0])  # term id >= dim
+        ix.add(1, i32(0, 1), u16(10), f32(1.0))  # term id >= dim
+    # add takes exactly-typed arrays, as the SWIG typemaps do (nsparse_typemaps.i);
+    # the reference fails cleanly only for values and crashes for the other two
+    with pytest.raises(TypeError, match="float32 array for values"):
+        ix.add(1, i32(0, 1), u16(1), np.array([1.0]))
+    with pytest.raises(TypeError, match="int32 array for indptr"):
+        ix.add(1, np.array([0, 1]), u16(1), f32(1.0))
+    with pytest.raises(TypeError, match="uint16 array for indices"):
+        ix.add(1, i32(0, 1), [1], f32(1.0))
+    with pytest.raises(TypeError, match="integer"):
+        ix.add(1.0, i32(0, 1), u16(1), f32(1.0))
     with pytest.raises(RuntimeError):
         ix.add_with_ids(1, [0, 1], [1], [1.0], [5])
     with pytest.raises(ValueError):
