@@ -450,6 +450,23 @@ def main():
     if n_plain:
         log(f"plain: {plain_qps:,.0f} QPS, p50 {pct(0.5):.3f} ms, p99 {pct(0.99):.3f} ms")
 
+    # ---- the same queries through Index::search over host arrays (PCIe both ways)
+    lat_h = []
+    for i in range(n_plain):
+        k = int(ks[i])
+        a, b = int(queries[0][i]), int(queries[0][i + 1])
+        hip_ = np.array([0, b - a], dtype=np.int32)
+        t1 = time.perf_counter()
+        _, hl = index.search(1, hip_, queries[1][a:b], queries[2][a:b], k, params)
+        lat_h.append(time.perf_counter() - t1)
+        if i < 20:
+            assert (hl[0] == labels[i, :k]).all(), "host-array and batched results differ"
+    lat_h = np.sort(np.array(lat_h))
+    host_qps = n_plain / float(lat_h.sum()) if n_plain else float("nan")
+    pct_h = lambda f: float(lat_h[min(len(lat_h) - 1, int(math.floor(f * len(lat_h))))] * 1e3) if n_plain else None  # noqa
+    if n_plain:
+        log(f"plain over host arrays: {host_qps:,.0f} QPS, p50 {pct_h(0.5):.3f} ms, p99 {pct_h(0.99):.3f} ms")
+
     # ---- recall@10 against exact FP32 scores on the unquantized corpus (GPU)
     import ctypes
 
@@ -610,6 +627,8 @@ def main():
         "cpu_baseline": cpu,
         "plain": {"qps": plain_qps if n_plain else None, "p50_ms": pct(0.5), "p99_ms": pct(0.99),
                   "queries": n_plain},
+        "plain_host_arrays": {"qps": host_qps if n_plain else None, "p50_ms": pct_h(0.5), "p99_ms": pct_h(0.99),
+                              "queries": n_plain},
         "batched_over_plain": qps / plain_qps if n_plain else None,
         "batched_over_cpu": (qps / cpu["value"]) if cpu else None,
         "recall_at_10": recall if n_rec else None, "recall_queries": n_rec,

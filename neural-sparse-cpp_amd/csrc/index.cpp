@@ -440,33 +440,39 @@ void SeismicIndex::search_host(idx_t n, const idx_t* indptr, const term_t* indic
         if (indptr[i + 1] - indptr[i] > kDeviceQueryMax)
             throw std::invalid_argument("query has more non-zeros than the device path supports (" +
                                         std::to_string(kDeviceQueryMax) + ")");
-    // Queries: rebase so the device copy starts at 0.
+    // One upload of the queries (rebased so the device copy starts at 0) and
+    // the selector bitmap, one download of the results and the error flag.
     const int64_t q0 = indptr[0], qnnz = indptr[n] - q0;
-    std::vector<int32_t> qip(n + 1);
+    const size_t nk = static_cast<size_t>(n) * k;
+    auto st = stages_.take();
+    st->begin();
+    const int s_qip = st->add(HostStage::kIn, static_cast<size_t>(n + 1) * 4);
+    const int s_qix = st->add(HostStage::kIn, static_cast<size_t>(qnnz) * 2);
+    const int s_qv = st->add(HostStage::kIn, static_cast<size_t>(qnnz) * 4);
+    const int64_t nd = fwd_.n();
+    const int s_sel = sel.active() ? st->add(HostStage::kIn, static_cast<size_t>((nd + 31) / 32) * 4) : -1;
+    const int s_out_d = st->add(HostStage::kOut, nk * 4);
+    const int s_out_l = st->add(HostStage::kOut, nk * 4);
+    const int s_err = st->add(HostStage::kOut, 4);
+    const int s_stats =
+        stats_out != nullptr ? st->add(HostStage::kOut, static_cast<size_t>(n) * kStatFields * 8) : -1;
+    st->commit();
+    int32_t* qip = st->host<int32_t>(s_qip);
     for (idx_t i = 0; i <= n; ++i) qip[i] = static_cast<int32_t>(indptr[i] - q0);
-    DeviceBuffer d_qip, d_qix, d_qv, d_out_d, d_out_l, d_err, d_sel, d_stats;
-    d_qip.upload(qip);
-    d_qix.upload(indices + q0, qnnz * 2);
-    d_qv.upload(values + q0, qnnz * 4);
-    d_out_d.allocate(static_cast<size_t>(n) * k * 4);
-    d_out_l.allocate(static_cast<size_t>(n) * k * 4);
-    d_err.allocate(4);
-    hip_check(hipMemset(d_err.get(), 0, 4), "hipMemset");
-    const uint32_t* sel_all = nullptr;
-    if (sel.active()) {
-        const int64_t nd = fwd_.n();
-        std::vector<uint32_t> bm((nd + 31) / 32, 0);
+    st->put(s_qix, indices + q0, static_cast<size_t>(qnnz) * 2);
+    st->put(s_qv, values + q0, static_cast<size_t>(qnnz) * 4);
+    if (s_sel >= 0) {
+        uint32_t* bm = st->host<uint32_t>(s_sel);
+        std::fill(bm, bm + (nd + 31) / 32, 0U);
         for (int64_t d = 0; d < nd; ++d)
             if (sel.is_member(d)) bm[d >> 5] |= 1U << (d & 31);
-        d_sel.upload(bm);
-        sel_all = d_sel.get<uint32_t>();
     }
-    if (stats_out != nullptr) d_stats.allocate(static_cast<size_t>(n) * kStatFields * 8);
+    st->upload();
     SearchArgs a{};
     a.ix = view_;
-    a.q_indptr = d_qip.get<int32_t>();
-    a.q_indices = d_qix.get<uint16_t>();
-    a.q_values = d_qv.get<float>();
+    a.q_indptr = st->dev<int32_t>(s_qip);
+    a.q_indices = st->dev<uint16_t>(s_qix);
+    a.q_values = st->dev<float>(s_qv);
     a.ks = nullptr;
     a.k_max = k;
     a.cut = req.cut;
@@ -484,21 +490,20 @@ void SeismicIndex::search_host(idx_t n, const idx_t* indptr, const term_t* indic
     }
     a.sel_of_query = nullptr;
     a.sel_bitmaps = nullptr;
-    a.sel_all = sel_all;
-    a.out_d = d_out_d.get<float>();
-    a.out_l = d_out_l.get<int32_t>();
-    a.stats = stats_out != nullptr ? d_stats.get<int64_t>() : nullptr;
-    a.error_flag = d_err.get<int32_t>();
+    a.sel_all = s_sel >= 0 ? st->dev<uint32_t>(s_sel) : nullptr;
+    a.out_d = st->dev<float>(s_out_d);
+    a.out_l = st->dev<int32_t>(s_out_l);
+    a.stats = s_stats >= 0 ? st->dev<int64_t>(s_stats) : nullptr;
+    a.error_flag = st->dev<int32_t>(s_err);
     a.n_queries = n;
     a.query_base = 0;
-    launch_search_checked(a, max_clusters_, k, sq_, nullptr);
-    hip_check(hipDeviceSynchronize(), "search kernel");
-    int32_t err = 0;
-    d_err.download(&err, 4);
+    launch_search_checked(a, max_clusters_, k, sq_, st->stream());
+    st->download_and_wait();
+    const int32_t err = *st->host<int32_t>(s_err);
     if (err != 0) throw std::invalid_argument(device_error_message(err));
-    d_out_d.download(distances, static_cast<size_t>(n) * k * 4);
-    d_out_l.download(labels, static_cast<size_t>(n) * k * 4);
-    if (stats_out != nullptr) d_stats.download(stats_out, static_cast<size_t>(n) * kStatFields * 8);
+    std::memcpy(distances, st->host(s_out_d), nk * 4);
+    std::memcpy(labels, st->host(s_out_l), nk * 4);
+    if (stats_out != nullptr) std::memcpy(stats_out, st->host(s_stats), static_cast<size_t>(n) * kStatFields * 8);
 }
 
 void SeismicIndex::search_device(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices,
@@ -655,19 +660,26 @@ void SeismicIndex::exact_search(idx_t n, const idx_t* indptr, const term_t* indi
         if (d < 0 || d >= fwd_.n()) throw std::invalid_argument("selector id out of range");
     require_device();
     ensure_device();
-    const int64_t q0 = indptr[0], qnnz = indptr[n] - q0;
-    std::vector<int32_t> qip(n + 1);
-    for (idx_t i = 0; i <= n; ++i) qip[i] = static_cast<int32_t>(indptr[i] - q0);
     for (idx_t i = 0; i < n; ++i)
-        if (qip[i + 1] - qip[i] > kDeviceQueryMax)
+        if (indptr[i + 1] - indptr[i] > kDeviceQueryMax)
             throw std::invalid_argument("query has more non-zeros than the device path supports");
-    DeviceBuffer d_qip, d_qix, d_qv, d_ids, d_out_d, d_out_l;
-    d_qip.upload(qip);
-    d_qix.upload(indices + q0, qnnz * 2);
-    d_qv.upload(values + q0, qnnz * 4);
-    d_ids.upload(ids);
-    d_out_d.allocate(static_cast<size_t>(n) * k * 4);
-    d_out_l.allocate(static_cast<size_t>(n) * k * 4);
+    const int64_t q0 = indptr[0], qnnz = indptr[n] - q0;
+    const size_t nk = static_cast<size_t>(n) * k;
+    auto st = stages_.take();
+    st->begin();
+    const int s_qip = st->add(HostStage::kIn, static_cast<size_t>(n + 1) * 4);
+    const int s_qix = st->add(HostStage::kIn, static_cast<size_t>(qnnz) * 2);
+    const int s_qv = st->add(HostStage::kIn, static_cast<size_t>(qnnz) * 4);
+    const int s_ids = st->add(HostStage::kIn, ids.size() * 4);
+    const int s_out_d = st->add(HostStage::kOut, nk * 4);
+    const int s_out_l = st->add(HostStage::kOut, nk * 4);
+    st->commit();
+    int32_t* qip = st->host<int32_t>(s_qip);
+    for (idx_t i = 0; i <= n; ++i) qip[i] = static_cast<int32_t>(indptr[i] - q0);
+    st->put(s_qix, indices + q0, static_cast<size_t>(qnnz) * 2);
+    st->put(s_qv, values + q0, static_cast<size_t>(qnnz) * 4);
+    st->put(s_ids, ids.data(), ids.size() * 4);
+    st->upload();
     const float qmin = req.has_sq_range ? req.vmin : vmin_;
     const float qmax = req.has_sq_range ? req.vmax : vmax_;
     const float ingest_range = vmax_ - vmin_;
@@ -675,22 +687,22 @@ void SeismicIndex::exact_search(idx_t n, const idx_t* indptr, const term_t* indi
     const float max_q = 255.0F;
     ExactArgs a{};
     a.ix = view_;
-    a.q_indptr = d_qip.get<int32_t>();
-    a.q_indices = d_qix.get<uint16_t>();
-    a.q_values = d_qv.get<float>();
-    a.ids = d_ids.get<int32_t>();
+    a.q_indptr = st->dev<int32_t>(s_qip);
+    a.q_indices = st->dev<uint16_t>(s_qix);
+    a.q_values = st->dev<float>(s_qv);
+    a.ids = st->dev<int32_t>(s_ids);
     a.n_ids = static_cast<int32_t>(ids.size());
     a.k = k;
     a.q_vmin = qmin;
     a.q_scale = 255.0F / (qmax - qmin);
     a.decode_scale = (ingest_range * query_range) / (max_q * max_q);
-    a.out_d = d_out_d.get<float>();
-    a.out_l = d_out_l.get<int32_t>();
+    a.out_d = st->dev<float>(s_out_d);
+    a.out_l = st->dev<int32_t>(s_out_l);
     a.n_queries = n;
-    launch_exact_checked(a, sq_, nullptr);
-    hip_check(hipDeviceSynchronize(), "exact-match kernel");
-    d_out_d.download(distances, static_cast<size_t>(n) * k * 4);
-    d_out_l.download(labels, static_cast<size_t>(n) * k * 4);
+    launch_exact_checked(a, sq_, st->stream());
+    st->download_and_wait();
+    std::memcpy(distances, st->host(s_out_d), nk * 4);
+    std::memcpy(labels, st->host(s_out_l), nk * 4);
 }
 
 // ---------------------------------------------------------------- layout IO
@@ -799,6 +811,39 @@ void SeismicIndex::read_payload(ByteReader& r) {
     params_ = Params{-1, -1, 0.4F, -1};
 }
 
+// ========================================================= staged host queries
+namespace {
+// Segments of a brute-force or inverted search over host arrays.
+struct StagedQueries {
+    int qip, qix, qv, extra, out_d, out_l, scratch;
+};
+
+// Lays a batch of host queries (rebased so the device copy starts at 0), an
+// optional extra input (an allow bitmap), the two outputs and a kernel's
+// scratch out in a stage, and fills the input mirror.
+StagedQueries stage_queries(HostStage& st, idx_t n, const idx_t* indptr, const term_t* indices, const float* values,
+                            int k, size_t scratch_bytes, const void* extra = nullptr, size_t extra_bytes = 0) {
+    const int64_t q0 = indptr[0], qnnz = indptr[n] - q0;
+    const size_t nk = static_cast<size_t>(n) * k;
+    StagedQueries q{};
+    st.begin();
+    q.qip = st.add(HostStage::kIn, static_cast<size_t>(n + 1) * 4);
+    q.qix = st.add(HostStage::kIn, static_cast<size_t>(qnnz) * 2);
+    q.qv = st.add(HostStage::kIn, static_cast<size_t>(qnnz) * 4);
+    q.extra = extra != nullptr ? st.add(HostStage::kIn, extra_bytes) : -1;
+    q.out_d = st.add(HostStage::kOut, nk * 4);
+    q.out_l = st.add(HostStage::kOut, nk * 4);
+    q.scratch = st.add(HostStage::kScratch, scratch_bytes);
+    st.commit();
+    int32_t* qip = st.host<int32_t>(q.qip);
+    for (idx_t i = 0; i <= n; ++i) qip[i] = static_cast<int32_t>(indptr[i] - q0);
+    st.put(q.qix, indices + q0, static_cast<size_t>(qnnz) * 2);
+    st.put(q.qv, values + q0, static_cast<size_t>(qnnz) * 4);
+    if (extra != nullptr) st.put(q.extra, extra, extra_bytes);
+    return q;
+}
+}  // namespace
+
 // ============================================================== BrutalIndex
 
 BrutalIndex::BrutalIndex(int dim) : Index(dim) {
@@ -853,22 +898,15 @@ void BrutalIndex::search(idx_t n, const idx_t* indptr, const term_t* indices, co
             dev_valid_ = true;
         }
     }
-    const int64_t q0 = indptr[0], qnnz = indptr[n] - q0;
-    std::vector<int32_t> qip(n + 1);
-    for (idx_t i = 0; i <= n; ++i) qip[i] = static_cast<int32_t>(indptr[i] - q0);
-    DeviceBuffer d_qip, d_qix, d_qv, d_out_d, d_out_l, d_scores;
-    d_qip.upload(qip);
-    d_qix.upload(indices + q0, qnnz * 2);
-    d_qv.upload(values + q0, qnnz * 4);
-    d_out_d.allocate(static_cast<size_t>(n) * k * 4);
-    d_out_l.allocate(static_cast<size_t>(n) * k * 4);
-    d_scores.allocate(brute_search_scratch_bytes(fwd_.n(), n));
+    auto st = stages_.take();
+    const StagedQueries q = stage_queries(*st, n, indptr, indices, values, k, brute_search_scratch_bytes(fwd_.n(), n));
+    st->upload();
     launch_brute_search(d_indptr_.get<int32_t>(), d_indices_.get<uint16_t>(), d_values_.get<float>(), fwd_.n(), dim_,
-                        d_qip.get<int32_t>(), d_qix.get<uint16_t>(), d_qv.get<float>(), n, k, d_out_d.get<float>(),
-                        d_out_l.get<int32_t>(), d_scores.get<float>(), nullptr);
-    hip_check(hipDeviceSynchronize(), "brute-force search");
-    d_out_d.download(distances, static_cast<size_t>(n) * k * 4);
-    d_out_l.download(labels, static_cast<size_t>(n) * k * 4);
+                        st->dev<int32_t>(q.qip), st->dev<uint16_t>(q.qix), st->dev<float>(q.qv), n, k,
+                        st->dev<float>(q.out_d), st->dev<int32_t>(q.out_l), st->dev<float>(q.scratch), st->stream());
+    st->download_and_wait();
+    std::memcpy(distances, st->host(q.out_d), static_cast<size_t>(n) * k * 4);
+    std::memcpy(labels, st->host(q.out_l), static_cast<size_t>(n) * k * 4);
 }
 
 // write_index / read_index (io/index_io.cpp:120-141): BrutalIndex is not an
@@ -1003,23 +1041,17 @@ void InvertedIndex::search_allowed(idx_t n, const idx_t* indptr, const term_t* i
         return;
     }
     ensure_device();
-    const int64_t q0 = indptr[0], qnnz = indptr[n] - q0;
-    std::vector<int32_t> qip(n + 1);
-    for (idx_t i = 0; i <= n; ++i) qip[i] = static_cast<int32_t>(indptr[i] - q0);
-    DeviceBuffer d_qip, d_qix, d_qv, d_out_d, d_out_l, d_scores, d_allow;
-    d_qip.upload(qip);
-    d_qix.upload(indices + q0, qnnz * 2);
-    d_qv.upload(values + q0, qnnz * 4);
-    if (allow != nullptr) d_allow.upload(*allow);
-    d_out_d.allocate(static_cast<size_t>(n) * k * 4);
-    d_out_l.allocate(static_cast<size_t>(n) * k * 4);
-    d_scores.allocate(brute_search_scratch_bytes(list_docs_, n));
-    launch_inverted_search(dev_, list_docs_, d_qip.get<int32_t>(), d_qix.get<uint16_t>(), d_qv.get<float>(), n, k,
-                           allow != nullptr ? d_allow.get<uint32_t>() : nullptr, d_out_d.get<float>(),
-                           d_out_l.get<int32_t>(), d_scores.get<float>(), nullptr);
-    hip_check(hipDeviceSynchronize(), "inverted search");
-    d_out_d.download(distances, static_cast<size_t>(n) * k * 4);
-    d_out_l.download(labels, static_cast<size_t>(n) * k * 4);
+    auto st = stages_.take();
+    const StagedQueries q = stage_queries(*st, n, indptr, indices, values, k, brute_search_scratch_bytes(list_docs_, n),
+                                          allow != nullptr ? allow->data() : nullptr,
+                                          allow != nullptr ? allow->size() * 4 : 0);
+    st->upload();
+    launch_inverted_search(dev_, list_docs_, st->dev<int32_t>(q.qip), st->dev<uint16_t>(q.qix), st->dev<float>(q.qv),
+                           n, k, allow != nullptr ? st->dev<uint32_t>(q.extra) : nullptr, st->dev<float>(q.out_d),
+                           st->dev<int32_t>(q.out_l), st->dev<float>(q.scratch), st->stream());
+    st->download_and_wait();
+    std::memcpy(distances, st->host(q.out_d), static_cast<size_t>(n) * k * 4);
+    std::memcpy(labels, st->host(q.out_l), static_cast<size_t>(n) * k * 4);
 }
 
 // InvertedIndex::write_index (inverted_index.cpp:367-402): document count,

@@ -13,6 +13,7 @@
 
 #include "device_index.h"
 #include "device_mem.h"
+#include "host_stage.h"
 #include "kernels.h"
 #include "nsparse_c.h"
 
@@ -152,6 +153,8 @@ private:
     // Guards the lazy upload of host state, so concurrent searches on one
     // built index (which the reference allows) find it resident exactly once.
     std::mutex dev_mu_;
+    // Staging for searches over host arrays, one stage per concurrent caller.
+    HostStagePool stages_;
 };
 
 // BrutalIndex (brutal_index.h:22-44, brutal_index.cpp): FP32 vectors kept as
@@ -175,6 +178,7 @@ private:
     bool dev_valid_ = false;
     DeviceBuffer d_indptr_, d_indices_, d_values_;
     std::mutex dev_mu_;  // the lazy upload, as in SeismicIndex
+    HostStagePool stages_;
 };
 
 // InvertedIndex (inverted_index.h:25-61, inverted_index.cpp): FP32 posting
@@ -220,6 +224,7 @@ private:
     mutable std::vector<float> h_vals_, h_max_;  // h_max_: max_term_scores_
     DevicePostings dev_;  // max_score is filled by a build only
     mutable std::mutex mu_;  // the lazy copies between host and device
+    HostStagePool stages_;
 };
 
 // A batched search over fixed device buffers, replayed from a hipGraph: the

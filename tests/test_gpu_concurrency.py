@@ -56,3 +56,46 @@ def test_concurrent_searches_equal_sequential(nsparse, spec):
     fresh = nsparse.read_index_bytes(nsparse.serialize_index(idx))
     for d, lab in _concurrent(fresh, queries, 10, params):
         assert np.array_equal(lab, want_l) and np.array_equal(d, want_d)
+
+
+@pytest.mark.parametrize("spec", ["seismic_sq", "inverted", "brutal"])
+def test_reused_staging_equals_fresh_index(nsparse, spec):
+    """Searches over host arrays reuse one index's staging (a stream, a device
+    arena, a pinned mirror): calls that grow, shrink and switch the selector on
+    one index must equal the same calls on indexes searched for the first time."""
+    from nsparse import synth
+
+    print(f"staging reuse on {spec}", flush=True)
+    docs = synth.make_corpus(synth.doc_spec(seed=51), 12000)
+    qip, qix, qv = synth.make_corpus(synth.query_spec(seed=52), 48)
+    vmin, vmax = float(docs[2].min()), float(docs[2].max())
+    desc = {"seismic_sq": f"seismic_sq,quantizer=8bit|vmin={vmin:.9g}|vmax={vmax:.9g}|lambda=200|beta=20|alpha=0.4|seed=0"
+            }.get(spec, spec)
+
+    def fresh():
+        idx = nsparse.index_factory(DIM, desc)
+        idx.add(len(docs[0]) - 1, *docs)
+        if spec != "brutal":  # BrutalIndex has no build step (Index::build throws)
+            idx.build()
+        return idx
+
+    def params(sel=None):
+        p = nsparse.SeismicSearchParameters(10, 0.8) if spec == "seismic_sq" else (
+            nsparse.SearchParameters() if sel is not None else None)
+        if sel is not None:
+            p.set_id_selector(sel)
+        return p
+
+    sub = lambda a, b: (qip[a:b + 1] - qip[a], qix[qip[a]:qip[b]], qv[qip[a]:qip[b]])  # noqa: E731
+    few = np.array([3, 17, 4000, 9001, 11999], dtype=np.int32)
+    calls = [(sub(0, 48), 50, None), (sub(5, 6), 10, None), (sub(10, 14), 10, "few"), (sub(0, 48), 20, "many"),
+             (sub(7, 8), 1, None)]
+    shared = fresh()
+    for q, k, sel_kind in calls:
+        if spec == "brutal" and sel_kind is not None:
+            continue
+        sel = {"few": lambda: nsparse.SetIDSelector(few),
+               "many": lambda: nsparse.SetIDSelector(np.arange(0, 12000, 3, dtype=np.int32))}.get(sel_kind)
+        got = shared.search(len(q[0]) - 1, *q, k, params(sel() if sel else None))
+        want = fresh().search(len(q[0]) - 1, *q, k, params(sel() if sel else None))
+        assert np.array_equal(got[1], want[1]) and np.array_equal(got[0], want[0]), (spec, k, sel_kind)
