@@ -99,3 +99,50 @@ def test_reused_staging_equals_fresh_index(nsparse, spec):
         got = shared.search(len(q[0]) - 1, *q, k, params(sel() if sel else None))
         want = fresh().search(len(q[0]) - 1, *q, k, params(sel() if sel else None))
         assert np.array_equal(got[1], want[1]) and np.array_equal(got[0], want[0]), (spec, k, sel_kind)
+
+
+@pytest.mark.parametrize("spec", ["seismic_sq", "inverted"])
+def test_concurrent_mixed_sizes_equal_sequential(nsparse, spec):
+    """Threads searching one index at once with batches of different sizes and
+    k: each takes a stage from the index's pool (growing it as needed), and
+    every result equals the same call made alone beforehand."""
+    from nsparse import synth
+
+    print(f"concurrent mixed-size searches on {spec}", flush=True)
+    docs = synth.make_corpus(synth.doc_spec(seed=61), 15000)
+    qip, qix, qv = synth.make_corpus(synth.query_spec(seed=62), 40)
+    vmin, vmax = float(docs[2].min()), float(docs[2].max())
+    desc = {"seismic_sq": f"seismic_sq,quantizer=8bit|vmin={vmin:.9g}|vmax={vmax:.9g}|lambda=200|beta=20|alpha=0.4|seed=0"
+            }.get(spec, spec)
+    idx = nsparse.index_factory(DIM, desc)
+    idx.add(len(docs[0]) - 1, *docs)
+    idx.build()
+    params = nsparse.SeismicSearchParameters(10, 0.8) if spec == "seismic_sq" else None
+    sub = lambda a, b: (qip[a:b + 1] - qip[a], qix[qip[a]:qip[b]], qv[qip[a]:qip[b]])  # noqa: E731
+    calls = [(sub(0, 40), 30), (sub(3, 4), 10), (sub(8, 20), 100), (sub(21, 22), 1), (sub(0, 10), 10),
+             (sub(30, 40), 50)]
+    want = [idx.search(len(q[0]) - 1, *q, k, params) for q, k in calls]
+    n_threads = 6
+    got = [[None] * len(calls) for _ in range(n_threads)]
+    errors = []
+    start = threading.Barrier(n_threads)
+
+    def run(t):
+        try:
+            start.wait()
+            for i in range(len(calls)):
+                c = (i + t) % len(calls)  # each thread walks the calls from its own offset
+                q, k = calls[c]
+                got[t][c] = idx.search(len(q[0]) - 1, *q, k, params)
+        except Exception as e:  # surfaced below
+            errors.append(e)
+
+    ts = [threading.Thread(target=run, args=(t,)) for t in range(n_threads)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors
+    for t in range(n_threads):
+        for c, (d, lab) in enumerate(got[t]):
+            assert np.array_equal(lab, want[c][1]) and np.array_equal(d, want[c][0]), (t, c)
