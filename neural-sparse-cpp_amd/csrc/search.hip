@@ -1238,6 +1238,11 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
         // i - 2) are free.
         if (wave == 0) {
             bool have_prev = false, stop = false;
+            // wave 0's control work (replay, plan, gather, scan) is the chain
+            // every other wave waits on, so wave 0 issues first on its SIMD
+            // (+0.9% batched throughput at 8.8M docs; dropping back to the
+            // scoring waves' priority while it scores measured 0.1% lower)
+            __builtin_amdgcn_s_setprio(2);
             for (int i = 0, cb = 0, cm = 0;; ++i, cb ^= 1, cm = cm == 2 ? 0 : cm + 1) {
                 const Chunk cur = chunk_at(cb, cm);
                 const Chunk prev = chunk_at(cb ^ 1, cm == 0 ? 2 : cm - 1);
@@ -1277,6 +1282,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                 tick(5);
                 have_prev = true;
             }
+            __builtin_amdgcn_s_setprio(0);
         } else {
             for (int i = 0, cb = 0, cm = 0;; ++i, cb ^= 1, cm = cm == 2 ? 0 : cm + 1) {
                 const Chunk cur = chunk_at(cb, cm);
