@@ -12,11 +12,34 @@ namespace nsparse_amd {
 
 namespace {
 
-template <class T>
-T* pinned(size_t count) {
+char* pinned(size_t bytes) {
     void* p = nullptr;
-    hip_check(hipHostMalloc(&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault), "hipHostMalloc");
-    return static_cast<T*>(p);
+    hip_check(hipHostMalloc(&p, std::max<size_t>(bytes, 1), hipHostMallocDefault), "hipHostMalloc");
+    return static_cast<char*>(p);
+}
+
+// Arrays inside a slot's blocks start on 256 bytes.
+size_t up256(size_t b) { return (b + 255) & ~size_t{255}; }
+
+// Zero bytes after the term ids and after every other input array (vector
+// loads may run past an array's end), and the offset of the first output array.
+constexpr size_t kIxPad = 64, kPad = 16, kOutHead = 256;
+
+// Input block bytes for n queries with nnz non-zeros in all; the byte offsets
+// of the k, selector, term-id and value arrays go to ks, sel, ix and v when
+// they are given (the query offsets start the block).
+size_t input_layout(int64_t n, int64_t nnz, size_t* ks, size_t* sel, size_t* ix, size_t* v) {
+    const size_t o_ks = up256((n + 1) * 4 + kPad);
+    const size_t o_sel = o_ks + up256(n * 4 + kPad);
+    const size_t o_ix = o_sel + up256(n * 4 + kPad);
+    const size_t o_v = o_ix + up256(nnz * 2 + kIxPad);
+    if (ks != nullptr) {
+        *ks = o_ks;
+        *sel = o_sel;
+        *ix = o_ix;
+        *v = o_v;
+    }
+    return o_v + nnz * 4 + kPad;
 }
 
 }  // namespace
@@ -37,22 +60,13 @@ Server::Server(SeismicIndex& index, const ServerConfig& cfg) : index_(index), cf
         // default priority: a stream of any other priority is never used here
         hip_check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipEventCreateWithFlags(&s.done, hipEventDisableTiming), "hipEventCreate");
-        s.h_ip = pinned<int32_t>(B + 1);
-        s.h_ks = pinned<int32_t>(B);
-        s.h_sel = pinned<int32_t>(B);
-        s.h_ix = pinned<uint16_t>(Q);
-        s.h_v = pinned<float>(Q);
-        s.h_dist = pinned<float>(O);
-        s.h_lab = pinned<int32_t>(O);
-        s.h_err = pinned<int32_t>(1);
-        s.d_err.allocate(4);
-        s.d_ip.allocate((B + 1) * 4);
-        s.d_ks.allocate(B * 4);
-        s.d_sel.allocate(B * 4);
-        s.d_ix.allocate(Q * 2, 64);
-        s.d_v.allocate(Q * 4);
-        s.d_dist.allocate(O * 4);
-        s.d_lab.allocate(O * 4);
+        const size_t in_max = input_layout(static_cast<int64_t>(B), static_cast<int64_t>(Q), nullptr, nullptr,
+                                           nullptr, nullptr);
+        const size_t out_max = kOutHead + up256(O * 4) + O * 4;
+        s.h_in = pinned(in_max);
+        s.h_out = pinned(out_max);
+        s.d_in.allocate(in_max);
+        s.d_out.allocate(out_max);
     }
     worker_ = std::thread([this] { run(); });
 }
@@ -63,14 +77,8 @@ Server::~Server() {
     if (worker_.joinable()) worker_.join();
     for (Slot& s : slots_) {
         if (s.stream != nullptr) (void)hipStreamSynchronize(s.stream);
-        (void)hipHostFree(s.h_ip);
-        (void)hipHostFree(s.h_ks);
-        (void)hipHostFree(s.h_sel);
-        (void)hipHostFree(s.h_ix);
-        (void)hipHostFree(s.h_v);
-        (void)hipHostFree(s.h_dist);
-        (void)hipHostFree(s.h_lab);
-        (void)hipHostFree(s.h_err);
+        (void)hipHostFree(s.h_in);
+        (void)hipHostFree(s.h_out);
         if (s.done != nullptr) (void)hipEventDestroy(s.done);
         if (s.stream != nullptr) (void)hipStreamDestroy(s.stream);
     }
@@ -175,14 +183,17 @@ bool Server::reap() {
         const hipError_t e = hipEventQuery(s.done);
         if (e == hipErrorNotReady) continue;
         hip_check(e, "server batch");
-        if (*s.h_err != 0) throw std::runtime_error("server batch: " + device_error_message(*s.h_err));
+        const int32_t err = *reinterpret_cast<const int32_t*>(s.h_out);
+        if (err != 0) throw std::runtime_error("server batch: " + device_error_message(err));
         const int64_t t_done = now_ns();
+        const float* h_dist = reinterpret_cast<const float*>(s.h_out + kOutHead);
+        const int32_t* h_lab = reinterpret_cast<const int32_t*>(s.h_out + s.off_lab);
         std::lock_guard<std::mutex> lk(mu_);
         for (size_t b = 0; b < s.tickets.size(); ++b) {
             Request& r = requests_[s.tickets[b] - first_ticket_];
             const size_t row = b * static_cast<size_t>(cfg_.k_max);
-            r.dist.assign(s.h_dist + row, s.h_dist + row + r.k);
-            r.lab.assign(s.h_lab + row, s.h_lab + row + r.k);
+            r.dist.assign(h_dist + row, h_dist + row + r.k);
+            r.lab.assign(h_lab + row, h_lab + row + r.k);
             r.done_ns = t_done;
         }
         stats_.completed += static_cast<int64_t>(s.tickets.size());
@@ -204,28 +215,45 @@ bool Server::admit() {
         {
             std::lock_guard<std::mutex> lk(mu_);
             if (pending_.empty()) return any;
+            // the oldest requests that fit the batch size and the scratch budget
             int64_t bytes = 0, nnz = 0;
-            while (!pending_.empty() && static_cast<int32_t>(batch.size()) < cfg_.max_batch) {
-                const Request& r = requests_[pending_.front() - first_ticket_];
+            size_t n = 0;
+            while (n < pending_.size() && static_cast<int32_t>(n) < cfg_.max_batch) {
+                const Request& r = requests_[pending_[n] - first_ticket_];
                 const int64_t need = request_bytes(static_cast<int32_t>(r.indices.size()));
                 if (inflight_bytes_ + bytes + need > cfg_.scratch_budget) {
                     ++stats_.deferred_by_budget;
                     break;
                 }
-                // pack into this stream's pinned staging
-                const size_t b = batch.size();
-                s.h_ip[b] = static_cast<int32_t>(nnz);
-                std::memcpy(s.h_ix + nnz, r.indices.data(), r.indices.size() * 2);
-                std::memcpy(s.h_v + nnz, r.values.data(), r.values.size() * 4);
-                s.h_ks[b] = r.k;
-                s.h_sel[b] = r.bitmap_id;
                 nnz += static_cast<int64_t>(r.indices.size());
                 bytes += need;
+                ++n;
+            }
+            if (n == 0) return any;
+            // pack them into this stream's pinned input block, zero padded
+            s.in_bytes = input_layout(static_cast<int64_t>(n), nnz, &s.off_ks, &s.off_sel, &s.off_ix, &s.off_v);
+            std::memset(s.h_in, 0, s.off_ks);
+            std::memset(s.h_in + s.off_ks, 0, s.off_ix - s.off_ks);
+            std::memset(s.h_in + s.off_ix + nnz * 2, 0, s.off_v - s.off_ix - nnz * 2);
+            std::memset(s.h_in + s.off_v + nnz * 4, 0, kPad);
+            int32_t* h_ip = reinterpret_cast<int32_t*>(s.h_in);
+            int32_t* h_ks = reinterpret_cast<int32_t*>(s.h_in + s.off_ks);
+            int32_t* h_sel = reinterpret_cast<int32_t*>(s.h_in + s.off_sel);
+            uint16_t* h_ix = reinterpret_cast<uint16_t*>(s.h_in + s.off_ix);
+            float* h_v = reinterpret_cast<float*>(s.h_in + s.off_v);
+            int64_t at = 0;
+            for (size_t b = 0; b < n; ++b) {
+                const Request& r = requests_[pending_.front() - first_ticket_];
+                h_ip[b] = static_cast<int32_t>(at);
+                std::memcpy(h_ix + at, r.indices.data(), r.indices.size() * 2);
+                std::memcpy(h_v + at, r.values.data(), r.values.size() * 4);
+                h_ks[b] = r.k;
+                h_sel[b] = r.bitmap_id;
+                at += static_cast<int64_t>(r.indices.size());
                 batch.push_back(pending_.front());
                 pending_.pop_front();
             }
-            if (batch.empty()) return any;
-            s.h_ip[batch.size()] = static_cast<int32_t>(nnz);
+            h_ip[n] = static_cast<int32_t>(at);
             s.bytes = bytes;
             inflight_bytes_ += bytes;
             ++stats_.batches;
@@ -243,23 +271,22 @@ bool Server::admit() {
 
 void Server::launch(Slot& s, const std::vector<int64_t>& batch) {
     const int64_t n = static_cast<int64_t>(batch.size());
-    const int64_t nnz = s.h_ip[n];
-    hip_check(hipMemcpyAsync(s.d_ip.get(), s.h_ip, (n + 1) * 4, hipMemcpyHostToDevice, s.stream), "H2D");
-    if (nnz > 0) {
-        hip_check(hipMemcpyAsync(s.d_ix.get(), s.h_ix, nnz * 2, hipMemcpyHostToDevice, s.stream), "H2D");
-        hip_check(hipMemcpyAsync(s.d_v.get(), s.h_v, nnz * 4, hipMemcpyHostToDevice, s.stream), "H2D");
-    }
-    hip_check(hipMemcpyAsync(s.d_ks.get(), s.h_ks, n * 4, hipMemcpyHostToDevice, s.stream), "H2D");
-    const bool with_sel = !bitmaps_.empty();
-    if (with_sel) hip_check(hipMemcpyAsync(s.d_sel.get(), s.h_sel, n * 4, hipMemcpyHostToDevice, s.stream), "H2D");
-    index_.search_device(n, s.d_ip.get<int32_t>(), s.d_ix.get<uint16_t>(), s.d_v.get<float>(), s.d_ks.get<int32_t>(),
-                         cfg_.k_max, cfg_.req, with_sel ? s.d_sel.get<int32_t>() : nullptr,
-                         with_sel ? d_bitmap_table_.get<const uint32_t*>() : nullptr, s.d_dist.get<float>(),
-                         s.d_lab.get<int32_t>(), nullptr, s.stream, s.d_err.get<int32_t>());
-    hip_check(hipMemcpyAsync(s.h_err, s.d_err.get(), 4, hipMemcpyDeviceToHost, s.stream), "D2H");
+    hip_check(hipMemcpyAsync(s.d_in.get(), s.h_in, s.in_bytes, hipMemcpyHostToDevice, s.stream), "H2D");
     const size_t out = static_cast<size_t>(n) * cfg_.k_max;
-    hip_check(hipMemcpyAsync(s.h_dist, s.d_dist.get(), out * 4, hipMemcpyDeviceToHost, s.stream), "D2H");
-    hip_check(hipMemcpyAsync(s.h_lab, s.d_lab.get(), out * 4, hipMemcpyDeviceToHost, s.stream), "D2H");
+    s.off_lab = kOutHead + up256(out * 4);
+    s.out_bytes = s.off_lab + out * 4;
+    char* d_in = s.d_in.get<char>();
+    char* d_out = s.d_out.get<char>();
+    const bool with_sel = !bitmaps_.empty();
+    index_.search_device(n, reinterpret_cast<const int32_t*>(d_in), reinterpret_cast<const uint16_t*>(d_in + s.off_ix),
+                         reinterpret_cast<const float*>(d_in + s.off_v),
+                         reinterpret_cast<const int32_t*>(d_in + s.off_ks), cfg_.k_max, cfg_.req,
+                         with_sel ? reinterpret_cast<const int32_t*>(d_in + s.off_sel) : nullptr,
+                         with_sel ? d_bitmap_table_.get<const uint32_t*>() : nullptr,
+                         reinterpret_cast<float*>(d_out + kOutHead), reinterpret_cast<int32_t*>(d_out + s.off_lab),
+                         nullptr, s.stream, reinterpret_cast<int32_t*>(d_out));
+    // the error flag, the distances and the labels in one copy
+    hip_check(hipMemcpyAsync(s.h_out, d_out, s.out_bytes, hipMemcpyDeviceToHost, s.stream), "D2H");
     hip_check(hipEventRecord(s.done, s.stream), "hipEventRecord");
     s.tickets = batch;
     s.busy = true;

@@ -84,12 +84,17 @@ private:
         bool busy = false;
         int64_t bytes = 0;
         std::vector<int64_t> tickets;
-        // pinned host staging and device buffers, sized for max_batch full-length queries
-        int32_t *h_ip = nullptr, *h_ks = nullptr, *h_sel = nullptr, *h_lab = nullptr;
-        uint16_t* h_ix = nullptr;
-        float *h_v = nullptr, *h_dist = nullptr;
-        int32_t* h_err = nullptr;  // the batch's device error flag, copied back
-        DeviceBuffer d_ip, d_ix, d_v, d_ks, d_sel, d_dist, d_lab, d_err;
+        // One pinned block and one device block for a batch's inputs, one of
+        // each for its outputs, sized for max_batch full-length queries. A
+        // batch packs its arrays from the start of each block, so it moves in
+        // one copy each way.
+        char *h_in = nullptr, *h_out = nullptr;
+        DeviceBuffer d_in, d_out;
+        // byte offsets of the current batch's arrays: inputs (query offsets,
+        // k, selector, term ids, values) and outputs (error flag at 0,
+        // distances, labels)
+        size_t off_ks = 0, off_sel = 0, off_ix = 0, off_v = 0, in_bytes = 0;
+        size_t off_lab = 0, out_bytes = 0;
     };
 
     void run();
