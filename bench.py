@@ -322,7 +322,7 @@ def main():
     ap.add_argument("--serve-batch", type=int, default=1024)
     ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--serve-budget-mb", type=int, default=256)
-    ap.add_argument("--rescore-candidates", type=int, default=100,
+    ap.add_argument("--rescore-candidates", type=int, default=20,
                     help="two-stage operating point: SQ candidates per query re-scored in FP32 (0 skips)")
     ap.add_argument("--inverted-queries", type=int, default=300,
                     help="exact InvertedIndex on the same corpus: drop-in search of this many queries (0 skips)")
