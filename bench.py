@@ -324,6 +324,9 @@ def main():
     ap.add_argument("--serve-budget-mb", type=int, default=256)
     ap.add_argument("--rescore-candidates", type=int, default=20,
                     help="two-stage operating point: SQ candidates per query re-scored in FP32 (0 skips)")
+    ap.add_argument("--rescore-cut", type=int, default=8, help="two-stage operating point: the SQ search's cut")
+    ap.add_argument("--rescore-hf", type=float, default=0.8,
+                    help="two-stage operating point: the SQ search's heap_factor")
     ap.add_argument("--inverted-queries", type=int, default=300,
                     help="exact InvertedIndex on the same corpus: drop-in search of this many queries (0 skips)")
     ap.add_argument("--schedule", default="auto",
@@ -497,7 +500,8 @@ def main():
         cand_l = torch.empty((args.queries, kc), dtype=torch.int32, device=dev)
         fp32 = torch.empty((args.queries, kc), dtype=torch.float32, device=dev)
         err = torch.zeros(1, dtype=torch.int32, device=dev)
-        cplan = index.search_plan(qip, qix, qv, kc, params=params, out_d=cand_d, out_l=cand_l, stream=stream)
+        rparams = nsparse.SeismicSearchParameters(args.rescore_cut, args.rescore_hf)
+        cplan = index.search_plan(qip, qix, qv, kc, params=rparams, out_d=cand_d, out_l=cand_l, stream=stream)
 
         def two_stage():
             cplan.launch(stream)
@@ -517,12 +521,15 @@ def main():
         torch.cuda.synchronize()
         r_ms = e0.elapsed_time(e1) / args.steps
         best = torch.gather(cand_l, 1, top).cpu().numpy()
-        rescored = {"candidates": kc, "qps": args.queries / (r_ms * 1e-3), "ms_per_step": r_ms,
-                    "recall_at_10": recall_of(best), "candidate_schedule": cplan.config,
-                    "note": f"SQ batched search for {kc} candidates per query, their exact FP32 scores from the "
+        rescored = {"candidates": kc, "cut": args.rescore_cut, "heap_factor": args.rescore_hf,
+                    "qps": args.queries / (r_ms * 1e-3), "ms_per_step": r_ms, "recall_at_10": recall_of(best),
+                    "recall_queries": n_rec, "candidate_schedule": cplan.config,
+                    "note": f"SQ batched search (cut {args.rescore_cut}, heap_factor {args.rescore_hf}) for {kc} "
+                            "candidates per query, their exact FP32 scores from the "
                             "re-scoring kernel, top-10 by FP32 score (torch.topk): an operating point beyond the "
                             "reference's arithmetic"}
-        log(f"two-stage: SQ top-{kc} + FP32 re-score: {rescored['qps']:,.0f} QPS ({r_ms:.2f} ms/step), "
+        log(f"two-stage: SQ top-{kc} (cut {args.rescore_cut}, hf {args.rescore_hf}) + FP32 re-score: "
+            f"{rescored['qps']:,.0f} QPS ({r_ms:.2f} ms/step), "
             f"recall@10 {rescored['recall_at_10']:.4f}")
         del d_ip, d_ix, d_v, cand_d, cand_l, fp32, cplan
 
