@@ -131,3 +131,55 @@ def test_info_nce_gradient_matches_finite_differences(nsparse, setup):
         b[p] -= eps
         fd = (loss64(qv64, a) - loss64(qv64, b)) / (2 * eps)
         assert abs(dv.grad[p].item() - fd) <= 1e-4 * max(abs(fd), 1e-2), (p, dv.grad[p].item(), fd)
+
+
+def test_two_stage_composition(nsparse):
+    """The two-stage search as composed from the public API: an SQ search plan
+    mines 20 candidates, `rescore` scores them in FP32, `torch.topk` keeps 10.
+    The result is each query's exact FP32 top-10 among its candidates
+    (float64 CPU scores; ties in FP32 accepted by score), and its recall
+    against exact FP32 brute force is not below the SQ search's own top-10
+    (a k=20 traversal prunes differently from a k=10 one, so its candidates
+    need not contain that top-10: 0.05 of slack)."""
+    import torch
+
+    from nsparse import synth
+    from nsparse.rescore import rescore
+
+    print("two-stage composition", flush=True)
+    docs = synth.make_corpus(synth.doc_spec(seed=70), 6000)
+    queries = synth.make_corpus(synth.query_spec(seed=71), 24)
+    vmin, vmax = float(docs[2].min()), float(docs[2].max())
+    idx = nsparse.index_factory(
+        DIM, f"seismic_sq,quantizer=8bit|vmin={vmin:.9g}|vmax={vmax:.9g}|lambda=300|beta=30|alpha=0.4|seed=0")
+    idx.add(len(docs[0]) - 1, *docs)
+    idx.build()
+    dev = torch.device("cuda")
+    q_ip = torch.from_numpy(queries[0]).to(dev)
+    q_ix = torch.from_numpy(queries[1].view(np.int16)).to(dev)
+    q_v = torch.from_numpy(queries[2]).to(dev)
+    d_ip = torch.from_numpy(docs[0]).to(dev)
+    d_ix = torch.from_numpy(docs[1].view(np.int16)).to(dev)
+    d_v = torch.from_numpy(docs[2]).to(dev)
+    plan = idx.search_plan(q_ip, q_ix, q_v, 20, params=nsparse.SeismicSearchParameters(8, 0.8))
+    _, cand = plan.launch()
+    with torch.no_grad():
+        s = rescore(q_v, d_v, q_ip, q_ix, cand, d_ip, d_ix, DIM)
+    top = torch.topk(s.masked_fill(cand < 0, float("-inf")), 10, dim=1)
+    labels = torch.gather(cand, 1, top.indices).cpu().numpy()
+    torch.cuda.synchronize()
+    c = cand.cpu().numpy()
+    exact = cpu_scores(docs, queries, c, queries[2].astype(np.float64), docs[2].astype(np.float64))
+    for q in range(c.shape[0]):
+        valid = c[q] >= 0
+        want = np.sort(exact[q][valid])[::-1][:10]
+        got = np.array([exact[q][list(c[q]).index(d)] for d in labels[q] if d >= 0])
+        np.testing.assert_allclose(got, want[:len(got)], rtol=1e-5)
+    # recall against exact FP32 brute force over all docs, against the SQ search's own top-10
+    _, sq_lab = idx.search(len(queries[0]) - 1, *queries, 10, nsparse.SeismicSearchParameters(8, 0.8))
+    brute = nsparse.index_factory(DIM, "brutal")
+    brute.add(len(docs[0]) - 1, *docs)
+    _, truth = brute.search(len(queries[0]) - 1, *queries, 10)
+    rec = lambda lab: np.mean([len(set(lab[i]) & set(truth[i])) / 10 for i in range(len(truth))])  # noqa: E731
+    print(f"recall@10 two-stage {rec(labels):.3f}, SQ {rec(sq_lab):.3f}", flush=True)
+    assert rec(labels) >= rec(sq_lab) - 0.05
