@@ -50,7 +50,8 @@ constexpr int kQMax = 512;    // longest query handled on device
 // within four workgroups per CU. A 1024-lane workgroup (alone on its CU) could
 // stage 2048, but smaller chunks speculate less past the prune point and turn
 // over sooner: one query per launch at 8.8M docs took p50 1.27 ms with 2048,
-// 1.17 ms with 1024, 1.14 ms with 768 or 512. A multiple of 64 up to 4096
+// 1.17 ms with 1024, 1.14 ms with 768 or 512; 384 was 2% slower there and
+// 1.1% slower for the batched 512-lane launch. A multiple of 64 up to 4096
 // (the first-doc search reads every budget/64-th word start with one lane each).
 __host__ __device__ constexpr int chunk_budget(bool dense, int /*block*/) {
     return dense ? 512 : 256;
