@@ -131,9 +131,11 @@ int nsparse_search(nsparse_index* index, nsparse_idx_t n, const nsparse_idx_t* i
  * call). Queries and outputs are DEVICE pointers already resident in HBM; query i
  * asks for ks[i] results (ks may be NULL: all k_max), written to row i of an
  * n x k_max output and padded with (-1.0f, -1). selector_of_query[i] (may be NULL)
- * indexes bitmaps[] (device pointers, NULL entry = no filter) or is -1. Runs as one
- * launch (one workgroup per query) on `stream` (hipStream_t, NULL = default) and does
- * not synchronize. cut/heap_factor/vmin/vmax come from params (selector field ignored). */
+ * indexes bitmaps[] (n_bitmaps device pointers, NULL entry = no filter) or is -1;
+ * an index >= n_bitmaps pads that query's row and reports device error 6
+ * (nsparse_device_error). Runs as one launch (one workgroup per query) on `stream`
+ * (hipStream_t, NULL = default) and does not synchronize. cut/heap_factor/vmin/vmax
+ * come from params (selector field ignored). */
 int nsparse_search_device(nsparse_index* index, int64_t n, const int32_t* d_indptr,
                           const uint16_t* d_indices, const float* d_values, const int32_t* d_ks,
                           int k_max, const nsparse_search_params* params,

@@ -217,7 +217,7 @@ int nsparse_search_device(nsparse_index* index, int64_t n, const int32_t* d_indp
         if (d_selector_of_query != nullptr && (d_bitmaps == nullptr || n_bitmaps <= 0))
             throw std::invalid_argument("per-query selectors need a bitmap table");
         s.search_device(n, d_indptr, d_indices, d_values, d_ks, k_max, device_request(params), d_selector_of_query,
-                        d_bitmaps, d_distances, d_labels, nullptr, static_cast<hipStream_t>(stream));
+                        d_bitmaps, n_bitmaps, d_distances, d_labels, nullptr, static_cast<hipStream_t>(stream));
     });
 }
 
@@ -239,7 +239,8 @@ int nsparse_search_plan_create(nsparse_index* index, int64_t n, const int32_t* d
             throw std::invalid_argument("per-query selectors need a bitmap table");
         require_device();
         const SearchArgs a = s.search_args(n, d_indptr, d_indices, d_values, d_ks, k_max, device_request(params),
-                                           d_selector_of_query, d_bitmaps, d_distances, d_labels, nullptr);
+                                           d_selector_of_query, d_bitmaps, n_bitmaps, d_distances, d_labels,
+                                           nullptr);
         SearchConfig c;
         if (config != nullptr) {
             c.lookup = config->lookup;
@@ -290,7 +291,7 @@ int nsparse_search_device_stats(nsparse_index* index, int64_t n, const int32_t* 
                                 int64_t* d_stats, void* stream) {
     return guarded([&] {
         SeismicIndex& s = seismic_of(index);
-        s.search_device(n, d_indptr, d_indices, d_values, d_ks, k_max, device_request(params), nullptr, nullptr,
+        s.search_device(n, d_indptr, d_indices, d_values, d_ks, k_max, device_request(params), nullptr, nullptr, 0,
                         d_distances, d_labels, d_stats, static_cast<hipStream_t>(stream));
     });
 }

@@ -94,6 +94,7 @@ struct SearchArgs {
     float decode_scale;        // (ingest_range * query_range) / (255 * 255)
     const int32_t* sel_of_query;        // device, per-query selector slot or -1 (nullptr: use sel_all)
     const uint32_t* const* sel_bitmaps; // device array of device bitmap pointers
+    int32_t n_bitmaps;                  // entries of sel_bitmaps; a slot >= n_bitmaps is rejected
     const uint32_t* sel_all;            // one bitmap for every query (nullptr: none)
     float* out_d;              // device, n x k_max
     int32_t* out_l;
@@ -122,7 +123,8 @@ enum DeviceError : int32_t {
     kDevTermOutOfRange = 2, // query term id >= dim
     kDevTooManyClusters = 3, // list with more clusters than the LDS plan allows
     kDevBadK = 4,           // per-query k outside [1, k_max]
-    kDevChunkWait = 5       // a wait between the waves of a search timed out
+    kDevChunkWait = 5,      // a wait between the waves of a search timed out
+    kDevBadSelector = 6     // per-query selector slot >= the bitmap table's size
 };
 
 }  // namespace nsparse_amd

@@ -508,11 +508,12 @@ void SeismicIndex::search_host(idx_t n, const idx_t* indptr, const term_t* indic
 
 void SeismicIndex::search_device(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices,
                                  const float* d_values, const int32_t* d_ks, int k_max, const SearchRequest& req,
-                                 const int32_t* d_sel_of_query, const uint32_t* const* d_bitmaps, float* d_out_d,
-                                 int32_t* d_out_l, int64_t* d_stats, hipStream_t stream, int32_t* d_err) {
+                                 const int32_t* d_sel_of_query, const uint32_t* const* d_bitmaps, int32_t n_bitmaps,
+                                 float* d_out_d, int32_t* d_out_l, int64_t* d_stats, hipStream_t stream,
+                                 int32_t* d_err) {
     if (!has_vectors_ && n > 0 && k_max > 0) return;
     SearchArgs a = search_args(n, d_indptr, d_indices, d_values, d_ks, k_max, req, d_sel_of_query, d_bitmaps,
-                               d_out_d, d_out_l, d_stats);
+                               n_bitmaps, d_out_d, d_out_l, d_stats);
     if (d_err != nullptr) a.error_flag = d_err;
     launch_search_checked(a, max_clusters_, k_max, sq_, stream);
 }
@@ -520,7 +521,7 @@ void SeismicIndex::search_device(int64_t n, const int32_t* d_indptr, const uint1
 SearchArgs SeismicIndex::search_args(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices,
                                      const float* d_values, const int32_t* d_ks, int k_max, const SearchRequest& req,
                                      const int32_t* d_sel_of_query, const uint32_t* const* d_bitmaps,
-                                     float* d_out_d, int32_t* d_out_l, int64_t* d_stats) {
+                                     int32_t n_bitmaps, float* d_out_d, int32_t* d_out_l, int64_t* d_stats) {
     if (n <= 0) throw std::invalid_argument("value must be positive");
     if (k_max <= 0) throw std::invalid_argument("value must be positive");
     throw_if_any_null(d_indptr, d_indices, d_values, d_out_d, d_out_l);
@@ -547,7 +548,10 @@ SearchArgs SeismicIndex::search_args(int64_t n, const int32_t* d_indptr, const u
     const float max_q = 255.0F;
     a.decode_scale = (ingest_range * query_range) / (max_q * max_q);
     a.sel_of_query = d_sel_of_query;
+    if (d_sel_of_query != nullptr && (d_bitmaps == nullptr || n_bitmaps <= 0))
+        throw std::invalid_argument("per-query selectors need a bitmap table");
     a.sel_bitmaps = d_bitmaps;
+    a.n_bitmaps = d_sel_of_query != nullptr ? n_bitmaps : 0;
     a.sel_all = nullptr;
     a.out_d = d_out_d;
     a.out_l = d_out_l;

@@ -102,12 +102,13 @@ public:
     // last_device_error reads). Launches that may overlap pass flags of their own.
     void search_device(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices, const float* d_values,
                        const int32_t* d_ks, int k_max, const SearchRequest& req, const int32_t* d_sel_of_query,
-                       const uint32_t* const* d_bitmaps, float* d_out_d, int32_t* d_out_l, int64_t* d_stats,
-                       hipStream_t stream, int32_t* d_err = nullptr);
+                       const uint32_t* const* d_bitmaps, int32_t n_bitmaps, float* d_out_d, int32_t* d_out_l,
+                       int64_t* d_stats, hipStream_t stream, int32_t* d_err = nullptr);
     // The kernel arguments of a batched search (validated), as search_device uses them.
     SearchArgs search_args(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices, const float* d_values,
                            const int32_t* d_ks, int k_max, const SearchRequest& req, const int32_t* d_sel_of_query,
-                           const uint32_t* const* d_bitmaps, float* d_out_d, int32_t* d_out_l, int64_t* d_stats);
+                           const uint32_t* const* d_bitmaps, int32_t n_bitmaps, float* d_out_d, int32_t* d_out_l,
+                           int64_t* d_stats);
     // The search proper over host queries (IDMap passes its mapped selector).
     void search_host(idx_t n, const idx_t* indptr, const term_t* indices, const float* values, int k,
                      float* distances, idx_t* labels, const SearchRequest& req, int64_t* stats_out);

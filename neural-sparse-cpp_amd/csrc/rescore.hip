@@ -1,12 +1,17 @@
 // Differentiable exact re-scoring of mined candidates (training-style use of
 // the SEISMIC scorer, compute_similarity in seismic_common.h:78-103 on FP32).
 //
-//   score[q, c] = sum_i qv[i] * d(cand[q, c])[t_i]      (i over query q's entries)
-//   dL/dqv[i]   = sum_c g[q, c] * d(cand[q, c])[t_i]
-//   dL/dd[e]   += g[q, c] * sum_{i: t_i == term(e)} qv[i]   (e an entry of the candidate row)
+//   score[q, c] = sum_e d_e * Q[term(e)]     (e over the entries of row cand[q, c])
+//   dL/dqv[i]   = sum_c g[q, c] * d(cand[q, c])[t_i]   if i is the last entry of term t_i, else 0
+//   dL/dd[e]   += g[q, c] * Q[term(e)]
+//
+// Q is the query as a dense vector: a term repeated in the query takes the
+// value of its last entry, as SparseVectors::get_dense_vector builds it for
+// compute_similarity and as the SEISMIC and brute-force searches score it, so
+// the earlier entries of a repeated term have no influence and get gradient 0.
 //
 // One workgroup per query. The query's distinct terms sit in LDS (bitmap +
-// popcount rank table) with the per-term sum of its entries' values and a
+// popcount rank table) with the value of each term's last entry and a
 // per-term gradient accumulator; one wavefront per candidate row gathers the
 // row from HBM once for the forward and once for the backward. Gradients of
 // document weights are scatter-added with float atomics (a document can be a
@@ -28,8 +33,9 @@ constexpr int kRQMax = 512;
 struct RescoreLds {
     uint32_t* qbits;
     uint16_t* qpref;
-    float* qsum;   // per distinct term: sum of the query's values for it
-    float* gsum;   // per distinct term: accumulated gradient
+    float* qval;    // per distinct term: the value of its last entry
+    float* gsum;    // per distinct term: accumulated gradient
+    int32_t* qlast; // per distinct term: index of its last entry
     int32_t* wsum;
 };
 
@@ -44,14 +50,16 @@ __device__ __forceinline__ int rank_of(const RescoreLds& L, uint32_t t, bool& pr
 __device__ void setup_query(RescoreLds& L, unsigned char* smem, int nw, const uint16_t* qi, const float* qv, int qn) {
     L.qbits = reinterpret_cast<uint32_t*>(smem);
     L.qpref = reinterpret_cast<uint16_t*>(L.qbits + nw);
-    L.qsum = reinterpret_cast<float*>(smem + ((nw * 6 + 15) & ~15));
-    L.gsum = L.qsum + kRQMax;
-    L.wsum = reinterpret_cast<int32_t*>(L.gsum + kRQMax);
+    L.qval = reinterpret_cast<float*>(smem + ((nw * 6 + 15) & ~15));
+    L.gsum = L.qval + kRQMax;
+    L.qlast = reinterpret_cast<int32_t*>(L.gsum + kRQMax);
+    L.wsum = L.qlast + kRQMax;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int w = tid; w < nw; w += kRB) L.qbits[w] = 0;
     for (int r = tid; r < kRQMax; r += kRB) {
-        L.qsum[r] = 0.0F;
+        L.qval[r] = 0.0F;
         L.gsum[r] = 0.0F;
+        L.qlast[r] = -1;
     }
     __syncthreads();
     for (int i = tid; i < qn; i += kRB)
@@ -78,8 +86,11 @@ __device__ void setup_query(RescoreLds& L, unsigned char* smem, int nw, const ui
     for (int i = tid; i < qn; i += kRB) {
         if ((qi[i] >> 5) >= nw) continue;
         bool p;
-        atomicAdd(&L.qsum[rank_of(L, qi[i], p)], qv[i]);
+        atomicMax(&L.qlast[rank_of(L, qi[i], p)], i);
     }
+    __syncthreads();
+    for (int r = tid; r < kRQMax; r += kRB)
+        if (L.qlast[r] >= 0) L.qval[r] = qv[L.qlast[r]];
     __syncthreads();
 }
 
@@ -111,7 +122,7 @@ __global__ __launch_bounds__(kRB) void rescore_fwd_kernel(const int32_t* __restr
                 const uint32_t t = d_indices[p];
                 if ((t >> 5) >= static_cast<uint32_t>(nw)) continue;
                 const int r = rank_of(L, t, pr);
-                if (pr) acc += d_values[p] * L.qsum[r];
+                if (pr) acc += d_values[p] * L.qval[r];
             }
         }
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
@@ -150,7 +161,7 @@ __global__ __launch_bounds__(kRB) void rescore_bwd_kernel(const int32_t* __restr
             if ((t >> 5) >= static_cast<uint32_t>(nw)) continue;
             const int r = rank_of(L, t, pr);
             if (!pr) continue;
-            if (grad_d != nullptr) atomicAdd(&grad_d[p], g * L.qsum[r]);
+            if (grad_d != nullptr) atomicAdd(&grad_d[p], g * L.qval[r]);
             if (grad_q != nullptr) atomicAdd(&L.gsum[r], g * d_values[p]);
         }
     }
@@ -159,11 +170,16 @@ __global__ __launch_bounds__(kRB) void rescore_bwd_kernel(const int32_t* __restr
         for (int i = threadIdx.x; i < qn; i += kRB) {
             bool p;
             const uint32_t t = q_indices[s0 + i];
-            grad_q[s0 + i] = (t >> 5) < static_cast<uint32_t>(nw) ? L.gsum[rank_of(L, t, p)] : 0.0F;
+            float gi = 0.0F;
+            if ((t >> 5) < static_cast<uint32_t>(nw)) {
+                const int r = rank_of(L, t, p);
+                gi = L.qlast[r] == i ? L.gsum[r] : 0.0F;
+            }
+            grad_q[s0 + i] = gi;
         }
 }
 
-int lds_bytes(int nw) { return ((nw * 6 + 15) & ~15) + kRQMax * 8 + 4 * 4; }
+int lds_bytes(int nw) { return ((nw * 6 + 15) & ~15) + kRQMax * 12 + 4 * 4; }
 
 }  // namespace
 

@@ -381,14 +381,17 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
     float* out_d = a.out_d + q * a.k_max;
     int32_t* out_l = a.out_l + q * a.k_max;
     const uint32_t* sel = a.sel_all;
+    bool bad_sel = false;
     if (a.sel_of_query != nullptr) {
         const int s = a.sel_of_query[q];
-        sel = s >= 0 ? a.sel_bitmaps[s] : nullptr;
+        bad_sel = s >= a.n_bitmaps;
+        sel = s >= 0 && !bad_sel ? a.sel_bitmaps[s] : nullptr;
     }
     const float hf = a.heap_factor;
 
-    if (qn > kQMax || k <= 0 || k > P.kmax) {
-        if (tid == 0) atomicCAS(a.error_flag, 0, qn > kQMax ? kDevQueryTooLong : kDevBadK);
+    if (bad_sel || qn > kQMax || k <= 0 || k > P.kmax) {
+        if (tid == 0)
+            atomicCAS(a.error_flag, 0, bad_sel ? kDevBadSelector : qn > kQMax ? kDevQueryTooLong : kDevBadK);
         for (int i = tid; i < a.k_max; i += kBlock) {
             out_d[i] = -1.0F;
             out_l[i] = -1;
@@ -801,11 +804,15 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
         // `overlap` runs exactly once, while the first round's doc-id and row
         // loads are in flight (rounds are uniform over the threads); it must not
         // touch the slots being gathered.
+        // `abandoned` (wave 0 only) is set by `overlap` when the other waves
+        // may still be reading the staging buffer the slots go to; the slots
+        // are then left untouched.
+        bool abandoned = false;
         auto gather_slots = [&](auto unroll_tag, const Chunk& C, int t0, int nt, auto&& overlap) {
             constexpr int kUnroll = decltype(unroll_tag)::value;
             const int nseg = C.meta[kChunkNseg], total = C.meta[kChunkTotal];
             if (total == 0) overlap();
-            for (int r0 = 0; r0 < total; r0 += nt * kUnroll) {
+            for (int r0 = 0; r0 < total && !abandoned; r0 += nt * kUnroll) {
                 const int j0 = r0 + t0;
                 int64_t pp[kUnroll];
 #pragma unroll
@@ -828,7 +835,10 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                     d[u] = ix.docs[pp[u]];
                     ref[u] = blk ? ix.row_ref[pp[u]] : 0;
                 }
-                if (r0 == 0) overlap();
+                if (r0 == 0) {
+                    overlap();
+                    if (abandoned) break;  // wave-uniform
+                }
 #pragma unroll
                 for (int u = 0; u < kUnroll; ++u) {
                     const int j = j0 + u * nt;
@@ -1254,7 +1264,15 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                 auto replay_prev = [&] {
                     if (!have_prev) return;
                     tick(4);
-                    wait_for(&prev.meta[kChunkCheckin], kWaves);
+                    // A wait that gave up (error 5 is set, the launch's
+                    // results are void) leaves waves that may still score
+                    // chunk i - 1 from the staging buffer chunk i + 1 would
+                    // reuse: stop without gathering or scanning into it and
+                    // publish an end marker instead.
+                    if (!wait_for(&prev.meta[kChunkCheckin], kWaves)) {
+                        stop = true;
+                        abandoned = true;
+                    }
                     if (!stop) {
                         stop = replay_chunk(prev, tkn);
                         if (lane == 0) misc[kMiscTkN] = tkn;
@@ -1274,7 +1292,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                 tick(3);
                 gather_slots(std::integral_constant<int, (kBud / 64 < 8 ? kBud / 64 : 8)>{}, next, lane, 64,
                              replay_prev);
-                scan_words(next);
+                if (!abandoned) scan_words(next);
                 if (lane == 0) {
                     next.meta[kChunkEnd] = (stop || next.meta[kChunkNseg] == 0) ? 1 : 0;
                     lds_store_release(&next.meta[kChunkStamp], i + 1);
@@ -1997,6 +2015,7 @@ std::string device_error_message(int32_t code) {
         case kDevTooManyClusters: return "posting list has more clusters than the device plan";
         case kDevBadK: return "per-query k outside [1, k_max]";
         case kDevChunkWait: return "a wait between the waves of a search workgroup timed out";
+        case kDevBadSelector: return "per-query selector index outside the bitmap table";
         default: return "device search reported error " + std::to_string(code);
     }
 }

@@ -283,7 +283,7 @@ void Server::launch(Slot& s, const std::vector<int64_t>& batch) {
                          reinterpret_cast<const int32_t*>(d_in + s.off_ks), cfg_.k_max, cfg_.req,
                          with_sel ? reinterpret_cast<const int32_t*>(d_in + s.off_sel) : nullptr,
                          with_sel ? d_bitmap_table_.get<const uint32_t*>() : nullptr,
-                         reinterpret_cast<float*>(d_out + kOutHead), reinterpret_cast<int32_t*>(d_out + s.off_lab),
+                         static_cast<int32_t>(bitmaps_.size()), reinterpret_cast<float*>(d_out + kOutHead), reinterpret_cast<int32_t*>(d_out + s.off_lab),
                          nullptr, s.stream, reinterpret_cast<int32_t*>(d_out));
     // the error flag, the distances and the labels in one copy
     hip_check(hipMemcpyAsync(s.h_out, d_out, s.out_bytes, hipMemcpyDeviceToHost, s.stream), "D2H");

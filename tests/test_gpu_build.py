@@ -1,11 +1,23 @@
 """GPU build parity: add() + build() on the device, then write_index, must give
 the oracle's bytes exactly (id tie mode: value ties broken by doc id / term
 id). For FP32 data, where values do not tie, that is also the reference's
-own layout, checked against oracle/_ref when it is present."""
+own layout: the device's bytes hash to the digests of the reference's own
+build in tests/golden/ref_fixtures.json."""
+import hashlib
+import json
+import os
+import sys
+
 import numpy as np
 import pytest
 
 import _oracle as O
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+from golden_cases import BUILD_CASES, build_corpus  # noqa: E402
+
+with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_fixtures.json")) as _f:
+    FIX = json.load(_f)
 
 pytestmark = pytest.mark.gpu
 
@@ -42,7 +54,7 @@ def test_sq_build_matches_oracle(nsparse, n, lam, beta):
 
 
 @pytest.mark.parametrize("n,lam,beta,alpha", [(3000, 200, 20, 0.4), (6000, 500, 50, 0.5), (1000, 100, 0, 0.3)])
-def test_f32_build_matches_oracle_and_reference(nsparse, n, lam, beta, alpha):
+def test_f32_build_matches_oracle(nsparse, n, lam, beta, alpha):
     from nsparse import synth
 
     docs = synth.make_corpus(synth.doc_spec(seed=100 + n), n)
@@ -50,11 +62,24 @@ def test_f32_build_matches_oracle_and_reference(nsparse, n, lam, beta, alpha):
     got = nsparse.serialize_index(gpu_build(nsparse, desc, docs))
     want = oracle_bytes(docs, 0, lam, beta, alpha, 9)
     assert got == want
-    # FP32 weights can still tie inside one posting list (float32 collisions);
-    # where no tie reaches the layout, the two tie modes agree and the device
-    # output is the reference's own.
-    if O.ref is not None and want == oracle_bytes(docs, 0, lam, beta, alpha, 9, tie_mode=0):
-        assert got == O.ref_build_bytes(DIM, 0, *docs, lam, beta, alpha, 9)
+
+
+F32_GOLDEN = [i for i, c in enumerate(BUILD_CASES) if c["sq"] == 0]
+
+
+@pytest.mark.parametrize("i", F32_GOLDEN, ids=[f"n{BUILD_CASES[i]['n']}-l{BUILD_CASES[i]['lam']}" for i in F32_GOLDEN])
+def test_f32_build_hashes_to_reference(nsparse, i):
+    """FP32 corpora whose values never tie in a way that reaches the layout:
+    the device's index bytes are the reference's own (SHA-256 and size of the
+    reference build's write_index output, kept in the golden fixtures)."""
+    case, fx = BUILD_CASES[i], FIX["build"][i]
+    ip, ix, v = build_corpus(case)
+    idx = nsparse.index_factory(case["dim"], f"seismic,lambda={case['lam']}|beta={case['beta']}|"
+                                             f"alpha={case['alpha']}|seed={case['seed']}")
+    idx.add(len(ip) - 1, ip, ix, v)
+    idx.build()
+    got = nsparse.serialize_index(idx)
+    assert (hashlib.sha256(got).hexdigest(), len(got)) == (fx["sha256"], fx["size"])
 
 
 def test_build_then_search_matches_oracle(nsparse):

@@ -2,7 +2,9 @@
 exact CPU computation, gradients against a CPU reference gradient and FP64
 central finite differences (eps 1e-3, relative error <= 1e-4), on candidates
 mined by the GPU SEISMIC search, with an InfoNCE loss (temperature 0.05, first
-candidate positive)."""
+candidate positive). A query term given twice takes the value of its last
+entry (the dense query of compute_similarity, seismic_common.h:78-103, built
+by SparseVectors::get_dense_vector): its earlier entries get gradient 0."""
 import numpy as np
 import pytest
 
@@ -31,22 +33,27 @@ def setup(nsparse):
     return docs, queries, cand, t
 
 
+def dense_query(qix, qv, lo, hi):
+    """term -> (value, entry index) of the last entry of each term."""
+    qmap = {}
+    for i in range(lo, hi):
+        qmap[int(qix[i])] = (qv[i], i)
+    return qmap
+
+
 def cpu_scores(docs, queries, cand, qv, dv):
-    """float64 exact sparse dot products and their gradients w.r.t. a unit
-    upstream gradient weighted by g (returned as functions)."""
+    """float64 exact sparse dot products against the dense query."""
     dip, dix, _ = docs
     qip, qix, _ = queries
     n_q, n_c = cand.shape
     S = np.zeros((n_q, n_c))
     for q in range(n_q):
-        qmap = {}
-        for i in range(qip[q], qip[q + 1]):
-            qmap[int(qix[i])] = qmap.get(int(qix[i]), 0.0) + qv[i]
+        qmap = dense_query(qix, qv, qip[q], qip[q + 1])
         for c in range(n_c):
             d = cand[q, c]
             if d < 0:
                 continue
-            S[q, c] = sum(dv[p] * qmap.get(int(dix[p]), 0.0) for p in range(dip[d], dip[d + 1]))
+            S[q, c] = sum(dv[p] * qmap.get(int(dix[p]), (0.0, -1))[0] for p in range(dip[d], dip[d + 1]))
     return S
 
 
@@ -56,18 +63,17 @@ def cpu_grads(docs, queries, cand, qv, dv, G):
     gq = np.zeros(len(qv))
     gd = np.zeros(len(dv))
     for q in range(cand.shape[0]):
-        qmap = {}
-        for i in range(qip[q], qip[q + 1]):
-            qmap[int(qix[i])] = qmap.get(int(qix[i]), 0.0) + qv[i]
+        qmap = dense_query(qix, qv, qip[q], qip[q + 1])
+        last = {i for _, i in qmap.values()}
         for c in range(cand.shape[1]):
             d = cand[q, c]
             if d < 0:
                 continue
-            dmap = {int(dix[p]): dv[p] for p in range(dip[d], dip[d + 1])}
             for i in range(qip[q], qip[q + 1]):
-                gq[i] += G[q, c] * dmap.get(int(qix[i]), 0.0)
+                if i in last:
+                    gq[i] += G[q, c] * sum(dv[p] for p in range(dip[d], dip[d + 1]) if int(dix[p]) == int(qix[i]))
             for p in range(dip[d], dip[d + 1]):
-                gd[p] += G[q, c] * qmap.get(int(dix[p]), 0.0)
+                gd[p] += G[q, c] * qmap.get(int(dix[p]), (0.0, -1))[0]
     return gq, gd
 
 
@@ -183,3 +189,48 @@ def test_two_stage_composition(nsparse):
     rec = lambda lab: np.mean([len(set(lab[i]) & set(truth[i])) / 10 for i in range(len(truth))])  # noqa: E731
     print(f"recall@10 two-stage {rec(labels):.3f}, SQ {rec(sq_lab):.3f}", flush=True)
     assert rec(labels) >= rec(sq_lab) - 0.05
+
+
+def test_repeated_query_terms_take_the_last_entry(nsparse):
+    """Queries that repeat terms (with different values) score as the
+    brute-force index scores them (the last entry of a term wins), within
+    1e-5 relative; the gradient of an earlier entry of a repeated term is 0 and
+    that of the last entry is the sum over candidates of g * d_t."""
+    import torch
+
+    from nsparse import synth
+    from nsparse.rescore import rescore
+
+    docs = synth.make_corpus(synth.doc_spec(seed=44), 3000)
+    qip, qix, qv = synth.make_corpus(synth.query_spec(seed=45), 12)
+    rng = np.random.default_rng(46)
+    # every query gets two extra entries repeating its first two terms with new values
+    ip2, ix2, v2 = [0], [], []
+    for q in range(len(qip) - 1):
+        t = list(qix[qip[q]:qip[q + 1]])
+        w = list(qv[qip[q]:qip[q + 1]])
+        t += t[:2]
+        w += list(rng.uniform(0.5, 3.0, 2).astype(np.float32))
+        ix2 += t
+        v2 += w
+        ip2.append(len(ix2))
+    queries = (np.array(ip2, np.int32), np.array(ix2, np.uint16), np.array(v2, np.float32))
+    brute = nsparse.index_factory(DIM, "brutal")
+    brute.add(len(docs[0]) - 1, *docs)
+    bd, cand = brute.search(len(ip2) - 1, *queries, 30)
+    dev = torch.device("cuda")
+    qv_t = torch.from_numpy(queries[2]).to(dev).requires_grad_(True)
+    dv_t = torch.from_numpy(docs[2]).to(dev)
+    S = rescore(qv_t, dv_t, torch.from_numpy(queries[0]).to(dev), torch.from_numpy(queries[1].view(np.int16)).to(dev),
+                torch.from_numpy(cand).to(dev), torch.from_numpy(docs[0]).to(dev),
+                torch.from_numpy(docs[1].view(np.int16)).to(dev), DIM)
+    valid = cand >= 0
+    np.testing.assert_allclose(S.detach().cpu().numpy()[valid], bd[valid], rtol=1e-5, atol=1e-6)
+    G = rng.standard_normal(S.shape).astype(np.float32)
+    S.backward(torch.from_numpy(G).to(dev))
+    gq, _ = cpu_grads(docs, queries, cand, queries[2].astype(np.float64), docs[2].astype(np.float64),
+                      G.astype(np.float64))
+    got = qv_t.grad.cpu().numpy()
+    np.testing.assert_allclose(got, gq, rtol=1e-4, atol=1e-5)
+    for q in range(len(ip2) - 1):
+        assert got[ip2[q]] == 0.0 and got[ip2[q] + 1] == 0.0  # shadowed by the repeats at the end

@@ -304,3 +304,42 @@ def test_waits_between_waves_end_with_a_one_poll_limit(nsparse, data, sq_pair, m
     assert dev.device_error() == 0
     assert np.array_equal(gl, wl) and np.array_equal(gd, wd)
     assert n > 0
+
+
+def test_selector_index_outside_the_bitmap_table(nsparse, data, sq_pair):
+    """A per-query selector index >= the bitmap table's size is rejected by the
+    kernel: error 6, that query's row padded with (-1.0f, -1), the other
+    queries searched as usual (their results equal the oracle's)."""
+    import torch
+
+    o, dev, vmax = sq_pair
+    ip, ix, v = data[1]
+    n = len(ip) - 1
+    n_docs = len(data[0][0]) - 1
+    mask = np.random.default_rng(11).random(n_docs) >= 0.2
+    words = np.packbits(mask.astype(np.uint8), bitorder="little")
+    words = np.pad(words, (0, (-len(words)) % 4)).view(np.uint32)
+    g = torch.device("cuda:0")
+    bm = torch.from_numpy(words.copy()).to(g)
+    table = torch.tensor([bm.data_ptr()], dtype=torch.int64, device=g)
+    slots = np.where(np.arange(n) % 5 == 0, 0, -1).astype(np.int32)
+    slots[3] = 1  # one past the table
+    slots[7] = 1000
+    q = (torch.from_numpy(ip).to(g), torch.from_numpy(ix.view(np.int16)).to(g), torch.from_numpy(v).to(g))
+    p = params(nsparse, 1, 10, 0.8, vmax)
+    gd, gl = dev.search_device(*q, 10, params=p, selector_of_query=torch.from_numpy(slots).to(g), bitmaps=table)
+    gd, gl = gd.cpu().numpy(), gl.cpu().numpy()
+    assert dev.device_error() == 6
+    for r in (3, 7):
+        assert (gl[r] == -1).all() and (gd[r] == -1.0).all()
+    wd, wl = o.search(ip, ix, v, 10, cut=10, hf=0.8)
+    sd, sl = o.search(ip, ix, v, 10, cut=10, hf=0.8, selector=("bitmap", words, n_docs))
+    for r in range(n):
+        if r in (3, 7):
+            continue
+        want = (sd[r], sl[r]) if slots[r] == 0 else (wd[r], wl[r])
+        assert (gl[r] == want[1]).all() and (gd[r] == want[0]).all(), r
+    # a clean launch afterwards resets the flag
+    dev.search_device(*q, 10, params=p)
+    torch.cuda.synchronize()
+    assert dev.device_error() == 0
