@@ -142,6 +142,24 @@ int nsparse_search_device(nsparse_index* index, int64_t n, const int32_t* d_indp
                           const int32_t* d_selector_of_query, const uint32_t* const* d_bitmaps,
                           int n_bitmaps, float* d_distances, int32_t* d_labels, void* stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). nsparse_search_device's search, computing identical results, with the
+ * persistent longest-first schedule; its queue counter, histogram and query order
+ * live in a static device array of the library (about 4 MiB per GPU, held until the
+ * process ends), so the call allocates nothing. Limits: n <= 1048576 queries per
+ * call (larger n: NSPARSE_ERR_INVALID_ARGUMENT); the index must already be resident
+ * on the device (after nsparse_build, or one search of an index read from a file),
+ * else NSPARSE_ERR_INVALID_ARGUMENT. The launch writes its device error code (0 =
+ * none, else the codes nsparse_device_error reports) to d_error (device int32).
+ * Calls that overlap on one GPU share the array and race: serialize them. */
+int nsparse_search_device_noalloc(nsparse_index* index, int64_t n, const int32_t* d_indptr,
+                                  const uint16_t* d_indices, const float* d_values, const int32_t* d_ks,
+                                  int k_max, const nsparse_search_params* params,
+                                  const int32_t* d_selector_of_query, const uint32_t* const* d_bitmaps,
+                                  int n_bitmaps, float* d_distances, int32_t* d_labels, int32_t* d_error,
+                                  void* stream);
+
 /* nsparse_search_device plus per-query traversal counters: d_stats (device,
  * n x 16 int64) receives lists visited, query entries found in summaries,
  * summary entries read, prune tests, clusters opened, doc ids read, rows scored,

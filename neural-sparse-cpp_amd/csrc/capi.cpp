@@ -221,6 +221,19 @@ int nsparse_search_device(nsparse_index* index, int64_t n, const int32_t* d_indp
     });
 }
 
+int nsparse_search_device_noalloc(nsparse_index* index, int64_t n, const int32_t* d_indptr,
+                                  const uint16_t* d_indices, const float* d_values, const int32_t* d_ks, int k_max,
+                                  const nsparse_search_params* params, const int32_t* d_selector_of_query,
+                                  const uint32_t* const* d_bitmaps, int n_bitmaps, float* d_distances,
+                                  int32_t* d_labels, int32_t* d_error, void* stream) {
+    return guarded([&] {
+        SeismicIndex& s = device_search_index(index);
+        s.search_device_noalloc(n, d_indptr, d_indices, d_values, d_ks, k_max, device_request(params),
+                                d_selector_of_query, d_bitmaps, n_bitmaps, d_distances, d_labels, d_error,
+                                static_cast<hipStream_t>(stream));
+    });
+}
+
 struct nsparse_search_plan {
     std::unique_ptr<SearchPlan> impl;
 };

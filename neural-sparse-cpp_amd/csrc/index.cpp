@@ -518,6 +518,26 @@ void SeismicIndex::search_device(int64_t n, const int32_t* d_indptr, const uint1
     launch_search_checked(a, max_clusters_, k_max, sq_, stream);
 }
 
+void SeismicIndex::search_device_noalloc(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices,
+                                         const float* d_values, const int32_t* d_ks, int k_max,
+                                         const SearchRequest& req, const int32_t* d_sel_of_query,
+                                         const uint32_t* const* d_bitmaps, int32_t n_bitmaps, float* d_out_d,
+                                         int32_t* d_out_l, int32_t* d_err, hipStream_t stream) {
+    if (d_err == nullptr) throw std::invalid_argument("the allocation-free search needs a device error flag");
+    if (!has_vectors_ && n > 0 && k_max > 0) return;
+    {
+        // ensure_device (called by search_args) must find nothing to upload
+        std::lock_guard<std::mutex> lock(dev_mu_);
+        if (!dev_fwd_valid_ || (!dev_lists_valid_ && host_lists_valid_) || !d_err_.bytes())
+            throw std::invalid_argument(
+                "the allocation-free search needs the index resident on the device: build it, or search it once");
+    }
+    SearchArgs a = search_args(n, d_indptr, d_indices, d_values, d_ks, k_max, req, d_sel_of_query, d_bitmaps,
+                               n_bitmaps, d_out_d, d_out_l, nullptr);
+    a.error_flag = d_err;
+    launch_search_noalloc(a, max_clusters_, k_max, sq_, stream);
+}
+
 SearchArgs SeismicIndex::search_args(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices,
                                      const float* d_values, const int32_t* d_ks, int k_max, const SearchRequest& req,
                                      const int32_t* d_sel_of_query, const uint32_t* const* d_bitmaps,

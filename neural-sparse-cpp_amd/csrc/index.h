@@ -104,6 +104,14 @@ public:
                        const int32_t* d_ks, int k_max, const SearchRequest& req, const int32_t* d_sel_of_query,
                        const uint32_t* const* d_bitmaps, int32_t n_bitmaps, float* d_out_d, int32_t* d_out_l,
                        int64_t* d_stats, hipStream_t stream, int32_t* d_err = nullptr);
+    // search_device without any device allocation: the index must already be
+    // resident on the device (after build(), or after a first search of an
+    // index read from a file), the error code goes to d_err, and the
+    // schedule's scratch is the library's static device array.
+    void search_device_noalloc(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices, const float* d_values,
+                               const int32_t* d_ks, int k_max, const SearchRequest& req,
+                               const int32_t* d_sel_of_query, const uint32_t* const* d_bitmaps, int32_t n_bitmaps,
+                               float* d_out_d, int32_t* d_out_l, int32_t* d_err, hipStream_t stream);
     // The kernel arguments of a batched search (validated), as search_device uses them.
     SearchArgs search_args(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices, const float* d_values,
                            const int32_t* d_ks, int k_max, const SearchRequest& req, const int32_t* d_sel_of_query,

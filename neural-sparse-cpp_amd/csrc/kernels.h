@@ -64,6 +64,12 @@ size_t search_scratch_bytes(int64_t n);
 void launch_search(const SearchArgs& a, int max_clusters, int kmax, bool sq, const SearchConfig& cfg, void* scratch,
                    hipStream_t stream);
 void launch_search_checked(const SearchArgs& a, int max_clusters, int kmax, bool sq, hipStream_t stream);
+// The batched search with the persistent, longest-first schedule whose queue
+// counter, histogram and query order live in a file-scope device array of the
+// library (no allocation): at most kNoAllocMaxQueries queries per call, and
+// calls that overlap on one GPU race on that array.
+constexpr int64_t kNoAllocMaxQueries = int64_t{1} << 20;
+void launch_search_noalloc(const SearchArgs& a, int max_clusters, int kmax, bool sq, hipStream_t stream);
 void launch_exact_checked(const ExactArgs& a, bool sq, hipStream_t stream);
 int search_lds_bytes(int dim, int max_clusters, int kmax, bool sq);
 std::string device_error_message(int32_t code);

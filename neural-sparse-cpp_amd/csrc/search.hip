@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <limits>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -2189,6 +2190,52 @@ void launch_search_checked(const SearchArgs& args, int max_clusters, int kmax, b
         args.n_queries <= std::max(1, device_cus() / 4))
         c.block = 1024;
     launch_search(args, max_clusters, kmax, sq, c, nullptr, stream);
+}
+
+namespace {
+
+// Scratch of the allocation-free batched search: the persistent schedule's
+// queue counter, its nnz-bucket histogram and the longest-first query order,
+// in the layout launch_search gives a caller's scratch buffer. Every GPU the
+// library runs on holds its own copy until the process ends.
+constexpr int64_t kNoAllocWords = static_cast<int64_t>((16 + 4 * ((kOrderBuckets + 15) & ~15)) / 4) +
+                                  kNoAllocMaxQueries;
+static __device__ __align__(256) int32_t scratch[kNoAllocWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (the array's name is
+// only a host-side stand-in), resolved once per device.
+void* scratch_on_current_device() {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (p == nullptr) throw device_error("hipGetSymbolAddress(scratch) returned no address");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+void launch_search_noalloc(const SearchArgs& a, int max_clusters, int kmax, bool sq, hipStream_t stream) {
+    if (a.n_queries > kNoAllocMaxQueries)
+        throw std::invalid_argument("the allocation-free search takes at most " + std::to_string(kNoAllocMaxQueries) +
+                                    " queries per call");
+    static_assert(static_cast<size_t>(kNoAllocWords) * 4 >= 16 + 4 * ((kOrderBuckets + 15) & ~15) +
+                                                                  4 * static_cast<size_t>(kNoAllocMaxQueries),
+                  "scratch holds the persistent schedule's queue, histogram and order");
+    void* ptr = scratch_on_current_device();
+    SearchConfig c = default_search_config(sq);
+    c.persistent = 1;
+    c.longest_first = 1;
+    launch_search(a, max_clusters, kmax, sq, c, ptr, stream);
 }
 
 void launch_exact_checked(const ExactArgs& a, bool sq, hipStream_t stream) {

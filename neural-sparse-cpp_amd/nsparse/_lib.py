@@ -115,6 +115,9 @@ _sig("nsparse_search", [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_
                         P(SearchParams)])
 _sig("nsparse_search_device", [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                P(SearchParams), c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p])
+_sig("nsparse_search_device_noalloc", [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                       P(SearchParams), c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                       c_void_p])
 _sig("nsparse_search_device_stats", [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                      P(SearchParams), c_void_p, c_void_p, c_void_p, c_void_p])
 _sig("nsparse_device_error", [c_void_p, P(ctypes.c_int32)])

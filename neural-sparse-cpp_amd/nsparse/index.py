@@ -230,6 +230,23 @@ class Index:
                                             s))
         return out_d, out_l
 
+    def search_device_noalloc(self, indptr, indices, values, k_max, out_d, out_l, error, ks=None, params=None,
+                              stream=None, selector_of_query=None, bitmaps=None):
+        """search_device without any device allocation (nsparse_search_device_noalloc):
+        the caller passes the outputs and a one-element int32 device tensor for
+        the error code; the index must already be resident on the device."""
+        n = indptr.numel() - 1
+        ps = ctypes.byref(params._struct()) if params is not None else None
+        s = ctypes.c_void_p(stream.cuda_stream if stream is not None else 0)
+        ksp = ctypes.c_void_p(ks.data_ptr()) if ks is not None else None
+        sel = ctypes.c_void_p(selector_of_query.data_ptr()) if selector_of_query is not None else None
+        bm = ctypes.c_void_p(bitmaps.data_ptr()) if bitmaps is not None else None
+        nb = int(bitmaps.numel()) if bitmaps is not None else 0
+        check(lib.nsparse_search_device_noalloc(self._h, n, indptr.data_ptr(), indices.data_ptr(),
+                                                values.data_ptr(), ksp, int(k_max), ps, sel, bm, nb,
+                                                out_d.data_ptr(), out_l.data_ptr(), error.data_ptr(), s))
+        return out_d, out_l
+
     def search_plan(self, indptr, indices, values, k_max, ks=None, params=None, out_d=None, out_l=None,
                     config=None, stream=None, selector_of_query=None, bitmaps=None):
         """A SearchPlan: the batched search over these GPU tensors captured in a
