@@ -58,15 +58,21 @@ def cpu_model():
     return name, max(len(sockets), 1)
 
 
-def algorithmic_bytes(stats):
-    """Bytes the reference's traversal of one query reads (per row of the device
-    counters), uint8 values: per visited list the binary-search probes over its
-    term ids (q_len * 2 B * ceil(log2 T), counter 8); per query entry found in
-    the summaries its term_ptr pair (8 B) and entries (2 B cluster + 1 B value);
-    per prune test the cluster's offsets pair (8 B); per doc id read 4 B; per row
-    scored its indptr pair (8 B) and nnz * (2 B term + 1 B code)."""
+def algorithmic_bytes(stats, probe_bytes=None):
+    """Bytes one query's traversal reads, per row of traversal counters in the
+    order lists, terms_found, csc_entries, clusters_tested, clusters_opened,
+    docs_read, docs_scored, nnz_scored[, probe_bytes], uint8 values: per
+    visited list the binary-search probes over its term ids (q_len * 2 B *
+    ceil(log2 T)); per query entry found in the summaries its term_ptr pair
+    (8 B) and entries (2 B cluster + 1 B value); per prune test the cluster's
+    offsets pair (8 B); per doc id read 4 B; per row scored its indptr pair
+    (8 B) and nnz * (2 B term + 1 B code). With the oracle's counters (the
+    reference's traversal, whose visited set skips a doc already scored in an
+    earlier list) these are the reference's bytes; with the device's, they
+    include the rows it scores again for such docs."""
     s = stats.astype(np.float64)
-    return (s[:, 8] + 8 * s[:, 1] + 3 * s[:, 2] + 8 * s[:, 3] + 4 * s[:, 5] + 8 * s[:, 6] + 3 * s[:, 7])
+    probe = s[:, 8] if probe_bytes is None else np.asarray(probe_bytes, np.float64)
+    return (probe + 8 * s[:, 1] + 3 * s[:, 2] + 8 * s[:, 3] + 4 * s[:, 5] + 8 * s[:, 6] + 3 * s[:, 7])
 
 
 def serving_bench(args, nsparse, index, n_docs, threads):
@@ -132,15 +138,18 @@ def measured_traffic(workload):
 
 
 def build_bytes(n_docs, nnz, st, v=1):
-    """Bytes the reference's build moves, uint8 values (v = 1): the forward
-    index read once for list construction ((N + 1) * 4 + nnz * (2 + v)), the
-    postings written and read back (2 * nnz * (4 + v)), and every kept
+    """Bytes the reference's build reads and writes, uint8 values (v = 1): the
+    forward index read once for list construction ((N + 1) * 4 + nnz * (2 +
+    v)), the postings written and read back (2 * nnz * (4 + v)), every kept
     posting's doc row gathered twice, for the assignment and the summaries
-    (2 * (8 * kept + row_nnz * (2 + v)))."""
+    (2 * (8 * kept + row_nnz * (2 + v))), and one centroid-index entry (2 B
+    cluster id + v) per (doc term, centroid posting) pair the assignment
+    scores (kmeans_utils.cpp:107-156). Returns (total, the pairs' share)."""
     fwd = (n_docs + 1) * 4 + nnz * (2 + v)
     postings = 2 * nnz * (4 + v)
     rows = 2 * (8 * st["kept_postings"] + st["kept_row_nnz"] * (2 + v))
-    return float(fwd + postings + rows)
+    pairs = st["assign_pairs"] * (2 + v)
+    return float(fwd + postings + rows + pairs), float(pairs)
 
 
 def copy_bandwidth(dev, nbytes=2 << 30, reps=5):
@@ -299,6 +308,113 @@ def exact_truth(args, nsparse, index, docs, queries, qix, qv, vmin, vmax, n_rec,
     return truth, truth_sq
 
 
+def fp32_truth(nsparse, docs, queries, n_rec, dev, stream):
+    """Exact FP32 top-10 (score desc, id asc, positive scores) of the first
+    n_rec queries on the unquantized corpus, scored on the GPU."""
+    import ctypes
+
+    import torch
+
+    n_docs = len(docs[0]) - 1
+    d_ip = torch.from_numpy(docs[0]).to(dev)
+    d_ix = torch.from_numpy(docs[1].view(np.int16)).to(dev)
+    d_v = torch.from_numpy(docs[2]).to(dev)
+    q_ix = torch.from_numpy(queries[1].view(np.int16)).to(dev)
+    q_v = torch.from_numpy(queries[2]).to(dev)
+    scores = torch.empty(n_docs, dtype=torch.float32, device=dev)
+    truth = []
+    for i in range(n_rec):
+        a, b = int(queries[0][i]), int(queries[0][i + 1])
+        nsparse._lib.check(nsparse._lib.lib.nsparse_exact_scores_device(
+            n_docs, d_ip.data_ptr(), d_ix.data_ptr(), d_v.data_ptr(), 30522, b - a,
+            ctypes.c_void_p(q_ix.data_ptr() + 2 * a), ctypes.c_void_p(q_v.data_ptr() + 4 * a), scores.data_ptr(),
+            ctypes.c_void_p(stream.cuda_stream)))
+        top_v, top_i = torch.topk(scores, 64)
+        tv, ti = top_v.cpu().numpy(), top_i.cpu().numpy().astype(np.int64)
+        keep = tv > 0
+        tv, ti = tv[keep], ti[keep]
+        truth.append(set(ti[np.lexsort((ti, -tv))[:10]].tolist()))
+    return truth
+
+
+def c1_reference_order(args, nsparse, threads, dev, stream):
+    """The 1M-doc SEISMIC-SQ configuration (lambda 6000, beta 600, alpha 0.4;
+    queries of nnz lognormal median 40, k = 10, cut and heap_factor as the
+    benchmark's): the index built in the reference's own tie order by the
+    oracle (its bytes are the reference build's) on all host threads -- the
+    CPU build baseline -- beside the GPU build of the same corpus. Then
+    recall@10 against exact FP32 brute force for the reference path (the
+    oracle's traversal with libstdc++ ties) on the reference-order index, for
+    the device on that same index (read from its bytes), and for the device on
+    its own build (ties broken by id)."""
+    if args.c1_docs <= 0 or args.no_cpu:
+        return None
+    import torch
+
+    from nsparse import synth
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle
+
+    n, nq = args.c1_docs, args.c1_queries
+    docs = synth.make_corpus(synth.doc_spec(seed=0), n, threads=threads)
+    queries = synth.make_corpus(synth.query_spec(seed=1), nq, threads=threads)
+    vmin, vmax = float(docs[2].min()), float(docs[2].max())
+    desc = (f"seismic_sq,quantizer=8bit|vmin={vmin:.9g}|vmax={vmax:.9g}|lambda={args.lam}|beta={args.beta}|"
+            f"alpha={args.alpha}|seed=0")
+    gpu = nsparse.index_factory(30522, desc)
+    gpu.add(n, *docs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gpu.build()
+    torch.cuda.synchronize()
+    gpu_build_s = time.perf_counter() - t0
+    o = _oracle.OracleIndex(30522, 1, args.lam, args.beta, args.alpha, 0, np.float32(vmin), np.float32(vmax),
+                            tie_mode=0)
+    o.add(*docs)
+    t0 = time.perf_counter()
+    o.build(threads=threads)
+    cpu_build_s = time.perf_counter() - t0
+    log(f"C1 {n} docs: GPU build {gpu_build_s:.2f}s, oracle build in the reference's tie order "
+        f"{cpu_build_s:.1f}s on {threads} threads ({cpu_build_s / gpu_build_s:.0f}x)")
+    blob = o.serialize()
+    ref_dev = nsparse.read_index_bytes(blob)
+    del blob
+    qip, qix, qv = queries
+    p = nsparse.SeismicSearchParameters(args.cut, args.hf)
+    d_rd, l_rd = ref_dev.search(nq, qip, qix, qv, 10, p)
+    d_g, l_g = gpu.search(nq, qip, qix, qv, 10, p)
+    d_c, l_c = o.search(qip, qix, qv, 10, cut=args.cut, hf=args.hf, threads=threads)
+    o1 = _oracle.OracleIndex(30522, 1, args.lam, args.beta, args.alpha, 0, np.float32(vmin), np.float32(vmax),
+                             tie_mode=1)
+    o1.copy_from(o)
+    d_1, l_1 = o1.search(qip, qix, qv, 10, cut=args.cut, hf=args.hf, threads=threads)
+    exact = float(np.mean((l_rd == l_1).all(axis=1) & (d_rd == d_1).all(axis=1)))
+    up_to_ties = float(np.mean([same_topk_up_to_ties(d_c[i], l_c[i], d_rd[i], l_rd[i]) for i in range(nq)]))
+    n_rec = min(args.c1_recall_queries, nq)
+    truth = fp32_truth(nsparse, docs, queries, n_rec, dev, stream)
+
+    def recall(lab):
+        hits = [len(set(int(x) for x in lab[i] if x >= 0) & t) / len(t) for i, t in enumerate(truth) if t]
+        return float(np.mean(hits))
+
+    r_ref, r_rd, r_g = recall(l_c), recall(l_rd), recall(l_g)
+    log(f"C1 recall@10 over {n_rec} queries: reference path {r_ref:.4f}, GPU on the reference-order index "
+        f"{r_rd:.4f} ({r_rd - r_ref:+.4f}), GPU on its own build {r_g:.4f} ({r_g - r_ref:+.4f}); GPU rows equal "
+        f"to the oracle (id ties) {exact:.4f}, to the reference path up to ties {up_to_ties:.4f}")
+    del gpu, ref_dev, o, o1
+    return {"docs": n, "queries": nq, "k": 10, "cut": args.cut, "heap_factor": args.hf,
+            "gpu_build_s": gpu_build_s,
+            "cpu_build_baseline": {"value": cpu_build_s, "unit": "s", "cores": threads, "kind": "port",
+                                   "sample": f"the whole {n}-doc build, the oracle in the reference's tie order "
+                                             "(its write_index bytes are the reference build's), OpenMP over "
+                                             "posting lists"},
+            "recall_queries": n_rec, "recall_reference_path": r_ref,
+            "recall_gpu_on_reference_order_index": r_rd, "recall_gpu_on_own_build": r_g,
+            "gpu_minus_reference_same_index": r_rd - r_ref, "gpu_own_build_minus_reference": r_g - r_ref,
+            "gpu_rows_equal_oracle_id_ties": exact, "gpu_rows_equal_reference_up_to_ties": up_to_ties}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -329,6 +445,10 @@ def main():
                     help="two-stage operating point: the SQ search's heap_factor")
     ap.add_argument("--inverted-queries", type=int, default=300,
                     help="exact InvertedIndex on the same corpus: drop-in search of this many queries (0 skips)")
+    ap.add_argument("--c1-docs", type=int, default=1_000_000,
+                    help="reference-order build at this size: CPU build baseline and recall parity (0 skips)")
+    ap.add_argument("--c1-queries", type=int, default=10_000)
+    ap.add_argument("--c1-recall-queries", type=int, default=1000)
     ap.add_argument("--schedule", default="auto",
                     help='"auto" (autotuner) or a JSON search config, e.g. {"lookup":1,"block":512,"persistent":1}')
     args = ap.parse_args()
@@ -362,9 +482,10 @@ def main():
     torch.cuda.synchronize()
     build_s = time.time() - t0
     bst = index.build_stats()
-    b_bytes = build_bytes(args.docs, int(docs[0][-1]), bst)
-    build = dict(bst, seconds=build_s, algorithmic_bytes=b_bytes, gbps=b_bytes / build_s / 1e9,
-                 frac=b_bytes / build_s / 1e9 / HBM_PEAK_GBS, assign_pairs_per_s=bst["assign_pairs"] / build_s)
+    b_bytes, pair_bytes = build_bytes(args.docs, int(docs[0][-1]), bst)
+    build = dict(bst, seconds=build_s, algorithmic_bytes=b_bytes, assign_pair_bytes=pair_bytes,
+                 gbps=b_bytes / build_s / 1e9, frac=b_bytes / build_s / 1e9 / HBM_PEAK_GBS,
+                 assign_pairs_per_s=bst["assign_pairs"] / build_s)
     log(f"add {add_s:.1f}s, GPU build {build_s:.1f}s: {b_bytes / 1e9:.1f} GB algorithmic "
         f"({build['gbps']:.1f} GB/s), {bst['assign_pairs'] / 1e9:.2f}G assignment pairs "
         f"({build['assign_pairs_per_s'] / 1e9:.1f}G/s), {bst['kept_postings']:,} kept postings, "
@@ -423,13 +544,14 @@ def main():
     index.search_device(qip, qix, qv, k_max, ks=d_ks, params=params, stats=stats, stream=stream)
     torch.cuda.synchronize()
     st = stats.cpu().numpy()
-    bytes_launch = float(algorithmic_bytes(st).sum())
+    dev_bytes = algorithmic_bytes(st)  # per query, by the device's own counters
+    bytes_launch = float(dev_bytes.sum())
     achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
     tick_names = ["setup", "summary", "order", "plan", "gather", "score", "replay"]
     tick_means = st[:, 9:16].mean(axis=0) * 10.0 / 1e3  # 100 MHz device wall clock -> us
     log("per-query phase time (us, one workgroup): " + ", ".join(f"{n}={v:.0f}" for n, v in zip(tick_names,
                                                                                                tick_means)))
-    log(f"algorithmic {bytes_launch / 1e9:.3f} GB/launch -> {achieved:.0f} GB/s; per-query means "
+    log(f"device-counted {bytes_launch / 1e9:.3f} GB/launch -> {achieved:.0f} GB/s; per-query means "
         + ", ".join(f"{n}={v:.0f}" for n, v in zip(["lists", "found", "csc", "tested", "opened", "read", "scored",
                                                       "nnz", "probe_bytes"], st.mean(axis=0))))
 
@@ -567,8 +689,10 @@ def main():
         ref = _oracle.OracleIndex.load_from_index(nsparse, index, tie_mode=0)
         log(f"oracle copy of the index loaded in {time.time() - t1:.1f}s")
         cpu_labels = np.full((n_rec, 10), -1, np.int64)  # the CPU's top-10 of the recall queries
+        ostats = np.zeros((args.queries, 8), np.int64)  # the reference traversal's counters per query
+        covered = np.zeros(args.queries, bool)
 
-        def cpu_run(n_threads, seconds, chunk):
+        def cpu_run(n_threads, seconds, chunk, keep_stats=False):
             """Searches the benchmark queries in order, chunk by chunk, until
             `seconds` of search time; returns (queries, seconds, agree, total)."""
             done, t_cpu, agree, total = 0, 0.0, 0, 0
@@ -582,9 +706,12 @@ def main():
                     vals = [queries[2][queries[0][j]:queries[0][j + 1]] for j in sel]
                     sip = np.r_[0, np.cumsum([len(r) for r in rows])].astype(np.int32)
                     t1 = time.perf_counter()
-                    cd, cl = ref.search(sip, np.concatenate(rows), np.concatenate(vals), k, cut=args.cut,
-                                        hf=args.hf, threads=n_threads)
+                    cd, cl, cs = ref.search(sip, np.concatenate(rows), np.concatenate(vals), k, cut=args.cut,
+                                            hf=args.hf, threads=n_threads, with_stats=True)
                     t_cpu += time.perf_counter() - t1
+                    if keep_stats:
+                        ostats[sel] = cs
+                        covered[sel] = True
                     for r, j in enumerate(sel):
                         total += 1
                         agree += int(same_topk_up_to_ties(cd[r], cl[r], dists[j, :k], labels[j, :k]))
@@ -595,7 +722,7 @@ def main():
 
         # each call carries 128 queries per thread (per k), so OpenMP's
         # dynamic(64) chunks keep every thread busy
-        done, t_cpu, agree, total = cpu_run(threads, args.cpu_seconds, 256 * threads)
+        done, t_cpu, agree, total = cpu_run(threads, args.cpu_seconds, 256 * threads, keep_stats=True)
         done1, t_cpu1, _, _ = cpu_run(1, args.cpu_seconds / 3, 8)
         model, sockets = cpu_model()
         simd = "avx512" if _oracle.lib.oracle_set_simd(1) else "generic"
@@ -613,7 +740,27 @@ def main():
             f"{cpu['single_thread_qps']:,.1f} QPS on 1 thread ({done1} queries); {model}, {sockets} socket(s), "
             f"{simd} u8 dot; CPU recall@10 {cpu['recall_at_10']}")
 
+    # the roofline's bytes are the reference's: its traversal's counters from the
+    # oracle over the same queries (the device re-scores a doc it already saw in
+    # an earlier list; the reference's visited set skips it); queries the CPU
+    # leg did not reach are scaled by the covered queries' ratio
+    ref_bytes_src = "device counters (no CPU leg)"
+    ref_bytes_launch = bytes_launch
+    if cpu is not None and covered.any():
+        ref_q = algorithmic_bytes(ostats[covered], probe_bytes=st[covered, 8])
+        ratio = float(ref_q.sum() / dev_bytes[covered].sum())
+        ref_bytes_launch = float(ref_q.sum() + ratio * dev_bytes[~covered].sum())
+        ref_bytes_src = (f"oracle traversal counters over {int(covered.sum())} of {args.queries} queries"
+                         + (f", the rest scaled by their ratio {ratio:.4f}" if (~covered).any() else ""))
+        mismatch = int((ostats[covered, :6] != st[covered, :6]).any(axis=1).sum())
+        ref_bytes_src += f"; queries whose first six counters differ from the device's: {mismatch}"
+    achieved_dev = achieved
+    achieved = ref_bytes_launch / (kernel_ms * 1e-3) / 1e9
+    log(f"reference bytes {ref_bytes_launch / 1e9:.3f} GB/launch ({ref_bytes_src}) -> {achieved:.0f} GB/s, "
+        f"{achieved / HBM_PEAK_GBS:.3f} of peak; device re-scored share "
+        f"{1 - ref_bytes_launch / bytes_launch:.4f}")
     traffic, traffic_src = measured_traffic(f"seismic_sq_batched_{args.docs}docs/{args.queries}q")
+    c1 = c1_reference_order(args, nsparse, threads, dev, stream)
     copy_gbs = copy_bandwidth(dev)
     log(f"device copy (2 GiB): {copy_gbs:,.0f} GB/s read+write; search achieves {achieved / copy_gbs:.3f} of it")
     result = {
@@ -629,7 +776,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "bytes_per_launch": bytes_launch, "kernel_ms": kernel_ms,
+                     "bytes_per_launch": ref_bytes_launch, "bytes_source": ref_bytes_src,
+                     "device_bytes_per_launch": bytes_launch, "device_achieved": achieved_dev,
+                     "device_rescored_share": 1 - ref_bytes_launch / bytes_launch,
+                     "traffic_over_bytes": (traffic / ref_bytes_launch) if traffic else None,
+                     "kernel_ms": kernel_ms,
                      "copy_gbps": copy_gbs, "frac_of_copy": achieved / copy_gbs},
         "cpu_baseline": cpu,
         "plain": {"qps": plain_qps if n_plain else None, "p50_ms": pct(0.5), "p99_ms": pct(0.99),
@@ -658,6 +809,10 @@ def main():
         result["inverted"] = inverted
     if rescored:
         result["two_stage_fp32_rescore"] = rescored
+    if c1:
+        result["c1_reference_order"] = c1
+        build["cpu_baseline_c1"] = c1["cpu_build_baseline"]
+        build["gpu_c1_s"] = c1["gpu_build_s"]
     print(json.dumps(result), flush=True)
 
 

@@ -32,6 +32,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <unordered_set>
 #include <utility>
 #include <vector>
@@ -501,24 +502,53 @@ void score_summaries(const ListClusters& lc, const term_t* qi, const QT* qv, siz
     }
 }
 
+// id_selector.h:22-86; kind 4 is the IDMapIndex wrapper of id_map_index.h:27-75
+// (IDSelectorWithIDMap / IDSelectorEnumerableWithIDMap): membership of an
+// internal id is the inner selector's on its external id; an enumerable inner
+// selector stays enumerable, with the inner size() and, as ids(), the internal
+// ids of the external ids the map knows.
 struct Selector {
-    int kind = -1;  // -1 none, 0 array, 1 set, 2 not, 3 bitmap
+    int kind = -1;  // -1 none, 0 array, 1 set, 2 not, 3 bitmap, 4 idmap
     std::vector<idx_t> ids;
     std::unordered_set<idx_t> set;
     const uint32_t* bitmap = nullptr;
     int64_t nbits = 0;
     const Selector* inner = nullptr;
+    const std::vector<idx_t>* i2e = nullptr;                  // kind 4
+    const std::unordered_map<idx_t, idx_t>* e2i = nullptr;    // kind 4
     bool is_member(idx_t id) const {
         switch (kind) {
             case 0: return std::find(ids.begin(), ids.end(), id) != ids.end();
             case 1: return set.count(id) != 0;
             case 2: return !inner->is_member(id);
             case 3: return id >= 0 && id < nbits && ((bitmap[id >> 5] >> (id & 31)) & 1U);
+            case 4:
+                // the reference indexes its map unchecked; a doc without an
+                // external id is outside what it defines
+                if (id < 0 || static_cast<size_t>(id) >= i2e->size())
+                    throw std::out_of_range("doc without an external id under an IDMap selector");
+                return inner->is_member((*i2e)[id]);
             default: return true;
         }
     }
-    bool enumerable() const { return kind == 0 || kind == 1; }
-    size_t size() const { return kind == 0 ? ids.size() : set.size(); }
+    bool enumerable() const { return kind == 0 || kind == 1 || (kind == 4 && inner->enumerable()); }
+    size_t size() const { return kind == 0 ? ids.size() : kind == 1 ? set.size() : inner->size(); }
+    // IDSelectorEnumerable::ordered_ids (id_selector.h:33-37): ids(), sorted
+    std::vector<idx_t> ordered_ids() const {
+        std::vector<idx_t> out;
+        if (kind == 0) {
+            out = ids;
+        } else if (kind == 1) {
+            out.assign(set.begin(), set.end());
+        } else if (kind == 4) {
+            for (idx_t e : inner->ordered_ids()) {
+                auto it = e2i->find(e);
+                if (it != e2i->end()) out.push_back(it->second);
+            }
+        }
+        std::ranges::sort(out);
+        return out;
+    }
 };
 
 template <class TopK>
@@ -689,8 +719,7 @@ void single_query(const Index& ix, const std::vector<VT>& dense, const term_t* q
 // exact_matcher.h:23-41
 template <class TopK>
 void exact_query(const Index& ix, const Selector& sel, const std::vector<uint8_t>& qbytes_dense, int k, TopK& h) {
-    std::vector<idx_t> ids = sel.kind == 0 ? sel.ids : std::vector<idx_t>(sel.set.begin(), sel.set.end());
-    std::ranges::sort(ids);
+    const std::vector<idx_t> ids = sel.ordered_ids();
     const Fwd& f = ix.fwd;
     for (idx_t d : ids) {
         if (d < 0 || static_cast<size_t>(d) >= f.n()) throw std::invalid_argument("selector id out of range");
@@ -1150,6 +1179,114 @@ int oracle_search(void* h, int n, const idx_t* indptr, const term_t* indices, co
             search_impl<HeapTopK<idx_t>>(*ix, n, indptr, indices, values, k, cut, hf, qq, sp, D, L, st, n_threads);
         else
             search_impl<IdTopK<idx_t>>(*ix, n, indptr, indices, values, k, cut, hf, qq, sp, D, L, st, n_threads);
+    });
+}
+
+// IDMapIndex::search (id_map_index.cpp:29-60) over this index as the
+// delegate: i2e[internal id] = external id (add_with_ids order,
+// id_map_index.cpp:63-75; a repeated external id maps to its last doc). The
+// selector, given on external ids, is wrapped as the reference wraps it, and
+// every label >= 0 is mapped to its external id.
+int oracle_search_idmap(void* h, const idx_t* i2e, int64_t n_map, int n, const idx_t* indptr, const term_t* indices,
+                        const float* values, int k, int cut, float hf, int qsq, float qvmin, float qvmax,
+                        int sel_kind, int64_t sel_n, const idx_t* sel_ids, const uint32_t* sel_bitmap,
+                        int sel_negate, float* D, idx_t* L, int n_threads) {
+    return guard([&] {
+        auto* ix = static_cast<Index*>(h);
+        if (n <= 0 || k <= 0) throw std::invalid_argument("value must be positive");
+        if (!ix->has_vectors) return;
+        Quantizer qq = ix->q;
+        if (ix->sq && qsq) qq = Quantizer(qvmin, qvmax);
+        const std::vector<idx_t> map(i2e, i2e + n_map);
+        std::unordered_map<idx_t, idx_t> rev;
+        for (int64_t i = 0; i < n_map; ++i) rev[map[i]] = static_cast<idx_t>(i);
+        Selector inner_not;
+        Selector user = make_selector(sel_kind, sel_n, sel_ids, sel_bitmap, sel_negate, inner_not);
+        Selector wrap;
+        const Selector* sp = nullptr;
+        if (user.kind >= 0) {
+            wrap.kind = 4;
+            wrap.inner = &user;
+            wrap.i2e = &map;
+            wrap.e2i = &rev;
+            sp = &wrap;
+        }
+        if (ix->tie_mode == 0)
+            search_impl<HeapTopK<idx_t>>(*ix, n, indptr, indices, values, k, cut, hf, qq, sp, D, L, nullptr,
+                                         n_threads);
+        else
+            search_impl<IdTopK<idx_t>>(*ix, n, indptr, indices, values, k, cut, hf, qq, sp, D, L, nullptr,
+                                       n_threads);
+        for (int64_t i = 0; i < static_cast<int64_t>(n) * k; ++i) {
+            if (L[i] < 0) continue;
+            if (static_cast<size_t>(L[i]) >= map.size()) throw std::out_of_range("result doc without an external id");
+            L[i] = map[static_cast<size_t>(L[i])];
+        }
+    });
+}
+
+// Summary scores of the oracle's lists, as the search computes them
+// (score_summaries, restating inverted_list_clusters.cpp:295-331): query q, all
+// its entries in CSR order (SQ: codes of the index's quantizer), against list
+// lists[q]. out holds cap floats per query; out_n[q] receives the cluster count.
+int oracle_score_summaries(void* h, int nq, const idx_t* q_indptr, const term_t* q_indices, const float* q_values,
+                           const int32_t* lists, int cap, float* out, int32_t* out_n) {
+    return guard([&] {
+        auto* ix = static_cast<Index*>(h);
+        std::vector<float> scores;
+        for (int qi = 0; qi < nq; ++qi) {
+            const idx_t s0 = q_indptr[qi];
+            const size_t len = static_cast<size_t>(q_indptr[qi + 1] - s0);
+            const ListClusters& lc = ix->lists.at(static_cast<size_t>(lists[qi]));
+            Stats st{};
+            if (ix->sq) {
+                std::vector<uint8_t> codes(len);
+                for (size_t i = 0; i < len; ++i) codes[i] = ix->q.encode(q_values[s0 + i]);
+                score_summaries<uint8_t>(lc, q_indices + s0, codes.data(), len, scores, st);
+            } else {
+                score_summaries<float>(lc, q_indices + s0, q_values + s0, len, scores, st);
+            }
+            out_n[qi] = static_cast<int32_t>(scores.size());
+            for (size_t c = 0; c < scores.size() && c < static_cast<size_t>(cap); ++c)
+                out[static_cast<size_t>(qi) * cap + c] = scores[c];
+        }
+    });
+}
+
+// Replaces the index's posting lists with hand-made ones (known-answer tests):
+// list l has nc[l] clusters; cluster c (numbered across all lists) holds
+// cdocs[c] doc ids from `docs` and a summary of cterms[c] (term, value) pairs
+// from sum_terms / sum_vals (terms ascending; SQ values are codes 0..255). The
+// term-major transpose is built as build_transpose does.
+int oracle_set_lists(void* h, int n_lists, const int32_t* nc, const int32_t* cdocs, const idx_t* docs,
+                     const int32_t* cterms, const term_t* sum_terms, const float* sum_vals) {
+    return guard([&] {
+        auto* ix = static_cast<Index*>(h);
+        std::vector<ListClusters> out(static_cast<size_t>(n_lists));
+        size_t c = 0, d = 0, t = 0;
+        for (int l = 0; l < n_lists; ++l) {
+            ListClusters& lc = out[l];
+            lc.esz = ix->fwd.esz;
+            std::vector<std::vector<term_t>> st;
+            std::vector<std::vector<float>> svf;
+            std::vector<std::vector<uint8_t>> svq;
+            if (nc[l] > 0) lc.offsets.push_back(0);
+            for (int j = 0; j < nc[l]; ++j, ++c) {
+                lc.docs.insert(lc.docs.end(), docs + d, docs + d + cdocs[c]);
+                d += static_cast<size_t>(cdocs[c]);
+                lc.offsets.push_back(static_cast<idx_t>(lc.docs.size()));
+                st.emplace_back(sum_terms + t, sum_terms + t + cterms[c]);
+                svf.emplace_back(sum_vals + t, sum_vals + t + cterms[c]);
+                svq.emplace_back();
+                for (int e = 0; e < cterms[c]; ++e) svq.back().push_back(static_cast<uint8_t>(sum_vals[t + e]));
+                t += static_cast<size_t>(cterms[c]);
+            }
+            if (ix->sq)
+                build_transpose<uint8_t>(lc, st, svq);
+            else
+                build_transpose<float>(lc, st, svf);
+        }
+        ix->lists = std::move(out);
     });
 }
 

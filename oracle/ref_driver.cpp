@@ -1,8 +1,9 @@
 // ORACLE (test infrastructure): a thin C entry layer over the reference's own
 // build-side sources (cluster/, invlists/, sparse_vectors.cpp, io/), compiled
 // straight from /root/reference into oracle/_ref/ by oracle/Makefile. Used only
-// by tests/ to check the restatement in oracle/oracle.cpp. It calls the
-// reference's functions; the only code here is argument plumbing plus the
+// by tests/ to check the restatement in oracle/oracle.cpp (build bytes, summary
+// scores, SQ codes, dot products). It calls the reference's functions; the
+// only code here is argument plumbing plus the
 // per-list loop of build_inverted_lists_clusters (seismic_common.h:146-184) and
 // the header writes of write_index (io/index_io.cpp:88-95,
 // seismic_scalar_quantized_index.cpp:421-428), which live in files that pull in
@@ -11,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -36,6 +38,47 @@ int guard(F&& f) {
         return 1;
     }
 }
+
+// The reference's forward index and per-term InvertedListClusters, built with
+// final lambda and beta (already resolved): the per-list loop of
+// build_inverted_lists_clusters (seismic_common.h:146-184).
+struct RefBuilt {
+    SparseVectors vectors;
+    ScalarQuantizer q;
+    std::vector<InvertedListClusters> lists;
+};
+
+std::unique_ptr<RefBuilt> ref_build(int dim, int sq, float vmin, float vmax, int n, const int32_t* indptr,
+                                    const uint16_t* indices, const float* values, int lambda, int beta, float alpha,
+                                    uint32_t base_seed) {
+    const size_t esz = sq ? 1 : 4;
+    const size_t nnz = indptr[n];
+    auto b = std::unique_ptr<RefBuilt>(new RefBuilt{
+        SparseVectors({.element_size = esz, .dimension = static_cast<size_t>(dim)}),
+        ScalarQuantizer(QuantizerType::QT_8bit, sq ? vmin : 0.0F, sq ? vmax : 1.0F), {}});
+    std::vector<uint8_t> codes;
+    if (sq) {
+        codes.resize(nnz);
+        b->q.encode(values, codes.data(), nnz);
+    } else {
+        codes.resize(nnz * 4);
+        std::memcpy(codes.data(), values, nnz * 4);
+    }
+    b->vectors.add_vectors(indptr, n + 1, indices, nnz, codes.data(), codes.size());
+    auto inv = ArrayInvertedLists::build_inverted_lists(dim, esz, &b->vectors);
+    b->lists.resize(inv->size());
+    for (size_t idx = 0; idx < inv->size(); ++idx) {
+        auto& il = (*inv)[idx];
+        const auto doc_ids = il.prune_and_keep_doc_ids(lambda);
+        InvertedListClusters c(
+            detail::RandomKMeans::train(&b->vectors, doc_ids, beta, base_seed + static_cast<uint32_t>(idx)));
+        c.summarize(&b->vectors, alpha);
+        b->lists[idx] = std::move(c);
+        il.clear();
+    }
+    return b;
+}
+
 }  // namespace
 
 extern "C" {
@@ -48,32 +91,10 @@ int ref_build_serialize(int dim, int sq, float vmin, float vmax, int n, const in
                         const uint16_t* indices, const float* values, int lambda, int beta, float alpha,
                         uint32_t base_seed, uint8_t** out, size_t* out_size) {
     return guard([&] {
-        const size_t esz = sq ? 1 : 4;
-        const size_t nnz = indptr[n];
-        SparseVectors vectors({.element_size = esz, .dimension = static_cast<size_t>(dim)});
-        ScalarQuantizer q(QuantizerType::QT_8bit, sq ? vmin : 0.0F, sq ? vmax : 1.0F);
-        std::vector<uint8_t> codes;
-        if (sq) {
-            codes.resize(nnz);
-            q.encode(values, codes.data(), nnz);
-        } else {
-            codes.resize(nnz * 4);
-            std::memcpy(codes.data(), values, nnz * 4);
-        }
-        vectors.add_vectors(indptr, n + 1, indices, nnz, codes.data(), codes.size());
-
-        auto inv = ArrayInvertedLists::build_inverted_lists(dim, esz, &vectors);
-        std::vector<InvertedListClusters> lists(inv->size());
-        for (size_t idx = 0; idx < inv->size(); ++idx) {
-            auto& il = (*inv)[idx];
-            const auto doc_ids = il.prune_and_keep_doc_ids(lambda);
-            InvertedListClusters c(
-                detail::RandomKMeans::train(&vectors, doc_ids, beta, base_seed + static_cast<uint32_t>(idx)));
-            c.summarize(&vectors, alpha);
-            lists[idx] = std::move(c);
-            il.clear();
-        }
-
+        auto b = ref_build(dim, sq, vmin, vmax, n, indptr, indices, values, lambda, beta, alpha, base_seed);
+        const SparseVectors& vectors = b->vectors;
+        const ScalarQuantizer& q = b->q;
+        const std::vector<InvertedListClusters>& lists = b->lists;
         BufferedIOWriter w;
         const char* id = sq ? "SESQ" : "SEIS";
         uint32_t fcc = static_cast<uint32_t>(id[0]) | static_cast<uint32_t>(id[1]) << 8 |
@@ -99,6 +120,39 @@ int ref_build_serialize(int dim, int sq, float vmin, float vmax, int n, const in
 }
 
 void ref_buffer_free(uint8_t* p) { std::free(p); }
+
+// InvertedListClusters::score_summaries_transposed (inverted_list_clusters.cpp:295-331)
+// of the reference's own build: query q (all its entries, in CSR order) against
+// the summaries of list lists[q]. As the reference's search does, SQ queries go
+// in as codes of the index's quantizer (seismic_scalar_quantized_index.cpp:
+// 182-191, 87-88), FP32 queries as their float bytes. out holds cap floats per
+// query; out_n[q] receives the list's cluster count (scores past cap are dropped).
+int ref_score_summaries(int dim, int sq, float vmin, float vmax, int n, const int32_t* indptr,
+                        const uint16_t* indices, const float* values, int lambda, int beta, float alpha,
+                        uint32_t base_seed, int nq, const int32_t* q_indptr, const uint16_t* q_indices,
+                        const float* q_values, const int32_t* lists, int cap, float* out, int32_t* out_n) {
+    return guard([&] {
+        auto b = ref_build(dim, sq, vmin, vmax, n, indptr, indices, values, lambda, beta, alpha, base_seed);
+        std::vector<float> scores;
+        for (int qi = 0; qi < nq; ++qi) {
+            const int32_t s0 = q_indptr[qi];
+            const size_t len = static_cast<size_t>(q_indptr[qi + 1] - s0);
+            std::vector<uint8_t> qbytes;
+            if (sq) {
+                qbytes.resize(len);
+                b->q.encode(q_values + s0, qbytes.data(), len);
+            } else {
+                qbytes.resize(len * 4);
+                std::memcpy(qbytes.data(), q_values + s0, len * 4);
+            }
+            const auto& lc = b->lists.at(static_cast<size_t>(lists[qi]));
+            lc.score_summaries_transposed(q_indices + s0, qbytes.data(), len, scores);
+            out_n[qi] = static_cast<int32_t>(scores.size());
+            for (size_t c = 0; c < scores.size() && c < static_cast<size_t>(cap); ++c)
+                out[static_cast<size_t>(qi) * cap + c] = scores[c];
+        }
+    });
+}
 
 // The posting lists and per-term maxima of InvertedIndex::build
 // (inverted_index.cpp:236-252), made by the reference's own

@@ -57,7 +57,7 @@ class OracleIndex:
                                ctypes.byref(self.h)))
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:
             lib.oracle_free(self.h)
 
     def add(self, ip, ix, v):
@@ -96,6 +96,65 @@ class OracleIndex:
             return self
         finally:
             nsparse._lib.lib.nsparse_buffer_free(buf)
+
+    def search_idmap(self, i2e, ip, ix, v, k, cut=10, hf=1.0, sq_range=None, selector=None, negate=False,
+                     threads=8):
+        """IDMapIndex::search with this index as the delegate: i2e[internal] =
+        external id; the selector is on external ids; labels come back external."""
+        ip = np.ascontiguousarray(ip, np.int32)
+        ix = np.ascontiguousarray(ix, np.uint16)
+        v = np.ascontiguousarray(v, np.float32)
+        i2e = np.ascontiguousarray(i2e, np.int32)
+        n = len(ip) - 1
+        d = np.empty((n, k), np.float32)
+        lab = np.empty((n, k), np.int32)
+        kind, sn, ids, bm = -1, 0, None, None
+        if selector is not None:
+            if selector[0] in ("array", "set"):
+                kind = 0 if selector[0] == "array" else 1
+                ids = np.ascontiguousarray(selector[1], np.int32)
+                sn = len(ids)
+            else:
+                kind = 3
+                bm = np.ascontiguousarray(selector[1], np.uint32)
+                sn = int(selector[2])
+        qsq = 1 if sq_range is not None else 0
+        lo, hi = sq_range if sq_range is not None else (0.0, 1.0)
+        _chk(lib.oracle_search_idmap(self.h, _p(i2e), ctypes.c_int64(len(i2e)), n, _p(ip), _p(ix), _p(v), k, cut,
+                                     f32(hf), qsq, f32(lo), f32(hi), kind, ctypes.c_int64(sn),
+                                     _p(ids) if ids is not None else None, _p(bm) if bm is not None else None,
+                                     int(negate), _p(d), _p(lab), threads))
+        return d, lab
+
+    def score_summaries(self, ip, ix, v, lists, cap=4096):
+        """Summary scores of query q against list lists[q] (all its entries, SQ
+        as codes): a list of float32 arrays, one per query."""
+        ip = np.ascontiguousarray(ip, np.int32)
+        ix = np.ascontiguousarray(ix, np.uint16)
+        v = np.ascontiguousarray(v, np.float32)
+        lists = np.ascontiguousarray(lists, np.int32)
+        nq = len(ip) - 1
+        out = np.zeros((nq, cap), np.float32)
+        n = np.zeros(nq, np.int32)
+        _chk(lib.oracle_score_summaries(self.h, nq, _p(ip), _p(ix), _p(v), _p(lists), cap, _p(out), _p(n)))
+        return [out[q, : n[q]].copy() for q in range(nq)]
+
+    def set_lists(self, lists):
+        """Hand-made posting lists: lists[l] is a list of clusters, each a pair
+        (doc ids, {term: summary value}); SQ summary values are codes."""
+        nc, cdocs, docs, cterms, terms, vals = [], [], [], [], [], []
+        for clusters in lists:
+            nc.append(len(clusters))
+            for ds, summ in clusters:
+                cdocs.append(len(ds))
+                docs.extend(ds)
+                cterms.append(len(summ))
+                for t in sorted(summ):
+                    terms.append(t)
+                    vals.append(summ[t])
+        a = [np.ascontiguousarray(x, dt) for x, dt in ((nc, np.int32), (cdocs, np.int32), (docs, np.int32),
+                                                        (cterms, np.int32), (terms, np.uint16), (vals, np.float32))]
+        _chk(lib.oracle_set_lists(self.h, len(lists), *[_p(x) for x in a]))
 
     def copy_from(self, other):
         _chk(lib.oracle_copy_index(self.h, other.h))
@@ -137,6 +196,42 @@ class OracleIndex:
         return (d, lab, stats) if with_stats else (d, lab)
 
 
+def search_mixed(o, queries, ks, k_max, cut, hf, sel_of_query=None, bitmaps=(), n_docs=0, threads=8):
+    """The oracle's results for a batch with a k per query and, optionally, a
+    bitmap selector per query (sel_of_query[i] indexes `bitmaps`, uint32 words
+    over n_docs ids; -1 = none), laid out as the device's batched search writes
+    them: row i holds ks[i] results, padded with (-1.0, -1) to k_max."""
+    ip, ix, v = (np.asarray(a) for a in queries)
+    n = len(ip) - 1
+    ks = np.asarray(ks, np.int32)
+    sel = np.full(n, -1, np.int32) if sel_of_query is None else np.asarray(sel_of_query, np.int32)
+    D = np.full((n, k_max), -1.0, np.float32)
+    L = np.full((n, k_max), -1, np.int32)
+    for k, s in sorted({(int(a), int(b)) for a, b in zip(ks, sel)}):
+        rows = np.nonzero((ks == k) & (sel == s))[0]
+        lens = ip[rows + 1] - ip[rows]
+        sip = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        take = np.concatenate([np.arange(ip[r], ip[r + 1]) for r in rows]) if len(rows) else np.zeros(0, int)
+        selector = None if s < 0 else ("bitmap", bitmaps[s], n_docs)
+        d, lab = o.search(sip, ix[take], v[take], k, cut=cut, hf=hf, selector=selector, threads=threads)
+        D[rows, :k] = d
+        L[rows, :k] = lab
+    return D, L
+
+
+def same_up_to_ties(d_ref, l_ref, d_got, l_got):
+    """One result row against another, apart from exact score ties: the same
+    scores in the same order, the same doc ids at every score above the k-th,
+    and padding in the same places (docs tied with the k-th may be any of the
+    tied ones, and equal scores may list their docs in any order)."""
+    if not np.array_equal(d_ref, d_got):
+        return False
+    above = d_ref > d_ref[-1]
+    if not np.array_equal(np.sort(l_ref[above]), np.sort(l_got[above])):
+        return False
+    return bool(np.array_equal(l_ref < 0, l_got < 0))
+
+
 def ref_build_bytes(dim, sq, ip, ix, v, lam, beta, alpha, seed, vmin=0.0, vmax=1.0):
     """write_index bytes produced by the reference's own build-side code."""
     ip = np.ascontiguousarray(ip, np.int32)
@@ -151,6 +246,23 @@ def ref_build_bytes(dim, sq, ip, ix, v, lam, beta, alpha, seed, vmin=0.0, vmax=1
     out = ctypes.string_at(buf, sz.value)
     ref.ref_buffer_free(buf)
     return out
+
+
+def ref_score_summaries(dim, sq, docs, lam, beta, alpha, seed, queries, lists, vmin=0.0, vmax=1.0, cap=4096):
+    """Summary scores from the reference's own InvertedListClusters::
+    score_summaries_transposed over its own build (see OracleIndex.score_summaries)."""
+    ip, ix, v = (np.ascontiguousarray(a, dt) for a, dt in zip(docs, (np.int32, np.uint16, np.float32)))
+    qip, qix, qv = (np.ascontiguousarray(a, dt) for a, dt in zip(queries, (np.int32, np.uint16, np.float32)))
+    lists = np.ascontiguousarray(lists, np.int32)
+    nq = len(qip) - 1
+    out = np.zeros((nq, cap), np.float32)
+    n = np.zeros(nq, np.int32)
+    rc = ref.ref_score_summaries(dim, int(sq), f32(vmin), f32(vmax), len(ip) - 1, _p(ip), _p(ix), _p(v), lam, beta,
+                                 f32(alpha), ctypes.c_uint32(seed), nq, _p(qip), _p(qix), _p(qv), _p(lists), cap,
+                                 _p(out), _p(n))
+    if rc != 0:
+        raise RuntimeError(ref.ref_last_error().decode())
+    return [out[q, : n[q]].copy() for q in range(nq)]
 
 
 def cluster_order(keys):

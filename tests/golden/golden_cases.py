@@ -1,5 +1,7 @@
 """The cases behind tests/golden/ref_fixtures.json: seeded corpora (numpy PCG64,
 stable across numpy releases) and the parameters each was built with."""
+import hashlib
+
 import numpy as np
 
 
@@ -58,3 +60,38 @@ def sq_values():
     rng = np.random.default_rng(0)
     return np.concatenate([rng.lognormal(0, 1.5, 20000), [0.0, -1.0, 1e9, 3.0, 0.5, 2.5],
                            np.float32(np.arange(0, 256) / 255.0 * 3.0)]).astype(np.float32)
+
+
+def summary_pairs(case, n_queries=12, lists_per_query=5):
+    """(query, posting list) pairs for summary scoring on a build case: seeded
+    queries over the case's vocabulary; each query is paired with the lists of
+    its lists_per_query largest-weight terms and of its first term. Query 0
+    repeats its first term at the end with another weight (a repeated entry
+    adds again in score_summaries). Returns the queries one row per pair (CSR)
+    and the list of each row."""
+    qip, qix, qv = corpus(n_queries, case["dim"], max(4, case["nnz"] // 2), seed=case["corpus_seed"] + 1)
+    rows, lists = [], []
+    for q in range(n_queries):
+        t = qix[qip[q]:qip[q + 1]].copy()
+        w = qv[qip[q]:qip[q + 1]].copy()
+        if q == 0:
+            t = np.append(t, t[0]).astype(np.uint16)
+            w = np.append(w, np.float32(2.5) * w[0]).astype(np.float32)
+        picks = list(t[np.argsort(-w, kind="stable")[:lists_per_query]]) + [t[0]]
+        for term in picks:
+            rows.append((t, w))
+            lists.append(int(term))
+    ip = np.zeros(len(rows) + 1, np.int32)
+    ip[1:] = np.cumsum([len(r[0]) for r in rows])
+    ix = np.concatenate([r[0] for r in rows]).astype(np.uint16)
+    v = np.concatenate([r[1] for r in rows]).astype(np.float32)
+    return ip, ix, v, np.array(lists, np.int32)
+
+
+def summary_digest(scores):
+    """SHA-256 over every pair's cluster count (int32) and scores (float32 bits)."""
+    h = hashlib.sha256()
+    for sc in scores:
+        h.update(np.int32(len(sc)).tobytes())
+        h.update(np.ascontiguousarray(sc, np.float32).tobytes())
+    return h.hexdigest()

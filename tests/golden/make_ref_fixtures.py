@@ -10,7 +10,11 @@ The fixture holds data only:
     the reference from seeded generator corpora (the inputs are regenerated
     from the seeds by the test), for the cases of test_oracle.py;
   * the same for the posting lists + per-term maxima of InvertedIndex::build;
-  * 8-bit SQ codes of fixed values under three ranges, and decoded scores.
+  * 8-bit SQ codes of fixed values under three ranges, and decoded scores;
+  * per build case, the summary scores InvertedListClusters::
+    score_summaries_transposed gives seeded queries against their lists
+    (golden_cases.summary_pairs): SHA-256 of every pair's cluster count
+    (int32) and scores (float32 bits), and the first pair's scores.
 tests/test_golden.py checks the oracle against it without the reference.
 """
 import ctypes
@@ -25,7 +29,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 
 import _oracle as O  # noqa: E402
-from golden_cases import BUILD_CASES, INVERTED_CASES, SQ_RANGES, sq_values, build_corpus, inverted_corpus  # noqa: E402
+from golden_cases import (BUILD_CASES, INVERTED_CASES, SQ_RANGES, build_corpus, inverted_corpus,  # noqa: E402
+                          sq_values, summary_digest, summary_pairs)
 
 
 def digest(b):
@@ -35,13 +40,22 @@ def digest(b):
 def main():
     if O.ref is None:
         raise SystemExit("oracle/_ref/libnsparse_ref.so is not built here")
-    out = {"build": [], "inverted": [], "sq_encode": [], "sq_decode": []}
+    out = {"build": [], "inverted": [], "sq_encode": [], "sq_decode": [], "summaries": []}
     for case in BUILD_CASES:
         ip, ix, v = build_corpus(case)
         vmin, vmax = float(v.min()), float(v.max())
         b = O.ref_build_bytes(case["dim"], case["sq"], ip, ix, v, case["lam_resolved"], case["beta_resolved"],
                               case["alpha"], case["seed"], vmin, vmax)
         out["build"].append(dict(case, **digest(b)))
+    for case in BUILD_CASES:
+        ip, ix, v = build_corpus(case)
+        vmin, vmax = float(v.min()), float(v.max())
+        qip, qix, qv, lists = summary_pairs(case)
+        sc = O.ref_score_summaries(case["dim"], case["sq"], (ip, ix, v), case["lam_resolved"],
+                                   case["beta_resolved"], case["alpha"], case["seed"], (qip, qix, qv), lists, vmin,
+                                   vmax)
+        out["summaries"].append({"pairs": len(lists), "sha256": summary_digest(sc),
+                                 "first_bits": sc[0].view(np.uint32).tolist()})
     for case in INVERTED_CASES:
         ip, ix, v = inverted_corpus(case)
         out["inverted"].append(dict(case, **digest(O.ref_inverted_bytes(case["dim"], ip, ix, v))))

@@ -5,7 +5,8 @@ so the oracle stays pinned where the reference tree is absent.
     tie mode for FP32, whose corpora here have no value ties that reach the
     layout -- the oracle's bytes hash to the reference's;
   * InvertedIndex lists + maxima, byte for byte;
-  * 8-bit SQ codes and decoded scores."""
+  * 8-bit SQ codes and decoded scores;
+  * summary scores (score_summaries_transposed) of seeded query / list pairs."""
 import ctypes
 import hashlib
 import json
@@ -19,7 +20,8 @@ import _oracle as O
 from _oracle import OracleIndex, OracleInverted
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
-from golden_cases import BUILD_CASES, INVERTED_CASES, SQ_RANGES, build_corpus, inverted_corpus, sq_values  # noqa: E402
+from golden_cases import (BUILD_CASES, INVERTED_CASES, SQ_RANGES, build_corpus, inverted_corpus,  # noqa: E402
+                          sq_values, summary_digest, summary_pairs)
 
 with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_fixtures.json")) as _f:
     FIX = json.load(_f)
@@ -82,3 +84,21 @@ def test_sq_codes_and_decode_match_reference():
     for fx in FIX["sq_decode"]:
         d = O.lib.oracle_sq_decode(*fx["ingest"], *fx["query"], fx["score"])
         assert int(np.float32(d).view(np.uint32)) == fx["decoded_bits"]
+
+
+@pytest.mark.parametrize("idx", range(len(BUILD_CASES)), ids=[_case_id(c) for c in BUILD_CASES])
+def test_summary_scores_hash_to_reference(idx):
+    """The oracle's summary scores (libstdc++ tie mode build, which is the
+    reference's layout) for every (query, list) pair hash to the reference's."""
+    case, fx = BUILD_CASES[idx], FIX["summaries"][idx]
+    ip, ix, v = build_corpus(case)
+    vmin, vmax = float(v.min()), float(v.max())
+    o = OracleIndex(case["dim"], case["sq"], case["lam"], case["beta"], case["alpha"], case["seed"], vmin, vmax,
+                    tie_mode=0)
+    o.add(ip, ix, v)
+    o.build()
+    qip, qix, qv, lists = summary_pairs(case)
+    assert len(lists) == fx["pairs"]
+    sc = o.score_summaries(qip, qix, qv, lists)
+    assert sc[0].view(np.uint32).tolist() == fx["first_bits"]
+    assert summary_digest(sc) == fx["sha256"]

@@ -1,9 +1,12 @@
 """Batched search plans (hipGraph replay + autotuned schedule): every schedule
-must give exactly the results of nsparse_search_device, on every replay, for
-mixed k and per-query bitmap selectors; query buffers rewritten in place between
-replays must be searched afresh."""
+must give exactly the oracle's results on the same index (scores bit for bit,
+labels in the id tie mode), on every replay, for mixed k and per-query bitmap
+selectors -- as must nsparse_search_device; query buffers rewritten in place
+between replays must be searched afresh."""
 import numpy as np
 import pytest
+
+import _oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -40,11 +43,15 @@ def setup(nsparse):
     table = torch.tensor([bm.data_ptr()], dtype=torch.int64, device=dev)
     sel = torch.from_numpy(np.where(np.arange(n) % 7 == 0, 0, -1).astype(np.int32)).to(dev)
     params = nsparse.SeismicSearchParameters(10, 0.8)
-    want = idx.search_device(q["ip"], q["ix"], q["v"], 100, ks=ks, params=params, selector_of_query=sel,
-                             bitmaps=table)
+    o = O.OracleIndex.load_from_index(nsparse, idx, tie_mode=1)
+    want = O.search_mixed(o, (qip, qix, qv), ks.cpu().numpy(), 100, 10, 0.8, sel.cpu().numpy(), [words],
+                          len(ip) - 1)
+    dev_d, dev_l = idx.search_device(q["ip"], q["ix"], q["v"], 100, ks=ks, params=params, selector_of_query=sel,
+                                     bitmaps=table)
     torch.cuda.synchronize()
-    keep = (bm,)
-    return idx, q, ks, sel, table, params, [t.cpu().numpy() for t in want], keep
+    assert np.array_equal(dev_l.cpu().numpy(), want[1]) and np.array_equal(dev_d.cpu().numpy(), want[0])
+    keep = (bm, o, (qip, qix, qv))
+    return idx, q, ks, sel, table, params, list(want), keep
 
 
 def same(got, want):
@@ -90,16 +97,17 @@ def test_autotuned_plan(setup):
 def test_rewritten_queries_are_searched_afresh(setup):
     import torch
 
-    idx, q, ks, sel, table, params, want, _ = setup
+    idx, q, ks, sel, table, params, want, keep = setup
+    o, (qip, qix, qv) = keep[1], keep[2]
     v2 = q["v"].clone()
     plan = idx.search_plan(q["ip"], q["ix"], v2, 100, ks=ks, params=params,
                            config=dict(lookup=1, block=512, persistent=1, longest_first=1))
-    base = [t.cpu().numpy() for t in idx.search_device(q["ip"], q["ix"], q["v"], 100, ks=ks, params=params)]
+    base = O.search_mixed(o, (qip, qix, qv), ks.cpu().numpy(), 100, 10, 0.8)
     same(plan.launch(), base)
     # new weights in the same buffer: reversed order within the batch
     v2.copy_(torch.flip(q["v"], dims=[0]))
-    fresh = [t.cpu().numpy() for t in idx.search_device(q["ip"], q["ix"], v2, 100, ks=ks, params=params)]
     torch.cuda.synchronize()
+    fresh = O.search_mixed(o, (qip, qix, qv[::-1].copy()), ks.cpu().numpy(), 100, 10, 0.8)
     same(plan.launch(), fresh)
     plan.free()
 
@@ -119,7 +127,8 @@ def test_fp32_plan(nsparse):
     b = torch.from_numpy(qix.view(np.int16)).to(dev)
     c = torch.from_numpy(qv).to(dev)
     p = nsparse.SeismicSearchParameters(10, 0.9)
-    want = [t.cpu().numpy() for t in idx.search_device(a, b, c, 10, params=p)]
+    want = O.OracleIndex.load_from_index(nsparse, idx, tie_mode=1).search(qip, qix, qv, 10, cut=10, hf=0.9)
+    same(idx.search_device(a, b, c, 10, params=p), want)
     for cfg in (None, dict(lookup=1, block=512, persistent=1, longest_first=1)):
         plan = idx.search_plan(a, b, c, 10, params=p, config=cfg)
         assert plan.config["lookup"] == 0 and plan.config["block"] == 256

@@ -234,32 +234,53 @@ def test_batched_device_mixed_k_equals_single(nsparse, data, sq_pair):
         assert st[q, 6] >= ws[q, 6]
 
 
-def test_idmap_search(nsparse):
-    """IDMapIndex over a GPU-built SEISMIC-SQ index: labels come back external,
-    selectors are evaluated on external ids (id_map_index.cpp:29-60)."""
+@pytest.mark.parametrize("sq", [1, 0])
+def test_idmap_search_matches_oracle(nsparse, sq):
+    """IDMapIndex over a GPU-built SEISMIC index against the oracle's IDMapIndex
+    (id_map_index.cpp:29-75) on the same index: labels come back external,
+    selectors (Set, Array, NotIDSelector, bitmap over external ids, and small
+    enumerable selectors with unknown external ids that take the exact path)
+    are evaluated on external ids; scores and labels identical."""
     from nsparse import synth
 
-    ip, ix, v = synth.make_corpus(synth.doc_spec(seed=3), 3000)
-    ids = (np.arange(3000, dtype=np.int32) * 7 + 100)
-    im = nsparse.index_factory(DIM, f"idmap,seismic_sq,quantizer=8bit|vmin=0|vmax={float(v.max()):.9g}|"
-                                    "lambda=200|beta=20|alpha=0.4|seed=0")
-    im.add_with_ids(3000, ip, ix, v, ids)
+    n = 3000
+    ip, ix, v = synth.make_corpus(synth.doc_spec(seed=3), n)
+    vmax = float(v.max())
+    ids = (np.random.default_rng(5).permutation(n).astype(np.int32) * 7 + 100)
+    spec = (f"seismic_sq,quantizer=8bit|vmin=0|vmax={vmax:.9g}|lambda=200|beta=20|alpha=0.4|seed=0" if sq
+            else "seismic,lambda=200|beta=20|alpha=0.4|seed=0")
+    im = nsparse.index_factory(DIM, "idmap," + spec)
+    im.add_with_ids(n, ip, ix, v, ids)
     im.build()
-    plain = nsparse.index_factory(DIM, f"seismic_sq,quantizer=8bit|vmin=0|vmax={float(v.max()):.9g}|"
-                                       "lambda=200|beta=20|alpha=0.4|seed=0")
-    plain.add(3000, ip, ix, v)
-    plain.build()
+    o = O.OracleIndex(DIM, sq, 200, 20, 0.4, 0, 0.0, vmax, tie_mode=1)
+    o.add(ip, ix, v)
+    o.build()
     qip, qix, qv = synth.make_corpus(synth.query_spec(seed=4), 30)
-    p = nsparse.SeismicSearchParameters(10, 0.9)
-    d1, l1 = im.search(30, qip, qix, qv, 10, p)
-    d0, l0 = plain.search(30, qip, qix, qv, 10, p)
-    assert (d1 == d0).all()
-    assert (l1 == np.where(l0 >= 0, l0 * 7 + 100, -1)).all()
-    # selector on external ids
-    allowed_ext = ids[::3]
-    p.set_id_selector(nsparse.SetIDSelector(allowed_ext))
-    d2, l2 = im.search(30, qip, qix, qv, 10, p)
-    assert np.isin(l2[l2 >= 0], allowed_ext).all()
+    rng = np.random.default_rng(6)
+    some = np.sort(rng.choice(ids, n // 3, replace=False)).astype(np.int32)
+    banned = np.sort(rng.choice(ids, n // 5, replace=False)).astype(np.int32)
+    nbits = int(ids.max()) + 1
+    mask = np.zeros(nbits, bool)
+    mask[some] = True
+    bm = nsparse.BitmapIDSelector(mask)
+    small = np.array([ids[17], ids[2999], 5, ids[400]], np.int32)  # 5 is no external id
+    cases = [
+        (None, None, 10),
+        (nsparse.SetIDSelector(some), ("set", some), 10),
+        (nsparse.ArrayIDSelector(some), ("array", some), 10),
+        (nsparse.NotIDSelector(nsparse.SetIDSelector(banned)), ("set", banned), 10),
+        (bm, ("bitmap", bm.words, nbits), 10),
+        (nsparse.SetIDSelector(small), ("set", small), 10),   # size 4 <= k: exact path
+        (nsparse.ArrayIDSelector(small), ("array", small), 3),  # size 4 > k: filtered traversal
+    ]
+    for dev_sel, o_sel, k in cases:
+        p = nsparse.SeismicSearchParameters(10, 0.9)
+        if dev_sel is not None:
+            p.set_id_selector(dev_sel)
+        got = im.search(30, qip, qix, qv, k, p)
+        neg = dev_sel is not None and isinstance(dev_sel, nsparse.NotIDSelector)
+        want = o.search_idmap(ids, qip, qix, qv, k, cut=10, hf=0.9, selector=o_sel, negate=neg)
+        assert_same(got, want)
 
 
 def test_sq_search_with_and_without_block_rows(nsparse, data, monkeypatch):
@@ -343,3 +364,42 @@ def test_selector_index_outside_the_bitmap_table(nsparse, data, sq_pair):
     dev.search_device(*q, 10, params=p)
     torch.cuda.synchronize()
     assert dev.device_error() == 0
+
+
+@pytest.mark.parametrize("name", sorted(__import__("_hand_cases").CASES))
+def test_hand_derived_traversal_on_device(nsparse, name):
+    """The hand-made indexes of tests/_hand_cases.py (results derived by hand
+    from the reference's traversal), written by the oracle and read by the
+    device: the device returns the hand-derived labels and scores (id tie
+    mode)."""
+    from _hand_cases import CASES, build_oracle, query_arrays
+
+    case = CASES[name]
+    dev = nsparse.read_index_bytes(build_oracle(case, 1).serialize())
+    d, lab = dev.search(1, *query_arrays(case), case["k"], nsparse.SeismicSearchParameters(case["cut"], case["hf"]))
+    want_l, want_d = case["want"][1]
+    assert list(lab[0]) == want_l
+    assert list(d[0]) == want_d
+
+
+@pytest.mark.parametrize("sq", [1, 0])
+def test_reference_order_index(nsparse, data, sq):
+    """An index in the reference's own tie order (the oracle's libstdc++ tie
+    mode, whose write_index bytes are the reference build's) read by the
+    device: its results equal the oracle's id-tie search on those bytes
+    exactly, and the reference path's (libstdc++ result ties) up to exact
+    score ties."""
+    ip, ix, v = data[0]
+    vmax = float(v.max())
+    o0 = O.OracleIndex(DIM, sq, 300, 30, 0.4, 0, 0.0, vmax, tie_mode=0)
+    o0.add(ip, ix, v)
+    o0.build()
+    blob = o0.serialize()
+    dev = nsparse.read_index_bytes(blob)
+    o1 = O.OracleIndex.load(blob, tie_mode=1)
+    for qs in (data[1], data[2]):
+        got = dev.search(len(qs[0]) - 1, *qs, 10, params(nsparse, sq, 10, 0.8, vmax))
+        assert_same(got, o1.search(*qs, 10, cut=10, hf=0.8))
+        d0, l0 = o0.search(*qs, 10, cut=10, hf=0.8)
+        for r in range(len(d0)):
+            assert O.same_up_to_ties(d0[r], l0[r], got[0][r], got[1][r]), r

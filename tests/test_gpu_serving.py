@@ -1,9 +1,13 @@
 """Serving: continuous batching over a device-resident index. Every request's
-result must equal the search of that query alone (nsparse_search_device with
-n = 1, same k and selector), whatever batch and stream it landed in; a tight
-scratch budget must hold requests back without losing any."""
+result must equal the oracle's search of that query on the same index (same k
+and selector; scores bit for bit, labels in the id tie mode) and the device's
+search of that query alone (nsparse_search_device with n = 1), whatever batch
+and stream it landed in; a tight scratch budget must hold requests back
+without losing any."""
 import numpy as np
 import pytest
+
+import _oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -20,7 +24,7 @@ def served(nsparse):
                                      "alpha=0.4|seed=4")
     idx.add(len(ip) - 1, ip, ix, v)
     idx.build()
-    return idx, len(ip) - 1
+    return idx, len(ip) - 1, O.OracleIndex.load_from_index(nsparse, idx, tie_mode=1)
 
 
 def bitmap_words(n_docs, seed):
@@ -50,7 +54,7 @@ def alone(nsparse, idx, qi, qv, k, words):
 def test_each_result_is_the_same_alone_and_in_the_mix(nsparse, served):
     from nsparse import synth
 
-    idx, n_docs = served
+    idx, n_docs, o = served
     qip, qix, qv = synth.make_corpus(synth.query_spec(seed=13, log_uniform=True), 400)
     n = len(qip) - 1
     rng = np.random.default_rng(14)
@@ -69,6 +73,8 @@ def test_each_result_is_the_same_alone_and_in_the_mix(nsparse, served):
     assert st["completed"] == n and st["batches"] > 1
     assert st["max_inflight_bytes"] <= 3 * 24 * per_req
     assert (lat > 0).all()
+    od, ol = O.search_mixed(o, (qip, qix, qv), ks, 100, 10, 0.8, bids, [w for w, _ in pool], n_docs)
+    assert np.array_equal(lab, ol) and np.array_equal(d, od)
     for i in range(n):
         a, b = qip[i], qip[i + 1]
         k = int(ks[i])
@@ -84,8 +90,9 @@ def test_each_result_is_the_same_alone_and_in_the_mix(nsparse, served):
 def test_submit_wait_and_budget_backpressure(nsparse, served):
     from nsparse import synth
 
-    idx, _ = served
+    idx, _, o = served
     qip, qix, qv = synth.make_corpus(synth.query_spec(seed=15, log_uniform=True), 120)
+    od, ol = o.search(qip, qix, qv, 10, cut=10, hf=0.8)
     per_req = 512 * 6 + 12 + 8 * 10
     # room for exactly two full-length requests in flight
     srv = nsparse.Server(idx, max_batch=64, streams=4, scratch_budget=2 * per_req, k_max=10, cut=10,
@@ -95,6 +102,7 @@ def test_submit_wait_and_budget_backpressure(nsparse, served):
         d, lab, us = srv.wait(t, 10)
         wd, wl = alone(nsparse, idx, qix[qip[i]:qip[i + 1]], qv[qip[i]:qip[i + 1]], 10, None)
         assert (lab == wl).all() and (d == wd).all() and us > 0
+        assert (lab == ol[i]).all() and (d == od[i]).all()
     st = srv.stats()
     srv.close()
     assert st["completed"] == len(tickets)
@@ -103,7 +111,7 @@ def test_submit_wait_and_budget_backpressure(nsparse, served):
 
 
 def test_bad_requests_are_rejected(nsparse, served):
-    idx, _ = served
+    idx, _, _ = served
     srv = nsparse.Server(idx, max_batch=8, streams=1, k_max=10)
     with pytest.raises(ValueError):
         srv.submit(np.array([1], np.uint16), np.array([1.0], np.float32), 11)  # k above k_max

@@ -14,10 +14,14 @@ What pins each part of the oracle:
     exact brute force on a tiny corpus (heap_factor so large nothing is
     pruned => SEISMIC must equal brute force over the visited lists), and the
     reference python tests' recall floor on a 2000-doc corpus.
-The search traversal itself (cluster order ties, first-list break, heap
-mechanics) has no compiled reference here (its sources need Abseil): beyond the
-cases above it is "parity unpinned" against the reference and rests on the
-oracle using the very libstdc++ routines the reference uses.
+  * summary scoring: bit for bit against the reference's compiled
+    InvertedListClusters::score_summaries_transposed (and its golden digests).
+  * traversal (first-list break, later-list continue, first_list after an
+    empty list and after an out-of-range term, cluster-order ties, the
+    result-tie rule, repeated query terms): hand-derived known answers,
+    tests/test_oracle_traversal.py.
+The traversal's own source needs Abseil, so it is not compiled here; the
+known answers above pin each of its branches instead.
 """
 import numpy as np
 import pytest
