@@ -927,55 +927,82 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                     const uint2* val8 = BLK ? reinterpret_cast<const uint2*>(ix.blk_codes) + lwb
                                             : reinterpret_cast<const uint2*>(ix.values);
                     const int64_t zero_word = BLK ? ix.blk_words - lwb : 0;
+                    // every kS1-th word start, read once per chunk: the first
+                    // ballot of each window's doc search needs no LDS read
+                    const int wb1 = lane * kS1 < total ? wbase[lane * kS1] : INT32_MAX;
                     while (true) {
                         int xs = 0;
                         if (lane == 0) xs = atomicAdd(&C.meta[kChunkNext], kWinPerStep);
                         xs = __shfl(xs, 0, 64);
                         if (xs >= nwin) break;
                         int jj[kWinPerStep], j1s[kWinPerStep], lo[kWinPerStep], hi[kWinPerStep];
+                        int sbs[kWinPerStep], bns[kWinPerStep];
                         int64_t ww[kWinPerStep];
                         uint4 ii[kWinPerStep];
                         uint2 vv[kWinPerStep];
-                        // 1. each lane's doc and word address
+                        // 1. each lane's doc and word address. The windows of a
+                        // step are independent, so each phase issues its LDS
+                        // reads for all of them before any result is used; a
+                        // window past the last one repeats the last window and
+                        // is masked out (no word, no sums).
+                        // 1a. the starts after every kS1-th one <= the window's first word
 #pragma unroll
                         for (int u = 0; u < kWinPerStep; ++u) {
-                            const int x = xs + u;
-                            jj[u] = -1;
-                            j1s[u] = -1;
+                            const int F = min(xs + u, nwin - 1) << 6;
+                            sbs[u] = (__popcll(ballot(wb1 <= F)) - 1) * kS1;
+                            bns[u] = wbase[sbs[u] + lane];  // sentinels cover sb + 63
+                        }
+                        // 1b. the window's first doc, then the doc boundaries inside it
+#pragma unroll
+                        for (int u = 0; u < kWinPerStep; ++u) {
+                            const bool valid = xs + u < nwin;  // wave-uniform
+                            const int F = min(xs + u, nwin - 1) << 6, f = F + lane;
+                            const int sb = sbs[u], bn0 = bns[u];
+                            // the last j < total with wbase[j] <= F, among sb .. sb + kS1 - 1
+                            const uint64_t m2 = ballot(lane < kS1 && sb + lane < total && bn0 <= F);
+                            int j = sb + __popcll(m2) - 1;
+                            // boundaries after j: lanes j - sb + 1 .. 63 of bn0 hold
+                            // wbase[j + 1 .. sb + 63]; a window with more starts
+                            // than that reads on
+                            const int r0 = j - sb + 1;
+                            const uint64_t inside = ballot(lane >= r0 && bn0 <= F + 63);
+                            const int nb0 = __popcll(inside);
+                            for (int i = 0; i < nb0; ++i) j += f >= __builtin_amdgcn_readlane(bn0, r0 + i);
+                            if (r0 + nb0 == 64) {
+                                for (int base = sb + 64;; base += 64) {
+                                    const int bn = wbase[base + lane];
+                                    const uint64_t bm = ballot(bn <= F + 63);
+                                    const int nb = __popcll(bm);
+                                    for (int i = 0; i < nb; ++i) j += f >= __builtin_amdgcn_readlane(bn, i);
+                                    if (nb < 64) break;
+                                }
+                            }
+                            // the doc of word f + 1, from the next lane (a DPP
+                            // wavefront shift left by one lane; lane 63 reads 0)
+                            const int j1 = __builtin_amdgcn_update_dpp(0, j, 0x130, 0xF, 0xF, false);
+                            const bool has = valid && f < W;
+                            jj[u] = has ? j : -1;
+                            // run end (the next word is another doc's, or the window
+                            // ends): j1s = the doc whose run follows in this window,
+                            // INT32_MAX when none does, -1 when not a run end
+                            const bool run_end = j1 != j || lane == 63;
+                            j1s[u] = !has || !run_end ? -1 : (j1 != j && f + 1 < W && lane != 63) ? j1 : INT32_MAX;
+                        }
+                        // 1c. row references of the lanes' docs
+#pragma unroll
+                        for (int u = 0; u < kWinPerStep; ++u) {
+                            const int F = min(xs + u, nwin - 1) << 6, f = F + lane;
+                            const int j = jj[u];
                             ww[u] = zero_word;
                             lo[u] = 0;
                             hi[u] = 8;
-                            if (x >= nwin) continue;  // wave-uniform
-                            const int F = x << 6, f = F + lane;
-                            // the window's first doc: the last j < total with wbase[j] <= F
-                            const int i1 = lane * kS1;
-                            const uint64_t m1 = ballot(i1 < total && wbase[i1] <= F);
-                            const int sb = (__popcll(m1) - 1) * kS1;
-                            const int i2 = sb + lane;
-                            const uint64_t m2 = ballot(lane < kS1 && i2 < total && wbase[i2] <= F);
-                            int j = sb + __popcll(m2) - 1, j1 = j;
-                            for (int base = j + 1;; base += 64) {
-                                const int bn = wbase[base + lane];
-                                const uint64_t bm = ballot(bn <= F + 63);
-                                const int nb = __popcll(bm);
-                                for (int i = 0; i < nb; ++i) {
-                                    const int B = __builtin_amdgcn_readlane(bn, i);
-                                    j += f >= B;
-                                    j1 += f + 1 >= B;
-                                }
-                                if (nb < 64) break;
-                            }
-                            if (f < W) {
-                                jj[u] = j;
-                                // run end (the next word is another doc's, or the window
-                                // ends): j1s = the doc whose run follows in this window,
-                                // INT32_MAX when none does, -1 when not a run end
-                                const bool run_end = j1 != j || lane == 63;
-                                j1s[u] = !run_end ? -1 : (j1 != j && f + 1 < W && lane != 63) ? j1 : INT32_MAX;
-                                if constexpr (BLK) {
-                                    ww[u] = f + C.rs[j];
-                                } else {
-                                    const int rs = C.rs[j], re = rs + C.len[j], wb = wbase[j];
+                            if constexpr (BLK) {
+                                const int r = C.rs[max(j, 0)];
+                                if (j >= 0) ww[u] = f + r;
+                            } else {
+                                const int jc = max(j, 0);
+                                const int rs = C.rs[jc], re = rs + C.len[jc], wb = wbase[jc];
+                                if (j >= 0) {
                                     const int w = (rs >> 3) + (f - wb);
                                     ww[u] = w;
                                     lo[u] = max(rs - (w << 3), 0);
@@ -989,10 +1016,13 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                             ii[u] = idx8[ww[u]];
                             vv[u] = val8[ww[u]];
                         }
-                        // 3. scores
+                        // 3. scores (a masked window scores the zero word), then
+                        // the windows' prefix sums, then their atomics: three
+                        // passes, so the windows' LDS reads and DPP chains
+                        // interleave
+                        uint32_t accs[kWinPerStep];
 #pragma unroll
                         for (int u = 0; u < kWinPerStep; ++u) {
-                            if (xs + u >= nwin) continue;  // wave-uniform
                             uint64_t codes = (static_cast<uint64_t>(vv[u].y) << 32) | vv[u].x;
                             if constexpr (!BLK) {
                                 // bytes outside the doc's [rs, re) -- and every byte of a
@@ -1018,7 +1048,13 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                                     acc += pr ? c * qv : 0U;
                                 }
                             }
-                            const uint32_t S = wave_incl_scan_dpp(acc);
+                            accs[u] = acc;
+                        }
+#pragma unroll
+                        for (int u = 0; u < kWinPerStep; ++u) accs[u] = wave_incl_scan_dpp(accs[u]);
+#pragma unroll
+                        for (int u = 0; u < kWinPerStep; ++u) {
+                            const uint32_t S = accs[u];
                             if (j1s[u] >= 0 && S != 0) {
                                 atomicAdd(&C.isc[jj[u]], S);
                                 if (j1s[u] != INT32_MAX) atomicSub(&C.isc[j1s[u]], S);
