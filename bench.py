@@ -377,9 +377,7 @@ def c1_reference_order(args, nsparse, threads, dev, stream):
     cpu_build_s = time.perf_counter() - t0
     log(f"C1 {n} docs: GPU build {gpu_build_s:.2f}s, oracle build in the reference's tie order "
         f"{cpu_build_s:.1f}s on {threads} threads ({cpu_build_s / gpu_build_s:.0f}x)")
-    blob = o.serialize()
-    ref_dev = nsparse.read_index_bytes(blob)
-    del blob
+    ref_dev = o.to_library_index(nsparse)
     qip, qix, qv = queries
     p = nsparse.SeismicSearchParameters(args.cut, args.hf)
     d_rd, l_rd = ref_dev.search(nq, qip, qix, qv, 10, p)

@@ -159,6 +159,20 @@ class OracleIndex:
     def copy_from(self, other):
         _chk(lib.oracle_copy_index(self.h, other.h))
 
+    def to_library_index(self, nsparse):
+        """The library's copy of this index: the write_index stream handed to
+        nsparse_read_index_buffer as a raw buffer (no Python bytes object, so no
+        2 GiB limit)."""
+        buf = ctypes.POINTER(ctypes.c_uint8)()
+        sz = ctypes.c_size_t()
+        _chk(lib.oracle_serialize(self.h, ctypes.byref(buf), ctypes.byref(sz)))
+        try:
+            h = ctypes.c_void_p()
+            nsparse._lib.check(nsparse._lib.lib.nsparse_read_index_buffer(ctypes.cast(buf, P), sz, 0, ctypes.byref(h)))
+            return nsparse.Index(h.value)
+        finally:
+            lib.oracle_buffer_free(buf)
+
     def serialize(self):
         buf = ctypes.POINTER(ctypes.c_uint8)()
         sz = ctypes.c_size_t()
