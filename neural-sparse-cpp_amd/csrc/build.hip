@@ -1127,6 +1127,39 @@ __global__ __launch_bounds__(kB) void block_copy_kernel(const int32_t* __restric
     }
 }
 
+// One workgroup per posting list: its summary terms as a bitmap over the
+// vocabulary (in LDS, then written out) and the exclusive prefix of the
+// words' set bits.
+__global__ __launch_bounds__(kB) void summary_bitmap_kernel(const int32_t* __restrict__ ltb,
+                                                            const uint16_t* __restrict__ tids, int nw,
+                                                            uint32_t* __restrict__ bits, uint16_t* __restrict__ rank) {
+    extern __shared__ uint32_t sbits[];  // nw words, then kB / 64 wave sums
+    int32_t* wsum = reinterpret_cast<int32_t*>(sbits + nw);
+    const int l = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int w = tid; w < nw; w += kB) sbits[w] = 0;
+    __syncthreads();
+    for (int32_t i = ltb[l] + tid; i < ltb[l + 1]; i += kB) atomicOr(&sbits[tids[i] >> 5], 1u << (tids[i] & 31));
+    __syncthreads();
+    const int per = (nw + kB - 1) / kB, w0 = tid * per;
+    int local = 0;
+    for (int w = w0; w < w0 + per && w < nw; ++w) local += __popc(sbits[w]);
+    int incl = local;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int x = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += x;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int run = incl - local;
+    for (int v = 0; v < wave; ++v) run += wsum[v];
+    const size_t row = static_cast<size_t>(l) * nw;
+    for (int w = w0; w < w0 + per && w < nw; ++w) {
+        bits[row + w] = sbits[w];
+        rank[row + w] = static_cast<uint16_t>(run);
+        run += __popc(sbits[w]);
+    }
+}
+
 // InvertedIndex postings: the term-sorted positions become (doc, value) pairs.
 __global__ void gather_postings_kernel(const int32_t* __restrict__ spos, const int32_t* __restrict__ doc_of,
                                        const float* __restrict__ values, size_t n, int32_t* __restrict__ docs,
@@ -1215,6 +1248,30 @@ void build_postings_on_device(const int32_t* indptr, const uint16_t* indices, co
     int32_t h_dup = 0;
     dup.download(&h_dup, 4);
     out.duplicate_docs = h_dup != 0;
+}
+
+bool build_summary_bitmaps(const DevIndexView& ix, DeviceBuffer& bits, DeviceBuffer& rank) {
+    bits.release();
+    rank.release();
+    if (ix.n_lists <= 0 || ix.list_term_base == nullptr || ix.sum_term_ids == nullptr) return false;
+    const int nw = (ix.dim + 31) / 32;
+    const size_t L = static_cast<size_t>(ix.n_lists);
+    // rank prefixes are 16-bit: every list must have fewer than 65536 summary terms
+    std::vector<int32_t> ltb(L + 1);
+    hip_check(hipMemcpy(ltb.data(), ix.list_term_base, (L + 1) * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    for (size_t l = 0; l < L; ++l)
+        if (ltb[l + 1] - ltb[l] >= 65536) return false;
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    const size_t need = L * nw * 6;
+    if (need + (size_t{1} << 30) > free_b) return false;
+    bits.allocate(L * nw * 4);
+    rank.allocate(L * nw * 2);
+    hipLaunchKernelGGL(summary_bitmap_kernel, dim3(static_cast<unsigned>(L)), dim3(kB),
+                       nw * 4 + (kB / 64) * 4, nullptr, ix.list_term_base, ix.sum_term_ids, nw,
+                       bits.get<uint32_t>(), rank.get<uint16_t>());
+    sync("summary bitmaps");
+    return true;
 }
 
 bool build_block_rows(const DevIndexView& ix, int64_t n_postings, DeviceBuffer& row_ref, DeviceBuffer& blk_indices,

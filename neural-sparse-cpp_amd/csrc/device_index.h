@@ -45,6 +45,13 @@ struct DevIndexView {
     const uint16_t* blk_indices = nullptr;
     const uint8_t* blk_codes = nullptr;
     int64_t blk_words = 0;  // words in the block copies; word blk_words is zero padding
+    // Per-list summary-term bitmaps (optional): bit t of list l's sum_nw words
+    // is set when t is one of its summary terms; sum_rank[l * sum_nw + w] counts
+    // the list's summary terms in words < w, so a term's column is
+    // list_term_base[l] + sum_rank + the set bits below it in its word.
+    const uint32_t* sum_bits = nullptr;
+    const uint16_t* sum_rank = nullptr;
+    int32_t sum_nw = 0;
 };
 
 // Per-query traversal counters (the algorithmic-byte model): one row of

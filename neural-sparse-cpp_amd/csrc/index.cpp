@@ -222,6 +222,7 @@ void SeismicIndex::build() {
     view_.sum_cluster = d_sum_cluster_.get<uint16_t>();
     view_.sum_value = d_sum_value_.get<void>();
     refresh_block_rows();
+    refresh_summary_bitmaps();
 }
 
 // Block-contiguous row copies for the uint8 search (device_index.h). They cost
@@ -247,6 +248,27 @@ void SeismicIndex::refresh_block_rows() {
     view_.blk_indices = d_blk_indices_.get<uint16_t>();
     view_.blk_codes = d_blk_codes_.get<uint8_t>();
     view_.blk_words = words;
+}
+
+// Per-list summary-term bitmaps (device_index.h): the search finds a query
+// term's summary column with one bitmap word and one rank prefix instead of
+// scanning the list's summary terms. About 6 B per 32 vocabulary terms per
+// list (175 MB for 30522 lists of a 30522-term vocabulary);
+// NSPARSE_SUMMARY_BITMAPS=0 turns them off, and they are skipped when HBM
+// cannot hold them. Both paths find the same columns.
+void SeismicIndex::refresh_summary_bitmaps() {
+    view_.sum_bits = nullptr;
+    view_.sum_rank = nullptr;
+    view_.sum_nw = 0;
+    const char* e = std::getenv("NSPARSE_SUMMARY_BITMAPS");
+    if ((e != nullptr && std::string(e) == "0") || !build_summary_bitmaps(view_, d_sum_bits_, d_sum_rank_)) {
+        d_sum_bits_.release();
+        d_sum_rank_.release();
+        return;
+    }
+    view_.sum_bits = d_sum_bits_.get<uint32_t>();
+    view_.sum_rank = d_sum_rank_.get<uint16_t>();
+    view_.sum_nw = (dim_ + 31) / 32;
 }
 
 void SeismicIndex::ensure_device() {
@@ -370,6 +392,7 @@ void SeismicIndex::upload_lists() {
     view_.sum_value = d_sum_value_.get<void>();
     dev_lists_valid_ = true;
     refresh_block_rows();
+    refresh_summary_bitmaps();
 }
 
 // Rebuilds the per-list host structures from the HBM layout (after a GPU build).

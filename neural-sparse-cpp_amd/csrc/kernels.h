@@ -53,6 +53,11 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
 bool build_block_rows(const DevIndexView& ix, int64_t n_postings, DeviceBuffer& row_ref, DeviceBuffer& blk_indices,
                       DeviceBuffer& blk_codes, int64_t& n_words);
 
+// Per-list summary-term bitmaps and rank prefixes (device_index.h) of the
+// lists in `ix`. Returns false, leaving the buffers empty, when HBM cannot hold
+// them or a list has 65536 or more summary terms.
+bool build_summary_bitmaps(const DevIndexView& ix, DeviceBuffer& bits, DeviceBuffer& rank);
+
 // The default (NSPARSE_SEARCH_LOOKUP / NSPARSE_SEARCH_BLOCK override it).
 SearchConfig default_search_config(bool sq);
 // `c` made valid for this index: the dense table is dropped when it does not fit in LDS.

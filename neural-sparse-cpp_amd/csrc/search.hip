@@ -534,7 +534,18 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
         }
         __syncthreads();
         // ---- summary columns matching the query ----
-        {
+        if (ix.sum_bits != nullptr) {
+            // the list's summary-term bitmap: one bitmap word and one rank
+            // prefix per query entry (a repeated term writes the same column)
+            const uint32_t* lbits = ix.sum_bits + static_cast<int64_t>(term) * ix.sum_nw;
+            const uint16_t* lrank = ix.sum_rank + static_cast<int64_t>(term) * ix.sum_nw;
+            for (int i = tid; i < qn; i += kBlock) {
+                const uint32_t t = eterm[i];
+                const uint32_t w = lbits[t >> 5], bit = 1u << (t & 31);
+                const int32_t below = lrank[t >> 5];
+                if (w & bit) rcol[erank[i]] = tb + below + __popc(w & (bit - 1));
+            }
+        } else {
             constexpr int kU = 4;  // independent term loads in flight per lane
             for (int i0 = tb + tid; i0 < te; i0 += kBlock * kU) {
                 uint32_t t[kU];

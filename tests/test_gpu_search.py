@@ -403,3 +403,29 @@ def test_reference_order_index(nsparse, data, sq):
         d0, l0 = o0.search(*qs, 10, cut=10, hf=0.8)
         for r in range(len(d0)):
             assert O.same_up_to_ties(d0[r], l0[r], got[0][r], got[1][r]), r
+
+
+@pytest.mark.parametrize("sq", [1, 0])
+def test_search_with_and_without_summary_bitmaps(nsparse, data, monkeypatch, sq):
+    """The summary columns of a query's terms are found through per-list
+    summary-term bitmaps by default, or by scanning each list's summary terms
+    with NSPARSE_SUMMARY_BITMAPS=0. Both must equal the oracle, traversal
+    counters included."""
+    import torch
+
+    o, _, vmax = build_pair(nsparse, data[0], sq)
+    blob = o.serialize()
+    for flag in ("0", "1"):
+        monkeypatch.setenv("NSPARSE_SUMMARY_BITMAPS", flag)
+        dev = nsparse.read_index_bytes(blob)
+        for qs in (data[1], data[2]):
+            ip, ix, v = qs
+            want = o.search(ip, ix, v, 10, cut=10, hf=0.8, with_stats=True)
+            n = len(ip) - 1
+            stats = torch.zeros((n, 16), dtype=torch.int64, device="cuda")
+            d, lab = dev.search_device(torch.from_numpy(ip).cuda(), torch.from_numpy(ix.view(np.int16)).cuda(),
+                                       torch.from_numpy(v).cuda(), 10, params=params(nsparse, sq, 10, 0.8, vmax),
+                                       stats=stats)
+            assert_same((d.cpu().numpy(), lab.cpu().numpy()), want[:2])
+            st = stats.cpu().numpy()
+            assert (st[:, :3] == want[2][:, :3]).all()  # lists, query entries found, summary entries
