@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -78,6 +79,34 @@ public:
 private:
     void* ptr_ = nullptr;
     size_t bytes_ = 0;
+};
+
+// Device scratch for launches that return before their kernels finish: a
+// buffer is handed out again only once the event recorded after its last
+// launch has completed, so callers on different streams never share one in
+// flight. Buffers are kept until the pool is destroyed.
+class ScratchPool {
+public:
+    ScratchPool() = default;
+    ScratchPool(const ScratchPool&) = delete;
+    ScratchPool& operator=(const ScratchPool&) = delete;
+    ~ScratchPool() {
+        for (auto& s : slots_)
+            if (s.done != nullptr) (void)hipEventDestroy(s.done);
+    }
+    // A free buffer of at least `bytes`; its index goes to `slot`.
+    void* acquire(size_t bytes, size_t& slot);
+    // Records, on `stream`, the point after which buffer `slot` is free again.
+    void release(size_t slot, hipStream_t stream);
+
+private:
+    struct Slot {
+        DeviceBuffer buf;
+        hipEvent_t done = nullptr;
+        bool busy = false;
+    };
+    std::mutex mu_;
+    std::vector<Slot> slots_;
 };
 
 // Makes sure a HIP device exists and is usable; throws device_error otherwise.

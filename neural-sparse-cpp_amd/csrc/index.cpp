@@ -129,6 +129,35 @@ void require_device() {
         throw device_error("no HIP device available: this index runs its search and build on the GPU");
 }
 
+void* ScratchPool::acquire(size_t bytes, size_t& slot) {
+    std::lock_guard<std::mutex> lock(mu_);
+    for (size_t i = 0; i < slots_.size(); ++i) {
+        Slot& s = slots_[i];
+        if (s.busy || s.buf.bytes() < bytes) continue;
+        const hipError_t e = hipEventQuery(s.done);
+        if (e == hipErrorNotReady) continue;
+        hip_check(e, "hipEventQuery");
+        s.busy = true;
+        slot = i;
+        return s.buf.get();
+    }
+    Slot s;
+    s.buf.allocate(bytes);
+    hip_check(hipEventCreateWithFlags(&s.done, hipEventDisableTiming), "hipEventCreate");
+    s.busy = true;
+    slots_.push_back(std::move(s));
+    slot = slots_.size() - 1;
+    return slots_.back().buf.get();
+}
+
+void ScratchPool::release(size_t slot, hipStream_t stream) {
+    std::lock_guard<std::mutex> lock(mu_);
+    Slot& s = slots_.at(slot);
+    const hipError_t e = hipEventRecord(s.done, stream);
+    s.busy = false;
+    hip_check(e, "hipEventRecord");
+}
+
 void Index::add_with_ids(idx_t, const idx_t*, const term_t*, const float*, const idx_t*) {
     throw std::runtime_error("add_with_ids not implemented in Index");
 }
@@ -223,6 +252,7 @@ void SeismicIndex::build() {
     view_.sum_value = d_sum_value_.get<void>();
     refresh_block_rows();
     refresh_summary_bitmaps();
+    max_list_docs_ = max_list_docs(view_);
 }
 
 // Block-contiguous row copies for the uint8 search (device_index.h). They cost
@@ -393,6 +423,7 @@ void SeismicIndex::upload_lists() {
     dev_lists_valid_ = true;
     refresh_block_rows();
     refresh_summary_bitmaps();
+    max_list_docs_ = max_list_docs(view_);
 }
 
 // Rebuilds the per-list host structures from the HBM layout (after a GPU build).
@@ -479,6 +510,11 @@ void SeismicIndex::search_host(idx_t n, const idx_t* indptr, const term_t* indic
     const int s_err = st->add(HostStage::kOut, 4);
     const int s_stats =
         stats_out != nullptr ? st->add(HostStage::kOut, static_cast<size_t>(n) * kStatFields * 8) : -1;
+    const bool spread = use_spread(n, k, req.cut, stats_out != nullptr, st->stream());
+    const int s_spread =
+        spread ? st->add(HostStage::kScratch,
+                         search_spread_scratch_bytes(view_, max_clusters_, k, req.cut, max_list_docs_, n))
+               : -1;
     st->commit();
     int32_t* qip = st->host<int32_t>(s_qip);
     for (idx_t i = 0; i <= n; ++i) qip[i] = static_cast<int32_t>(indptr[i] - q0);
@@ -520,7 +556,10 @@ void SeismicIndex::search_host(idx_t n, const idx_t* indptr, const term_t* indic
     a.error_flag = st->dev<int32_t>(s_err);
     a.n_queries = n;
     a.query_base = 0;
-    launch_search_checked(a, max_clusters_, k, sq_, st->stream());
+    if (spread)
+        launch_search_spread(a, max_clusters_, k, max_list_docs_, st->dev<void>(s_spread), st->stream());
+    else
+        launch_search_checked(a, max_clusters_, k, sq_, st->stream());
     st->download_and_wait();
     const int32_t err = *st->host<int32_t>(s_err);
     if (err != 0) throw std::invalid_argument(device_error_message(err));
@@ -538,7 +577,36 @@ void SeismicIndex::search_device(int64_t n, const int32_t* d_indptr, const uint1
     SearchArgs a = search_args(n, d_indptr, d_indices, d_values, d_ks, k_max, req, d_sel_of_query, d_bitmaps,
                                n_bitmaps, d_out_d, d_out_l, d_stats);
     if (d_err != nullptr) a.error_flag = d_err;
+    if (use_spread(n, k_max, req.cut, d_stats != nullptr, stream)) {
+        size_t slot = 0;
+        void* scratch = spread_scratch_.acquire(
+            search_spread_scratch_bytes(view_, max_clusters_, k_max, req.cut, max_list_docs_, n), slot);
+        try {
+            launch_search_spread(a, max_clusters_, k_max, max_list_docs_, scratch, stream);
+        } catch (...) {
+            spread_scratch_.release(slot, stream);
+            throw;
+        }
+        spread_scratch_.release(slot, stream);
+        return;
+    }
     launch_search_checked(a, max_clusters_, k_max, sq_, stream);
+}
+
+// A launch of one query, or of a few, leaves most CUs idle with one workgroup
+// per query; the spread search puts each query on the whole GPU. Batches of at
+// most NSPARSE_SPREAD_MAX_QUERIES queries (default 2) take it; NSPARSE_SPREAD=0
+// turns it off. Launches under stream capture, or with traversal counters,
+// keep the per-query workgroup.
+bool SeismicIndex::use_spread(int64_t n, int k_max, int cut, bool stats, hipStream_t stream) const {
+    const char* off = std::getenv("NSPARSE_SPREAD");
+    if (off != nullptr && std::string(off) == "0") return false;
+    const char* e = std::getenv("NSPARSE_SPREAD_MAX_QUERIES");
+    const int64_t max_n = e != nullptr ? std::atoll(e) : 2;
+    if (stats || !sq_ || n > max_n || view_.row_ref == nullptr) return false;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
+    return search_spread_fits(view_, max_clusters_, k_max, cut, max_list_docs_);
 }
 
 void SeismicIndex::search_device_noalloc(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices,

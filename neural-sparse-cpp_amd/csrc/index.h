@@ -139,6 +139,8 @@ private:
     void download_lists();
     void refresh_block_rows();
     void refresh_summary_bitmaps();
+    // Whether a launch of n queries runs as the spread search (search_spread.hip).
+    bool use_spread(int64_t n, int k_max, int cut, bool stats, hipStream_t stream) const;
 
     bool sq_;
     float vmin_ = 0.0F, vmax_ = 1.0F;
@@ -152,6 +154,7 @@ private:
     bool dev_fwd_valid_ = false;
     int32_t n_lists_ = 0;
     int max_clusters_ = 0;
+    int64_t max_list_docs_ = 0;  // most docs of any device posting list
     BuildStats build_stats_{};
     // device copies
     DeviceBuffer d_indptr_, d_indices_, d_values_;
@@ -166,6 +169,8 @@ private:
     std::mutex dev_mu_;
     // Staging for searches over host arrays, one stage per concurrent caller.
     HostStagePool stages_;
+    // Scratch of spread searches launched by search_device.
+    ScratchPool spread_scratch_;
 };
 
 // BrutalIndex (brutal_index.h:22-44, brutal_index.cpp): FP32 vectors kept as

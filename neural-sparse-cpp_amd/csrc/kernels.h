@@ -1,4 +1,5 @@
-// Host-callable entry points of the HIP translation units (search.hip, build.hip).
+// Host-callable entry points of the HIP translation units (search.hip, search_spread.hip,
+// build.hip, rescore.hip).
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -76,6 +77,17 @@ void launch_search_checked(const SearchArgs& a, int max_clusters, int kmax, bool
 constexpr int64_t kNoAllocMaxQueries = int64_t{1} << 20;
 void launch_search_noalloc(const SearchArgs& a, int max_clusters, int kmax, bool sq, hipStream_t stream);
 void launch_exact_checked(const ExactArgs& a, bool sq, hipStream_t stream);
+// The spread search (search_spread.hip): each query of a small batch on the
+// whole GPU, one workgroup per (cut list, slice of its clusters), then an exact
+// replay; uint8 indexes with block rows only, no traversal counters. Same
+// results as launch_search. Scratch: search_spread_scratch_bytes, per launch.
+bool search_spread_fits(const DevIndexView& ix, int max_clusters, int kmax, int cut, int64_t max_list_docs);
+size_t search_spread_scratch_bytes(const DevIndexView& ix, int max_clusters, int kmax, int cut,
+                                   int64_t max_list_docs, int64_t n_queries);
+void launch_search_spread(const SearchArgs& a, int max_clusters, int kmax, int64_t max_list_docs, void* scratch,
+                          hipStream_t stream);
+// Most docs any posting list of `ix` holds (device lists).
+int64_t max_list_docs(const DevIndexView& ix);
 int search_lds_bytes(int dim, int max_clusters, int kmax, bool sq);
 std::string device_error_message(int32_t code);
 // Exhaustive search (BrutalIndex): every document scored, the k best kept per

@@ -36,14 +36,13 @@
 
 #include "device_index.h"
 #include "kernels.h"
+#include "search_common.h"
 #include "stl_order.h"
 
 namespace nsparse_amd {
 
 constexpr int kBlock = 256;  // exact-match, brute-force and default search workgroups
 constexpr int kWaves = kBlock / 64;
-constexpr int kMaxWaves = 16;
-constexpr int kQMax = 512;    // longest query handled on device
 // Doc slots per speculative chunk. Two chunks are staged at once (one is
 // scored while the next is gathered), 20 B per slot each plus a word-start
 // table: 512 slots with the dense code table keep its LDS plan within two
@@ -141,185 +140,6 @@ struct LdsPlan {
     int32_t total;
 };
 
-__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
-
-__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
-    return v;
-}
-
-// Inclusive prefix sum over the 64 lanes with DPP moves only (no LDS
-// crossbar): Hillis-Steele inside each 16-lane row (row_shr 1, 2, 4, 8), then
-// row_bcast:15 and row_bcast:31 carry the row totals upward. Lanes whose DPP
-// source is outside the row or masked off read the `old` operand, 0.
-__device__ __forceinline__ uint32_t wave_incl_scan_dpp(uint32_t v) {
-    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x111, 0xf, 0xf, false));
-    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x112, 0xf, 0xf, false));
-    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x114, 0xf, 0xf, false));
-    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x118, 0xf, 0xf, false));
-    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x142, 0xa, 0xf, false));
-    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x143, 0xc, 0xf, false));
-    return v;
-}
-
-__device__ __forceinline__ bool better(float s, int32_t i, float ts, int32_t ti) {
-    return s > ts || (s == ts && i < ti);
-}
-
-struct QueryLds {
-    const uint32_t* qbits;
-    const uint16_t* qpref;
-    const float* rval;
-    __device__ __forceinline__ int rank_of(uint32_t t, bool& present) const {
-        const uint32_t w = qbits[t >> 5];
-        const uint32_t bit = 1u << (t & 31);
-        present = (w & bit) != 0;
-        return qpref[t >> 5] + __popc(w & (bit - 1));
-    }
-    __device__ __forceinline__ float value(uint32_t t) const {
-        bool p;
-        const int r = rank_of(t, p);
-        return p ? rval[r] : 0.0F;
-    }
-};
-
-// std::sort(A, A + n, greater_key) by one wavefront, step-exact with the
-// libstdc++ introsort that stl_order.h restates for one lane (median-of-3
-// pivot, unguarded Hoare partition, depth limit 2*floor(log2 n) with heap-sort
-// fallback, 16-element threshold, final insertion sort).
-//
-// One Hoare partition of [first, last) with pivot p = A[first] (after the
-// median move) is computed in parallel: with L_0 < L_1 < ... the positions in
-// (first, last) whose key is <= p's ("left stoppers") and R_0 > R_1 > ... those
-// whose key is >= p's ("right stoppers"), the sequential scan swaps
-// A[L_i] <-> A[R_i] exactly while L_i < R_i (a swapped element stops the other
-// scanner no later than the next original stopper), and returns
-// min(L_m, R_{m-1}) after m swaps (L_0 when m = 0). The swaps are disjoint.
-// Ranges are pushed on a lane-held stack (lane sp holds entry sp); the order in
-// which disjoint ranges are finished does not change the result. Lpos and Rasc
-// hold n positions each.
-//
-// The final insertion sort is left to the caller: after the partitions every
-// key of an earlier leaf range (<= 16 elements, or a heap-sorted range) is >=
-// every key of a later one, so __final_insertion_sort never moves an element
-// out of its leaf and amounts to a stable sort of each leaf. `leaf_bits` (n
-// bits) marks the first position of every leaf; heap-sorted ranges, already in
-// order, mark every position.
-__device__ void introsort_desc_wave(KeyItem* A, int n, uint16_t* Lpos, uint16_t* Rasc, uint32_t* leaf_bits,
-                                    int lane) {
-    for (int w = lane; w < (n + 31) / 32; w += 64) leaf_bits[w] = w == 0 ? 1u : 0u;
-    __builtin_amdgcn_wave_barrier();
-    if (n <= 1) return;
-    const uint64_t lt = (1ULL << lane) - 1;
-    int s_lo = 0, s_hi = n, s_d = floor_log2(n) * 2;  // entry `lane` of the stack
-    int sp = 1;
-    while (sp > 0) {
-        --sp;
-        int lo = __builtin_amdgcn_readlane(s_lo, sp);
-        int hi = __builtin_amdgcn_readlane(s_hi, sp);
-        int depth = __builtin_amdgcn_readlane(s_d, sp);
-        bool heap = false;
-        while (hi - lo > kIntrosortThreshold) {
-            if (depth == 0) {
-                if (lane == 0) {
-                    heap_sort_range(A, lo, hi);
-                    for (int x = lo; x < hi; ++x) leaf_bits[x >> 5] |= 1u << (x & 31);
-                }
-                __builtin_amdgcn_wave_barrier();
-                heap = true;
-                break;
-            }
-            --depth;
-            if (lane == 0) median_to_first(A, lo, lo + 1, lo + (hi - lo) / 2, hi - 1);
-            __builtin_amdgcn_wave_barrier();
-            const float pk = A[lo].key;
-            int nl = 0, nr = 0;
-            for (int x0 = lo + 1; x0 < hi; x0 += 64) {
-                const int x = x0 + lane;
-                const bool in = x < hi;
-                const float key = in ? A[x].key : 0.0F;
-                const bool isl = in && !(key > pk);
-                const bool isr = in && !(pk > key);
-                const uint64_t ml = ballot(isl), mr = ballot(isr);
-                if (isl) Lpos[nl + __popcll(ml & lt)] = static_cast<uint16_t>(x);
-                if (isr) Rasc[nr + __popcll(mr & lt)] = static_cast<uint16_t>(x);
-                nl += __popcll(ml);
-                nr += __popcll(mr);
-            }
-            __builtin_amdgcn_wave_barrier();
-            // R_i = Rasc[nr - 1 - i]; L_i < R_i holds for a prefix i < m
-            const int pairs = min(nl, nr);
-            int m = 0;
-            for (int i0 = 0; i0 < pairs; i0 += 64) {
-                const int i = i0 + lane;
-                const bool c = i < pairs && Lpos[i] < Rasc[nr - 1 - i];
-                m += __popcll(ballot(c));
-            }
-            for (int i0 = 0; i0 < m; i0 += 64) {
-                const int i = i0 + lane;
-                KeyItem a{}, b{};
-                int pa = 0, pb = 0;
-                if (i < m) {
-                    pa = Lpos[i];
-                    pb = Rasc[nr - 1 - i];
-                    a = A[pa];
-                    b = A[pb];
-                }
-                __builtin_amdgcn_wave_barrier();
-                if (i < m) {
-                    A[pa] = b;
-                    A[pb] = a;
-                }
-            }
-            int cut = m < nl ? static_cast<int>(Lpos[m]) : 0x7FFFFFFF;
-            if (m > 0) cut = min(cut, static_cast<int>(Rasc[nr - m]));
-            __builtin_amdgcn_wave_barrier();
-            // push the right part, continue with the left one
-            if (lane == sp) {
-                s_lo = cut;
-                s_hi = hi;
-                s_d = depth;
-            }
-            ++sp;
-            hi = cut;
-        }
-        if (!heap && lane == 0) leaf_bits[lo >> 5] |= 1u << (lo & 31);
-        __builtin_amdgcn_wave_barrier();
-    }
-}
-
-// The final insertion sort as a stable sort of each leaf (see above): element
-// i goes to leaf_start + (elements of its leaf with a greater key, or an equal
-// key before it). Writes out[pos] = A[i].id. Any number of lanes.
-__device__ void introsort_final_pass(const KeyItem* A, int n, const uint32_t* leaf_bits, uint16_t* out, int tid,
-                                     int nthreads) {
-    for (int i = tid; i < n; i += nthreads) {
-        int w = i >> 5;
-        uint32_t m = leaf_bits[w] & ((2u << (i & 31)) - 1u);  // marks at or before i
-        while (m == 0) m = leaf_bits[--w];
-        const int start = w * 32 + 31 - __clz(m);
-        int end = n;
-        const int i1 = i + 1;
-        if (i1 < n) {
-            w = i1 >> 5;
-            m = leaf_bits[w] & ~((1u << (i1 & 31)) - 1u);  // marks at or after i + 1
-            while (m == 0 && (w + 1) * 32 < n) m = leaf_bits[++w];
-            if (m != 0) end = min(n, w * 32 + __ffs(m) - 1);
-        }
-        const float ki = A[i].key;
-        int r = 0;
-        for (int j = start; j < end; ++j) {
-            const float kj = A[j].key;
-            r += (kj > ki || (kj == ki && j < i)) ? 1 : 0;
-        }
-        out[start + r] = static_cast<uint16_t>(A[i].id);
-    }
-}
-
 // DENSE (uint8 codes only): the query's codes also sit in a dense
 // vocabulary-sized byte table at LDS offset 0, so the doc-row scoring loop
 // does one byte read per element instead of bitmap + rank + value reads; it
@@ -403,33 +223,9 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
     // ---- query setup ----
     if (tid < kMiscCount) misc[tid] = 0;
     if (tid < kStatFields) qstat[tid] = 0;
-    for (int w = tid; w < P.nw; w += kBlock) qbits[w] = 0;
-    for (int r = tid; r < qn; r += kBlock) {
-        rlast[r] = -1;
-        rsum[r] = 0;
-    }
-    __syncthreads();
-    for (int i = tid; i < qn; i += kBlock) {
-        const uint32_t t = a.q_indices[s0 + i];
-        const float v = a.q_values[s0 + i];
-        if (t >= static_cast<uint32_t>(ix.dim)) {
-            misc[kMiscErr] = 1;
-            continue;
-        }
-        eterm[i] = static_cast<uint16_t>(t);
-        if constexpr (SQ) {
-            // ScalarQuantizer::encode_8bit (utils/scalar_quantizer.h:97-101)
-            float sc = (v - a.q_vmin) * a.q_scale;
-            sc = (sc < 255.0F) ? sc : 255.0F;
-            sc = (0.0F < sc) ? sc : 0.0F;
-            evalf[i] = static_cast<float>(static_cast<uint32_t>(roundf(sc)));
-        } else {
-            evalf[i] = v;
-        }
-        atomicOr(&qbits[t >> 5], 1u << (t & 31));
-    }
-    __syncthreads();
-    if (misc[kMiscErr] != 0) {
+    const QueryTables QT{qbits, qpref, eterm, erank, evalf, rlast, rval, rsum, qcode, wsum, &misc[kMiscErr],
+                         &misc[kMiscNu]};
+    if (!load_query_lds<SQ, DENSE, BLOCK>(a, s0, qn, P.nw, QT)) {
         if (tid == 0) atomicCAS(a.error_flag, 0, kDevTermOutOfRange);
         for (int i = tid; i < a.k_max; i += kBlock) {
             out_d[i] = -1.0F;
@@ -437,44 +233,8 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
         }
         return;
     }
-    // exclusive popcount prefix over the bitmap words
-    {
-        const int per = (P.nw + kBlock - 1) / kBlock;
-        const int w0 = tid * per;
-        int local = 0;
-        for (int w = w0; w < w0 + per && w < P.nw; ++w) local += __popc(qbits[w]);
-        const int incl = wave_incl_scan(local, lane);
-        if (lane == 63) wsum[wave] = incl;
-        __syncthreads();
-        int base = 0;
-        for (int w = 0; w < wave; ++w) base += wsum[w];
-        int run = base + incl - local;
-        for (int w = w0; w < w0 + per && w < P.nw; ++w) {
-            qpref[w] = static_cast<uint16_t>(run);
-            run += __popc(qbits[w]);
-        }
-        if (tid == kBlock - 1) misc[kMiscNu] = base + incl;
-    }
-    __syncthreads();
     const QueryLds QL{qbits, qpref, rval};
-    for (int i = tid; i < qn; i += kBlock) {
-        bool p;
-        const int r = QL.rank_of(eterm[i], p);
-        erank[i] = static_cast<uint16_t>(r);
-        atomicMax(&rlast[r], i);
-        if constexpr (SQ) atomicAdd(&rsum[r], static_cast<uint32_t>(evalf[i]));
-    }
-    __syncthreads();
     const int nu = misc[kMiscNu];
-    for (int r = tid; r < nu; r += kBlock) rval[r] = evalf[rlast[r]];
-    if constexpr (DENSE) {
-        uint32_t* q32 = reinterpret_cast<uint32_t*>(qcode);
-        for (int w = tid; w < (ix.dim + 3) / 4; w += kBlock) q32[w] = 0;
-        __syncthreads();
-        // get_dense_vector: the last entry of a repeated term wins
-        for (int i = tid; i < qn; i += kBlock)
-            if (rlast[erank[i]] == i) qcode[eterm[i]] = static_cast<uint8_t>(evalf[i]);
-    }
     if (tid == 0) {
         // top_k_tokens (utils/vector_process.h:18-30) on codes (SQ) or raw values.
         KeyItem* heap = reinterpret_cast<KeyItem*>(uni);
@@ -534,35 +294,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
         }
         __syncthreads();
         // ---- summary columns matching the query ----
-        if (ix.sum_bits != nullptr) {
-            // the list's summary-term bitmap: one bitmap word and one rank
-            // prefix per query entry (a repeated term writes the same column)
-            const uint32_t* lbits = ix.sum_bits + static_cast<int64_t>(term) * ix.sum_nw;
-            const uint16_t* lrank = ix.sum_rank + static_cast<int64_t>(term) * ix.sum_nw;
-            for (int i = tid; i < qn; i += kBlock) {
-                const uint32_t t = eterm[i];
-                const uint32_t w = lbits[t >> 5], bit = 1u << (t & 31);
-                const int32_t below = lrank[t >> 5];
-                if (w & bit) rcol[erank[i]] = tb + below + __popc(w & (bit - 1));
-            }
-        } else {
-            constexpr int kU = 4;  // independent term loads in flight per lane
-            for (int i0 = tb + tid; i0 < te; i0 += kBlock * kU) {
-                uint32_t t[kU];
-#pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    const int i = i0 + u * kBlock;
-                    t[u] = i < te ? ix.sum_term_ids[i] : 0xFFFFFFFFu;
-                }
-#pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    if (t[u] == 0xFFFFFFFFu) continue;
-                    bool p;
-                    const int r = QL.rank_of(t[u], p);
-                    if (p) rcol[r] = i0 + u * kBlock;
-                }
-            }
-        }
+        match_summary_columns<BLOCK>(ix, term, tb, te, qn, eterm, erank, QL, rcol);
         __syncthreads();
         if (a.stats != nullptr) {
             // query entries found in this list and their summary entries (a
@@ -607,71 +339,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
         stat_add(kStatTermsFound, misc[kMiscStatFound]);
         stat_add(kStatCscEntries, misc[kMiscStatCsc]);
         if constexpr (SQ) {
-            // The found columns' entries, flattened: one pass loads every
-            // column's range and query-code sum, a prefix over their lengths
-            // numbers the entries, then all lanes take entries with kU loads in
-            // flight, each finding its column by binary search. Integer atomics
-            // make the per-cluster sums exact in any order.
-            const uint8_t* sv = static_cast<const uint8_t*>(ix.sum_value);
-            int32_t* fe0 = reinterpret_cast<int32_t*>(uni);  // column start, relative to the list's first entry
-            int32_t* fpre = fe0 + kQMax;                      // [nf + 1] exclusive prefix of column lengths
-            uint32_t* fqs = reinterpret_cast<uint32_t*>(fpre + kQMax + 1);
-            const int64_t lbase = te > tb ? ix.sum_term_ptr[tb] : 0;
-            for (int f = tid; f < nf; f += kBlock) {
-                const int r = flist[f];
-                const int col = rcol[r];
-                const int64_t e0 = ix.sum_term_ptr[col], e1 = ix.sum_term_ptr[col + 1];
-                fe0[f] = static_cast<int32_t>(e0 - lbase);
-                fpre[f + 1] = static_cast<int32_t>(e1 - e0);
-                fqs[f] = rsum[r];
-            }
-            __syncthreads();
-            if (wave == 0) {
-                int run = 0;
-                for (int b = 0; b < nf; b += 64) {
-                    const int f = b + lane;
-                    const int len = f < nf ? fpre[f + 1] : 0;
-                    const int incl = wave_incl_scan(len, lane);
-                    if (f < nf) fpre[f + 1] = run + incl;
-                    run += __shfl(incl, 63, 64);
-                }
-                if (lane == 0) fpre[0] = 0;
-            }
-            __syncthreads();
-            const int total_e = fpre[nf];
-            constexpr int kU = 4;
-            for (int x0 = tid; x0 < total_e; x0 += kBlock * kU) {
-                int fx[kU];
-                int64_t ex[kU];
-#pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    const int x = x0 + u * kBlock;
-                    fx[u] = -1;
-                    ex[u] = lbase;
-                    if (x < total_e) {
-                        int lo = 0, hi = nf - 1;
-                        while (lo < hi) {
-                            const int mid = (lo + hi + 1) >> 1;
-                            if (fpre[mid] <= x)
-                                lo = mid;
-                            else
-                                hi = mid - 1;
-                        }
-                        fx[u] = lo;
-                        ex[u] = lbase + fe0[lo] + (x - fpre[lo]);
-                    }
-                }
-                uint32_t cl[kU], vv[kU];
-#pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    cl[u] = ix.sum_cluster[ex[u]];
-                    vv[u] = sv[ex[u]];
-                }
-#pragma unroll
-                for (int u = 0; u < kU; ++u)
-                    if (fx[u] >= 0) atomicAdd(&sacc[cl[u]], fqs[fx[u]] * vv[u]);
-            }
-            __syncthreads();
+            accumulate_summaries_sq<BLOCK>(ix, tb, te, nf, flist, rcol, rsum, uni, sacc);
             for (int c = tid; c < nc; c += kBlock) score[c] = static_cast<float>(sacc[c]);
         } else {
             const float* sv = static_cast<const float*>(ix.sum_value);
@@ -1126,94 +794,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                     else
                         sc = C.sc[j];
                 }
-                const int32_t id = in ? C.id[j] : 0;
-                const bool fullk = tkn == k;
-                const float ks = fullk ? tks[k - 1] : 0.0F;
-                const int32_t ki = fullk ? tki[k - 1] : 0;
-                const bool cand = in && (!fullk || better(sc, id, ks, ki));
-                const uint64_t cm = ballot(cand);
-                if (cm == 0) return;
-                const auto lane_f = [](float v, int l) {
-                    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
-                };
-                int rt = 0;
-                bool dup = false;
-                const bool small = tkn <= 128;
-                const bool l0 = lane < tkn, l1 = lane + 64 < tkn;
-                float e0s = 0.0F, e1s = 0.0F;
-                int32_t e0i = 0, e1i = 0;
-                if (small) {
-                    if (l0) {
-                        e0s = tks[lane];
-                        e0i = tki[lane];
-                    }
-                    if (l1) {
-                        e1s = tks[lane + 64];
-                        e1i = tki[lane + 64];
-                    }
-                    for (uint64_t m = cm; m != 0; m &= m - 1) {
-                        const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
-                        const float cs = lane_f(sc, bl);
-                        const int32_t ci = __builtin_amdgcn_readlane(id, bl);
-                        const int cnt = __popcll(ballot(l0 && better(e0s, e0i, cs, ci))) +
-                                        __popcll(ballot(l1 && better(e1s, e1i, cs, ci)));
-                        const bool dp = ballot((l0 && e0i == ci) || (l1 && e1i == ci)) != 0;
-                        if (lane == bl) {
-                            rt = cnt;
-                            dup = dp;
-                        }
-                    }
-                } else if (cand) {
-                    int hi = tkn;
-                    while (rt < hi) {
-                        const int mid = (rt + hi) >> 1;
-                        if (better(tks[mid], tki[mid], sc, id))
-                            rt = mid + 1;
-                        else
-                            hi = mid;
-                    }
-                    dup = rt < tkn && tki[rt] == id;
-                }
-                const bool v = cand && !dup;
-                const uint64_t vm = ballot(v);
-                if (vm == 0) return;
-                int rc = 0;
-                for (uint64_t m = vm; m != 0; m &= m - 1) {
-                    const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
-                    rc += better(lane_f(sc, bl), __builtin_amdgcn_readlane(id, bl), sc, id) ? 1 : 0;
-                }
-                // top-k entries move up past the candidates better than them,
-                // highest tile first (a tile is read whole before it is written)
-                for (int t0 = ((tkn - 1) >> 6) << 6; t0 >= 0 && tkn > 0; t0 -= 64) {
-                    const int t = t0 + lane;
-                    const bool live = t < tkn;
-                    float es;
-                    int32_t ei;
-                    if (small) {
-                        es = t0 == 0 ? e0s : e1s;
-                        ei = t0 == 0 ? e0i : e1i;
-                    } else {
-                        es = live ? tks[t] : 0.0F;
-                        ei = live ? tki[t] : 0;
-                    }
-                    int up = 0;
-                    for (uint64_t m = vm; m != 0; m &= m - 1) {
-                        const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
-                        up += better(lane_f(sc, bl), __builtin_amdgcn_readlane(id, bl), es, ei) ? 1 : 0;
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                    if (live && up > 0 && t + up < k) {
-                        tks[t + up] = es;
-                        tki[t + up] = ei;
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                }
-                if (v && rt + rc < k) {
-                    tks[rt + rc] = sc;
-                    tki[rt + rc] = id;
-                }
-                __builtin_amdgcn_wave_barrier();
-                tkn = min(k, tkn + __popcll(vm));
+                merge_topk_wave(in, sc, in ? C.id[j] : 0, k, tkn, tks, tki, lane);
             };
             const int nseg = C.meta[kChunkNseg];
             for (int si = 0; si < nseg; ++si) {

@@ -1,0 +1,504 @@
+// Device helpers shared by the search kernels (search.hip, search_spread.hip):
+// wave-level scans and ballots, the query's LDS tables, the step-exact
+// std::sort emulation for the first list's visiting order, and the top-k merge.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_index.h"
+#include "stl_order.h"
+
+namespace nsparse_amd {
+
+constexpr int kQMax = 512;  // longest query handled on device
+constexpr int kMaxWaves = 16;
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+// Inclusive prefix sum over the 64 lanes with DPP moves only (no LDS
+// crossbar): Hillis-Steele inside each 16-lane row (row_shr 1, 2, 4, 8), then
+// row_bcast:15 and row_bcast:31 carry the row totals upward. Lanes whose DPP
+// source is outside the row or masked off read the `old` operand, 0.
+__device__ __forceinline__ uint32_t wave_incl_scan_dpp(uint32_t v) {
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x111, 0xf, 0xf, false));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x112, 0xf, 0xf, false));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x114, 0xf, 0xf, false));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x118, 0xf, 0xf, false));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x142, 0xa, 0xf, false));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x143, 0xc, 0xf, false));
+    return v;
+}
+
+__device__ __forceinline__ bool better(float s, int32_t i, float ts, int32_t ti) {
+    return s > ts || (s == ts && i < ti);
+}
+
+struct QueryLds {
+    const uint32_t* qbits;
+    const uint16_t* qpref;
+    const float* rval;
+    __device__ __forceinline__ int rank_of(uint32_t t, bool& present) const {
+        const uint32_t w = qbits[t >> 5];
+        const uint32_t bit = 1u << (t & 31);
+        present = (w & bit) != 0;
+        return qpref[t >> 5] + __popc(w & (bit - 1));
+    }
+    __device__ __forceinline__ float value(uint32_t t) const {
+        bool p;
+        const int r = rank_of(t, p);
+        return p ? rval[r] : 0.0F;
+    }
+};
+
+// std::sort(A, A + n, greater_key) by one wavefront, step-exact with the
+// libstdc++ introsort that stl_order.h restates for one lane (median-of-3
+// pivot, unguarded Hoare partition, depth limit 2*floor(log2 n) with heap-sort
+// fallback, 16-element threshold, final insertion sort).
+//
+// One Hoare partition of [first, last) with pivot p = A[first] (after the
+// median move) is computed in parallel: with L_0 < L_1 < ... the positions in
+// (first, last) whose key is <= p's ("left stoppers") and R_0 > R_1 > ... those
+// whose key is >= p's ("right stoppers"), the sequential scan swaps
+// A[L_i] <-> A[R_i] exactly while L_i < R_i (a swapped element stops the other
+// scanner no later than the next original stopper), and returns
+// min(L_m, R_{m-1}) after m swaps (L_0 when m = 0). The swaps are disjoint.
+// Ranges are pushed on a lane-held stack (lane sp holds entry sp); the order in
+// which disjoint ranges are finished does not change the result. Lpos and Rasc
+// hold n positions each.
+//
+// The final insertion sort is left to the caller: after the partitions every
+// key of an earlier leaf range (<= 16 elements, or a heap-sorted range) is >=
+// every key of a later one, so __final_insertion_sort never moves an element
+// out of its leaf and amounts to a stable sort of each leaf. `leaf_bits` (n
+// bits) marks the first position of every leaf; heap-sorted ranges, already in
+// order, mark every position.
+inline __device__ void introsort_desc_wave(KeyItem* A, int n, uint16_t* Lpos, uint16_t* Rasc,
+                                           uint32_t* leaf_bits, int lane) {
+    for (int w = lane; w < (n + 31) / 32; w += 64) leaf_bits[w] = w == 0 ? 1u : 0u;
+    __builtin_amdgcn_wave_barrier();
+    if (n <= 1) return;
+    const uint64_t lt = (1ULL << lane) - 1;
+    int s_lo = 0, s_hi = n, s_d = floor_log2(n) * 2;  // entry `lane` of the stack
+    int sp = 1;
+    while (sp > 0) {
+        --sp;
+        int lo = __builtin_amdgcn_readlane(s_lo, sp);
+        int hi = __builtin_amdgcn_readlane(s_hi, sp);
+        int depth = __builtin_amdgcn_readlane(s_d, sp);
+        bool heap = false;
+        while (hi - lo > kIntrosortThreshold) {
+            if (depth == 0) {
+                if (lane == 0) {
+                    heap_sort_range(A, lo, hi);
+                    for (int x = lo; x < hi; ++x) leaf_bits[x >> 5] |= 1u << (x & 31);
+                }
+                __builtin_amdgcn_wave_barrier();
+                heap = true;
+                break;
+            }
+            --depth;
+            if (lane == 0) median_to_first(A, lo, lo + 1, lo + (hi - lo) / 2, hi - 1);
+            __builtin_amdgcn_wave_barrier();
+            const float pk = A[lo].key;
+            int nl = 0, nr = 0;
+            for (int x0 = lo + 1; x0 < hi; x0 += 64) {
+                const int x = x0 + lane;
+                const bool in = x < hi;
+                const float key = in ? A[x].key : 0.0F;
+                const bool isl = in && !(key > pk);
+                const bool isr = in && !(pk > key);
+                const uint64_t ml = ballot(isl), mr = ballot(isr);
+                if (isl) Lpos[nl + __popcll(ml & lt)] = static_cast<uint16_t>(x);
+                if (isr) Rasc[nr + __popcll(mr & lt)] = static_cast<uint16_t>(x);
+                nl += __popcll(ml);
+                nr += __popcll(mr);
+            }
+            __builtin_amdgcn_wave_barrier();
+            // R_i = Rasc[nr - 1 - i]; L_i < R_i holds for a prefix i < m
+            const int pairs = min(nl, nr);
+            int m = 0;
+            for (int i0 = 0; i0 < pairs; i0 += 64) {
+                const int i = i0 + lane;
+                const bool c = i < pairs && Lpos[i] < Rasc[nr - 1 - i];
+                m += __popcll(ballot(c));
+            }
+            for (int i0 = 0; i0 < m; i0 += 64) {
+                const int i = i0 + lane;
+                KeyItem a{}, b{};
+                int pa = 0, pb = 0;
+                if (i < m) {
+                    pa = Lpos[i];
+                    pb = Rasc[nr - 1 - i];
+                    a = A[pa];
+                    b = A[pb];
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (i < m) {
+                    A[pa] = b;
+                    A[pb] = a;
+                }
+            }
+            int cut = m < nl ? static_cast<int>(Lpos[m]) : 0x7FFFFFFF;
+            if (m > 0) cut = min(cut, static_cast<int>(Rasc[nr - m]));
+            __builtin_amdgcn_wave_barrier();
+            // push the right part, continue with the left one
+            if (lane == sp) {
+                s_lo = cut;
+                s_hi = hi;
+                s_d = depth;
+            }
+            ++sp;
+            hi = cut;
+        }
+        if (!heap && lane == 0) leaf_bits[lo >> 5] |= 1u << (lo & 31);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// The final insertion sort as a stable sort of each leaf (see above): element
+// i goes to leaf_start + (elements of its leaf with a greater key, or an equal
+// key before it). Writes out[pos] = A[i].id. Any number of lanes.
+inline __device__ void introsort_final_pass(const KeyItem* A, int n, const uint32_t* leaf_bits, uint16_t* out,
+                                            int tid, int nthreads) {
+    for (int i = tid; i < n; i += nthreads) {
+        int w = i >> 5;
+        uint32_t m = leaf_bits[w] & ((2u << (i & 31)) - 1u);  // marks at or before i
+        while (m == 0) m = leaf_bits[--w];
+        const int start = w * 32 + 31 - __clz(m);
+        int end = n;
+        const int i1 = i + 1;
+        if (i1 < n) {
+            w = i1 >> 5;
+            m = leaf_bits[w] & ~((1u << (i1 & 31)) - 1u);  // marks at or after i + 1
+            while (m == 0 && (w + 1) * 32 < n) m = leaf_bits[++w];
+            if (m != 0) end = min(n, w * 32 + __ffs(m) - 1);
+        }
+        const float ki = A[i].key;
+        int r = 0;
+        for (int j = start; j < end; ++j) {
+            const float kj = A[j].key;
+            r += (kj > ki || (kj == ki && j < i)) ? 1 : 0;
+        }
+        out[start + r] = static_cast<uint16_t>(A[i].id);
+    }
+}
+
+// Merges one wave's candidates (lane l: `in`, score `sc`, doc `id`) into the
+// top-k: tks/tki hold tkn entries sorted by (score desc, doc id asc), at most
+// k. Inserting the candidates one by one with the reference's reject rule keeps
+// the k best of (top-k U candidates) under that order -- an order-free result
+// -- so they are merged in parallel: each candidate's rank among the top-k
+// (ballots against the top-k held in registers, or a binary search when it has
+// more than 128 entries; an equal id is the same doc seen in an earlier list
+// and is not inserted again) plus its rank among the other candidates. The
+// candidates must be distinct docs. Uniform over the wave; tkn is updated.
+__device__ __forceinline__ void merge_topk_wave(bool in, float sc, int32_t id, int k, int& tkn, float* tks,
+                                                int32_t* tki, int lane) {
+    const bool fullk = tkn == k;
+    const float ks = fullk ? tks[k - 1] : 0.0F;
+    const int32_t ki = fullk ? tki[k - 1] : 0;
+    const bool cand = in && (!fullk || better(sc, id, ks, ki));
+    const uint64_t cm = ballot(cand);
+    if (cm == 0) return;
+    const auto lane_f = [](float v, int l) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+    };
+    int rt = 0;
+    bool dup = false;
+    const bool small = tkn <= 128;
+    const bool l0 = lane < tkn, l1 = lane + 64 < tkn;
+    float e0s = 0.0F, e1s = 0.0F;
+    int32_t e0i = 0, e1i = 0;
+    if (small) {
+        if (l0) {
+            e0s = tks[lane];
+            e0i = tki[lane];
+        }
+        if (l1) {
+            e1s = tks[lane + 64];
+            e1i = tki[lane + 64];
+        }
+        for (uint64_t m = cm; m != 0; m &= m - 1) {
+            const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
+            const float cs = lane_f(sc, bl);
+            const int32_t ci = __builtin_amdgcn_readlane(id, bl);
+            const int cnt = __popcll(ballot(l0 && better(e0s, e0i, cs, ci))) +
+                            __popcll(ballot(l1 && better(e1s, e1i, cs, ci)));
+            const bool dp = ballot((l0 && e0i == ci) || (l1 && e1i == ci)) != 0;
+            if (lane == bl) {
+                rt = cnt;
+                dup = dp;
+            }
+        }
+    } else if (cand) {
+        int hi = tkn;
+        while (rt < hi) {
+            const int mid = (rt + hi) >> 1;
+            if (better(tks[mid], tki[mid], sc, id))
+                rt = mid + 1;
+            else
+                hi = mid;
+        }
+        dup = rt < tkn && tki[rt] == id;
+    }
+    const bool v = cand && !dup;
+    const uint64_t vm = ballot(v);
+    if (vm == 0) return;
+    int rc = 0;
+    for (uint64_t m = vm; m != 0; m &= m - 1) {
+        const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
+        rc += better(lane_f(sc, bl), __builtin_amdgcn_readlane(id, bl), sc, id) ? 1 : 0;
+    }
+    // top-k entries move up past the candidates better than them, highest
+    // tile first (a tile is read whole before it is written)
+    for (int t0 = ((tkn - 1) >> 6) << 6; t0 >= 0 && tkn > 0; t0 -= 64) {
+        const int t = t0 + lane;
+        const bool live = t < tkn;
+        float es;
+        int32_t ei;
+        if (small) {
+            es = t0 == 0 ? e0s : e1s;
+            ei = t0 == 0 ? e0i : e1i;
+        } else {
+            es = live ? tks[t] : 0.0F;
+            ei = live ? tki[t] : 0;
+        }
+        int up = 0;
+        for (uint64_t m = vm; m != 0; m &= m - 1) {
+            const int bl = __ffsll(static_cast<unsigned long long>(m)) - 1;
+            up += better(lane_f(sc, bl), __builtin_amdgcn_readlane(id, bl), es, ei) ? 1 : 0;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (live && up > 0 && t + up < k) {
+            tks[t + up] = es;
+            tki[t + up] = ei;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (v && rt + rc < k) {
+        tks[rt + rc] = sc;
+        tki[rt + rc] = id;
+    }
+    __builtin_amdgcn_wave_barrier();
+    tkn = min(k, tkn + __popcll(vm));
+}
+
+// LDS tables of one query (all pointers into the workgroup's LDS).
+struct QueryTables {
+    uint32_t* qbits;  // presence bitmap over the vocabulary (nw words)
+    uint16_t* qpref;  // set bits in the words before each word
+    uint16_t* eterm;  // per query entry: term id
+    uint16_t* erank;  //   its rank among the distinct terms
+    float* evalf;     //   its uint8 code (SQ) or value
+    int32_t* rlast;   // per distinct term: its last entry
+    float* rval;      //   that entry's code or value (the reference's dense vector)
+    uint32_t* rsum;   //   sum of its entries' codes (SQ)
+    uint8_t* qcode;   // DENSE: vocabulary-sized code table (last entry wins)
+    int32_t* wsum;    // one int per wave
+    int32_t* err;     // zero before the call
+    int32_t* nu;      // distinct terms, written here
+};
+
+// Loads query entries [s0, s0 + qn) into the tables. Returns false (uniformly,
+// after a barrier) when a term id is >= dim; the tables are then partial.
+// Codes are ScalarQuantizer::encode_8bit of the values
+// (utils/scalar_quantizer.h:97-101). No barrier follows the last writes.
+template <bool SQ, bool DENSE, int BLOCK>
+__device__ __forceinline__ bool load_query_lds(const SearchArgs& a, int64_t s0, int qn, int nw,
+                                               const QueryTables& T) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int w = tid; w < nw; w += BLOCK) T.qbits[w] = 0;
+    for (int r = tid; r < qn; r += BLOCK) {
+        T.rlast[r] = -1;
+        T.rsum[r] = 0;
+    }
+    __syncthreads();
+    for (int i = tid; i < qn; i += BLOCK) {
+        const uint32_t t = a.q_indices[s0 + i];
+        const float v = a.q_values[s0 + i];
+        if (t >= static_cast<uint32_t>(a.ix.dim)) {
+            *T.err = 1;
+            continue;
+        }
+        T.eterm[i] = static_cast<uint16_t>(t);
+        if constexpr (SQ) {
+            float sc = (v - a.q_vmin) * a.q_scale;
+            sc = (sc < 255.0F) ? sc : 255.0F;
+            sc = (0.0F < sc) ? sc : 0.0F;
+            T.evalf[i] = static_cast<float>(static_cast<uint32_t>(roundf(sc)));
+        } else {
+            T.evalf[i] = v;
+        }
+        atomicOr(&T.qbits[t >> 5], 1u << (t & 31));
+    }
+    __syncthreads();
+    if (*T.err != 0) return false;
+    // exclusive popcount prefix over the bitmap words
+    {
+        const int per = (nw + BLOCK - 1) / BLOCK;
+        const int w0 = tid * per;
+        int local = 0;
+        for (int w = w0; w < w0 + per && w < nw; ++w) local += __popc(T.qbits[w]);
+        const int incl = wave_incl_scan(local, lane);
+        if (lane == 63) T.wsum[wave] = incl;
+        __syncthreads();
+        int base = 0;
+        for (int w = 0; w < wave; ++w) base += T.wsum[w];
+        int run = base + incl - local;
+        for (int w = w0; w < w0 + per && w < nw; ++w) {
+            T.qpref[w] = static_cast<uint16_t>(run);
+            run += __popc(T.qbits[w]);
+        }
+        if (tid == BLOCK - 1) *T.nu = base + incl;
+    }
+    __syncthreads();
+    const QueryLds QL{T.qbits, T.qpref, T.rval};
+    for (int i = tid; i < qn; i += BLOCK) {
+        bool p;
+        const int r = QL.rank_of(T.eterm[i], p);
+        T.erank[i] = static_cast<uint16_t>(r);
+        atomicMax(&T.rlast[r], i);
+        if constexpr (SQ) atomicAdd(&T.rsum[r], static_cast<uint32_t>(T.evalf[i]));
+    }
+    __syncthreads();
+    const int nu = *T.nu;
+    for (int r = tid; r < nu; r += BLOCK) T.rval[r] = T.evalf[T.rlast[r]];
+    if constexpr (DENSE) {
+        uint32_t* q32 = reinterpret_cast<uint32_t*>(T.qcode);
+        for (int w = tid; w < (a.ix.dim + 3) / 4; w += BLOCK) q32[w] = 0;
+        __syncthreads();
+        // get_dense_vector: the last entry of a repeated term wins
+        for (int i = tid; i < qn; i += BLOCK)
+            if (T.rlast[T.erank[i]] == i) T.qcode[T.eterm[i]] = static_cast<uint8_t>(T.evalf[i]);
+    }
+    return true;
+}
+
+// The summary columns of list `term` (summary terms [tb, te)) that the query's
+// terms select: rcol[rank of a distinct query term] = its column, left as the
+// caller preset it (-1) for terms the list's summaries lack. With per-list
+// summary-term bitmaps each query entry reads one bitmap word and one rank
+// prefix (a repeated term writes the same column); without, the list's term
+// ids are scanned against the query bitmap.
+template <int BLOCK>
+__device__ __forceinline__ void match_summary_columns(const DevIndexView& ix, uint32_t term, int32_t tb, int32_t te,
+                                                      int qn, const uint16_t* eterm, const uint16_t* erank,
+                                                      const QueryLds& QL, int32_t* rcol) {
+    const int tid = threadIdx.x;
+    if (ix.sum_bits != nullptr) {
+        const uint32_t* lbits = ix.sum_bits + static_cast<int64_t>(term) * ix.sum_nw;
+        const uint16_t* lrank = ix.sum_rank + static_cast<int64_t>(term) * ix.sum_nw;
+        for (int i = tid; i < qn; i += BLOCK) {
+            const uint32_t t = eterm[i];
+            const uint32_t w = lbits[t >> 5], bit = 1u << (t & 31);
+            const int32_t below = lrank[t >> 5];
+            if (w & bit) rcol[erank[i]] = tb + below + __popc(w & (bit - 1));
+        }
+    } else {
+        constexpr int kU = 4;  // independent term loads in flight per lane
+        for (int i0 = tb + tid; i0 < te; i0 += BLOCK * kU) {
+            uint32_t t[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int i = i0 + u * BLOCK;
+                t[u] = i < te ? ix.sum_term_ids[i] : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                if (t[u] == 0xFFFFFFFFu) continue;
+                bool p;
+                const int r = QL.rank_of(t[u], p);
+                if (p) rcol[r] = i0 + u * BLOCK;
+            }
+        }
+    }
+}
+
+// Integer summary scores of one list (uint8 codes; InvertedListClusters::
+// score_summaries_transposed, cluster/inverted_list_clusters.cpp:295-331):
+// sacc[cluster] += rsum[r] * code over every entry of the found columns
+// (flist[0, nf) holds their distinct-term ranks, rcol their columns). The
+// entries are flattened: one pass loads every column's range and query-code
+// sum, a prefix over their lengths numbers the entries, then all lanes take
+// entries with kU loads in flight, each finding its column by binary search.
+// Integer atomics make the per-cluster sums exact in any order. `uni` holds
+// kQMax * 12 + 4 bytes of scratch. Ends with a barrier.
+template <int BLOCK>
+__device__ __forceinline__ void accumulate_summaries_sq(const DevIndexView& ix, int32_t tb, int32_t te, int nf,
+                                                        const int32_t* flist, const int32_t* rcol,
+                                                        const uint32_t* rsum, unsigned char* uni, uint32_t* sacc) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint8_t* sv = static_cast<const uint8_t*>(ix.sum_value);
+    int32_t* fe0 = reinterpret_cast<int32_t*>(uni);  // column start, relative to the list's first entry
+    int32_t* fpre = fe0 + kQMax;                      // [nf + 1] exclusive prefix of column lengths
+    uint32_t* fqs = reinterpret_cast<uint32_t*>(fpre + kQMax + 1);
+    const int64_t lbase = te > tb ? ix.sum_term_ptr[tb] : 0;
+    for (int f = tid; f < nf; f += BLOCK) {
+        const int r = flist[f];
+        const int col = rcol[r];
+        const int64_t e0 = ix.sum_term_ptr[col], e1 = ix.sum_term_ptr[col + 1];
+        fe0[f] = static_cast<int32_t>(e0 - lbase);
+        fpre[f + 1] = static_cast<int32_t>(e1 - e0);
+        fqs[f] = rsum[r];
+    }
+    __syncthreads();
+    if (wave == 0) {
+        int run = 0;
+        for (int b = 0; b < nf; b += 64) {
+            const int f = b + lane;
+            const int len = f < nf ? fpre[f + 1] : 0;
+            const int incl = wave_incl_scan(len, lane);
+            if (f < nf) fpre[f + 1] = run + incl;
+            run += __shfl(incl, 63, 64);
+        }
+        if (lane == 0) fpre[0] = 0;
+    }
+    __syncthreads();
+    const int total_e = fpre[nf];
+    constexpr int kU = 4;
+    for (int x0 = tid; x0 < total_e; x0 += BLOCK * kU) {
+        int fx[kU];
+        int64_t ex[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int x = x0 + u * BLOCK;
+            fx[u] = -1;
+            ex[u] = lbase;
+            if (x < total_e) {
+                int lo = 0, hi = nf - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (fpre[mid] <= x)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                fx[u] = lo;
+                ex[u] = lbase + fe0[lo] + (x - fpre[lo]);
+            }
+        }
+        uint32_t cl[kU], vv[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            cl[u] = ix.sum_cluster[ex[u]];
+            vv[u] = sv[ex[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (fx[u] >= 0) atomicAdd(&sacc[cl[u]], fqs[fx[u]] * vv[u]);
+    }
+    __syncthreads();
+}
+
+}  // namespace nsparse_amd

@@ -1,0 +1,646 @@
+// SEISMIC-SQ search of a few queries spread over the whole GPU (uint8 codes,
+// block-contiguous rows). The per-query workgroup of search.hip keeps one
+// query's traversal on one CU; a single query then reads its rows at a small
+// fraction of HBM bandwidth. Here each query takes two launches:
+//
+//  1. spread_lists_kernel, grid (cut slots, 1 + G, queries): every workgroup
+//     loads the query and its cut (top_k_tokens). For cut list ci, workgroup
+//     y = 0 computes the list's summary scores (and, for the first list, the
+//     visiting order with the step-exact std::sort emulation); workgroups
+//     y = 1..G each score the rows of a slice of the list's clusters, writing
+//     one key per doc -- (score bits << 32 | ~doc id), 0 for a doc the selector
+//     rejects -- and the largest key of each cluster.
+//  2. spread_replay_kernel, one workgroup per query: walks the lists in cut
+//     order exactly as the reference does (query_single_inverted_list,
+//     seismic_scalar_quantized_index.cpp:67-135; single_query :307-357):
+//     prune test against the current k-th score, break on the first list,
+//     continue on later ones. A cluster can change the top-k only if it
+//     passes its test and its largest key beats the current k-th entry (or
+//     the top-k is not full), so 64 clusters are tested at once by one wave,
+//     and only those clusters' doc keys are merged, in visiting order; the
+//     k-th score moves only at such a merge, so every other cluster's test
+//     sees the value it would see in the sequential walk. The top-k equals
+//     the reference's under (score desc, doc id asc); a doc seen in an
+//     earlier list is either in the top-k (not inserted twice) or worse than
+//     its k-th entry, as in the reference, whose visited set skips it.
+//
+// Every doc of every cut list is scored, so a query reads somewhat more row
+// bytes than the pruned walk; the batched search (search.hip) stays the path
+// for large batches, where bytes bound the time.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+
+#include "device_index.h"
+#include "kernels.h"
+#include "search_common.h"
+#include "stl_order.h"
+
+namespace nsparse_amd {
+
+namespace {
+
+constexpr int kLBlock = 256;   // list workgroups
+constexpr int kRBlock = 1024;  // replay workgroup
+constexpr int kSubBatch = 2048;  // doc slots per scoring pass of a list workgroup
+constexpr int kWinPerStep = 4;   // 64-word windows per wave per step (loads in flight per lane)
+constexpr int kMaxCutSlots = 64;
+constexpr int kLdsCap = 160 * 1024;
+
+enum ListMisc { kLErr = 0, kLNu, kLNcut, kLFirst, kLNf, kLMiscCount = 16 };
+enum ReplayMisc { kRErr = 0, kRTkn, kRMiscCount = 16 };
+
+// LDS and scratch layouts (computed on the host).
+struct SpreadPlan {
+    int32_t nw;      // query bitmap words
+    int32_t mc;      // cluster capacity of a list
+    int32_t kmax;    // top-k capacity
+    int32_t ncap;    // cut slots (grid.x of the list launch)
+    int32_t keycap;  // doc keys of one list the replay stages in LDS
+    // list workgroups
+    int32_t o_qbits, o_qpref, o_eterm, o_erank, o_eval, o_rlast, o_rval, o_rsum, o_flist, o_cuts, o_wsum, o_misc;
+    int32_t o_cstart, o_acc, o_uni, total_list;
+    // replay workgroup (two buffers per list section)
+    int32_t r_cuts, r_misc, r_tks, r_tki, r_ord, r_cst, r_ssc, r_cmx, r_key, total_replay;
+    int32_t r_cst_stride, r_ssc_stride, r_cmx_stride, r_key_stride;  // bytes between the two buffers
+    // per-query scratch, bytes from the query's base
+    int64_t s_order, s_sum, s_cmax, s_key, s_bytes;
+    int64_t mld;  // doc capacity of a list (row stride of the key section)
+};
+
+__device__ __forceinline__ uint64_t doc_key(float s, int32_t id) {
+    return (static_cast<uint64_t>(__float_as_uint(s)) << 32) | static_cast<uint32_t>(~id);
+}
+__device__ __forceinline__ float key_score(uint64_t key) { return __uint_as_float(static_cast<uint32_t>(key >> 32)); }
+__device__ __forceinline__ int32_t key_doc(uint64_t key) { return static_cast<int32_t>(~static_cast<uint32_t>(key)); }
+
+// first i in [lo, hi) with a[i] >= v, hi when none
+__device__ __forceinline__ int first_ge(const int32_t* a, int lo, int hi, int v) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] >= v)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+// last i in [lo, hi) with a[i] <= v (a[lo] <= v is assumed)
+__device__ __forceinline__ int last_le(const int32_t* a, int lo, int hi, int v) {
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] <= v)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Copies n 16-byte units from global to LDS by threads t0, t0 + nt, ...,
+// four loads in flight per thread (a thread past the end reloads unit n - 1).
+__device__ __forceinline__ void copy_units(uint4* __restrict__ dst, const uint4* __restrict__ src, int n, int t0,
+                                           int nt) {
+    constexpr int kU = 4;
+    for (int i0 = t0; i0 < n; i0 += nt * kU) {
+        uint4 v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) v[u] = src[min(i0 + u * nt, n - 1)];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = i0 + u * nt;
+            if (i < n) dst[i] = v[u];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- launch 1
+
+__global__ __launch_bounds__(kLBlock) void spread_lists_kernel(SearchArgs a, SpreadPlan P, unsigned char* scratch) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int64_t q = a.query_base + blockIdx.z;
+    if (q >= a.n_queries) return;
+    const DevIndexView& ix = a.ix;
+    const int64_t s0 = a.q_indptr[q];
+    const int qn = static_cast<int>(a.q_indptr[q + 1] - s0);
+    const int k = a.ks != nullptr ? a.ks[q] : a.k_max;
+    const uint32_t* sel = a.sel_all;
+    if (a.sel_of_query != nullptr) {
+        const int s = a.sel_of_query[q];
+        if (s >= a.n_bitmaps) return;  // the replay reports a rejected query
+        sel = s >= 0 ? a.sel_bitmaps[s] : nullptr;
+    }
+    if (qn > kQMax || k <= 0 || k > P.kmax) return;
+    unsigned char* qs = scratch + static_cast<int64_t>(blockIdx.z) * P.s_bytes;
+
+    uint8_t* qcode = smem;
+    uint32_t* qbits = reinterpret_cast<uint32_t*>(smem + P.o_qbits);
+    uint16_t* qpref = reinterpret_cast<uint16_t*>(smem + P.o_qpref);
+    uint16_t* eterm = reinterpret_cast<uint16_t*>(smem + P.o_eterm);
+    uint16_t* erank = reinterpret_cast<uint16_t*>(smem + P.o_erank);
+    float* evalf = reinterpret_cast<float*>(smem + P.o_eval);
+    int32_t* rlast = reinterpret_cast<int32_t*>(smem + P.o_rlast);
+    float* rval = reinterpret_cast<float*>(smem + P.o_rval);
+    uint32_t* rsum = reinterpret_cast<uint32_t*>(smem + P.o_rsum);
+    int32_t* flist = reinterpret_cast<int32_t*>(smem + P.o_flist);
+    uint16_t* cuts = reinterpret_cast<uint16_t*>(smem + P.o_cuts);
+    int32_t* wsum = reinterpret_cast<int32_t*>(smem + P.o_wsum);
+    int32_t* misc = reinterpret_cast<int32_t*>(smem + P.o_misc);
+    int32_t* cstart = reinterpret_cast<int32_t*>(smem + P.o_cstart);
+    unsigned char* acc = smem + P.o_acc;
+    unsigned char* uni = smem + P.o_uni;
+
+    if (tid < kLMiscCount) misc[tid] = 0;
+    const QueryTables QT{qbits, qpref, eterm, erank, evalf, rlast, rval, rsum, qcode, wsum, &misc[kLErr], &misc[kLNu]};
+    if (!load_query_lds<true, true, kLBlock>(a, s0, qn, P.nw, QT)) return;  // the replay reports it
+    if (tid == 0) {
+        // top_k_tokens (utils/vector_process.h:18-30) on the codes
+        const int ncut = topk_tokens(
+            qn, a.cut, [&](int i) { return evalf[i]; }, [&](int i) { return static_cast<uint32_t>(eterm[i]); },
+            reinterpret_cast<KeyItem*>(uni), cuts);
+        int first = -1;  // the first cut list the walk visits (single_query :328-339)
+        for (int c = 0; c < ncut && first < 0; ++c)
+            if (cuts[c] < static_cast<uint32_t>(ix.n_lists)) first = c;
+        misc[kLNcut] = ncut;
+        misc[kLFirst] = first;
+    }
+    __syncthreads();
+    const int ncut = misc[kLNcut], first = misc[kLFirst];
+    const int ci = blockIdx.x;
+    if (ci == 0 && blockIdx.y == 0) {
+        int32_t* hdr = reinterpret_cast<int32_t*>(qs);
+        uint16_t* hcuts = reinterpret_cast<uint16_t*>(qs + 8);
+        if (tid == 0) {
+            hdr[0] = ncut;
+            hdr[1] = first;
+        }
+        for (int i = tid; i < ncut; i += kLBlock) hcuts[i] = cuts[i];
+    }
+    if (ci >= ncut) return;
+    const uint32_t term = cuts[ci];
+    if (term >= static_cast<uint32_t>(ix.n_lists)) return;
+    const int32_t lb = ix.list_cluster_base[term];
+    const int nc = ix.list_cluster_base[term + 1] - lb;
+    if (nc == 0 || nc > P.mc) return;
+
+    if (blockIdx.y == 0) {
+        // ---- summary scores of list ci (and the first list's visiting order) ----
+        const int32_t tb = ix.list_term_base[term], te = ix.list_term_base[term + 1];
+        int32_t* rcol = rlast;  // free after the query setup
+        uint32_t* sacc = reinterpret_cast<uint32_t*>(acc);
+        const int nu = misc[kLNu];
+        for (int r = tid; r < nu; r += kLBlock) rcol[r] = -1;
+        for (int c = tid; c < nc; c += kLBlock) sacc[c] = 0;
+        __syncthreads();
+        const QueryLds QL{qbits, qpref, rval};
+        match_summary_columns<kLBlock>(ix, term, tb, te, qn, eterm, erank, QL, rcol);
+        __syncthreads();
+        for (int r = tid; r < nu; r += kLBlock)
+            if (rcol[r] >= 0) flist[atomicAdd(&misc[kLNf], 1)] = r;
+        __syncthreads();
+        accumulate_summaries_sq<kLBlock>(ix, tb, te, misc[kLNf], flist, rcol, rsum, uni, sacc);
+        uint32_t* gsum = reinterpret_cast<uint32_t*>(qs + P.s_sum) + static_cast<int64_t>(ci) * P.mc;
+        for (int c = tid; c < nc; c += kLBlock) gsum[c] = sacc[c];
+        if (ci == first) {
+            // reorder_clusters (seismic_common.h:105-115)
+            KeyItem* A = reinterpret_cast<KeyItem*>(uni);
+            for (int c = tid; c < nc; c += kLBlock) A[c] = KeyItem{static_cast<float>(sacc[c]), static_cast<uint32_t>(c)};
+            __syncthreads();
+            uint32_t* leaf_bits = reinterpret_cast<uint32_t*>(flist);
+            if (wave == 0)
+                introsort_desc_wave(A, nc, reinterpret_cast<uint16_t*>(A + P.mc),
+                                    reinterpret_cast<uint16_t*>(A + P.mc) + P.mc, leaf_bits, lane);
+            __syncthreads();
+            introsort_final_pass(A, nc, leaf_bits, reinterpret_cast<uint16_t*>(qs + P.s_order), tid, kLBlock);
+        }
+        return;
+    }
+
+    // ---- doc keys of a slice of list ci's clusters ----
+    const int G = static_cast<int>(gridDim.y) - 1, g = static_cast<int>(blockIdx.y) - 1;
+    const int64_t dbase = ix.cluster_doc_ptr[lb];
+    for (int j = tid; j <= nc; j += kLBlock) cstart[j] = static_cast<int32_t>(ix.cluster_doc_ptr[lb + j] - dbase);
+    __syncthreads();
+    // the clusters whose first doc falls in this workgroup's share of the list
+    const int nd_list = cstart[nc];
+    const int b0 = static_cast<int>(static_cast<int64_t>(nd_list) * g / G);
+    const int b1 = static_cast<int>(static_cast<int64_t>(nd_list) * (g + 1) / G);
+    const int c0 = first_ge(cstart, 0, nc, b0);
+    const int c1 = g == G - 1 ? nc : first_ge(cstart, 0, nc, b1);
+    if (c0 >= c1) return;
+    unsigned long long* lcmax = reinterpret_cast<unsigned long long*>(acc);
+    for (int c = c0 + tid; c < c1; c += kLBlock) lcmax[c - c0] = 0;
+    int32_t* wst = reinterpret_cast<int32_t*>(uni);                // kSubBatch + 1 word starts
+    uint32_t* isc = reinterpret_cast<uint32_t*>(wst + kSubBatch + 1);  // kSubBatch sums
+    uint64_t* gkey = reinterpret_cast<uint64_t*>(qs + P.s_key) + static_cast<int64_t>(ci) * P.mld;
+    const uint4* idx8 = reinterpret_cast<const uint4*>(ix.blk_indices);
+    const uint2* val8 = reinterpret_cast<const uint2*>(ix.blk_codes);
+    const int p0 = cstart[c0], p1 = cstart[c1];
+    for (int pa = p0; pa < p1; pa += kSubBatch) {
+        const int nd = min(p1 - pa, kSubBatch);
+        __syncthreads();  // the previous pass is done with wst / isc
+        // the rows of consecutive postings are consecutive words
+        const int64_t w0 = ix.row_ref[dbase + pa] >> 16;
+        for (int j = tid; j < nd; j += kLBlock) {
+            const int64_t ref = ix.row_ref[dbase + pa + j];
+            const int ws = static_cast<int>((ref >> 16) - w0);
+            wst[j] = ws;
+            isc[j] = 0;
+            if (j == nd - 1) wst[nd] = ws + static_cast<int>(((ref & 0xFFFF) + 7) >> 3);
+        }
+        __syncthreads();
+        // Windows of 64 words, one per lane; a DPP prefix sum S over the window
+        // gives each doc run's sum: the last lane of a run adds S to its doc
+        // and subtracts S from the next run's doc (integer LDS atomics, exact
+        // in any order). Rows are zero padded to whole words; lanes past the
+        // end read the zero word after the last row.
+        const int W = wst[nd];
+        const int nwin = (W + 63) >> 6;
+        for (int x0 = wave * kWinPerStep; x0 < nwin; x0 += (kLBlock / 64) * kWinPerStep) {
+            int jv[kWinPerStep];
+            int64_t wv[kWinPerStep];
+#pragma unroll
+            for (int u = 0; u < kWinPerStep; ++u) {
+                const int f = ((x0 + u) << 6) + lane;
+                const bool has = f < W;
+                jv[u] = has ? last_le(wst, 0, nd, f) : -1;
+                wv[u] = has ? w0 + f : ix.blk_words;
+            }
+            uint4 ii[kWinPerStep];
+            uint2 vv[kWinPerStep];
+#pragma unroll
+            for (int u = 0; u < kWinPerStep; ++u) {
+                ii[u] = idx8[wv[u]];
+                vv[u] = val8[wv[u]];
+            }
+            uint32_t sums[kWinPerStep];
+#pragma unroll
+            for (int u = 0; u < kWinPerStep; ++u) {
+                const uint64_t codes = (static_cast<uint64_t>(vv[u].y) << 32) | vv[u].x;
+                const uint32_t tw[4] = {ii[u].x, ii[u].y, ii[u].z, ii[u].w};
+                uint32_t s = 0;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const uint32_t tm = (t & 1) ? (tw[t >> 1] >> 16) : (tw[t >> 1] & 0xFFFFu);
+                    s += (static_cast<uint32_t>(codes >> (8 * t)) & 0xFFu) * static_cast<uint32_t>(qcode[tm]);
+                }
+                sums[u] = s;
+            }
+#pragma unroll
+            for (int u = 0; u < kWinPerStep; ++u) sums[u] = wave_incl_scan_dpp(sums[u]);
+#pragma unroll
+            for (int u = 0; u < kWinPerStep; ++u) {
+                const int f = ((x0 + u) << 6) + lane;
+                const int j = jv[u];
+                // the doc of the next lane's word (a DPP shift left by one lane)
+                const int j1 = __builtin_amdgcn_update_dpp(0, j, 0x130, 0xF, 0xF, false);
+                const uint32_t S = sums[u];
+                if (j >= 0 && S != 0 && (j1 != j || lane == 63 || f + 1 >= W)) {
+                    atomicAdd(&isc[j], S);
+                    if (lane != 63 && f + 1 < W && j1 != j) atomicSub(&isc[j1], S);
+                }
+            }
+        }
+        __syncthreads();
+        for (int j = tid; j < nd; j += kLBlock) {
+            const int p = pa + j;
+            const int32_t id = ix.docs[dbase + p];
+            const bool keep = sel == nullptr || ((sel[id >> 5] >> (id & 31)) & 1u);
+            const uint64_t key = keep ? doc_key(static_cast<float>(isc[j]), id) : 0;
+            gkey[p] = key;
+            atomicMax(&lcmax[last_le(cstart, c0, c1, p) - c0], static_cast<unsigned long long>(key));
+        }
+    }
+    __syncthreads();
+    uint64_t* gcmax = reinterpret_cast<uint64_t*>(qs + P.s_cmax) + static_cast<int64_t>(ci) * P.mc;
+    for (int c = c0 + tid; c < c1; c += kLBlock) gcmax[c] = lcmax[c - c0];
+}
+
+// ---------------------------------------------------------------- launch 2
+
+__global__ __launch_bounds__(kRBlock) void spread_replay_kernel(SearchArgs a, SpreadPlan P,
+                                                              const unsigned char* scratch) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int64_t q = a.query_base + blockIdx.x;
+    if (q >= a.n_queries) return;
+    const DevIndexView& ix = a.ix;
+    const int64_t s0 = a.q_indptr[q];
+    const int qn = static_cast<int>(a.q_indptr[q + 1] - s0);
+    const int k = a.ks != nullptr ? a.ks[q] : a.k_max;
+    float* out_d = a.out_d + q * a.k_max;
+    int32_t* out_l = a.out_l + q * a.k_max;
+    bool bad_sel = false;
+    if (a.sel_of_query != nullptr) bad_sel = a.sel_of_query[q] >= a.n_bitmaps;
+    int32_t* misc = reinterpret_cast<int32_t*>(smem + P.r_misc);
+    if (tid < kRMiscCount) misc[tid] = 0;
+    __syncthreads();
+    int err = bad_sel ? kDevBadSelector : qn > kQMax ? kDevQueryTooLong : (k <= 0 || k > P.kmax) ? kDevBadK : 0;
+    if (err == 0) {
+        for (int i = tid; i < qn; i += kRBlock)
+            if (a.q_indices[s0 + i] >= static_cast<uint32_t>(ix.dim)) misc[kRErr] = 1;
+        __syncthreads();
+        if (misc[kRErr] != 0) err = kDevTermOutOfRange;
+    }
+    if (err != 0) {
+        if (tid == 0) atomicCAS(a.error_flag, 0, err);
+        for (int i = tid; i < a.k_max; i += kRBlock) {
+            out_d[i] = -1.0F;
+            out_l[i] = -1;
+        }
+        return;
+    }
+    const unsigned char* qs = scratch + static_cast<int64_t>(blockIdx.x) * P.s_bytes;
+    const int ncut = reinterpret_cast<const int32_t*>(qs)[0];
+    const int first_ci = reinterpret_cast<const int32_t*>(qs)[1];
+    uint16_t* cuts = reinterpret_cast<uint16_t*>(smem + P.r_cuts);
+    float* tks = reinterpret_cast<float*>(smem + P.r_tks);
+    int32_t* tki = reinterpret_cast<int32_t*>(smem + P.r_tki);
+    uint16_t* ord = reinterpret_cast<uint16_t*>(smem + P.r_ord);
+    for (int i = tid; i < ncut; i += kRBlock) cuts[i] = reinterpret_cast<const uint16_t*>(qs + 8)[i];
+    __syncthreads();
+    const float hf = a.heap_factor;
+
+    // the list sections of buffer b
+    auto cst_of = [&](int b) { return reinterpret_cast<int32_t*>(smem + P.r_cst + b * P.r_cst_stride); };
+    auto ssc_of = [&](int b) { return reinterpret_cast<float*>(smem + P.r_ssc + b * P.r_ssc_stride); };
+    auto cmx_of = [&](int b) { return reinterpret_cast<uint64_t*>(smem + P.r_cmx + b * P.r_cmx_stride); };
+    auto key_of = [&](int b) { return reinterpret_cast<uint64_t*>(smem + P.r_key + b * P.r_key_stride); };
+    const uint32_t* gsum_all = reinterpret_cast<const uint32_t*>(qs + P.s_sum);
+    const uint64_t* gcmax_all = reinterpret_cast<const uint64_t*>(qs + P.s_cmax);
+    const uint64_t* gkey_all = reinterpret_cast<const uint64_t*>(qs + P.s_key);
+
+    // Stages list ci's cluster starts, summary scores, largest keys, doc keys
+    // (when they fit) and, for the first list, its visiting order into buffer
+    // b, by threads t0, t0 + nt, ...
+    auto stage = [&](int ci, int b, int t0, int nt) {
+        const uint32_t term = cuts[ci];
+        if (term >= static_cast<uint32_t>(ix.n_lists)) return;
+        const int32_t lb = ix.list_cluster_base[term];
+        const int nc = ix.list_cluster_base[term + 1] - lb;
+        if (nc == 0 || nc > P.mc) return;
+        const int64_t dbase = ix.cluster_doc_ptr[lb];
+        const int64_t nd = ix.cluster_doc_ptr[lb + nc] - dbase;
+        int32_t* cst = cst_of(b);
+        float* ssc = ssc_of(b);
+        for (int c = t0; c <= nc; c += nt) cst[c] = static_cast<int32_t>(ix.cluster_doc_ptr[lb + c] - dbase);
+        const uint32_t* gs = gsum_all + static_cast<int64_t>(ci) * P.mc;
+        for (int c = t0; c < nc; c += nt) ssc[c] = static_cast<float>(gs[c]);
+        // 16-byte units: P.mc and P.mld are even, the sections 16-byte aligned
+        copy_units(reinterpret_cast<uint4*>(cmx_of(b)),
+                   reinterpret_cast<const uint4*>(gcmax_all + static_cast<int64_t>(ci) * P.mc), (nc + 1) >> 1, t0,
+                   nt);
+        if (nd <= P.keycap)
+            copy_units(reinterpret_cast<uint4*>(key_of(b)),
+                       reinterpret_cast<const uint4*>(gkey_all + static_cast<int64_t>(ci) * P.mld),
+                       static_cast<int>((nd + 1) >> 1), t0, nt);
+        if (ci == first_ci) {
+            const uint16_t* go = reinterpret_cast<const uint16_t*>(qs + P.s_order);
+            for (int c = t0; c < nc; c += nt) ord[c] = go[c];
+        }
+    };
+
+    // The exact walk of list ci from buffer b (wave 0).
+    int tkn = 0;
+    auto replay = [&](int ci, int b) {
+        const uint32_t term = cuts[ci];
+        if (term >= static_cast<uint32_t>(ix.n_lists)) return;
+        const int32_t lb = ix.list_cluster_base[term];
+        const int nc = ix.list_cluster_base[term + 1] - lb;
+        if (nc == 0 || nc > P.mc) return;
+        const int64_t nd = ix.cluster_doc_ptr[lb + nc] - ix.cluster_doc_ptr[lb];
+        const bool first = ci == first_ci;
+        const int32_t* cst = cst_of(b);
+        const float* ssc = ssc_of(b);
+        const uint64_t* cmx = cmx_of(b);
+        const uint64_t* keys = nd <= P.keycap ? key_of(b) : gkey_all + static_cast<int64_t>(ci) * P.mld;
+        for (int pos = 0; pos < nc; pos += 64) {
+            const int p = pos + lane;
+            const bool valid = p < nc;
+            const int c = valid ? (first ? static_cast<int>(ord[p]) : p) : 0;
+            const float s = valid ? ssc[c] : 0.0F;
+            const uint64_t mx = valid ? cmx[c] : 0;
+            uint64_t done = 0;  // lanes (visiting positions) already walked
+            bool brk = false;
+            while (true) {
+                // every test below sees the k-th score the sequential walk
+                // would see at that cluster: it changes only at a merge
+                const bool full = tkn == k;
+                const float kth = full ? tks[k - 1] : 0.0F;
+                const uint64_t tkey = full ? doc_key(kth, tki[k - 1]) : 0;
+                const bool fail = valid && full && (s * hf < kth);
+                const bool live = valid && !fail && mx != 0 && (!full || mx > tkey);
+                const uint64_t mf = ballot(fail) & ~done;
+                uint64_t m = ballot(live) & ~done;
+                if (first && mf != 0) {
+                    // the first list stops at its first failed test
+                    m &= (1ULL << (__ffsll(static_cast<unsigned long long>(mf)) - 1)) - 1;
+                    if (m == 0) {
+                        brk = true;
+                        break;
+                    }
+                }
+                if (m == 0) break;
+                const int i = __ffsll(static_cast<unsigned long long>(m)) - 1;
+                const int cc = __builtin_amdgcn_readlane(c, i);
+                const int d1 = cst[cc + 1];
+                for (int d = cst[cc]; d < d1; d += 64) {
+                    const int dd = d + lane;
+                    const uint64_t key = dd < d1 ? keys[dd] : 0;
+                    merge_topk_wave(key != 0, key_score(key), key_doc(key), k, tkn, tks, tki, lane);
+                }
+                done |= (2ULL << i) - 1;  // lanes <= i (all 64 when i = 63)
+                if (done == ~0ULL) break;
+            }
+            if (brk) break;
+        }
+    };
+
+    // List ci is walked from buffer ci & 1 while the other waves stage list
+    // ci + 1 into the other buffer.
+    if (ncut > 0) stage(0, 0, tid, kRBlock);
+    __syncthreads();
+    for (int ci = 0; ci < ncut; ++ci) {
+        if (wave == 0)
+            replay(ci, ci & 1);
+        else if (ci + 1 < ncut)
+            stage(ci + 1, (ci + 1) & 1, tid - 64, kRBlock - 64);
+        __syncthreads();
+    }
+    if (tid == 0) misc[kRTkn] = tkn;
+    __syncthreads();
+    const int n_out = misc[kRTkn];
+    for (int i = tid; i < a.k_max; i += kRBlock) {
+        float d = -1.0F;
+        int32_t l = -1;
+        if (i < k && i < n_out) {
+            d = tks[i] * a.decode_scale;
+            l = tki[i];
+        }
+        out_d[i] = d;
+        out_l[i] = l;
+    }
+}
+
+__global__ void spread_zero_flag_kernel(int32_t* p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
+__global__ void list_docs_max_kernel(const int32_t* __restrict__ lcb, const int64_t* __restrict__ cdp, int n_lists,
+                                     unsigned long long* __restrict__ out) {
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < n_lists; l += gridDim.x * blockDim.x)
+        atomicMax(out, static_cast<unsigned long long>(cdp[lcb[l + 1]] - cdp[lcb[l]]));
+}
+
+int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+
+SpreadPlan make_spread_plan(int dim, int max_clusters, int kmax, int cut, int64_t max_list_docs) {
+    SpreadPlan P{};
+    P.nw = (dim + 31) / 32;
+    P.mc = std::max(2, (max_clusters + 1) & ~1);
+    P.kmax = kmax;
+    P.ncap = std::max(1, std::min(cut, kQMax));
+    P.mld = std::max<int64_t>(2, (max_list_docs + 1) & ~int64_t{1});
+    int o = 0;
+    auto take = [&](int bytes) {
+        const int at = o;
+        o += (bytes + 15) & ~15;
+        return at;
+    };
+    take(dim);  // the dense code table at offset 0
+    P.o_qbits = take(P.nw * 4);
+    P.o_qpref = take(P.nw * 2);
+    P.o_eterm = take(kQMax * 2);
+    P.o_erank = take(kQMax * 2);
+    P.o_eval = take(kQMax * 4);
+    P.o_rlast = take(kQMax * 4);
+    P.o_rval = take(kQMax * 4);
+    P.o_rsum = take(kQMax * 4);
+    P.o_flist = take(kQMax * 4);
+    P.o_cuts = take(kQMax * 2);
+    P.o_wsum = take(kMaxWaves * 4);
+    P.o_misc = take(kLMiscCount * 4);
+    P.o_cstart = take((P.mc + 1) * 4);
+    P.o_acc = take(P.mc * 8);
+    // summary scratch, the cut heap, the sort buffers, or one scoring pass
+    const int uni = std::max(std::max(kQMax * 12 + 16, kQMax * 8), std::max(P.mc * 12, (2 * kSubBatch + 1) * 4));
+    P.o_uni = take(uni);
+    P.total_list = o;
+
+    o = 0;
+    P.r_cuts = take(kQMax * 2);
+    P.r_misc = take(kRMiscCount * 4);
+    P.r_tks = take(kmax * 4);
+    P.r_tki = take(kmax * 4);
+    P.r_ord = take(P.mc * 2);
+    P.r_cst_stride = ((P.mc + 1) * 4 + 15) & ~15;
+    P.r_ssc_stride = (P.mc * 4 + 15) & ~15;
+    P.r_cmx_stride = P.mc * 8;
+    P.r_cst = take(2 * P.r_cst_stride);
+    P.r_ssc = take(2 * P.r_ssc_stride);
+    P.r_cmx = take(2 * P.r_cmx_stride);
+    // doc keys: as many as fit, up to the longest list
+    const int64_t room = (kLdsCap - o) / 16;  // per buffer, in keys
+    P.keycap = static_cast<int32_t>(std::max<int64_t>(0, std::min<int64_t>(room & ~int64_t{1}, P.mld)));
+    P.r_key_stride = P.keycap * 8;
+    P.r_key = take(2 * P.r_key_stride);
+    P.total_replay = o;
+
+    int64_t s = align_up(8 + kQMax * 2, 256);
+    P.s_order = s;
+    s = align_up(s + P.mc * 2, 256);
+    P.s_sum = s;
+    s = align_up(s + static_cast<int64_t>(P.ncap) * P.mc * 4, 256);
+    P.s_cmax = s;
+    s = align_up(s + static_cast<int64_t>(P.ncap) * P.mc * 8, 256);
+    P.s_key = s;
+    s = align_up(s + static_cast<int64_t>(P.ncap) * P.mld * 8, 256);
+    P.s_bytes = s;
+    return P;
+}
+
+int cu_count() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    }
+    return cus;
+}
+
+}  // namespace
+
+bool search_spread_fits(const DevIndexView& ix, int max_clusters, int kmax, int cut, int64_t max_list_docs) {
+    if (ix.esz != 1 || ix.row_ref == nullptr || cut <= 0 || std::min(cut, kQMax) > kMaxCutSlots) return false;
+    // cluster ids are 16-bit in the visiting order; doc positions 32-bit
+    if (max_clusters <= 0 || max_clusters > 16384 || max_list_docs >= (int64_t{1} << 31)) return false;
+    const SpreadPlan P = make_spread_plan(ix.dim, max_clusters, kmax, cut, max_list_docs);
+    return P.total_list <= kLdsCap && P.total_replay <= kLdsCap && P.keycap > 0;
+}
+
+size_t search_spread_scratch_bytes(const DevIndexView& ix, int max_clusters, int kmax, int cut,
+                                   int64_t max_list_docs, int64_t n_queries) {
+    const SpreadPlan P = make_spread_plan(ix.dim, max_clusters, kmax, cut, max_list_docs);
+    return static_cast<size_t>(P.s_bytes) * static_cast<size_t>(n_queries);
+}
+
+void launch_search_spread(const SearchArgs& args, int max_clusters, int kmax, int64_t max_list_docs, void* scratch,
+                          hipStream_t stream) {
+    const SpreadPlan P = make_spread_plan(args.ix.dim, max_clusters, kmax, args.cut, max_list_docs);
+    if (!search_spread_fits(args.ix, max_clusters, kmax, args.cut, max_list_docs))
+        throw std::invalid_argument("the spread search does not fit this index");
+    if (args.error_flag == nullptr || scratch == nullptr)
+        throw std::invalid_argument("spread search launch without an error flag or scratch");
+    if (args.stats != nullptr) throw std::invalid_argument("the spread search keeps no traversal counters");
+    if (args.n_queries <= 0 || args.n_queries > 65535) throw std::invalid_argument("spread search batch size");
+    // doc-scoring workgroups per cut list: about two list workgroups per CU
+    int groups = std::max(1, 2 * cu_count() / P.ncap - 1);
+    if (const char* e = std::getenv("NSPARSE_SPREAD_GROUPS")) groups = std::max(1, std::atoi(e));
+    groups = std::min(groups, 1024);
+    static bool attrs = false;
+    if (!attrs) {
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(spread_lists_kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, kLdsCap),
+                  "hipFuncSetAttribute");
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(spread_replay_kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, kLdsCap),
+                  "hipFuncSetAttribute");
+        attrs = true;
+    }
+    SearchArgs a = args;
+    a.query_base = 0;
+    hipLaunchKernelGGL(spread_zero_flag_kernel, dim3(1), dim3(64), 0, stream, a.error_flag);
+    auto* s = static_cast<unsigned char*>(scratch);
+    hipLaunchKernelGGL(spread_lists_kernel, dim3(static_cast<unsigned>(P.ncap), static_cast<unsigned>(groups + 1),
+                                               static_cast<unsigned>(a.n_queries)),
+                       dim3(kLBlock), P.total_list, stream, a, P, s);
+    hipLaunchKernelGGL(spread_replay_kernel, dim3(static_cast<unsigned>(a.n_queries)), dim3(kRBlock), P.total_replay,
+                       stream, a, P, static_cast<const unsigned char*>(s));
+    hip_check(hipGetLastError(), "spread search launch");
+}
+
+int64_t max_list_docs(const DevIndexView& ix) {
+    if (ix.n_lists <= 0 || ix.list_cluster_base == nullptr || ix.cluster_doc_ptr == nullptr) return 0;
+    DeviceBuffer out(8);
+    hip_check(hipMemset(out.get(), 0, 8), "hipMemset");
+    const unsigned grid = static_cast<unsigned>(std::min((ix.n_lists + 255) / 256, 1024));
+    hipLaunchKernelGGL(list_docs_max_kernel, dim3(grid), dim3(256), 0, nullptr, ix.list_cluster_base,
+                       ix.cluster_doc_ptr, ix.n_lists, out.get<unsigned long long>());
+    hip_check(hipGetLastError(), "list size launch");
+    int64_t v = 0;
+    out.download(&v, 8);
+    return v;
+}
+
+}  // namespace nsparse_amd
