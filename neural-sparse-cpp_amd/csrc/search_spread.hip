@@ -7,9 +7,10 @@
 //     loads the query and its cut (top_k_tokens). For cut list ci, workgroup
 //     y = 0 computes the list's summary scores (and, for the first list, the
 //     visiting order with the step-exact std::sort emulation); workgroups
-//     y = 1..G each score the rows of a slice of the list's clusters, writing
-//     one key per doc -- (score bits << 32 | ~doc id), 0 for a doc the selector
-//     rejects -- and the largest key of each cluster.
+//     y = 1..G each score the rows of an equal share of the list's docs,
+//     writing one key per doc -- (score bits << 32 | ~doc id), 0 for a doc the
+//     selector rejects -- and the largest key of each cluster (an atomic max
+//     over the shares that hold its docs).
 //  2. spread_replay_kernel, one workgroup per query: walks the lists in cut
 //     order exactly as the reference does (query_single_inverted_list,
 //     seismic_scalar_quantized_index.cpp:67-135; single_query :307-357):
@@ -77,18 +78,6 @@ __device__ __forceinline__ uint64_t doc_key(float s, int32_t id) {
 }
 __device__ __forceinline__ float key_score(uint64_t key) { return __uint_as_float(static_cast<uint32_t>(key >> 32)); }
 __device__ __forceinline__ int32_t key_doc(uint64_t key) { return static_cast<int32_t>(~static_cast<uint32_t>(key)); }
-
-// first i in [lo, hi) with a[i] >= v, hi when none
-__device__ __forceinline__ int first_ge(const int32_t* a, int lo, int hi, int v) {
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a[mid] >= v)
-            hi = mid;
-        else
-            lo = mid + 1;
-    }
-    return lo;
-}
 
 // last i in [lo, hi) with a[i] <= v (a[lo] <= v is assumed)
 __device__ __forceinline__ int last_le(const int32_t* a, int lo, int hi, int v) {
@@ -224,18 +213,19 @@ __global__ __launch_bounds__(kLBlock) void spread_lists_kernel(SearchArgs a, Spr
         return;
     }
 
-    // ---- doc keys of a slice of list ci's clusters ----
+    // ---- doc keys of a slice of list ci's docs ----
     const int G = static_cast<int>(gridDim.y) - 1, g = static_cast<int>(blockIdx.y) - 1;
     const int64_t dbase = ix.cluster_doc_ptr[lb];
     for (int j = tid; j <= nc; j += kLBlock) cstart[j] = static_cast<int32_t>(ix.cluster_doc_ptr[lb + j] - dbase);
     __syncthreads();
-    // the clusters whose first doc falls in this workgroup's share of the list
+    // docs [p0, p1): an equal share of the list (a cluster can span shares);
+    // clusters [c0, c1) hold them
     const int nd_list = cstart[nc];
-    const int b0 = static_cast<int>(static_cast<int64_t>(nd_list) * g / G);
-    const int b1 = static_cast<int>(static_cast<int64_t>(nd_list) * (g + 1) / G);
-    const int c0 = first_ge(cstart, 0, nc, b0);
-    const int c1 = g == G - 1 ? nc : first_ge(cstart, 0, nc, b1);
-    if (c0 >= c1) return;
+    const int p0 = static_cast<int>(static_cast<int64_t>(nd_list) * g / G);
+    const int p1 = static_cast<int>(static_cast<int64_t>(nd_list) * (g + 1) / G);
+    if (p0 >= p1) return;
+    const int c0 = last_le(cstart, 0, nc, p0);
+    const int c1 = last_le(cstart, 0, nc, p1 - 1) + 1;
     unsigned long long* lcmax = reinterpret_cast<unsigned long long*>(acc);
     for (int c = c0 + tid; c < c1; c += kLBlock) lcmax[c - c0] = 0;
     int32_t* wst = reinterpret_cast<int32_t*>(uni);                // kSubBatch + 1 word starts
@@ -243,7 +233,6 @@ __global__ __launch_bounds__(kLBlock) void spread_lists_kernel(SearchArgs a, Spr
     uint64_t* gkey = reinterpret_cast<uint64_t*>(qs + P.s_key) + static_cast<int64_t>(ci) * P.mld;
     const uint4* idx8 = reinterpret_cast<const uint4*>(ix.blk_indices);
     const uint2* val8 = reinterpret_cast<const uint2*>(ix.blk_codes);
-    const int p0 = cstart[c0], p1 = cstart[c1];
     for (int pa = p0; pa < p1; pa += kSubBatch) {
         const int nd = min(p1 - pa, kSubBatch);
         __syncthreads();  // the previous pass is done with wst / isc
@@ -320,8 +309,11 @@ __global__ __launch_bounds__(kLBlock) void spread_lists_kernel(SearchArgs a, Spr
         }
     }
     __syncthreads();
-    uint64_t* gcmax = reinterpret_cast<uint64_t*>(qs + P.s_cmax) + static_cast<int64_t>(ci) * P.mc;
-    for (int c = c0 + tid; c < c1; c += kLBlock) gcmax[c] = lcmax[c - c0];
+    // the largest key of each cluster, over the shares that hold its docs
+    // (zeroed before the launch)
+    unsigned long long* gcmax = reinterpret_cast<unsigned long long*>(qs + P.s_cmax) + static_cast<int64_t>(ci) * P.mc;
+    for (int c = c0 + tid; c < c1; c += kLBlock)
+        if (lcmax[c - c0] != 0) atomicMax(&gcmax[c], lcmax[c - c0]);
 }
 
 // ---------------------------------------------------------------- launch 2
@@ -492,8 +484,14 @@ __global__ __launch_bounds__(kRBlock) void spread_replay_kernel(SearchArgs a, Sp
     }
 }
 
-__global__ void spread_zero_flag_kernel(int32_t* p) {
-    if (threadIdx.x == 0) *p = 0;
+// Clears the error flag and every query's largest-key section.
+__global__ void spread_prepare_kernel(int32_t* flag, unsigned char* scratch, int64_t s_bytes, int64_t s_cmax,
+                                      int64_t words, int64_t n) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *flag = 0;
+    const int64_t total = words * n;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < total;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        reinterpret_cast<uint64_t*>(scratch + (i / words) * s_bytes + s_cmax)[i % words] = 0;
 }
 
 __global__ void list_docs_max_kernel(const int32_t* __restrict__ lcb, const int64_t* __restrict__ cdp, int n_lists,
@@ -620,8 +618,11 @@ void launch_search_spread(const SearchArgs& args, int max_clusters, int kmax, in
     }
     SearchArgs a = args;
     a.query_base = 0;
-    hipLaunchKernelGGL(spread_zero_flag_kernel, dim3(1), dim3(64), 0, stream, a.error_flag);
     auto* s = static_cast<unsigned char*>(scratch);
+    const int64_t words = static_cast<int64_t>(P.ncap) * P.mc;
+    const unsigned zgrid = static_cast<unsigned>(std::min<int64_t>((words * a.n_queries + 255) / 256, 1024));
+    hipLaunchKernelGGL(spread_prepare_kernel, dim3(zgrid), dim3(256), 0, stream, a.error_flag, s, P.s_bytes,
+                       P.s_cmax, words, a.n_queries);
     hipLaunchKernelGGL(spread_lists_kernel, dim3(static_cast<unsigned>(P.ncap), static_cast<unsigned>(groups + 1),
                                                static_cast<unsigned>(a.n_queries)),
                        dim3(kLBlock), P.total_list, stream, a, P, s);
