@@ -1,7 +1,7 @@
 /*
- * Host-side hooks that expose the library's re-implementations of libstdc++
- * orderings so tests can compare them with std::ranges::sort and
- * std::priority_queue (no reference counterpart: test surface only).
+ * Hooks that expose the library's re-implementations of libstdc++ orderings,
+ * on the host and on the GPU, so tests can compare them with std::ranges::sort
+ * and std::priority_queue (no reference counterpart: test surface only).
  */
 #ifndef NSPARSE_TESTING_H
 #define NSPARSE_TESTING_H
@@ -20,6 +20,15 @@ int nsparse_testing_cluster_order(const float* keys, int n, int64_t* out_order);
  * terms best-first and their count. */
 int nsparse_testing_top_k_tokens(const uint16_t* terms, const float* weights, int n, int k, uint16_t* out,
                                  int* out_n);
+
+/* The first-list order as the GPU search computes it, on the GPU (host arrays,
+ * n <= 4096): workgroup 0 runs the one-wavefront sort, 1 the whole-workgroup one. */
+int nsparse_testing_device_cluster_order(const float* keys, int n, int workgroup, int64_t* out_order);
+
+/* The query cut as the GPU search computes it (one wavefront, registers), on
+ * the GPU: n <= 512, 1 <= k <= 64. */
+int nsparse_testing_device_top_k_tokens(const uint16_t* terms, const float* weights, int n, int k, uint16_t* out,
+                                        int* out_n);
 
 #ifdef __cplusplus
 }

@@ -592,3 +592,22 @@ extern "C" int nsparse_testing_top_k_tokens(const uint16_t* terms, const float* 
             heap.data(), out);
     });
 }
+
+extern "C" int nsparse_testing_device_cluster_order(const float* keys, int n, int workgroup, int64_t* out_order) {
+    return guarded([&] {
+        if (n < 0 || (n > 0 && (keys == nullptr || out_order == nullptr)))
+            throw std::invalid_argument("unexpected nullptr");
+        require_device();
+        device_cluster_order(keys, n, out_order, workgroup != 0);
+    });
+}
+
+extern "C" int nsparse_testing_device_top_k_tokens(const uint16_t* terms, const float* weights, int n, int k,
+                                                   uint16_t* out, int* out_n) {
+    return guarded([&] {
+        if (n < 0 || (n > 0 && (terms == nullptr || weights == nullptr || out == nullptr)) || out_n == nullptr)
+            throw std::invalid_argument("unexpected nullptr");
+        require_device();
+        *out_n = device_top_k_tokens(terms, weights, n, k, out);
+    });
+}

@@ -1,5 +1,5 @@
 // Host-callable entry points of the HIP translation units (search.hip, search_spread.hip,
-// build.hip, rescore.hip).
+// build.hip, rescore.hip, testing.hip).
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -88,6 +88,12 @@ void launch_search_spread(const SearchArgs& a, int max_clusters, int kmax, int64
                           hipStream_t stream);
 // Most docs any posting list of `ix` holds (device lists).
 int64_t max_list_docs(const DevIndexView& ix);
+// Test surface (testing.hip): the GPU's first-list order (one wavefront, or the
+// whole-workgroup form) of at most kMaxTestOrder keys, and its query cut
+// (topk_tokens_wave) of at most 512 entries with k <= 64; returns the count.
+constexpr int kMaxTestOrder = 4096;
+void device_cluster_order(const float* keys, int n, int64_t* out_order, bool block);
+int device_top_k_tokens(const uint16_t* terms, const float* weights, int n, int k, uint16_t* out);
 int search_lds_bytes(int dim, int max_clusters, int kmax, bool sq);
 std::string device_error_message(int32_t code);
 // Exhaustive search (BrutalIndex): every document scored, the k best kept per

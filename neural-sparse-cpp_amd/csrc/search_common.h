@@ -166,6 +166,103 @@ inline __device__ void introsort_desc_wave(KeyItem* A, int n, uint16_t* Lpos, ui
     }
 }
 
+// introsort_desc_wave by a whole workgroup, with the same result: the
+// partitions of one recursion level touch disjoint ranges, so each level's
+// ranges are shared out over the waves, one level after another (the order in
+// which disjoint ranges are finished does not change the result). A range
+// [lo, hi) uses positions lo .. hi - 1 of Lpos and Rasc (n each). `rng` holds
+// 2 * cap ranges (a level has at most 2 * n / 17 + 1 of them) and `cnt` two
+// ints of LDS. Barriers inside; leaf_bits as for introsort_desc_wave.
+template <int BLOCK>
+__device__ void introsort_desc_block(KeyItem* A, int n, uint16_t* Lpos, uint16_t* Rasc, uint32_t* leaf_bits, int4* rng,
+                                     int cap, int32_t* cnt) {
+    constexpr int kW = BLOCK / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t lt = (1ULL << lane) - 1;
+    for (int w = tid; w < (n + 31) / 32; w += BLOCK) leaf_bits[w] = w == 0 ? 1u : 0u;
+    if (tid == 0) {
+        cnt[0] = n > 1 ? 1 : 0;
+        cnt[1] = 0;
+        rng[0] = make_int4(0, n, floor_log2(max(n, 1)) * 2, 0);
+    }
+    __syncthreads();
+    for (int cur = 0;; cur ^= 1) {
+        const int ncur = cnt[cur];
+        if (ncur == 0) break;
+        for (int r = wave; r < ncur; r += kW) {
+            const int4 R = rng[cur * cap + r];
+            const int lo = R.x, hi = R.y;
+            int depth = R.z;
+            if (hi - lo <= kIntrosortThreshold) {
+                if (lane == 0 && lo < n) atomicOr(&leaf_bits[lo >> 5], 1u << (lo & 31));
+                continue;
+            }
+            if (depth == 0) {
+                if (lane == 0) {
+                    heap_sort_range(A, lo, hi);
+                    for (int x = lo; x < hi; ++x) atomicOr(&leaf_bits[x >> 5], 1u << (x & 31));
+                }
+                __builtin_amdgcn_wave_barrier();
+                continue;
+            }
+            --depth;
+            if (lane == 0) median_to_first(A, lo, lo + 1, lo + (hi - lo) / 2, hi - 1);
+            __builtin_amdgcn_wave_barrier();
+            uint16_t* L = Lpos + lo;
+            uint16_t* Ra = Rasc + lo;
+            const float pk = A[lo].key;
+            int nl = 0, nr = 0;
+            for (int x0 = lo + 1; x0 < hi; x0 += 64) {
+                const int x = x0 + lane;
+                const bool in = x < hi;
+                const float key = in ? A[x].key : 0.0F;
+                const bool isl = in && !(key > pk);
+                const bool isr = in && !(pk > key);
+                const uint64_t ml = ballot(isl), mr = ballot(isr);
+                if (isl) L[nl + __popcll(ml & lt)] = static_cast<uint16_t>(x);
+                if (isr) Ra[nr + __popcll(mr & lt)] = static_cast<uint16_t>(x);
+                nl += __popcll(ml);
+                nr += __popcll(mr);
+            }
+            __builtin_amdgcn_wave_barrier();
+            const int pairs = min(nl, nr);
+            int m = 0;
+            for (int i0 = 0; i0 < pairs; i0 += 64) {
+                const int i = i0 + lane;
+                const bool c = i < pairs && L[i] < Ra[nr - 1 - i];
+                m += __popcll(ballot(c));
+            }
+            for (int i0 = 0; i0 < m; i0 += 64) {
+                const int i = i0 + lane;
+                KeyItem va{}, vb{};
+                int pa = 0, pb = 0;
+                if (i < m) {
+                    pa = L[i];
+                    pb = Ra[nr - 1 - i];
+                    va = A[pa];
+                    vb = A[pb];
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (i < m) {
+                    A[pa] = vb;
+                    A[pb] = va;
+                }
+            }
+            int cut = m < nl ? static_cast<int>(L[m]) : 0x7FFFFFFF;
+            if (m > 0) cut = min(cut, static_cast<int>(Ra[nr - m]));
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) {
+                const int at = atomicAdd(&cnt[cur ^ 1], 2);
+                rng[(cur ^ 1) * cap + at] = make_int4(lo, cut, depth, 0);
+                rng[(cur ^ 1) * cap + at + 1] = make_int4(cut, hi, depth, 0);
+            }
+        }
+        __syncthreads();
+        if (tid == 0) cnt[cur] = 0;
+        __syncthreads();
+    }
+}
+
 // The final insertion sort as a stable sort of each leaf (see above): element
 // i goes to leaf_start + (elements of its leaf with a greater key, or an equal
 // key before it). Writes out[pos] = A[i].id. Any number of lanes.
@@ -192,6 +289,82 @@ inline __device__ void introsort_final_pass(const KeyItem* A, int n, const uint3
         }
         out[start + r] = static_cast<uint16_t>(A[i].id);
     }
+}
+
+// A heap of at most 64 KeyItems in one wavefront's registers: element j sits
+// in lane j, read with readlane and written by a lane-select at a
+// wave-uniform index, so every lane runs the heap code in step.
+struct LaneHeap {
+    float key = 0.0F;
+    uint32_t id = 0;
+    int lane = 0;
+};
+struct LaneSlot {
+    LaneHeap* h;
+    int x;
+    __device__ __forceinline__ operator KeyItem() const {
+        return KeyItem{__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, h->key), x)),
+                       static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(h->id), x))};
+    }
+    __device__ __forceinline__ LaneSlot& operator=(const KeyItem& v) {
+        const bool mine = h->lane == x;
+        h->key = mine ? v.key : h->key;
+        h->id = mine ? v.id : h->id;
+        return *this;
+    }
+    __device__ __forceinline__ LaneSlot& operator=(const LaneSlot& o) { return *this = static_cast<KeyItem>(o); }
+};
+struct LaneHeapRef {
+    LaneHeap* h;
+    __device__ __forceinline__ LaneSlot operator[](int x) const { return LaneSlot{h, x}; }
+};
+
+// topk_tokens (stl_order.h) by one wavefront, k <= 64, with its heap in
+// registers and the same heap steps. Entries are taken 64 at a time; only
+// those a ballot finds above the heap's threshold at the start of the 64 (all
+// of them while the heap fills) go through the sequential steps, where each is
+// tested again: the threshold only rises, so the others would be rejected by
+// the sequential loop too. Lane 0 writes `out`; returns the count.
+__device__ __forceinline__ int topk_tokens_wave(int n, int k, const float* weight, const uint16_t* term,
+                                                uint16_t* out, int lane) {
+    if (k >= n) {
+        for (int i = lane; i < n; i += 64) out[i] = term[i];
+        return n;
+    }
+    LaneHeap h;
+    h.lane = lane;
+    const LaneHeapRef heap{&h};
+    int size = 0;
+    float threshold = __builtin_inff();
+    for (int c0 = 0; c0 < n; c0 += 64) {
+        const int i = c0 + lane;
+        const bool in = i < n;
+        const float s = in ? weight[i] : 0.0F;
+        const int t = in ? static_cast<int>(term[i]) : 0;
+        uint64_t cand = ballot(in && !(size >= k && s <= threshold));
+        while (cand != 0) {
+            const int l = __ffsll(static_cast<unsigned long long>(cand)) - 1;
+            cand &= cand - 1;
+            const float sv = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, s), l));
+            const uint32_t tv = static_cast<uint32_t>(__builtin_amdgcn_readlane(t, l));
+            if (size >= k && sv <= threshold) continue;
+            if (size < k) {
+                pq_push(heap, size, KeyItem{sv, tv});
+                if (size == k) threshold = KeyItem(heap[0]).key;
+            } else {
+                pq_pop(heap, size);
+                pq_push(heap, size, KeyItem{sv, tv});
+                threshold = KeyItem(heap[0]).key;
+            }
+        }
+    }
+    const int total = size;
+    for (int idx = total - 1; idx >= 0; --idx) {
+        const uint32_t top = KeyItem(heap[0]).id;
+        if (lane == 0) out[idx] = static_cast<uint16_t>(top);
+        pq_pop(heap, size);
+    }
+    return total;
 }
 
 // Merges one wave's candidates (lane l: `in`, score `sc`, doc `id`) into the

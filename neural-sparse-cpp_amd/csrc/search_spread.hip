@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdlib>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -45,14 +46,14 @@ namespace nsparse_amd {
 
 namespace {
 
-constexpr int kLBlock = 256;   // list workgroups
+constexpr int kLBlock = 1024;  // list workgroups (16 waves share the first list's sort)
 constexpr int kRBlock = 1024;  // replay workgroup
 constexpr int kSubBatch = 2048;  // doc slots per scoring pass of a list workgroup
 constexpr int kWinPerStep = 4;   // 64-word windows per wave per step (loads in flight per lane)
 constexpr int kMaxCutSlots = 64;
 constexpr int kLdsCap = 160 * 1024;
 
-enum ListMisc { kLErr = 0, kLNu, kLNcut, kLFirst, kLNf, kLMiscCount = 16 };
+enum ListMisc { kLErr = 0, kLNu, kLNcut, kLFirst, kLNf, kLSortCnt, kLSortCnt1, kLMiscCount = 16 };
 enum ReplayMisc { kRErr = 0, kRTkn, kRMiscCount = 16 };
 
 // LDS and scratch layouts (computed on the host).
@@ -65,11 +66,12 @@ struct SpreadPlan {
     // list workgroups
     int32_t o_qbits, o_qpref, o_eterm, o_erank, o_eval, o_rlast, o_rval, o_rsum, o_flist, o_cuts, o_wsum, o_misc;
     int32_t o_cstart, o_acc, o_uni, total_list;
+    int32_t sort_rng, sort_cap;  // the first-list sort's range lists: byte offset in the union, ranges per level
     // replay workgroup (two buffers per list section)
-    int32_t r_cuts, r_misc, r_tks, r_tki, r_ord, r_cst, r_ssc, r_cmx, r_key, total_replay;
+    int32_t r_cuts, r_misc, r_lmax, r_tks, r_tki, r_ord, r_cst, r_ssc, r_cmx, r_key, total_replay;
     int32_t r_cst_stride, r_ssc_stride, r_cmx_stride, r_key_stride;  // bytes between the two buffers
     // per-query scratch, bytes from the query's base
-    int64_t s_order, s_sum, s_cmax, s_key, s_bytes;
+    int64_t s_order, s_sum, s_cmax, s_lmax, s_key, s_bytes;  // s_lmax directly follows the s_cmax section
     int64_t mld;  // doc capacity of a list (row stride of the key section)
 };
 
@@ -150,16 +152,17 @@ __global__ __launch_bounds__(kLBlock) void spread_lists_kernel(SearchArgs a, Spr
     if (tid < kLMiscCount) misc[tid] = 0;
     const QueryTables QT{qbits, qpref, eterm, erank, evalf, rlast, rval, rsum, qcode, wsum, &misc[kLErr], &misc[kLNu]};
     if (!load_query_lds<true, true, kLBlock>(a, s0, qn, P.nw, QT)) return;  // the replay reports it
-    if (tid == 0) {
-        // top_k_tokens (utils/vector_process.h:18-30) on the codes
-        const int ncut = topk_tokens(
-            qn, a.cut, [&](int i) { return evalf[i]; }, [&](int i) { return static_cast<uint32_t>(eterm[i]); },
-            reinterpret_cast<KeyItem*>(uni), cuts);
-        int first = -1;  // the first cut list the walk visits (single_query :328-339)
-        for (int c = 0; c < ncut && first < 0; ++c)
-            if (cuts[c] < static_cast<uint32_t>(ix.n_lists)) first = c;
-        misc[kLNcut] = ncut;
-        misc[kLFirst] = first;
+    if (wave == 0) {
+        // top_k_tokens (utils/vector_process.h:18-30) on the codes (cut <= 64)
+        const int ncut = topk_tokens_wave(qn, a.cut, evalf, eterm, cuts, lane);
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) {
+            int first = -1;  // the first cut list the walk visits (single_query :328-339)
+            for (int c = 0; c < ncut && first < 0; ++c)
+                if (cuts[c] < static_cast<uint32_t>(ix.n_lists)) first = c;
+            misc[kLNcut] = ncut;
+            misc[kLFirst] = first;
+        }
     }
     __syncthreads();
     const int ncut = misc[kLNcut], first = misc[kLFirst];
@@ -204,10 +207,9 @@ __global__ __launch_bounds__(kLBlock) void spread_lists_kernel(SearchArgs a, Spr
             for (int c = tid; c < nc; c += kLBlock) A[c] = KeyItem{static_cast<float>(sacc[c]), static_cast<uint32_t>(c)};
             __syncthreads();
             uint32_t* leaf_bits = reinterpret_cast<uint32_t*>(flist);
-            if (wave == 0)
-                introsort_desc_wave(A, nc, reinterpret_cast<uint16_t*>(A + P.mc),
-                                    reinterpret_cast<uint16_t*>(A + P.mc) + P.mc, leaf_bits, lane);
-            __syncthreads();
+            uint16_t* Lpos = reinterpret_cast<uint16_t*>(A + P.mc);
+            introsort_desc_block<kLBlock>(A, nc, Lpos, Lpos + P.mc, leaf_bits,
+                                          reinterpret_cast<int4*>(uni + P.sort_rng), P.sort_cap, &misc[kLSortCnt]);
             introsort_final_pass(A, nc, leaf_bits, reinterpret_cast<uint16_t*>(qs + P.s_order), tid, kLBlock);
         }
         return;
@@ -309,11 +311,24 @@ __global__ __launch_bounds__(kLBlock) void spread_lists_kernel(SearchArgs a, Spr
         }
     }
     __syncthreads();
-    // the largest key of each cluster, over the shares that hold its docs
-    // (zeroed before the launch)
+    // the largest key of each cluster, over the shares that hold its docs,
+    // and of the list (zeroed before the launch)
     unsigned long long* gcmax = reinterpret_cast<unsigned long long*>(qs + P.s_cmax) + static_cast<int64_t>(ci) * P.mc;
-    for (int c = c0 + tid; c < c1; c += kLBlock)
-        if (lcmax[c - c0] != 0) atomicMax(&gcmax[c], lcmax[c - c0]);
+    unsigned long long lmx = 0;
+    for (int c = c0 + tid; c < c1; c += kLBlock) {
+        const unsigned long long v = lcmax[c - c0];
+        if (v != 0) atomicMax(&gcmax[c], v);
+        lmx = v > lmx ? v : lmx;
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        const uint32_t hi = __shfl_xor(static_cast<uint32_t>(lmx >> 32), d, 64);
+        const uint32_t lo = __shfl_xor(static_cast<uint32_t>(lmx), d, 64);
+        const unsigned long long o = (static_cast<unsigned long long>(hi) << 32) | lo;
+        lmx = o > lmx ? o : lmx;
+    }
+    if (lane == 0 && lmx != 0)
+        atomicMax(reinterpret_cast<unsigned long long*>(qs + P.s_lmax) + ci, lmx);
 }
 
 // ---------------------------------------------------------------- launch 2
@@ -359,7 +374,15 @@ __global__ __launch_bounds__(kRBlock) void spread_replay_kernel(SearchArgs a, Sp
     float* tks = reinterpret_cast<float*>(smem + P.r_tks);
     int32_t* tki = reinterpret_cast<int32_t*>(smem + P.r_tki);
     uint16_t* ord = reinterpret_cast<uint16_t*>(smem + P.r_ord);
-    for (int i = tid; i < ncut; i += kRBlock) cuts[i] = reinterpret_cast<const uint16_t*>(qs + 8)[i];
+    uint64_t* lmax = reinterpret_cast<uint64_t*>(smem + P.r_lmax);
+    // snap[j & 1]: the k-th entry's key before list j is walked, 0 while the
+    // top-k is not full
+    uint64_t* snap = reinterpret_cast<uint64_t*>(misc + 4);
+    for (int i = tid; i < ncut; i += kRBlock) {
+        cuts[i] = reinterpret_cast<const uint16_t*>(qs + 8)[i];
+        lmax[i] = reinterpret_cast<const uint64_t*>(qs + P.s_lmax)[i];
+    }
+    if (tid < 2) snap[tid] = 0;
     __syncthreads();
     const float hf = a.heap_factor;
 
@@ -459,14 +482,23 @@ __global__ __launch_bounds__(kRBlock) void spread_replay_kernel(SearchArgs a, Sp
     };
 
     // List ci is walked from buffer ci & 1 while the other waves stage list
-    // ci + 1 into the other buffer.
+    // ci + 1 into the other buffer. Once the top-k is full, a list whose
+    // largest key does not beat its k-th entry can change nothing: it is
+    // neither staged nor walked. The stagers decide on the k-th entry before
+    // list ci, the walk on the current one (never lower), so a list that is
+    // walked has been staged.
     if (ncut > 0) stage(0, 0, tid, kRBlock);
     __syncthreads();
     for (int ci = 0; ci < ncut; ++ci) {
-        if (wave == 0)
-            replay(ci, ci & 1);
-        else if (ci + 1 < ncut)
-            stage(ci + 1, (ci + 1) & 1, tid - 64, kRBlock - 64);
+        if (wave == 0) {
+            const uint64_t now = tkn == k ? doc_key(tks[k - 1], tki[k - 1]) : 0;
+            if (now == 0 || lmax[ci] > now) replay(ci, ci & 1);
+            const uint64_t after = tkn == k ? doc_key(tks[k - 1], tki[k - 1]) : 0;
+            if (lane == 0) snap[(ci + 1) & 1] = after;
+        } else if (ci + 1 < ncut) {
+            const uint64_t before = snap[ci & 1];
+            if (before == 0 || lmax[ci + 1] > before) stage(ci + 1, (ci + 1) & 1, tid - 64, kRBlock - 64);
+        }
         __syncthreads();
     }
     if (tid == 0) misc[kRTkn] = tkn;
@@ -484,7 +516,7 @@ __global__ __launch_bounds__(kRBlock) void spread_replay_kernel(SearchArgs a, Sp
     }
 }
 
-// Clears the error flag and every query's largest-key section.
+// Clears the error flag and every query's largest-key sections.
 __global__ void spread_prepare_kernel(int32_t* flag, unsigned char* scratch, int64_t s_bytes, int64_t s_cmax,
                                       int64_t words, int64_t n) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *flag = 0;
@@ -531,13 +563,17 @@ SpreadPlan make_spread_plan(int dim, int max_clusters, int kmax, int cut, int64_
     P.o_cstart = take((P.mc + 1) * 4);
     P.o_acc = take(P.mc * 8);
     // summary scratch, the cut heap, the sort buffers, or one scoring pass
-    const int uni = std::max(std::max(kQMax * 12 + 16, kQMax * 8), std::max(P.mc * 12, (2 * kSubBatch + 1) * 4));
+    P.sort_cap = 2 * P.mc / 17 + 2;
+    P.sort_rng = (P.mc * 12 + 15) & ~15;
+    const int sort_bytes = P.sort_rng + 2 * P.sort_cap * 16;
+    const int uni = std::max(std::max(kQMax * 12 + 16, kQMax * 8), std::max(sort_bytes, (2 * kSubBatch + 1) * 4));
     P.o_uni = take(uni);
     P.total_list = o;
 
     o = 0;
     P.r_cuts = take(kQMax * 2);
     P.r_misc = take(kRMiscCount * 4);
+    P.r_lmax = take(P.ncap * 8);
     P.r_tks = take(kmax * 4);
     P.r_tki = take(kmax * 4);
     P.r_ord = take(P.mc * 2);
@@ -560,7 +596,8 @@ SpreadPlan make_spread_plan(int dim, int max_clusters, int kmax, int cut, int64_
     P.s_sum = s;
     s = align_up(s + static_cast<int64_t>(P.ncap) * P.mc * 4, 256);
     P.s_cmax = s;
-    s = align_up(s + static_cast<int64_t>(P.ncap) * P.mc * 8, 256);
+    P.s_lmax = s + static_cast<int64_t>(P.ncap) * P.mc * 8;
+    s = align_up(P.s_lmax + static_cast<int64_t>(P.ncap) * 8, 256);
     P.s_key = s;
     s = align_up(s + static_cast<int64_t>(P.ncap) * P.mld * 8, 256);
     P.s_bytes = s;
@@ -568,12 +605,12 @@ SpreadPlan make_spread_plan(int dim, int max_clusters, int kmax, int cut, int64_
 }
 
 int cu_count() {
-    static int cus = 0;
-    if (cus == 0) {
-        int dev = 0;
+    static const int cus = [] {
+        int dev = 0, n = 0;
         hip_check(hipGetDevice(&dev), "hipGetDevice");
-        hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
-    }
+        hip_check(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+        return n;
+    }();
     return cus;
 }
 
@@ -602,24 +639,37 @@ void launch_search_spread(const SearchArgs& args, int max_clusters, int kmax, in
         throw std::invalid_argument("spread search launch without an error flag or scratch");
     if (args.stats != nullptr) throw std::invalid_argument("the spread search keeps no traversal counters");
     if (args.n_queries <= 0 || args.n_queries > 65535) throw std::invalid_argument("spread search batch size");
-    // doc-scoring workgroups per cut list: about two list workgroups per CU
-    int groups = std::max(1, 2 * cu_count() / P.ncap - 1);
+    // doc-scoring workgroups per cut list: one wave of list workgroups fills the GPU
+    int per_cu = 1;
+    {
+        static std::mutex mu;
+        static bool attrs = false;
+        static int cached = 0, cached_lds = -1;
+        std::lock_guard<std::mutex> lock(mu);
+        if (!attrs) {
+            hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(spread_lists_kernel),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, kLdsCap),
+                      "hipFuncSetAttribute");
+            hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(spread_replay_kernel),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, kLdsCap),
+                      "hipFuncSetAttribute");
+            attrs = true;
+        }
+        if (cached_lds != P.total_list) {
+            hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &cached, reinterpret_cast<const void*>(spread_lists_kernel), kLBlock, P.total_list),
+                      "occupancy");
+            cached_lds = P.total_list;
+        }
+        per_cu = std::max(cached, 1);
+    }
+    int groups = std::max(1, per_cu * cu_count() / P.ncap - 1);
     if (const char* e = std::getenv("NSPARSE_SPREAD_GROUPS")) groups = std::max(1, std::atoi(e));
     groups = std::min(groups, 1024);
-    static bool attrs = false;
-    if (!attrs) {
-        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(spread_lists_kernel),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, kLdsCap),
-                  "hipFuncSetAttribute");
-        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(spread_replay_kernel),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, kLdsCap),
-                  "hipFuncSetAttribute");
-        attrs = true;
-    }
     SearchArgs a = args;
     a.query_base = 0;
     auto* s = static_cast<unsigned char*>(scratch);
-    const int64_t words = static_cast<int64_t>(P.ncap) * P.mc;
+    const int64_t words = static_cast<int64_t>(P.ncap) * (P.mc + 1);  // the cluster and list maxima
     const unsigned zgrid = static_cast<unsigned>(std::min<int64_t>((words * a.n_queries + 255) / 256, 1024));
     hipLaunchKernelGGL(spread_prepare_kernel, dim3(zgrid), dim3(256), 0, stream, a.error_flag, s, P.s_bytes,
                        P.s_cmax, words, a.n_queries);

@@ -45,8 +45,12 @@ NS_HD int floor_log2(int n) {
     return r;
 }
 
+// The heap routines take any random-access handle F over KeyItem (a pointer,
+// or on the GPU a heap held in a wavefront's registers).
+
 // bits/stl_heap.h __push_heap
-NS_HD void heap_push_hole(KeyItem* f, int hole, int top, KeyItem value) {
+template <class F>
+NS_HD void heap_push_hole(F f, int hole, int top, KeyItem value) {
     int parent = (hole - 1) / 2;
     while (hole > top && greater_key(f[parent], value)) {
         f[hole] = f[parent];
@@ -57,7 +61,8 @@ NS_HD void heap_push_hole(KeyItem* f, int hole, int top, KeyItem value) {
 }
 
 // bits/stl_heap.h __adjust_heap
-NS_HD void heap_adjust(KeyItem* f, int hole, int len, KeyItem value) {
+template <class F>
+NS_HD void heap_adjust(F f, int hole, int len, KeyItem value) {
     const int top = hole;
     int second = hole;
     while (second < (len - 1) / 2) {
@@ -75,17 +80,19 @@ NS_HD void heap_adjust(KeyItem* f, int hole, int len, KeyItem value) {
 }
 
 // std::priority_queue<Item, vector, greater-on-score>::push
-NS_HD void pq_push(KeyItem* heap, int& size, KeyItem v) {
+template <class F>
+NS_HD void pq_push(F heap, int& size, KeyItem v) {
     heap[size] = v;
     ++size;
     heap_push_hole(heap, size - 1, 0, v);
 }
 
 // std::priority_queue::pop (std::pop_heap, then pop_back)
-NS_HD void pq_pop(KeyItem* heap, int& size) {
+template <class F>
+NS_HD void pq_pop(F heap, int& size) {
     if (size > 1) {
         const int last = size - 1;
-        KeyItem value = heap[last];
+        const KeyItem value = heap[last];
         heap[last] = heap[0];
         heap_adjust(heap, 0, last, value);
     }
@@ -96,8 +103,8 @@ NS_HD void pq_pop(KeyItem* heap, int& size) {
 // keeps `k` items of the `n` (weights[i], terms[i]) pairs and writes their
 // terms best-first into `out`; returns the count. `heap` needs k slots.
 // When k >= n the reference returns all terms in input order.
-template <class GetW, class GetT>
-NS_HD int topk_tokens(int n, int k, GetW weight, GetT term, KeyItem* heap, uint16_t* out) {
+template <class GetW, class GetT, class F>
+NS_HD int topk_tokens(int n, int k, GetW weight, GetT term, F heap, uint16_t* out) {
     if (k >= n) {
         for (int i = 0; i < n; ++i) out[i] = static_cast<uint16_t>(term(i));
         return n;
@@ -109,16 +116,16 @@ NS_HD int topk_tokens(int n, int k, GetW weight, GetT term, KeyItem* heap, uint1
         if (size >= k && s <= threshold) continue;
         if (size < k) {
             pq_push(heap, size, KeyItem{s, static_cast<uint32_t>(term(i))});
-            if (size == k) threshold = heap[0].key;
+            if (size == k) threshold = KeyItem(heap[0]).key;
         } else {
             pq_pop(heap, size);
             pq_push(heap, size, KeyItem{s, static_cast<uint32_t>(term(i))});
-            threshold = heap[0].key;
+            threshold = KeyItem(heap[0]).key;
         }
     }
     const int total = size;
     for (int idx = total - 1; idx >= 0; --idx) {
-        out[idx] = static_cast<uint16_t>(heap[0].id);
+        out[idx] = static_cast<uint16_t>(KeyItem(heap[0]).id);
         pq_pop(heap, size);
     }
     return total;

@@ -154,6 +154,8 @@ _sig("nsparse_synth_row_nnz", [P(SynthSpec), c_int64, c_int64, c_void_p, c_int])
 _sig("nsparse_synth_fill", [P(SynthSpec), c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int])
 _sig("nsparse_testing_cluster_order", [c_void_p, c_int, c_void_p])
 _sig("nsparse_testing_top_k_tokens", [c_void_p, c_void_p, c_int, c_int, c_void_p, P(c_int)])
+_sig("nsparse_testing_device_cluster_order", [c_void_p, c_int, c_int, c_void_p])
+_sig("nsparse_testing_device_top_k_tokens", [c_void_p, c_void_p, c_int, c_int, c_void_p, P(c_int)])
 
 # nsparse_status -> the exception the reference's SWIG layer raises
 # (nsparse/python/swignsparse.swig:47-59).
