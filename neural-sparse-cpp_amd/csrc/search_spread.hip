@@ -469,10 +469,24 @@ __global__ __launch_bounds__(kRBlock) void spread_replay_kernel(SearchArgs a, Sp
                 const int i = __ffsll(static_cast<unsigned long long>(m)) - 1;
                 const int cc = __builtin_amdgcn_readlane(c, i);
                 const int d1 = cst[cc + 1];
-                for (int d = cst[cc]; d < d1; d += 64) {
-                    const int dd = d + lane;
-                    const uint64_t key = dd < d1 ? keys[dd] : 0;
-                    merge_topk_wave(key != 0, key_score(key), key_doc(key), k, tkn, tks, tki, lane);
+                // The cluster's docs, 64 at a time with kU loads in flight: a
+                // merge runs only for 64 that hold a doc better than the
+                // current k-th entry (or while the top-k fills).
+                constexpr int kU = 4;
+                uint64_t kk = tkey;
+                for (int d = cst[cc]; d < d1; d += 64 * kU) {
+                    uint64_t kv[kU];
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        const int dd = d + u * 64 + lane;
+                        kv[u] = dd < d1 ? keys[dd] : 0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        if (ballot(kv[u] != 0 && (tkn < k || kv[u] > kk)) == 0) continue;
+                        merge_topk_wave(kv[u] != 0, key_score(kv[u]), key_doc(kv[u]), k, tkn, tks, tki, lane);
+                        kk = tkn == k ? doc_key(tks[k - 1], tki[k - 1]) : 0;
+                    }
                 }
                 done |= (2ULL << i) - 1;  // lanes <= i (all 64 when i = 63)
                 if (done == ~0ULL) break;
