@@ -1072,15 +1072,94 @@ __global__ void list_bases_kernel(const int32_t* __restrict__ lcb, const int64_t
 
 // ---------------------------------------------------------------- block rows
 
-// 8-element words of each posting's row copy; flags rows too long for the
-// 16-bit length field of row_ref.
-__global__ void block_words_kernel(const int32_t* __restrict__ indptr, const int32_t* __restrict__ docs, size_t n,
-                                   int64_t* __restrict__ words, int32_t* __restrict__ too_long) {
+// Encodes one uint8 row into block words (the layout in device_index.h) at
+// `out` and returns their count; with out == nullptr it only counts. One
+// thread per row: the low terms first, 8 slots to a word, then the other
+// terms in row order under word bases.
+__device__ int encode_block_row(const uint16_t* __restrict__ t, const uint8_t* __restrict__ c, int len,
+                                uint4* __restrict__ out) {
+    int nw = 0, ns = 0;
+    uint64_t lo = 0, hi = 0;
+    auto put = [&](uint32_t slot) {
+        if (ns < 4)
+            lo |= static_cast<uint64_t>(slot) << (16 * ns);
+        else
+            hi |= static_cast<uint64_t>(slot) << (16 * (ns - 4));
+        ++ns;
+    };
+    auto flush = [&]() {
+        if (out != nullptr)
+            out[nw] = make_uint4(static_cast<uint32_t>(lo), static_cast<uint32_t>(lo >> 32),
+                                 static_cast<uint32_t>(hi), static_cast<uint32_t>(hi >> 32));
+        ++nw;
+        ns = 0;
+        lo = hi = 0;
+    };
+    for (int i = 0; i < len; ++i) {
+        const uint32_t term = t[i];
+        uint32_t code = c[i];
+        if (term >= static_cast<uint32_t>(kBlkTermSpan)) continue;
+        while (code > 0) {
+            const uint32_t part = min(code, 15U);
+            put((term << 4) | part);
+            code -= part;
+            if (ns == 8) flush();
+        }
+    }
+    if (ns > 0) flush();
+    uint32_t base = 0;
+    for (int i = 0; i < len; ++i) {
+        const uint32_t term = t[i];
+        uint32_t code = c[i];
+        if (term < static_cast<uint32_t>(kBlkTermSpan)) continue;
+        while (code > 0) {
+            if (ns > 0 && (term < base || term - base >= static_cast<uint32_t>(kBlkTermSpan))) flush();
+            if (ns == 0) {
+                base = term & ~15U;
+                put(base);  // slot 0: the base, code nibble 0
+            }
+            const uint32_t part = min(code, 15U);
+            put(((term - base) << 4) | part);
+            code -= part;
+            if (ns == 8) flush();
+        }
+    }
+    if (ns > 0) flush();
+    return nw;
+}
+
+// Block words of each doc's encoded row, rounded up to whole pairs (the
+// search reads a row as 32-byte pairs of words); flags rows whose nnz or word
+// count exceed the row_ref fields.
+__global__ void doc_block_words_kernel(const int32_t* __restrict__ indptr, const uint16_t* __restrict__ indices,
+                                       const uint8_t* __restrict__ values, size_t n, int64_t* __restrict__ words,
+                                       int32_t* __restrict__ too_long) {
+    GRID_STRIDE(d, n) {
+        const int32_t r0 = indptr[d], len = indptr[d + 1] - r0;
+        const int nw = (encode_block_row(indices + r0, values + r0, len, nullptr) + 1) & ~1;
+        words[d] = nw;
+        if (len > kBlkMaxField || nw > kBlkMaxField) atomicOr(too_long, 1);
+    }
+}
+
+// Each doc's encoded row at its offset dws[d] of `rows`, a zero word
+// completing an odd count.
+__global__ void doc_block_encode_kernel(const int32_t* __restrict__ indptr, const uint16_t* __restrict__ indices,
+                                        const uint8_t* __restrict__ values, size_t n, const int64_t* __restrict__ dws,
+                                        uint4* __restrict__ rows) {
+    GRID_STRIDE(d, n) {
+        const int32_t r0 = indptr[d], len = indptr[d + 1] - r0;
+        const int nw = encode_block_row(indices + r0, values + r0, len, rows + dws[d]);
+        if (nw & 1) rows[dws[d] + nw] = make_uint4(0, 0, 0, 0);
+    }
+}
+
+// Word count of each posting's row copy.
+__global__ void block_words_kernel(const int32_t* __restrict__ docs, const int64_t* __restrict__ dws, size_t n,
+                                   int64_t* __restrict__ words) {
     GRID_STRIDE(p, n) {
         const int32_t d = docs[p];
-        const int32_t len = indptr[d + 1] - indptr[d];
-        words[p] = (len + 7) >> 3;
-        if (len > 0xFFFF) atomicOr(too_long, 1);
+        words[p] = dws[d + 1] - dws[d];
     }
 }
 
@@ -1094,36 +1173,24 @@ __global__ void list_word_span_kernel(const int32_t* __restrict__ lcb, const int
     }
 }
 
-// One 16-lane group per posting copies its row into zero-padded 8-element
-// words (16 B of terms + 8 B of codes, each written whole) and writes row_ref.
+// One 16-lane group per posting copies its doc's encoded words (16 B each)
+// and writes row_ref.
 __global__ __launch_bounds__(kB) void block_copy_kernel(const int32_t* __restrict__ indptr,
-                                                        const uint16_t* __restrict__ indices,
-                                                        const uint8_t* __restrict__ values,
                                                         const int32_t* __restrict__ docs,
+                                                        const int64_t* __restrict__ dws,
+                                                        const uint4* __restrict__ doc_rows,
                                                         const int64_t* __restrict__ ws, size_t n,
-                                                        uint4* __restrict__ blk_idx, uint2* __restrict__ blk_val,
+                                                        uint4* __restrict__ blk_rows,
                                                         int64_t* __restrict__ row_ref) {
     const int g = threadIdx.x >> 4, gl = threadIdx.x & 15;
     for (size_t p = static_cast<size_t>(blockIdx.x) * (kB / 16) + g; p < n;
          p += static_cast<size_t>(gridDim.x) * (kB / 16)) {
         const int32_t d = docs[p];
-        const int32_t r0 = indptr[d], len = indptr[d + 1] - r0;
+        const int64_t s0 = dws[d];
+        const int nw = static_cast<int>(dws[d + 1] - s0);
         const int64_t w0 = ws[p];
-        if (gl == 0) row_ref[p] = (w0 << 16) | len;
-        const int nw = (len + 7) >> 3;
-        for (int w = gl; w < nw; w += 16) {
-            uint32_t t[4] = {0, 0, 0, 0}, c[2] = {0, 0};
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int i = w * 8 + e;
-                if (i < len) {
-                    t[e >> 1] |= static_cast<uint32_t>(indices[r0 + i]) << (16 * (e & 1));
-                    c[e >> 2] |= static_cast<uint32_t>(values[r0 + i]) << (8 * (e & 3));
-                }
-            }
-            blk_idx[w0 + w] = make_uint4(t[0], t[1], t[2], t[3]);
-            blk_val[w0 + w] = make_uint2(c[0], c[1]);
-        }
+        if (gl == 0) row_ref[p] = blk_ref(w0, nw, indptr[d + 1] - indptr[d]);
+        for (int w = gl; w < nw; w += 16) blk_rows[w0 + w] = doc_rows[s0 + w];
     }
 }
 
@@ -1274,22 +1341,41 @@ bool build_summary_bitmaps(const DevIndexView& ix, DeviceBuffer& bits, DeviceBuf
     return true;
 }
 
-bool build_block_rows(const DevIndexView& ix, int64_t n_postings, DeviceBuffer& row_ref, DeviceBuffer& blk_indices,
-                      DeviceBuffer& blk_codes, int64_t& n_words) {
+bool build_block_rows(const DevIndexView& ix, int64_t n_postings, DeviceBuffer& row_ref, DeviceBuffer& blk_rows,
+                      int64_t& n_words) {
     n_words = 0;
     row_ref.release();
-    blk_indices.release();
-    blk_codes.release();
-    if (ix.esz != 1 || n_postings <= 0) return false;
+    blk_rows.release();
+    if (ix.esz != 1 || n_postings <= 0 || ix.n_docs <= 0) return false;
     if (ix.indptr == nullptr || ix.indices == nullptr || ix.values == nullptr || ix.docs == nullptr ||
         ix.list_cluster_base == nullptr || ix.cluster_doc_ptr == nullptr)
         throw std::logic_error("block rows need the forward index and the posting lists on the device");
-    const size_t D = static_cast<size_t>(n_postings);
-    DeviceBuffer words(D * 8), ws((D + 1) * 8), flag(4), span(8);
+    const size_t N = static_cast<size_t>(ix.n_docs), D = static_cast<size_t>(n_postings);
+    const auto* values = static_cast<const uint8_t*>(ix.values);
+    // each doc's row, encoded once
+    DeviceBuffer dw(N * 8), dws((N + 1) * 8), flag(4), span(8);
     hip_check(hipMemset(flag.get(), 0, 4), "hipMemset");
     hip_check(hipMemset(span.get(), 0, 8), "hipMemset");
-    hipLaunchKernelGGL(block_words_kernel, dim3(grid_for(D)), dim3(kB), 0, nullptr, ix.indptr, ix.docs, D,
-                       words.get<int64_t>(), flag.get<int32_t>());
+    hipLaunchKernelGGL(doc_block_words_kernel, dim3(grid_for(N)), dim3(kB), 0, nullptr, ix.indptr, ix.indices, values,
+                       N, dw.get<int64_t>(), flag.get<int32_t>());
+    exclusive_scan_i64(dw.get<int64_t>(), dws.get<int64_t>(), N);
+    sync("block row encoding sizes");
+    dw.release();
+    int32_t too_long = 0;
+    int64_t doc_words = 0;
+    flag.download(&too_long, 4);
+    dws.download(&doc_words, 8, N * 8);
+    if (too_long != 0) return false;
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    if (static_cast<size_t>(doc_words) * 16 + D * 24 + (size_t{1} << 30) > free_b) return false;
+    DeviceBuffer doc_rows(static_cast<size_t>(doc_words) * 16, 64);
+    hipLaunchKernelGGL(doc_block_encode_kernel, dim3(grid_for(N)), dim3(kB), 0, nullptr, ix.indptr, ix.indices,
+                       values, N, dws.get<int64_t>(), doc_rows.get<uint4>());
+    // then every posting's copy of its doc's words
+    DeviceBuffer words(D * 8), ws((D + 1) * 8);
+    hipLaunchKernelGGL(block_words_kernel, dim3(grid_for(D)), dim3(kB), 0, nullptr, ix.docs, dws.get<int64_t>(), D,
+                       words.get<int64_t>());
     exclusive_scan_i64(words.get<int64_t>(), ws.get<int64_t>(), D);
     hipLaunchKernelGGL(list_word_span_kernel, dim3(grid_for(static_cast<size_t>(ix.n_lists))), dim3(kB), 0, nullptr,
                        ix.list_cluster_base, ix.cluster_doc_ptr, ws.get<int64_t>(), ix.n_lists,
@@ -1297,23 +1383,18 @@ bool build_block_rows(const DevIndexView& ix, int64_t n_postings, DeviceBuffer& 
     sync("block row sizes");
     words.release();
     int64_t W = 0, max_span = 0;
-    int32_t too_long = 0;
     ws.download(&W, 8, D * 8);
     span.download(&max_span, 8);
-    flag.download(&too_long, 4);
-    // the search keeps a list's row offsets as 32-bit element counts
-    if (too_long != 0 || max_span * 8 >= (int64_t{1} << 31)) return false;
-    size_t free_b = 0, total_b = 0;
+    // the search keeps a list's row offsets as 32-bit counts of 8ths of words
+    if (max_span * 8 >= (int64_t{1} << 31)) return false;
     hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    const size_t need = static_cast<size_t>(W) * 24 + D * 8 + (size_t{1} << 30);
-    if (need > free_b) return false;
+    if (static_cast<size_t>(W) * 16 + D * 8 + (size_t{1} << 30) > free_b) return false;
     row_ref.allocate(D * 8);
-    blk_indices.allocate(static_cast<size_t>(W) * 16, 64);
-    blk_codes.allocate(static_cast<size_t>(W) * 8, 64);
+    blk_rows.allocate(static_cast<size_t>(W) * 16, 64);
     const unsigned grid = static_cast<unsigned>(std::min<size_t>((D + kB / 16 - 1) / (kB / 16), 1u << 20));
-    hipLaunchKernelGGL(block_copy_kernel, dim3(std::max(grid, 1u)), dim3(kB), 0, nullptr, ix.indptr, ix.indices,
-                       static_cast<const uint8_t*>(ix.values), ix.docs, ws.get<int64_t>(), D,
-                       blk_indices.get<uint4>(), blk_codes.get<uint2>(), row_ref.get<int64_t>());
+    hipLaunchKernelGGL(block_copy_kernel, dim3(std::max(grid, 1u)), dim3(kB), 0, nullptr, ix.indptr, ix.docs,
+                       dws.get<int64_t>(), doc_rows.get<uint4>(), ws.get<int64_t>(), D, blk_rows.get<uint4>(),
+                       row_ref.get<int64_t>());
     sync("block row copy");
     n_words = W;
     return true;

@@ -14,15 +14,40 @@
 //   sum_cluster       uint16[E]   -> local cluster id per entry (csc_cluster_)
 //   sum_value         uint8/float[E] (csc_value_)
 // Block-contiguous row copies (uint8 indexes, optional): for posting position p
-// (the order of `docs`), row_ref[p] = word_start << 16 | nnz of docs[p]; its row
-// is copied to 8-element words [word_start, word_start + ceil(nnz / 8)) of
-// blk_indices (8 uint16 per word) and blk_codes (8 bytes per word), zero padded,
-// so one cluster's rows are one contiguous, line-aligned stream.
+// (the order of `docs`), row_ref[p] packs the row's first word, its word count
+// and its nnz (blk_ref_* below); the row of docs[p] is re-encoded into 16-byte
+// words [first word, first word + word count) of blk_rows, so one cluster's
+// rows are one contiguous, line-aligned stream. A word is 8 slots of 16 bits
+// (slot s = bits 16s..16s+15), each slot (offset << 4) | code with a code of
+// 1..15; an unused slot is 0:
+//   * low-term word (slot 0's code nonzero): term = offset; terms < 4096;
+//   * based word (slot 0's code zero): slot 0 holds the base term (a multiple
+//     of 16), slots 1..7 hold term = base + offset, offsets <= 4095.
+// An element whose code exceeds 15 becomes several slots of one term whose
+// codes add up to it, and an element of code 0 none: the integer score
+// sum(code * query code) is unchanged. A row's low terms come first, 8 to a
+// word, then its other terms in row order, a new based word starting whenever
+// a term falls outside [base, base + 4095] or 7 slots are full. A row holds an
+// even number of words (a zero word completes an odd count), so every row
+// starts on a 32-byte pair and is read as whole pairs.
 #pragma once
 
 #include <cstdint>
 
+#include <hip/hip_runtime.h>
+
 namespace nsparse_amd {
+
+// row_ref fields: nnz (bits 0-14), word count (bits 15-29), first word (bits 30-63).
+constexpr int kBlkMaxField = 0x7FFF;
+__host__ __device__ inline int64_t blk_ref(int64_t first_word, int words, int nnz) {
+    return (first_word << 30) | (static_cast<int64_t>(words) << 15) | nnz;
+}
+__host__ __device__ inline int64_t blk_ref_first(int64_t r) { return r >> 30; }
+__host__ __device__ inline int blk_ref_words(int64_t r) { return static_cast<int>((r >> 15) & kBlkMaxField); }
+__host__ __device__ inline int blk_ref_nnz(int64_t r) { return static_cast<int>(r & kBlkMaxField); }
+// Low-term words hold terms below this; based-word offsets stay below it.
+constexpr int kBlkTermSpan = 4096;
 
 struct DevIndexView {
     int32_t n_docs = 0;
@@ -42,8 +67,7 @@ struct DevIndexView {
     const uint16_t* sum_cluster = nullptr;
     const void* sum_value = nullptr;
     const int64_t* row_ref = nullptr;
-    const uint16_t* blk_indices = nullptr;
-    const uint8_t* blk_codes = nullptr;
+    const uint4* blk_rows = nullptr;
     int64_t blk_words = 0;  // words in the block copies; word blk_words is zero padding
     // Per-list summary-term bitmaps (optional): bit t of list l's sum_nw words
     // is set when t is one of its summary terms; sum_rank[l * sum_nw + w] counts

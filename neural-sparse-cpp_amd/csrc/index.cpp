@@ -256,27 +256,25 @@ void SeismicIndex::build() {
 }
 
 // Block-contiguous row copies for the uint8 search (device_index.h). They cost
-// about 24 B per 8 non-zeros of every posting; NSPARSE_BLOCK_ROWS=0 turns them
+// about 2.4 B per non-zero of every posting on a 30522-term vocabulary with
+// rows sorted by term (16-byte words of 7-8 slots); NSPARSE_BLOCK_ROWS=0 turns them
 // off, and they are skipped when HBM cannot hold them. The search reads the
 // forward index directly when they are absent.
 void SeismicIndex::refresh_block_rows() {
     view_.row_ref = nullptr;
-    view_.blk_indices = nullptr;
-    view_.blk_codes = nullptr;
+    view_.blk_rows = nullptr;
     view_.blk_words = 0;
     const char* e = std::getenv("NSPARSE_BLOCK_ROWS");
     const bool on = sq_ && !(e != nullptr && std::string(e) == "0");
     const int64_t postings = static_cast<int64_t>(d_docs_.bytes() / 4);
     int64_t words = 0;
-    if (!on || !build_block_rows(view_, postings, d_row_ref_, d_blk_indices_, d_blk_codes_, words)) {
+    if (!on || !build_block_rows(view_, postings, d_row_ref_, d_blk_rows_, words)) {
         d_row_ref_.release();
-        d_blk_indices_.release();
-        d_blk_codes_.release();
+        d_blk_rows_.release();
         return;
     }
     view_.row_ref = d_row_ref_.get<int64_t>();
-    view_.blk_indices = d_blk_indices_.get<uint16_t>();
-    view_.blk_codes = d_blk_codes_.get<uint8_t>();
+    view_.blk_rows = d_blk_rows_.get<uint4>();
     view_.blk_words = words;
 }
 

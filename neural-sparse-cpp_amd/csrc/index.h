@@ -160,7 +160,7 @@ private:
     DeviceBuffer d_indptr_, d_indices_, d_values_;
     DeviceBuffer d_list_cluster_base_, d_cluster_doc_ptr_, d_docs_, d_list_term_base_, d_sum_term_ids_,
         d_sum_term_ptr_, d_sum_cluster_, d_sum_value_, d_list_entry_base_;
-    DeviceBuffer d_row_ref_, d_blk_indices_, d_blk_codes_;  // block-contiguous row copies (uint8)
+    DeviceBuffer d_row_ref_, d_blk_rows_;                  // block-contiguous row copies (uint8)
     DeviceBuffer d_sum_bits_, d_sum_rank_;                  // per-list summary-term bitmaps
     DeviceBuffer d_err_;
     DevIndexView view_{};

@@ -48,11 +48,11 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
                      int seed, DeviceLists& out);
 
 // Block-contiguous row copies of every posting (uint8 indexes) for the device
-// search: row_ref (D int64), blk_indices / blk_codes (8-element words). Returns
-// false, leaving the buffers empty, when HBM cannot hold them or a list's rows
-// exceed 32-bit element offsets.
-bool build_block_rows(const DevIndexView& ix, int64_t n_postings, DeviceBuffer& row_ref, DeviceBuffer& blk_indices,
-                      DeviceBuffer& blk_codes, int64_t& n_words);
+// search: row_ref (D int64) and blk_rows (16-byte words, the layout in
+// device_index.h). Returns false, leaving the buffers empty, when HBM cannot
+// hold them, a row exceeds the row_ref fields or a list's rows exceed 2^28 words.
+bool build_block_rows(const DevIndexView& ix, int64_t n_postings, DeviceBuffer& row_ref, DeviceBuffer& blk_rows,
+                      int64_t& n_words);
 
 // Per-list summary-term bitmaps and rank prefixes (device_index.h) of the
 // lists in `ix`. Returns false, leaving the buffers empty, when HBM cannot hold

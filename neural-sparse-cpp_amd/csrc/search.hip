@@ -58,13 +58,18 @@ __host__ __device__ constexpr int chunk_budget(bool dense, int /*block*/) {
 }
 constexpr int kSegMax = 64;   // clusters examined per chunk (one wave pass)
 constexpr int kTileFloats = 4096;
-// windows per wave per scoring step (loads in flight per lane): 5 measured
-// best at 8.8M docs (3: -4%, 4: -1.1%, 6: -0.1%); the kernel stays within 128
-// VGPRs, i.e. four waves per SIMD
+// windows per wave per scoring step (loads in flight per lane) for
+// forward-index rows: 5 measured best at 8.8M docs with 24-byte row words
+// (3: -4%, 4: -1.1%, 6: -0.1%); the kernel stays within 128 VGPRs, i.e. four
+// waves per SIMD
 template <bool DENSE>
 constexpr int win_per_step() {
     return 5;
 }
+// the same for block rows, whose lanes each hold a 32-byte pair of words: 4
+// measured best at 8.8M docs (29.04-29.10 ms per 10k-query launch; 3:
+// 29.45 ms, 5: 29.12-29.18 ms)
+constexpr int kBlkWinPerStep = 4;
 
 // A piece of one cluster's docs in a chunk. Cluster ids and positions fit 16
 // bits: the LDS plan holds 10 bytes per cluster, so a list has < 16384.
@@ -276,7 +281,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
         // block-contiguous row copies: word offsets are kept relative to the
         // list's first row
         const bool blk = SQ && ix.row_ref != nullptr;
-        const int64_t lwb = blk ? (ix.row_ref[dbase] >> 16) : 0;
+        const int64_t lwb = blk ? blk_ref_first(ix.row_ref[dbase]) : 0;
         for (int j = tid; j <= nc; j += kBlock) cstart[j] = static_cast<int32_t>(ix.cluster_doc_ptr[lb + j] - dbase);
         const int32_t tb = ix.list_term_base[term];
         const int32_t te = ix.list_term_base[term + 1];
@@ -525,8 +530,10 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                     const bool keep = sel == nullptr || ((sel[d[u] >> 5] >> (d[u] & 31)) & 1u);
                     int32_t rs, len;
                     if (blk) {
-                        rs = static_cast<int32_t>(((ref[u] >> 16) - lwb) << 3);
-                        len = static_cast<int32_t>(ref[u] & 0xFFFF);
+                        // rs in 8ths of words, as for forward-index rows; len
+                        // packs the word count over the nnz
+                        rs = static_cast<int32_t>((blk_ref_first(ref[u]) - lwb) << 3);
+                        len = (blk_ref_words(ref[u]) << 16) | blk_ref_nnz(ref[u]);
                     } else {
                         rs = ix.indptr[d[u]];
                         len = ix.indptr[d[u] + 1] - rs;
@@ -555,13 +562,14 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                     int nwj = 0;
                     if (j < total) {
                         const int len = C.len[j];
-                        if (len > 0) nwj = ((C.rs[j] + len + 7) >> 3) - (C.rs[j] >> 3);
+                        // block rows: 32-byte pairs of words
+                        if (len > 0) nwj = blk ? (len >> 17) : ((C.rs[j] + len + 7) >> 3) - (C.rs[j] >> 3);
                         C.isc[j] = 0;
                     }
                     const int incl = static_cast<int>(wave_incl_scan_dpp(static_cast<uint32_t>(nwj)));
                     if (j < total) {
                         C.wbase[j] = run + incl - nwj;
-                        if (blk) C.rs[j] = (C.rs[j] >> 3) - (run + incl - nwj);
+                        if (blk) C.rs[j] = (C.rs[j] >> 4) - (run + incl - nwj);
                     }
                     run += __builtin_amdgcn_readlane(incl, 63);
                 }
@@ -590,10 +598,13 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                 // over the window yields each doc run's sum: the last lane of a
                 // run adds S to its doc and subtracts S from the next run's doc
                 // (LDS integer atomics: exact in any order, modulo 2^32, and the
-                // true sums fit). Block-contiguous rows start on a word and are
-                // zero padded, so they need no masks; lanes without a word read
-                // the zero word past the end. Forward-index rows mask the bytes
-                // outside the row.
+                // true sums fit). For block-contiguous rows a "word" here is a
+                // 32-byte pair of 16-byte block words (rows are whole pairs), so
+                // the doc search, prefix sum and atomics of a window cover 128
+                // block words: with 8-slot block words the per-window work, not
+                // HBM, bounded the step. Block rows need no masks (unused slots
+                // are zero); lanes without a word read the zero pair past the
+                // end. Forward-index rows mask the bytes outside the row.
                 constexpr int kWinPerStep = win_per_step<DENSE>();
                 constexpr int kS1 = kBud / 64;
                 const int W = C.meta[kChunkWords];
@@ -601,24 +612,24 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                 const int32_t* wbase = C.wbase;
                 auto score_windows = [&](auto blk_tag) {
                     constexpr bool BLK = decltype(blk_tag)::value;
-                    const uint4* idx8 = BLK ? reinterpret_cast<const uint4*>(ix.blk_indices) + lwb
-                                            : reinterpret_cast<const uint4*>(ix.indices);
-                    const uint2* val8 = BLK ? reinterpret_cast<const uint2*>(ix.blk_codes) + lwb
-                                            : reinterpret_cast<const uint2*>(ix.values);
-                    const int64_t zero_word = BLK ? ix.blk_words - lwb : 0;
+                    // block rows: a lane scores one 32-byte pair of words
+                    constexpr int kWin = BLK ? kBlkWinPerStep : kWinPerStep;
+                    const uint4* idx8 = BLK ? ix.blk_rows + lwb : reinterpret_cast<const uint4*>(ix.indices);
+                    const uint2* val8 = reinterpret_cast<const uint2*>(ix.values);  // forward-index rows
+                    const int64_t zero_word = BLK ? (ix.blk_words - lwb) >> 1 : 0;
                     // every kS1-th word start, read once per chunk: the first
                     // ballot of each window's doc search needs no LDS read
                     const int wb1 = lane * kS1 < total ? wbase[lane * kS1] : INT32_MAX;
                     while (true) {
                         int xs = 0;
-                        if (lane == 0) xs = atomicAdd(&C.meta[kChunkNext], kWinPerStep);
+                        if (lane == 0) xs = atomicAdd(&C.meta[kChunkNext], kWin);
                         xs = __shfl(xs, 0, 64);
                         if (xs >= nwin) break;
-                        int jj[kWinPerStep], j1s[kWinPerStep], lo[kWinPerStep], hi[kWinPerStep];
-                        int sbs[kWinPerStep], bns[kWinPerStep];
-                        int64_t ww[kWinPerStep];
-                        uint4 ii[kWinPerStep];
-                        uint2 vv[kWinPerStep];
+                        int jj[kWin], j1s[kWin], lo[kWin], hi[kWin];
+                        int sbs[kWin], bns[kWin];
+                        int64_t ww[kWin];
+                        uint4 ii[kWin], ii2[kWin];
+                        uint2 vv[kWin];
                         // 1. each lane's doc and word address. The windows of a
                         // step are independent, so each phase issues its LDS
                         // reads for all of them before any result is used; a
@@ -626,14 +637,14 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                         // is masked out (no word, no sums).
                         // 1a. the starts after every kS1-th one <= the window's first word
 #pragma unroll
-                        for (int u = 0; u < kWinPerStep; ++u) {
+                        for (int u = 0; u < kWin; ++u) {
                             const int F = min(xs + u, nwin - 1) << 6;
                             sbs[u] = (__popcll(ballot(wb1 <= F)) - 1) * kS1;
                             bns[u] = wbase[sbs[u] + lane];  // sentinels cover sb + 63
                         }
                         // 1b. the window's first doc, then the doc boundaries inside it
 #pragma unroll
-                        for (int u = 0; u < kWinPerStep; ++u) {
+                        for (int u = 0; u < kWin; ++u) {
                             const bool valid = xs + u < nwin;  // wave-uniform
                             const int F = min(xs + u, nwin - 1) << 6, f = F + lane;
                             const int sb = sbs[u], bn0 = bns[u];
@@ -669,7 +680,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                         }
                         // 1c. row references of the lanes' docs
 #pragma unroll
-                        for (int u = 0; u < kWinPerStep; ++u) {
+                        for (int u = 0; u < kWin; ++u) {
                             const int F = min(xs + u, nwin - 1) << 6, f = F + lane;
                             const int j = jj[u];
                             ww[u] = zero_word;
@@ -691,33 +702,25 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                         }
                         // 2. all loads of the step back to back, without branches
 #pragma unroll
-                        for (int u = 0; u < kWinPerStep; ++u) {
-                            ii[u] = idx8[ww[u]];
-                            vv[u] = val8[ww[u]];
+                        for (int u = 0; u < kWin; ++u) {
+                            if constexpr (BLK) {
+                                ii[u] = idx8[2 * ww[u]];
+                                ii2[u] = idx8[2 * ww[u] + 1];
+                            } else {
+                                ii[u] = idx8[ww[u]];
+                                vv[u] = val8[ww[u]];
+                            }
                         }
                         // 3. scores (a masked window scores the zero word), then
                         // the windows' prefix sums, then their atomics: three
                         // passes, so the windows' LDS reads and DPP chains
                         // interleave
-                        uint32_t accs[kWinPerStep];
+                        uint32_t accs[kWin];
 #pragma unroll
-                        for (int u = 0; u < kWinPerStep; ++u) {
-                            uint64_t codes = (static_cast<uint64_t>(vv[u].y) << 32) | vv[u].x;
-                            if constexpr (!BLK) {
-                                // bytes outside the doc's [rs, re) -- and every byte of a
-                                // lane without a word -- are masked; their term ids are
-                                // still valid ids, so the lookups stay in bounds
-                                const bool has = jj[u] >= 0;
-                                const uint64_t mlo = has ? (~0ULL << (8 * lo[u])) : 0ULL;
-                                const uint64_t mhi = hi[u] < 8 ? ((1ULL << (8 * hi[u])) - 1) : ~0ULL;
-                                codes &= mlo & mhi;
-                            }
+                        for (int u = 0; u < kWin; ++u) {
                             const uint32_t tw[4] = {ii[u].x, ii[u].y, ii[u].z, ii[u].w};
                             uint32_t acc = 0;
-#pragma unroll
-                            for (int t = 0; t < 8; ++t) {
-                                const uint32_t term = (t & 1) ? (tw[t >> 1] >> 16) : (tw[t >> 1] & 0xFFFFu);
-                                const uint32_t c = static_cast<uint32_t>(codes >> (8 * t)) & 0xFFu;
+                            auto add = [&](uint32_t term, uint32_t c) {
                                 if constexpr (DENSE) {
                                     acc += c * static_cast<uint32_t>(qcode[term]);
                                 } else {
@@ -726,13 +729,45 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                                     const uint32_t qv = static_cast<uint32_t>(rval[r]);
                                     acc += pr ? c * qv : 0U;
                                 }
+                            };
+                            if constexpr (BLK) {
+                                if constexpr (DENSE) {
+                                    acc = blk_word_dot(ii2[u], qcode, blk_word_dot(ii[u], qcode, 0U));
+                                } else {
+                                    // 8 slots of (offset << 4 | code) under each word's base
+#pragma unroll
+                                    for (int h = 0; h < 2; ++h) {
+                                        const uint4 w = h == 0 ? ii[u] : ii2[u];
+                                        const uint32_t tv[4] = {w.x, w.y, w.z, w.w};
+                                        const uint32_t base = blk_word_base(tv[0]);
+#pragma unroll
+                                        for (int t = 0; t < 8; ++t) {
+                                            const uint32_t slot = (t & 1) ? (tv[t >> 1] >> 16) : (tv[t >> 1] & 0xFFFFU);
+                                            add((t == 0 ? 0U : base) + (slot >> 4), slot & 15U);
+                                        }
+                                    }
+                                }
+                            } else {
+                                // bytes outside the doc's [rs, re) -- and every byte of a
+                                // lane without a word -- are masked; their term ids are
+                                // still valid ids, so the lookups stay in bounds
+                                uint64_t codes = (static_cast<uint64_t>(vv[u].y) << 32) | vv[u].x;
+                                const bool has = jj[u] >= 0;
+                                const uint64_t mlo = has ? (~0ULL << (8 * lo[u])) : 0ULL;
+                                const uint64_t mhi = hi[u] < 8 ? ((1ULL << (8 * hi[u])) - 1) : ~0ULL;
+                                codes &= mlo & mhi;
+#pragma unroll
+                                for (int t = 0; t < 8; ++t) {
+                                    const uint32_t term = (t & 1) ? (tw[t >> 1] >> 16) : (tw[t >> 1] & 0xFFFFu);
+                                    add(term, static_cast<uint32_t>(codes >> (8 * t)) & 0xFFu);
+                                }
                             }
                             accs[u] = acc;
                         }
 #pragma unroll
-                        for (int u = 0; u < kWinPerStep; ++u) accs[u] = wave_incl_scan_dpp(accs[u]);
+                        for (int u = 0; u < kWin; ++u) accs[u] = wave_incl_scan_dpp(accs[u]);
 #pragma unroll
-                        for (int u = 0; u < kWinPerStep; ++u) {
+                        for (int u = 0; u < kWin; ++u) {
                             const uint32_t S = accs[u];
                             if (j1s[u] >= 0 && S != 0) {
                                 atomicAdd(&C.isc[jj[u]], S);
@@ -817,7 +852,8 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                     const int len = (j >= s.slot && j < send) ? C.len[j] : -1;
                     if (timing) {
                         stat_add(kStatDocsScored, __popcll(ballot(len >= 0)));
-                        const uint32_t incl = wave_incl_scan_dpp(static_cast<uint32_t>(max(len, 0)));
+                        const int nnz = len < 0 ? 0 : blk ? (len & 0xFFFF) : len;
+                        const uint32_t incl = wave_incl_scan_dpp(static_cast<uint32_t>(nnz));
                         stat_add(kStatNnzScored,
                                  static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63)));
                     }

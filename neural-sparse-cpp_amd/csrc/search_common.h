@@ -40,6 +40,31 @@ __device__ __forceinline__ uint32_t wave_incl_scan_dpp(uint32_t v) {
     return v;
 }
 
+typedef unsigned short ushort2_t __attribute__((ext_vector_type(2)));
+
+// Base term of a block-row word (the layout in device_index.h): 0 for a
+// low-term word (slot 0's code nonzero), else slot 0 itself.
+__device__ __forceinline__ uint32_t blk_word_base(uint32_t w0) { return (w0 & 15U) != 0 ? 0U : (w0 & 0xFFF0U); }
+
+// acc + the integer dot product of one block-row word with the query's
+// vocabulary-sized code table: all 8 lookups first, then one dot2 per slot
+// pair (each dword holds two slots, codes at bits 0-3 and 16-19). Slot 0 of a
+// based word has code 0, so its lookup (of term base / 16) adds nothing.
+__device__ __forceinline__ uint32_t blk_word_dot(const uint4& w, const uint8_t* qcode, uint32_t acc) {
+    const uint32_t tw[4] = {w.x, w.y, w.z, w.w};
+    const uint32_t base = blk_word_base(tw[0]);
+    ushort2_t qp[4];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        qp[h].x = qcode[(h == 0 ? 0U : base) + ((tw[h] >> 4) & 0xFFFU)];
+        qp[h].y = qcode[base + (tw[h] >> 20)];
+    }
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+        acc = __builtin_amdgcn_udot2(__builtin_bit_cast(ushort2_t, tw[h] & 0x000F000FU), qp[h], acc, false);
+    return acc;
+}
+
 __device__ __forceinline__ bool better(float s, int32_t i, float ts, int32_t ti) {
     return s > ts || (s == ts && i < ti);
 }

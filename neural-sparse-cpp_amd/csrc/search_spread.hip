@@ -233,19 +233,18 @@ __global__ __launch_bounds__(kLBlock) void spread_lists_kernel(SearchArgs a, Spr
     int32_t* wst = reinterpret_cast<int32_t*>(uni);                // kSubBatch + 1 word starts
     uint32_t* isc = reinterpret_cast<uint32_t*>(wst + kSubBatch + 1);  // kSubBatch sums
     uint64_t* gkey = reinterpret_cast<uint64_t*>(qs + P.s_key) + static_cast<int64_t>(ci) * P.mld;
-    const uint4* idx8 = reinterpret_cast<const uint4*>(ix.blk_indices);
-    const uint2* val8 = reinterpret_cast<const uint2*>(ix.blk_codes);
+    const uint4* rows = ix.blk_rows;
     for (int pa = p0; pa < p1; pa += kSubBatch) {
         const int nd = min(p1 - pa, kSubBatch);
         __syncthreads();  // the previous pass is done with wst / isc
         // the rows of consecutive postings are consecutive words
-        const int64_t w0 = ix.row_ref[dbase + pa] >> 16;
+        const int64_t w0 = blk_ref_first(ix.row_ref[dbase + pa]);
         for (int j = tid; j < nd; j += kLBlock) {
             const int64_t ref = ix.row_ref[dbase + pa + j];
-            const int ws = static_cast<int>((ref >> 16) - w0);
+            const int ws = static_cast<int>(blk_ref_first(ref) - w0);
             wst[j] = ws;
             isc[j] = 0;
-            if (j == nd - 1) wst[nd] = ws + static_cast<int>(((ref & 0xFFFF) + 7) >> 3);
+            if (j == nd - 1) wst[nd] = ws + blk_ref_words(ref);
         }
         __syncthreads();
         // Windows of 64 words, one per lane; a DPP prefix sum S over the window
@@ -266,23 +265,12 @@ __global__ __launch_bounds__(kLBlock) void spread_lists_kernel(SearchArgs a, Spr
                 wv[u] = has ? w0 + f : ix.blk_words;
             }
             uint4 ii[kWinPerStep];
-            uint2 vv[kWinPerStep];
 #pragma unroll
-            for (int u = 0; u < kWinPerStep; ++u) {
-                ii[u] = idx8[wv[u]];
-                vv[u] = val8[wv[u]];
-            }
+            for (int u = 0; u < kWinPerStep; ++u) ii[u] = rows[wv[u]];
             uint32_t sums[kWinPerStep];
 #pragma unroll
             for (int u = 0; u < kWinPerStep; ++u) {
-                const uint64_t codes = (static_cast<uint64_t>(vv[u].y) << 32) | vv[u].x;
-                const uint32_t tw[4] = {ii[u].x, ii[u].y, ii[u].z, ii[u].w};
-                uint32_t s = 0;
-#pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    const uint32_t tm = (t & 1) ? (tw[t >> 1] >> 16) : (tw[t >> 1] & 0xFFFFu);
-                    s += (static_cast<uint32_t>(codes >> (8 * t)) & 0xFFu) * static_cast<uint32_t>(qcode[tm]);
-                }
+                const uint32_t s = blk_word_dot(ii[u], qcode, 0U);
                 sums[u] = s;
             }
 #pragma unroll

@@ -299,6 +299,45 @@ def test_sq_search_with_and_without_block_rows(nsparse, data, monkeypatch):
             assert_same(got, want)
 
 
+@pytest.mark.parametrize("dim", [3000, 30522, 65535])
+def test_block_row_encoding_edge_cases(nsparse, dim):
+    """Block rows re-encode each uint8 row into 16-byte words of 4-bit codes
+    (low-term words, and based words of offsets < 4096): codes above 15 are
+    split over several slots of one term, code-0 elements dropped, and a
+    word starts over whenever a term leaves its base's span. Rows here are
+    unsorted, codes cover 0..255 (some rows all zero), terms spread over the
+    whole vocabulary and long rows reach 1500 elements. The batched search
+    and the single-query (spread) search on block rows must equal the
+    oracle, as must the forward-index search."""
+    rng = np.random.default_rng(dim)
+    n = 3000
+    lens = rng.integers(1, 400, n)
+    lens[::97] = 1500
+    ip = np.zeros(n + 1, np.int32)
+    ip[1:] = np.cumsum(lens)
+    ix = np.concatenate([rng.choice(dim, size=m, replace=False) for m in lens]).astype(np.uint16)
+    v = rng.uniform(0.0, 10.0, ip[-1]).astype(np.float32)
+    v[ip[5]:ip[6]] = 0.0
+    v[rng.random(ip[-1]) < 0.05] = 0.0
+    qn = rng.integers(5, 120, 40)
+    qip = np.zeros(41, np.int32)
+    qip[1:] = np.cumsum(qn)
+    qix = np.concatenate([rng.choice(dim, size=m, replace=False) for m in qn]).astype(np.uint16)
+    qv = rng.uniform(0.01, 3.0, qip[-1]).astype(np.float32)
+    o = O.OracleIndex(dim, 1, 300, 30, 0.4, 0, 0.0, 10.0, tie_mode=1)
+    o.add(ip, ix, v)
+    o.build()
+    blob = o.serialize()
+    p = nsparse.SeismicSearchParameters(10, 0.8)
+    want = o.search(qip, qix, qv, 10, cut=10, hf=0.8)
+    dev = nsparse.read_index_bytes(blob)
+    assert_same(dev.search(40, qip, qix, qv, 10, p), want)
+    for i in range(0, 40, 7):
+        one = np.array([0, qip[i + 1] - qip[i]], np.int32)
+        got = dev.search(1, one, qix[qip[i]:qip[i + 1]], qv[qip[i]:qip[i + 1]], 10, p)
+        assert_same(got, (want[0][i:i + 1], want[1][i:i + 1]))
+
+
 def test_waits_between_waves_end_with_a_one_poll_limit(nsparse, data, sq_pair, monkeypatch):
     """The waves of a search workgroup wait for each other on LDS flags, each
     wait bounded. With the bound forced to one poll the launch still ends:
