@@ -467,6 +467,311 @@ __global__ __launch_bounds__(kB) void assign_kernel(
     if (lane == 0 && n_pairs != 0) atomicAdd(pairs, static_cast<unsigned long long>(n_pairs));
 }
 
+// ---- uint8 assignment with a dense part on the matrix cores
+//
+// map_docs_to_clusters_typed (cluster/kmeans_utils.cpp:107-158) for uint8
+// codes. A list's (doc term, centroid posting) pairs concentrate on the few
+// terms that most of its centroids share: at 8.8M synthetic docs the 256 terms
+// with the most centroid postings carry about 96% of a list's pairs. For those
+// "hot" terms sims[d][c] = sum_t doc[d][t] * cent[c][t] is a dense product of a
+// (docs x hot terms) tile and a (hot terms x centroids) matrix, computed with
+// v_mfma_f32_16x16x32_bf16: codes (< 256) are exact in bfloat16, every product
+// (< 2^16) and every partial sum over at most 256 terms (< 2^24) is an exact
+// float, so the dense part is the exact integer in any order. The remaining
+// "cold" pairs are added with the sparse pair loop of assign_kernel. The sum of
+// both is the reference's integer similarity; the argmax keeps the strict >
+// (lowest cluster on ties, cluster 0 when all are zero).
+
+constexpr int kHot = 256;        // hot term slots (the dense product's K)
+constexpr int kTileDocs = 16;    // docs per dense tile (the MFMA's M)
+constexpr int kHotHist = 1024;   // centroid-posting count histogram bins (counts clamp)
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+// Bytes of dynamic LDS the hot assignment needs for lists of up to mc clusters.
+size_t assign_hot_lds(int mc) {
+    int mc2 = 1;
+    while (mc2 < mc) mc2 <<= 1;
+    const size_t cent = (static_cast<size_t>(mc2) * 4 + 15) & ~size_t{15};
+    const size_t tile = static_cast<size_t>(kTileDocs) * kHot * 2;
+    const size_t sims = std::max(static_cast<size_t>(kTileDocs) * mc * 4, static_cast<size_t>(kHotHist) * 4);
+    return cent + tile + ((sims + 15) & ~size_t{15}) + 64;
+}
+
+// One workgroup per list of [l_begin, l_end) (grid-stride). `tmap` is this
+// workgroup's term -> CSC range table (as in assign_kernel); during a list a
+// hot term's entry holds (-2 - slot, its centroid postings). `bmat` is this
+// workgroup's (centroids x kHot) bfloat16 matrix of the hot terms' codes, rows
+// padded to a multiple of 16. Lists must have at most mc clusters, and the
+// dynamic LDS assign_hot_lds(mc) bytes.
+__global__ __launch_bounds__(kB) void assign_hot_kernel(
+    const int32_t* __restrict__ indptr, const uint16_t* __restrict__ indices, const uint8_t* __restrict__ values,
+    const int64_t* __restrict__ kept_base, const int32_t* __restrict__ kept, const int32_t* __restrict__ lcb,
+    const int32_t* __restrict__ centroid_doc, const int64_t* __restrict__ cent_off, int64_t e_base,
+    const uint32_t* __restrict__ ckey, const uint16_t* __restrict__ cc, const uint8_t* __restrict__ cw,
+    int2* __restrict__ tmap_all, int dim, __bf16* __restrict__ bmat_all, int32_t* __restrict__ assign, int l_begin,
+    int l_end, int mc, unsigned long long* __restrict__ pairs) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    int mc2 = 1;
+    while (mc2 < mc) mc2 <<= 1;
+    const int mcp = (mc + 15) & ~15;
+    int32_t* cent = reinterpret_cast<int32_t*>(smem);  // sorted centroid ids [mc2]
+    __bf16* atile = reinterpret_cast<__bf16*>(smem + ((mc2 * 4 + 15) & ~15));  // [kTileDocs][kHot]
+    int32_t* sims = reinterpret_cast<int32_t*>(reinterpret_cast<unsigned char*>(atile) + kTileDocs * kHot * 2);
+    int32_t* hist = sims;  // [kHotHist], list set-up only
+    const int sims_words = kTileDocs * mc > kHotHist ? kTileDocs * mc : kHotHist;
+    int32_t* misc = sims + ((sims_words + 3) & ~3);  // [0] hot slots used, [1] threshold
+    int2* tmap = tmap_all + static_cast<size_t>(blockIdx.x) * dim;
+    __bf16* bmat = bmat_all + static_cast<size_t>(blockIdx.x) * mcp * kHot;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint64_t n_pairs = 0;  // cold pairs (wave-uniform count)
+    uint64_t hot_pairs = 0;  // this lane's hot pairs
+    for (int l = l_begin + blockIdx.x; l < l_end; l += gridDim.x) {
+        const int64_t kb = kept_base[l], n = kept_base[l + 1] - kb;
+        const int c0 = lcb[l], nc = lcb[l + 1] - c0;
+        if (n == 0 || nc == 0) continue;
+        if (n == nc) {  // every kept doc is a centroid: nothing to assign
+            for (int64_t j = tid; j < n; j += kB) assign[kb + j] = -1;
+            continue;
+        }
+        const int64_t e0 = cent_off[c0] - e_base, e1 = cent_off[c0 + nc] - e_base;
+        for (int i = tid; i < kHotHist; i += kB) hist[i] = 0;
+        if (tid == 0) misc[0] = 0;
+        for (int64_t e = e0 + tid; e < e1; e += kB) {
+            const uint32_t k = ckey[e];
+            const int t = static_cast<int>(k & 0xFFFFu);
+            if (e == e0 || ckey[e - 1] != k) tmap[t].x = static_cast<int>(e);
+            if (e == e1 - 1 || ckey[e + 1] != k) tmap[t].y = static_cast<int>(e + 1);
+        }
+        __syncthreads();
+        // histogram of the terms' centroid-posting counts
+        for (int64_t e = e0 + tid; e < e1; e += kB) {
+            const uint32_t k = ckey[e];
+            if (e == e0 || ckey[e - 1] != k) {
+                const int2 col = tmap[k & 0xFFFFu];
+                atomicAdd(&hist[min(col.y - col.x, kHotHist - 1)], 1);
+            }
+        }
+        __syncthreads();
+        // threshold: the smallest count >= 2 whose terms (those with at least
+        // that many postings) fit the kHot slots; lane i holds bins 16i..16i+15
+        if (wave == 0) {
+            int bins[16], s = 0;
+#pragma unroll
+            for (int b = 0; b < 16; ++b) {
+                bins[b] = hist[lane * 16 + b];
+                s += bins[b];
+            }
+            // suffix sums over the lanes: a prefix over the reversed order
+            const int rev = __shfl(s, 63 - lane, 64);
+            const int rincl = static_cast<int>(wave_incl_scan_dpp(static_cast<uint32_t>(rev)));
+            const int suf = __shfl(rincl, 63 - lane, 64);  // terms in bins >= 16 * lane
+            const int above = lane == 63 ? 0 : __shfl(suf, lane + 1, 64);  // terms in bins >= 16 (lane + 1)
+            // each lane's smallest bin b with S(b) <= kHot, if any of its bins fits
+            int acc = above, tl = 16 * (lane + 1);
+            for (int b = 15; b >= 0; --b) {
+                if (acc + bins[b] > kHot) break;
+                acc += bins[b];
+                tl = 16 * lane + b;
+            }
+            // the lowest lane whose bins above its own fit holds the threshold
+            const uint64_t ok = __ballot(above <= kHot);
+            int theta = __shfl(tl, __ffsll(static_cast<unsigned long long>(ok)) - 1, 64);  // lane 63 always fits
+            if (theta >= kHotHist) theta = INT32_MAX;  // even the top bin's terms overflow: no hot terms
+            if (lane == 0) misc[1] = theta < 2 ? 2 : theta;
+        }
+        __syncthreads();
+        const int theta = misc[1];
+        // hot terms: a slot each, marked in tmap as (-2 - slot, postings)
+        for (int64_t e = e0 + tid; e < e1; e += kB) {
+            const uint32_t k = ckey[e];
+            if (e == e0 || ckey[e - 1] != k) {
+                const int t = static_cast<int>(k & 0xFFFFu);
+                const int2 col = tmap[t];
+                const int cnt = col.y - col.x;
+                if (cnt >= theta) {
+                    const int slot = atomicAdd(&misc[0], 1);  // < kHot by the threshold's choice
+                    if (slot < kHot) tmap[t] = make_int2(-2 - slot, cnt);
+                }
+            }
+        }
+        // zero the rows of bmat this list uses (16-byte stores), and sort the
+        // centroid ids ascending (bitonic, padded to a power of two)
+        const int ncp = (nc + 15) & ~15;
+        {
+            uint4* bz = reinterpret_cast<uint4*>(bmat);
+            const int nz = ncp * kHot / 8;
+            for (int i = tid; i < nz; i += kB) bz[i] = make_uint4(0, 0, 0, 0);
+        }
+        int n2 = 1;
+        while (n2 < nc) n2 <<= 1;
+        for (int i = tid; i < n2; i += kB) cent[i] = i < nc ? centroid_doc[c0 + i] : INT32_MAX;
+        __syncthreads();
+        for (int size = 2; size <= n2; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int t = tid; t < n2 / 2; t += kB) {
+                    const int i = 2 * t - (t & (stride - 1)), j = i + stride;
+                    const int32_t a = cent[i], b = cent[j];
+                    if ((a > b) == ((i & size) == 0)) {
+                        cent[i] = b;
+                        cent[j] = a;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // the hot terms' centroid codes into bmat[centroid][slot]
+        for (int64_t e = e0 + tid; e < e1; e += kB) {
+            const int2 col = tmap[ckey[e] & 0xFFFFu];
+            if (col.x <= -2) bmat[static_cast<size_t>(cc[e]) * kHot + (-2 - col.x)] = static_cast<__bf16>(cw[e]);
+        }
+        __syncthreads();
+        const int nhot = misc[0] < kHot ? misc[0] : kHot;
+        const int ksteps = (nhot + 31) >> 5;  // 32 slots per MFMA
+        for (int64_t j0 = 0; j0 < n; j0 += kTileDocs) {
+            const int nt = n - j0 < kTileDocs ? static_cast<int>(n - j0) : kTileDocs;
+            {
+                uint4* az = reinterpret_cast<uint4*>(atile);
+                for (int i = tid; i < kTileDocs * kHot / 8; i += kB) az[i] = make_uint4(0, 0, 0, 0);
+                for (int i = tid; i < kTileDocs * nc; i += kB) sims[i] = 0;
+            }
+            __syncthreads();
+            // each wave: its docs' hot codes into the tile, cold pairs into sims
+            for (int r = wave; r < nt; r += kB / 64) {
+                const int32_t d = kept[kb + j0 + r];
+                int lo = 0, hi = nc;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (cent[mid] < d)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                if (lo < nc && cent[lo] == d) continue;  // a centroid
+                int32_t* srow = sims + r * nc;
+                __bf16* arow = atile + r * kHot;
+                const int32_t r0 = indptr[d], r1 = indptr[d + 1];
+                // the cold pairs are numbered by a prefix over the terms' posting
+                // counts and dealt to the lanes, kW windows of 64 with their loads
+                // in flight together (as in assign_kernel)
+                constexpr int kW = 16;
+                for (int b = r0; b < r1; b += 64) {
+                    const int p = b + lane;
+                    int cs = 0, cnt = 0, dv = 0;
+                    if (p < r1) {
+                        const int2 col = tmap[indices[p]];
+                        dv = values[p];
+                        if (col.x <= -2) {
+                            arow[-2 - col.x] = static_cast<__bf16>(dv);
+                            hot_pairs += static_cast<uint64_t>(col.y);
+                        } else if (col.x >= 0) {
+                            cs = col.x;
+                            cnt = col.y - col.x;
+                        }
+                    }
+                    const int incl = static_cast<int>(wave_incl_scan_dpp(static_cast<uint32_t>(cnt)));
+                    const int excl = incl - cnt;
+                    const int total = __builtin_amdgcn_readlane(incl, 63);
+                    n_pairs += static_cast<uint64_t>(total);
+                    for (int i0 = 0; i0 < total; i0 += 64 * kW) {
+                        int ev[kW], wv[kW];
+                        bool ok[kW];
+#pragma unroll
+                        for (int u = 0; u < kW; ++u) {
+                            const int w0 = i0 + 64 * u, i = w0 + lane;
+                            ok[u] = i < total;
+                            ev[u] = 0;
+                            wv[u] = 0;
+                            if (w0 >= total) continue;  // wave-uniform
+                            int owner = __popcll(__ballot(excl <= w0)) - 1;
+                            const uint64_t bm = __ballot(excl > w0 && excl <= w0 + 63 && cnt > 0);
+                            for (uint64_t m = bm; m != 0; m &= m - 1) {
+                                const int o = __ffsll(static_cast<unsigned long long>(m)) - 1;
+                                if (i >= __builtin_amdgcn_readlane(excl, o)) owner = o;
+                            }
+                            const int ocs = __shfl(cs, owner, 64), oex = __shfl(excl, owner, 64);
+                            wv[u] = __shfl(dv, owner, 64);
+                            ev[u] = ocs + (i - oex);
+                        }
+                        uint32_t cl[kW], cwv[kW];
+#pragma unroll
+                        for (int u = 0; u < kW; ++u) {
+                            cl[u] = ok[u] ? cc[ev[u]] : 0u;
+                            cwv[u] = ok[u] ? static_cast<uint32_t>(cw[ev[u]]) : 0u;
+                        }
+#pragma unroll
+                        for (int u = 0; u < kW; ++u)
+                            if (ok[u]) atomicAdd(&srow[cl[u]], wv[u] * static_cast<int32_t>(cwv[u]));
+                    }
+                }
+            }
+            __syncthreads();
+            // dense part: 16 docs x 16 centroids per MFMA tile, K = 32 hot slots
+            // per step; lane l holds A[l % 16][8 (l / 16) ..], B[8 (l / 16) ..][l % 16]
+            // and D[4 (l / 16) + v][l % 16]
+            if (ksteps > 0) {
+                const int arow = lane & 15, kq = 8 * (lane >> 4);
+                for (int ct = wave; ct * 16 < nc; ct += kB / 64) {
+                    const int c = ct * 16 + (lane & 15);
+                    const __bf16* brow = bmat + static_cast<size_t>(c) * kHot + kq;
+                    f32x4_t acc = {0.0F, 0.0F, 0.0F, 0.0F};
+                    for (int ks = 0; ks < ksteps; ++ks) {
+                        const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(atile + arow * kHot + ks * 32 + kq);
+                        const bf16x8_t bv = *reinterpret_cast<const bf16x8_t*>(brow + ks * 32);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc, 0, 0, 0);
+                    }
+                    if (c < nc) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int r = 4 * (lane >> 4) + v;
+                            if (r < nt) sims[r * nc + c] += static_cast<int32_t>(acc[v]);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            // argmax per doc (strict >: the lowest cluster wins ties)
+            for (int r = wave; r < nt; r += kB / 64) {
+                const int32_t d = kept[kb + j0 + r];
+                int lo = 0, hi = nc;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (cent[mid] < d)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                if (lo < nc && cent[lo] == d) {
+                    if (lane == 0) assign[kb + j0 + r] = -1;
+                    continue;
+                }
+                const int32_t* srow = sims + r * nc;
+                int32_t best = srow[0];
+                int bi = 0;
+                for (int c = lane; c < nc; c += 64) argmax_merge(best, bi, srow[c], c);
+                for (int off = 32; off > 0; off >>= 1) {
+                    const int32_t ov = __shfl_xor(best, off, 64);
+                    const int oi = __shfl_xor(bi, off, 64);
+                    argmax_merge(best, bi, ov, oi);
+                }
+                if (lane == 0) assign[kb + j0 + r] = bi;
+            }
+            __syncthreads();
+        }
+        for (int64_t e = e0 + tid; e < e1; e += kB) tmap[ckey[e] & 0xFFFFu] = make_int2(-1, -1);
+        __syncthreads();
+    }
+    // hot pairs: a wave sum of the lanes' counts
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t lo = __shfl_xor(static_cast<uint32_t>(hot_pairs), off, 64);
+        const uint32_t hi = __shfl_xor(static_cast<uint32_t>(hot_pairs >> 32), off, 64);
+        hot_pairs += (static_cast<uint64_t>(hi) << 32) | lo;
+    }
+    const uint64_t all = n_pairs + hot_pairs;
+    if (lane == 0 && all != 0) atomicAdd(pairs, static_cast<unsigned long long>(all));
+}
+
 // Non-zeros of the kept postings' rows (for the build's byte count).
 __global__ void kept_row_nnz_kernel(const int32_t* __restrict__ indptr, const int32_t* __restrict__ kept, size_t n,
                                     unsigned long long* __restrict__ out) {
@@ -1599,6 +1904,13 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(assign_lds)),
                   "hipFuncSetAttribute");
     }
+    // uint8 lists take the hybrid assignment (dense hot terms on the matrix
+    // cores) when its LDS plan keeps three workgroups per CU
+    const bool hot = sq && assign_hot_lds(mc) <= 64 * 1024;
+    const int hot_grid = std::min(L, 1024);
+    const size_t mcp = static_cast<size_t>((mc + 15) & ~15);
+    DeviceBuffer bmat;
+    if (hot) bmat.allocate(static_cast<size_t>(hot_grid) * mcp * kHot * 2);
     for (size_t bi = 0; bi + 1 < cbatch.size(); ++bi) {
         const int l0 = cbatch[bi], l1 = cbatch[bi + 1];
         const size_t g0 = static_cast<size_t>(lcb[l0]), ng = static_cast<size_t>(lcb[l1]) - g0;
@@ -1628,7 +1940,14 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
                                    static_cast<size_t>(n), cc.get<uint16_t>(), cw.get<float>());
         }
         const int grid = std::min(l1 - l0, assign_grid);
-        if (sq)
+        if (hot)
+            hipLaunchKernelGGL(assign_hot_kernel, dim3(std::min(l1 - l0, hot_grid)), dim3(kB), assign_hot_lds(mc),
+                               nullptr, fwd.indptr, fwd.indices, static_cast<const uint8_t*>(fwd.values),
+                               d_kept_base.get<int64_t>(), kept.get<int32_t>(), d_lcb.get<int32_t>(),
+                               centroid_doc.get<int32_t>(), cent_off.get<int64_t>(), eb, ckey.get<uint32_t>(),
+                               cc.get<uint16_t>(), cw.get<uint8_t>(), tmap.get<int2>(), L, bmat.get<__bf16>(),
+                               assign.get<int32_t>(), l0, l1, mc, bstat.get<unsigned long long>());
+        else if (sq)
             hipLaunchKernelGGL((assign_kernel<uint8_t, int32_t>), dim3(grid), dim3(kB), assign_lds, nullptr,
                                fwd.indptr, fwd.indices, static_cast<const uint8_t*>(fwd.values),
                                d_kept_base.get<int64_t>(), kept.get<int32_t>(), d_lcb.get<int32_t>(),
@@ -1645,6 +1964,7 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
         sync("assignment");
     }
     tmap.release();
+    bmat.release();
     ckey_in.release();
     cval_in.release();
     ent_c.release();
