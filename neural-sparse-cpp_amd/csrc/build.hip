@@ -711,15 +711,20 @@ __global__ __launch_bounds__(kB) void assign_hot_kernel(
             // per step; lane l holds A[l % 16][8 (l / 16) ..], B[8 (l / 16) ..][l % 16]
             // and D[4 (l / 16) + v][l % 16]
             if (ksteps > 0) {
+                // all K steps' B fragments are loaded before the first MFMA (slots
+                // past the hot ones are zero in both operands)
                 const int arow = lane & 15, kq = 8 * (lane >> 4);
                 for (int ct = wave; ct * 16 < nc; ct += kB / 64) {
                     const int c = ct * 16 + (lane & 15);
                     const __bf16* brow = bmat + static_cast<size_t>(c) * kHot + kq;
+                    bf16x8_t bv[kHot / 32];
+#pragma unroll
+                    for (int ks = 0; ks < kHot / 32; ++ks) bv[ks] = *reinterpret_cast<const bf16x8_t*>(brow + ks * 32);
                     f32x4_t acc = {0.0F, 0.0F, 0.0F, 0.0F};
-                    for (int ks = 0; ks < ksteps; ++ks) {
+#pragma unroll
+                    for (int ks = 0; ks < kHot / 32; ++ks) {
                         const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(atile + arow * kHot + ks * 32 + kq);
-                        const bf16x8_t bv = *reinterpret_cast<const bf16x8_t*>(brow + ks * 32);
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv[ks], acc, 0, 0, 0);
                     }
                     if (c < nc) {
 #pragma unroll
