@@ -53,6 +53,12 @@ constexpr int kWaves = kBlock / 64;
 // 1.17 ms with 1024, 1.14 ms with 768 or 512; 384 was 2% slower there and
 // 1.1% slower for the batched 512-lane launch. A multiple of 64 up to 4096
 // (the first-doc search reads every budget/64-th word start with one lane each).
+// The first-list sort's range lists (introsort_desc_block) in the union
+// scratch, after KeyItem[mc] and two uint16[mc] position lists: two levels of
+// at most 2 * mc / 17 + 1 ranges each.
+__host__ __device__ constexpr int sort_range_cap(int mc) { return 2 * mc / 17 + 2; }
+__host__ __device__ constexpr int sort_rng_offset(int mc) { return (mc * 12 + 15) & ~15; }
+
 __host__ __device__ constexpr int chunk_budget(bool dense, int /*block*/) {
     return dense ? 512 : 256;
 }
@@ -130,6 +136,7 @@ enum Misc {
     kMiscBreakPending,
     kMiscStatFound,  // counters of the current list (stats runs only)
     kMiscStatCsc,
+    kMiscSortCnt,    // two ints: the first-list sort's range counts
     kMiscCount = 16
 };
 
@@ -379,9 +386,11 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
             for (int c = tid; c < nc; c += kBlock) A[c] = KeyItem{score[c], static_cast<uint32_t>(c)};
             __syncthreads();
             uint32_t* leaf_bits = reinterpret_cast<uint32_t*>(flist);  // free until the next list
-            if (wave == 0)
-                introsort_desc_wave(A, nc, reinterpret_cast<uint16_t*>(A + P.mc),
-                                    reinterpret_cast<uint16_t*>(A + P.mc) + P.mc, leaf_bits, lane);
+            // every wave takes ranges of each recursion level (one wave alone
+            // took 84 us per query at 8.8M docs while the others waited)
+            uint16_t* Lpos = reinterpret_cast<uint16_t*>(A + P.mc);
+            introsort_desc_block<BLOCK>(A, nc, Lpos, Lpos + P.mc, leaf_bits, reinterpret_cast<int4*>(uni + sort_rng_offset(P.mc)),
+                                        sort_range_cap(P.mc), &misc[kMiscSortCnt]);
             __syncthreads();
             introsort_final_pass(A, nc, leaf_bits, order, tid, kBlock);
         } else {
@@ -1060,7 +1069,8 @@ LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq, bool dense, int 
     P.kmax = kmax;
     P.budget = chunk_budget(dense, block);
     const int staging = 2 * (5 * P.budget + 128) * 4;  // two chunk buffers
-    const int sortbuf = P.mc * 12;  // KeyItem[mc] + two uint16[mc] position lists
+    // KeyItem[mc] + two uint16[mc] position lists + two levels of sort ranges
+    const int sortbuf = sort_rng_offset(P.mc) + 2 * sort_range_cap(P.mc) * 16;
     const int tile = sq ? 0 : kTileFloats * 4;
     P.uni_bytes = max(max(staging, sortbuf), max(tile, kQMax * 8));
     int o = 0;
