@@ -426,12 +426,15 @@ def main():
     ap.add_argument("--cut", type=int, default=10)
     ap.add_argument("--hf", type=float, default=0.8)
     ap.add_argument("--plain-queries", type=int, default=300)
-    ap.add_argument("--recall-queries", type=int, default=200)
+    ap.add_argument("--recall-queries", type=int, default=1000,
+                    help="queries whose recall@10 is measured against exact FP32 brute force")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--sweep", default="", help="comma-separated cut:heap_factor pairs to time and score")
-    ap.add_argument("--serve-rates", default="1000,10000,50000", help="serving loop: offered QPS levels")
+    ap.add_argument("--serve-rates", default="1000,10000,50000,200000",
+                    help="serving loop: offered QPS levels (the workload's 1k, 10k and 50k/s, and a load near the "
+                         "loop's capacity)")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--serve-batch", type=int, default=1024)
     ap.add_argument("--serve-streams", type=int, default=4)
