@@ -478,34 +478,66 @@ __global__ __launch_bounds__(kB) void assign_kernel(
 // v_mfma_f32_16x16x32_bf16: codes (< 256) are exact in bfloat16, every product
 // (< 2^16) and every partial sum over at most 256 terms (< 2^24) is an exact
 // float, so the dense part is the exact integer in any order. The remaining
-// "cold" pairs are added with the sparse pair loop of assign_kernel. The sum of
-// both is the reference's integer similarity; the argmax keeps the strict >
-// (lowest cluster on ties, cluster 0 when all are zero).
+// "cold" pairs are added with integer LDS atomics. The sum of both is the
+// reference's integer similarity; the argmax keeps the strict > (lowest
+// cluster on ties, cluster 0 when all are zero).
 
 constexpr int kHot = 256;        // hot term slots (the dense product's K)
 constexpr int kTileDocs = 16;    // docs per dense tile (the MFMA's M)
 constexpr int kHotHist = 1024;   // centroid-posting count histogram bins (counts clamp)
+constexpr int kHB = 512;         // hot assignment workgroup
+constexpr int kColdCap = 2048;   // cold (doc term) entries of a tile staged at once
+constexpr int kHotMaxClusters = 831;  // largest list the LDS plan takes
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
-// Bytes of dynamic LDS the hot assignment needs for lists of up to mc clusters.
-size_t assign_hot_lds(int mc) {
+// Dynamic LDS of the hot assignment for lists of up to mc clusters: sorted
+// centroid ids, the hot tile, the tile's sims (or the count histogram), the
+// cold list (term range start, prefix of posting counts, row | code << 8) and
+// per-tile doc tables.
+struct HotLds {
+    int o_cent, o_tile, o_sims, o_ccs, o_cpre, o_crd, o_tdoc, o_tstart, o_tpre, o_misc, total;
+};
+__host__ __device__ inline HotLds hot_lds(int mc) {
     int mc2 = 1;
     while (mc2 < mc) mc2 <<= 1;
-    const size_t cent = (static_cast<size_t>(mc2) * 4 + 15) & ~size_t{15};
-    const size_t tile = static_cast<size_t>(kTileDocs) * kHot * 2;
-    const size_t sims = std::max(static_cast<size_t>(kTileDocs) * mc * 4, static_cast<size_t>(kHotHist) * 4);
-    return cent + tile + ((sims + 15) & ~size_t{15}) + 64;
+    HotLds L{};
+    int o = 0;
+    auto take = [&](int bytes) {
+        const int at = o;
+        o += (bytes + 15) & ~15;
+        return at;
+    };
+    L.o_cent = take(mc2 * 4);
+    L.o_tile = take(kTileDocs * kHot * 2);
+    L.o_sims = take((kTileDocs * mc > kHotHist ? kTileDocs * mc : kHotHist) * 4);
+    L.o_ccs = take(kColdCap * 4);
+    L.o_cpre = take((kColdCap + 1) * 4);
+    L.o_crd = take(kColdCap * 4);
+    L.o_tdoc = take(kTileDocs * 4);
+    L.o_tstart = take(kTileDocs * 4);
+    L.o_tpre = take((kTileDocs + 1) * 4);
+    L.o_misc = take(16 * 4);
+    L.total = o;
+    return L;
 }
 
-// One workgroup per list of [l_begin, l_end) (grid-stride). `tmap` is this
-// workgroup's term -> CSC range table (as in assign_kernel); during a list a
-// hot term's entry holds (-2 - slot, its centroid postings). `bmat` is this
-// workgroup's (centroids x kHot) bfloat16 matrix of the hot terms' codes, rows
-// padded to a multiple of 16. Lists must have at most mc clusters, and the
-// dynamic LDS assign_hot_lds(mc) bytes.
-__global__ __launch_bounds__(kB) void assign_hot_kernel(
+// One 512-lane workgroup per list of [l_begin, l_end) (grid-stride). `tmap`
+// is this workgroup's term -> CSC range table (as in assign_kernel); during a
+// list a hot term's entry holds (-2 - slot, its centroid postings). `bmat` is
+// this workgroup's (centroids x kHot) bfloat16 matrix of the hot terms' codes,
+// rows padded to a multiple of 16. Lists have at most mc <= kHotMaxClusters
+// clusters; dynamic LDS hot_lds(mc).total bytes.
+//
+// Per tile of 16 docs: (1) every lane takes (doc, term) entries of the tile's
+// rows, four at a time with their loads in flight; a hot term's code goes to
+// the tile, a cold term with centroid postings joins the cold list; (2) the
+// cold list's postings are numbered by a prefix over their counts and dealt to
+// every lane, eight pairs at a time, each finding its entry by binary search,
+// and added into the tile's sims with LDS atomics (rounds of at most kColdCap
+// entries); (3) the dense part; (4) the argmax.
+__global__ __launch_bounds__(kHB, 4) void assign_hot_kernel(
     const int32_t* __restrict__ indptr, const uint16_t* __restrict__ indices, const uint8_t* __restrict__ values,
     const int64_t* __restrict__ kept_base, const int32_t* __restrict__ kept, const int32_t* __restrict__ lcb,
     const int32_t* __restrict__ centroid_doc, const int64_t* __restrict__ cent_off, int64_t e_base,
@@ -513,32 +545,36 @@ __global__ __launch_bounds__(kB) void assign_hot_kernel(
     int2* __restrict__ tmap_all, int dim, __bf16* __restrict__ bmat_all, int32_t* __restrict__ assign, int l_begin,
     int l_end, int mc, unsigned long long* __restrict__ pairs) {
     extern __shared__ __align__(16) unsigned char smem[];
-    int mc2 = 1;
-    while (mc2 < mc) mc2 <<= 1;
+    const HotLds LP = hot_lds(mc);
     const int mcp = (mc + 15) & ~15;
-    int32_t* cent = reinterpret_cast<int32_t*>(smem);  // sorted centroid ids [mc2]
-    __bf16* atile = reinterpret_cast<__bf16*>(smem + ((mc2 * 4 + 15) & ~15));  // [kTileDocs][kHot]
-    int32_t* sims = reinterpret_cast<int32_t*>(reinterpret_cast<unsigned char*>(atile) + kTileDocs * kHot * 2);
-    int32_t* hist = sims;  // [kHotHist], list set-up only
-    const int sims_words = kTileDocs * mc > kHotHist ? kTileDocs * mc : kHotHist;
-    int32_t* misc = sims + ((sims_words + 3) & ~3);  // [0] hot slots used, [1] threshold
+    int32_t* cent = reinterpret_cast<int32_t*>(smem + LP.o_cent);     // sorted centroid ids
+    __bf16* atile = reinterpret_cast<__bf16*>(smem + LP.o_tile);        // [kTileDocs][kHot]
+    int32_t* sims = reinterpret_cast<int32_t*>(smem + LP.o_sims);      // [kTileDocs][nc]
+    int32_t* hist = sims;                                              // [kHotHist], list set-up only
+    int32_t* ccs = reinterpret_cast<int32_t*>(smem + LP.o_ccs);        // cold: CSC range start
+    int32_t* cpre = reinterpret_cast<int32_t*>(smem + LP.o_cpre);      // cold: posting counts, then their prefix
+    int32_t* crd = reinterpret_cast<int32_t*>(smem + LP.o_crd);        // cold: row | code << 8
+    int32_t* tdoc = reinterpret_cast<int32_t*>(smem + LP.o_tdoc);      // tile row -> doc (-1: none or a centroid)
+    int32_t* tstart = reinterpret_cast<int32_t*>(smem + LP.o_tstart);  // tile row -> first element
+    int32_t* tpre = reinterpret_cast<int32_t*>(smem + LP.o_tpre);      // tile rows' element prefix
+    int32_t* misc = reinterpret_cast<int32_t*>(smem + LP.o_misc);      // [0] hot slots, [1] threshold, [2] cold entries
     int2* tmap = tmap_all + static_cast<size_t>(blockIdx.x) * dim;
     __bf16* bmat = bmat_all + static_cast<size_t>(blockIdx.x) * mcp * kHot;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    uint64_t n_pairs = 0;  // cold pairs (wave-uniform count)
-    uint64_t hot_pairs = 0;  // this lane's hot pairs
+    constexpr int kWaves = kHB / 64;
+    uint64_t n_pairs = 0;  // this lane's pairs, hot and cold
     for (int l = l_begin + blockIdx.x; l < l_end; l += gridDim.x) {
         const int64_t kb = kept_base[l], n = kept_base[l + 1] - kb;
         const int c0 = lcb[l], nc = lcb[l + 1] - c0;
         if (n == 0 || nc == 0) continue;
         if (n == nc) {  // every kept doc is a centroid: nothing to assign
-            for (int64_t j = tid; j < n; j += kB) assign[kb + j] = -1;
+            for (int64_t j = tid; j < n; j += kHB) assign[kb + j] = -1;
             continue;
         }
         const int64_t e0 = cent_off[c0] - e_base, e1 = cent_off[c0 + nc] - e_base;
-        for (int i = tid; i < kHotHist; i += kB) hist[i] = 0;
+        for (int i = tid; i < kHotHist; i += kHB) hist[i] = 0;
         if (tid == 0) misc[0] = 0;
-        for (int64_t e = e0 + tid; e < e1; e += kB) {
+        for (int64_t e = e0 + tid; e < e1; e += kHB) {
             const uint32_t k = ckey[e];
             const int t = static_cast<int>(k & 0xFFFFu);
             if (e == e0 || ckey[e - 1] != k) tmap[t].x = static_cast<int>(e);
@@ -546,7 +582,7 @@ __global__ __launch_bounds__(kB) void assign_hot_kernel(
         }
         __syncthreads();
         // histogram of the terms' centroid-posting counts
-        for (int64_t e = e0 + tid; e < e1; e += kB) {
+        for (int64_t e = e0 + tid; e < e1; e += kHB) {
             const uint32_t k = ckey[e];
             if (e == e0 || ckey[e - 1] != k) {
                 const int2 col = tmap[k & 0xFFFFu];
@@ -557,16 +593,16 @@ __global__ __launch_bounds__(kB) void assign_hot_kernel(
         // threshold: the smallest count >= 2 whose terms (those with at least
         // that many postings) fit the kHot slots; lane i holds bins 16i..16i+15
         if (wave == 0) {
-            int bins[16], s = 0;
+            int bins[16], sum = 0;
 #pragma unroll
             for (int b = 0; b < 16; ++b) {
                 bins[b] = hist[lane * 16 + b];
-                s += bins[b];
+                sum += bins[b];
             }
             // suffix sums over the lanes: a prefix over the reversed order
-            const int rev = __shfl(s, 63 - lane, 64);
+            const int rev = __shfl(sum, 63 - lane, 64);
             const int rincl = static_cast<int>(wave_incl_scan_dpp(static_cast<uint32_t>(rev)));
-            const int suf = __shfl(rincl, 63 - lane, 64);  // terms in bins >= 16 * lane
+            const int suf = __shfl(rincl, 63 - lane, 64);                    // terms in bins >= 16 lane
             const int above = lane == 63 ? 0 : __shfl(suf, lane + 1, 64);  // terms in bins >= 16 (lane + 1)
             // each lane's smallest bin b with S(b) <= kHot, if any of its bins fits
             int acc = above, tl = 16 * (lane + 1);
@@ -584,7 +620,7 @@ __global__ __launch_bounds__(kB) void assign_hot_kernel(
         __syncthreads();
         const int theta = misc[1];
         // hot terms: a slot each, marked in tmap as (-2 - slot, postings)
-        for (int64_t e = e0 + tid; e < e1; e += kB) {
+        for (int64_t e = e0 + tid; e < e1; e += kHB) {
             const uint32_t k = ckey[e];
             if (e == e0 || ckey[e - 1] != k) {
                 const int t = static_cast<int>(k & 0xFFFFu);
@@ -602,15 +638,15 @@ __global__ __launch_bounds__(kB) void assign_hot_kernel(
         {
             uint4* bz = reinterpret_cast<uint4*>(bmat);
             const int nz = ncp * kHot / 8;
-            for (int i = tid; i < nz; i += kB) bz[i] = make_uint4(0, 0, 0, 0);
+            for (int i = tid; i < nz; i += kHB) bz[i] = make_uint4(0, 0, 0, 0);
         }
         int n2 = 1;
         while (n2 < nc) n2 <<= 1;
-        for (int i = tid; i < n2; i += kB) cent[i] = i < nc ? centroid_doc[c0 + i] : INT32_MAX;
+        for (int i = tid; i < n2; i += kHB) cent[i] = i < nc ? centroid_doc[c0 + i] : INT32_MAX;
         __syncthreads();
         for (int size = 2; size <= n2; size <<= 1) {
             for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                for (int t = tid; t < n2 / 2; t += kB) {
+                for (int t = tid; t < n2 / 2; t += kHB) {
                     const int i = 2 * t - (t & (stride - 1)), j = i + stride;
                     const int32_t a = cent[i], b = cent[j];
                     if ((a > b) == ((i & size) == 0)) {
@@ -622,99 +658,169 @@ __global__ __launch_bounds__(kB) void assign_hot_kernel(
             }
         }
         // the hot terms' centroid codes into bmat[centroid][slot]
-        for (int64_t e = e0 + tid; e < e1; e += kB) {
+        for (int64_t e = e0 + tid; e < e1; e += kHB) {
             const int2 col = tmap[ckey[e] & 0xFFFFu];
             if (col.x <= -2) bmat[static_cast<size_t>(cc[e]) * kHot + (-2 - col.x)] = static_cast<__bf16>(cw[e]);
         }
         __syncthreads();
         const int nhot = misc[0] < kHot ? misc[0] : kHot;
-        const int ksteps = (nhot + 31) >> 5;  // 32 slots per MFMA
         for (int64_t j0 = 0; j0 < n; j0 += kTileDocs) {
             const int nt = n - j0 < kTileDocs ? static_cast<int>(n - j0) : kTileDocs;
+            // the tile's docs (centroids and rows past the list are empty) and
+            // the prefix of their row lengths
+            if (wave == 0) {
+                int d = -1, r0 = 0, len = 0;
+                if (lane < nt) {
+                    d = kept[kb + j0 + lane];
+                    int lo = 0, hi = nc;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (cent[mid] < d)
+                            lo = mid + 1;
+                        else
+                            hi = mid;
+                    }
+                    if (lo < nc && cent[lo] == d) d = -1;
+                    if (d >= 0) {
+                        r0 = indptr[d];
+                        len = indptr[d + 1] - r0;
+                    }
+                }
+                const int incl = static_cast<int>(wave_incl_scan_dpp(static_cast<uint32_t>(len)));
+                if (lane < kTileDocs) {
+                    tdoc[lane] = d;
+                    tstart[lane] = r0;
+                    tpre[lane] = incl - len;
+                }
+                if (lane == kTileDocs - 1) tpre[kTileDocs] = incl;
+            }
             {
                 uint4* az = reinterpret_cast<uint4*>(atile);
-                for (int i = tid; i < kTileDocs * kHot / 8; i += kB) az[i] = make_uint4(0, 0, 0, 0);
-                for (int i = tid; i < kTileDocs * nc; i += kB) sims[i] = 0;
+                for (int i = tid; i < kTileDocs * kHot / 8; i += kHB) az[i] = make_uint4(0, 0, 0, 0);
+                for (int i = tid; i < kTileDocs * nc; i += kHB) sims[i] = 0;
             }
             __syncthreads();
-            // each wave: its docs' hot codes into the tile, cold pairs into sims
-            for (int r = wave; r < nt; r += kB / 64) {
-                const int32_t d = kept[kb + j0 + r];
-                int lo = 0, hi = nc;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (cent[mid] < d)
-                        lo = mid + 1;
-                    else
-                        hi = mid;
-                }
-                if (lo < nc && cent[lo] == d) continue;  // a centroid
-                int32_t* srow = sims + r * nc;
-                __bf16* arow = atile + r * kHot;
-                const int32_t r0 = indptr[d], r1 = indptr[d + 1];
-                // the cold pairs are numbered by a prefix over the terms' posting
-                // counts and dealt to the lanes, kW windows of 64 with their loads
-                // in flight together (as in assign_kernel)
-                constexpr int kW = 16;
-                for (int b = r0; b < r1; b += 64) {
-                    const int p = b + lane;
-                    int cs = 0, cnt = 0, dv = 0;
-                    if (p < r1) {
-                        const int2 col = tmap[indices[p]];
-                        dv = values[p];
-                        if (col.x <= -2) {
-                            arow[-2 - col.x] = static_cast<__bf16>(dv);
-                            hot_pairs += static_cast<uint64_t>(col.y);
-                        } else if (col.x >= 0) {
-                            cs = col.x;
-                            cnt = col.y - col.x;
+            const int E = tpre[kTileDocs];
+            // rounds of at most kColdCap entries, so the cold list never overflows
+            for (int s0 = 0; s0 < E; s0 += kColdCap) {
+                const int s1 = min(E, s0 + kColdCap);
+                if (tid == 0) misc[2] = 0;
+                __syncthreads();
+                // (1) entries: hot codes into the tile, cold terms into the list
+                constexpr int kA = 4;
+                for (int f0 = s0; f0 < s1; f0 += kHB * kA) {
+                    int rr[kA], term[kA], val[kA];
+#pragma unroll
+                    for (int u = 0; u < kA; ++u) {
+                        const int f = f0 + u * kHB + tid;
+                        rr[u] = -1;
+                        term[u] = 0;
+                        val[u] = 0;
+                        if (f < s1) {
+                            int r = 0;
+#pragma unroll
+                            for (int st = 8; st > 0; st >>= 1)
+                                if (tpre[r + st] <= f) r += st;
+                            const int p = tstart[r] + (f - tpre[r]);
+                            rr[u] = r;
+                            term[u] = indices[p];
+                            val[u] = values[p];
                         }
                     }
-                    const int incl = static_cast<int>(wave_incl_scan_dpp(static_cast<uint32_t>(cnt)));
-                    const int excl = incl - cnt;
-                    const int total = __builtin_amdgcn_readlane(incl, 63);
-                    n_pairs += static_cast<uint64_t>(total);
-                    for (int i0 = 0; i0 < total; i0 += 64 * kW) {
-                        int ev[kW], wv[kW];
-                        bool ok[kW];
+                    int2 col[kA];
 #pragma unroll
-                        for (int u = 0; u < kW; ++u) {
-                            const int w0 = i0 + 64 * u, i = w0 + lane;
-                            ok[u] = i < total;
-                            ev[u] = 0;
-                            wv[u] = 0;
-                            if (w0 >= total) continue;  // wave-uniform
-                            int owner = __popcll(__ballot(excl <= w0)) - 1;
-                            const uint64_t bm = __ballot(excl > w0 && excl <= w0 + 63 && cnt > 0);
-                            for (uint64_t m = bm; m != 0; m &= m - 1) {
-                                const int o = __ffsll(static_cast<unsigned long long>(m)) - 1;
-                                if (i >= __builtin_amdgcn_readlane(excl, o)) owner = o;
-                            }
-                            const int ocs = __shfl(cs, owner, 64), oex = __shfl(excl, owner, 64);
-                            wv[u] = __shfl(dv, owner, 64);
-                            ev[u] = ocs + (i - oex);
+                    for (int u = 0; u < kA; ++u) col[u] = rr[u] >= 0 ? tmap[term[u]] : make_int2(-1, -1);
+#pragma unroll
+                    for (int u = 0; u < kA; ++u) {
+                        const int r = rr[u];
+                        const int2 cl = col[u];
+                        if (r >= 0 && cl.x <= -2) {
+                            atile[r * kHot + (-2 - cl.x)] = static_cast<__bf16>(val[u]);
+                            n_pairs += static_cast<uint64_t>(cl.y);
                         }
-                        uint32_t cl[kW], cwv[kW];
-#pragma unroll
-                        for (int u = 0; u < kW; ++u) {
-                            cl[u] = ok[u] ? cc[ev[u]] : 0u;
-                            cwv[u] = ok[u] ? static_cast<uint32_t>(cw[ev[u]]) : 0u;
+                        const bool cold = r >= 0 && cl.x >= 0 && cl.y > cl.x;
+                        const uint64_t m = __ballot(cold);
+                        int base = 0;
+                        if (lane == 0 && m != 0) base = atomicAdd(&misc[2], __popcll(m));
+                        base = __shfl(base, 0, 64);
+                        if (cold) {
+                            const int at = base + __popcll(m & ((1ULL << lane) - 1));
+                            ccs[at] = cl.x;
+                            cpre[at] = cl.y - cl.x;
+                            crd[at] = r | (val[u] << 8);
                         }
-#pragma unroll
-                        for (int u = 0; u < kW; ++u)
-                            if (ok[u]) atomicAdd(&srow[cl[u]], wv[u] * static_cast<int32_t>(cwv[u]));
                     }
                 }
+                __syncthreads();
+                // (2) the cold postings: exclusive prefix of the counts (four
+                // entries per lane, then across lanes and waves), then pairs
+                const int M = misc[2];
+                int loc[4], lsum = 0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int i = tid * 4 + u;
+                    loc[u] = i < M ? cpre[i] : 0;
+                    lsum += loc[u];
+                }
+                const int wincl = static_cast<int>(wave_incl_scan_dpp(static_cast<uint32_t>(lsum)));
+                int* wtot = misc + 4;  // [kWaves]
+                if (lane == 63) wtot[wave] = wincl;
+                __syncthreads();
+                int wbase = 0;
+                for (int w = 0; w < wave; ++w) wbase += wtot[w];
+                int run = wbase + wincl - lsum;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int i = tid * 4 + u;
+                    if (i < M) cpre[i] = run;
+                    run += loc[u];
+                }
+                if (tid == kHB - 1) cpre[M] = run;  // every count is in by the last lane
+                __syncthreads();
+                const int P = cpre[M];
+                constexpr int kU = 8;
+                for (int i0 = 0; i0 < P; i0 += kHB * kU) {
+                    int ent[kU], off[kU];
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        const int i = i0 + u * kHB + tid;
+                        int lo = 0, hi = M;  // the last entry whose prefix is <= i
+                        while (hi - lo > 1) {
+                            const int mid = (lo + hi) >> 1;
+                            if (cpre[mid] <= i)
+                                lo = mid;
+                            else
+                                hi = mid;
+                        }
+                        ent[u] = i < P ? lo : -1;
+                        off[u] = i - cpre[lo];
+                    }
+                    uint32_t cl[kU], cwv[kU];
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        const int64_t e = ent[u] >= 0 ? static_cast<int64_t>(ccs[ent[u]]) + off[u] : 0;
+                        cl[u] = ent[u] >= 0 ? cc[e] : 0u;
+                        cwv[u] = ent[u] >= 0 ? static_cast<uint32_t>(cw[e]) : 0u;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        if (ent[u] < 0) continue;
+                        const int rd = crd[ent[u]];
+                        atomicAdd(&sims[(rd & 0xFF) * nc + static_cast<int>(cl[u])],
+                                  (rd >> 8) * static_cast<int32_t>(cwv[u]));
+                    }
+                }
+                n_pairs += static_cast<uint64_t>(tid == 0 ? P : 0);
+                __syncthreads();
             }
-            __syncthreads();
-            // dense part: 16 docs x 16 centroids per MFMA tile, K = 32 hot slots
-            // per step; lane l holds A[l % 16][8 (l / 16) ..], B[8 (l / 16) ..][l % 16]
-            // and D[4 (l / 16) + v][l % 16]
-            if (ksteps > 0) {
-                // all K steps' B fragments are loaded before the first MFMA (slots
-                // past the hot ones are zero in both operands)
+            // (3) dense part: 16 docs x 16 centroids per MFMA tile, K = 32 hot
+            // slots per step; lane l holds A[l % 16][8 (l / 16) ..],
+            // B[8 (l / 16) ..][l % 16] and D[4 (l / 16) + v][l % 16]. All K
+            // steps' B fragments are loaded before the first MFMA (slots past
+            // the hot ones are zero in both operands).
+            if (nhot > 0) {
                 const int arow = lane & 15, kq = 8 * (lane >> 4);
-                for (int ct = wave; ct * 16 < nc; ct += kB / 64) {
+                for (int ct = wave; ct * 16 < nc; ct += kWaves) {
                     const int c = ct * 16 + (lane & 15);
                     const __bf16* brow = bmat + static_cast<size_t>(c) * kHot + kq;
                     bf16x8_t bv[kHot / 32];
@@ -723,8 +829,8 @@ __global__ __launch_bounds__(kB) void assign_hot_kernel(
                     f32x4_t acc = {0.0F, 0.0F, 0.0F, 0.0F};
 #pragma unroll
                     for (int ks = 0; ks < kHot / 32; ++ks) {
-                        const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(atile + arow * kHot + ks * 32 + kq);
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv[ks], acc, 0, 0, 0);
+                        const bf16x8_t av = *reinterpret_cast<const bf16x8_t*>(atile + arow * kHot + ks * 32 + kq);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[ks], acc, 0, 0, 0);
                     }
                     if (c < nc) {
 #pragma unroll
@@ -736,19 +842,10 @@ __global__ __launch_bounds__(kB) void assign_hot_kernel(
                 }
             }
             __syncthreads();
-            // argmax per doc (strict >: the lowest cluster wins ties)
-            for (int r = wave; r < nt; r += kB / 64) {
-                const int32_t d = kept[kb + j0 + r];
-                int lo = 0, hi = nc;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (cent[mid] < d)
-                        lo = mid + 1;
-                    else
-                        hi = mid;
-                }
-                if (lo < nc && cent[lo] == d) {
-                    if (lane == 0) assign[kb + j0 + r] = -1;
+            // (4) argmax per doc (strict >: the lowest cluster wins ties)
+            for (int r = wave; r < nt; r += kWaves) {
+                if (tdoc[r] < 0) {
+                    if (lane == 0) assign[kb + j0 + r] = -1;  // a centroid
                     continue;
                 }
                 const int32_t* srow = sims + r * nc;
@@ -764,17 +861,16 @@ __global__ __launch_bounds__(kB) void assign_hot_kernel(
             }
             __syncthreads();
         }
-        for (int64_t e = e0 + tid; e < e1; e += kB) tmap[ckey[e] & 0xFFFFu] = make_int2(-1, -1);
+        for (int64_t e = e0 + tid; e < e1; e += kHB) tmap[ckey[e] & 0xFFFFu] = make_int2(-1, -1);
         __syncthreads();
     }
-    // hot pairs: a wave sum of the lanes' counts
+    // the lanes' pair counts, summed per wave
     for (int off = 32; off > 0; off >>= 1) {
-        const uint32_t lo = __shfl_xor(static_cast<uint32_t>(hot_pairs), off, 64);
-        const uint32_t hi = __shfl_xor(static_cast<uint32_t>(hot_pairs >> 32), off, 64);
-        hot_pairs += (static_cast<uint64_t>(hi) << 32) | lo;
+        const uint32_t lo = __shfl_xor(static_cast<uint32_t>(n_pairs), off, 64);
+        const uint32_t hi = __shfl_xor(static_cast<uint32_t>(n_pairs >> 32), off, 64);
+        n_pairs += (static_cast<uint64_t>(hi) << 32) | lo;
     }
-    const uint64_t all = n_pairs + hot_pairs;
-    if (lane == 0 && all != 0) atomicAdd(pairs, static_cast<unsigned long long>(all));
+    if (lane == 0 && n_pairs != 0) atomicAdd(pairs, static_cast<unsigned long long>(n_pairs));
 }
 
 // Non-zeros of the kept postings' rows (for the build's byte count).
@@ -1911,11 +2007,16 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
     }
     // uint8 lists take the hybrid assignment (dense hot terms on the matrix
     // cores) when its LDS plan keeps three workgroups per CU
-    const bool hot = sq && assign_hot_lds(mc) <= 64 * 1024;
+    const bool hot = sq && mc <= kHotMaxClusters;
     const int hot_grid = std::min(L, 1024);
     const size_t mcp = static_cast<size_t>((mc + 15) & ~15);
     DeviceBuffer bmat;
-    if (hot) bmat.allocate(static_cast<size_t>(hot_grid) * mcp * kHot * 2);
+    if (hot) {
+        bmat.allocate(static_cast<size_t>(hot_grid) * mcp * kHot * 2);
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(assign_hot_kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, hot_lds(mc).total),
+                  "hipFuncSetAttribute");
+    }
     for (size_t bi = 0; bi + 1 < cbatch.size(); ++bi) {
         const int l0 = cbatch[bi], l1 = cbatch[bi + 1];
         const size_t g0 = static_cast<size_t>(lcb[l0]), ng = static_cast<size_t>(lcb[l1]) - g0;
@@ -1946,7 +2047,7 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
         }
         const int grid = std::min(l1 - l0, assign_grid);
         if (hot)
-            hipLaunchKernelGGL(assign_hot_kernel, dim3(std::min(l1 - l0, hot_grid)), dim3(kB), assign_hot_lds(mc),
+            hipLaunchKernelGGL(assign_hot_kernel, dim3(std::min(l1 - l0, hot_grid)), dim3(kHB), hot_lds(mc).total,
                                nullptr, fwd.indptr, fwd.indices, static_cast<const uint8_t*>(fwd.values),
                                d_kept_base.get<int64_t>(), kept.get<int32_t>(), d_lcb.get<int32_t>(),
                                centroid_doc.get<int32_t>(), cent_off.get<int64_t>(), eb, ckey.get<uint32_t>(),

@@ -19,6 +19,9 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <cstdlib>
+#include <string>
+
 #include <algorithm>
 #include <cstddef>
 #include <cstring>
@@ -175,12 +178,20 @@ public:
                 free_.pop_back();
                 return Lease(*this, std::move(s));
             }
-            first = made_++ == 0;
+            first = made_++ == 0 && null_stream_first();
         }
         return Lease(*this, std::make_unique<HostStage>(!first));
     }
 
 private:
+    // The null stream synchronizes with every blocking stream of the process
+    // (a search waits for work queued there before it, and work queued there
+    // after it waits for the search); NSPARSE_STAGE_NULL_STREAM=0 gives the
+    // first stage a non-blocking stream of its own instead.
+    static bool null_stream_first() {
+        const char* e = std::getenv("NSPARSE_STAGE_NULL_STREAM");
+        return !(e != nullptr && std::string(e) == "0");
+    }
     // The null-stream stage goes to the back, where take() looks first.
     void give_back(std::unique_ptr<HostStage> s) {
         std::lock_guard<std::mutex> lock(mu_);

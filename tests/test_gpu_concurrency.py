@@ -146,3 +146,30 @@ def test_concurrent_mixed_sizes_equal_sequential(nsparse, spec):
     for t in range(n_threads):
         for c, (d, lab) in enumerate(got[t]):
             assert np.array_equal(lab, want[c][1]) and np.array_equal(d, want[c][0]), (t, c)
+
+
+@pytest.mark.parametrize("spec", ["seismic_sq", "inverted"])
+def test_first_stage_on_its_own_stream(nsparse, spec, monkeypatch):
+    """With NSPARSE_STAGE_NULL_STREAM=0 an index's first host-array stage runs
+    on a non-blocking stream of its own instead of the null stream: single and
+    concurrent searches equal the default's."""
+    from nsparse import synth
+
+    print(f"first stage on its own stream, {spec}", flush=True)
+    docs = synth.make_corpus(synth.doc_spec(seed=61), 12000)
+    queries = synth.make_corpus(synth.query_spec(seed=62), 40)
+    vmin, vmax = float(docs[2].min()), float(docs[2].max())
+    desc = {"seismic_sq": f"seismic_sq,quantizer=8bit|vmin={vmin:.9g}|vmax={vmax:.9g}|lambda=200|beta=20|alpha=0.4|seed=0"
+            }.get(spec, spec)
+    idx = nsparse.index_factory(DIM, desc)
+    idx.add(len(docs[0]) - 1, *docs)
+    idx.build()
+    params = nsparse.SeismicSearchParameters(10, 0.8) if spec == "seismic_sq" else None
+    blob = nsparse.serialize_index(idx)
+    want_d, want_l = nsparse.read_index_bytes(blob).search(len(queries[0]) - 1, *queries, 10, params)
+    monkeypatch.setenv("NSPARSE_STAGE_NULL_STREAM", "0")
+    own = nsparse.read_index_bytes(blob)
+    d, lab = own.search(len(queries[0]) - 1, *queries, 10, params)
+    assert np.array_equal(lab, want_l) and np.array_equal(d, want_d)
+    for d, lab in _concurrent(own, queries, 10, params, n_threads=4):
+        assert np.array_equal(lab, want_l) and np.array_equal(d, want_d)
