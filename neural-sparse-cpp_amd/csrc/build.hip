@@ -239,26 +239,30 @@ __global__ void row_len_kernel(const int32_t* __restrict__ indptr, const int32_t
 
 // One posting per (centroid term): key (list, term), value = entry id. Covers
 // clusters [g0, g0 + n_clusters); entry ids are relative to e_base, the first
-// entry of cluster g0, so one batch of lists fits 32-bit sort values.
+// entry of cluster g0, so one batch of lists fits 32-bit sort values. A group
+// of 16 lanes writes one centroid's row, so the stores are contiguous.
 template <class T>
-__global__ void centroid_entries_kernel(const int32_t* __restrict__ indptr, const uint16_t* __restrict__ indices,
-                                        const T* __restrict__ values, const int32_t* __restrict__ centroid_doc,
-                                        const int32_t* __restrict__ cluster_list, const int32_t* __restrict__ lcb,
-                                        const int64_t* __restrict__ cent_off, size_t g0, size_t n_clusters,
-                                        int64_t e_base, uint32_t* __restrict__ key, int32_t* __restrict__ val,
-                                        uint16_t* __restrict__ ent_c, T* __restrict__ ent_w) {
-    GRID_STRIDE(i, n_clusters) {
+__global__ __launch_bounds__(kB) void centroid_entries_kernel(
+    const int32_t* __restrict__ indptr, const uint16_t* __restrict__ indices, const T* __restrict__ values,
+    const int32_t* __restrict__ centroid_doc, const int32_t* __restrict__ cluster_list, const int32_t* __restrict__ lcb,
+    const int64_t* __restrict__ cent_off, size_t g0, size_t n_clusters, int64_t e_base, uint32_t* __restrict__ key,
+    int32_t* __restrict__ val, uint16_t* __restrict__ ent_c, T* __restrict__ ent_w) {
+    const int grp = threadIdx.x >> 4, gl = threadIdx.x & 15;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * (kB / 16) + grp; i < n_clusters;
+         i += static_cast<size_t>(gridDim.x) * (kB / 16)) {
         const size_t g = g0 + i;
+        const int64_t e0 = cent_off[g] - e_base;
+        if (cent_off[g + 1] - e_base == e0) continue;  // list without docs to assign
         const int32_t d = centroid_doc[g];
         const int l = cluster_list[g];
         const uint16_t c = static_cast<uint16_t>(g - lcb[l]);
-        int64_t e = cent_off[g] - e_base;
-        if (cent_off[g + 1] - e_base == e) continue;  // list without docs to assign
-        for (int32_t p = indptr[d]; p < indptr[d + 1]; ++p, ++e) {
-            key[e] = (static_cast<uint32_t>(l) << 16) | indices[p];
+        const int32_t r0 = indptr[d], len = indptr[d + 1] - r0;
+        for (int j = gl; j < len; j += 16) {
+            const int64_t e = e0 + j;
+            key[e] = (static_cast<uint32_t>(l) << 16) | indices[r0 + j];
             val[e] = static_cast<int32_t>(e);
             ent_c[e] = c;
-            ent_w[e] = values[p];
+            ent_w[e] = values[r0 + j];
         }
     }
 }
@@ -2023,13 +2027,13 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
         const int64_t eb = list_ce[l0], n = list_ce[l1] - eb;
         if (n > 0) {
             if (sq)
-                hipLaunchKernelGGL((centroid_entries_kernel<uint8_t>), dim3(grid_for(ng)), dim3(kB), 0, nullptr,
+                hipLaunchKernelGGL((centroid_entries_kernel<uint8_t>), dim3(grid_for(ng, kB / 16)), dim3(kB), 0, nullptr,
                                    fwd.indptr, fwd.indices, static_cast<const uint8_t*>(fwd.values),
                                    centroid_doc.get<int32_t>(), cluster_list.get<int32_t>(), d_lcb.get<int32_t>(),
                                    cent_off.get<int64_t>(), g0, ng, eb, ckey_in.get<uint32_t>(),
                                    cval_in.get<int32_t>(), ent_c.get<uint16_t>(), ent_w.get<uint8_t>());
             else
-                hipLaunchKernelGGL((centroid_entries_kernel<float>), dim3(grid_for(ng)), dim3(kB), 0, nullptr,
+                hipLaunchKernelGGL((centroid_entries_kernel<float>), dim3(grid_for(ng, kB / 16)), dim3(kB), 0, nullptr,
                                    fwd.indptr, fwd.indices, static_cast<const float*>(fwd.values),
                                    centroid_doc.get<int32_t>(), cluster_list.get<int32_t>(), d_lcb.get<int32_t>(),
                                    cent_off.get<int64_t>(), g0, ng, eb, ckey_in.get<uint32_t>(),
