@@ -488,10 +488,17 @@ void SeismicIndex::search_host(idx_t n, const idx_t* indptr, const term_t* indic
     if (req.cut <= 0) throw std::invalid_argument("value must be positive");
     require_device();
     ensure_device();
-    for (idx_t i = 0; i < n; ++i)
-        if (indptr[i + 1] - indptr[i] > kDeviceQueryMax)
-            throw std::invalid_argument("query has more non-zeros than the device path supports (" +
-                                        std::to_string(kDeviceQueryMax) + ")");
+    // Queries longer than the per-query workgroup's tables take the spread
+    // search, whose tables are sized for the batch's longest query.
+    idx_t longest = 0;
+    for (idx_t i = 0; i < n; ++i) longest = std::max(longest, indptr[i + 1] - indptr[i]);
+    const int qcap = static_cast<int>(std::min<idx_t>(longest, INT32_MAX));
+    const bool long_queries = longest > kDeviceQueryMax;
+    if (long_queries && (stats_out != nullptr || !sq_ || view_.row_ref == nullptr ||
+                         !search_spread_fits(view_, max_clusters_, k, req.cut, max_list_docs_, qcap)))
+        throw std::invalid_argument("query has more non-zeros than the device path supports (" +
+                                    std::to_string(kDeviceQueryMax) +
+                                    "; longer ones need a uint8 index with block rows and a cut of at most 64)");
     // One upload of the queries (rebased so the device copy starts at 0) and
     // the selector bitmap, one download of the results and the error flag.
     const int64_t q0 = indptr[0], qnnz = indptr[n] - q0;
@@ -508,10 +515,10 @@ void SeismicIndex::search_host(idx_t n, const idx_t* indptr, const term_t* indic
     const int s_err = st->add(HostStage::kOut, 4);
     const int s_stats =
         stats_out != nullptr ? st->add(HostStage::kOut, static_cast<size_t>(n) * kStatFields * 8) : -1;
-    const bool spread = use_spread(n, k, req.cut, stats_out != nullptr, st->stream());
+    const bool spread = long_queries || use_spread(n, k, req.cut, stats_out != nullptr, st->stream());
     const int s_spread =
         spread ? st->add(HostStage::kScratch,
-                         search_spread_scratch_bytes(view_, max_clusters_, k, req.cut, max_list_docs_, n))
+                         search_spread_scratch_bytes(view_, max_clusters_, k, req.cut, max_list_docs_, n, qcap))
                : -1;
     st->commit();
     int32_t* qip = st->host<int32_t>(s_qip);
@@ -555,7 +562,7 @@ void SeismicIndex::search_host(idx_t n, const idx_t* indptr, const term_t* indic
     a.n_queries = n;
     a.query_base = 0;
     if (spread)
-        launch_search_spread(a, max_clusters_, k, max_list_docs_, st->dev<void>(s_spread), st->stream());
+        launch_search_spread(a, max_clusters_, k, max_list_docs_, st->dev<void>(s_spread), st->stream(), qcap);
     else
         launch_search_checked(a, max_clusters_, k, sq_, st->stream());
     st->download_and_wait();

@@ -81,11 +81,15 @@ void launch_exact_checked(const ExactArgs& a, bool sq, hipStream_t stream);
 // whole GPU, one workgroup per (cut list, slice of its clusters), then an exact
 // replay; uint8 indexes with block rows only, no traversal counters. Same
 // results as launch_search. Scratch: search_spread_scratch_bytes, per launch.
-bool search_spread_fits(const DevIndexView& ix, int max_clusters, int kmax, int cut, int64_t max_list_docs);
+// `qcap` is the longest query (non-zeros) the launch must take: its query
+// tables are sized for max(qcap, kDeviceQueryMax), as far as LDS allows, so
+// longer queries than the per-query workgroup's can run here.
+bool search_spread_fits(const DevIndexView& ix, int max_clusters, int kmax, int cut, int64_t max_list_docs,
+                        int qcap = kDeviceQueryMax);
 size_t search_spread_scratch_bytes(const DevIndexView& ix, int max_clusters, int kmax, int cut,
-                                   int64_t max_list_docs, int64_t n_queries);
+                                   int64_t max_list_docs, int64_t n_queries, int qcap = kDeviceQueryMax);
 void launch_search_spread(const SearchArgs& a, int max_clusters, int kmax, int64_t max_list_docs, void* scratch,
-                          hipStream_t stream);
+                          hipStream_t stream, int qcap = kDeviceQueryMax);
 // Most docs any posting list of `ix` holds (device lists).
 int64_t max_list_docs(const DevIndexView& ix);
 // Test surface (testing.hip): the GPU's first-list order (one wavefront, or the

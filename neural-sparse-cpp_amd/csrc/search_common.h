@@ -631,16 +631,18 @@ __device__ __forceinline__ void match_summary_columns(const DevIndexView& ix, ui
 // sum, a prefix over their lengths numbers the entries, then all lanes take
 // entries with kU loads in flight, each finding its column by binary search.
 // Integer atomics make the per-cluster sums exact in any order. `uni` holds
-// kQMax * 12 + 4 bytes of scratch. Ends with a barrier.
+// qcap * 12 + 4 bytes of scratch (qcap >= the query's distinct terms). Ends
+// with a barrier.
 template <int BLOCK>
 __device__ __forceinline__ void accumulate_summaries_sq(const DevIndexView& ix, int32_t tb, int32_t te, int nf,
                                                         const int32_t* flist, const int32_t* rcol,
-                                                        const uint32_t* rsum, unsigned char* uni, uint32_t* sacc) {
+                                                        const uint32_t* rsum, unsigned char* uni, uint32_t* sacc,
+                                                        int qcap = kQMax) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint8_t* sv = static_cast<const uint8_t*>(ix.sum_value);
     int32_t* fe0 = reinterpret_cast<int32_t*>(uni);  // column start, relative to the list's first entry
-    int32_t* fpre = fe0 + kQMax;                      // [nf + 1] exclusive prefix of column lengths
-    uint32_t* fqs = reinterpret_cast<uint32_t*>(fpre + kQMax + 1);
+    int32_t* fpre = fe0 + qcap;                       // [nf + 1] exclusive prefix of column lengths
+    uint32_t* fqs = reinterpret_cast<uint32_t*>(fpre + qcap + 1);
     const int64_t lbase = te > tb ? ix.sum_term_ptr[tb] : 0;
     for (int f = tid; f < nf; f += BLOCK) {
         const int r = flist[f];

@@ -59,6 +59,7 @@ enum ReplayMisc { kRErr = 0, kRTkn, kRMiscCount = 16 };
 // LDS and scratch layouts (computed on the host).
 struct SpreadPlan {
     int32_t nw;      // query bitmap words
+    int32_t qcap;    // longest query (non-zeros) the tables hold
     int32_t mc;      // cluster capacity of a list
     int32_t kmax;    // top-k capacity
     int32_t ncap;    // cut slots (grid.x of the list launch)
@@ -129,7 +130,7 @@ __global__ __launch_bounds__(kLBlock) void spread_lists_kernel(SearchArgs a, Spr
         if (s >= a.n_bitmaps) return;  // the replay reports a rejected query
         sel = s >= 0 ? a.sel_bitmaps[s] : nullptr;
     }
-    if (qn > kQMax || k <= 0 || k > P.kmax) return;
+    if (qn > P.qcap || k <= 0 || k > P.kmax) return;
     unsigned char* qs = scratch + static_cast<int64_t>(blockIdx.z) * P.s_bytes;
 
     uint8_t* qcode = smem;
@@ -198,7 +199,7 @@ __global__ __launch_bounds__(kLBlock) void spread_lists_kernel(SearchArgs a, Spr
         for (int r = tid; r < nu; r += kLBlock)
             if (rcol[r] >= 0) flist[atomicAdd(&misc[kLNf], 1)] = r;
         __syncthreads();
-        accumulate_summaries_sq<kLBlock>(ix, tb, te, misc[kLNf], flist, rcol, rsum, uni, sacc);
+        accumulate_summaries_sq<kLBlock>(ix, tb, te, misc[kLNf], flist, rcol, rsum, uni, sacc, P.qcap);
         uint32_t* gsum = reinterpret_cast<uint32_t*>(qs + P.s_sum) + static_cast<int64_t>(ci) * P.mc;
         for (int c = tid; c < nc; c += kLBlock) gsum[c] = sacc[c];
         if (ci == first) {
@@ -340,7 +341,7 @@ __global__ __launch_bounds__(kRBlock) void spread_replay_kernel(SearchArgs a, Sp
     int32_t* misc = reinterpret_cast<int32_t*>(smem + P.r_misc);
     if (tid < kRMiscCount) misc[tid] = 0;
     __syncthreads();
-    int err = bad_sel ? kDevBadSelector : qn > kQMax ? kDevQueryTooLong : (k <= 0 || k > P.kmax) ? kDevBadK : 0;
+    int err = bad_sel ? kDevBadSelector : qn > P.qcap ? kDevQueryTooLong : (k <= 0 || k > P.kmax) ? kDevBadK : 0;
     if (err == 0) {
         for (int i = tid; i < qn; i += kRBlock)
             if (a.q_indices[s0 + i] >= static_cast<uint32_t>(ix.dim)) misc[kRErr] = 1;
@@ -536,12 +537,13 @@ __global__ void list_docs_max_kernel(const int32_t* __restrict__ lcb, const int6
 
 int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
-SpreadPlan make_spread_plan(int dim, int max_clusters, int kmax, int cut, int64_t max_list_docs) {
+SpreadPlan make_spread_plan(int dim, int max_clusters, int kmax, int cut, int64_t max_list_docs, int qcap) {
     SpreadPlan P{};
     P.nw = (dim + 31) / 32;
+    P.qcap = std::max(kQMax, (qcap + 63) & ~63);
     P.mc = std::max(2, (max_clusters + 1) & ~1);
     P.kmax = kmax;
-    P.ncap = std::max(1, std::min(cut, kQMax));
+    P.ncap = std::max(1, std::min(cut, P.qcap));
     P.mld = std::max<int64_t>(2, (max_list_docs + 1) & ~int64_t{1});
     int o = 0;
     auto take = [&](int bytes) {
@@ -552,14 +554,14 @@ SpreadPlan make_spread_plan(int dim, int max_clusters, int kmax, int cut, int64_
     take(dim);  // the dense code table at offset 0
     P.o_qbits = take(P.nw * 4);
     P.o_qpref = take(P.nw * 2);
-    P.o_eterm = take(kQMax * 2);
-    P.o_erank = take(kQMax * 2);
-    P.o_eval = take(kQMax * 4);
-    P.o_rlast = take(kQMax * 4);
-    P.o_rval = take(kQMax * 4);
-    P.o_rsum = take(kQMax * 4);
-    P.o_flist = take(kQMax * 4);
-    P.o_cuts = take(kQMax * 2);
+    P.o_eterm = take(P.qcap * 2);
+    P.o_erank = take(P.qcap * 2);
+    P.o_eval = take(P.qcap * 4);
+    P.o_rlast = take(P.qcap * 4);
+    P.o_rval = take(P.qcap * 4);
+    P.o_rsum = take(P.qcap * 4);
+    P.o_flist = take(P.qcap * 4);
+    P.o_cuts = take(P.qcap * 2);
     P.o_wsum = take(kMaxWaves * 4);
     P.o_misc = take(kLMiscCount * 4);
     P.o_cstart = take((P.mc + 1) * 4);
@@ -568,12 +570,12 @@ SpreadPlan make_spread_plan(int dim, int max_clusters, int kmax, int cut, int64_
     P.sort_cap = 2 * P.mc / 17 + 2;
     P.sort_rng = (P.mc * 12 + 15) & ~15;
     const int sort_bytes = P.sort_rng + 2 * P.sort_cap * 16;
-    const int uni = std::max(std::max(kQMax * 12 + 16, kQMax * 8), std::max(sort_bytes, (2 * kSubBatch + 1) * 4));
+    const int uni = std::max(std::max(P.qcap * 12 + 16, P.qcap * 8), std::max(sort_bytes, (2 * kSubBatch + 1) * 4));
     P.o_uni = take(uni);
     P.total_list = o;
 
     o = 0;
-    P.r_cuts = take(kQMax * 2);
+    P.r_cuts = take(P.qcap * 2);
     P.r_misc = take(kRMiscCount * 4);
     P.r_lmax = take(P.ncap * 8);
     P.r_tks = take(kmax * 4);
@@ -592,7 +594,7 @@ SpreadPlan make_spread_plan(int dim, int max_clusters, int kmax, int cut, int64_
     P.r_key = take(2 * P.r_key_stride);
     P.total_replay = o;
 
-    int64_t s = align_up(8 + kQMax * 2, 256);
+    int64_t s = align_up(8 + P.qcap * 2, 256);
     P.s_order = s;
     s = align_up(s + P.mc * 2, 256);
     P.s_sum = s;
@@ -618,24 +620,26 @@ int cu_count() {
 
 }  // namespace
 
-bool search_spread_fits(const DevIndexView& ix, int max_clusters, int kmax, int cut, int64_t max_list_docs) {
-    if (ix.esz != 1 || ix.row_ref == nullptr || cut <= 0 || std::min(cut, kQMax) > kMaxCutSlots) return false;
+bool search_spread_fits(const DevIndexView& ix, int max_clusters, int kmax, int cut, int64_t max_list_docs,
+                        int qcap) {
+    if (ix.esz != 1 || ix.row_ref == nullptr || cut <= 0 || std::min(cut, std::max(qcap, kQMax)) > kMaxCutSlots)
+        return false;
     // cluster ids are 16-bit in the visiting order; doc positions 32-bit
     if (max_clusters <= 0 || max_clusters > 16384 || max_list_docs >= (int64_t{1} << 31)) return false;
-    const SpreadPlan P = make_spread_plan(ix.dim, max_clusters, kmax, cut, max_list_docs);
+    const SpreadPlan P = make_spread_plan(ix.dim, max_clusters, kmax, cut, max_list_docs, qcap);
     return P.total_list <= kLdsCap && P.total_replay <= kLdsCap && P.keycap > 0;
 }
 
 size_t search_spread_scratch_bytes(const DevIndexView& ix, int max_clusters, int kmax, int cut,
-                                   int64_t max_list_docs, int64_t n_queries) {
-    const SpreadPlan P = make_spread_plan(ix.dim, max_clusters, kmax, cut, max_list_docs);
+                                   int64_t max_list_docs, int64_t n_queries, int qcap) {
+    const SpreadPlan P = make_spread_plan(ix.dim, max_clusters, kmax, cut, max_list_docs, qcap);
     return static_cast<size_t>(P.s_bytes) * static_cast<size_t>(n_queries);
 }
 
 void launch_search_spread(const SearchArgs& args, int max_clusters, int kmax, int64_t max_list_docs, void* scratch,
-                          hipStream_t stream) {
-    const SpreadPlan P = make_spread_plan(args.ix.dim, max_clusters, kmax, args.cut, max_list_docs);
-    if (!search_spread_fits(args.ix, max_clusters, kmax, args.cut, max_list_docs))
+                          hipStream_t stream, int qcap) {
+    const SpreadPlan P = make_spread_plan(args.ix.dim, max_clusters, kmax, args.cut, max_list_docs, qcap);
+    if (!search_spread_fits(args.ix, max_clusters, kmax, args.cut, max_list_docs, qcap))
         throw std::invalid_argument("the spread search does not fit this index");
     if (args.error_flag == nullptr || scratch == nullptr)
         throw std::invalid_argument("spread search launch without an error flag or scratch");

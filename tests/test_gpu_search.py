@@ -186,8 +186,9 @@ def test_clusters_larger_than_a_chunk(nsparse, data, big_cluster_pairs, monkeypa
 
 
 def test_longest_queries(nsparse, data, sq_pair, f32_pair):
-    """Queries of 480..512 non-zeros (the device maximum) against the oracle;
-    513 is rejected."""
+    """Queries of 480..512 non-zeros (the per-query workgroup's maximum)
+    against the oracle; 513 goes through the spread search on a uint8 index
+    with block rows and is rejected on an FP32 index."""
     from nsparse import synth
 
     spec = synth.SynthSpec(7, DIM, 1.1, 1, 480.0, 512.0, 480, 512, 0.0, 0.7)
@@ -197,12 +198,15 @@ def test_longest_queries(nsparse, data, sq_pair, f32_pair):
         want = o.search(ip, ix, v, 10, cut=10, hf=0.8)
         got = dev.search(len(ip) - 1, ip, ix, v, 10, params(nsparse, sq, 10, 0.8, vmax))
         assert_same(got, want)
-    o, dev, vmax = sq_pair
     ip2 = np.array([0, 513], np.int32)
     ix2 = np.arange(513, dtype=np.uint16)
     v2 = np.ones(513, np.float32)
-    with pytest.raises(Exception):
-        dev.search(1, ip2, ix2, v2, 10, params(nsparse, 1, 10, 0.8, vmax))
+    o, dev, vmax = sq_pair
+    assert_same(dev.search(1, ip2, ix2, v2, 10, params(nsparse, 1, 10, 0.8, vmax)),
+                o.search(ip2, ix2, v2, 10, cut=10, hf=0.8))
+    o, dev, vmax = f32_pair
+    with pytest.raises(ValueError):
+        dev.search(1, ip2, ix2, v2, 10, params(nsparse, 0, 10, 0.8, vmax))
 
 
 def test_batched_device_mixed_k_equals_single(nsparse, data, sq_pair):

@@ -132,6 +132,32 @@ def test_repeated_terms_unsorted_rows_and_empty_queries(nsparse, pair, forced):
     assert (got[1][5] == -1).all() and (got[0][5] == -1.0).all()
 
 
+def test_queries_longer_than_the_workgroup_tables(nsparse, data, pair):
+    """Queries of more than 512 non-zeros (the per-query workgroup's tables)
+    go through the spread search, whose tables are sized for the call's
+    longest query: single calls and a batch mixing short and long queries
+    equal the oracle; a query longer than its LDS can hold is rejected."""
+    o, dev, _ = pair
+    rng = np.random.default_rng(23)
+    lens = [700, 1500, 12, 2400, 513]
+    rows = [np.sort(rng.choice(DIM, m, replace=False)).astype(np.uint16) for m in lens]
+    ip = np.r_[0, np.cumsum(lens)].astype(np.int32)
+    ix = np.concatenate(rows)
+    v = rng.lognormal(0, 0.7, len(ix)).astype(np.float32)
+    for k in (10, 100):
+        want = o.search(ip, ix, v, k, cut=10, hf=0.8)
+        got = dev.search(len(lens), ip, ix, v, k, params(nsparse, 10, 0.8))
+        assert_same(got, want)
+        for i in (0, 1, 3):
+            one = np.array([0, lens[i]], np.int32)
+            got1 = dev.search(1, one, rows[i], v[ip[i]:ip[i + 1]], k, params(nsparse, 10, 0.8))
+            assert_same(got1, (want[0][i:i + 1], want[1][i:i + 1]))
+    huge = np.sort(rng.choice(DIM, 20000, replace=False)).astype(np.uint16)
+    with pytest.raises(ValueError):
+        dev.search(1, np.array([0, len(huge)], np.int32), huge, np.ones(len(huge), np.float32), 10,
+                   params(nsparse, 10, 0.8))
+
+
 @pytest.mark.parametrize("lam,beta", [(6000, 2), (20000, 10)])
 def test_large_clusters_and_long_lists(nsparse, data, forced, lam, beta):
     """Clusters of thousands of docs (a list workgroup scores its slice in
