@@ -92,7 +92,9 @@ typedef enum nsparse_selector_kind {
     NSPARSE_SELECTOR_ARRAY = 0,  /* ArrayIDSelector(n, ids)  id_selector.h:51-66 */
     NSPARSE_SELECTOR_SET = 1,    /* SetIDSelector(n, ids)    id_selector.h:33-49 */
     NSPARSE_SELECTOR_NOT = 2,    /* NotIDSelector(inner)     id_selector.h:68-78 */
-    NSPARSE_SELECTOR_BITMAP = 3  /* membership bitmap over ids; not enumerable (extension) */
+    NSPARSE_SELECTOR_BITMAP = 3,  /* membership bitmap over ids; not enumerable (extension) */
+    NSPARSE_SELECTOR_CALLBACK = 4 /* is_member(ctx, id): any other IDSelector, id_selector.h:22-27;
+                                     not enumerable (extension) */
 } nsparse_selector_kind;
 
 typedef struct nsparse_selector {
@@ -101,6 +103,12 @@ typedef struct nsparse_selector {
     const nsparse_idx_t* ids;              /* ARRAY/SET */
     const uint32_t* bitmap;                /* BITMAP: bit i of word i/32 set => member */
     const struct nsparse_selector* inner;  /* NOT: non-owning, like the reference */
+    /* CALLBACK: returns 1 for a member, 0 for a non-member, negative on failure
+     * (the search then reports NSPARSE_ERR_RUNTIME). Called on the calling
+     * thread, during the call only, with the ids the index holds (external ids
+     * under an IDMap). */
+    int (*is_member)(const void* ctx, nsparse_idx_t id);
+    const void* ctx;
 } nsparse_selector;
 
 typedef enum nsparse_params_kind {

@@ -62,6 +62,8 @@ public:
             inner_ = std::make_unique<SelectorEval>(s->inner);
         } else if (s->kind == NSPARSE_SELECTOR_BITMAP) {
             if (s->n < 0 || (s->n > 0 && s->bitmap == nullptr)) throw std::invalid_argument("selector bitmap is null");
+        } else if (s->kind == NSPARSE_SELECTOR_CALLBACK) {
+            if (s->is_member == nullptr) throw std::invalid_argument("selector callback is null");
         } else {
             throw std::invalid_argument("unknown selector kind");
         }
@@ -72,6 +74,11 @@ public:
             case NSPARSE_SELECTOR_ARRAY:
             case NSPARSE_SELECTOR_SET: return set_.count(static_cast<idx_t>(id)) != 0;
             case NSPARSE_SELECTOR_NOT: return !inner_->is_member(id);
+            case NSPARSE_SELECTOR_CALLBACK: {
+                const int r = s_->is_member(s_->ctx, static_cast<idx_t>(id));
+                if (r < 0) throw std::runtime_error("IDSelector::is_member failed");
+                return r != 0;
+            }
             default: return id >= 0 && id < s_->n && ((s_->bitmap[id >> 5] >> (id & 31)) & 1U);
         }
     }

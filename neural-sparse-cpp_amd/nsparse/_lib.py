@@ -42,6 +42,8 @@ Selector._fields_ = [
     ("ids", P(ctypes.c_int32)),
     ("bitmap", P(ctypes.c_uint32)),
     ("inner", P(Selector)),
+    ("is_member", c_void_p),
+    ("ctx", c_void_p),
 ]
 
 
