@@ -47,24 +47,42 @@ using term_t = uint16_t;
 constexpr idx_t kInvalid = -1;
 
 // ---------------------------------------------------------------- quantizer
-// utils/scalar_quantizer.h:65-77, 97-101, 118-126 (8-bit only).
+// utils/scalar_quantizer.h:32-44, 52-93, 97-126 (8-bit and 16-bit codes).
 struct Quantizer {
     float vmin = 0.0F, vmax = 1.0F;
+    int bits = 8;  // QuantizerType: QT_8bit (8) or QT_16bit (16)
     Quantizer() = default;
-    Quantizer(float lo, float hi) : vmin(lo), vmax(hi) {
+    Quantizer(float lo, float hi, int b = 8) : vmin(lo), vmax(hi), bits(b) {
         if (!std::isfinite(lo) || !std::isfinite(hi)) throw std::invalid_argument("vmin and vmax must be finite");
         if (hi <= lo) throw std::invalid_argument("vmax must be greater than vmin");
     }
-    uint8_t encode(float val) const {
-        constexpr float kMax = 255.0F;
+    // kMax8bit / kMax16bit: numeric_limits<uint8_t / uint16_t>::max() as float
+    float kmax() const { return bits == 16 ? 65535.0F : 255.0F; }
+    size_t bytes() const { return bits == 16 ? 2 : 1; }
+    // encode_8bit / encode_16bit (:97-101, :109-113)
+    uint32_t encode(float val) const {
+        const float kMax = kmax();
         float scaled = (val - vmin) * (kMax / (vmax - vmin));
         scaled = std::max(0.0F, std::min(kMax, scaled));
-        return static_cast<uint8_t>(std::lround(scaled));
+        return static_cast<uint32_t>(std::lround(scaled));
     }
+    // codes of n values, bytes() each, little-endian
+    void encode_n(const float* v, size_t n, uint8_t* out) const {
+        for (size_t i = 0; i < n; ++i) {
+            const uint32_t c = encode(v[i]);
+            if (bits == 16) {
+                const uint16_t c16 = static_cast<uint16_t>(c);
+                std::memcpy(out + 2 * i, &c16, 2);
+            } else {
+                out[i] = static_cast<uint8_t>(c);
+            }
+        }
+    }
+    // decode_dot_product (:85-93): the ingest quantizer's width sets max_q
     float decode_dot_product(float s, const Quantizer& q) const {
         const float ingest_range = vmax - vmin;
         const float query_range = q.vmax - q.vmin;
-        const float max_q = 255.0F;
+        const float max_q = kmax();
         const float scale = (ingest_range * query_range) / (max_q * max_q);
         return s * scale;
     }
@@ -225,6 +243,11 @@ float value_as_float(const uint8_t* p, size_t esz) {
         float f;
         std::memcpy(&f, p, 4);
         return f;
+    }
+    if (esz == 2) {
+        uint16_t c;
+        std::memcpy(&c, p, 2);
+        return static_cast<float>(c);
     }
     return static_cast<float>(*p);
 }
@@ -465,6 +488,8 @@ std::vector<ListClusters> build_lists(const Fwd& f, int lambda_p, int beta_p, fl
         const uint32_t s = base_seed + static_cast<uint32_t>(idx);
         if (f.esz == 4)
             out[idx] = build_one_list<float, float>(f, kept, beta, s, alpha, tie_mode);
+        else if (f.esz == 2)
+            out[idx] = build_one_list<uint16_t, int64_t>(f, kept, beta, s, alpha, tie_mode);
         else
             out[idx] = build_one_list<uint8_t, int64_t>(f, kept, beta, s, alpha, tie_mode);
     }
@@ -700,6 +725,12 @@ void single_query(const Index& ix, const std::vector<VT>& dense, const term_t* q
                         float r = 0.0F;
                         for (idx_t j = s0; j < s1; ++j) r += w[j] * dense[f.indices[j]];
                         score = r;
+                    } else if constexpr (std::is_same_v<VT, uint16_t>) {
+                        // dot_product_uint16_dense (utils/distance.h:49-57): int64 sum
+                        const uint16_t* w = reinterpret_cast<const uint16_t*>(f.values.data());
+                        int64_t r = 0;
+                        for (idx_t j = s0; j < s1; ++j) r += static_cast<int64_t>(w[j]) * dense[f.indices[j]];
+                        score = static_cast<float>(r);
                     } else {
                         const size_t n = static_cast<size_t>(s1 - s0);
                         const int r = avx512 ? dot_u8_avx512(f.indices.data() + s0, f.values.data() + s0, n,
@@ -730,6 +761,12 @@ void exact_query(const Index& ix, const Selector& sel, const std::vector<uint8_t
             float r = 0.0F;
             for (idx_t j = f.indptr[d]; j < f.indptr[d + 1]; ++j) r += w[j] * dn[f.indices[j]];
             score = r;
+        } else if (f.esz == 2) {
+            const uint16_t* w = reinterpret_cast<const uint16_t*>(f.values.data());
+            const uint16_t* dn = reinterpret_cast<const uint16_t*>(qbytes_dense.data());
+            int64_t r = 0;
+            for (idx_t j = f.indptr[d]; j < f.indptr[d + 1]; ++j) r += static_cast<int64_t>(w[j]) * dn[f.indices[j]];
+            score = static_cast<float>(r);
         } else {
             int r = 0;
             for (idx_t j = f.indptr[d]; j < f.indptr[d + 1]; ++j) r += f.values[j] * qbytes_dense[f.indices[j]];
@@ -744,10 +781,15 @@ void search_impl(const Index& ix, int n, const idx_t* ip, const term_t* qi, cons
                  const Quantizer& qq, const Selector* sel, float* D, idx_t* L, Stats* stats, int n_threads) {
     const size_t nnz = ip[n];
     const bool sq = ix.sq;
+    const bool w16 = sq && ix.q.bits == 16;
     std::vector<uint8_t> codes;
-    if (sq) {
+    std::vector<uint16_t> codes16;
+    if (w16) {
+        codes16.resize(nnz);
+        for (size_t i = 0; i < nnz; ++i) codes16[i] = static_cast<uint16_t>(qq.encode(qv[i]));
+    } else if (sq) {
         codes.resize(nnz);
-        for (size_t i = 0; i < nnz; ++i) codes[i] = qq.encode(qv[i]);
+        for (size_t i = 0; i < nnz; ++i) codes[i] = static_cast<uint8_t>(qq.encode(qv[i]));
     }
     for (size_t i = 0; i < nnz; ++i)
         if (qi[i] >= ix.fwd.dim) throw std::invalid_argument("query term id out of range");
@@ -757,7 +799,9 @@ void search_impl(const Index& ix, int n, const idx_t* ip, const term_t* qi, cons
         for (int qidx = 0; qidx < n; ++qidx) {
             std::vector<uint8_t> dense(ix.fwd.dim * esz, 0);
             for (idx_t j = ip[qidx]; j < ip[qidx + 1]; ++j) {
-                if (sq)
+                if (w16)
+                    std::memcpy(dense.data() + static_cast<size_t>(qi[j]) * 2, &codes16[j], 2);
+                else if (sq)
                     dense[qi[j]] = codes[j];
                 else
                     std::memcpy(dense.data() + static_cast<size_t>(qi[j]) * 4, qv + j, 4);
@@ -786,8 +830,11 @@ void search_impl(const Index& ix, int n, const idx_t* ip, const term_t* qi, cons
     {
         std::vector<float> densef;
         std::vector<uint8_t> denseq;
+        std::vector<uint16_t> dense16;
         DocSet visited(static_cast<size_t>(std::max(k, 1)) * 4096);
-        if (sq)
+        if (w16)
+            dense16.assign(ix.fwd.dim, 0);
+        else if (sq)
             denseq.assign(ix.fwd.dim + 3, 0);  // + the 4-byte gathers' padding
         else
             densef.assign(ix.fwd.dim, 0.0F);
@@ -797,7 +844,13 @@ void search_impl(const Index& ix, int n, const idx_t* ip, const term_t* qi, cons
             const size_t len = ip[qidx + 1] - s0;
             Stats st{};
             TopK heap(k);
-            if (sq) {
+            if (w16) {
+                const uint16_t* qc = codes16.data() + s0;
+                auto cuts = top_k_tokens<uint16_t>(qi + s0, qc, static_cast<int>(len), cut);
+                for (size_t i = 0; i < len; ++i) dense16[qi[s0 + i]] = qc[i];
+                single_query<TopK, uint16_t>(ix, dense16, qi + s0, qc, len, cuts, hf, sel, heap, st, visited);
+                for (size_t i = 0; i < len; ++i) dense16[qi[s0 + i]] = 0;
+            } else if (sq) {
                 const uint8_t* qc = codes.data() + s0;
                 auto cuts = top_k_tokens<uint8_t>(qi + s0, qc, static_cast<int>(len), cut);
                 for (size_t i = 0; i < len; ++i) denseq[qi[s0 + i]] = qc[i];
@@ -848,7 +901,7 @@ void write_index(const Index& ix, Writer& w) {
     w.scalar<uint32_t>(fcc);
     w.scalar<int>(static_cast<int>(ix.fwd.dim));
     if (ix.sq) {
-        w.scalar<uint8_t>(0);  // QuantizerType::QT_8bit
+        w.scalar<uint8_t>(ix.q.bits == 16 ? 1 : 0);  // QuantizerType: QT_8bit 0, QT_16bit 1
         w.scalar<float>(ix.q.vmin);
         w.scalar<float>(ix.q.vmax);
     }
@@ -1117,19 +1170,19 @@ int oracle_set_simd(int on) {
     return __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512bw") ? 1 : 0;
 }
 
-// kind: 0 = seismic (FP32), 1 = seismic_sq (8-bit).
+// sq: 0 = seismic (FP32), 1 = seismic_sq 8-bit, 2 = seismic_sq 16-bit.
 int oracle_create(int dim, int sq, float vmin, float vmax, int lambda, int beta, float alpha, int seed, int tie_mode,
                   void** out) {
     return guard([&] {
         auto* ix = new Index;
         ix->sq = sq != 0;
-        if (ix->sq) ix->q = Quantizer(vmin, vmax);
+        if (ix->sq) ix->q = Quantizer(vmin, vmax, sq == 2 ? 16 : 8);
         ix->lambda = lambda;
         ix->beta = beta;
         ix->alpha = alpha;
         ix->seed = seed;
         ix->tie_mode = tie_mode;
-        ix->fwd.esz = ix->sq ? 1 : 4;
+        ix->fwd.esz = ix->sq ? ix->q.bytes() : 4;
         ix->fwd.dim = static_cast<size_t>(dim);
         *out = ix;
     });
@@ -1143,8 +1196,8 @@ int oracle_add(void* h, int n, const idx_t* indptr, const term_t* indices, const
         if (n <= 0) throw std::invalid_argument("value must be positive");
         const size_t nnz = indptr[n];
         if (ix->sq) {
-            std::vector<uint8_t> codes(nnz);
-            for (size_t i = 0; i < nnz; ++i) codes[i] = ix->q.encode(values[i]);
+            std::vector<uint8_t> codes(nnz * ix->q.bytes());
+            ix->q.encode_n(values, nnz, codes.data());
             ix->fwd.add(indptr, n + 1, indices, nnz, codes.data());
         } else {
             ix->fwd.add(indptr, n + 1, indices, nnz, reinterpret_cast<const uint8_t*>(values));
@@ -1170,7 +1223,7 @@ int oracle_search(void* h, int n, const idx_t* indptr, const term_t* indices, co
         if (n <= 0 || k <= 0) throw std::invalid_argument("value must be positive");
         if (!ix->has_vectors) return;
         Quantizer qq = ix->q;
-        if (ix->sq && qsq) qq = Quantizer(qvmin, qvmax);
+        if (ix->sq && qsq) qq = Quantizer(qvmin, qvmax, ix->q.bits);
         Selector inner;
         Selector sel = make_selector(sel_kind, sel_n, sel_ids, sel_bitmap, sel_negate, inner);
         const Selector* sp = sel.kind < 0 ? nullptr : &sel;
@@ -1196,7 +1249,7 @@ int oracle_search_idmap(void* h, const idx_t* i2e, int64_t n_map, int n, const i
         if (n <= 0 || k <= 0) throw std::invalid_argument("value must be positive");
         if (!ix->has_vectors) return;
         Quantizer qq = ix->q;
-        if (ix->sq && qsq) qq = Quantizer(qvmin, qvmax);
+        if (ix->sq && qsq) qq = Quantizer(qvmin, qvmax, ix->q.bits);
         const std::vector<idx_t> map(i2e, i2e + n_map);
         std::unordered_map<idx_t, idx_t> rev;
         for (int64_t i = 0; i < n_map; ++i) rev[map[i]] = static_cast<idx_t>(i);
@@ -1239,9 +1292,13 @@ int oracle_score_summaries(void* h, int nq, const idx_t* q_indptr, const term_t*
             const size_t len = static_cast<size_t>(q_indptr[qi + 1] - s0);
             const ListClusters& lc = ix->lists.at(static_cast<size_t>(lists[qi]));
             Stats st{};
-            if (ix->sq) {
+            if (ix->sq && ix->q.bits == 16) {
+                std::vector<uint16_t> codes(len);
+                for (size_t i = 0; i < len; ++i) codes[i] = static_cast<uint16_t>(ix->q.encode(q_values[s0 + i]));
+                score_summaries<uint16_t>(lc, q_indices + s0, codes.data(), len, scores, st);
+            } else if (ix->sq) {
                 std::vector<uint8_t> codes(len);
-                for (size_t i = 0; i < len; ++i) codes[i] = ix->q.encode(q_values[s0 + i]);
+                for (size_t i = 0; i < len; ++i) codes[i] = static_cast<uint8_t>(ix->q.encode(q_values[s0 + i]));
                 score_summaries<uint8_t>(lc, q_indices + s0, codes.data(), len, scores, st);
             } else {
                 score_summaries<float>(lc, q_indices + s0, q_values + s0, len, scores, st);
@@ -1270,6 +1327,7 @@ int oracle_set_lists(void* h, int n_lists, const int32_t* nc, const int32_t* cdo
             std::vector<std::vector<term_t>> st;
             std::vector<std::vector<float>> svf;
             std::vector<std::vector<uint8_t>> svq;
+            std::vector<std::vector<uint16_t>> svw;
             if (nc[l] > 0) lc.offsets.push_back(0);
             for (int j = 0; j < nc[l]; ++j, ++c) {
                 lc.docs.insert(lc.docs.end(), docs + d, docs + d + cdocs[c]);
@@ -1278,10 +1336,16 @@ int oracle_set_lists(void* h, int n_lists, const int32_t* nc, const int32_t* cdo
                 st.emplace_back(sum_terms + t, sum_terms + t + cterms[c]);
                 svf.emplace_back(sum_vals + t, sum_vals + t + cterms[c]);
                 svq.emplace_back();
-                for (int e = 0; e < cterms[c]; ++e) svq.back().push_back(static_cast<uint8_t>(sum_vals[t + e]));
+                svw.emplace_back();
+                for (int e = 0; e < cterms[c]; ++e) {
+                    svq.back().push_back(static_cast<uint8_t>(sum_vals[t + e]));
+                    svw.back().push_back(static_cast<uint16_t>(sum_vals[t + e]));
+                }
                 t += static_cast<size_t>(cterms[c]);
             }
-            if (ix->sq)
+            if (ix->sq && ix->q.bits == 16)
+                build_transpose<uint16_t>(lc, st, svw);
+            else if (ix->sq)
                 build_transpose<uint8_t>(lc, st, svq);
             else
                 build_transpose<float>(lc, st, svf);
@@ -1323,9 +1387,10 @@ int oracle_load(const uint8_t* buf, size_t size, int tie_mode, void** out) {
             scalar(qt);
             scalar(lo);
             scalar(hi);
-            ix->q = Quantizer(lo, hi);
+            if (qt > 1) throw std::runtime_error("index file declares an unknown quantizer type");
+            ix->q = Quantizer(lo, hi, qt == 1 ? 16 : 8);
         }
-        ix->fwd.esz = ix->sq ? 1 : 4;
+        ix->fwd.esz = ix->sq ? ix->q.bytes() : 4;
         ix->fwd.dim = static_cast<size_t>(dim);
         size_t count = 0;
         scalar(count);
@@ -1415,14 +1480,20 @@ int oracle_cluster_order(const float* scores, int n, int64_t* out) {
 }
 
 int oracle_sq_encode(float vmin, float vmax, const float* v, size_t n, uint8_t* out) {
-    return guard([&] {
-        Quantizer q(vmin, vmax);
-        for (size_t i = 0; i < n; ++i) out[i] = q.encode(v[i]);
-    });
+    return guard([&] { Quantizer(vmin, vmax).encode_n(v, n, out); });
+}
+
+// 16-bit codes (QT_16bit), n uint16 values into out.
+int oracle_sq_encode16(float vmin, float vmax, const float* v, size_t n, uint16_t* out) {
+    return guard([&] { Quantizer(vmin, vmax, 16).encode_n(v, n, reinterpret_cast<uint8_t*>(out)); });
 }
 
 float oracle_sq_decode(float ivmin, float ivmax, float qvmin, float qvmax, float s) {
     return Quantizer(ivmin, ivmax).decode_dot_product(s, Quantizer(qvmin, qvmax));
+}
+
+float oracle_sq_decode16(float ivmin, float ivmax, float qvmin, float qvmax, float s) {
+    return Quantizer(ivmin, ivmax, 16).decode_dot_product(s, Quantizer(qvmin, qvmax, 16));
 }
 
 int oracle_select_centroids(int n_docs, int n_clusters, uint32_t seed, int32_t* out_positions) {

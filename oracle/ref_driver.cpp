@@ -51,14 +51,16 @@ struct RefBuilt {
 std::unique_ptr<RefBuilt> ref_build(int dim, int sq, float vmin, float vmax, int n, const int32_t* indptr,
                                     const uint16_t* indices, const float* values, int lambda, int beta, float alpha,
                                     uint32_t base_seed) {
-    const size_t esz = sq ? 1 : 4;
+    const size_t esz = sq == 2 ? 2 : sq ? 1 : 4;
     const size_t nnz = indptr[n];
     auto b = std::unique_ptr<RefBuilt>(new RefBuilt{
         SparseVectors({.element_size = esz, .dimension = static_cast<size_t>(dim)}),
-        ScalarQuantizer(QuantizerType::QT_8bit, sq ? vmin : 0.0F, sq ? vmax : 1.0F), {}});
+        ScalarQuantizer(sq == 2 ? QuantizerType::QT_16bit : QuantizerType::QT_8bit, sq ? vmin : 0.0F,
+                        sq ? vmax : 1.0F),
+        {}});
     std::vector<uint8_t> codes;
     if (sq) {
-        codes.resize(nnz);
+        codes.resize(nnz * esz);
         b->q.encode(values, codes.data(), nnz);
     } else {
         codes.resize(nnz * 4);
@@ -85,7 +87,7 @@ extern "C" {
 
 const char* ref_last_error() { return g_err.c_str(); }
 
-// Builds a SEIS (sq == 0) or SESQ (sq == 1, 8-bit) index with final lambda and
+// Builds a SEIS (sq == 0) or SESQ (sq == 1: 8-bit, sq == 2: 16-bit) index with final lambda and
 // beta (already resolved) and writes its full write_index byte stream.
 int ref_build_serialize(int dim, int sq, float vmin, float vmax, int n, const int32_t* indptr,
                         const uint16_t* indices, const float* values, int lambda, int beta, float alpha,
@@ -139,7 +141,7 @@ int ref_score_summaries(int dim, int sq, float vmin, float vmax, int n, const in
             const size_t len = static_cast<size_t>(q_indptr[qi + 1] - s0);
             std::vector<uint8_t> qbytes;
             if (sq) {
-                qbytes.resize(len);
+                qbytes.resize(len * (sq == 2 ? 2 : 1));
                 b->q.encode(q_values + s0, qbytes.data(), len);
             } else {
                 qbytes.resize(len * 4);
@@ -218,13 +220,25 @@ float ref_sq_decode(float ivmin, float ivmax, float qvmin, float qvmax, float s)
     ScalarQuantizer a(QuantizerType::QT_8bit, ivmin, ivmax), b(QuantizerType::QT_8bit, qvmin, qvmax);
     return a.decode_dot_product(s, b);
 }
+int ref_sq_encode16(float vmin, float vmax, const float* v, size_t n, uint16_t* out) {
+    return guard([&] {
+        ScalarQuantizer(QuantizerType::QT_16bit, vmin, vmax).encode(v, reinterpret_cast<uint8_t*>(out), n);
+    });
+}
+float ref_sq_decode16(float ivmin, float ivmax, float qvmin, float qvmax, float s) {
+    ScalarQuantizer a(QuantizerType::QT_16bit, ivmin, ivmax), b(QuantizerType::QT_16bit, qvmin, qvmax);
+    return a.decode_dot_product(s, b);
+}
 
-// detail::dot_product_{uint8,float}_dense (utils/distance.h:29-47), generic build.
+// detail::dot_product_{uint8,uint16,float}_dense (utils/distance.h:29-57), generic build.
 float ref_dot_u8(const uint16_t* idx, const uint8_t* w, size_t len, const uint8_t* dense) {
     return detail::dot_product_uint8_dense(idx, w, len, dense);
 }
 float ref_dot_f32(const uint16_t* idx, const float* w, size_t len, const float* dense) {
     return detail::dot_product_float_dense(idx, w, len, dense);
+}
+float ref_dot_u16(const uint16_t* idx, const uint16_t* w, size_t len, const uint16_t* dense) {
+    return detail::dot_product_uint16_dense(idx, w, len, dense);
 }
 
 }  // extern "C"

@@ -26,10 +26,15 @@ lib.oracle_sq_decode.restype = ctypes.c_float
 lib.oracle_sq_decode.argtypes = [f32, f32, f32, f32, f32]
 lib.oracle_mt19937_nth.restype = ctypes.c_uint32
 lib.oracle_mt19937_nth.argtypes = [ctypes.c_uint32, ctypes.c_int]
+lib.oracle_sq_decode16.restype = ctypes.c_float
+lib.oracle_sq_decode16.argtypes = [f32, f32, f32, f32, f32]
 ref = ctypes.CDLL(REF_SO) if os.path.exists(REF_SO) else None
 if ref is not None:
     ref.ref_sq_decode.restype = ctypes.c_float
     ref.ref_sq_decode.argtypes = [f32, f32, f32, f32, f32]
+    ref.ref_sq_decode16.restype = ctypes.c_float
+    ref.ref_sq_decode16.argtypes = [f32, f32, f32, f32, f32]
+    ref.ref_dot_u16.restype = ctypes.c_float
     ref.ref_dot_u8.restype = ctypes.c_float
     ref.ref_dot_f32.restype = ctypes.c_float
 
@@ -47,8 +52,9 @@ def _chk(rc):
 
 
 class OracleIndex:
-    """The CPU restatement. tie_mode 0 = libstdc++ ties (the reference's),
-    1 = exact float ties broken by id (what the device path implements)."""
+    """The CPU restatement. sq: 0 FP32, 1 8-bit codes, 2 16-bit codes.
+    tie_mode 0 = libstdc++ ties (the reference's), 1 = exact float ties broken
+    by id (what the device path implements)."""
 
     def __init__(self, dim, sq, lam, beta, alpha, seed, vmin=0.0, vmax=1.0, tie_mode=1):
         self.h = P()

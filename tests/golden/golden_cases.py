@@ -30,7 +30,7 @@ def _resolve(lam, beta, n):
 
 
 BUILD_CASES = []
-for _sq in (0, 1):
+for _sq in (0, 1, 2):  # FP32, 8-bit codes, 16-bit codes
     for _n, _dim, _nnz, _lam, _beta in [(2000, 1024, 30, 100, 10), (3000, 4096, 60, 300, 30), (500, 256, 20, -1, -1),
                                         (400, 300, 12, 0, 0)]:
         _lr, _br = _resolve(_lam, _beta, _n)

@@ -10,7 +10,8 @@ The fixture holds data only:
     the reference from seeded generator corpora (the inputs are regenerated
     from the seeds by the test), for the cases of test_oracle.py;
   * the same for the posting lists + per-term maxima of InvertedIndex::build;
-  * 8-bit SQ codes of fixed values under three ranges, and decoded scores;
+  * 8-bit and 16-bit SQ codes of fixed values under three ranges, and decoded
+    scores;
   * per build case, the summary scores InvertedListClusters::
     score_summaries_transposed gives seeded queries against their lists
     (golden_cases.summary_pairs): SHA-256 of every pair's cluster count
@@ -40,7 +41,8 @@ def digest(b):
 def main():
     if O.ref is None:
         raise SystemExit("oracle/_ref/libnsparse_ref.so is not built here")
-    out = {"build": [], "inverted": [], "sq_encode": [], "sq_decode": [], "summaries": []}
+    out = {"build": [], "inverted": [], "sq_encode": [], "sq_decode": [], "summaries": [], "sq_encode16": [],
+           "sq_decode16": []}
     for case in BUILD_CASES:
         ip, ix, v = build_corpus(case)
         vmin, vmax = float(v.min()), float(v.max())
@@ -70,6 +72,17 @@ def main():
         d = O.ref.ref_sq_decode(0.01, 43.9, 0.02, 20.0, s)
         out["sq_decode"].append({"ingest": [0.01, 43.9], "query": [0.02, 20.0], "score": s,
                                  "decoded_bits": int(np.float32(d).view(np.uint32))})
+    for lo, hi in SQ_RANGES:
+        codes = np.empty(len(vals), np.uint16)
+        O.ref.ref_sq_encode16(ctypes.c_float(lo), ctypes.c_float(hi), vals.ctypes.data_as(O.P),
+                              ctypes.c_size_t(len(vals)), codes.ctypes.data_as(O.P))
+        out["sq_encode16"].append({"vmin": lo, "vmax": hi,
+                                   "codes_sha256": hashlib.sha256(codes.tobytes()).hexdigest(),
+                                   "first_codes": codes[:64].tolist()})
+    for s in [0.0, 1.0, 12345.0, 3.5e6, 4.0e12]:
+        d = O.ref.ref_sq_decode16(0.01, 43.9, 0.02, 20.0, s)
+        out["sq_decode16"].append({"ingest": [0.01, 43.9], "query": [0.02, 20.0], "score": s,
+                                   "decoded_bits": int(np.float32(d).view(np.uint32))})
     with open(os.path.join(HERE, "ref_fixtures.json"), "w") as f:
         json.dump(out, f, indent=1)
     print(f"wrote {len(out['build'])} build, {len(out['inverted'])} inverted digests")

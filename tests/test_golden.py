@@ -5,7 +5,7 @@ so the oracle stays pinned where the reference tree is absent.
     tie mode for FP32, whose corpora here have no value ties that reach the
     layout -- the oracle's bytes hash to the reference's;
   * InvertedIndex lists + maxima, byte for byte;
-  * 8-bit SQ codes and decoded scores;
+  * 8-bit and 16-bit SQ codes and decoded scores;
   * summary scores (score_summaries_transposed) of seeded query / list pairs."""
 import ctypes
 import hashlib
@@ -83,6 +83,19 @@ def test_sq_codes_and_decode_match_reference():
         assert hashlib.sha256(codes.tobytes()).hexdigest() == fx["codes_sha256"]
     for fx in FIX["sq_decode"]:
         d = O.lib.oracle_sq_decode(*fx["ingest"], *fx["query"], fx["score"])
+        assert int(np.float32(d).view(np.uint32)) == fx["decoded_bits"]
+
+
+def test_sq16_codes_and_decode_match_reference():
+    vals = sq_values()
+    for (lo, hi), fx in zip(SQ_RANGES, FIX["sq_encode16"]):
+        codes = np.empty(len(vals), np.uint16)
+        O.lib.oracle_sq_encode16(ctypes.c_float(lo), ctypes.c_float(hi), vals.ctypes.data_as(O.P),
+                                 ctypes.c_size_t(len(vals)), codes.ctypes.data_as(O.P))
+        assert codes[:64].tolist() == fx["first_codes"]
+        assert hashlib.sha256(codes.tobytes()).hexdigest() == fx["codes_sha256"]
+    for fx in FIX["sq_decode16"]:
+        d = O.lib.oracle_sq_decode16(*fx["ingest"], *fx["query"], fx["score"])
         assert int(np.float32(d).view(np.uint32)) == fx["decoded_bits"]
 
 

@@ -6,8 +6,9 @@ What pins each part of the oracle:
     the reference's own build-side sources compiled into oracle/_ref
     (test_build_bytes_match_reference_*), for FP32 and SQ, in the
     libstdc++ tie mode; the id tie mode equals it wherever values do not tie.
-  * SQ encode / decode and the uint8 / FP32 dot products: against the same
-    compiled reference functions, on random and edge inputs.
+  * SQ encode / decode (8-bit and 16-bit) and the uint8 / uint16 / FP32 dot
+    products: against the same compiled reference functions, on random and
+    edge inputs, and the 16-bit codes against their closed form.
   * top_k_tokens: the reference's vector_process_test.cpp known answers.
   * std::mt19937: the standard's 10000th-output value.
   * search: the reference's hand-computed seismic_index_test.cpp cases, an
@@ -48,7 +49,7 @@ needs_ref = pytest.mark.skipif(O.ref is None, reason="reference build-side libra
 
 
 @needs_ref
-@pytest.mark.parametrize("sq", [0, 1])
+@pytest.mark.parametrize("sq", [0, 1, 2])
 @pytest.mark.parametrize("n,dim,nnz,lam,beta", [(2000, 1024, 30, 100, 10), (3000, 4096, 60, 300, 30),
                                                 (500, 256, 20, -1, -1), (400, 300, 12, 0, 0)])
 def test_build_bytes_match_reference(sq, n, dim, nnz, lam, beta):
@@ -93,11 +94,50 @@ def test_sq_encode_decode_match_reference():
 
 
 @needs_ref
+def test_sq16_encode_decode_match_reference():
+    import ctypes
+
+    rng = np.random.default_rng(2)
+    vals = np.concatenate([rng.lognormal(0, 1.5, 20000), [0.0, -1.0, 1e9, 3.0, 0.5, 2.5],
+                           np.float32(np.arange(0, 65536, 7) / 65535.0 * 3.0)]).astype(np.float32)
+    for lo, hi in [(0.0, 3.0), (0.01, 43.9), (-2.0, 5.0)]:
+        a = np.empty(len(vals), np.uint16)
+        b = np.empty(len(vals), np.uint16)
+        O.lib.oracle_sq_encode16(ctypes.c_float(lo), ctypes.c_float(hi), vals.ctypes.data_as(O.P),
+                                 ctypes.c_size_t(len(vals)), a.ctypes.data_as(O.P))
+        O.ref.ref_sq_encode16(ctypes.c_float(lo), ctypes.c_float(hi), vals.ctypes.data_as(O.P),
+                              ctypes.c_size_t(len(vals)), b.ctypes.data_as(O.P))
+        assert (a == b).all()
+    for s in [0.0, 1.0, 12345.0, 3.5e6, 4.0e12]:
+        assert (O.lib.oracle_sq_decode16(0.01, 43.9, 0.02, 20.0, s) ==
+                O.ref.ref_sq_decode16(0.01, 43.9, 0.02, 20.0, s))
+
+
+def test_sq16_codes_closed_form():
+    """encode_16bit (scalar_quantizer.h:109-113): lround of the clamped
+    (v - vmin) * (65535 / (vmax - vmin)) in float32, half away from zero."""
+    import ctypes
+
+    lo, hi = np.float32(0.25), np.float32(2.0)
+    scale = np.float32(65535.0) / (hi - lo)
+    vals = np.concatenate([np.float32(lo) + np.arange(0, 65536, 3, dtype=np.float32) / scale,
+                           [-5.0, 0.25, 2.0, 9.0]]).astype(np.float32)
+    got = np.empty(len(vals), np.uint16)
+    O.lib.oracle_sq_encode16(ctypes.c_float(lo), ctypes.c_float(hi), vals.ctypes.data_as(O.P),
+                             ctypes.c_size_t(len(vals)), got.ctypes.data_as(O.P))
+    x = np.clip((vals - lo).astype(np.float32) * scale, np.float32(0), np.float32(65535)).astype(np.float64)
+    want = np.floor(x + 0.5).astype(np.uint16)  # x >= 0: half away from zero
+    assert (got == want).all()
+    assert got[-4:].tolist() == [0, 0, 65535, 65535]
+
+
+@needs_ref
 def test_dot_products_match_reference():
     import ctypes
 
     rng = np.random.default_rng(1)
     dense_u8 = rng.integers(0, 30, 2000).astype(np.uint8)
+    dense_u16 = rng.integers(0, 65536, 2000).astype(np.uint16)
     dense_f = rng.lognormal(0, 0.7, 2000).astype(np.float32)
     for _ in range(50):
         m = int(rng.integers(1, 300))
@@ -107,6 +147,10 @@ def test_dot_products_match_reference():
         got8 = O.ref.ref_dot_u8(idx.ctypes.data_as(O.P), w8.ctypes.data_as(O.P), ctypes.c_size_t(m),
                                 dense_u8.ctypes.data_as(O.P))
         assert got8 == float(np.sum(w8.astype(np.int64) * dense_u8[idx]))
+        w16 = rng.integers(0, 65536, m).astype(np.uint16)
+        got16 = O.ref.ref_dot_u16(idx.ctypes.data_as(O.P), w16.ctypes.data_as(O.P), ctypes.c_size_t(m),
+                                  dense_u16.ctypes.data_as(O.P))
+        assert got16 == np.float32(np.sum(w16.astype(np.int64) * dense_u16[idx].astype(np.int64)))
         gotf = O.ref.ref_dot_f32(idx.ctypes.data_as(O.P), wf.ctypes.data_as(O.P), ctypes.c_size_t(m),
                                  dense_f.ctypes.data_as(O.P))
         acc = np.float32(0)
@@ -180,6 +224,46 @@ def test_selector_filters(tie_mode):
     assert list(lab[0]) == [2, 0, -1]
 
 
+@pytest.mark.parametrize("tie_mode", [0, 1])
+def test_sq16_search_finds_matching_doc(tie_mode):
+    # seismic_scalar_quantized_index_test.cpp:291-311 (QT_16bit, range [0, 1])
+    o = OracleIndex(10, 2, 10, 2, 0.5, 3, 0.0, 1.0, tie_mode=tie_mode)
+    add_docs(o, [{0: 1.0, 1: 0.5}])
+    o.build()
+    d, lab = o.search([0, 1], [0], [1.0], 1, cut=5, hf=1.0)
+    assert lab[0][0] == 0
+    # codes 65535 (doc) x 65535 (query), decoded with (1 * 1) / 65535^2
+    assert d[0][0] == np.float32(np.float32(65535 * 65535) * np.float32(1.0 / (65535.0 * 65535.0)))
+
+
+def test_sq16_search_equals_brute_force_over_codes():
+    """heap_factor huge and lambda above every list: the 16-bit search scores
+    every doc sharing a cut term, each as the int64 sum of code products turned
+    to float once (dot_product_uint16_dense), decoded by the 16-bit scale."""
+    ip, ix, v = corpus(600, 200, 12, seed=21)
+    vmax = np.float32(v.max())
+    o = OracleIndex(200, 2, 100000, 8, 0.4, 3, 0.0, vmax, tie_mode=1)
+    o.add(ip, ix, v)
+    o.build()
+    qip, qix, qv = corpus(30, 200, 6, seed=22)
+    d, lab = o.search(qip, qix, qv, 10, cut=1000, hf=1e9)
+
+    def codes(x):
+        s = np.clip((x - np.float32(0)) * (np.float32(65535) / (vmax - np.float32(0))), 0, 65535)
+        return np.floor(s.astype(np.float64) + 0.5).astype(np.int64)
+
+    dc, qc = codes(v), codes(qv)
+    scale = np.float32((vmax * vmax) / np.float32(65535.0 * 65535.0))
+    for q in range(30):
+        dense = np.zeros(200, np.int64)
+        dense[qix[qip[q]:qip[q + 1]]] = qc[qip[q]:qip[q + 1]]
+        sc = np.array([np.float32(np.sum(dc[ip[i]:ip[i + 1]] * dense[ix[ip[i]:ip[i + 1]]])) for i in range(600)])
+        touched = np.array([np.any(dense[ix[ip[i]:ip[i + 1]]] > 0) for i in range(600)])
+        order = sorted(np.flatnonzero(touched), key=lambda i: (-sc[i], i))[:10]
+        assert list(lab[q][: len(order)]) == order
+        assert [np.float32(sc[i] * scale) for i in order] == list(d[q][: len(order)])
+
+
 def test_simd_and_generic_u8_dot_agree():
     """The oracle's AVX-512 uint8 dot product (taken when the CPU has it) and
     the generic loop give identical searches: integer sums, any order."""
@@ -215,7 +299,7 @@ def test_no_pruning_equals_brute_force_over_visited_lists():
         assert list(keep[: len(want)]) == list(want)
 
 
-@pytest.mark.parametrize("sq", [0, 1])
+@pytest.mark.parametrize("sq", [0, 1, 2])
 def test_recall_floor(sq):
     # python_tests/conftest.py:32-38 corpus shape, floors 0.80 / 0.82
     rng = np.random.default_rng(0xC0FFEE)
