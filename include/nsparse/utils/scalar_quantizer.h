@@ -1,6 +1,5 @@
-// nsparse/utils/scalar_quantizer.h:21-24: the quantizer widths. The device
-// path implements QT_8bit; an index asked for QT_16bit is refused with
-// std::runtime_error when it is made.
+// nsparse/utils/scalar_quantizer.h:21-24: the quantizer widths (8-bit and
+// 16-bit codes).
 #ifndef NSPARSE_DROPIN_SCALAR_QUANTIZER_H
 #define NSPARSE_DROPIN_SCALAR_QUANTIZER_H
 

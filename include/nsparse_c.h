@@ -45,8 +45,8 @@ const char* nsparse_last_error(void);
  * "seismic_sq,quantizer=8bit|vmin=..|vmax=..|lambda=..|beta=..|alpha=..|seed=..",
  * "brutal" (BrutalIndex, nsparse/brutal_index.cpp), "inverted" (InvertedIndex,
  * nsparse/inverted_index.cpp) and the "idmap,<delegate>" prefix. Unknown keys
- * are ignored, as there. "quantizer=16bit" is rejected with NSPARSE_ERR_RUNTIME
- * (8-bit only on this device path). */
+ * are ignored, as there; "quantizer=16bit" selects 16-bit codes, any other
+ * value 8-bit. */
 int nsparse_index_factory(int dim, const char* description, nsparse_index** out);
 
 /* delete index (SWIG %newobject ownership, nsparse/python/swignsparse.swig:182-184). */

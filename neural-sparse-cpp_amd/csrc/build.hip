@@ -85,17 +85,19 @@ void sync(const char* what) {
 
 // ------------------------------------------------------------------ kernels
 
-// ScalarQuantizer::encode_8bit (utils/scalar_quantizer.h:97-101): the scale is
-// computed on the host as `255.0f / (vmax - vmin)`, exactly as the reference
-// evaluates it; the device applies (v - vmin) * scale, clamps with
-// std::min/std::max semantics and rounds half away from zero (lround).
-__global__ void sq_encode_kernel(const float* __restrict__ v, uint8_t* __restrict__ out, size_t n, float vmin,
-                                 float scale) {
+// ScalarQuantizer::encode_8bit / encode_16bit (utils/scalar_quantizer.h:97-113):
+// the scale is computed on the host as `kMax / (vmax - vmin)` (kMax 255 or
+// 65535), exactly as the reference evaluates it; the device applies
+// (v - vmin) * scale, clamps with std::min/std::max semantics and rounds half
+// away from zero (lround).
+template <class T>
+__global__ void sq_encode_kernel(const float* __restrict__ v, T* __restrict__ out, size_t n, float vmin, float scale) {
+    constexpr float kMax = sizeof(T) == 1 ? 255.0F : 65535.0F;
     GRID_STRIDE(i, n) {
         float s = (v[i] - vmin) * scale;
-        s = (s < 255.0F) ? s : 255.0F;
+        s = (s < kMax) ? s : kMax;
         s = (0.0F < s) ? s : 0.0F;
-        out[i] = static_cast<uint8_t>(static_cast<uint32_t>(roundf(s)));
+        out[i] = static_cast<T>(static_cast<uint32_t>(roundf(s)));
     }
 }
 
@@ -123,6 +125,11 @@ __global__ void doc_of_kernel(const int32_t* __restrict__ indptr, int64_t n_docs
 __global__ void prune_key_u8_kernel(const uint16_t* __restrict__ idx, const uint8_t* __restrict__ v, size_t n,
                                     uint32_t* __restrict__ key) {
     GRID_STRIDE(i, n) key[i] = (static_cast<uint32_t>(idx[i]) << 8) | (255u - v[i]);
+}
+
+__global__ void prune_key_u16_kernel(const uint16_t* __restrict__ idx, const uint16_t* __restrict__ v, size_t n,
+                                     uint32_t* __restrict__ key) {
+    GRID_STRIDE(i, n) key[i] = (static_cast<uint32_t>(idx[i]) << 16) | (65535u - v[i]);
 }
 
 __device__ __forceinline__ uint32_t float_desc_bits(float f) {
@@ -374,7 +381,7 @@ __global__ __launch_bounds__(kB) void assign_kernel(
             for (int c = lane; c < nc; c += 64) sims[c] = Acc(0);
             __builtin_amdgcn_wave_barrier();
             const int32_t r0 = indptr[d], r1 = indptr[d + 1];
-            if constexpr (std::is_same_v<T, uint8_t>) {
+            if constexpr (!std::is_same_v<T, float>) {
                 // Integer sims are exact in any order: the doc's (term, centroid
                 // posting) pairs are numbered by a prefix over the terms' posting
                 // counts and dealt to the lanes, kW windows of 64 with their loads
@@ -1273,21 +1280,24 @@ __global__ __launch_bounds__(kB) void summarize_u8_kernel(const int32_t* __restr
     }
 }
 
-// ---- FP32 summaries: the reference's sequential algorithm (one lane walks the
-// cluster's events in order so the running float sum rounds exactly as the
-// CPU's does), an LDS dense accumulator, and a workgroup bitonic sort of the
-// (max desc, term asc) pairs -- in LDS when they fit, else in this
-// workgroup's global scratch (`spill`, next_pow2(dim) entries).
-__global__ __launch_bounds__(kB) void summarize_f32_kernel(const int32_t* __restrict__ indptr,
+// ---- FP32 and 16-bit summaries: the reference's sequential algorithm (one
+// lane walks the cluster's events in order so the running float sum rounds
+// exactly as the CPU's does), an LDS dense accumulator, and a workgroup bitonic
+// sort of the (max desc, term asc) pairs -- in LDS when they fit, else in this
+// workgroup's global scratch (`spill`, next_pow2(dim) entries). 16-bit codes
+// sit in the float accumulator exactly, and the sum adds the same floats as the
+// reference's (float += code, float += integer increase).
+template <class T>
+__global__ __launch_bounds__(kB) void summarize_seq_kernel(const int32_t* __restrict__ indptr,
                                                            const uint16_t* __restrict__ indices,
-                                                           const float* __restrict__ values,
+                                                           const T* __restrict__ values,
                                                            const int64_t* __restrict__ cdp,
                                                            const int32_t* __restrict__ docs,
                                                            const int32_t* __restrict__ cluster_list,
                                                            const int32_t* __restrict__ lcb, int64_t n_clusters, int dim,
                                                            int cap, float alpha, int64_t* __restrict__ count,
                                                            const int64_t* __restrict__ out_off, bool write,
-                                                           uint16_t* __restrict__ out_term, float* __restrict__ out_val,
+                                                           uint16_t* __restrict__ out_term, T* __restrict__ out_val,
                                                            uint16_t* __restrict__ out_cl, KeyItem* __restrict__ spill,
                                                            int spill_stride) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1314,7 +1324,7 @@ __global__ __launch_bounds__(kB) void summarize_f32_kernel(const int32_t* __rest
                 const int32_t d = docs[q];
                 for (int32_t p = indptr[d]; p < indptr[d + 1]; ++p) {
                     const int t = indices[p];
-                    const float v = values[p];
+                    const float v = static_cast<float>(values[p]);
                     if (!((touch[t >> 5] >> (t & 31)) & 1u)) {
                         touch[t >> 5] |= 1u << (t & 31);
                         const float value = (0.0F < v) ? v : 0.0F;  // std::max(0.0f, v)
@@ -1409,7 +1419,7 @@ __global__ __launch_bounds__(kB) void summarize_f32_kernel(const int32_t* __rest
                     bits &= bits - 1;
                     const int t = w * 32 + b;
                     out_term[pos] = static_cast<uint16_t>(t);
-                    out_val[pos] = acc[t];
+                    out_val[pos] = static_cast<T>(acc[t]);
                     out_cl[pos] = local_c;
                     ++pos;
                 }
@@ -1854,25 +1864,33 @@ int calc_beta(int beta, int lambda) {
 
 }  // namespace
 
-void encode_sq_on_device(const float* host_values, size_t n, float vmin, float vmax, uint8_t* host_codes) {
+void encode_sq_on_device(const float* host_values, size_t n, float vmin, float vmax, uint8_t* host_codes,
+                         int bits) {
     if (n == 0) return;
-    const float scale = 255.0F / (vmax - vmin);
+    if (bits != 8 && bits != 16) throw std::invalid_argument("codes are 8 or 16 bits");
+    const size_t cb = bits / 8;
+    const float scale = (bits == 16 ? 65535.0F : 255.0F) / (vmax - vmin);
     constexpr size_t kChunk = size_t{1} << 28;
-    DeviceBuffer dv(std::min(n, kChunk) * 4), dc(std::min(n, kChunk));
+    DeviceBuffer dv(std::min(n, kChunk) * 4), dc(std::min(n, kChunk) * cb);
     for (size_t off = 0; off < n; off += kChunk) {
         const size_t m = std::min(kChunk, n - off);
         hip_check(hipMemcpy(dv.get(), host_values + off, m * 4, hipMemcpyHostToDevice), "hipMemcpy H2D");
-        hipLaunchKernelGGL(sq_encode_kernel, dim3(grid_for(m)), dim3(kB), 0, nullptr, dv.get<float>(),
-                           dc.get<uint8_t>(), m, vmin, scale);
+        if (bits == 16)
+            hipLaunchKernelGGL((sq_encode_kernel<uint16_t>), dim3(grid_for(m)), dim3(kB), 0, nullptr, dv.get<float>(),
+                               dc.get<uint16_t>(), m, vmin, scale);
+        else
+            hipLaunchKernelGGL((sq_encode_kernel<uint8_t>), dim3(grid_for(m)), dim3(kB), 0, nullptr, dv.get<float>(),
+                               dc.get<uint8_t>(), m, vmin, scale);
         hip_check(hipGetLastError(), "sq_encode_kernel launch");
-        hip_check(hipMemcpy(host_codes + off, dc.get(), m, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        hip_check(hipMemcpy(host_codes + off * cb, dc.get(), m * cb, hipMemcpyDeviceToHost), "hipMemcpy D2H");
     }
 }
 
 void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int lambda_p, int beta_p, float alpha,
                      int seed, DeviceLists& out) {
     const int L = fwd.dim;
-    const bool sq = fwd.esz == 1;
+    const bool sq = fwd.esz == 1;   // uint8 codes
+    const bool w16 = fwd.esz == 2;  // uint16 codes: the FP32 flow over integer codes
     const int lambda = calc_lambda(lambda_p, n_docs);
     const int beta = calc_beta(beta_p, lambda);
     const uint32_t base_seed = seed == -1 ? std::random_device{}() : static_cast<uint32_t>(seed);
@@ -1932,6 +1950,12 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
                                static_cast<const uint8_t*>(fwd.values), Z, k_in.get<uint32_t>());
             radix_sort_pairs(k_in.get<uint32_t>(), k_out.get<uint32_t>(), pos_iota.get<int32_t>(),
                              spos2.get<int32_t>(), Z, 24);
+        } else if (w16) {
+            DeviceBuffer k_in(Z * 4), k_out(Z * 4);
+            hipLaunchKernelGGL(prune_key_u16_kernel, dim3(grid_for(Z)), dim3(kB), 0, nullptr, fwd.indices,
+                               static_cast<const uint16_t*>(fwd.values), Z, k_in.get<uint32_t>());
+            radix_sort_pairs(k_in.get<uint32_t>(), k_out.get<uint32_t>(), pos_iota.get<int32_t>(),
+                             spos2.get<int32_t>(), Z, 32);
         } else {
             DeviceBuffer k_in(Z * 8), k_out(Z * 8);
             hipLaunchKernelGGL(prune_key_f32_kernel, dim3(grid_for(Z)), dim3(kB), 0, nullptr, fwd.indices,
@@ -1990,7 +2014,7 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
     }
     const std::vector<int> cbatch = list_batches(list_ce, batch_entries());
     const int64_t cmax = max_batch_entries(list_ce, cbatch);
-    const size_t ew = sq ? 1 : 4;
+    const size_t ew = sq ? 1 : w16 ? 2 : 4;
     DeviceBuffer ckey_in(cmax * 4), cval_in(cmax * 4), ent_c(cmax * 2), ent_w(cmax * ew), ckey(cmax * 4),
         cperm(cmax * 4), cc(cmax * 2), cw(cmax * ew);
     const int assign_grid = std::min(L, 2048);
@@ -2000,12 +2024,16 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
     const int mc = std::max(1, max_clusters);
     int mc2 = 1;
     while (mc2 < mc) mc2 <<= 1;
-    const size_t assign_lds = ((mc2 * 4 + 15) & ~15) + static_cast<size_t>(4) * mc * 4;
+    // sims: 4 waves x mc accumulators (int64 for 16-bit codes, whose sums pass 2^32)
+    const size_t assign_lds = ((mc2 * 4 + 15) & ~15) + static_cast<size_t>(4) * mc * (w16 ? 8 : 4);
     if (assign_lds > 64 * 1024) {
         hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(assign_kernel<uint8_t, int32_t>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(assign_lds)),
                   "hipFuncSetAttribute");
         hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(assign_kernel<float, float>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(assign_lds)),
+                  "hipFuncSetAttribute");
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(assign_kernel<uint16_t, unsigned long long>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(assign_lds)),
                   "hipFuncSetAttribute");
     }
@@ -2032,6 +2060,12 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
                                    centroid_doc.get<int32_t>(), cluster_list.get<int32_t>(), d_lcb.get<int32_t>(),
                                    cent_off.get<int64_t>(), g0, ng, eb, ckey_in.get<uint32_t>(),
                                    cval_in.get<int32_t>(), ent_c.get<uint16_t>(), ent_w.get<uint8_t>());
+            else if (w16)
+                hipLaunchKernelGGL((centroid_entries_kernel<uint16_t>), dim3(grid_for(ng, kB / 16)), dim3(kB), 0,
+                                   nullptr, fwd.indptr, fwd.indices, static_cast<const uint16_t*>(fwd.values),
+                                   centroid_doc.get<int32_t>(), cluster_list.get<int32_t>(), d_lcb.get<int32_t>(),
+                                   cent_off.get<int64_t>(), g0, ng, eb, ckey_in.get<uint32_t>(),
+                                   cval_in.get<int32_t>(), ent_c.get<uint16_t>(), ent_w.get<uint16_t>());
             else
                 hipLaunchKernelGGL((centroid_entries_kernel<float>), dim3(grid_for(ng, kB / 16)), dim3(kB), 0, nullptr,
                                    fwd.indptr, fwd.indices, static_cast<const float*>(fwd.values),
@@ -2044,6 +2078,10 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
                 hipLaunchKernelGGL((gather_csc_kernel<uint8_t>), dim3(grid_for(n)), dim3(kB), 0, nullptr,
                                    cperm.get<int32_t>(), ent_c.get<uint16_t>(), ent_w.get<uint8_t>(),
                                    static_cast<size_t>(n), cc.get<uint16_t>(), cw.get<uint8_t>());
+            else if (w16)
+                hipLaunchKernelGGL((gather_csc_kernel<uint16_t>), dim3(grid_for(n)), dim3(kB), 0, nullptr,
+                                   cperm.get<int32_t>(), ent_c.get<uint16_t>(), ent_w.get<uint16_t>(),
+                                   static_cast<size_t>(n), cc.get<uint16_t>(), cw.get<uint16_t>());
             else
                 hipLaunchKernelGGL((gather_csc_kernel<float>), dim3(grid_for(n)), dim3(kB), 0, nullptr,
                                    cperm.get<int32_t>(), ent_c.get<uint16_t>(), ent_w.get<float>(),
@@ -2063,6 +2101,13 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
                                d_kept_base.get<int64_t>(), kept.get<int32_t>(), d_lcb.get<int32_t>(),
                                centroid_doc.get<int32_t>(), cent_off.get<int64_t>(), eb, ckey.get<uint32_t>(),
                                cc.get<uint16_t>(), cw.get<uint8_t>(), tmap.get<int2>(), L, assign.get<int32_t>(), l0,
+                               l1, mc, bstat.get<unsigned long long>());
+        else if (w16)
+            hipLaunchKernelGGL((assign_kernel<uint16_t, unsigned long long>), dim3(grid), dim3(kB), assign_lds, nullptr,
+                               fwd.indptr, fwd.indices, static_cast<const uint16_t*>(fwd.values),
+                               d_kept_base.get<int64_t>(), kept.get<int32_t>(), d_lcb.get<int32_t>(),
+                               centroid_doc.get<int32_t>(), cent_off.get<int64_t>(), eb, ckey.get<uint32_t>(),
+                               cc.get<uint16_t>(), cw.get<uint16_t>(), tmap.get<int2>(), L, assign.get<int32_t>(), l0,
                                l1, mc, bstat.get<unsigned long long>());
         else
             hipLaunchKernelGGL((assign_kernel<float, float>), dim3(grid), dim3(kB), assign_lds, nullptr, fwd.indptr,
@@ -2115,9 +2160,10 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
     DeviceBuffer spill;
     if (!sq) spill.allocate(static_cast<size_t>(sgrid) * spill_stride * sizeof(KeyItem));
     if (!sq && ldsf > 160 * 1024)
-        throw std::invalid_argument("vocabulary too large for the FP32 device summarizer");
+        throw std::invalid_argument("vocabulary too large for the FP32 / 16-bit device summarizer");
     if (!sq) {
-        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(summarize_f32_kernel),
+        hip_check(hipFuncSetAttribute(w16 ? reinterpret_cast<const void*>(summarize_seq_kernel<uint16_t>)
+                                          : reinterpret_cast<const void*>(summarize_seq_kernel<float>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(ldsf)),
                   "hipFuncSetAttribute");
     }
@@ -2168,11 +2214,19 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
                                    scount.get<int64_t>(), out_off.get<int64_t>(), sterm.get<uint16_t>(),
                                    sval.get<uint8_t>(), scl.get<uint16_t>(), big.get<int32_t>(), n_big.get<int32_t>());
             }
+        } else if (w16) {
+            hipLaunchKernelGGL((summarize_seq_kernel<uint16_t>), dim3(sgrid), dim3(kB), ldsf, nullptr, fwd.indptr,
+                               fwd.indices, static_cast<const uint16_t*>(fwd.values), cdp.get<int64_t>(),
+                               docs.get<int32_t>(), cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), C, L, f32_cap,
+                               alpha, scount.get<int64_t>(), out_off.get<int64_t>(), pass == 1,
+                               pass == 1 ? sterm.get<uint16_t>() : nullptr,
+                               pass == 1 ? sval.get<uint16_t>() : nullptr, pass == 1 ? scl.get<uint16_t>() : nullptr,
+                               spill.get<KeyItem>(), spill_stride);
         } else {
-            hipLaunchKernelGGL(summarize_f32_kernel, dim3(sgrid), dim3(kB), ldsf, nullptr, fwd.indptr, fwd.indices,
-                               static_cast<const float*>(fwd.values), cdp.get<int64_t>(), docs.get<int32_t>(),
-                               cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), C, L, f32_cap, alpha,
-                               scount.get<int64_t>(), out_off.get<int64_t>(), pass == 1,
+            hipLaunchKernelGGL((summarize_seq_kernel<float>), dim3(sgrid), dim3(kB), ldsf, nullptr, fwd.indptr,
+                               fwd.indices, static_cast<const float*>(fwd.values), cdp.get<int64_t>(),
+                               docs.get<int32_t>(), cluster_list.get<int32_t>(), d_lcb.get<int32_t>(), C, L, f32_cap,
+                               alpha, scount.get<int64_t>(), out_off.get<int64_t>(), pass == 1,
                                pass == 1 ? sterm.get<uint16_t>() : nullptr, pass == 1 ? sval.get<float>() : nullptr,
                                pass == 1 ? scl.get<uint16_t>() : nullptr, spill.get<KeyItem>(), spill_stride);
         }
@@ -2211,6 +2265,11 @@ void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int l
                                        perm.get<int32_t>(), scl.get<uint16_t>() + eb, sval.get<uint8_t>() + eb,
                                        static_cast<size_t>(n), out.sum_cluster.get<uint16_t>() + eb,
                                        out.sum_value.get<uint8_t>() + eb);
+                else if (w16)
+                    hipLaunchKernelGGL((gather_summary_kernel<uint16_t>), dim3(grid_for(n)), dim3(kB), 0, nullptr,
+                                       perm.get<int32_t>(), scl.get<uint16_t>() + eb, sval.get<uint16_t>() + eb,
+                                       static_cast<size_t>(n), out.sum_cluster.get<uint16_t>() + eb,
+                                       out.sum_value.get<uint16_t>() + eb);
                 else
                     hipLaunchKernelGGL((gather_summary_kernel<float>), dim3(grid_for(n)), dim3(kB), 0, nullptr,
                                        perm.get<int32_t>(), scl.get<uint16_t>() + eb, sval.get<float>() + eb,

@@ -172,10 +172,10 @@ void Index::build() { throw std::runtime_error("not implemented"); }
 
 // ============================================================== SeismicIndex
 
-SeismicIndex::SeismicIndex(int dim, bool sq, float vmin, float vmax, Params p)
-    : Index(dim), sq_(sq), vmin_(vmin), vmax_(vmax), params_(p) {
+SeismicIndex::SeismicIndex(int dim, bool sq, float vmin, float vmax, Params p, bool w16)
+    : Index(dim), sq_(sq), w16_(sq && w16), vmin_(vmin), vmax_(vmax), params_(p) {
     if (sq_) check_quantizer_range(vmin, vmax);
-    fwd_.esz = sq_ ? 1 : 4;
+    fwd_.esz = sq_ ? (w16_ ? 2 : 1) : 4;
     fwd_.dim = static_cast<size_t>(dim);
 }
 
@@ -201,7 +201,8 @@ void SeismicIndex::add(idx_t n, const idx_t* indptr, const term_t* indices, cons
     fwd_.values.resize(old_vals + static_cast<size_t>(nnz) * fwd_.esz);
     if (sq_) {
         require_device();
-        encode_sq_on_device(values, static_cast<size_t>(nnz), vmin_, vmax_, fwd_.values.data() + old_vals);
+        encode_sq_on_device(values, static_cast<size_t>(nnz), vmin_, vmax_, fwd_.values.data() + old_vals,
+                            w16_ ? 16 : 8);
     } else {
         std::memcpy(fwd_.values.data() + old_vals, values, static_cast<size_t>(nnz) * 4);
     }
@@ -272,7 +273,7 @@ void SeismicIndex::refresh_block_rows() {
     view_.blk_rows = nullptr;
     view_.blk_words = 0;
     const char* e = std::getenv("NSPARSE_BLOCK_ROWS");
-    const bool on = sq_ && !(e != nullptr && std::string(e) == "0");
+    const bool on = u8() && !(e != nullptr && std::string(e) == "0");
     const int64_t postings = static_cast<int64_t>(d_docs_.bytes() / 4);
     int64_t words = 0;
     if (!on || !build_block_rows(view_, postings, d_row_ref_, d_blk_rows_, words)) {
@@ -501,7 +502,7 @@ void SeismicIndex::search_host(idx_t n, const idx_t* indptr, const term_t* indic
     for (idx_t i = 0; i < n; ++i) longest = std::max(longest, indptr[i + 1] - indptr[i]);
     const int qcap = static_cast<int>(std::min<idx_t>(longest, INT32_MAX));
     const bool long_queries = longest > kDeviceQueryMax;
-    if (long_queries && (stats_out != nullptr || !sq_ || view_.row_ref == nullptr ||
+    if (long_queries && (stats_out != nullptr || !u8() || view_.row_ref == nullptr ||
                          !search_spread_fits(view_, max_clusters_, k, req.cut, max_list_docs_, qcap)))
         throw std::invalid_argument("query has more non-zeros than the device path supports (" +
                                     std::to_string(kDeviceQueryMax) +
@@ -551,12 +552,12 @@ void SeismicIndex::search_host(idx_t n, const idx_t* indptr, const term_t* indic
     const float qmin = req.has_sq_range ? req.vmin : vmin_;
     const float qmax = req.has_sq_range ? req.vmax : vmax_;
     a.q_vmin = qmin;
-    a.q_scale = 255.0F / (qmax - qmin);
+    a.q_scale = code_max() / (qmax - qmin);
     {
-        // ScalarQuantizer::decode_dot_product (utils/scalar_quantizer.h:118-126)
+        // ScalarQuantizer::decode_dot_product (utils/scalar_quantizer.h:85-93)
         const float ingest_range = vmax_ - vmin_;
         const float query_range = qmax - qmin;
-        const float max_q = 255.0F;
+        const float max_q = code_max();
         a.decode_scale = (ingest_range * query_range) / (max_q * max_q);
     }
     a.sel_of_query = nullptr;
@@ -571,7 +572,7 @@ void SeismicIndex::search_host(idx_t n, const idx_t* indptr, const term_t* indic
     if (spread)
         launch_search_spread(a, max_clusters_, k, max_list_docs_, st->dev<void>(s_spread), st->stream(), qcap);
     else
-        launch_search_checked(a, max_clusters_, k, sq_, st->stream());
+        launch_search_checked(a, max_clusters_, k, u8(), st->stream());
     st->download_and_wait();
     const int32_t err = *st->host<int32_t>(s_err);
     if (err != 0) throw std::invalid_argument(device_error_message(err));
@@ -602,7 +603,7 @@ void SeismicIndex::search_device(int64_t n, const int32_t* d_indptr, const uint1
         spread_scratch_.release(slot, stream);
         return;
     }
-    launch_search_checked(a, max_clusters_, k_max, sq_, stream);
+    launch_search_checked(a, max_clusters_, k_max, u8(), stream);
 }
 
 // A launch of one query, or of a few, leaves most CUs idle with one workgroup
@@ -615,7 +616,7 @@ bool SeismicIndex::use_spread(int64_t n, int k_max, int cut, bool stats, hipStre
     if (off != nullptr && std::string(off) == "0") return false;
     const char* e = std::getenv("NSPARSE_SPREAD_MAX_QUERIES");
     const int64_t max_n = e != nullptr ? std::atoll(e) : 2;
-    if (stats || !sq_ || n > max_n || view_.row_ref == nullptr) return false;
+    if (stats || !u8() || n > max_n || view_.row_ref == nullptr) return false;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
     return search_spread_fits(view_, max_clusters_, k_max, cut, max_list_docs_);
@@ -638,7 +639,7 @@ void SeismicIndex::search_device_noalloc(int64_t n, const int32_t* d_indptr, con
     SearchArgs a = search_args(n, d_indptr, d_indices, d_values, d_ks, k_max, req, d_sel_of_query, d_bitmaps,
                                n_bitmaps, d_out_d, d_out_l, nullptr);
     a.error_flag = d_err;
-    launch_search_noalloc(a, max_clusters_, k_max, sq_, stream);
+    launch_search_noalloc(a, max_clusters_, k_max, u8(), stream);
 }
 
 SearchArgs SeismicIndex::search_args(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices,
@@ -665,10 +666,10 @@ SearchArgs SeismicIndex::search_args(int64_t n, const int32_t* d_indptr, const u
     const float qmin = req.has_sq_range ? req.vmin : vmin_;
     const float qmax = req.has_sq_range ? req.vmax : vmax_;
     a.q_vmin = qmin;
-    a.q_scale = 255.0F / (qmax - qmin);
+    a.q_scale = code_max() / (qmax - qmin);
     const float ingest_range = vmax_ - vmin_;
     const float query_range = qmax - qmin;
-    const float max_q = 255.0F;
+    const float max_q = code_max();
     a.decode_scale = (ingest_range * query_range) / (max_q * max_q);
     a.sel_of_query = d_sel_of_query;
     if (d_sel_of_query != nullptr && (d_bitmaps == nullptr || n_bitmaps <= 0))
@@ -712,7 +713,7 @@ float time_launch(const std::function<void()>& launch, hipStream_t stream, int r
 
 SearchPlan::SearchPlan(SeismicIndex& index, const SearchArgs& args, int k_max, const SearchConfig* config,
                        hipStream_t stream)
-    : args_(args), k_max_(k_max), max_clusters_(index.max_clusters()), sq_(index.sq()) {
+    : args_(args), k_max_(k_max), max_clusters_(index.max_clusters()), sq_(index.u8()) {
     scratch_.allocate(search_scratch_bytes(args.n_queries));
     const int dim = args.ix.dim;
     if (config != nullptr) {
@@ -811,7 +812,7 @@ void SeismicIndex::exact_search(idx_t n, const idx_t* indptr, const term_t* indi
     const float qmax = req.has_sq_range ? req.vmax : vmax_;
     const float ingest_range = vmax_ - vmin_;
     const float query_range = qmax - qmin;
-    const float max_q = 255.0F;
+    const float max_q = code_max();
     ExactArgs a{};
     a.ix = view_;
     a.q_indptr = st->dev<int32_t>(s_qip);
@@ -821,12 +822,12 @@ void SeismicIndex::exact_search(idx_t n, const idx_t* indptr, const term_t* indi
     a.n_ids = static_cast<int32_t>(ids.size());
     a.k = k;
     a.q_vmin = qmin;
-    a.q_scale = 255.0F / (qmax - qmin);
+    a.q_scale = code_max() / (qmax - qmin);
     a.decode_scale = (ingest_range * query_range) / (max_q * max_q);
     a.out_d = st->dev<float>(s_out_d);
     a.out_l = st->dev<int32_t>(s_out_l);
     a.n_queries = n;
-    launch_exact_checked(a, sq_, st->stream());
+    launch_exact_checked(a, u8(), st->stream());
     st->download_and_wait();
     std::memcpy(distances, st->host(s_out_d), nk * 4);
     std::memcpy(labels, st->host(s_out_l), nk * 4);
@@ -839,7 +840,7 @@ void SeismicIndex::exact_search(idx_t n, const idx_t* indptr, const term_t* indi
 // InvertedListClusters::serialize (cluster/inverted_list_clusters.cpp:333-363).
 void SeismicIndex::write_payload(ByteWriter& w) const {
     if (sq_) {
-        w.scalar<uint8_t>(0);  // QuantizerType::QT_8bit
+        w.scalar<uint8_t>(w16_ ? 1 : 0);  // QuantizerType: QT_8bit 0, QT_16bit 1
         w.scalar<float>(vmin_);
         w.scalar<float>(vmax_);
     }
@@ -882,13 +883,13 @@ void SeismicIndex::read_payload(ByteReader& r) {
         const float hi = r.scalar<float>();
         if (qt != 0 && qt != 1) throw std::runtime_error("index file declares an unknown quantizer type");
         check_quantizer_range(lo, hi);
-        if (qt == 1) throw std::runtime_error("16-bit quantized indexes are not supported on the device path");
+        w16_ = qt == 1;
         vmin_ = lo;
         vmax_ = hi;
     }
     const size_t count = r.scalar<size_t>();
     fwd_ = HostCsr{};
-    fwd_.esz = sq_ ? 1 : 4;
+    fwd_.esz = sq_ ? (w16_ ? 2 : 1) : 4;
     fwd_.dim = static_cast<size_t>(dim_);
     if (count > 0) {
         const size_t fdim = r.scalar<size_t>();
@@ -1412,6 +1413,7 @@ std::unique_ptr<Index> index_factory(int dim, const char* description) {
     if (type == "seismic" || type == "seismic_sq") {
         SeismicIndex::Params p;
         bool sq = type == "seismic_sq";
+        bool w16 = false;
         float vmin = 0.0F, vmax = 1.0F;
         if (sq) {
             const std::string qs = get("quantizer", "8bit");
@@ -1422,15 +1424,14 @@ std::unique_ptr<Index> index_factory(int dim, const char* description) {
             p.alpha = std::stof(get("alpha", "0.5"));
             p.seed = std::stoi(get("seed", "-1"));
             check_quantizer_range(vmin, vmax);
-            if (qs == "16bit")
-                throw std::runtime_error("16-bit scalar quantization is not implemented on the device path");
+            w16 = qs == "16bit";  // any other value is 8-bit, as in the reference
         } else {
             p.lambda = std::stoi(get("lambda", "10"));
             p.beta = std::stoi(get("beta", "5"));
             p.alpha = std::stof(get("alpha", "0.5"));
             p.seed = std::stoi(get("seed", "-1"));
         }
-        return std::make_unique<SeismicIndex>(dim, sq, vmin, vmax, p);
+        return std::make_unique<SeismicIndex>(dim, sq, vmin, vmax, p, w16);
     }
     if (type == "idmap") {
         if (segments.size() < 2) throw std::invalid_argument("idmap requires a delegate index type");

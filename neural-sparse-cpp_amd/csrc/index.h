@@ -77,8 +77,9 @@ protected:
     int dim_;
 };
 
-// SeismicIndex (FP32) and SeismicScalarQuantizedIndex (8-bit) in one class:
-// the two differ only in element width, query encoding and score decoding.
+// SeismicIndex (FP32) and SeismicScalarQuantizedIndex (8-bit or 16-bit codes)
+// in one class: they differ only in element width, query encoding and score
+// decoding.
 class SeismicIndex : public Index {
 public:
     struct Params {
@@ -86,7 +87,8 @@ public:
         float alpha = 0.5F;
         int seed = -1;
     };
-    SeismicIndex(int dim, bool sq, float vmin, float vmax, Params p);
+    // sq: scalar-quantized (SESQ); w16: its codes are 16-bit (QT_16bit)
+    SeismicIndex(int dim, bool sq, float vmin, float vmax, Params p, bool w16 = false);
     std::array<char, 4> id() const override;
     int64_t num_vectors() const override { return fwd_.n(); }
     void add(idx_t n, const idx_t* indptr, const term_t* indices, const float* values) override;
@@ -126,6 +128,8 @@ public:
     // Error code a kernel left from the last search_device call (0 = none).
     int32_t last_device_error();
     bool sq() const { return sq_; }
+    // uint8 codes: the search kernels' code-table and block-row paths
+    bool u8() const { return sq_ && !w16_; }
     const DevIndexView& view();
     int max_clusters() const { return max_clusters_; }
     // Counters of the last build() (zero for an index read from a file).
@@ -141,8 +145,12 @@ private:
     void refresh_summary_bitmaps();
     // Whether a launch of n queries runs as the spread search (search_spread.hip).
     bool use_spread(int64_t n, int k_max, int cut, bool stats, hipStream_t stream) const;
+    // kMax8bit / kMax16bit of the quantizer (utils/scalar_quantizer.h): the
+    // largest code, as float
+    float code_max() const { return w16_ ? 65535.0F : 255.0F; }
 
     bool sq_;
+    bool w16_ = false;
     float vmin_ = 0.0F, vmax_ = 1.0F;
     Params params_;
     HostCsr fwd_;

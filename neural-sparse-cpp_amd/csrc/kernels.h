@@ -40,8 +40,10 @@ struct DeviceLists {
     BuildStats stats;
 };
 
-// ScalarQuantizer::encode (8-bit) of host values on the GPU; codes come back to the host.
-void encode_sq_on_device(const float* host_values, size_t n, float vmin, float vmax, uint8_t* host_codes);
+// ScalarQuantizer::encode of host values on the GPU (bits 8: one byte per code,
+// 16: two, little-endian); codes come back to the host.
+void encode_sq_on_device(const float* host_values, size_t n, float vmin, float vmax, uint8_t* host_codes,
+                         int bits = 8);
 
 // build_inverted_lists_clusters (seismic_common.h:146-184) on the GPU.
 void build_on_device(const DevIndexView& fwd, int64_t n_docs, int64_t nnz, int lambda, int beta, float alpha,

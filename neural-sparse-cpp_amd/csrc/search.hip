@@ -156,9 +156,15 @@ struct LdsPlan {
 // vocabulary-sized byte table at LDS offset 0, so the doc-row scoring loop
 // does one byte read per element instead of bitmap + rank + value reads; it
 // costs dim bytes of LDS per workgroup.
-template <bool SQ, bool DENSE, int BLOCK>
+// W16 (with SQ false): 16-bit codes (QuantizerType::QT_16bit). The traversal
+// is the FP32 one (bitmap + rank lookup, summary scores as float sums in query
+// order, as score_summaries_typed<uint16_t> adds them), over codes held as
+// exact floats; a row's score is the int64 sum of its code products turned to
+// float once (dot_product_uint16_dense, utils/distance.h:49-57).
+template <bool SQ, bool DENSE, int BLOCK, bool W16 = false>
 __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan& P, int64_t q, unsigned char* smem) {
     static_assert(SQ || !DENSE, "the dense code table holds uint8 codes");
+    static_assert(!(SQ && W16), "one code width");
     static_assert(BLOCK % 64 == 0 && BLOCK <= 1024, "whole wavefronts, at most 16");
     constexpr int kBlock = BLOCK;
     constexpr int kWaves = BLOCK / 64;
@@ -237,7 +243,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
     if (tid < kStatFields) qstat[tid] = 0;
     const QueryTables QT{qbits, qpref, eterm, erank, evalf, rlast, rval, rsum, qcode, wsum, &misc[kMiscErr],
                          &misc[kMiscNu]};
-    if (!load_query_lds<SQ, DENSE, BLOCK>(a, s0, qn, P.nw, QT)) {
+    if (!load_query_lds<SQ, DENSE, BLOCK, W16>(a, s0, qn, P.nw, QT)) {
         if (tid == 0) atomicCAS(a.error_flag, 0, kDevTermOutOfRange);
         for (int i = tid; i < a.k_max; i += kBlock) {
             out_d[i] = -1.0F;
@@ -355,6 +361,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
             for (int c = tid; c < nc; c += kBlock) score[c] = static_cast<float>(sacc[c]);
         } else {
             const float* sv = static_cast<const float*>(ix.sum_value);
+            const uint16_t* sv16 = static_cast<const uint16_t*>(ix.sum_value);
             float* tile = reinterpret_cast<float*>(uni);
             const int per = max(1, (P.uni_bytes / 4) / nc);
             for (int f0 = 0; f0 < nf; f0 += per) {
@@ -367,7 +374,8 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                     const float qv = evalf[i];
                     const int64_t e0 = ix.sum_term_ptr[col], e1 = ix.sum_term_ptr[col + 1];
                     float* row = tile + (f - f0) * nc;
-                    for (int64_t e = e0 + lane; e < e1; e += 64) row[ix.sum_cluster[e]] = qv * sv[e];
+                    for (int64_t e = e0 + lane; e < e1; e += 64)
+                        row[ix.sum_cluster[e]] = qv * (W16 ? static_cast<float>(sv16[e]) : sv[e]);
                 }
                 __syncthreads();
                 for (int c = tid; c < nc; c += kBlock) {
@@ -826,9 +834,18 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                     const int len = j < total ? C.len[j] : -1;
                     if (len < 0) continue;
                     const int32_t rs = C.rs[j];
-                    float acc = 0.0F;
-                    for (int pp = rs; pp < rs + len; ++pp) acc += vf[pp] * QL.value(ix.indices[pp]);
-                    C.sc[j] = acc;
+                    if constexpr (W16) {
+                        const uint16_t* v16 = static_cast<const uint16_t*>(ix.values);
+                        uint64_t acc = 0;
+                        for (int pp = rs; pp < rs + len; ++pp)
+                            acc += static_cast<uint64_t>(static_cast<uint32_t>(v16[pp]) *
+                                                         static_cast<uint32_t>(QL.value(ix.indices[pp])));
+                        C.sc[j] = static_cast<float>(acc);
+                    } else {
+                        float acc = 0.0F;
+                        for (int pp = rs; pp < rs + len; ++pp) acc += vf[pp] * QL.value(ix.indices[pp]);
+                        C.sc[j] = acc;
+                    }
                 }
                 if (lane == 0)
                     __hip_atomic_fetch_add(&C.meta[kChunkCheckin], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1018,7 +1035,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
         float d = -1.0F;
         int32_t l = -1;
         if (i < k && i < tkn) {
-            d = SQ ? tks[i] * a.decode_scale : tks[i];
+            d = (SQ || W16) ? tks[i] * a.decode_scale : tks[i];
             l = tki[i];
         }
         out_d[i] = d;
@@ -1032,19 +1049,19 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
 
 // One workgroup per query: query query_base + blockIdx.x. Both kernels are held
 // to 128 VGPRs (4 waves per SIMD: two 512-lane or four 256-lane queries per CU).
-template <bool SQ, bool DENSE, int BLOCK>
+template <bool SQ, bool DENSE, int BLOCK, bool W16 = false>
 __global__ __launch_bounds__(BLOCK, 4) void seismic_search_kernel(SearchArgs a, LdsPlan P) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int64_t q = a.query_base + blockIdx.x;
     if (q >= a.n_queries) return;
-    search_query<SQ, DENSE, BLOCK>(a, P, q, smem);
+    search_query<SQ, DENSE, BLOCK, W16>(a, P, q, smem);
 }
 
 // Persistent workgroups (as many as fit on the device at once): each takes the
 // next query index from the device counter a.queue (zero at launch) until all
 // n_queries are handed out; a.order, when set, maps the index to a query (for
 // longest-first scheduling). Every workgroup exits once the counter passes n.
-template <bool SQ, bool DENSE, int BLOCK>
+template <bool SQ, bool DENSE, int BLOCK, bool W16 = false>
 __global__ __launch_bounds__(BLOCK, 4) void seismic_search_persistent_kernel(SearchArgs a, LdsPlan P) {
     extern __shared__ __align__(16) unsigned char smem[];
     int32_t* slot = reinterpret_cast<int32_t*>(smem + P.o_queue);
@@ -1055,7 +1072,7 @@ __global__ __launch_bounds__(BLOCK, 4) void seismic_search_persistent_kernel(Sea
         __syncthreads();
         if (i >= a.n_queries) break;
         const int64_t q = a.order != nullptr ? a.order[i] : i;
-        search_query<SQ, DENSE, BLOCK>(a, P, q, smem);
+        search_query<SQ, DENSE, BLOCK, W16>(a, P, q, smem);
         __syncthreads();
     }
 }
@@ -1112,7 +1129,7 @@ LdsPlan make_plan(int dim, int max_clusters, int kmax, bool sq, bool dense, int 
 // ExactMatcher::single_query (exact_matcher.h:23-41): every id of a small
 // enumerable selector is scored against the dense query and the <= k results
 // are written best first, ties by doc id. One workgroup per query.
-template <bool SQ>
+template <bool SQ, bool W16 = false>
 __global__ __launch_bounds__(kBlock) void exact_match_kernel(ExactArgs a, int nw) {
     extern __shared__ __align__(16) unsigned char smem[];
     uint32_t* qbits = reinterpret_cast<uint32_t*>(smem);
@@ -1162,9 +1179,10 @@ __global__ __launch_bounds__(kBlock) void exact_match_kernel(ExactArgs a, int nw
         const int i = rlast[r];
         if (i < 0) continue;
         const float v = a.q_values[s0 + i];
-        if constexpr (SQ) {
+        if constexpr (SQ || W16) {
+            constexpr float kMax = W16 ? 65535.0F : 255.0F;
             float x = (v - a.q_vmin) * a.q_scale;
-            x = (x < 255.0F) ? x : 255.0F;
+            x = (x < kMax) ? x : kMax;
             x = (0.0F < x) ? x : 0.0F;
             rval[r] = static_cast<float>(static_cast<uint32_t>(roundf(x)));
         } else {
@@ -1179,6 +1197,13 @@ __global__ __launch_bounds__(kBlock) void exact_match_kernel(ExactArgs a, int nw
             const uint8_t* vv = static_cast<const uint8_t*>(ix.values);
             int acc = 0;
             for (int p = r0; p < r1; ++p) acc += static_cast<int>(vv[p]) * static_cast<int>(QL.value(ix.indices[p]));
+            sc[j] = static_cast<float>(acc);
+        } else if constexpr (W16) {
+            const uint16_t* vv = static_cast<const uint16_t*>(ix.values);
+            uint64_t acc = 0;
+            for (int p = r0; p < r1; ++p)
+                acc += static_cast<uint64_t>(static_cast<uint32_t>(vv[p]) *
+                                             static_cast<uint32_t>(QL.value(ix.indices[p])));
             sc[j] = static_cast<float>(acc);
         } else {
             const float* vv = static_cast<const float*>(ix.values);
@@ -1199,7 +1224,7 @@ __global__ __launch_bounds__(kBlock) void exact_match_kernel(ExactArgs a, int nw
             const int32_t oi = a.ids[o];
             if (better(os, oi, s, id) || (os == s && oi == id && o < j)) ++rank;
         }
-        out_d[rank] = SQ ? s * a.decode_scale : s;
+        out_d[rank] = (SQ || W16) ? s * a.decode_scale : s;
         out_l[rank] = id;
     }
     for (int j = a.n_ids + tid; j < a.k; j += kBlock) {
@@ -1767,10 +1792,11 @@ int device_cus() {
     return cus;
 }
 
-template <bool SQ, bool DENSE, int BLOCK>
+template <bool SQ, bool DENSE, int BLOCK, bool W16 = false>
 void launch_variant(const SearchArgs& args, const LdsPlan& P, bool persistent, hipStream_t stream) {
-    const void* kfn = persistent ? reinterpret_cast<const void*>(seismic_search_persistent_kernel<SQ, DENSE, BLOCK>)
-                                 : reinterpret_cast<const void*>(seismic_search_kernel<SQ, DENSE, BLOCK>);
+    const void* kfn = persistent
+                          ? reinterpret_cast<const void*>(seismic_search_persistent_kernel<SQ, DENSE, BLOCK, W16>)
+                          : reinterpret_cast<const void*>(seismic_search_kernel<SQ, DENSE, BLOCK, W16>);
     if (P.total > 64 * 1024)
         hip_check(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, P.total),
                   "hipFuncSetAttribute");
@@ -1779,8 +1805,8 @@ void launch_variant(const SearchArgs& args, const LdsPlan& P, bool persistent, h
         hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, BLOCK, P.total), "occupancy");
         const int64_t want = static_cast<int64_t>(std::max(per_cu, 1)) * device_cus();
         const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(want, args.n_queries)));
-        hipLaunchKernelGGL((seismic_search_persistent_kernel<SQ, DENSE, BLOCK>), dim3(grid), dim3(BLOCK), P.total,
-                           stream, args, P);
+        hipLaunchKernelGGL((seismic_search_persistent_kernel<SQ, DENSE, BLOCK, W16>), dim3(grid), dim3(BLOCK),
+                           P.total, stream, args, P);
         hip_check(hipGetLastError(), "search kernel launch");
         return;
     }
@@ -1789,8 +1815,8 @@ void launch_variant(const SearchArgs& args, const LdsPlan& P, bool persistent, h
         SearchArgs a = args;
         a.query_base = base;
         const int64_t nb = args.n_queries - base < kMaxGrid ? args.n_queries - base : kMaxGrid;
-        hipLaunchKernelGGL((seismic_search_kernel<SQ, DENSE, BLOCK>), dim3(static_cast<unsigned>(nb)), dim3(BLOCK),
-                           P.total, stream, a, P);
+        hipLaunchKernelGGL((seismic_search_kernel<SQ, DENSE, BLOCK, W16>), dim3(static_cast<unsigned>(nb)),
+                           dim3(BLOCK), P.total, stream, a, P);
         hip_check(hipGetLastError(), "search kernel launch");
     }
 }
@@ -1862,7 +1888,9 @@ void launch_search(const SearchArgs& args_in, int max_clusters, int kmax, bool s
         hip_check(hipGetLastError(), "search queue launch");
     }
     const bool pers = cfg.persistent != 0;
-    if (!sq)
+    if (!sq && a.ix.esz == 2)
+        launch_variant<false, false, 256, true>(a, P, pers, stream);
+    else if (!sq)
         launch_variant<false, false, 256>(a, P, pers, stream);
     else if (cfg.lookup == 1 && cfg.block == 1024)
         launch_variant<true, true, 1024>(a, P, pers, stream);
@@ -1940,6 +1968,8 @@ void launch_exact_checked(const ExactArgs& a, bool sq, hipStream_t stream) {
     const dim3 grid(static_cast<unsigned>(a.n_queries));
     if (sq)
         hipLaunchKernelGGL((exact_match_kernel<true>), grid, dim3(kBlock), lds, stream, a, nw);
+    else if (a.ix.esz == 2)
+        hipLaunchKernelGGL((exact_match_kernel<false, true>), grid, dim3(kBlock), lds, stream, a, nw);
     else
         hipLaunchKernelGGL((exact_match_kernel<false>), grid, dim3(kBlock), lds, stream, a, nw);
     hip_check(hipGetLastError(), "exact-match kernel launch");

@@ -510,9 +510,10 @@ struct QueryTables {
 
 // Loads query entries [s0, s0 + qn) into the tables. Returns false (uniformly,
 // after a barrier) when a term id is >= dim; the tables are then partial.
-// Codes are ScalarQuantizer::encode_8bit of the values
-// (utils/scalar_quantizer.h:97-101). No barrier follows the last writes.
-template <bool SQ, bool DENSE, int BLOCK>
+// Codes are ScalarQuantizer::encode_8bit (SQ) or encode_16bit (W16) of the
+// values (utils/scalar_quantizer.h:97-113), held as exact floats. No barrier
+// follows the last writes.
+template <bool SQ, bool DENSE, int BLOCK, bool W16 = false>
 __device__ __forceinline__ bool load_query_lds(const SearchArgs& a, int64_t s0, int qn, int nw,
                                                const QueryTables& T) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -530,9 +531,10 @@ __device__ __forceinline__ bool load_query_lds(const SearchArgs& a, int64_t s0, 
             continue;
         }
         T.eterm[i] = static_cast<uint16_t>(t);
-        if constexpr (SQ) {
+        if constexpr (SQ || W16) {
+            constexpr float kMax = W16 ? 65535.0F : 255.0F;
             float sc = (v - a.q_vmin) * a.q_scale;
-            sc = (sc < 255.0F) ? sc : 255.0F;
+            sc = (sc < kMax) ? sc : kMax;
             sc = (0.0F < sc) ? sc : 0.0F;
             T.evalf[i] = static_cast<float>(static_cast<uint32_t>(roundf(sc)));
         } else {

@@ -131,8 +131,8 @@ def test_class_constructors(nsparse):
     assert nsparse.SeismicScalarQuantizedIndex(100).id() == "SESQ"
     sq = nsparse.SeismicScalarQuantizedIndex(nsparse.QuantizerType.QT_8bit, 0.5, 3.0, p, 50)
     assert sq.id() == "SESQ" and sq.get_dimension() == 50
-    with pytest.raises(RuntimeError, match="16-bit"):
-        nsparse.SeismicScalarQuantizedIndex(nsparse.QuantizerType.QT_16bit, 0.0, 1.0, p, 10)
+    q16 = nsparse.SeismicScalarQuantizedIndex(nsparse.QuantizerType.QT_16bit, 0.0, 1.0, p, 10)
+    assert q16.id() == "SESQ" and q16.get_dimension() == 10
     with pytest.raises(TypeError):
         nsparse.SeismicScalarQuantizedIndex(1, 2)
     assert nsparse.BrutalIndex(5).id() == "BRUT" and nsparse.InvertedIndex(9).id() == "INVT"
@@ -204,7 +204,7 @@ def test_layout_round_trip_is_byte_identical(nsparse):
     ip = np.arange(0, 12 * n + 1, 12, dtype=np.int32)
     ix = np.concatenate([np.sort(rng.choice(dim, 12, replace=False)) for _ in range(n)]).astype(np.uint16)
     v = rng.lognormal(0, 0.7, 12 * n).astype(np.float32)
-    for sq in (0, 1):
+    for sq in (0, 1, 2):  # FP32, 8-bit codes, 16-bit codes
         o = O.OracleIndex(dim, sq, 50, 5, 0.4, 1, 0.0, float(v.max()), tie_mode=1)
         o.add(ip, ix, v)
         o.build()
@@ -213,6 +213,14 @@ def test_layout_round_trip_is_byte_identical(nsparse):
         assert got.id() == ("SESQ" if sq else "SEIS")
         assert got.num_vectors() == n
         assert nsparse.serialize_index(got) == blob
+    # a 16-bit file whose element size says 8-bit is corrupt
+    o = O.OracleIndex(dim, 2, 50, 5, 0.4, 1, 0.0, float(v.max()), tie_mode=1)
+    o.add(ip, ix, v)
+    o.build()
+    bad = bytearray(o.serialize())
+    bad[8] = 0  # QuantizerType byte after fourcc + dim: QT_8bit with 2-byte values
+    with pytest.raises(RuntimeError, match="element size"):
+        nsparse.read_index_bytes(bytes(bad))
 
 
 def test_corrupt_files_are_rejected(nsparse):
