@@ -152,6 +152,41 @@ def build_bytes(n_docs, nnz, st, v=1):
     return float(fwd + postings + rows + pairs), float(pairs)
 
 
+def read_bandwidth(nsparse, dev, nbytes=4 << 30, reps=5):
+    """Rate of a streaming read of device memory (GB/s): the library's read
+    probe (non-temporal 16-byte loads, XOR of the words; the fastest read shape
+    measured on this device) over a buffer far larger than the last-level
+    cache, timed with HIP events on the stream it runs on."""
+    import ctypes
+
+    import torch
+
+    buf = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    out = torch.zeros(1, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+
+    def go():
+        rc = nsparse._lib.lib.nsparse_testing_read_probe(ctypes.c_void_p(buf.data_ptr()), ctypes.c_size_t(nbytes),
+                                                        ctypes.c_void_p(out.data_ptr()), sp)
+        if rc != 0:
+            raise RuntimeError(nsparse._lib.lib.nsparse_last_error().decode())
+
+    go()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        go()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del buf, out
+    torch.cuda.empty_cache()
+    return nbytes / (ms * 1e-3) / 1e9
+
+
 def copy_bandwidth(dev, nbytes=2 << 30, reps=5):
     """Rate of a plain device copy (bytes read + bytes written per second, GB/s)
     over buffers far larger than the last-level cache: the measured ceiling
@@ -764,6 +799,11 @@ def main():
     c1 = c1_reference_order(args, nsparse, threads, dev, stream)
     copy_gbs = copy_bandwidth(dev)
     log(f"device copy (2 GiB): {copy_gbs:,.0f} GB/s read+write; search achieves {achieved / copy_gbs:.3f} of it")
+    read_gbs = read_bandwidth(nsparse, dev)
+    traffic_gbs = traffic / (kernel_ms * 1e-3) / 1e9 if traffic else None
+    log(f"device streaming read (4 GiB): {read_gbs:,.0f} GB/s"
+        + (f"; the search's measured HBM traffic moves {traffic_gbs:,.0f} GB/s, {traffic_gbs / read_gbs:.3f} of it"
+           if traffic_gbs else ""))
     result = {
         "metric": METRIC, "value": qps, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
@@ -782,7 +822,9 @@ def main():
                      "device_rescored_share": 1 - ref_bytes_launch / bytes_launch,
                      "traffic_over_bytes": (traffic / ref_bytes_launch) if traffic else None,
                      "kernel_ms": kernel_ms,
-                     "copy_gbps": copy_gbs, "frac_of_copy": achieved / copy_gbs},
+                     "copy_gbps": copy_gbs, "frac_of_copy": achieved / copy_gbs,
+                     "read_gbps": read_gbs, "traffic_gbps": traffic_gbs,
+                     "traffic_frac_of_read": (traffic_gbs / read_gbs) if traffic_gbs else None},
         "cpu_baseline": cpu,
         "plain": {"qps": plain_qps if n_plain else None, "p50_ms": pct(0.5), "p99_ms": pct(0.99),
                   "queries": n_plain},

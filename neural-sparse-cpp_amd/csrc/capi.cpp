@@ -602,6 +602,13 @@ extern "C" int nsparse_testing_device_cluster_order(const float* keys, int n, in
     });
 }
 
+extern "C" int nsparse_testing_read_probe(const void* d_buf, size_t bytes, uint32_t* d_out, void* stream) {
+    return guarded([&] {
+        require_device();
+        device_read_probe(d_buf, bytes, d_out, static_cast<hipStream_t>(stream));
+    });
+}
+
 extern "C" int nsparse_testing_device_top_k_tokens(const uint16_t* terms, const float* weights, int n, int k,
                                                    uint16_t* out, int* out_n) {
     return guarded([&] {

@@ -100,6 +100,9 @@ int64_t max_list_docs(const DevIndexView& ix);
 constexpr int kMaxTestOrder = 4096;
 void device_cluster_order(const float* keys, int n, int64_t* out_order, bool block);
 int device_top_k_tokens(const uint16_t* terms, const float* weights, int n, int k, uint16_t* out);
+// One streaming read of `bytes` (a multiple of 16, 16-byte aligned) of device
+// memory; *d_out receives the XOR of its 32-bit words. Measures the plain read rate.
+void device_read_probe(const void* d_buf, size_t bytes, uint32_t* d_out, hipStream_t stream);
 int search_lds_bytes(int dim, int max_clusters, int kmax, bool sq);
 std::string device_error_message(int32_t code);
 // Exhaustive search (BrutalIndex): every document scored, the k best kept per

@@ -89,6 +89,56 @@ void device_cluster_order(const float* keys, int n, int64_t* out_order, bool blo
     dout.download(out_order, static_cast<size_t>(n) * 8);
 }
 
+namespace {
+
+// Streams a buffer once and XORs its words into *out: the device's streaming
+// read rate, for the bandwidth a search's HBM traffic is compared with. Of the
+// shapes measured on MI355X (16-byte loads two to eight in flight per lane,
+// grid-stride or a contiguous chunk per workgroup, 8 to 64 workgroups per CU,
+// plain or non-temporal loads), four non-temporal loads in flight with 64
+// workgroups per CU read fastest.
+typedef unsigned int probe_u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void read_probe_kernel(const probe_u32x4* __restrict__ p, size_t n16,
+                                                         uint32_t* __restrict__ out) {
+    __shared__ uint32_t part[4];
+    uint32_t acc = 0;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        const probe_u32x4 a = __builtin_nontemporal_load(p + i);
+        const probe_u32x4 b = __builtin_nontemporal_load(p + i + stride);
+        const probe_u32x4 c = __builtin_nontemporal_load(p + i + 2 * stride);
+        const probe_u32x4 d = __builtin_nontemporal_load(p + i + 3 * stride);
+        const probe_u32x4 x = a ^ b ^ c ^ d;
+        acc ^= x.x ^ x.y ^ x.z ^ x.w;
+    }
+    for (; i < n16; i += stride) {
+        const probe_u32x4 a = __builtin_nontemporal_load(p + i);
+        acc ^= a.x ^ a.y ^ a.z ^ a.w;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc ^= __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicXor(out, part[0] ^ part[1] ^ part[2] ^ part[3]);
+}
+
+__global__ void zero_word_kernel(uint32_t* p) { *p = 0; }
+
+}  // namespace
+
+void device_read_probe(const void* d_buf, size_t bytes, uint32_t* d_out, hipStream_t stream) {
+    if (d_buf == nullptr || d_out == nullptr) throw std::invalid_argument("unexpected nullptr");
+    if (bytes % 16 != 0 || reinterpret_cast<uintptr_t>(d_buf) % 16 != 0)
+        throw std::invalid_argument("the read probe takes a 16-byte aligned buffer of whole 16-byte words");
+    int dev = 0, cus = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    hipLaunchKernelGGL(zero_word_kernel, dim3(1), dim3(1), 0, stream, d_out);
+    hipLaunchKernelGGL(read_probe_kernel, dim3(static_cast<unsigned>(std::max(cus, 1) * 64)), dim3(256), 0, stream,
+                       static_cast<const probe_u32x4*>(d_buf), bytes / 16, d_out);
+    hip_check(hipGetLastError(), "read probe launch");
+}
+
 int device_top_k_tokens(const uint16_t* terms, const float* weights, int n, int k, uint16_t* out) {
     if (n < 0 || n > kQMax || k <= 0 || k > 64)
         throw std::invalid_argument("device top_k_tokens takes at most 512 entries and k in [1, 64]");

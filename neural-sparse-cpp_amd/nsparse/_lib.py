@@ -158,6 +158,7 @@ _sig("nsparse_testing_cluster_order", [c_void_p, c_int, c_void_p])
 _sig("nsparse_testing_top_k_tokens", [c_void_p, c_void_p, c_int, c_int, c_void_p, P(c_int)])
 _sig("nsparse_testing_device_cluster_order", [c_void_p, c_int, c_int, c_void_p])
 _sig("nsparse_testing_device_top_k_tokens", [c_void_p, c_void_p, c_int, c_int, c_void_p, P(c_int)])
+_sig("nsparse_testing_read_probe", [c_void_p, c_size_t, c_void_p, c_void_p])
 
 # nsparse_status -> the exception the reference's SWIG layer raises
 # (nsparse/python/swignsparse.swig:47-59).
