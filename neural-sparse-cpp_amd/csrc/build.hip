@@ -1735,6 +1735,7 @@ void build_postings_on_device(const int32_t* indptr, const uint16_t* indices, co
     int32_t h_dup = 0;
     dup.download(&h_dup, 4);
     out.duplicate_docs = h_dup != 0;
+    build_window_starts(out, n_docs, dim);
 }
 
 bool build_summary_bitmaps(const DevIndexView& ix, DeviceBuffer& bits, DeviceBuffer& rank) {

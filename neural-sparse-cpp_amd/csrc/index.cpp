@@ -1115,6 +1115,7 @@ void InvertedIndex::ensure_device() {
                 dev_.duplicate_docs = true;
                 break;
             }
+    build_window_starts(dev_, list_docs_, static_cast<int>(h_list_ptr_.size()) - 1);
     dev_valid_ = true;
 }
 

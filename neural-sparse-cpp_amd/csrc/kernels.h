@@ -125,7 +125,18 @@ struct DevicePostings {
     DeviceBuffer list_ptr, docs, vals, max_score;
     int64_t n_postings = 0;
     bool duplicate_docs = false;  // some row repeats a term, so a list holds that doc twice
+    // Search table: window_start[t * (n_windows + 1) + w] is the position of
+    // list t's first posting with doc id >= w * kInvertedWindow (int32; w =
+    // n_windows gives the list's end). Made by build_window_starts.
+    DeviceBuffer window_start;
+    int64_t n_windows = 0;
 };
+
+// Doc ids per window of the exact inverted search (its LDS score tile).
+constexpr int kInvertedWindow = 16384;
+
+// The window_start table of `p` for lists over doc ids [0, n_docs).
+void build_window_starts(DevicePostings& p, int64_t n_docs, int dim);
 
 // ArrayInvertedLists::build_inverted_lists + InvertedIndex::build's per-term
 // maxima (inverted_index.cpp:236-252) over a forward CSR already in HBM.

@@ -1313,7 +1313,8 @@ unsigned exact_scores_blocks(int64_t n_docs) {
 // desc, doc id asc), the order in which the reference's reject-when-not-better
 // heap keeps them (earlier docs win ties at the k-th).
 constexpr int kSelectSlots = 4096;  // LDS keys: the sorted top-k, then candidates
-constexpr int kSelectRound = kBlock * 4;
+constexpr int kSelectPer = 16;      // keys a thread holds per round (loads in flight)
+constexpr int kSelectRound = kBlock * kSelectPer;
 
 __device__ __forceinline__ uint64_t score_key(float s, int64_t d) {
     uint32_t u = __float_as_uint(s);
@@ -1327,11 +1328,11 @@ __device__ __forceinline__ float key_score(uint64_t key) {
     return __uint_as_float(u);
 }
 
-// Descending bitonic sort of keys[0, kSelectSlots) by all kBlock threads.
-__device__ void bitonic_desc(uint64_t* keys) {
-    for (int size = 2; size <= kSelectSlots; size <<= 1) {
+// Descending bitonic sort of keys[0, n) (n a power of two) by all kBlock threads.
+__device__ void bitonic_desc(uint64_t* keys, int n) {
+    for (int size = 2; size <= n; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int t = threadIdx.x; t < kSelectSlots / 2; t += kBlock) {
+            for (int t = threadIdx.x; t < n / 2; t += kBlock) {
                 const int i = 2 * t - (t & (stride - 1));
                 const int j = i + stride;
                 const bool desc = (i & size) == 0;
@@ -1348,8 +1349,11 @@ __device__ void bitonic_desc(uint64_t* keys) {
 
 // Keeps the kcap best of n candidate keys (key_of(i); 0 = not a candidate) in
 // keys[0, kcap), best first: the candidates stream through a filter against
-// the current k-th key; survivors collect in LDS behind the sorted top-kcap and
-// are merged in by a bitonic sort whenever the next round could overflow them.
+// the current k-th key, kSelectPer keys per thread per round held in
+// registers; survivors collect in LDS behind the sorted top-kcap. When the
+// candidate slots run out, the placed keys are merged in by a bitonic sort
+// (which raises the filter) and the round's unplaced keys are filtered and
+// placed again. A key at or below the k-th key can never enter the top-k.
 template <class KeyOf>
 __device__ void stream_topk(uint64_t* keys, int* cnt, int64_t n, int k, int kcap, KeyOf key_of) {
     const int cap = kSelectSlots - kcap;  // candidate slots
@@ -1393,27 +1397,45 @@ __device__ void stream_topk(uint64_t* keys, int* cnt, int64_t n, int k, int kcap
     if (threadIdx.x == 0) *cnt = 0;
     __syncthreads();
     auto merge = [&]() {
-        const int used = kcap + *cnt;
-        for (int i = used + threadIdx.x; i < kSelectSlots; i += kBlock) keys[i] = 0;
+        // only the used slots, padded with zero keys to a power of two, are sorted
+        const int used = kcap + min(*cnt, cap);
+        int n2 = kcap;
+        while (n2 < used) n2 <<= 1;
+        for (int i = used + threadIdx.x; i < n2; i += kBlock) keys[i] = 0;
         __syncthreads();
-        bitonic_desc(keys);
+        bitonic_desc(keys, n2);
         if (threadIdx.x == 0) *cnt = 0;
         thresh = keys[k - 1];
         __syncthreads();
     };
     for (int64_t base = 0; base < n; base += kSelectRound) {
+        uint64_t v[kSelectPer];
 #pragma unroll
-        for (int u = 0; u < kSelectRound / kBlock; ++u) {
+        for (int u = 0; u < kSelectPer; ++u) {
             const int64_t i = base + u * kBlock + threadIdx.x;
-            if (i < n) {
-                const uint64_t key = key_of(i);
-                if (key > thresh) keys[kcap + atomicAdd(cnt, 1)] = key;
-            }
+            v[u] = i < n ? key_of(i) : 0;
         }
-        __syncthreads();
-        const int c = *cnt;
-        __syncthreads();  // every thread has read cnt before the next round adds to it
-        if (c > cap - kSelectRound) merge();
+        uint32_t pending = 0;  // bit u: v[u] passed the filter and has no slot yet
+#pragma unroll
+        for (int u = 0; u < kSelectPer; ++u)
+            if (v[u] > thresh) pending |= 1u << u;
+        while (true) {
+#pragma unroll
+            for (int u = 0; u < kSelectPer; ++u) {
+                if (((pending >> u) & 1u) == 0) continue;
+                if (v[u] > thresh) {
+                    const int slot = atomicAdd(cnt, 1);
+                    if (slot >= cap) continue;  // no room: placed after the next merge
+                    keys[kcap + slot] = v[u];
+                }
+                pending &= ~(1u << u);
+            }
+            __syncthreads();
+            const int c = *cnt;
+            __syncthreads();  // every thread has read cnt before a merge resets it
+            if (c < cap) break;  // every survivor has a slot
+            merge();
+        }
     }
     if (*cnt > 0) merge();
 }
@@ -1537,161 +1559,161 @@ namespace {
 
 // ---- InvertedIndex: exact term-at-a-time scores in LDS windows of doc ids
 constexpr int kInvBlock = 1024;
-constexpr int kInvWaves = kInvBlock / 64;
-constexpr int kInvWindow = 16384;     // doc ids per window: 64 KB of float scores
-constexpr int kInvChunks = 4;         // a round takes up to kInvChunks * kInvBlock postings of one term
-constexpr int kInvSlabTarget = 2048;  // workgroups a group of queries aims for
+constexpr int kInvWindow = kInvertedWindow;  // 64 KB of float scores
+constexpr int kInvChunks = 4;                // a round takes up to kInvChunks * kInvBlock postings of one term
+constexpr int kInvRound = kInvChunks * kInvBlock;
+constexpr int kInvSlabTarget = 2048;         // workgroups a group of queries aims for
+
+// window_start[t * (nw + 1) + w] = lower_bound(list t's doc ids, w * kInvWindow),
+// one thread per (list, window boundary).
+__global__ void window_start_kernel(const int64_t* __restrict__ list_ptr, const int32_t* __restrict__ docs, int dim,
+                                    int64_t nw, int32_t* __restrict__ out) {
+    const int64_t total = static_cast<int64_t>(dim) * (nw + 1);
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < total;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t t = i / (nw + 1), w = i % (nw + 1);
+        const int64_t key = w * kInvWindow;
+        int64_t a = list_ptr[t], b = list_ptr[t + 1];
+        while (a < b) {
+            const int64_t m = (a + b) >> 1;
+            if (docs[m] < key)
+                a = m + 1;
+            else
+                b = m;
+        }
+        out[i] = static_cast<int32_t>(a);
+    }
+}
 
 // Postings a round holds in registers: lane tid has positions
-// p0 + j * kInvBlock + tid for j < kInvChunks, loaded below p0 + n and the
-// list end e; the others read as doc INT32_MAX (never inside a window).
+// p0 + j * kInvBlock + tid for j < kInvChunks below p0 + n; the others read as
+// doc INT32_MAX (never inside a window).
 struct InvRound {
     int32_t d[kInvChunks];
     float v[kInvChunks];
 };
 
 __device__ __forceinline__ void inv_load(const int32_t* __restrict__ docs, const float* __restrict__ vals, int64_t p0,
-                                         int64_t e, int n, int tid, InvRound& r) {
+                                         int n, int tid, InvRound& r) {
 #pragma unroll
     for (int j = 0; j < kInvChunks; ++j) {
         const int off = j * kInvBlock + tid;
-        const int64_t p = p0 + off;
-        const bool ok = off < n && p < e;
-        r.d[j] = ok ? docs[p] : INT32_MAX;
-        r.v[j] = ok ? vals[p] : 0.0F;
+        const bool ok = off < n;
+        r.d[j] = ok ? docs[p0 + off] : INT32_MAX;
+        r.v[j] = ok ? vals[p0 + off] : 0.0F;
     }
 }
 
-// One workgroup per (slab of consecutive windows, query). Per query term, LDS
-// keeps the cursor into its posting list (found once per slab by binary
-// search), the list end and the doc id under the cursor. Each window is
-// scored term by term in query order: a round of the whole workgroup takes the
-// term's next n postings (the count its remaining list's density predicts in
-// the window, with a margin, up to kInvChunks * kInvBlock: what a round loads
-// past the window is read again by the next one), every posting inside the
-// window adds weight * value into
-// its doc's slot, and a barrier ends the round. Docs ascend within a list, so
-// the postings inside the window are a prefix of the round and the first one
-// past it is the new cursor. While a term's round is being added, the next
-// term's first round is already loading. A doc's score is therefore the float
-// sum of its products in query-term order, from 0 -- the same bits however the
-// work is split. The window's scores are then written to row blockIdx.y of
-// `scores` (row stride ld). kAtomic: some list holds a doc twice, so one round
-// may add to a slot twice.
+// One workgroup per (slab of consecutive windows, query). Each window is
+// scored term by term in query order. The window_start table gives each term's
+// postings inside the window exactly, so rounds of up to kInvRound postings
+// load only those (no posting is read twice). Every posting adds weight * value
+// into its doc's slot. A term's docs are distinct, so its rounds need no
+// barrier between them; a barrier ends each term, so a doc's score is the
+// float sum of its products in query-term order, from 0 -- the same bits
+// however the work is split. The round after the current one (the term's
+// next, or the next term's first) loads while the current one is added. The
+// window's scores are then written to row blockIdx.y of `scores` (row stride
+// ld). kAtomic: some list holds a doc twice, so one round may add to a slot
+// twice.
 template <bool kAtomic>
 __global__ __launch_bounds__(kInvBlock) void inverted_scores_kernel(
-    const int64_t* __restrict__ list_ptr, const int32_t* __restrict__ docs, const float* __restrict__ vals,
-    int64_t n_docs, int windows_per_slab, const int32_t* __restrict__ q_indptr,
+    const int32_t* __restrict__ wstart, int64_t n_windows, const int32_t* __restrict__ docs,
+    const float* __restrict__ vals, int64_t n_docs, int windows_per_slab, const int32_t* __restrict__ q_indptr,
     const uint16_t* __restrict__ q_indices, const float* __restrict__ q_values, int64_t q0,
     float* __restrict__ scores, int64_t ld) {
     __shared__ float win[kInvWindow];
-    __shared__ int32_t cur[kQMax];  // list positions fit 32 bits: the host caps postings at 2^31 - 1
-    __shared__ int32_t stop[kQMax];
-    __shared__ int32_t head[kQMax];
+    __shared__ int32_t lo[kQMax];  // list positions fit 32 bits: the host caps postings at 2^31 - 1
+    __shared__ int32_t hi[kQMax];
+    __shared__ int32_t term[kQMax];
     __shared__ float wt[kQMax];
-    __shared__ int32_t inside[2][kInvWaves];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x;
     const int64_t q = q0 + blockIdx.y;
     const int32_t qs = q_indptr[q];
     const int qn = q_indptr[q + 1] - qs;
-    const int64_t first = static_cast<int64_t>(blockIdx.x) * windows_per_slab * kInvWindow;
-    const int64_t last = min(n_docs, first + static_cast<int64_t>(windows_per_slab) * kInvWindow);
+    const int64_t w0 = static_cast<int64_t>(blockIdx.x) * windows_per_slab;
+    const int64_t w1 = min(n_windows, w0 + windows_per_slab);
     float* row = scores + static_cast<int64_t>(blockIdx.y) * ld;
     for (int t = tid; t < qn; t += kInvBlock) {
-        const int term = q_indices[qs + t];
-        int64_t a = list_ptr[term], b = list_ptr[term + 1];
-        const int64_t e = b;
-        while (a < b) {  // lower_bound(docs[a, b), first)
-            const int64_t m = (a + b) >> 1;
-            if (docs[m] < first)
-                a = m + 1;
-            else
-                b = m;
-        }
-        cur[t] = static_cast<int32_t>(a);
-        stop[t] = static_cast<int32_t>(e);
-        head[t] = a < e ? docs[a] : INT32_MAX;
+        term[t] = q_indices[qs + t];
         wt[t] = q_values[qs + t];
     }
     for (int i = tid; i < kInvWindow; i += kInvBlock) win[i] = 0.0F;
-    __syncthreads();
-    int round = 0;
-    for (int64_t base = first; base < last; base += kInvWindow) {
-        const int64_t wend = min(base + kInvWindow, last);
-        // postings a term's first round takes: its remaining list spread over
-        // the remaining doc ids predicts the count in this window; 1.5x that
-        // plus 64 rarely falls short (a short round is followed by full ones)
-        const float frac = static_cast<float>(wend - base) / static_cast<float>(n_docs - base);
-        auto round_size = [&](int t) {
-            const float est = static_cast<float>(stop[t] - cur[t]) * frac;
-            return static_cast<int>(min(1.5F * est + 64.0F, static_cast<float>(kInvChunks * kInvBlock)));
-        };
+    for (int64_t w = w0; w < w1; ++w) {
+        const int64_t base = w * kInvWindow;
+        const int64_t wend = min(base + kInvWindow, n_docs);
+        __syncthreads();  // term[] / wt[] written, the previous window's lo / hi read
+        for (int t = tid; t < qn; t += kInvBlock) {
+            const int32_t* ws = wstart + static_cast<int64_t>(term[t]) * (n_windows + 1) + w;
+            lo[t] = ws[0];
+            hi[t] = ws[1];
+        }
+        __syncthreads();
         int t = 0;
-        while (t < qn && head[t] >= wend) ++t;  // terms with nothing in the window are skipped
-        InvRound nx;
-        int nx_n = 0;
+        while (t < qn && lo[t] == hi[t]) ++t;  // terms with nothing in the window are skipped
+        InvRound c;
+        int64_t c_p0 = 0;
+        int c_n = 0;
         if (t < qn) {
-            nx_n = round_size(t);
-            inv_load(docs, vals, cur[t], stop[t], nx_n, tid, nx);
+            c_p0 = lo[t];
+            c_n = min(kInvRound, hi[t] - lo[t]);
+            inv_load(docs, vals, c_p0, c_n, tid, c);
         }
         while (t < qn) {
-            InvRound c = nx;
-            int n_ld = nx_n;
-            int64_t p0 = cur[t];
-            const int64_t e = stop[t];
-            const float w = wt[t];
-            int t2 = t + 1;
-            while (t2 < qn && head[t2] >= wend) ++t2;
-            if (t2 < qn) {
-                nx_n = round_size(t2);
-                inv_load(docs, vals, cur[t2], stop[t2], nx_n, tid, nx);
+            const int32_t tend = hi[t];
+            const float wgt = wt[t];
+            int t2 = t;
+            int64_t p2 = c_p0 + c_n;
+            if (p2 >= tend) {  // the term's last round: the next is the next term's first
+                t2 = t + 1;
+                while (t2 < qn && lo[t2] == hi[t2]) ++t2;
+                p2 = t2 < qn ? lo[t2] : 0;
             }
-            for (;;) {
-                int mine = 0;
+            const int n2 = t2 < qn ? min(kInvRound, static_cast<int>((t2 == t ? tend : hi[t2]) - p2)) : 0;
+            InvRound nx;
+            if (t2 < qn) inv_load(docs, vals, p2, n2, tid, nx);
 #pragma unroll
-                for (int j = 0; j < kInvChunks; ++j) {
-                    const bool in = c.d[j] < wend;
-                    if (in) {
-                        const float add = w * c.v[j];
-                        float* slot = &win[c.d[j] - base];
-                        if (kAtomic)
-                            atomicAdd(slot, add);
-                        else
-                            *slot += add;
-                    }
-                    mine += __popcll(__ballot(in));
+            for (int j = 0; j < kInvChunks; ++j) {
+                if (c.d[j] != INT32_MAX) {
+                    const float add = wgt * c.v[j];
+                    float* slot = &win[c.d[j] - base];
+                    if (kAtomic)
+                        atomicAdd(slot, add);
+                    else
+                        *slot += add;
                 }
-                if (lane == 0) inside[round & 1][wave] = mine;
-                __syncthreads();
-                int cnt = 0;
-#pragma unroll
-                for (int v = 0; v < kInvWaves; ++v) cnt += inside[round & 1][v];
-                ++round;
-                if (cnt < n_ld) {
-#pragma unroll
-                    for (int j = 0; j < kInvChunks; ++j)
-                        if (j * kInvBlock + tid == cnt) {  // the first posting past the window
-                            cur[t] = static_cast<int32_t>(p0 + cnt);
-                            head[t] = c.d[j];
-                        }
-                    break;
-                }
-                p0 += n_ld;
-                n_ld = kInvChunks * kInvBlock;  // still inside after a whole round: take whole rounds
-                inv_load(docs, vals, p0, e, n_ld, tid, c);
             }
+            if (t2 != t) __syncthreads();  // the next term adds to the same slots
             t = t2;
+            c = nx;
+            c_p0 = p2;
+            c_n = n2;
         }
         __syncthreads();
         for (int i = tid; i < wend - base; i += kInvBlock) {
             row[base + i] = win[i];
             win[i] = 0.0F;
         }
-        __syncthreads();
     }
 }
 
 }  // namespace
+
+void build_window_starts(DevicePostings& p, int64_t n_docs, int dim) {
+    p.window_start.release();
+    p.n_windows = 0;
+    // positions are int32 (the search refuses more postings anyway)
+    if (n_docs <= 0 || dim <= 0 || p.n_postings > std::numeric_limits<int32_t>::max()) return;
+    const int64_t nw = (n_docs + kInvWindow - 1) / kInvWindow;
+    p.window_start.allocate(static_cast<size_t>(dim) * static_cast<size_t>(nw + 1) * 4);
+    p.n_windows = nw;
+    const int64_t total = static_cast<int64_t>(dim) * (nw + 1);
+    const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 65536)));
+    hipLaunchKernelGGL(window_start_kernel, dim3(grid), dim3(256), 0, nullptr, p.list_ptr.get<int64_t>(),
+                       p.docs.get<int32_t>(), dim, nw, p.window_start.get<int32_t>());
+    hip_check(hipGetLastError(), "window start launch");
+    hip_check(hipDeviceSynchronize(), "window starts");
+}
 
 void launch_inverted_search(const DevicePostings& ps, int64_t n_docs, const int32_t* d_q_indptr,
                             const uint16_t* d_q_indices, const float* d_q_values, int64_t n_queries, int k,
@@ -1701,10 +1723,12 @@ void launch_inverted_search(const DevicePostings& ps, int64_t n_docs, const int3
     if (n_docs <= 0 || n_queries <= 0) throw std::invalid_argument("inverted search needs documents and queries");
     if (ps.n_postings > std::numeric_limits<int32_t>::max())
         throw std::invalid_argument("inverted search addresses at most 2^31 - 1 postings");
+    const int64_t windows = (n_docs + kInvWindow - 1) / kInvWindow;
+    if (ps.n_windows != windows || !ps.window_start.bytes())
+        throw std::logic_error("inverted search without the window table of its lists");
     int kcap = 1;
     while (kcap < k) kcap <<= 1;
     const int64_t group = brute_search_group(n_docs, n_queries);
-    const int64_t windows = (n_docs + kInvWindow - 1) / kInvWindow;
     for (int64_t q0 = 0; q0 < n_queries; q0 += group) {
         const int64_t g = std::min<int64_t>(group, n_queries - q0);
         const int64_t wps = std::clamp<int64_t>(windows * g / kInvSlabTarget, 1, windows);
@@ -1712,12 +1736,14 @@ void launch_inverted_search(const DevicePostings& ps, int64_t n_docs, const int3
         const dim3 grid(static_cast<unsigned>(slabs), static_cast<unsigned>(g));
         if (ps.duplicate_docs)
             hipLaunchKernelGGL(inverted_scores_kernel<true>, grid, dim3(kInvBlock), 0, stream,
-                               ps.list_ptr.get<int64_t>(), ps.docs.get<int32_t>(), ps.vals.get<float>(), n_docs,
-                               static_cast<int>(wps), d_q_indptr, d_q_indices, d_q_values, q0, d_scores, n_docs);
+                               ps.window_start.get<int32_t>(), windows, ps.docs.get<int32_t>(), ps.vals.get<float>(),
+                               n_docs, static_cast<int>(wps), d_q_indptr, d_q_indices, d_q_values, q0, d_scores,
+                               n_docs);
         else
             hipLaunchKernelGGL(inverted_scores_kernel<false>, grid, dim3(kInvBlock), 0, stream,
-                               ps.list_ptr.get<int64_t>(), ps.docs.get<int32_t>(), ps.vals.get<float>(), n_docs,
-                               static_cast<int>(wps), d_q_indptr, d_q_indices, d_q_values, q0, d_scores, n_docs);
+                               ps.window_start.get<int32_t>(), windows, ps.docs.get<int32_t>(), ps.vals.get<float>(),
+                               n_docs, static_cast<int>(wps), d_q_indptr, d_q_indices, d_q_values, q0, d_scores,
+                               n_docs);
         launch_topk(d_scores, n_docs, g, k, kcap, true, d_allow, partial_keys(d_scores, n_docs, n_queries),
                     d_out_d + q0 * k, d_out_l + q0 * k, stream);
         hip_check(hipGetLastError(), "inverted search launch");
