@@ -170,3 +170,26 @@ def test_reference_order_file(nsparse, data):
     print("16-bit reference-order search", flush=True)
     got = dev.search(len(qip) - 1, qip, qix, qv, 10, nsparse.SeismicSearchParameters(10, 0.8))
     assert_same(got, o1.search(qip, qix, qv, 10, cut=10, hf=0.8))
+
+
+def test_serving_loop(nsparse, data, built):
+    """The continuous-batching server over the 16-bit index: a Poisson burst of
+    mixed-k requests, 10% with a 20%-exclusion bitmap, equals the oracle."""
+    idx, o, _ = built
+    ip, ix, v = data[2]
+    n = len(ip) - 1
+    n_docs = len(data[0][0]) - 1
+    rng = np.random.default_rng(14)
+    ks = np.where(rng.random(n) < 0.5, 10, 100).astype(np.int32)
+    keep = rng.random(n_docs) >= 0.2
+    words = np.packbits(keep.astype(np.uint8), bitorder="little")
+    words = np.pad(words, (0, (-len(words)) % 4)).view(np.uint32)
+    bids = np.where(rng.random(n) < 0.1, 0, -1).astype(np.int32)
+    arrivals = np.cumsum(rng.exponential(1.0 / 20000.0, n))
+    print("16-bit serving replay", flush=True)
+    srv = nsparse.Server(idx, max_batch=24, streams=4, k_max=100, cut=10, heap_factor=0.8)
+    srv.add_bitmap(words)
+    d, lab, _ = srv.replay(arrivals, ip, ix, v, ks, bids)
+    srv.close()
+    want = O.search_mixed(o, (ip, ix, v), ks, 100, 10, 0.8, bids, [words], n_docs)
+    assert_same((d, lab), want)

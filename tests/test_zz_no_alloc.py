@@ -2,8 +2,8 @@
 the C ABI): its scratch is a static device array of the library, so its
 results must equal nsparse_search_device's bit for bit -- on two calls in a
 row, the second with fewer queries (the array keeps the first call's
-contents) -- with mixed k and per-query bitmap selectors, for uint8 and FP32
-indexes. Sizes beyond its limit and indexes not yet resident on the device
+contents) -- with mixed k and per-query bitmap selectors, for 8-bit, 16-bit
+and FP32 indexes. Sizes beyond its limit and indexes not yet resident on the device
 are rejected before anything is launched."""
 import ctypes
 
@@ -22,7 +22,7 @@ def _bitmap(n_docs, seed):
     return np.pad(words, (0, (-len(words)) % 4)).view(np.uint32)
 
 
-@pytest.fixture(scope="module", params=[1, 0], ids=["sq", "fp32"])
+@pytest.fixture(scope="module", params=[1, 0, 2], ids=["sq", "fp32", "sq16"])
 def setup(nsparse, request):
     import torch
 
@@ -32,7 +32,8 @@ def setup(nsparse, request):
     print(f"no-alloc setup sq={sq}", flush=True)
     ip, ix, v = synth.make_corpus(synth.doc_spec(seed=21), 20000)
     vmax = float(v.max())
-    desc = (f"seismic_sq,quantizer=8bit|vmin=0|vmax={vmax:.9g}|lambda=300|beta=30|alpha=0.4|seed=2" if sq
+    width = "16bit" if sq == 2 else "8bit"
+    desc = (f"seismic_sq,quantizer={width}|vmin=0|vmax={vmax:.9g}|lambda=300|beta=30|alpha=0.4|seed=2" if sq
             else "seismic,lambda=300|beta=30|alpha=0.4|seed=2")
     idx = nsparse.index_factory(DIM, desc)
     idx.add(len(ip) - 1, ip, ix, v)
