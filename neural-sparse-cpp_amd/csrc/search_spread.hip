@@ -473,7 +473,8 @@ __global__ __launch_bounds__(kRBlock) void spread_replay_kernel(SearchArgs a, Sp
 #pragma unroll
                     for (int u = 0; u < kU; ++u) {
                         if (ballot(kv[u] != 0 && (tkn < k || kv[u] > kk)) == 0) continue;
-                        merge_topk_wave(kv[u] != 0, key_score(kv[u]), key_doc(kv[u]), k, tkn, tks, tki, lane);
+                        merge_topk_wave<true>(kv[u] != 0, key_score(kv[u]), key_doc(kv[u]), k, tkn, tks, tki,
+                                              lane);
                         kk = tkn == k ? doc_key(tks[k - 1], tki[k - 1]) : 0;
                     }
                 }
