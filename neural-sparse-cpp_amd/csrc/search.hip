@@ -825,6 +825,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                 // FP32 rows: one doc per lane, 64-slot groups from the counter;
                 // each row is summed in order (as the reference's loop does)
                 const float* vf = static_cast<const float*>(ix.values);
+                const int64_t nnz_end = ix.indptr[ix.n_docs];
                 while (true) {
                     int g = 0;
                     if (lane == 0) g = atomicAdd(&C.meta[kChunkNext], 1);
@@ -834,16 +835,79 @@ __device__ __forceinline__ void search_query(const SearchArgs& a, const LdsPlan&
                     const int len = j < total ? C.len[j] : -1;
                     if (len < 0) continue;
                     const int32_t rs = C.rs[j];
+                    // the row as aligned words of 8 entries (16 bytes of term ids,
+                    // 32 of float or 16 of 16-bit values), kRowWords per step,
+                    // entries still in row order; a word past the arrays' last
+                    // entry is read entry by entry
+                    constexpr int kRowWords = 1;
+                    const uint4* iw = reinterpret_cast<const uint4*>(ix.indices);
+                    const int64_t r0 = rs, r1 = static_cast<int64_t>(rs) + len;
+                    const int64_t wend = (r1 + 7) >> 3;
                     if constexpr (W16) {
                         const uint16_t* v16 = static_cast<const uint16_t*>(ix.values);
+                        const uint4* vw = reinterpret_cast<const uint4*>(ix.values);
                         uint64_t acc = 0;
-                        for (int pp = rs; pp < rs + len; ++pp)
-                            acc += static_cast<uint64_t>(static_cast<uint32_t>(v16[pp]) *
-                                                         static_cast<uint32_t>(QL.value(ix.indices[pp])));
+                        for (int64_t w0 = r0 >> 3; w0 < wend; w0 += kRowWords) {
+                            if ((w0 + kRowWords) * 8 <= nnz_end) {
+                                uint4 ti[kRowWords], tv[kRowWords];
+#pragma unroll
+                                for (int u = 0; u < kRowWords; ++u) {
+                                    ti[u] = iw[w0 + u];
+                                    tv[u] = vw[w0 + u];
+                                }
+#pragma unroll
+                                for (int u = 0; u < kRowWords; ++u) {
+                                    const uint32_t tt[4] = {ti[u].x, ti[u].y, ti[u].z, ti[u].w};
+                                    const uint32_t vv[4] = {tv[u].x, tv[u].y, tv[u].z, tv[u].w};
+#pragma unroll
+                                    for (int e = 0; e < 8; ++e) {
+                                        const int64_t pp = (w0 + u) * 8 + e;
+                                        if (pp >= r0 && pp < r1) {
+                                            const uint32_t t = (e & 1) ? (tt[e >> 1] >> 16) : (tt[e >> 1] & 0xFFFFu);
+                                            const uint32_t c = (e & 1) ? (vv[e >> 1] >> 16) : (vv[e >> 1] & 0xFFFFu);
+                                            acc += static_cast<uint64_t>(c * static_cast<uint32_t>(QL.value(t)));
+                                        }
+                                    }
+                                }
+                            } else {
+                                for (int64_t pp = w0 * 8 > r0 ? w0 * 8 : r0; pp < r1; ++pp)
+                                    acc += static_cast<uint64_t>(static_cast<uint32_t>(v16[pp]) *
+                                                                 static_cast<uint32_t>(QL.value(ix.indices[pp])));
+                            }
+                        }
                         C.sc[j] = static_cast<float>(acc);
                     } else {
+                        const float4* vw = reinterpret_cast<const float4*>(ix.values);
                         float acc = 0.0F;
-                        for (int pp = rs; pp < rs + len; ++pp) acc += vf[pp] * QL.value(ix.indices[pp]);
+                        for (int64_t w0 = r0 >> 3; w0 < wend; w0 += kRowWords) {
+                            if ((w0 + kRowWords) * 8 <= nnz_end) {
+                                uint4 ti[kRowWords];
+                                float4 va[kRowWords], vb[kRowWords];
+#pragma unroll
+                                for (int u = 0; u < kRowWords; ++u) {
+                                    ti[u] = iw[w0 + u];
+                                    va[u] = vw[2 * (w0 + u)];
+                                    vb[u] = vw[2 * (w0 + u) + 1];
+                                }
+#pragma unroll
+                                for (int u = 0; u < kRowWords; ++u) {
+                                    const uint32_t tt[4] = {ti[u].x, ti[u].y, ti[u].z, ti[u].w};
+                                    const float vv[8] = {va[u].x, va[u].y, va[u].z, va[u].w,
+                                                         vb[u].x, vb[u].y, vb[u].z, vb[u].w};
+#pragma unroll
+                                    for (int e = 0; e < 8; ++e) {
+                                        const int64_t pp = (w0 + u) * 8 + e;
+                                        if (pp >= r0 && pp < r1) {
+                                            const uint32_t t = (e & 1) ? (tt[e >> 1] >> 16) : (tt[e >> 1] & 0xFFFFu);
+                                            acc += vv[e] * QL.value(t);
+                                        }
+                                    }
+                                }
+                            } else {
+                                for (int64_t pp = w0 * 8 > r0 ? w0 * 8 : r0; pp < r1; ++pp)
+                                    acc += vf[pp] * QL.value(ix.indices[pp]);
+                            }
+                        }
                         C.sc[j] = acc;
                     }
                 }
