@@ -585,12 +585,12 @@ void SeismicIndex::search_device(int64_t n, const int32_t* d_indptr, const uint1
                                  const float* d_values, const int32_t* d_ks, int k_max, const SearchRequest& req,
                                  const int32_t* d_sel_of_query, const uint32_t* const* d_bitmaps, int32_t n_bitmaps,
                                  float* d_out_d, int32_t* d_out_l, int64_t* d_stats, hipStream_t stream,
-                                 int32_t* d_err) {
+                                 int32_t* d_err, int64_t spread_cap) {
     if (!has_vectors_ && n > 0 && k_max > 0) return;
     SearchArgs a = search_args(n, d_indptr, d_indices, d_values, d_ks, k_max, req, d_sel_of_query, d_bitmaps,
                                n_bitmaps, d_out_d, d_out_l, d_stats);
     if (d_err != nullptr) a.error_flag = d_err;
-    if (use_spread(n, k_max, req.cut, d_stats != nullptr, stream)) {
+    if (use_spread(n, k_max, req.cut, d_stats != nullptr, stream, spread_cap)) {
         size_t slot = 0;
         void* scratch = spread_scratch_.acquire(
             search_spread_scratch_bytes(view_, max_clusters_, k_max, req.cut, max_list_docs_, n), slot);
@@ -606,16 +606,19 @@ void SeismicIndex::search_device(int64_t n, const int32_t* d_indptr, const uint1
     launch_search_checked(a, max_clusters_, k_max, u8(), stream);
 }
 
-// A launch of one query, or of a few, leaves most CUs idle with one workgroup
-// per query; the spread search puts each query on the whole GPU. Batches of at
-// most NSPARSE_SPREAD_MAX_QUERIES queries (default 2) take it; NSPARSE_SPREAD=0
-// turns it off. Launches under stream capture, or with traversal counters,
-// keep the per-query workgroup.
-bool SeismicIndex::use_spread(int64_t n, int k_max, int cut, bool stats, hipStream_t stream) const {
+// A launch of a few queries leaves most CUs idle with one workgroup per query
+// (about 1.5 ms for any batch up to a hundred at 8.8M docs); the spread search
+// puts each query on the whole GPU, 0.13 ms for one query and 0.8 ms for 32.
+// It reads every doc of the cut lists, so on a GPU shared with other launches
+// it costs more than it saves. Batches of at most cap queries take it (default
+// 32, for a caller alone on the GPU); NSPARSE_SPREAD_MAX_QUERIES overrides the
+// cap and NSPARSE_SPREAD=0 turns it off. Launches under stream capture, or with
+// traversal counters, keep the per-query workgroup.
+bool SeismicIndex::use_spread(int64_t n, int k_max, int cut, bool stats, hipStream_t stream, int64_t cap) const {
     const char* off = std::getenv("NSPARSE_SPREAD");
     if (off != nullptr && std::string(off) == "0") return false;
     const char* e = std::getenv("NSPARSE_SPREAD_MAX_QUERIES");
-    const int64_t max_n = e != nullptr ? std::atoll(e) : 2;
+    const int64_t max_n = e != nullptr ? std::atoll(e) : cap >= 0 ? cap : 32;
     if (stats || !u8() || n > max_n || view_.row_ref == nullptr) return false;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;

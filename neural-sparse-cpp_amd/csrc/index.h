@@ -102,10 +102,12 @@ public:
     // Batched search over queries already in HBM (nsparse_search_device).
     // d_err: the launch's device error flag; nullptr uses the index's (which
     // last_device_error reads). Launches that may overlap pass flags of their own.
+    // spread_cap >= 0 sets the largest launch that takes the spread search in
+    // place of the default (a caller that knows the GPU's load picks it).
     void search_device(int64_t n, const int32_t* d_indptr, const uint16_t* d_indices, const float* d_values,
                        const int32_t* d_ks, int k_max, const SearchRequest& req, const int32_t* d_sel_of_query,
                        const uint32_t* const* d_bitmaps, int32_t n_bitmaps, float* d_out_d, int32_t* d_out_l,
-                       int64_t* d_stats, hipStream_t stream, int32_t* d_err = nullptr);
+                       int64_t* d_stats, hipStream_t stream, int32_t* d_err = nullptr, int64_t spread_cap = -1);
     // search_device without any device allocation: the index must already be
     // resident on the device (after build(), or after a first search of an
     // index read from a file), the error code goes to d_err, and the
@@ -144,7 +146,7 @@ private:
     void refresh_block_rows();
     void refresh_summary_bitmaps();
     // Whether a launch of n queries runs as the spread search (search_spread.hip).
-    bool use_spread(int64_t n, int k_max, int cut, bool stats, hipStream_t stream) const;
+    bool use_spread(int64_t n, int k_max, int cut, bool stats, hipStream_t stream, int64_t cap = -1) const;
     // kMax8bit / kMax16bit of the quantizer (utils/scalar_quantizer.h): the
     // largest code, as float
     float code_max() const { return w16_ ? 65535.0F : 255.0F; }

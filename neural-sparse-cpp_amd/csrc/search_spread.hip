@@ -670,7 +670,13 @@ void launch_search_spread(const SearchArgs& args, int max_clusters, int kmax, in
         }
         per_cu = std::max(cached, 1);
     }
-    int groups = std::max(1, per_cu * cu_count() / P.ncap - 1);
+    // one query's lists fill the GPU; a launch of several shares one fill among
+    // them (at least 4 shares per list), since the per-workgroup query load and
+    // cut selection, not the row bytes, set the time of small shares
+    const int64_t fill = static_cast<int64_t>(per_cu) * cu_count();
+    int groups = static_cast<int>(std::max<int64_t>(1, fill / P.ncap - 1));
+    if (args.n_queries > 1)
+        groups = static_cast<int>(std::max<int64_t>(std::min(groups, 4), fill / (P.ncap * args.n_queries) - 1));
     if (const char* e = std::getenv("NSPARSE_SPREAD_GROUPS")) groups = std::max(1, std::atoi(e));
     groups = std::min(groups, 1024);
     SearchArgs a = args;

@@ -100,7 +100,7 @@ private:
     void run();
     bool reap();
     bool admit();
-    void launch(Slot& s, const std::vector<int64_t>& batch);
+    void launch(Slot& s, const std::vector<int64_t>& batch, int64_t load);
 
     SeismicIndex& index_;
     ServerConfig cfg_;
@@ -116,6 +116,9 @@ private:
     int64_t first_ticket_ = 0;
     std::deque<int64_t> pending_;
     int64_t inflight_bytes_ = 0;
+    // queries in flight up to which batches of up to 64 take the spread search
+    // (NSPARSE_SERVE_SPREAD_LOAD)
+    int64_t spread_load_ = 128;
     ServerStats stats_;
     std::string error_;  // set if the dispatcher failed; every wait then reports it
     std::atomic<bool> stop_{false};

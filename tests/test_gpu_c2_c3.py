@@ -86,13 +86,15 @@ def test_c3_serving_equals_oracle(nsparse, c2):
     r2 = np.random.default_rng(10000)
     ks = np.where(r2.random(n) < 0.5, 10, 100).astype(np.int32)
     bids = np.where(r2.random(n) < 0.1, r2.integers(0, len(pool), n), -1).astype(np.int32)
-    arrivals = np.cumsum(r2.exponential(1.0 / 10000.0, n))
+    arrivals = np.cumsum(r2.exponential(1.0 / 50000.0, n))
     per_req = 512 * 6 + 12 + 8 * 100
-    budget = 8 * per_req  # room for about 20 average requests: arrivals at 10k/s outrun it
+    # room for about 10 average requests: arrivals at 50k/s, each served in
+    # 0.2 ms or more, outrun it
+    budget = 4 * per_req
     srv = nsparse.Server(idx, max_batch=64, streams=4, scratch_budget=budget, k_max=100, cut=10, heap_factor=0.8)
     for w in pool:
         srv.add_bitmap(w)
-    _log(f"serving replay: {n} requests at 10k/s")
+    _log(f"serving replay: {n} requests at 50k/s")
     d, lab, lat = srv.replay(arrivals, qip, qix, qv, ks, bids)
     st = srv.stats()
     srv.close()

@@ -2,7 +2,7 @@
 launch on the whole GPU -- one workgroup per cut list for its summary scores
 and first-list order, several per list for its rows' scores -- then an exact
 replay of the reference's walk. It runs for launches of at most
-NSPARSE_SPREAD_MAX_QUERIES queries (default 2) over uint8 indexes with block
+NSPARSE_SPREAD_MAX_QUERIES queries (default 32) over uint8 indexes with block
 rows, so the plain drop-in (one query per call) takes it. These tests run
 queries one per call and also force it for whole batches; every result must
 equal the oracle's (scores bit for bit, labels in the id tie mode), as the
