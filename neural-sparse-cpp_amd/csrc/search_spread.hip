@@ -57,6 +57,9 @@ enum ListMisc { kLErr = 0, kLNu, kLNcut, kLFirst, kLNf, kLSortCnt, kLSortCnt1, k
 enum ReplayMisc { kRErr = 0, kRTkn, kRMiscCount = 16 };
 
 // LDS and scratch layouts (computed on the host).
+// doc keys the replay stages before one merge into the top-k
+constexpr int kStage = 64;
+
 struct SpreadPlan {
     int32_t nw;      // query bitmap words
     int32_t qcap;    // longest query (non-zeros) the tables hold
@@ -69,7 +72,7 @@ struct SpreadPlan {
     int32_t o_cstart, o_acc, o_uni, total_list;
     int32_t sort_rng, sort_cap;  // the first-list sort's range lists: byte offset in the union, ranges per level
     // replay workgroup (two buffers per list section)
-    int32_t r_cuts, r_misc, r_lmax, r_tks, r_tki, r_ord, r_cst, r_ssc, r_cmx, r_key, total_replay;
+    int32_t r_cuts, r_misc, r_lmax, r_tks, r_tki, r_ord, r_stg, r_cst, r_ssc, r_cmx, r_key, total_replay;
     int32_t r_cst_stride, r_ssc_stride, r_cmx_stride, r_key_stride;  // bytes between the two buffers
     // per-query scratch, bytes from the query's base
     int64_t s_order, s_sum, s_cmax, s_lmax, s_key, s_bytes;  // s_lmax directly follows the s_cmax section
@@ -414,8 +417,29 @@ __global__ __launch_bounds__(kRBlock) void spread_replay_kernel(SearchArgs a, Sp
         }
     };
 
-    // The exact walk of list ci from buffer b (wave 0).
+    // The exact walk of list ci from buffer b (wave 0). Doc keys that beat the
+    // k-th entry are staged (up to kStage) and merged into the top-k together:
+    // the top-k of a set of distinct keys does not depend on the order they
+    // arrive in, so a batch equals the one-by-one merges as long as the walk
+    // visits the same clusters. A test that fails against the merged top-k
+    // fails in the sequential walk too (its k-th score only rises). A test
+    // passes for sure if it passes against the highest k-th score the staged
+    // keys can lead to: with sm of them, the entry now at k - 1 - sm (a
+    // non-full top-k stays below k while tkn + sm < k). Where that is not
+    // certain the staged keys are merged first and the test runs exactly.
+    // Staging a doc that in the sequential walk would not beat the k-th entry
+    // changes nothing: it stays below every key of the final top-k.
     int tkn = 0;
+    int sm = 0;  // staged keys
+    uint64_t* stg = reinterpret_cast<uint64_t*>(smem + P.r_stg);
+    auto flush = [&] {
+        if (sm == 0) return;
+        __builtin_amdgcn_wave_barrier();
+        const uint64_t kv = lane < sm ? stg[lane] : 0;
+        __builtin_amdgcn_wave_barrier();
+        merge_topk_wave<true>(kv != 0, key_score(kv), key_doc(kv), k, tkn, tks, tki, lane);
+        sm = 0;
+    };
     auto replay = [&](int ci, int b) {
         const uint32_t term = cuts[ci];
         if (term >= static_cast<uint32_t>(ix.n_lists)) return;
@@ -437,8 +461,8 @@ __global__ __launch_bounds__(kRBlock) void spread_replay_kernel(SearchArgs a, Sp
             uint64_t done = 0;  // lanes (visiting positions) already walked
             bool brk = false;
             while (true) {
-                // every test below sees the k-th score the sequential walk
-                // would see at that cluster: it changes only at a merge
+                // tests against the merged top-k (certain fails; passes are
+                // checked against the staged keys below)
                 const bool full = tkn == k;
                 const float kth = full ? tks[k - 1] : 0.0F;
                 const uint64_t tkey = full ? doc_key(kth, tki[k - 1]) : 0;
@@ -456,11 +480,30 @@ __global__ __launch_bounds__(kRBlock) void spread_replay_kernel(SearchArgs a, Sp
                 }
                 if (m == 0) break;
                 const int i = __ffsll(static_cast<unsigned long long>(m)) - 1;
+                const uint64_t upto = (2ULL << i) - 1;  // lanes <= i (all 64 when i = 63)
+                if (sm > 0) {
+                    // lane i's pass, and on the first list every pass before
+                    // it (a failure there would end the walk), must hold
+                    // whatever the staged keys do to the k-th score
+                    bool sure;
+                    if (!full) {
+                        sure = tkn + sm < k;
+                    } else if (sm >= k) {
+                        sure = false;
+                    } else {
+                        const float bound = tks[k - 1 - sm];
+                        const uint64_t scope = first ? upto : (1ULL << i);
+                        sure = (ballot(valid && !fail && s * hf < bound) & ~done & scope) == 0;
+                    }
+                    if (!sure) {
+                        flush();
+                        continue;
+                    }
+                }
                 const int cc = __builtin_amdgcn_readlane(c, i);
                 const int d1 = cst[cc + 1];
-                // The cluster's docs, 64 at a time with kU loads in flight: a
-                // merge runs only for 64 that hold a doc better than the
-                // current k-th entry (or while the top-k fills).
+                // The cluster's docs, 64 at a time with kU loads in flight;
+                // those that beat the merged top-k's k-th entry are staged.
                 constexpr int kU = 4;
                 uint64_t kk = tkey;
                 for (int d = cst[cc]; d < d1; d += 64 * kU) {
@@ -472,17 +515,29 @@ __global__ __launch_bounds__(kRBlock) void spread_replay_kernel(SearchArgs a, Sp
                     }
 #pragma unroll
                     for (int u = 0; u < kU; ++u) {
-                        if (ballot(kv[u] != 0 && (tkn < k || kv[u] > kk)) == 0) continue;
-                        merge_topk_wave<true>(kv[u] != 0, key_score(kv[u]), key_doc(kv[u]), k, tkn, tks, tki,
-                                              lane);
-                        kk = tkn == k ? doc_key(tks[k - 1], tki[k - 1]) : 0;
+                        bool take = kv[u] != 0 && (tkn < k || kv[u] > kk);
+                        uint64_t tm = ballot(take);
+                        if (tm == 0) continue;
+                        if (sm + __popcll(tm) > kStage) {
+                            flush();
+                            kk = tkn == k ? doc_key(tks[k - 1], tki[k - 1]) : 0;
+                            take = kv[u] != 0 && (tkn < k || kv[u] > kk);
+                            tm = ballot(take);
+                            if (tm == 0) continue;
+                        }
+                        if (take) stg[sm + __popcll(tm & ((1ULL << lane) - 1))] = kv[u];
+                        sm += __popcll(tm);
                     }
                 }
-                done |= (2ULL << i) - 1;  // lanes <= i (all 64 when i = 63)
+                done |= upto;
                 if (done == ~0ULL) break;
             }
+            // the first list's next window (or its end) must see every test
+            // exact: a failure among this window's unchecked passes ends it
+            if (first) flush();
             if (brk) break;
         }
+        flush();
     };
 
     // List ci is walked from buffer ci & 1 while the other waves stage list
@@ -582,6 +637,7 @@ SpreadPlan make_spread_plan(int dim, int max_clusters, int kmax, int cut, int64_
     P.r_tks = take(kmax * 4);
     P.r_tki = take(kmax * 4);
     P.r_ord = take(P.mc * 2);
+    P.r_stg = take(kStage * 8);
     P.r_cst_stride = ((P.mc + 1) * 4 + 15) & ~15;
     P.r_ssc_stride = (P.mc * 4 + 15) & ~15;
     P.r_cmx_stride = P.mc * 8;
