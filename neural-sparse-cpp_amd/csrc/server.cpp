@@ -216,6 +216,7 @@ bool Server::admit() {
         std::vector<int64_t> batch;
         int64_t busy = 1;  // batches in flight, this one included
         int64_t load = 0;  // queries in flight, this batch's included
+        bool big = false;  // a batch of more than 64 queries is in flight
         {
             std::lock_guard<std::mutex> lk(mu_);
             if (pending_.empty()) return any;
@@ -267,10 +268,11 @@ bool Server::admit() {
             for (const Slot& o : slots_) {
                 busy += o.busy ? 1 : 0;
                 load += o.busy ? static_cast<int64_t>(o.tickets.size()) : 0;
+                big = big || (o.busy && o.tickets.size() > 64);
             }
             stats_.max_concurrent_batches = std::max(stats_.max_concurrent_batches, busy);
         }
-        launch(s, batch, load);
+        launch(s, batch, big || load > spread_load_ ? 8 : 64);
         any = true;
     }
     return any;
@@ -279,9 +281,10 @@ bool Server::admit() {
 // The spread search puts each query of a batch on the whole GPU, about 0.35 ms
 // plus 13 us per query at 8.8M docs, against about 1.5 ms on one CU per query
 // for the per-query workgroup. It costs more GPU time per query, so batches of
-// up to 64 take it while at most spread_load_ queries are in flight (this
-// batch's included), batches of up to 8 under heavier load.
-void Server::launch(Slot& s, const std::vector<int64_t>& batch, int64_t load) {
+// up to 64 take it under light load, batches of up to 8 under heavy load:
+// more than spread_load_ queries in flight (this batch's included), or a
+// batch of more than 64 in flight.
+void Server::launch(Slot& s, const std::vector<int64_t>& batch, int64_t spread_cap) {
     const int64_t n = static_cast<int64_t>(batch.size());
     hip_check(hipMemcpyAsync(s.d_in.get(), s.h_in, s.in_bytes, hipMemcpyHostToDevice, s.stream), "H2D");
     const size_t out = static_cast<size_t>(n) * cfg_.k_max;
@@ -296,7 +299,7 @@ void Server::launch(Slot& s, const std::vector<int64_t>& batch, int64_t load) {
                          with_sel ? reinterpret_cast<const int32_t*>(d_in + s.off_sel) : nullptr,
                          with_sel ? d_bitmap_table_.get<const uint32_t*>() : nullptr,
                          static_cast<int32_t>(bitmaps_.size()), reinterpret_cast<float*>(d_out + kOutHead), reinterpret_cast<int32_t*>(d_out + s.off_lab),
-                         nullptr, s.stream, reinterpret_cast<int32_t*>(d_out), load <= spread_load_ ? 64 : 8);
+                         nullptr, s.stream, reinterpret_cast<int32_t*>(d_out), spread_cap);
     // the error flag, the distances and the labels in one copy
     hip_check(hipMemcpyAsync(s.h_out, d_out, s.out_bytes, hipMemcpyDeviceToHost, s.stream), "D2H");
     hip_check(hipEventRecord(s.done, s.stream), "hipEventRecord");

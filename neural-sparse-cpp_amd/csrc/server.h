@@ -100,7 +100,7 @@ private:
     void run();
     bool reap();
     bool admit();
-    void launch(Slot& s, const std::vector<int64_t>& batch, int64_t load);
+    void launch(Slot& s, const std::vector<int64_t>& batch, int64_t spread_cap);
 
     SeismicIndex& index_;
     ServerConfig cfg_;
