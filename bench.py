@@ -489,7 +489,7 @@ def main():
                     help='"auto" (autotuner) or a JSON search config, e.g. {"lookup":1,"block":512,"persistent":1}')
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (query-sharded replicas are a "
+        print("bench.py measures one GPU: the library has no multi-GPU path (query-sharded replicas are a "
               "design note only)", file=sys.stderr)
         sys.exit(2)
 

@@ -16,7 +16,7 @@
 //
 // Pinning (what each part is checked against) is listed in the header comment
 // of tests/test_oracle.py. Parts pinned only by invariants there are marked
-// "parity unpinned" in that file and in DESIGN.md.
+// "parity unpinned" in that file.
 #include <algorithm>
 #include <array>
 #include <atomic>

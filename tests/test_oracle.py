@@ -58,7 +58,7 @@ def test_build_bytes_match_reference(sq, n, dim, nnz, lam, beta):
     o = OracleIndex(dim, sq, lam, beta, 0.4, 7, vmin, vmax, tie_mode=0)
     o.add(ip, ix, v)
     o.build()
-    # the driver resolves lambda/beta the way seismic_common.h:130-144 does
+    # oracle/ref_driver.cpp resolves lambda/beta the way seismic_common.h:130-144 does
     lam_r = max(int(np.float32(0.0005) * np.float32(n)), 160) if lam == -1 else lam
     beta_r = int(np.float32(lam_r) * np.float32(0.1)) if beta == -1 else beta
     want = O.ref_build_bytes(dim, sq, ip, ix, v, lam_r, beta_r, 0.4, 7, vmin, vmax)
