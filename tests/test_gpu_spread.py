@@ -85,7 +85,12 @@ def test_one_query_per_call_matches_oracle(nsparse, data, pair, cut, hf, k):
         assert_same(got, (want[0][:50], want[1][:50]))
 
 
-@pytest.mark.parametrize("cut,hf,k", [(10, 0.8, 10), (10, 1.0, 100), (3, 0.7, 10), (64, 0.9, 10), (1, 0.5, 5)])
+# k of 1, 2, 63 and 65 around the 64-key staging of the replay's merges (every
+# staged key forces an exact test at k = 1; more than 64 keys span several
+# merges), and k of 200 and 500 with tight and loose pruning
+@pytest.mark.parametrize("cut,hf,k", [(10, 0.8, 10), (10, 1.0, 100), (3, 0.7, 10), (64, 0.9, 10), (1, 0.5, 5),
+                                      (10, 0.5, 1), (10, 0.6, 2), (10, 0.7, 63), (10, 0.9, 65), (10, 0.4, 200),
+                                      (10, 0.95, 500)])
 def test_forced_batches_match_oracle(nsparse, data, pair, forced, cut, hf, k):
     o, dev, _ = pair
     for qs in (data[1], data[2]):
