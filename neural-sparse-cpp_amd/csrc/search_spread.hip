@@ -53,7 +53,7 @@ constexpr int kWinPerStep = 4;   // 64-word windows per wave per step (loads in 
 constexpr int kMaxCutSlots = 64;
 constexpr int kLdsCap = 160 * 1024;
 
-enum ListMisc { kLErr = 0, kLNu, kLNcut, kLFirst, kLNf, kLSortCnt, kLSortCnt1, kLMiscCount = 16 };
+enum ListMisc { kLErr = 0, kLNu, kLNcut, kLFirst, kLNf, kLSortCnt, kLSortCnt1, kLMax, kLMaxCnt, kLMaxAt, kLMiscCount = 16 };
 enum ReplayMisc { kRErr = 0, kRTkn, kRMiscCount = 16 };
 
 // LDS and scratch layouts (computed on the host).
@@ -156,7 +156,25 @@ __global__ __launch_bounds__(kLBlock) void spread_lists_kernel(SearchArgs a, Spr
     if (tid < kLMiscCount) misc[tid] = 0;
     const QueryTables QT{qbits, qpref, eterm, erank, evalf, rlast, rval, rsum, qcode, wsum, &misc[kLErr], &misc[kLNu]};
     if (!load_query_lds<true, true, kLBlock>(a, s0, qn, P.nw, QT)) return;  // the replay reports it
-    if (wave == 0) {
+    // top_k_tokens puts its largest entry first (out of more entries than the
+    // cut). With one largest code, whose term has a list, that entry's list is
+    // the first the walk visits: its summary workgroup (cut slot 0) starts at
+    // once, and another workgroup of slot 0 writes the cut for the replay.
+    int fast_term = -1;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && qn > a.cut) {
+        for (int i = tid; i < qn; i += kLBlock) atomicMax(&misc[kLMax], __float_as_int(evalf[i]));
+        __syncthreads();
+        const int mx = misc[kLMax];
+        for (int i = tid; i < qn; i += kLBlock)
+            if (__float_as_int(evalf[i]) == mx) {
+                atomicAdd(&misc[kLMaxCnt], 1);
+                misc[kLMaxAt] = i;
+            }
+        __syncthreads();
+        if (misc[kLMaxCnt] == 1 && eterm[misc[kLMaxAt]] < static_cast<uint32_t>(ix.n_lists))
+            fast_term = eterm[misc[kLMaxAt]];
+    }
+    if (fast_term < 0 && wave == 0) {
         // top_k_tokens (utils/vector_process.h:18-30) on the codes (cut <= 64)
         const int ncut = topk_tokens_wave(qn, a.cut, evalf, eterm, cuts, lane);
         __builtin_amdgcn_wave_barrier();
@@ -169,9 +187,9 @@ __global__ __launch_bounds__(kLBlock) void spread_lists_kernel(SearchArgs a, Spr
         }
     }
     __syncthreads();
-    const int ncut = misc[kLNcut], first = misc[kLFirst];
+    const int ncut = fast_term >= 0 ? 1 : misc[kLNcut], first = fast_term >= 0 ? 0 : misc[kLFirst];
     const int ci = blockIdx.x;
-    if (ci == 0 && blockIdx.y == 0) {
+    if (ci == 0 && blockIdx.y == 1) {  // the grid has 1 + G >= 2 workgroups per slot
         int32_t* hdr = reinterpret_cast<int32_t*>(qs);
         uint16_t* hcuts = reinterpret_cast<uint16_t*>(qs + 8);
         if (tid == 0) {
@@ -181,7 +199,7 @@ __global__ __launch_bounds__(kLBlock) void spread_lists_kernel(SearchArgs a, Spr
         for (int i = tid; i < ncut; i += kLBlock) hcuts[i] = cuts[i];
     }
     if (ci >= ncut) return;
-    const uint32_t term = cuts[ci];
+    const uint32_t term = fast_term >= 0 ? static_cast<uint32_t>(fast_term) : cuts[ci];
     if (term >= static_cast<uint32_t>(ix.n_lists)) return;
     const int32_t lb = ix.list_cluster_base[term];
     const int nc = ix.list_cluster_base[term + 1] - lb;
